@@ -1,0 +1,274 @@
+/*
+ * pierce_hip.h - C ABI of the MI355X (gfx950) spatial-join engine.
+ *
+ * Plain C types only: pointers, sizes, enums as int32.  Device work runs on
+ * the context's HIP stream; every call blocks until its results are on the
+ * host unless stated otherwise.  No call exits the process: failures return a
+ * negative pz_status and pz_last_error() describes the most recent failure of
+ * the calling thread.
+ *
+ * Reference interfaces these entry points replace (paths relative to the
+ * Pierce source tree):
+ *   - dataset load/upload: pierce/query/src/runtime/GeometryIO.cpp:6-20,
+ *     pierce/query/src/geometry/GeometryUploader.cpp:12-34,
+ *     pierce/query/src/geometry/PrecomputedEdgeData.cpp:5-29
+ *   - index build: pierce/query/src/optix/OptixAccelerationStructure.cpp:22-68
+ *   - query launches: MeshOverlapEdgesLauncher.h:49-51,
+ *     MeshIntersectionLauncher.h:17-22, MeshContainmentLauncher.h:26-30
+ *     (all under pierce/query/src/raytracing/)
+ *   - extern "C" helpers: pierce/query/src/cuda/mesh_query_deduplication.h:6-18,
+ *     pierce/query/src/cuda/scan_utils.h:7-9
+ *   - selectivity estimate: pierce/query/src/cuda/estimated_overlap.h:6-13
+ */
+#ifndef PIERCE_HIP_H
+#define PIERCE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+typedef int32_t pz_status;
+#define PZ_OK            0
+#define PZ_E_INVALID    -1  /* bad argument */
+#define PZ_E_HIP        -2  /* HIP runtime error */
+#define PZ_E_IO         -3  /* file missing / malformed / wrong version */
+#define PZ_E_OVERFLOW   -4  /* a sized buffer was too small (recoverable) */
+#define PZ_E_NOMEM      -5  /* device or host allocation failed */
+#define PZ_E_NO_EDGES   -6  /* dataset lacks the precomputed edge payload */
+#define PZ_E_LIMIT      -7  /* input beyond a documented engine limit */
+#define PZ_E_NODEVICE   -8  /* no HIP device available */
+
+/* Most recent failure message of the calling thread ("" if none). */
+const char* pz_last_error(void);
+
+/* ------------------------------------------------------------- host types */
+/* One result pair; layout of MeshQueryResult (pierce/common/include/common.h:34-37). */
+typedef struct {
+    int32_t object_id_mesh1;
+    int32_t object_id_mesh2;
+} pz_pair;
+
+/* Sparse grid cell of the .pre payload: GridCell (pierce/common/include/GridCell.h:5-10)
+ * preceded by its int3 index (SparseGridEntry, pierce/common/include/Geometry.h:15-18). */
+typedef struct {
+    int32_t ix, iy, iz;
+    uint32_t center_count;
+    uint32_t touch_count;
+    float avg_size_mean;
+    float vol_ratio;
+} pz_grid_cell;
+
+/* Borrowed view of one dataset: the .pre v4 payload
+ * (pierce/common/include/BinaryIO.h:14-151).  All arrays are AoS exactly as in
+ * the file: float3 vertices, uint3 indices, int32 triangle->object map, float3
+ * edge starts/ends and int32 edge source objects. */
+typedef struct {
+    const float*    vertices;        /* 3 * num_vertices */
+    uint64_t        num_vertices;
+    const uint32_t* indices;         /* 3 * num_triangles */
+    uint64_t        num_triangles;
+    const int32_t*  triangle_to_object; /* num_triangles */
+    const float*    edge_starts;     /* 3 * num_edges (may be NULL if num_edges == 0) */
+    const float*    edge_ends;       /* 3 * num_edges */
+    const int32_t*  edge_objects;    /* num_edges */
+    uint64_t        num_edges;
+    const pz_grid_cell* grid_cells;  /* num_grid_cells, may be NULL */
+    uint64_t        num_grid_cells;
+    float           grid_cell_size;  /* 0 if no grid */
+} pz_mesh_view;
+
+/* Host-owned dataset (generated or read from a .pre file). */
+typedef struct pz_host_mesh pz_host_mesh;
+
+pz_status pz_host_mesh_view(const pz_host_mesh* mesh, pz_mesh_view* out);
+void      pz_host_mesh_free(pz_host_mesh* mesh);
+
+/* Read a .pre v4 file (pierce/common/include/BinaryIO.h:84-151).  Wrong magic or
+ * version -> PZ_E_IO (the reference returns empty geometry and prints an error). */
+pz_status pz_pre_read(const char* path, pz_host_mesh** out);
+/* Write a .pre v4 file byte-compatible with BinaryIO.h:32-82. */
+pz_status pz_pre_write(const char* path, const pz_mesh_view* mesh);
+
+/* Copy a view into a new host mesh (owned arrays). */
+pz_status pz_host_mesh_from_view(const pz_mesh_view* view, pz_host_mesh** out);
+
+/* Replace the mesh's edge payload with the unique per-object edges of
+ * EdgePreprocessor::extractEdges (pierce/preprocess/src/EdgePreprocessor.cpp:44-111):
+ * 1e-5 quantised endpoint keys, first-seen float endpoints, key order. */
+pz_status pz_host_mesh_extract_edges(pz_host_mesh* mesh);
+/* Replace the mesh's grid payload with generateGridStats
+ * (pierce/preprocess/src/pierce_preprocess.cpp:49-167); cells in ascending
+ * (z, y, x) order, objects visited in ascending id order. */
+pz_status pz_host_mesh_compute_grid(pz_host_mesh* mesh, float cell_size);
+
+/* ------------------------------------------------------------- generators */
+#define PZ_GEN_ELLIPSOIDS     0  /* icosphere ellipsoids at one subdivision level */
+#define PZ_GEN_SPHERES_MIXED  1  /* spheres, level l in [level_min, level_max] with P(l) ~ 2^-l */
+#define PZ_GEN_CUBES          2  /* axis-aligned cubes, 12 triangles */
+#define PZ_GEN_TUBES          3  /* branching generalised-cylinder tubes */
+
+#define PZ_EDGES_NONE     0
+#define PZ_EDGES_GROUPED  1  /* each mesh's unique edges, meshes in id order (fast) */
+#define PZ_EDGES_PRE      2  /* EdgePreprocessor key order over the whole set (a .pre file's order) */
+
+typedef struct {
+    int32_t  kind;            /* PZ_GEN_* */
+    int32_t  edges_mode;      /* PZ_EDGES_* */
+    uint64_t count;           /* number of meshes */
+    uint64_t seed;
+    int32_t  level_min;       /* icosphere subdivision */
+    int32_t  level_max;
+    float    radius_min;      /* ellipsoid radii / sphere radius / cube edge / tube radius */
+    float    radius_max;
+    float    box;             /* centres uniform in [0, box)^3 */
+    float    jitter;          /* vertex jitter sigma as a fraction of the mean radius */
+    /* scaled copies of another generated set (containment workloads) */
+    float    copy_fraction;
+    float    copy_scale_min;
+    float    copy_scale_max;
+    uint64_t copy_source_count;
+    uint64_t copy_source_seed;
+    /* tubes */
+    int64_t  tri_median;      /* lognormal median of triangle counts */
+    float    tri_sigma;       /* lognormal sigma */
+    int64_t  tri_min;
+    int64_t  tri_max;
+    int32_t  ring_segments;
+    float    step;
+    float    branch_prob;
+    int32_t  max_branches;
+    float    degenerate_fraction; /* fraction replaced by near-coplanar/offset copies of the source set */
+    int32_t  reserved[4];
+} pz_gen_params;
+
+/* Fill *p with the defaults of a kind (ellipsoids at level 2, radii [1,3], box 100). */
+void      pz_gen_defaults(int32_t kind, pz_gen_params* p);
+pz_status pz_generate(const pz_gen_params* params, pz_host_mesh** out);
+
+/* ---------------------------------------------------------------- context */
+typedef struct pz_ctx pz_ctx;
+typedef struct pz_dataset pz_dataset;
+
+/* Number of visible HIP devices (0 when none; never fails). */
+int32_t   pz_device_count(void);
+pz_status pz_ctx_create(int32_t device, pz_ctx** out);
+void      pz_ctx_destroy(pz_ctx* ctx);
+
+/* Upload one dataset to HBM (GeometryUploader::upload + PrecomputedEdgeData).
+ * A dataset without edges is rejected with PZ_E_NO_EDGES
+ * (pierce/query/src/runtime/GeometryIO.cpp:11-20). */
+pz_status pz_dataset_upload(pz_ctx* ctx, const pz_mesh_view* view, pz_dataset** out);
+pz_status pz_dataset_load_pre(pz_ctx* ctx, const char* path, pz_dataset** out);
+void      pz_dataset_release(pz_dataset* ds);
+/* out[0..4] = vertices, triangles, edges, objects (max id + 1), grid cells */
+pz_status pz_dataset_info(const pz_dataset* ds, uint64_t out[5]);
+
+/* ------------------------------------------------------------------- join */
+#define PZ_PRED_OVERLAP       0  /* surfaces intersect (pierce_overlap*) */
+#define PZ_PRED_INTERSECTION  1  /* overlap or one solid contains the other (pierce_intersection) */
+#define PZ_PRED_CONTAINMENT   2  /* mesh2 object strictly inside mesh1 object, no contact (pierce_containment) */
+
+#define PZ_DIR_BOTH       0
+#define PZ_DIR_1TO2       1  /* mesh1 edges against mesh2 triangles only */
+#define PZ_DIR_2TO1       2
+
+#define PZ_STRATEGY_TWO_LEVEL 0  /* object broad phase + per-object BVHs, early exit per pair */
+#define PZ_STRATEGY_GLOBAL    1  /* every edge traced through a set-wide BVH (reference shape) */
+
+#define PZ_PREC_FP32_FILTERED 0  /* fp32 filter, fp64 filter, exact fallback */
+#define PZ_PREC_FP64          1  /* fp64 filter, exact fallback */
+
+#define PZ_SIZING_TWO_PASS 0  /* exact count pass, scan, write pass */
+#define PZ_SIZING_ESTIMATE 1  /* grid selectivity estimate (estimated_overlap.cu:29-40) */
+#define PZ_SIZING_LEARNED  2  /* MLP estimate (needs pz_join_desc.model) */
+#define PZ_SIZING_FIXED    3  /* capacity given by pz_join_desc.capacity */
+
+#define PZ_DEDUP_SORT 0  /* GLOBAL strategy: raw hits sorted + uniqued */
+#define PZ_DEDUP_HASH 1  /* GLOBAL strategy: open-addressing hash set, then compaction */
+
+typedef struct pz_mlp pz_mlp;
+
+typedef struct {
+    int32_t predicate;
+    int32_t direction;          /* overlap only */
+    int32_t strategy;
+    int32_t precision;
+    int32_t sizing;
+    int32_t dedup;              /* GLOBAL strategy only */
+    int32_t include_overlap_pairs; /* containment: also return overlap pairs (type 1) */
+    int32_t max_hits_per_edge;  /* 0 = unlimited (the reference caps at overlap_max_iterations) */
+    uint64_t capacity;          /* FIXED: buffer slots; ESTIMATE/LEARNED: 0 = computed */
+    float    load_factor;       /* hash sets: slots = expected / load_factor (0 -> 0.5) */
+    float    margin;            /* estimate-sized buffers hold expected * (1 + margin) (0 -> 0.25) */
+    float    gamma;             /* grid estimator exponent (0 -> 0.8) */
+    float    epsilon;           /* grid estimator size epsilon (0 -> 0.001) */
+    int32_t  no_alpha_correction;
+    int32_t  keep_device_result; /* reserved, must be 0 */
+    const pz_mlp* model;        /* PZ_SIZING_LEARNED */
+    int32_t  reserved[8];
+} pz_join_desc;
+
+/* Fill *d with the defaults: overlap, both directions, TWO_LEVEL, FP32_FILTERED, TWO_PASS. */
+void pz_join_desc_defaults(pz_join_desc* d);
+
+typedef struct {
+    char   name[48];
+    double ms;
+} pz_phase;
+
+#define PZ_MAX_PHASES 24
+
+typedef struct {
+    uint64_t num_pairs;
+    pz_pair* pairs;             /* host, library-owned; sorted by (mesh1, mesh2), unique */
+    uint8_t* pair_types;        /* containment + include_overlap_pairs: 0 = containment, 1 = overlap; else NULL */
+    uint64_t num_candidates;    /* TWO_LEVEL: object pairs with overlapping boxes; GLOBAL: raw edge hits */
+    uint64_t capacity;          /* slots of the sized buffer actually used */
+    uint64_t predicted;         /* sizing prediction (0 for TWO_PASS) */
+    int32_t  overflowed;        /* 1: the estimate-sized buffer overflowed; the exact count was used */
+    int32_t  num_phases;
+    uint64_t tests_total;       /* primitive predicate evaluations (segment-triangle and parity) */
+    uint64_t tests_fp64;        /* decided by the fp64 filter */
+    uint64_t tests_exact;       /* decided by exact arithmetic */
+    uint64_t tests_fp32_misclassified; /* escalated tests an unguarded fp32 evaluation gets wrong */
+    uint64_t edges_traced;      /* edge traversals started */
+    uint64_t stack_overflows;   /* must be 0: traversal stack exceeded (result invalid) */
+    double   ms_total;          /* device-resident geometry -> sorted host pair set */
+    pz_phase phases[PZ_MAX_PHASES];
+} pz_join_result;
+
+pz_status pz_join(pz_ctx* ctx, const pz_dataset* mesh1, const pz_dataset* mesh2,
+                  const pz_join_desc* desc, pz_join_result* out);
+void      pz_join_result_release(pz_join_result* r);
+
+/* Number of (edge, triangle) pairs over both directions whose closed fp32
+ * boxes overlap (the BVH-independent candidate-test count). */
+pz_status pz_count_candidate_tests(pz_ctx* ctx, const pz_dataset* mesh1, const pz_dataset* mesh2,
+                                   uint64_t* out);
+
+/* ------------------------------------------------ reference extern "C" helpers */
+/* pierce/query/src/cuda/mesh_query_deduplication.h:7-11: device inputs, caller-sized
+ * device output (>= n1 + n2); returns the number of unique pairs, sorted by
+ * (mesh1, mesh2).  Returns -1 on failure. */
+long long merge_and_deduplicate_pairs_gpu(const pz_pair* d_results1, long long num_results1,
+                                          const pz_pair* d_results2, long long num_results2,
+                                          pz_pair* d_merged_output);
+/* mesh_query_deduplication.h:13-16: device hash table of (id1 << 32 | id2) keys,
+ * empty = ~0ull; writes at most max_output_size pairs; returns the count written. */
+int compact_hash_table_pairs(const unsigned long long* d_hash_table, unsigned long long table_size,
+                             pz_pair* d_output, int max_output_size);
+/* scan_utils.h:7-9: exclusive scan of int -> long long on the device; returns the total. */
+long long exclusive_scan_gpu(const int* d_input, long long* d_output, int num_elements);
+
+/* estimated_overlap.h:6-13 (host inputs): sum over matched cells of the grid
+ * estimate; -1.0f on failure. */
+float pz_estimate_overlap_selectivity(const pz_grid_cell* cells_a, const pz_grid_cell* cells_b,
+                                      int32_t num_cells, float cell_volume, float epsilon, float gamma);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PIERCE_HIP_H */

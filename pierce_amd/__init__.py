@@ -1,0 +1,406 @@
+"""Python mirror of the pierce_hip C ABI (ctypes over libpierce_hip.so).
+
+The engine is the native library: this module only marshals numpy arrays in
+and out.  There is no Python or CPU fallback for the join: without the built
+library, or without a HIP device, every join call raises.
+
+Reference interfaces mirrored here: the ``.pre`` v4 payload
+(pierce/common/include/BinaryIO.h:14-151), MeshQueryResult pairs
+(pierce/common/include/common.h:34-37) and the four query applications'
+predicates (pierce/query/src/applications/*_main.cpp).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libpierce_hip.so")
+BIN_DIR = os.path.join(_HERE, "bin")
+
+# ---------------------------------------------------------------- constants
+PZ_OK = 0
+PZ_E_INVALID, PZ_E_HIP, PZ_E_IO, PZ_E_OVERFLOW, PZ_E_NOMEM, PZ_E_NO_EDGES, PZ_E_LIMIT, PZ_E_NODEVICE = (
+    -1, -2, -3, -4, -5, -6, -7, -8)
+
+GEN_ELLIPSOIDS, GEN_SPHERES_MIXED, GEN_CUBES, GEN_TUBES = 0, 1, 2, 3
+EDGES_NONE, EDGES_GROUPED, EDGES_PRE = 0, 1, 2
+
+PRED_OVERLAP, PRED_INTERSECTION, PRED_CONTAINMENT = 0, 1, 2
+DIR_BOTH, DIR_1TO2, DIR_2TO1 = 0, 1, 2
+STRATEGY_TWO_LEVEL, STRATEGY_GLOBAL = 0, 1
+PREC_FP32_FILTERED, PREC_FP64 = 0, 1
+SIZING_TWO_PASS, SIZING_ESTIMATE, SIZING_LEARNED, SIZING_FIXED = 0, 1, 2, 3
+DEDUP_SORT, DEDUP_HASH = 0, 1
+MAX_PHASES = 24
+
+
+class PierceError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"pierce_hip error {code}: {msg}")
+        self.code = code
+
+
+# ---------------------------------------------------------------- structs
+class GridCell(C.Structure):
+    _fields_ = [("ix", C.c_int32), ("iy", C.c_int32), ("iz", C.c_int32), ("center_count", C.c_uint32),
+                ("touch_count", C.c_uint32), ("avg_size_mean", C.c_float), ("vol_ratio", C.c_float)]
+
+
+GRID_DTYPE = np.dtype([("ix", "<i4"), ("iy", "<i4"), ("iz", "<i4"), ("center_count", "<u4"),
+                       ("touch_count", "<u4"), ("avg_size_mean", "<f4"), ("vol_ratio", "<f4")])
+
+
+class MeshView(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("num_vertices", C.c_uint64), ("indices", C.c_void_p),
+                ("num_triangles", C.c_uint64), ("triangle_to_object", C.c_void_p), ("edge_starts", C.c_void_p),
+                ("edge_ends", C.c_void_p), ("edge_objects", C.c_void_p), ("num_edges", C.c_uint64),
+                ("grid_cells", C.c_void_p), ("num_grid_cells", C.c_uint64), ("grid_cell_size", C.c_float)]
+
+
+class GenParams(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("edges_mode", C.c_int32), ("count", C.c_uint64), ("seed", C.c_uint64),
+                ("level_min", C.c_int32), ("level_max", C.c_int32), ("radius_min", C.c_float),
+                ("radius_max", C.c_float), ("box", C.c_float), ("jitter", C.c_float), ("copy_fraction", C.c_float),
+                ("copy_scale_min", C.c_float), ("copy_scale_max", C.c_float), ("copy_source_count", C.c_uint64),
+                ("copy_source_seed", C.c_uint64), ("tri_median", C.c_int64), ("tri_sigma", C.c_float),
+                ("tri_min", C.c_int64), ("tri_max", C.c_int64), ("ring_segments", C.c_int32), ("step", C.c_float),
+                ("branch_prob", C.c_float), ("max_branches", C.c_int32), ("degenerate_fraction", C.c_float),
+                ("reserved", C.c_int32 * 4)]
+
+
+class JoinDesc(C.Structure):
+    _fields_ = [("predicate", C.c_int32), ("direction", C.c_int32), ("strategy", C.c_int32),
+                ("precision", C.c_int32), ("sizing", C.c_int32), ("dedup", C.c_int32),
+                ("include_overlap_pairs", C.c_int32), ("max_hits_per_edge", C.c_int32), ("capacity", C.c_uint64),
+                ("load_factor", C.c_float), ("margin", C.c_float), ("gamma", C.c_float), ("epsilon", C.c_float),
+                ("no_alpha_correction", C.c_int32), ("keep_device_result", C.c_int32), ("model", C.c_void_p),
+                ("reserved", C.c_int32 * 8)]
+
+
+class Phase(C.Structure):
+    _fields_ = [("name", C.c_char * 48), ("ms", C.c_double)]
+
+
+class JoinResult(C.Structure):
+    _fields_ = [("num_pairs", C.c_uint64), ("pairs", C.c_void_p), ("pair_types", C.c_void_p),
+                ("num_candidates", C.c_uint64), ("capacity", C.c_uint64), ("predicted", C.c_uint64),
+                ("overflowed", C.c_int32), ("num_phases", C.c_int32), ("tests_total", C.c_uint64),
+                ("tests_fp64", C.c_uint64), ("tests_exact", C.c_uint64), ("tests_fp32_misclassified", C.c_uint64),
+                ("edges_traced", C.c_uint64), ("stack_overflows", C.c_uint64), ("ms_total", C.c_double),
+                ("phases", Phase * MAX_PHASES)]
+
+
+# Every function declared in include/pierce_hip.h: (name, restype, argtypes).
+_SIGS = [
+    ("pz_last_error", C.c_char_p, []),
+    ("pz_host_mesh_view", C.c_int32, [C.c_void_p, C.POINTER(MeshView)]),
+    ("pz_host_mesh_free", None, [C.c_void_p]),
+    ("pz_pre_read", C.c_int32, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    ("pz_pre_write", C.c_int32, [C.c_char_p, C.POINTER(MeshView)]),
+    ("pz_host_mesh_from_view", C.c_int32, [C.POINTER(MeshView), C.POINTER(C.c_void_p)]),
+    ("pz_host_mesh_extract_edges", C.c_int32, [C.c_void_p]),
+    ("pz_host_mesh_compute_grid", C.c_int32, [C.c_void_p, C.c_float]),
+    ("pz_gen_defaults", None, [C.c_int32, C.POINTER(GenParams)]),
+    ("pz_generate", C.c_int32, [C.POINTER(GenParams), C.POINTER(C.c_void_p)]),
+    ("pz_device_count", C.c_int32, []),
+    ("pz_ctx_create", C.c_int32, [C.c_int32, C.POINTER(C.c_void_p)]),
+    ("pz_ctx_destroy", None, [C.c_void_p]),
+    ("pz_dataset_upload", C.c_int32, [C.c_void_p, C.POINTER(MeshView), C.POINTER(C.c_void_p)]),
+    ("pz_dataset_load_pre", C.c_int32, [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p)]),
+    ("pz_dataset_release", None, [C.c_void_p]),
+    ("pz_dataset_info", C.c_int32, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    ("pz_join_desc_defaults", None, [C.POINTER(JoinDesc)]),
+    ("pz_join", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(JoinDesc), C.POINTER(JoinResult)]),
+    ("pz_join_result_release", None, [C.POINTER(JoinResult)]),
+    ("pz_count_candidate_tests", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]),
+    ("merge_and_deduplicate_pairs_gpu", C.c_longlong,
+     [C.c_void_p, C.c_longlong, C.c_void_p, C.c_longlong, C.c_void_p]),
+    ("compact_hash_table_pairs", C.c_int, [C.c_void_p, C.c_ulonglong, C.c_void_p, C.c_int]),
+    ("exclusive_scan_gpu", C.c_longlong, [C.c_void_p, C.c_void_p, C.c_int]),
+    ("pz_estimate_overlap_selectivity", C.c_float,
+     [C.POINTER(GridCell), C.POINTER(GridCell), C.c_int32, C.c_float, C.c_float, C.c_float]),
+]
+
+_lib: Optional[C.CDLL] = None
+
+
+def lib() -> C.CDLL:
+    """Load libpierce_hip.so (built by __graft_entry__.build()); raise if absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PierceError(PZ_E_INVALID, f"native library not built: {LIB_PATH}")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in _SIGS:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols() -> list:
+    return [n for n, _, _ in _SIGS]
+
+
+def _check(st: int) -> None:
+    if st != PZ_OK:
+        msg = lib().pz_last_error()
+        raise PierceError(st, msg.decode() if msg else "")
+
+
+def _ptr(a: Optional[np.ndarray]) -> Optional[int]:
+    return None if a is None or a.size == 0 else a.ctypes.data
+
+
+# ---------------------------------------------------------------- host meshes
+@dataclass
+class Mesh:
+    """One dataset's .pre payload as numpy arrays (AoS, as in the file)."""
+    vertices: np.ndarray            # (nV, 3) float32
+    indices: np.ndarray             # (nT, 3) uint32
+    triangle_to_object: np.ndarray  # (nT,) int32
+    edge_starts: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), np.float32))
+    edge_ends: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), np.float32))
+    edge_objects: np.ndarray = field(default_factory=lambda: np.zeros((0,), np.int32))
+    grid: np.ndarray = field(default_factory=lambda: np.zeros((0,), GRID_DTYPE))
+    grid_cell_size: float = 0.0
+
+    def __post_init__(self):
+        self.vertices = np.ascontiguousarray(self.vertices, np.float32).reshape(-1, 3)
+        self.indices = np.ascontiguousarray(self.indices, np.uint32).reshape(-1, 3)
+        self.triangle_to_object = np.ascontiguousarray(self.triangle_to_object, np.int32).reshape(-1)
+        self.edge_starts = np.ascontiguousarray(self.edge_starts, np.float32).reshape(-1, 3)
+        self.edge_ends = np.ascontiguousarray(self.edge_ends, np.float32).reshape(-1, 3)
+        self.edge_objects = np.ascontiguousarray(self.edge_objects, np.int32).reshape(-1)
+        self.grid = np.ascontiguousarray(self.grid, GRID_DTYPE).reshape(-1)
+
+    @property
+    def num_objects(self) -> int:
+        m = -1
+        if self.triangle_to_object.size:
+            m = max(m, int(self.triangle_to_object.max()))
+        if self.edge_objects.size:
+            m = max(m, int(self.edge_objects.max()))
+        return m + 1
+
+    def view(self) -> MeshView:
+        v = MeshView()
+        v.vertices = _ptr(self.vertices)
+        v.num_vertices = self.vertices.shape[0]
+        v.indices = _ptr(self.indices)
+        v.num_triangles = self.indices.shape[0]
+        v.triangle_to_object = _ptr(self.triangle_to_object)
+        v.edge_starts = _ptr(self.edge_starts)
+        v.edge_ends = _ptr(self.edge_ends)
+        v.edge_objects = _ptr(self.edge_objects)
+        v.num_edges = self.edge_objects.shape[0]
+        v.grid_cells = _ptr(self.grid)
+        v.num_grid_cells = self.grid.shape[0]
+        v.grid_cell_size = float(self.grid_cell_size)
+        return v
+
+
+def _mesh_from_handle(h: C.c_void_p) -> Mesh:
+    L = lib()
+    v = MeshView()
+    _check(L.pz_host_mesh_view(h, C.byref(v)))
+
+    def arr(p, n, dt, shape):
+        if n == 0 or not p:
+            return np.zeros(tuple(0 if x == -1 else x for x in shape), dt)
+        buf = (C.c_char * (n * np.dtype(dt).itemsize)).from_address(p)
+        return np.frombuffer(buf, dt).reshape(shape).copy()
+
+    m = Mesh(arr(v.vertices, 3 * v.num_vertices, np.float32, (-1, 3)),
+             arr(v.indices, 3 * v.num_triangles, np.uint32, (-1, 3)),
+             arr(v.triangle_to_object, v.num_triangles, np.int32, (-1,)),
+             arr(v.edge_starts, 3 * v.num_edges, np.float32, (-1, 3)),
+             arr(v.edge_ends, 3 * v.num_edges, np.float32, (-1, 3)),
+             arr(v.edge_objects, v.num_edges, np.int32, (-1,)),
+             arr(v.grid_cells, v.num_grid_cells, GRID_DTYPE, (-1,)),
+             float(v.grid_cell_size))
+    L.pz_host_mesh_free(h)
+    return m
+
+
+def gen_params(kind: int = GEN_ELLIPSOIDS, **kw) -> GenParams:
+    p = GenParams()
+    lib().pz_gen_defaults(kind, C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def generate(kind: int = GEN_ELLIPSOIDS, **kw) -> Mesh:
+    """Deterministic synthetic dataset (see pz_gen_params in include/pierce_hip.h)."""
+    p = gen_params(kind, **kw)
+    h = C.c_void_p()
+    _check(lib().pz_generate(C.byref(p), C.byref(h)))
+    return _mesh_from_handle(h)
+
+
+def read_pre(path: str) -> Mesh:
+    h = C.c_void_p()
+    _check(lib().pz_pre_read(path.encode(), C.byref(h)))
+    return _mesh_from_handle(h)
+
+
+def write_pre(path: str, mesh: Mesh) -> None:
+    v = mesh.view()
+    _check(lib().pz_pre_write(path.encode(), C.byref(v)))
+
+
+def _with_handle(mesh: Mesh, fn) -> Mesh:
+    v = mesh.view()
+    h = C.c_void_p()
+    _check(lib().pz_host_mesh_from_view(C.byref(v), C.byref(h)))
+    try:
+        _check(fn(h))
+    except Exception:
+        lib().pz_host_mesh_free(h)
+        raise
+    return _mesh_from_handle(h)
+
+
+def extract_edges(mesh: Mesh) -> Mesh:
+    """EdgePreprocessor::extractEdges semantics (pierce/preprocess/src/EdgePreprocessor.cpp:44-111)."""
+    return _with_handle(mesh, lambda h: lib().pz_host_mesh_extract_edges(h))
+
+
+def compute_grid(mesh: Mesh, cell_size: float) -> Mesh:
+    """generateGridStats semantics (pierce/preprocess/src/pierce_preprocess.cpp:49-167)."""
+    return _with_handle(mesh, lambda h: lib().pz_host_mesh_compute_grid(h, C.c_float(cell_size)))
+
+
+def estimate_overlap_selectivity(cells_a: np.ndarray, cells_b: np.ndarray, cell_volume: float,
+                                 epsilon: float = 0.001, gamma: float = 0.8) -> float:
+    a = np.ascontiguousarray(cells_a, GRID_DTYPE)
+    b = np.ascontiguousarray(cells_b, GRID_DTYPE)
+    return float(lib().pz_estimate_overlap_selectivity(
+        a.ctypes.data_as(C.POINTER(GridCell)), b.ctypes.data_as(C.POINTER(GridCell)), len(a),
+        cell_volume, epsilon, gamma))
+
+
+# ---------------------------------------------------------------- device
+def device_count() -> int:
+    return int(lib().pz_device_count())
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        _check(lib().pz_ctx_create(device, C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().pz_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, mesh: Mesh) -> "Dataset":
+        return Dataset(self, mesh)
+
+    def load_pre(self, path: str) -> "Dataset":
+        return Dataset(self, None, path)
+
+
+class Dataset:
+    def __init__(self, ctx: Context, mesh: Optional[Mesh], path: Optional[str] = None):
+        self.ctx = ctx
+        self._h = C.c_void_p()
+        if mesh is not None:
+            v = mesh.view()
+            _check(lib().pz_dataset_upload(ctx._h, C.byref(v), C.byref(self._h)))
+        else:
+            _check(lib().pz_dataset_load_pre(ctx._h, path.encode(), C.byref(self._h)))
+
+    def info(self) -> dict:
+        out = (C.c_uint64 * 5)()
+        _check(lib().pz_dataset_info(self._h, out))
+        return dict(vertices=out[0], triangles=out[1], edges=out[2], objects=out[3], grid_cells=out[4])
+
+    def close(self):
+        if self._h:
+            lib().pz_dataset_release(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class JoinOutput:
+    pairs: np.ndarray          # (n, 2) int32 sorted by (mesh1, mesh2)
+    types: Optional[np.ndarray]
+    num_candidates: int
+    capacity: int
+    predicted: int
+    overflowed: bool
+    tests_total: int
+    tests_fp64: int
+    tests_exact: int
+    tests_fp32_misclassified: int
+    edges_traced: int
+    ms_total: float
+    phases: list
+
+
+def join_desc(**kw) -> JoinDesc:
+    d = JoinDesc()
+    lib().pz_join_desc_defaults(C.byref(d))
+    for k, v in kw.items():
+        if not hasattr(d, k):
+            raise KeyError(k)
+        setattr(d, k, v)
+    return d
+
+
+def join(ctx: Context, a: Dataset, b: Dataset, **kw) -> JoinOutput:
+    """Run one join on the GPU; keyword arguments set pz_join_desc fields."""
+    d = join_desc(**kw)
+    r = JoinResult()
+    _check(lib().pz_join(ctx._h, a._h, b._h, C.byref(d), C.byref(r)))
+    try:
+        n = int(r.num_pairs)
+        if n:
+            buf = (C.c_int32 * (2 * n)).from_address(r.pairs)
+            pairs = np.frombuffer(buf, np.int32).reshape(n, 2).copy()
+        else:
+            pairs = np.zeros((0, 2), np.int32)
+        types = None
+        if r.pair_types:
+            types = np.frombuffer((C.c_uint8 * max(n, 1)).from_address(r.pair_types), np.uint8)[:n].copy()
+        phases = [(r.phases[i].name.decode(), r.phases[i].ms) for i in range(r.num_phases)]
+        return JoinOutput(pairs, types, int(r.num_candidates), int(r.capacity), int(r.predicted), bool(r.overflowed),
+                          int(r.tests_total), int(r.tests_fp64), int(r.tests_exact),
+                          int(r.tests_fp32_misclassified), int(r.edges_traced), float(r.ms_total), phases)
+    finally:
+        lib().pz_join_result_release(C.byref(r))
+
+
+def count_candidate_tests(ctx: Context, a: Dataset, b: Dataset) -> int:
+    out = C.c_uint64()
+    _check(lib().pz_count_candidate_tests(ctx._h, a._h, b._h, C.byref(out)))
+    return int(out.value)

@@ -1,0 +1,925 @@
+// C ABI implementation: context, datasets in HBM, and the join pipelines.
+//
+// Pipelines (all device work on the context stream; the host waits only where
+// a size must be known to launch the next step):
+//   two-level : object stats -> mesh2 object LBVH -> broad phase (two-pass or
+//               estimate-sized append with exact recount on overflow) ->
+//               per-object LBVHs + grouped edges of the objects in some pair ->
+//               narrow phase with early exit -> parity -> select -> sort.
+//   global    : set-wide triangle LBVH per set -> every edge of one set traced
+//               through the other's BVH (reference shape,
+//               pierce/query/src/cuda/mesh_overlap_edges.cu:105-208) -> two-pass
+//               count/scan/write/sort/unique
+//               (pierce/query/src/applications/pierce_overlap_two_pass_main.cpp:39-156)
+//               or hash-set dedup (pierce/query/src/applications/pierce_overlap_main.cpp:74-187).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "host/host_mesh.hpp"
+#include "kernels/pz_kernels_join.h"
+#include "pierce_hip.h"
+#include "pz_internal.h"
+
+using pz::fail;
+
+// ------------------------------------------------------------------ context
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+};
+
+struct pz_ctx {
+    int device = 0;
+    hipStream_t s = nullptr;
+    std::unordered_map<std::string, DevBuf> bufs;
+    void* pinned = nullptr;  // small readbacks
+    size_t pinned_n = 0;
+    hipEvent_t ev[PZ_MAX_PHASES + 1];
+};
+
+struct pz_dataset {
+    pz_ctx* ctx = nullptr;
+    float* v = nullptr;
+    uint32_t* idx = nullptr;
+    int32_t* t2o = nullptr;
+    float* es = nullptr;
+    float* ee = nullptr;
+    int32_t* eo = nullptr;
+    uint64_t nv = 0, nt = 0, ne = 0;
+    int32_t nobj = 0;
+    float max_abs = 0.f;
+    std::vector<pz_grid_cell> grid;
+    float cell_size = 0.f;
+};
+
+namespace {
+
+#define HIP_OK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            throw PzError(e_ == hipErrorOutOfMemory ? PZ_E_NOMEM : PZ_E_HIP,                           \
+                          std::string(#expr) + ": " + hipGetErrorString(e_));                          \
+    } while (0)
+
+struct PzError {
+    pz_status code;
+    std::string msg;
+    PzError(pz_status c, std::string m) : code(c), msg(std::move(m)) {}
+};
+
+template <class T>
+T* buf(pz_ctx* c, const std::string& name, size_t count) {
+    size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+    DevBuf& b = c->bufs[name];
+    if (b.n < bytes) {
+        if (b.p) HIP_OK(hipFree(b.p));
+        b.p = nullptr;
+        b.n = 0;
+        size_t want = bytes + bytes / 4;
+        HIP_OK(hipMalloc(&b.p, want));
+        b.n = want;
+    }
+    return reinterpret_cast<T*>(b.p);
+}
+
+void* pinned(pz_ctx* c, size_t bytes) {
+    if (c->pinned_n < bytes) {
+        if (c->pinned) HIP_OK(hipHostFree(c->pinned));
+        c->pinned = nullptr;
+        HIP_OK(hipHostMalloc(&c->pinned, bytes, hipHostMallocDefault));
+        c->pinned_n = bytes;
+    }
+    return c->pinned;
+}
+
+// Read n scalars (each from a device address) in one synchronisation.
+template <class T>
+std::vector<T> readback(pz_ctx* c, std::initializer_list<const T*> srcs) {
+    T* h = reinterpret_cast<T*>(pinned(c, srcs.size() * sizeof(T)));
+    size_t i = 0;
+    for (const T* p : srcs) HIP_OK(hipMemcpyAsync(h + i++, p, sizeof(T), hipMemcpyDeviceToHost, c->s));
+    HIP_OK(hipStreamSynchronize(c->s));
+    return std::vector<T>(h, h + srcs.size());
+}
+
+int bits_for(uint64_t n) {  // bits needed to hold values < n
+    int b = 0;
+    while (b < 64 && (1ull << b) < n) ++b;
+    return std::max(b, 1);
+}
+
+// Phase timer: events recorded on the stream at phase boundaries.
+struct Phases {
+    pz_ctx* c;
+    pz_join_result* r;
+    int n = 0;
+    explicit Phases(pz_ctx* ctx, pz_join_result* res) : c(ctx), r(res) {}
+    void begin(const char* name) {
+        if (n >= PZ_MAX_PHASES) return;
+        std::strncpy(r->phases[n].name, name, sizeof(r->phases[n].name) - 1);
+        HIP_OK(hipEventRecord(c->ev[n], c->s));
+        ++n;
+    }
+    void finish() {
+        if (n == 0) return;
+        HIP_OK(hipEventRecord(c->ev[n], c->s));
+        HIP_OK(hipEventSynchronize(c->ev[n]));
+        for (int i = 0; i < n; ++i) {
+            float ms = 0.f;
+            HIP_OK(hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
+            r->phases[i].ms = ms;
+        }
+        r->num_phases = n;
+    }
+};
+
+// ------------------------------------------------------------- set prep
+struct SetPrep {  // per-object statistics of one set (device)
+    uint32_t* obox;
+    uint32_t* tri_cnt;
+    uint32_t* first_tri;
+    uint32_t* edge_cnt;
+};
+
+SetPrep prep_set(pz_ctx* c, const pz_dataset* d, const std::string& tag) {
+    SetPrep p;
+    size_t n = std::max<int32_t>(d->nobj, 1);
+    p.obox = buf<uint32_t>(c, tag + ".obox", 6 * n);
+    p.tri_cnt = buf<uint32_t>(c, tag + ".tri_cnt", n);
+    p.first_tri = buf<uint32_t>(c, tag + ".first_tri", n);
+    p.edge_cnt = buf<uint32_t>(c, tag + ".edge_cnt", n);
+    pzk::object_stats(d->v, d->nv, d->idx, d->t2o, d->nt, d->nobj, p.obox, p.tri_cnt, p.first_tri, c->s);
+    pzk::edge_count(d->eo, d->ne, d->nobj, p.edge_cnt, c->s);
+    return p;
+}
+
+// LBVH over object boxes of one set (broad-phase target).
+struct ObjBvh {
+    pzd::Node* nodes;
+    uint32_t* order;
+    uint32_t* height;
+};
+
+ObjBvh build_object_bvh(pz_ctx* c, const pz_dataset* d, const SetPrep& p, const std::string& tag) {
+    const uint64_t n = std::max<int32_t>(d->nobj, 0);
+    ObjBvh b;
+    uint32_t* scene = buf<uint32_t>(c, tag + ".scene", 8);
+    uint64_t* keys = buf<uint64_t>(c, tag + ".okeys", n + 1);
+    uint32_t* vals = buf<uint32_t>(c, tag + ".ovals", n + 1);
+    uint64_t* keys2 = buf<uint64_t>(c, tag + ".okeys2", n + 1);
+    uint32_t* vals2 = buf<uint32_t>(c, tag + ".ovals2", n + 1);
+    b.nodes = buf<pzd::Node>(c, tag + ".onodes", n + 1);
+    uint32_t* pint = buf<uint32_t>(c, tag + ".opint", n + 1);
+    uint32_t* pleaf = buf<uint32_t>(c, tag + ".opleaf", n + 1);
+    uint32_t* cnt = buf<uint32_t>(c, tag + ".ocnt", n + 1);
+    b.height = buf<uint32_t>(c, tag + ".oheight", 1);
+    void* tmp = buf<char>(c, "sort.tmp.obj", pzk::radix_sort_temp_bytes(n));
+    pzk::scene_box(p.obox, d->nobj, scene, c->s);
+    pzk::object_keys(p.obox, d->nobj, scene, keys, vals, c->s);
+    bool alt = pzk::radix_sort_pairs(keys, vals, keys2, vals2, n, 0, 30, tmp, c->s);
+    uint64_t* sk = alt ? keys2 : keys;
+    b.order = alt ? vals2 : vals;
+    pzk::fill_u32(b.height, 0u, 1, c->s);
+    pzk::lbvh_karras(sk, n, nullptr, 0, b.nodes, pint, pleaf, cnt, c->s);
+    pzk::lbvh_refit_ord_boxes(sk, b.order, n, nullptr, 0, p.obox, b.nodes, pint, pleaf, cnt, b.height, c->s);
+    return b;
+}
+
+// Per-object LBVHs and grouped edges of the needed objects of one set.
+struct Needed {
+    uint32_t* need;
+    uint32_t* loc;
+    uint32_t* seg_off;
+    uint32_t* eseg_off;
+    int32_t* obj_of_loc;
+};
+
+Needed needed_layout(pz_ctx* c, const pz_dataset* d, const SetPrep& p, const std::string& tag) {
+    Needed nd;
+    size_t n = std::max<int32_t>(d->nobj, 1);
+    nd.need = buf<uint32_t>(c, tag + ".need", n);
+    nd.loc = buf<uint32_t>(c, tag + ".loc", n + 1);
+    nd.seg_off = buf<uint32_t>(c, tag + ".seg_off", n + 1);
+    nd.eseg_off = buf<uint32_t>(c, tag + ".eseg_off", n + 1);
+    nd.obj_of_loc = buf<int32_t>(c, tag + ".obj_of_loc", n);
+    return nd;
+}
+
+struct SetAccel {
+    pzd::Node* nodes;
+    pzd::LeafTri* leaves;
+    pzd::EdgeRec* edges;
+    uint32_t* height;
+};
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void init_result(pz_join_result* r) {
+    std::memset(r, 0, sizeof(*r));
+}
+
+// Final stage shared by both strategies: sort unique keys (+types), copy to host.
+void finish_pairs(pz_ctx* c, uint64_t* keys, uint32_t* types, uint64_t n, int32_t nA, bool want_types,
+                  pz_join_result* r) {
+    uint64_t* keys2 = buf<uint64_t>(c, "fin.keys2", n + 1);
+    uint32_t* types2 = buf<uint32_t>(c, "fin.types2", n + 1);
+    void* tmp = buf<char>(c, "sort.tmp.fin", pzk::radix_sort_temp_bytes(n));
+    int bits = 32 + bits_for((uint64_t)std::max<int32_t>(nA, 1));
+    bool alt = types ? pzk::radix_sort_pairs(keys, types, keys2, types2, n, 0, bits, tmp, c->s)
+                     : pzk::radix_sort_keys(keys, keys2, n, 0, bits, tmp, c->s);
+    uint64_t* sk = alt ? keys2 : keys;
+    uint32_t* st = alt ? types2 : types;
+    int32_t* dpairs = buf<int32_t>(c, "fin.pairs", 2 * n + 2);
+    pzk::keys_to_pairs(sk, n, dpairs, c->s);
+    r->num_pairs = n;
+    r->pairs = static_cast<pz_pair*>(std::malloc(std::max<uint64_t>(n, 1) * sizeof(pz_pair)));
+    if (!r->pairs) throw PzError(PZ_E_NOMEM, "host result allocation failed");
+    if (n) HIP_OK(hipMemcpyAsync(r->pairs, dpairs, n * sizeof(pz_pair), hipMemcpyDeviceToHost, c->s));
+    std::vector<uint32_t> ht;
+    if (want_types && n) {
+        ht.resize(n);
+        HIP_OK(hipMemcpyAsync(ht.data(), st, n * 4, hipMemcpyDeviceToHost, c->s));
+    }
+    HIP_OK(hipStreamSynchronize(c->s));
+    if (want_types) {
+        r->pair_types = static_cast<uint8_t*>(std::malloc(std::max<uint64_t>(n, 1)));
+        if (!r->pair_types) throw PzError(PZ_E_NOMEM, "host result allocation failed");
+        for (uint64_t i = 0; i < n; ++i) r->pair_types[i] = static_cast<uint8_t>(ht[i]);
+    }
+}
+
+// ------------------------------------------------------------- estimates
+// Grid estimate of pierce/query/src/applications/pierce_overlap_main.cpp:376-471 with the
+// per-cell formula of pierce/query/src/cuda/estimated_overlap.cu:29-40, summed in double.
+double grid_estimate(const pz_dataset* a, const pz_dataset* b, float gamma, float eps, bool alpha_corr) {
+    if (a->grid.empty() || b->grid.empty() || !(a->cell_size > 0)) return -1.0;
+    struct K {
+        int32_t x, y, z;
+        bool operator<(const K& o) const { return std::tie(x, y, z) < std::tie(o.x, o.y, o.z); }
+    };
+    std::map<K, const pz_grid_cell*> ma;
+    for (const auto& g : a->grid) ma[K{g.ix, g.iy, g.iz}] = &g;
+    std::vector<pz_grid_cell> A, B;
+    for (const auto& g : b->grid) {
+        auto it = ma.find(K{g.ix, g.iy, g.iz});
+        if (it != ma.end()) {
+            A.push_back(*it->second);
+            B.push_back(g);
+        }
+    }
+    float cs = a->cell_size;
+    float cell_vol = cs * cs * cs;
+    double raw = pz_estimate_overlap_selectivity(A.data(), B.data(), (int32_t)A.size(), cell_vol, eps, gamma);
+    auto avg = [](const std::vector<pz_grid_cell>& g, bool size) {
+        double t = 0;
+        long long n = 0;
+        for (const auto& c : g)
+            if (c.touch_count > 0) {
+                t += (double)(size ? c.avg_size_mean : c.vol_ratio) * c.touch_count;
+                n += c.touch_count;
+            }
+        return n ? (float)(t / n) : (size ? 0.0f : 1.0f);
+    };
+    float e1 = avg(a->grid, true) * std::cbrt(avg(a->grid, false));
+    float e2 = avg(b->grid, true) * std::cbrt(avg(b->grid, false));
+    float comb = e1 + e2;
+    float mink = comb * comb * comb;
+    if (cell_vol < 1e-9f) cell_vol = 1e-9f;
+    float alpha = 1.0f;
+    if (alpha_corr) alpha = std::max(1.0f, mink / cell_vol);
+    return raw / alpha;
+}
+
+// ------------------------------------------------------------- two-level
+void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const pz_join_desc& d, pz_join_result* r,
+                    Phases& ph) {
+    hipStream_t s = c->s;
+    unsigned long long* stats = buf<unsigned long long>(c, "stats", 8);
+    pzk::fill_ull(stats, 0ull, 8, s);
+    const int32_t nA = A->nobj, nB = B->nobj;
+
+    ph.begin("Object Bounds");
+    SetPrep pa = prep_set(c, A, "A"), pb = prep_set(c, B, "B");
+
+    ph.begin("Build Object Index");
+    ObjBvh ob = build_object_bvh(c, B, pb, "B");
+
+    ph.begin("Broad Phase");
+    // The broad-phase walk keeps a 64-entry stack: the object LBVH must be shallower.
+    const uint32_t oheight = readback<uint32_t>(c, {ob.height})[0];
+    if (oheight >= 64)
+        throw PzError(PZ_E_LIMIT, "object BVH height " + std::to_string(oheight) + " exceeds the broad-phase stack (63)");
+    uint64_t np = 0;
+    uint64_t* pairs = nullptr;
+    auto two_pass = [&]() {
+        uint32_t* cnt = buf<uint32_t>(c, "bp.cnt", (size_t)std::max(nA, 1));
+        uint64_t* off = buf<uint64_t>(c, "bp.off", (size_t)std::max(nA, 1) + 1);
+        void* tmp = buf<char>(c, "scan.tmp", pzk::scan_temp_bytes((uint64_t)std::max(nA, 1)));
+        pzk::broad_count(pa.obox, nA, pb.obox, ob.nodes, ob.order, nB, cnt, s);
+        pzk::exclusive_scan_u32_u64(cnt, off, (uint64_t)std::max(nA, 0), tmp, s);
+        np = readback<uint64_t>(c, {off + std::max(nA, 0)})[0];
+        pairs = buf<uint64_t>(c, "bp.pairs", np + 1);
+        pzk::broad_write(pa.obox, nA, pb.obox, ob.nodes, ob.order, nB, off, pairs, s);
+    };
+    if (d.sizing == PZ_SIZING_TWO_PASS) {
+        two_pass();
+        r->capacity = np;
+    } else {
+        uint64_t cap = d.capacity;
+        if (d.sizing != PZ_SIZING_FIXED) {
+            double pred = (double)r->predicted;
+            float margin = d.margin > 0 ? d.margin : 0.25f;
+            cap = (uint64_t)std::ceil(pred * (1.0 + margin)) + 64;
+        }
+        cap = std::max<uint64_t>(cap, 1);
+        r->capacity = cap;
+        pairs = buf<uint64_t>(c, "bp.pairs", cap + 1);
+        unsigned long long* cursor = buf<unsigned long long>(c, "bp.cursor", 1);
+        pzk::fill_ull(cursor, 0ull, 1, s);
+        pzk::broad_append(pa.obox, nA, pb.obox, ob.nodes, ob.order, nB, pairs, cursor, cap, s);
+        np = readback<unsigned long long>(c, {cursor})[0];
+        if (np > cap) {
+            r->overflowed = 1;
+            two_pass();
+        }
+    }
+    r->num_candidates = np;
+
+    ph.begin("Build Mesh Index");
+    const int dir_mask = d.predicate == PZ_PRED_OVERLAP
+                             ? (d.direction == PZ_DIR_1TO2 ? 1 : (d.direction == PZ_DIR_2TO1 ? 2 : 3))
+                             : 3;
+    Needed na = needed_layout(c, A, pa, "A"), nb = needed_layout(c, B, pb, "B");
+    const size_t sA = std::max(nA, 1), sB = std::max(nB, 1);
+    pzk::fill_u32(na.need, 0u, sA, s);
+    pzk::fill_u32(nb.need, 0u, sB, s);
+    pzk::mark_needed(pairs, np, na.need, nb.need, s);
+    struct Tot {
+        uint64_t nneed, ntri, nedge;
+    };
+    auto layout = [&](const pz_dataset* D, const SetPrep& P, Needed& N, const std::string& tag) {
+        size_t n = std::max(D->nobj, 1);
+        uint32_t* stc = buf<uint32_t>(c, tag + ".seg_tri_cnt", n);
+        uint32_t* sec = buf<uint32_t>(c, tag + ".seg_edge_cnt", n);
+        pzk::fill_u32(stc, 0u, n, s);
+        pzk::fill_u32(sec, 0u, n, s);
+        void* tmp = buf<char>(c, "scan.tmp", pzk::scan_temp_bytes(n + 1));
+        pzk::exclusive_scan_u32_u32(N.need, N.loc, (uint64_t)std::max(D->nobj, 0), tmp, s);
+        pzk::needed_layout(N.need, N.loc, D->nobj, P.tri_cnt, P.edge_cnt, N.obj_of_loc, stc, sec, s);
+        pzk::exclusive_scan_u32_u32(stc, N.seg_off, (uint64_t)std::max(D->nobj, 0), tmp, s);
+        pzk::exclusive_scan_u32_u32(sec, N.eseg_off, (uint64_t)std::max(D->nobj, 0), tmp, s);
+    };
+    layout(A, pa, na, "A");
+    layout(B, pb, nb, "B");
+    auto tot = readback<uint32_t>(c, {na.loc + std::max(nA, 0), na.seg_off + std::max(nA, 0),
+                                      na.eseg_off + std::max(nA, 0), nb.loc + std::max(nB, 0),
+                                      nb.seg_off + std::max(nB, 0), nb.eseg_off + std::max(nB, 0)});
+    Tot ta{tot[0], tot[1], tot[2]}, tb{tot[3], tot[4], tot[5]};
+
+    auto build = [&](const pz_dataset* D, const SetPrep& P, const Needed& N, const Tot& T,
+                     const std::string& tag) -> SetAccel {
+        SetAccel acc;
+        const uint64_t n = T.ntri;
+        uint64_t* keys = buf<uint64_t>(c, tag + ".tkeys", n + 1);
+        uint32_t* vals = buf<uint32_t>(c, tag + ".tvals", n + 1);
+        uint64_t* keys2 = buf<uint64_t>(c, tag + ".tkeys2", n + 1);
+        uint32_t* vals2 = buf<uint32_t>(c, tag + ".tvals2", n + 1);
+        uint32_t* cursor = buf<uint32_t>(c, tag + ".cursor", (size_t)std::max(D->nobj, 1));
+        pzk::fill_u32(cursor, 0u, (size_t)std::max(D->nobj, 1), s);
+        pzk::tri_keys_two_level(D->v, D->nv, D->idx, D->t2o, D->nt, D->nobj, N.need, N.loc, P.obox, N.seg_off,
+                                cursor, keys, vals, s);
+        void* tmp = buf<char>(c, "sort.tmp." + tag, pzk::radix_sort_temp_bytes(n));
+        bool alt = pzk::radix_sort_pairs(keys, vals, keys2, vals2, n, 0, 30 + bits_for(std::max<uint64_t>(T.nneed, 1)),
+                                         tmp, s);
+        uint64_t* sk = alt ? keys2 : keys;
+        uint32_t* order = alt ? vals2 : vals;
+        acc.nodes = buf<pzd::Node>(c, tag + ".nodes", n + 1);
+        uint32_t* pint = buf<uint32_t>(c, tag + ".pint", n + 1);
+        uint32_t* pleaf = buf<uint32_t>(c, tag + ".pleaf", n + 1);
+        uint32_t* cnt = buf<uint32_t>(c, tag + ".ncnt", n + 1);
+        acc.height = buf<uint32_t>(c, tag + ".height", 1);
+        pzk::fill_u32(acc.height, 0u, 1, s);
+        pzk::lbvh_karras(sk, n, N.seg_off, 30, acc.nodes, pint, pleaf, cnt, s);
+        pzk::lbvh_refit_tris(sk, order, n, N.seg_off, 30, D->v, D->idx, acc.nodes, pint, pleaf, cnt, acc.height, s);
+        acc.leaves = buf<pzd::LeafTri>(c, tag + ".leaves", n + 1);
+        pzk::gather_leaf_tris(order, n, D->v, D->idx, D->t2o, acc.leaves, s);
+        acc.edges = buf<pzd::EdgeRec>(c, tag + ".edges", T.nedge + 1);
+        pzk::fill_u32(cursor, 0u, (size_t)std::max(D->nobj, 1), s);
+        pzk::edge_scatter(D->es, D->ee, D->eo, D->ne, D->nobj, N.need, N.loc, N.eseg_off, cursor, acc.edges, s);
+        return acc;
+    };
+    SetAccel xa = build(A, pa, na, ta, "A");
+    SetAccel xb = build(B, pb, nb, tb, "B");
+
+    ph.begin("Narrow Phase");
+    uint32_t* tcnt = buf<uint32_t>(c, "np.tcnt", np + 1);
+    uint64_t* toff = buf<uint64_t>(c, "np.toff", np + 2);
+    uint32_t* found = buf<uint32_t>(c, "np.found", np + 1);
+    void* tmp = buf<char>(c, "scan.tmp.np", pzk::scan_temp_bytes(np + 1));
+    pzk::task_counts(pairs, np, na.loc, na.eseg_off, nb.loc, nb.eseg_off, dir_mask, tcnt, s);
+    pzk::exclusive_scan_u32_u64(tcnt, toff, np, tmp, s);
+    pzk::fill_u32(found, 0u, np + 1, s);
+    auto hv = readback<uint64_t>(c, {toff + np});
+    auto hh = readback<uint32_t>(c, {xa.height, xb.height});
+    const uint64_t ntasks = hv[0];
+    const uint32_t height = std::max(hh[0], hh[1]);
+    if (pzk::stack_for_height(height) == 0)
+        throw PzError(PZ_E_LIMIT, "per-object BVH height " + std::to_string(height) + " exceeds the traversal stack (63)");
+    pzd::SetView va{xa.nodes, xa.leaves, na.seg_off, xa.edges, na.eseg_off, na.loc, pa.obox};
+    pzd::SetView vb{xb.nodes, xb.leaves, nb.seg_off, xb.edges, nb.eseg_off, nb.loc, pb.obox};
+    const bool f32 = d.precision == PZ_PREC_FP32_FILTERED;
+    const int allow_f32 = (std::max(A->max_abs, B->max_abs) <= 0x1p39f) ? 1 : 0;
+    pzk::narrow(pairs, np, toff, ntasks, va, vb, found, dir_mask, f32, allow_f32, height, stats, s);
+
+    uint32_t* inside = nullptr;
+    if (d.predicate != PZ_PRED_OVERLAP) {
+        ph.begin("Containment Parity");
+        inside = buf<uint32_t>(c, "np.inside", np + 1);
+        pzk::fill_u32(inside, 0u, np + 1, s);
+        pzk::parity(pairs, np, found, va, vb, A->v, A->idx, pa.first_tri, B->v, B->idx, pb.first_tri,
+                    d.predicate == PZ_PRED_INTERSECTION ? 1 : 0, inside, height, stats, s);
+    }
+
+    ph.begin("Download Results");
+    uint64_t* okeys = buf<uint64_t>(c, "sel.keys", np + 1);
+    uint32_t* otype = buf<uint32_t>(c, "sel.type", np + 1);
+    unsigned long long* ocnt = buf<unsigned long long>(c, "sel.cnt", 1);
+    pzk::fill_ull(ocnt, 0ull, 1, s);
+    pzk::select_pairs(pairs, np, found, inside, d.predicate, d.include_overlap_pairs, okeys, otype, ocnt, s);
+    uint64_t nout = readback<unsigned long long>(c, {ocnt})[0];
+    const bool want_types = d.predicate == PZ_PRED_CONTAINMENT && d.include_overlap_pairs;
+    finish_pairs(c, okeys, want_types ? otype : nullptr, nout, nA, want_types, r);
+    auto st = readback<unsigned long long>(c, {stats, stats + 1, stats + 2, stats + 3, stats + 4});
+    r->tests_total = st[0];
+    r->tests_fp64 = st[1];
+    r->tests_exact = st[2];
+    r->edges_traced = st[4];
+    r->tests_fp32_misclassified = st[3];
+}
+
+// ------------------------------------------------------------- global
+struct GlobalBvh {
+    pzd::Node* nodes;
+    pzd::LeafTri* leaves;
+    uint32_t height;
+};
+
+GlobalBvh build_global(pz_ctx* c, const pz_dataset* D, const SetPrep& P, const std::string& tag) {
+    hipStream_t s = c->s;
+    const uint64_t n = D->nt;
+    GlobalBvh g;
+    uint32_t* scene = buf<uint32_t>(c, tag + ".gscene", 8);
+    pzk::scene_box(P.obox, D->nobj, scene, s);
+    uint64_t* keys = buf<uint64_t>(c, tag + ".gkeys", n + 1);
+    uint32_t* vals = buf<uint32_t>(c, tag + ".gvals", n + 1);
+    uint64_t* keys2 = buf<uint64_t>(c, tag + ".gkeys2", n + 1);
+    uint32_t* vals2 = buf<uint32_t>(c, tag + ".gvals2", n + 1);
+    pzk::tri_keys_global(D->v, D->idx, n, scene, keys, vals, s);
+    void* tmp = buf<char>(c, "sort.tmp." + tag, pzk::radix_sort_temp_bytes(n));
+    bool alt = pzk::radix_sort_pairs(keys, vals, keys2, vals2, n, 0, 30, tmp, s);
+    uint64_t* sk = alt ? keys2 : keys;
+    uint32_t* order = alt ? vals2 : vals;
+    g.nodes = buf<pzd::Node>(c, tag + ".gnodes", n + 1);
+    uint32_t* pint = buf<uint32_t>(c, tag + ".gpint", n + 1);
+    uint32_t* pleaf = buf<uint32_t>(c, tag + ".gpleaf", n + 1);
+    uint32_t* cnt = buf<uint32_t>(c, tag + ".gcnt", n + 1);
+    uint32_t* h = buf<uint32_t>(c, tag + ".gheight", 1);
+    pzk::fill_u32(h, 0u, 1, s);
+    pzk::lbvh_karras(sk, n, nullptr, 0, g.nodes, pint, pleaf, cnt, s);
+    pzk::lbvh_refit_tris(sk, order, n, nullptr, 0, D->v, D->idx, g.nodes, pint, pleaf, cnt, h, s);
+    g.leaves = buf<pzd::LeafTri>(c, tag + ".gleaves", n + 1);
+    pzk::gather_leaf_tris(order, n, D->v, D->idx, D->t2o, g.leaves, s);
+    g.height = readback<uint32_t>(c, {h})[0];
+    if (pzk::stack_for_height(g.height) == 0)
+        throw PzError(PZ_E_LIMIT, "set-wide BVH height " + std::to_string(g.height) + " exceeds the traversal stack (63)");
+    return g;
+}
+
+void join_global(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const pz_join_desc& d, pz_join_result* r,
+                 Phases& ph) {
+    if (d.predicate != PZ_PRED_OVERLAP)
+        throw PzError(PZ_E_INVALID, "the global strategy evaluates the overlap predicate only");
+    hipStream_t s = c->s;
+    unsigned long long* stats = buf<unsigned long long>(c, "stats", 8);
+    pzk::fill_ull(stats, 0ull, 8, s);
+    ph.begin("Build Mesh1 Index");
+    SetPrep pa = prep_set(c, A, "A"), pb = prep_set(c, B, "B");
+    const bool do12 = d.direction != PZ_DIR_2TO1, do21 = d.direction != PZ_DIR_1TO2;
+    GlobalBvh ga{}, gb{};
+    if (do21) ga = build_global(c, A, pa, "A");
+    ph.begin("Build Mesh2 Index");
+    if (do12) gb = build_global(c, B, pb, "B");
+    const bool f32 = d.precision == PZ_PREC_FP32_FILTERED;
+    const int allow_f32 = (std::max(A->max_abs, B->max_abs) <= 0x1p39f) ? 1 : 0;
+    uint32_t* overflow = buf<uint32_t>(c, "g.overflow", 1);
+
+    auto run_two_pass = [&](uint64_t* out_n) -> uint64_t* {
+        uint64_t n1 = do12 ? A->ne : 0, n2 = do21 ? B->ne : 0;
+        uint32_t* cnt = buf<uint32_t>(c, "g.cnt", n1 + n2 + 1);
+        uint64_t* off = buf<uint64_t>(c, "g.off", n1 + n2 + 2);
+        void* tmp = buf<char>(c, "scan.tmp.g", pzk::scan_temp_bytes(n1 + n2 + 1));
+        ph.begin("Raytrace_Mesh1ToMesh2_Pass1");
+        if (do12)
+            pzk::trace_global(0, A->es, A->ee, A->eo, n1, gb.nodes, gb.leaves, B->nt, gb.height, 0, f32, allow_f32,
+                              cnt, nullptr, nullptr, nullptr, 0, overflow, stats, s);
+        ph.begin("Raytrace_Mesh2ToMesh1_Pass1");
+        if (do21)
+            pzk::trace_global(0, B->es, B->ee, B->eo, n2, ga.nodes, ga.leaves, A->nt, ga.height, 1, f32, allow_f32,
+                              cnt + n1, nullptr, nullptr, nullptr, 0, overflow, stats, s);
+        pzk::exclusive_scan_u32_u64(cnt, off, n1 + n2, tmp, s);
+        uint64_t total = readback<uint64_t>(c, {off + n1 + n2})[0];
+        uint64_t* keys = buf<uint64_t>(c, "g.keys", total + 1);
+        ph.begin("Raytrace_Mesh1ToMesh2_Pass2");
+        if (do12)
+            pzk::trace_global(1, A->es, A->ee, A->eo, n1, gb.nodes, gb.leaves, B->nt, gb.height, 0, f32, allow_f32,
+                              nullptr, off, keys, nullptr, 0, overflow, stats, s);
+        ph.begin("Raytrace_Mesh2ToMesh1_Pass2");
+        if (do21)
+            pzk::trace_global(1, B->es, B->ee, B->eo, n2, ga.nodes, ga.leaves, A->nt, ga.height, 1, f32, allow_f32,
+                              nullptr, off + n1, keys, nullptr, 0, overflow, stats, s);
+        *out_n = total;
+        return keys;
+    };
+
+    uint64_t raw = 0;
+    uint64_t* keys = nullptr;
+    uint64_t nuniq = 0;
+    uint64_t* uniq = nullptr;
+    bool done = false;
+    if (d.dedup == PZ_DEDUP_HASH || d.sizing != PZ_SIZING_TWO_PASS) {
+        uint64_t expected = d.sizing == PZ_SIZING_FIXED ? d.capacity : std::max<uint64_t>(r->predicted, 1);
+        float lf = d.load_factor > 0 ? d.load_factor : 0.5f;
+        uint64_t slots = 1024;
+        if (d.sizing == PZ_SIZING_FIXED)
+            slots = std::max<uint64_t>(d.capacity, 1);
+        else
+            slots = (uint64_t)std::ceil((double)expected / lf);
+        uint64_t cap2 = 1024;
+        while (cap2 < slots && cap2 < (1ull << 34)) cap2 <<= 1;
+        r->capacity = cap2;
+        unsigned long long* table = buf<unsigned long long>(c, "g.table", cap2);
+        pzk::fill_ull(table, ~0ull, cap2, s);
+        pzk::fill_u32(overflow, 0u, 1, s);
+        ph.begin("Raytrace_Hash_Mesh1ToMesh2");
+        if (do12)
+            pzk::trace_global(2, A->es, A->ee, A->eo, A->ne, gb.nodes, gb.leaves, B->nt, gb.height, 0, f32,
+                              allow_f32, nullptr, nullptr, nullptr, table, cap2 - 1, overflow, stats, s);
+        ph.begin("Raytrace_Hash_Mesh2ToMesh1");
+        if (do21)
+            pzk::trace_global(2, B->es, B->ee, B->eo, B->ne, ga.nodes, ga.leaves, A->nt, ga.height, 1, f32,
+                              allow_f32, nullptr, nullptr, nullptr, table, cap2 - 1, overflow, stats, s);
+        ph.begin("compact_hash_table_pairs");
+        unsigned long long* cnt = buf<unsigned long long>(c, "g.ccnt", 1);
+        pzk::fill_ull(cnt, 0ull, 1, s);
+        uniq = buf<uint64_t>(c, "g.uniq", cap2 + 1);
+        pzk::compact_table(table, cap2, uniq, cap2, cnt, s);
+        HIP_OK(hipMemcpyAsync(pinned(c, 16), overflow, 4, hipMemcpyDeviceToHost, s));
+        uint32_t ovf = 0;
+        unsigned long long cc = 0;
+        HIP_OK(hipStreamSynchronize(s));
+        std::memcpy(&ovf, c->pinned, 4);
+        cc = readback<unsigned long long>(c, {cnt})[0];
+        if (ovf) {
+            r->overflowed = 1;  // fall back to the exact two-pass count
+        } else {
+            nuniq = cc;
+            done = true;
+            r->num_candidates = cc;
+        }
+    }
+    if (!done) {
+        keys = run_two_pass(&raw);
+        r->num_candidates = raw;
+        ph.begin("gpu deduplication");
+        uint64_t* keys2 = buf<uint64_t>(c, "g.keys2", raw + 1);
+        void* tmp = buf<char>(c, "sort.tmp.g", pzk::radix_sort_temp_bytes(raw));
+        bool alt = pzk::radix_sort_keys(keys, keys2, raw, 0, 32 + bits_for((uint64_t)std::max(A->nobj, 1)), tmp, s);
+        uint64_t* sk = alt ? keys2 : keys;
+        uniq = buf<uint64_t>(c, "g.uniq", raw + 1);
+        unsigned long long* ucnt = buf<unsigned long long>(c, "g.ucnt", 1);
+        void* utmp = buf<char>(c, "uniq.tmp", pzk::unique_temp_bytes(raw));
+        pzk::unique_u64(sk, raw, reinterpret_cast<uint64_t*>(uniq), reinterpret_cast<uint64_t*>(ucnt), utmp, s);
+        nuniq = readback<unsigned long long>(c, {ucnt})[0];
+        if (d.sizing == PZ_SIZING_TWO_PASS) r->capacity = raw;
+    }
+    ph.begin("Download Results");
+    finish_pairs(c, uniq, nullptr, nuniq, A->nobj, false, r);
+    auto st = readback<unsigned long long>(c, {stats, stats + 1, stats + 2, stats + 3, stats + 4});
+    r->tests_total = st[0];
+    r->tests_fp64 = st[1];
+    r->tests_exact = st[2];
+    r->edges_traced = st[4];
+    r->tests_fp32_misclassified = st[3];
+}
+
+template <class F>
+pz_status guarded(F f) {
+    try {
+        return f();
+    } catch (const PzError& e) {
+        return fail(e.code, e.msg);
+    } catch (const std::bad_alloc&) {
+        return fail(PZ_E_NOMEM, "host allocation failed");
+    }
+}
+
+}  // namespace
+
+// ===================================================================== C ABI
+extern "C" {
+
+int32_t pz_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+pz_status pz_ctx_create(int32_t device, pz_ctx** out) {
+    if (!out) return fail(PZ_E_INVALID, "pz_ctx_create: null output");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(PZ_E_NODEVICE, "no HIP device available");
+    if (device < 0 || device >= n) return fail(PZ_E_INVALID, "device index out of range");
+    return guarded([&]() -> pz_status {
+        std::unique_ptr<pz_ctx> c(new pz_ctx());
+        c->device = device;
+        HIP_OK(hipSetDevice(device));
+        HIP_OK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+        for (auto& e : c->ev) HIP_OK(hipEventCreate(&e));
+        *out = c.release();
+        return PZ_OK;
+    });
+}
+
+void pz_ctx_destroy(pz_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->s);
+    for (auto& kv : c->bufs)
+        if (kv.second.p) (void)hipFree(kv.second.p);
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    for (auto& e : c->ev) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->s);
+    delete c;
+}
+
+pz_status pz_dataset_upload(pz_ctx* c, const pz_mesh_view* v, pz_dataset** out) {
+    if (!c || !v || !out) return fail(PZ_E_INVALID, "pz_dataset_upload: null argument");
+    if (v->num_edges == 0 || !v->edge_starts || !v->edge_ends || !v->edge_objects)
+        return fail(PZ_E_NO_EDGES,
+                    "dataset has no precomputed edges; re-run preprocessing with edge extraction enabled");
+    if (v->num_triangles >= (1ull << 31) || v->num_edges >= (1ull << 31))
+        return fail(PZ_E_LIMIT, "a set is limited to 2^31 - 1 triangles and edges");
+    if (v->num_triangles && (!v->indices || !v->triangle_to_object || !v->vertices))
+        return fail(PZ_E_INVALID, "null geometry array");
+    return guarded([&]() -> pz_status {
+        HIP_OK(hipSetDevice(c->device));
+        std::unique_ptr<pz_dataset> d(new pz_dataset());
+        d->ctx = c;
+        d->nv = v->num_vertices;
+        d->nt = v->num_triangles;
+        d->ne = v->num_edges;
+        int32_t mx = -1;
+        for (uint64_t t = 0; t < d->nt; ++t) mx = std::max(mx, v->triangle_to_object[t]);
+        for (uint64_t i = 0; i < 3 * d->nt; ++i)
+            if (v->indices[i] >= d->nv) throw PzError(PZ_E_INVALID, "triangle index beyond the vertex array");
+        for (uint64_t e = 0; e < d->ne; ++e) mx = std::max(mx, v->edge_objects[e]);
+        d->nobj = mx + 1;
+        float ma = 0.f;
+        for (uint64_t i = 0; i < 3 * d->nv; ++i) ma = std::max(ma, std::fabs(v->vertices[i]));
+        for (uint64_t i = 0; i < 3 * d->ne; ++i)
+            ma = std::max(ma, std::max(std::fabs(v->edge_starts[i]), std::fabs(v->edge_ends[i])));
+        d->max_abs = std::isfinite(ma) ? ma : INFINITY;
+        auto up = [&](void** dst, const void* src, size_t bytes) {
+            HIP_OK(hipMalloc(dst, std::max<size_t>(bytes, 16)));
+            if (bytes) HIP_OK(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+        };
+        try {
+            up((void**)&d->v, v->vertices, 12 * d->nv);
+            up((void**)&d->idx, v->indices, 12 * d->nt);
+            up((void**)&d->t2o, v->triangle_to_object, 4 * d->nt);
+            up((void**)&d->es, v->edge_starts, 12 * d->ne);
+            up((void**)&d->ee, v->edge_ends, 12 * d->ne);
+            up((void**)&d->eo, v->edge_objects, 4 * d->ne);
+        } catch (...) {
+            pz_dataset_release(d.release());
+            throw;
+        }
+        if (v->grid_cells && v->num_grid_cells) d->grid.assign(v->grid_cells, v->grid_cells + v->num_grid_cells);
+        d->cell_size = v->grid_cell_size;
+        *out = d.release();
+        return PZ_OK;
+    });
+}
+
+pz_status pz_dataset_load_pre(pz_ctx* c, const char* path, pz_dataset** out) {
+    pz_host_mesh* m = nullptr;
+    pz_status st = pz_pre_read(path, &m);
+    if (st != PZ_OK) return st;
+    pz_mesh_view v = m->view();
+    st = pz_dataset_upload(c, &v, out);
+    pz_host_mesh_free(m);
+    return st;
+}
+
+void pz_dataset_release(pz_dataset* d) {
+    if (!d) return;
+    for (void* p : {(void*)d->v, (void*)d->idx, (void*)d->t2o, (void*)d->es, (void*)d->ee, (void*)d->eo})
+        if (p) (void)hipFree(p);
+    delete d;
+}
+
+pz_status pz_dataset_info(const pz_dataset* d, uint64_t out[5]) {
+    if (!d || !out) return fail(PZ_E_INVALID, "pz_dataset_info: null argument");
+    out[0] = d->nv;
+    out[1] = d->nt;
+    out[2] = d->ne;
+    out[3] = (uint64_t)std::max(d->nobj, 0);
+    out[4] = d->grid.size();
+    return PZ_OK;
+}
+
+void pz_join_desc_defaults(pz_join_desc* d) {
+    std::memset(d, 0, sizeof(*d));
+    d->predicate = PZ_PRED_OVERLAP;
+    d->direction = PZ_DIR_BOTH;
+    d->strategy = PZ_STRATEGY_TWO_LEVEL;
+    d->precision = PZ_PREC_FP32_FILTERED;
+    d->sizing = PZ_SIZING_TWO_PASS;
+    d->dedup = PZ_DEDUP_SORT;
+    d->load_factor = 0.5f;
+    d->margin = 0.25f;
+    d->gamma = 0.8f;
+    d->epsilon = 0.001f;
+}
+
+pz_status pz_join(pz_ctx* c, const pz_dataset* a, const pz_dataset* b, const pz_join_desc* desc, pz_join_result* r) {
+    if (!c || !a || !b || !desc || !r) return fail(PZ_E_INVALID, "pz_join: null argument");
+    if (a->ctx != c || b->ctx != c) return fail(PZ_E_INVALID, "datasets belong to another context");
+    pz_join_desc d = *desc;
+    if (d.predicate < 0 || d.predicate > 2) return fail(PZ_E_INVALID, "unknown predicate");
+    if (d.direction < 0 || d.direction > 2) return fail(PZ_E_INVALID, "unknown query direction");
+    if (d.sizing < 0 || d.sizing > 3) return fail(PZ_E_INVALID, "unknown sizing mode");
+    if (d.load_factor < 0 || d.load_factor > 1) return fail(PZ_E_INVALID, "load factor must be in (0, 1]");
+    if (d.sizing == PZ_SIZING_LEARNED && !d.model) return fail(PZ_E_INVALID, "learned sizing needs a model");
+    if (d.keep_device_result) return fail(PZ_E_INVALID, "keep_device_result is reserved");
+    init_result(r);
+    return guarded([&]() -> pz_status {
+        HIP_OK(hipSetDevice(c->device));
+        double t0 = now_ms();
+        Phases ph(c, r);
+        if (d.sizing == PZ_SIZING_ESTIMATE) {
+            ph.begin("Selectivity Estimation");
+            double est = grid_estimate(a, b, d.gamma > 0 ? d.gamma : 0.8f, d.epsilon > 0 ? d.epsilon : 0.001f,
+                                       !d.no_alpha_correction);
+            r->predicted = est > 0 ? (uint64_t)est : 0;
+        } else if (d.sizing == PZ_SIZING_LEARNED) {
+            ph.begin("Selectivity Estimation");
+            r->predicted = pz::mlp_predict_count(c, d.model, a, b, d.predicate);
+        }
+        if (d.strategy == PZ_STRATEGY_GLOBAL)
+            join_global(c, a, b, d, r, ph);
+        else
+            join_two_level(c, a, b, d, r, ph);
+        ph.finish();
+        r->ms_total = now_ms() - t0;
+        return PZ_OK;
+    });
+}
+
+void pz_join_result_release(pz_join_result* r) {
+    if (!r) return;
+    std::free(r->pairs);
+    std::free(r->pair_types);
+    r->pairs = nullptr;
+    r->pair_types = nullptr;
+    r->num_pairs = 0;
+}
+
+pz_status pz_count_candidate_tests(pz_ctx* c, const pz_dataset* a, const pz_dataset* b, uint64_t* out) {
+    if (!c || !a || !b || !out) return fail(PZ_E_INVALID, "pz_count_candidate_tests: null argument");
+    return guarded([&]() -> pz_status {
+        HIP_OK(hipSetDevice(c->device));
+        hipStream_t s = c->s;
+        unsigned long long* stats = buf<unsigned long long>(c, "stats", 8);
+        SetPrep pa = prep_set(c, a, "A"), pb = prep_set(c, b, "B");
+        GlobalBvh ga = build_global(c, a, pa, "A");
+        GlobalBvh gb = build_global(c, b, pb, "B");
+        uint64_t n1 = a->ne, n2 = b->ne;
+        uint32_t* cnt = buf<uint32_t>(c, "g.cnt", n1 + n2 + 1);
+        uint64_t* off = buf<uint64_t>(c, "g.off", n1 + n2 + 2);
+        void* tmp = buf<char>(c, "scan.tmp.g", pzk::scan_temp_bytes(n1 + n2 + 1));
+        uint32_t* overflow = buf<uint32_t>(c, "g.overflow", 1);
+        pzk::trace_global(3, a->es, a->ee, a->eo, n1, gb.nodes, gb.leaves, b->nt, gb.height, 0, false, 0, cnt,
+                          nullptr, nullptr, nullptr, 0, overflow, stats, s);
+        pzk::trace_global(3, b->es, b->ee, b->eo, n2, ga.nodes, ga.leaves, a->nt, ga.height, 1, false, 0, cnt + n1,
+                          nullptr, nullptr, nullptr, 0, overflow, stats, s);
+        pzk::exclusive_scan_u32_u64(cnt, off, n1 + n2, tmp, s);
+        *out = readback<uint64_t>(c, {off + n1 + n2})[0];
+        return PZ_OK;
+    });
+}
+
+// ----------------------------------------------- reference extern "C" helpers
+long long merge_and_deduplicate_pairs_gpu(const pz_pair* d1, long long n1, const pz_pair* d2, long long n2,
+                                          pz_pair* d_out) {
+    if (n1 < 0 || n2 < 0 || (n1 && !d1) || (n2 && !d2) || !d_out) return -1;
+    uint64_t n = (uint64_t)(n1 + n2);
+    void* mem = nullptr;
+    size_t bytes = 3 * (n + 1) * 8 + pzk::radix_sort_temp_bytes(n) + pzk::unique_temp_bytes(n) + 1024;
+    if (hipMalloc(&mem, bytes) != hipSuccess) return -1;
+    char* p = reinterpret_cast<char*>(mem);
+    uint64_t* k = reinterpret_cast<uint64_t*>(p);
+    uint64_t* k2 = k + (n + 1);
+    uint64_t* u = k2 + (n + 1);
+    char* tmp = reinterpret_cast<char*>(u + (n + 1));
+    tmp = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(tmp) + 255) & ~uintptr_t(255));
+    char* utmp = tmp + pzk::radix_sort_temp_bytes(n);
+    utmp = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(utmp) + 255) & ~uintptr_t(255));
+    hipStream_t s = nullptr;
+    pzk::pairs_to_keys(reinterpret_cast<const int32_t*>(d1), (uint64_t)n1, k, s);
+    pzk::pairs_to_keys(reinterpret_cast<const int32_t*>(d2), (uint64_t)n2, k + n1, s);
+    bool alt = pzk::radix_sort_keys(k, k2, n, 0, 64, tmp, s);
+    uint64_t* sk = alt ? k2 : k;
+    uint64_t* dcount = reinterpret_cast<uint64_t*>(alt ? k : k2);  // free buffer slot for the count
+    pzk::unique_u64(sk, n, u, dcount, utmp, s);
+    uint64_t cnt = 0;
+    long long result = -1;
+    if (hipMemcpy(&cnt, dcount, 8, hipMemcpyDeviceToHost) == hipSuccess) {
+        pzk::keys_to_pairs(u, cnt, reinterpret_cast<int32_t*>(d_out), s);
+        if (hipDeviceSynchronize() == hipSuccess) result = (long long)cnt;
+    }
+    (void)hipFree(mem);
+    return result;
+}
+
+int compact_hash_table_pairs(const unsigned long long* d_table, unsigned long long size, pz_pair* d_output,
+                             int max_output_size) {
+    if (!d_table || !d_output || max_output_size < 0) return -1;
+    void* mem = nullptr;
+    uint64_t cap = (uint64_t)max_output_size;
+    if (hipMalloc(&mem, (cap + 2) * 8) != hipSuccess) return -1;
+    unsigned long long* cnt = reinterpret_cast<unsigned long long*>(mem);
+    uint64_t* keys = reinterpret_cast<uint64_t*>(cnt + 1);
+    hipStream_t s = nullptr;
+    pzk::fill_ull(cnt, 0ull, 1, s);
+    pzk::compact_table(d_table, size, keys, cap, cnt, s);
+    unsigned long long total = 0;
+    int result = -1;
+    if (hipMemcpy(&total, cnt, 8, hipMemcpyDeviceToHost) == hipSuccess) {
+        uint64_t n = std::min<uint64_t>(total, cap);
+        pzk::keys_to_pairs(keys, n, reinterpret_cast<int32_t*>(d_output), s);
+        if (hipDeviceSynchronize() == hipSuccess) result = (int)n;
+    }
+    (void)hipFree(mem);
+    return result;
+}
+
+long long exclusive_scan_gpu(const int* d_input, long long* d_output, int num_elements) {
+    if (num_elements < 0 || (num_elements && (!d_input || !d_output))) return -1;
+    void* mem = nullptr;
+    uint64_t n = (uint64_t)num_elements;
+    // out needs n + 1 slots for the total; the caller's array holds n.
+    size_t bytes = (n + 1) * 8 + pzk::scan_temp_bytes(n) + 512;
+    if (hipMalloc(&mem, bytes) != hipSuccess) return -1;
+    int64_t* out = reinterpret_cast<int64_t*>(mem);
+    char* tmp = reinterpret_cast<char*>(out + n + 1);
+    tmp = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(tmp) + 255) & ~uintptr_t(255));
+    pzk::exclusive_scan_i32_i64(reinterpret_cast<const int32_t*>(d_input), out, n, tmp, nullptr);
+    long long total = -1;
+    if (n && hipMemcpy(d_output, out, n * 8, hipMemcpyDeviceToDevice) != hipSuccess) {
+        (void)hipFree(mem);
+        return -1;
+    }
+    if (hipMemcpy(&total, out + n, 8, hipMemcpyDeviceToHost) != hipSuccess) total = -1;
+    (void)hipFree(mem);
+    return total;
+}
+
+float pz_estimate_overlap_selectivity(const pz_grid_cell* a, const pz_grid_cell* b, int32_t n, float cell_volume,
+                                      float epsilon, float gamma) {
+    if (n < 0 || (n && (!a || !b)) || !(cell_volume > 0)) return -1.0f;
+    double total = 0.0;
+    for (int32_t i = 0; i < n; ++i) {
+        if (a[i].touch_count == 0 || b[i].touch_count == 0) continue;
+        float comb = a[i].avg_size_mean + b[i].avg_size_mean + epsilon;
+        float prob = comb * comb * comb / cell_volume;
+        prob *= std::pow(std::sqrt(a[i].vol_ratio * b[i].vol_ratio), gamma);
+        prob = std::min(prob, 1.0f);
+        total += (double)a[i].touch_count * (double)b[i].touch_count * prob;
+    }
+    return (float)total;
+}
+
+}  // extern "C"

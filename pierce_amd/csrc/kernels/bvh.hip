@@ -1,0 +1,242 @@
+// Linear BVH construction (T. Karras, "Maximizing Parallelism in the
+// Construction of BVHs, Octrees, and k-d Trees", HPG 2012) over primitives
+// sorted by 64-bit keys, built independently per segment: a segment is a run
+// of sorted positions [begin, end) (one object's triangles, or the whole set);
+// its internal nodes occupy slots [begin, end - 1) and its root is slot
+// `begin` (or the single leaf when end - begin == 1).  Keys tie-break on the
+// sorted position, so duplicate Morton codes still give a valid radix tree.
+// Boxes are filled bottom-up with one atomic counter per internal node.
+#include "pz_device.h"
+#include "pz_kernels_bvh.h"
+
+namespace {
+
+using pzd::Node;
+
+__device__ __forceinline__ int delta(const uint64_t* __restrict__ keys, int64_t i, int64_t j, int64_t b, int64_t e) {
+    if (j < b || j >= e) return -1;
+    uint64_t ki = keys[i], kj = keys[j];
+    if (ki != kj) return __clzll((long long)(ki ^ kj));
+    return 64 + __clz((int)((uint32_t)i ^ (uint32_t)j));
+}
+
+__device__ __forceinline__ void seg_range(const uint32_t* __restrict__ seg_off, const uint64_t* __restrict__ keys,
+                                          int seg_shift, uint64_t n, uint64_t i, int64_t& b, int64_t& e) {
+    if (seg_off) {
+        uint64_t s = keys[i] >> seg_shift;
+        b = seg_off[s];
+        e = seg_off[s + 1];
+    } else {
+        b = 0;
+        e = (int64_t)n;
+    }
+}
+
+__global__ void k_karras(const uint64_t* __restrict__ keys, uint64_t n, const uint32_t* __restrict__ seg_off,
+                         int seg_shift, Node* __restrict__ nodes, uint32_t* __restrict__ parent_int,
+                         uint32_t* __restrict__ parent_leaf, uint32_t* __restrict__ counters) {
+    uint64_t gi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= n) return;
+    int64_t b, e;
+    seg_range(seg_off, keys, seg_shift, n, gi, b, e);
+    const int64_t i = (int64_t)gi;
+    if (i == e - 1) return;  // last position of a segment owns no internal node
+    counters[i] = 0;
+    const int d = (delta(keys, i, i + 1, b, e) - delta(keys, i, i - 1, b, e)) >= 0 ? 1 : -1;
+    const int dmin = delta(keys, i, i - d, b, e);
+    int64_t lmax = 2;
+    while (delta(keys, i, i + lmax * d, b, e) > dmin) lmax <<= 1;
+    int64_t l = 0;
+    for (int64_t t = lmax >> 1; t >= 1; t >>= 1)
+        if (delta(keys, i, i + (l + t) * d, b, e) > dmin) l += t;
+    const int64_t j = i + l * d;
+    const int dnode = delta(keys, i, j, b, e);
+    int64_t s = 0;
+    int64_t div = 2;
+    while (true) {
+        int64_t t = (l + div - 1) / div;
+        if (delta(keys, i, i + (s + t) * d, b, e) > dnode) s += t;
+        if (t <= 1) break;
+        div <<= 1;
+    }
+    const int64_t g = i + s * d + (d < 0 ? -1 : 0);
+    const int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+    int c0, c1;
+    if (lo == g) {
+        c0 = ~(int)g;
+        parent_leaf[g] = (uint32_t)i;
+    } else {
+        c0 = (int)g;
+        parent_int[g] = (uint32_t)i;
+    }
+    if (hi == g + 1) {
+        c1 = ~(int)(g + 1);
+        parent_leaf[g + 1] = (uint32_t)i;
+    } else {
+        c1 = (int)(g + 1);
+        parent_int[g + 1] = (uint32_t)i;
+    }
+    nodes[i].d = make_int4(c0, c1, 0, 0);
+}
+
+__device__ __forceinline__ void store_child_box(Node* nd, int slot, const float* lo, const float* hi) {
+    float* f = reinterpret_cast<float*>(nd);
+    if (slot == 0) {
+        f[0] = lo[0];
+        f[1] = lo[1];
+        f[2] = lo[2];
+        f[3] = hi[0];
+        f[4] = hi[1];
+        f[5] = hi[2];
+    } else {
+        f[6] = lo[0];
+        f[7] = lo[1];
+        f[8] = lo[2];
+        f[9] = hi[0];
+        f[10] = hi[1];
+        f[11] = hi[2];
+    }
+}
+__device__ __forceinline__ void load_child_box(const Node* nd, int slot, float* lo, float* hi) {
+    const volatile float* f = reinterpret_cast<const volatile float*>(nd);
+    int o = slot == 0 ? 0 : 6;
+    lo[0] = f[o + 0];
+    lo[1] = f[o + 1];
+    lo[2] = f[o + 2];
+    hi[0] = f[o + 3];
+    hi[1] = f[o + 4];
+    hi[2] = f[o + 5];
+}
+
+template <class BoxFn>
+__device__ void refit_from_leaf(uint64_t p, const uint64_t* __restrict__ keys, uint64_t n,
+                                const uint32_t* __restrict__ seg_off, int seg_shift, Node* nodes,
+                                const uint32_t* __restrict__ parent_int, const uint32_t* __restrict__ parent_leaf,
+                                uint32_t* counters, uint32_t* max_depth, BoxFn boxfn) {
+    int64_t b, e;
+    seg_range(seg_off, keys, seg_shift, n, p, b, e);
+    float lo[3], hi[3];
+    boxfn(p, lo, hi);
+    if (e - b == 1) return;
+    uint32_t node = parent_leaf[p];
+    int child = ~(int)p;
+    int height = 0;  // height of the subtree carried upward (leaf = 0)
+    while (true) {
+        Node* nd = nodes + node;
+        int slot = (nd->d.x == child) ? 0 : 1;
+        store_child_box(nd, slot, lo, hi);
+        reinterpret_cast<int*>(nd)[14 + slot] = height;
+        __threadfence();
+        uint32_t prev = atomicAdd(&counters[node], 1u);
+        if (prev == 0) return;  // sibling not done yet; it continues upward
+        __threadfence();
+        float slo[3], shi[3];
+        load_child_box(nd, 1 - slot, slo, shi);
+        int sh = reinterpret_cast<const volatile int*>(nd)[14 + (1 - slot)];
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(lo[a], slo[a]);
+            hi[a] = fmaxf(hi[a], shi[a]);
+        }
+        height = 1 + (height > sh ? height : sh);
+        if ((int64_t)node == b) break;  // segment root
+        child = (int)node;
+        node = parent_int[node];
+    }
+    atomicMax(max_depth, (uint32_t)height);
+}
+
+struct TriBoxFn {
+    const float* verts;
+    const uint32_t* idx;
+    const uint32_t* order;
+    __device__ void operator()(uint64_t p, float* lo, float* hi) const {
+        uint32_t t = order[p];
+        const float* v0 = verts + 3ull * idx[3ull * t];
+        const float* v1 = verts + 3ull * idx[3ull * t + 1];
+        const float* v2 = verts + 3ull * idx[3ull * t + 2];
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(v0[a], fminf(v1[a], v2[a]));
+            hi[a] = fmaxf(v0[a], fmaxf(v1[a], v2[a]));
+        }
+    }
+};
+struct OrdBoxFn {  // boxes stored as ordered uints: 6 per primitive
+    const uint32_t* obox;
+    const uint32_t* order;
+    __device__ void operator()(uint64_t p, float* lo, float* hi) const {
+        const uint32_t* b = obox + 6ull * order[p];
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = pzd::ord2f(b[a]);
+            hi[a] = pzd::ord2f(b[3 + a]);
+        }
+    }
+};
+
+__global__ void k_refit_tris(const uint64_t* keys, uint64_t n, const uint32_t* seg_off, int seg_shift, Node* nodes,
+                             const uint32_t* parent_int, const uint32_t* parent_leaf, uint32_t* counters,
+                             uint32_t* max_depth, TriBoxFn fn) {
+    uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) refit_from_leaf(p, keys, n, seg_off, seg_shift, nodes, parent_int, parent_leaf, counters, max_depth, fn);
+}
+__global__ void k_refit_boxes(const uint64_t* keys, uint64_t n, const uint32_t* seg_off, int seg_shift, Node* nodes,
+                              const uint32_t* parent_int, const uint32_t* parent_leaf, uint32_t* counters,
+                              uint32_t* max_depth, OrdBoxFn fn) {
+    uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) refit_from_leaf(p, keys, n, seg_off, seg_shift, nodes, parent_int, parent_leaf, counters, max_depth, fn);
+}
+
+__global__ void k_gather_leaf_tris(const uint32_t* __restrict__ order, uint64_t n, const float* __restrict__ verts,
+                                   const uint32_t* __restrict__ idx, const int32_t* __restrict__ t2o,
+                                   pzd::LeafTri* __restrict__ out) {
+    uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    uint32_t t = order[p];
+    const float* v0 = verts + 3ull * idx[3ull * t];
+    const float* v1 = verts + 3ull * idx[3ull * t + 1];
+    const float* v2 = verts + 3ull * idx[3ull * t + 2];
+    pzd::LeafTri L;
+    L.v0 = make_float4(v0[0], v0[1], v0[2], __uint_as_float(t));
+    L.v1 = make_float4(v1[0], v1[1], v1[2], __int_as_float(t2o[t]));
+    L.v2 = make_float4(v2[0], v2[1], v2[2], 0.0f);
+    out[p] = L;
+}
+
+inline unsigned grid1(uint64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+namespace pzk {
+
+void lbvh_karras(const uint64_t* sorted_keys, uint64_t n, const uint32_t* seg_off, int seg_shift, pzd::Node* nodes,
+                 uint32_t* parent_int, uint32_t* parent_leaf, uint32_t* counters, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_karras, dim3(grid1(n)), dim3(256), 0, s, sorted_keys, n, seg_off, seg_shift, nodes, parent_int,
+                       parent_leaf, counters);
+}
+
+void lbvh_refit_tris(const uint64_t* sorted_keys, const uint32_t* order, uint64_t n, const uint32_t* seg_off,
+                     int seg_shift, const float* verts, const uint32_t* idx, pzd::Node* nodes,
+                     const uint32_t* parent_int, const uint32_t* parent_leaf, uint32_t* counters, uint32_t* max_depth,
+                     hipStream_t s) {
+    if (n == 0) return;
+    TriBoxFn fn{verts, idx, order};
+    hipLaunchKernelGGL(k_refit_tris, dim3(grid1(n)), dim3(256), 0, s, sorted_keys, n, seg_off, seg_shift, nodes,
+                       parent_int, parent_leaf, counters, max_depth, fn);
+}
+
+void lbvh_refit_ord_boxes(const uint64_t* sorted_keys, const uint32_t* order, uint64_t n, const uint32_t* seg_off,
+                          int seg_shift, const uint32_t* obox, pzd::Node* nodes, const uint32_t* parent_int,
+                          const uint32_t* parent_leaf, uint32_t* counters, uint32_t* max_depth, hipStream_t s) {
+    if (n == 0) return;
+    OrdBoxFn fn{obox, order};
+    hipLaunchKernelGGL(k_refit_boxes, dim3(grid1(n)), dim3(256), 0, s, sorted_keys, n, seg_off, seg_shift, nodes,
+                       parent_int, parent_leaf, counters, max_depth, fn);
+}
+
+void gather_leaf_tris(const uint32_t* order, uint64_t n, const float* verts, const uint32_t* idx, const int32_t* t2o,
+                      pzd::LeafTri* out, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_gather_leaf_tris, dim3(grid1(n)), dim3(256), 0, s, order, n, verts, idx, t2o, out);
+}
+
+}  // namespace pzk

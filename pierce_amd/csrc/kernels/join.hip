@@ -1,0 +1,968 @@
+// Join kernels for gfx950.
+//
+// Two strategies share these kernels:
+//  * two-level (default): object boxes -> object-level LBVH of mesh2 ->
+//    broad phase (object pairs whose boxes meet) -> per-object LBVHs of the
+//    objects that occur in a pair -> narrow phase: each wave takes a task
+//    (pair, direction, chunk of source edges) and stops as soon as any lane
+//    proves an intersection for that pair -> +Z parity for containment.
+//  * global (the reference's shape, pierce/query/src/cuda/mesh_overlap_edges.cu:105-208):
+//    every edge of one set is traced through one BVH over all triangles of
+//    the other set and every hit is kept, then deduplicated by sort+unique
+//    (two-pass sizing) or by an open-addressing hash set.
+// Traversal keeps a per-lane stack in LDS (64 lanes x STACK ints per wave).
+#include "pz_device.h"
+#include "pz_kernels_join.h"
+#include "pz_predicates.h"
+
+namespace {
+
+using namespace pzd;
+
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / PZ_WAVE;
+
+inline unsigned grid1(uint64_t n, int b = kBlock) { return (unsigned)((n + b - 1) / b); }
+
+__device__ __forceinline__ float3 vtx(const float* __restrict__ v, uint32_t i) {
+    return make_float3(v[3ull * i], v[3ull * i + 1], v[3ull * i + 2]);
+}
+
+// ------------------------------------------------------------ object stats
+// Per triangle: object box (ordered-uint atomics), triangle count and first
+// triangle per object; lanes of a wave that share one object reduce first.
+__global__ __launch_bounds__(kBlock) void k_object_stats(const float* __restrict__ verts, uint64_t nv,
+                                                         const uint32_t* __restrict__ idx,
+                                                         const int32_t* __restrict__ t2o, uint64_t nt, int32_t nobj,
+                                                         uint32_t* __restrict__ obox, uint32_t* __restrict__ tri_cnt,
+                                                         uint32_t* __restrict__ first_tri) {
+    uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    bool valid = t < nt;
+    int32_t o = -1;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    if (valid) {
+        o = t2o[t];
+        uint32_t i0 = idx[3 * t], i1 = idx[3 * t + 1], i2 = idx[3 * t + 2];
+        valid = o >= 0 && o < nobj && i0 < nv && i1 < nv && i2 < nv;
+        if (valid) {
+            float3 a = vtx(verts, i0), b = vtx(verts, i1), c = vtx(verts, i2);
+            lo[0] = fminf(a.x, fminf(b.x, c.x));
+            lo[1] = fminf(a.y, fminf(b.y, c.y));
+            lo[2] = fminf(a.z, fminf(b.z, c.z));
+            hi[0] = fmaxf(a.x, fmaxf(b.x, c.x));
+            hi[1] = fmaxf(a.y, fmaxf(b.y, c.y));
+            hi[2] = fmaxf(a.z, fmaxf(b.z, c.z));
+        }
+    }
+    uint64_t vm = __ballot(valid);
+    if (vm == 0) return;
+    int leader = __ffsll((long long)vm) - 1;
+    int32_t o0 = __shfl(o, leader, 64);
+    bool uniform = __ballot(valid && o == o0) == vm;
+    if (uniform) {
+        for (int a = 0; a < 3; ++a) {
+            float l = lo[a], h = hi[a];
+            for (int s = 32; s > 0; s >>= 1) {
+                l = fminf(l, __shfl_xor(l, s, 64));
+                h = fmaxf(h, __shfl_xor(h, s, 64));
+            }
+            lo[a] = l;
+            hi[a] = h;
+        }
+        uint32_t tmin = (uint32_t)__shfl((int)(uint32_t)t, leader, 64);
+        if (__lane_id() == leader) {
+            for (int a = 0; a < 3; ++a) {
+                atomicMin(&obox[6ull * o0 + a], f2ord(lo[a]));
+                atomicMax(&obox[6ull * o0 + 3 + a], f2ord(hi[a]));
+            }
+            atomicAdd(&tri_cnt[o0], (uint32_t)__popcll(vm));
+            atomicMin(&first_tri[o0], tmin);
+        }
+    } else if (valid) {
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(&obox[6ull * o + a], f2ord(lo[a]));
+            atomicMax(&obox[6ull * o + 3 + a], f2ord(hi[a]));
+        }
+        atomicAdd(&tri_cnt[o], 1u);
+        atomicMin(&first_tri[o], (uint32_t)t);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_edge_count(const int32_t* __restrict__ eo, uint64_t ne, int32_t nobj,
+                                                       uint32_t* __restrict__ edge_cnt) {
+    uint64_t e = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    int32_t o = e < ne ? eo[e] : -1;
+    bool valid = o >= 0 && o < nobj;
+    uint64_t vm = __ballot(valid);
+    if (vm == 0) return;
+    int leader = __ffsll((long long)vm) - 1;
+    int32_t o0 = __shfl(o, leader, 64);
+    if (__ballot(valid && o == o0) == vm) {
+        if (__lane_id() == leader) atomicAdd(&edge_cnt[o0], (uint32_t)__popcll(vm));
+    } else if (valid) {
+        atomicAdd(&edge_cnt[o], 1u);
+    }
+}
+
+// Union of object boxes (ordered uints) into scene[6].
+__global__ __launch_bounds__(kBlock) void k_scene_box(const uint32_t* __restrict__ obox, int32_t nobj,
+                                                      uint32_t* __restrict__ scene) {
+    int32_t o = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t v[6];
+    for (int a = 0; a < 3; ++a) {
+        v[a] = o < nobj ? obox[6ll * o + a] : 0xFFFFFFFFu;
+        v[3 + a] = o < nobj ? obox[6ll * o + 3 + a] : 0u;
+    }
+    for (int a = 0; a < 3; ++a)
+        for (int s = 32; s > 0; s >>= 1) {
+            v[a] = min(v[a], (uint32_t)__shfl_xor((int)v[a], s, 64));
+            v[3 + a] = max(v[3 + a], (uint32_t)__shfl_xor((int)v[3 + a], s, 64));
+        }
+    if (__lane_id() == 0)
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(&scene[a], v[a]);
+            atomicMax(&scene[3 + a], v[3 + a]);
+        }
+}
+
+__device__ __forceinline__ void load_scene(const uint32_t* scene, float* lo, float* inv) {
+    for (int a = 0; a < 3; ++a) {
+        float l = ord2f(scene[a]), h = ord2f(scene[3 + a]);
+        if (!(l <= h)) {  // empty scene
+            l = 0.f;
+            h = 0.f;
+        }
+        lo[a] = l;
+        float ext = h - l;
+        inv[a] = ext > 0.f ? 1.0f / ext : 0.0f;
+    }
+}
+
+// Object keys: Morton code of the box centre inside the scene box.
+__global__ __launch_bounds__(kBlock) void k_object_keys(const uint32_t* __restrict__ obox, int32_t nobj,
+                                                        const uint32_t* __restrict__ scene, uint64_t* __restrict__ keys,
+                                                        uint32_t* __restrict__ vals) {
+    int32_t o = blockIdx.x * kBlock + threadIdx.x;
+    if (o >= nobj) return;
+    float lo[3], inv[3];
+    load_scene(scene, lo, inv);
+    float c[3];
+    bool empty = false;
+    for (int a = 0; a < 3; ++a) {
+        float l = ord2f(obox[6ll * o + a]), h = ord2f(obox[6ll * o + 3 + a]);
+        empty |= !(l <= h);
+        c[a] = 0.5f * l + 0.5f * h;
+    }
+    keys[o] = empty ? 0ull : morton30(c[0], c[1], c[2], lo[0], lo[1], lo[2], inv[0], inv[1], inv[2]);
+    vals[o] = (uint32_t)o;
+}
+
+// --------------------------------------------------------------- broad phase
+// Thread per mesh1 object: walk the mesh2 object BVH with its box.
+// mode 0: count into cnt[a]; mode 1: write at off[a]; mode 2: append with
+// capacity (cursor counts every pair, writes only below cap).
+template <int Mode>
+__global__ __launch_bounds__(kBlock) void k_broad(const uint32_t* __restrict__ oboxA, int32_t nA,
+                                                  const uint32_t* __restrict__ oboxB,
+                                                  const Node* __restrict__ nodesB, const uint32_t* __restrict__ orderB,
+                                                  int32_t nB, uint32_t* __restrict__ cnt,
+                                                  const uint64_t* __restrict__ off, uint64_t* __restrict__ out,
+                                                  unsigned long long* __restrict__ cursor, uint64_t cap) {
+    int32_t a = blockIdx.x * kBlock + threadIdx.x;
+    if (a >= nA || nB <= 0) return;
+    float lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = ord2f(oboxA[6ll * a + k]);
+        hi[k] = ord2f(oboxA[6ll * a + 3 + k]);
+    }
+    if (!(lo[0] <= hi[0] && lo[1] <= hi[1] && lo[2] <= hi[2])) {
+        if (Mode == 0) cnt[a] = 0;
+        return;
+    }
+    uint32_t n = 0;
+    uint64_t base = Mode == 1 ? off[a] : 0;
+    auto emit = [&](int32_t b) {
+        if (Mode == 1) out[base + n] = pack_pair(a, b);
+        if (Mode == 2) {
+            unsigned long long slot = atomicAdd(cursor, 1ull);
+            if (slot < cap) out[slot] = pack_pair(a, b);
+        }
+        ++n;
+    };
+    if (nB == 1) {
+        // single object: no internal node; test its box directly
+        if (boxes_meet(lo, hi, ord2f(oboxB[0]), ord2f(oboxB[1]), ord2f(oboxB[2]), ord2f(oboxB[3]), ord2f(oboxB[4]),
+                       ord2f(oboxB[5])))
+            emit(0);
+    } else {
+        int stack[64];
+        int sp = 0;
+        int cur = 0;
+        while (true) {
+            const Node nd = nodesB[cur];
+            bool h0 = boxes_meet(lo, hi, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
+            bool h1 = boxes_meet(lo, hi, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w);
+            int c0 = nd.d.x, c1 = nd.d.y;
+            if (h0 && c0 < 0) {
+                emit((int32_t)orderB[~c0]);
+                h0 = false;
+            }
+            if (h1 && c1 < 0) {
+                emit((int32_t)orderB[~c1]);
+                h1 = false;
+            }
+            if (h0 && h1) {
+                stack[sp++] = c1;
+                cur = c0;
+            } else if (h0) {
+                cur = c0;
+            } else if (h1) {
+                cur = c1;
+            } else {
+                if (sp == 0) break;
+                cur = stack[--sp];
+            }
+        }
+    }
+    if (Mode == 0) cnt[a] = n;
+}
+
+// ------------------------------------------------- needed objects / layout
+__global__ void k_mark_needed(const uint64_t* __restrict__ pairs, uint64_t np, uint32_t* __restrict__ needA,
+                              uint32_t* __restrict__ needB) {
+    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= np) return;
+    uint64_t k = pairs[i];
+    needA[(uint32_t)(k >> 32)] = 1u;
+    needB[(uint32_t)k] = 1u;
+}
+
+// loc = exclusive scan of need; fills per-needed-object counts.
+__global__ void k_needed_layout(const uint32_t* __restrict__ need, const uint32_t* __restrict__ loc, int32_t nobj,
+                                const uint32_t* __restrict__ tri_cnt, const uint32_t* __restrict__ edge_cnt,
+                                int32_t* __restrict__ obj_of_loc, uint32_t* __restrict__ seg_tri_cnt,
+                                uint32_t* __restrict__ seg_edge_cnt) {
+    int32_t o = blockIdx.x * kBlock + threadIdx.x;
+    if (o >= nobj || !need[o]) return;
+    uint32_t l = loc[o];
+    obj_of_loc[l] = o;
+    seg_tri_cnt[l] = tri_cnt[o];
+    seg_edge_cnt[l] = edge_cnt[o];
+}
+
+// Wave-aggregated slot reservation inside object segments.
+__device__ __forceinline__ uint32_t reserve_slot(bool valid, uint32_t l, uint32_t* cursor) {
+    uint64_t vm = __ballot(valid);
+    if (vm == 0) return 0;
+    int leader = __ffsll((long long)vm) - 1;
+    uint32_t l0 = (uint32_t)__shfl((int)l, leader, 64);
+    uint32_t slot = 0;
+    if (__ballot(valid && l == l0) == vm) {
+        uint32_t base = 0;
+        if (__lane_id() == leader) base = atomicAdd(&cursor[l0], (uint32_t)__popcll(vm));
+        base = (uint32_t)__shfl((int)base, leader, 64);
+        slot = base + lanes_below(vm);
+    } else if (valid) {
+        slot = atomicAdd(&cursor[l], 1u);
+    }
+    return slot;
+}
+
+// Triangles of needed objects -> (loc << 30 | Morton in object box, triangle).
+__global__ __launch_bounds__(kBlock) void k_tri_keys_two_level(
+    const float* __restrict__ verts, const uint32_t* __restrict__ idx, const int32_t* __restrict__ t2o, uint64_t nt,
+    int32_t nobj, const uint32_t* __restrict__ need, const uint32_t* __restrict__ loc,
+    const uint32_t* __restrict__ obox, const uint32_t* __restrict__ seg_off, uint32_t* __restrict__ cursor,
+    uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, uint64_t nv) {
+    uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    int32_t o = t < nt ? t2o[t] : -1;
+    bool valid = o >= 0 && o < nobj && need[o];
+    uint32_t i0 = 0, i1 = 0, i2 = 0;
+    if (valid) {
+        i0 = idx[3 * t];
+        i1 = idx[3 * t + 1];
+        i2 = idx[3 * t + 2];
+        valid = i0 < nv && i1 < nv && i2 < nv;
+    }
+    uint32_t l = valid ? loc[o] : 0u;
+    uint32_t slot = reserve_slot(valid, l, cursor);
+    if (!valid) return;
+    float3 a = vtx(verts, i0), b = vtx(verts, i1), c = vtx(verts, i2);
+    float lo[3], inv[3];
+    for (int k = 0; k < 3; ++k) {
+        float bl = ord2f(obox[6ll * o + k]), bh = ord2f(obox[6ll * o + 3 + k]);
+        lo[k] = bl;
+        inv[k] = bh > bl ? 1.0f / (bh - bl) : 0.0f;
+    }
+    float cx = (a.x + b.x + c.x) * (1.0f / 3.0f), cy = (a.y + b.y + c.y) * (1.0f / 3.0f),
+          cz = (a.z + b.z + c.z) * (1.0f / 3.0f);
+    uint64_t pos = (uint64_t)seg_off[l] + slot;
+    keys[pos] = ((uint64_t)l << 30) | morton30(cx, cy, cz, lo[0], lo[1], lo[2], inv[0], inv[1], inv[2]);
+    vals[pos] = (uint32_t)t;
+}
+
+// Triangles of a whole set -> (Morton in scene box, triangle); invalid ones get
+// key 2^30 - 1 and an empty box at refit time is avoided by the caller
+// counting only valid triangles first.
+__global__ __launch_bounds__(kBlock) void k_tri_keys_global(const float* __restrict__ verts,
+                                                            const uint32_t* __restrict__ idx, uint64_t nt,
+                                                            const uint32_t* __restrict__ scene,
+                                                            uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= nt) return;
+    float lo[3], inv[3];
+    load_scene(scene, lo, inv);
+    float3 a = vtx(verts, idx[3 * t]), b = vtx(verts, idx[3 * t + 1]), c = vtx(verts, idx[3 * t + 2]);
+    float cx = (a.x + b.x + c.x) * (1.0f / 3.0f), cy = (a.y + b.y + c.y) * (1.0f / 3.0f),
+          cz = (a.z + b.z + c.z) * (1.0f / 3.0f);
+    keys[t] = morton30(cx, cy, cz, lo[0], lo[1], lo[2], inv[0], inv[1], inv[2]);
+    vals[t] = (uint32_t)t;
+}
+
+// Edges of needed objects -> EdgeRec in per-object segments.
+__global__ __launch_bounds__(kBlock) void k_edge_scatter(const float* __restrict__ es, const float* __restrict__ ee,
+                                                         const int32_t* __restrict__ eo, uint64_t ne, int32_t nobj,
+                                                         const uint32_t* __restrict__ need,
+                                                         const uint32_t* __restrict__ loc,
+                                                         const uint32_t* __restrict__ eseg_off,
+                                                         uint32_t* __restrict__ cursor, EdgeRec* __restrict__ out) {
+    uint64_t e = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    int32_t o = e < ne ? eo[e] : -1;
+    bool valid = o >= 0 && o < nobj && need[o];
+    uint32_t l = valid ? loc[o] : 0u;
+    uint32_t slot = reserve_slot(valid, l, cursor);
+    if (!valid) return;
+    EdgeRec r;
+    r.s = make_float4(es[3 * e], es[3 * e + 1], es[3 * e + 2], __int_as_float(o));
+    r.e = make_float4(ee[3 * e], ee[3 * e + 1], ee[3 * e + 2], 0.0f);
+    out[(uint64_t)eseg_off[l] + slot] = r;
+}
+
+// ------------------------------------------------------------- traversal
+struct LaneStats {
+    uint32_t tests, f64, exact, mis, traced;
+};
+
+__device__ __forceinline__ void flush_stats(LaneStats& st, unsigned long long* stats) {
+    uint32_t v[5] = {st.tests, st.f64, st.exact, st.mis, st.traced};
+    for (int k = 0; k < 5; ++k) {
+        uint32_t s = wave_sum(v[k]);
+        if (__lane_id() == 0 && s) atomicAdd(&stats[k], (unsigned long long)s);
+    }
+}
+
+template <bool F32>
+__device__ __forceinline__ bool leaf_test(const LeafTri& L, const float3& p, const float3& q, bool allow_f32,
+                                          LaneStats& st) {
+    float3 a = make_float3(L.v0.x, L.v0.y, L.v0.z), b = make_float3(L.v1.x, L.v1.y, L.v1.z),
+           c = make_float3(L.v2.x, L.v2.y, L.v2.z);
+    Esc esc{0};
+    bool hit = seg_tri<F32>(p, q, a, b, c, allow_f32, esc);
+    ++st.tests;
+    if (esc.level >= 1) {
+        if (esc.level == 1)
+            ++st.f64;
+        else
+            ++st.exact;
+        if (F32 && seg_tri_naive(p, q, a, b, c) != hit) ++st.mis;
+    }
+    return hit;
+}
+
+// Two-level narrow phase.  Task = (pair, direction, chunk of kChunk source
+// edges); one wave per task; early exit per pair.
+constexpr int kChunk = 256;
+
+template <int STACK, bool F32>
+__global__ __launch_bounds__(kBlock) void k_narrow(const uint64_t* __restrict__ pairs, uint64_t np,
+                                                   const uint64_t* __restrict__ task_off, SetView A, SetView B,
+                                                   uint32_t* __restrict__ found, int dir_mask, int allow_f32,
+                                                   unsigned long long* __restrict__ stats) {
+    __shared__ int stk[kWavesPerBlock * STACK * PZ_WAVE];
+    const int lane = __lane_id(), w = threadIdx.x / PZ_WAVE;
+    int* my = stk + (w * STACK) * PZ_WAVE + lane;
+    const uint64_t task = (uint64_t)blockIdx.x * kWavesPerBlock + w;
+    const uint64_t ntasks = task_off[np];
+    if (task >= ntasks) return;
+    // pair = upper_bound(task_off, task) - 1
+    uint64_t lo = 0, hi = np;
+    while (hi - lo > 1) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (task_off[mid] <= task)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const uint64_t p = lo;
+    if (found[p]) return;
+    const uint64_t key = pairs[p];
+    const int32_t oa = (int32_t)(key >> 32), ob = (int32_t)(uint32_t)key;
+    const uint32_t la = A.loc[oa], lb = B.loc[ob];
+    const uint32_t ea0 = A.eseg_off[la], ea1 = A.eseg_off[la + 1];
+    const uint32_t eb0 = B.eseg_off[lb], eb1 = B.eseg_off[lb + 1];
+    const uint64_t chunks_a = (dir_mask & 1) ? (ea1 - ea0 + kChunk - 1) / kChunk : 0;
+    uint64_t c = task - task_off[p];
+    // direction 0: edges of A against triangles of B; direction 1: the reverse
+    const bool rev = c >= chunks_a;
+    if (rev) c -= chunks_a;
+    const SetView& S = rev ? B : A;
+    const SetView& T = rev ? A : B;
+    const uint32_t e_begin = (rev ? eb0 : ea0) + (uint32_t)c * kChunk;
+    const uint32_t e_end = min(e_begin + kChunk, rev ? eb1 : ea1);
+    const int32_t ot = rev ? oa : ob;
+    const uint32_t lt = rev ? la : lb;
+    const uint32_t t0 = T.seg_off[lt], t1 = T.seg_off[lt + 1];
+    const int root = (t1 - t0 >= 2) ? (int)t0 : ~(int)t0;
+    float tlo[3], thi[3];
+    for (int k = 0; k < 3; ++k) {
+        tlo[k] = ord2f(T.obox[6ll * ot + k]);
+        thi[k] = ord2f(T.obox[6ll * ot + 3 + k]);
+    }
+    LaneStats st{0, 0, 0, 0, 0};
+    bool any_hit = false;
+    for (uint32_t base = e_begin; base < e_end && !any_hit; base += PZ_WAVE) {
+        const uint32_t e = base + lane;
+        bool act = e < e_end;
+        float3 pp = make_float3(0, 0, 0), qq = pp;
+        float elo[3] = {0, 0, 0}, ehi[3] = {0, 0, 0};
+        if (act) {
+            EdgeRec r = S.edges[e];
+            pp = make_float3(r.s.x, r.s.y, r.s.z);
+            qq = make_float3(r.e.x, r.e.y, r.e.z);
+            elo[0] = fminf(pp.x, qq.x);
+            elo[1] = fminf(pp.y, qq.y);
+            elo[2] = fminf(pp.z, qq.z);
+            ehi[0] = fmaxf(pp.x, qq.x);
+            ehi[1] = fmaxf(pp.y, qq.y);
+            ehi[2] = fmaxf(pp.z, qq.z);
+            act = boxes_meet(elo, ehi, tlo[0], tlo[1], tlo[2], thi[0], thi[1], thi[2]);
+        }
+        if (act) ++st.traced;
+        int cur = root;
+        int sp = 0;
+        bool hit = false;
+        while (__ballot(act) != 0) {
+            if (act) {
+                if (cur >= 0) {
+                    const Node nd = T.nodes[cur];
+                    bool h0 = boxes_meet(elo, ehi, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
+                    bool h1 = boxes_meet(elo, ehi, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w);
+                    if (h0 && h1) {
+                        my[sp * PZ_WAVE] = nd.d.y;
+                        ++sp;
+                        cur = nd.d.x;
+                    } else if (h0) {
+                        cur = nd.d.x;
+                    } else if (h1) {
+                        cur = nd.d.y;
+                    } else if (sp > 0) {
+                        --sp;
+                        cur = my[sp * PZ_WAVE];
+                    } else {
+                        act = false;
+                    }
+                } else {
+                    const LeafTri L = T.leaves[~cur];
+                    if (leaf_test<F32>(L, pp, qq, allow_f32 != 0, st)) {
+                        hit = true;
+                        act = false;
+                    } else if (sp > 0) {
+                        --sp;
+                        cur = my[sp * PZ_WAVE];
+                    } else {
+                        act = false;
+                    }
+                }
+            }
+            if (__ballot(hit) != 0) break;
+        }
+        any_hit = __ballot(hit) != 0;
+    }
+    if (any_hit && lane == 0) found[p] = 1u;
+    flush_stats(st, stats);
+}
+
+// Parity: thread per (pair, direction).  Counts +Z crossings of the
+// perturbed ray from the first vertex of one object against the other's
+// triangles.  kind 0: containment (point of mesh2 object in mesh1 object);
+// kind 1: intersection (both directions, sets inside[] bit 0 / bit 1).
+template <int STACK>
+__global__ __launch_bounds__(kBlock) void k_parity(const uint64_t* __restrict__ pairs, uint64_t np,
+                                                   const uint32_t* __restrict__ found, SetView A, SetView B,
+                                                   const float* __restrict__ vA, const uint32_t* __restrict__ iA,
+                                                   const uint32_t* __restrict__ ftA, const float* __restrict__ vB,
+                                                   const uint32_t* __restrict__ iB, const uint32_t* __restrict__ ftB,
+                                                   int kind, uint32_t* __restrict__ inside,
+                                                   unsigned long long* __restrict__ stats) {
+    __shared__ int stk[kWavesPerBlock * STACK * PZ_WAVE];
+    const int lane = __lane_id(), w = threadIdx.x / PZ_WAVE;
+    int* my = stk + (w * STACK) * PZ_WAVE + lane;
+    const uint64_t gid = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int ndir = kind == 1 ? 2 : 1;
+    const uint64_t p = gid / ndir;
+    const int dir = (int)(gid % ndir);  // 0: point of B in A; 1: point of A in B
+    bool act = p < np && !found[p];
+    LaneStats st{0, 0, 0, 0, 0};
+    float3 pt = make_float3(0, 0, 0);
+    int cur = 0, sp = 0;
+    bool odd = false;
+    const SetView* T = dir == 0 ? &A : &B;
+    int32_t ot = 0;
+    if (act) {
+        uint64_t key = pairs[p];
+        int32_t oa = (int32_t)(key >> 32), ob = (int32_t)(uint32_t)key;
+        if (dir == 0) {
+            pt = vtx(vB, iB[3ull * ftB[ob]]);
+            ot = oa;
+        } else {
+            pt = vtx(vA, iA[3ull * ftA[oa]]);
+            ot = ob;
+        }
+        uint32_t lt = T->loc[ot];
+        uint32_t t0 = T->seg_off[lt], t1 = T->seg_off[lt + 1];
+        cur = (t1 - t0 >= 2) ? (int)t0 : ~(int)t0;
+        // the ray must start inside the target's box footprint and below its top
+        float bx0 = ord2f(T->obox[6ll * ot]), by0 = ord2f(T->obox[6ll * ot + 1]);
+        float bx1 = ord2f(T->obox[6ll * ot + 3]), by1 = ord2f(T->obox[6ll * ot + 4]);
+        float bz1 = ord2f(T->obox[6ll * ot + 5]);
+        act = pt.x >= bx0 && pt.x <= bx1 && pt.y >= by0 && pt.y <= by1 && pt.z <= bz1;
+    }
+    const float rlo[3] = {pt.x, pt.y, pt.z}, rhi[3] = {pt.x, pt.y, INFINITY};
+    while (act) {
+        if (cur >= 0) {
+            const Node nd = T->nodes[cur];
+            bool h0 = boxes_meet(rlo, rhi, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
+            bool h1 = boxes_meet(rlo, rhi, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w);
+            if (h0 && h1) {
+                my[sp * PZ_WAVE] = nd.d.y;
+                ++sp;
+                cur = nd.d.x;
+                continue;
+            } else if (h0) {
+                cur = nd.d.x;
+                continue;
+            } else if (h1) {
+                cur = nd.d.y;
+                continue;
+            }
+        } else {
+            const LeafTri L = T->leaves[~cur];
+            Esc esc{0};
+            float3 a = make_float3(L.v0.x, L.v0.y, L.v0.z), b = make_float3(L.v1.x, L.v1.y, L.v1.z),
+                   c = make_float3(L.v2.x, L.v2.y, L.v2.z);
+            if (ray_cross(pt, a, b, c, esc)) odd = !odd;
+            ++st.tests;
+            if (esc.level == 2) ++st.exact;
+        }
+        if (sp == 0) break;
+        --sp;
+        cur = my[sp * PZ_WAVE];
+    }
+    if (p < np && odd) atomicOr(&inside[p], 1u << dir);
+    flush_stats(st, stats);
+}
+
+// Ballot + prefix-sum compaction of selected candidate pairs.
+// sel: 0 drop; else keep with type value (sel - 1) in vals.
+__global__ __launch_bounds__(kBlock) void k_select_pairs(const uint64_t* __restrict__ pairs, uint64_t np,
+                                                         const uint32_t* __restrict__ found,
+                                                         const uint32_t* __restrict__ inside, int predicate,
+                                                         int include_overlap, uint64_t* __restrict__ out_keys,
+                                                         uint32_t* __restrict__ out_type,
+                                                         unsigned long long* __restrict__ count) {
+    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    uint32_t type = 0;
+    bool keep = false;
+    if (i < np) {
+        bool ov = found[i] != 0;
+        uint32_t in = inside ? inside[i] : 0u;
+        if (predicate == 0) {
+            keep = ov;
+        } else if (predicate == 1) {
+            keep = ov || in != 0;
+        } else {
+            if (ov) {
+                keep = include_overlap != 0;
+                type = 1;
+            } else {
+                keep = (in & 1u) != 0;
+            }
+        }
+    }
+    uint64_t m = __ballot(keep);
+    if (m == 0) return;
+    unsigned long long base = 0;
+    const int leader = __ffsll((long long)m) - 1;
+    if (__lane_id() == leader) base = atomicAdd(count, (unsigned long long)__popcll(m));
+    base = __shfl((long long)base, leader, 64);
+    if (keep) {
+        uint64_t pos = base + lanes_below(m);
+        out_keys[pos] = pairs[i];
+        out_type[pos] = type;
+    }
+}
+
+// Tasks per pair: chunks of source edges in the enabled directions.
+__global__ void k_task_counts(const uint64_t* __restrict__ pairs, uint64_t np, const uint32_t* __restrict__ locA,
+                              const uint32_t* __restrict__ esegA, const uint32_t* __restrict__ locB,
+                              const uint32_t* __restrict__ esegB, int dir_mask, uint32_t* __restrict__ counts) {
+    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= np) return;
+    uint64_t key = pairs[i];
+    uint32_t la = locA[(uint32_t)(key >> 32)], lb = locB[(uint32_t)key];
+    uint32_t ea = esegA[la + 1] - esegA[la], eb = esegB[lb + 1] - esegB[lb];
+    uint32_t c = 0;
+    if (dir_mask & 1) c += (ea + kChunk - 1) / kChunk;
+    if (dir_mask & 2) c += (eb + kChunk - 1) / kChunk;
+    counts[i] = c;
+}
+
+// ------------------------------------------------- global (reference-shaped)
+__device__ __forceinline__ uint64_t fmix64(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return k;
+}
+
+// Insert into a power-of-two open-addressing set; false when the probe
+// budget runs out (the caller flags overflow and re-runs exactly).
+__device__ __forceinline__ bool hash_insert(unsigned long long* table, uint64_t mask, uint64_t key) {
+    uint64_t h = fmix64(key) & mask;
+    for (uint32_t probe = 0; probe < 4096; ++probe) {
+        unsigned long long cur = table[h];
+        if (cur == (unsigned long long)key) return true;
+        if (cur == ~0ull) {
+            unsigned long long prev = atomicCAS(&table[h], ~0ull, (unsigned long long)key);
+            if (prev == ~0ull || prev == (unsigned long long)key) return true;
+        }
+        h = (h + 1) & mask;
+    }
+    return false;
+}
+
+// Mode 0: count hits per edge; 1: write keys at off[e]; 2: hash insert;
+// 3: count leaves whose boxes meet the edge box (candidate tests, no predicate).
+// One lane per source edge; every hit on a distinct target object kept
+// (consecutive hits on one object collapse).
+template <int STACK, bool F32, int Mode>
+__global__ __launch_bounds__(kBlock) void k_trace_global(
+    const float* __restrict__ es, const float* __restrict__ ee, const int32_t* __restrict__ eo, uint64_t ne,
+    const Node* __restrict__ nodes, const LeafTri* __restrict__ leaves, uint64_t ntri, int swap, int allow_f32,
+    uint32_t* __restrict__ counts, const uint64_t* __restrict__ off, uint64_t* __restrict__ out,
+    unsigned long long* __restrict__ table, uint64_t mask, uint32_t* __restrict__ overflow,
+    unsigned long long* __restrict__ stats) {
+    __shared__ int stk[kWavesPerBlock * STACK * PZ_WAVE];
+    const int lane = __lane_id(), w = threadIdx.x / PZ_WAVE;
+    int* my = stk + (w * STACK) * PZ_WAVE + lane;
+    const uint64_t e = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    bool act = e < ne && ntri > 0 && eo[e] >= 0;
+    LaneStats st{0, 0, 0, 0, 0};
+    float3 pp = make_float3(0, 0, 0), qq = pp;
+    float elo[3] = {0, 0, 0}, ehi[3] = {0, 0, 0};
+    int32_t src = 0;
+    if (act) {
+        pp = make_float3(es[3 * e], es[3 * e + 1], es[3 * e + 2]);
+        qq = make_float3(ee[3 * e], ee[3 * e + 1], ee[3 * e + 2]);
+        src = eo[e];
+        elo[0] = fminf(pp.x, qq.x);
+        elo[1] = fminf(pp.y, qq.y);
+        elo[2] = fminf(pp.z, qq.z);
+        ehi[0] = fmaxf(pp.x, qq.x);
+        ehi[1] = fmaxf(pp.y, qq.y);
+        ehi[2] = fmaxf(pp.z, qq.z);
+        ++st.traced;
+    }
+    int cur = ntri >= 2 ? 0 : ~0;
+    if (ntri == 1 && act) {
+        // single-triangle set: the root is the leaf; its box is tested here
+        const LeafTri L = leaves[0];
+        float bl[3] = {fminf(L.v0.x, fminf(L.v1.x, L.v2.x)), fminf(L.v0.y, fminf(L.v1.y, L.v2.y)),
+                       fminf(L.v0.z, fminf(L.v1.z, L.v2.z))};
+        float bh[3] = {fmaxf(L.v0.x, fmaxf(L.v1.x, L.v2.x)), fmaxf(L.v0.y, fmaxf(L.v1.y, L.v2.y)),
+                       fmaxf(L.v0.z, fmaxf(L.v1.z, L.v2.z))};
+        act = boxes_meet(elo, ehi, bl[0], bl[1], bl[2], bh[0], bh[1], bh[2]);
+    }
+    int sp = 0;
+    uint32_t n = 0;
+    int32_t last = -1;
+    uint64_t base = (Mode == 1 && e < ne) ? off[e] : 0;
+    while (act) {
+        if (cur >= 0) {
+            const Node nd = nodes[cur];
+            bool h0 = boxes_meet(elo, ehi, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
+            bool h1 = boxes_meet(elo, ehi, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w);
+            if (h0 && h1) {
+                my[sp * PZ_WAVE] = nd.d.y;
+                ++sp;
+                cur = nd.d.x;
+                continue;
+            } else if (h0) {
+                cur = nd.d.x;
+                continue;
+            } else if (h1) {
+                cur = nd.d.y;
+                continue;
+            }
+        } else {
+            const LeafTri L = leaves[~cur];
+            const int32_t tgt = __float_as_int(L.v1.w);
+            if (Mode == 3) {
+                ++n;  // candidate count only: boxes meet
+            } else if (tgt >= 0 && tgt != last && leaf_test<F32>(L, pp, qq, allow_f32 != 0, st)) {
+                last = tgt;
+                uint64_t key = swap ? pack_pair(tgt, src) : pack_pair(src, tgt);
+                if (Mode == 1) out[base + n] = key;
+                if (Mode == 2 && !hash_insert(table, mask, key)) *overflow = 1u;
+                ++n;
+            }
+        }
+        if (sp == 0) break;
+        --sp;
+        cur = my[sp * PZ_WAVE];
+    }
+    if ((Mode == 0 || Mode == 3) && e < ne) counts[e] = n;
+    flush_stats(st, stats);
+}
+
+__global__ __launch_bounds__(kBlock) void k_compact_table(const unsigned long long* __restrict__ table, uint64_t size,
+                                                          uint64_t* __restrict__ out, uint64_t cap,
+                                                          unsigned long long* __restrict__ count) {
+    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    bool keep = i < size && table[i] != ~0ull;
+    uint64_t m = __ballot(keep);
+    if (m == 0) return;
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned long long base = 0;
+    if (__lane_id() == leader) base = atomicAdd(count, (unsigned long long)__popcll(m));
+    base = __shfl((long long)base, leader, 64);
+    if (keep) {
+        uint64_t pos = base + lanes_below(m);
+        if (pos < cap) out[pos] = table[i];
+    }
+}
+
+template <class T>
+__global__ void k_fill_any(T* p, T v, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_init_boxes(uint32_t* obox, int64_t n) {
+    int64_t o = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (o >= n) return;
+    for (int a = 0; a < 3; ++a) {
+        obox[6 * o + a] = 0xFF800000u;      // ordered +inf
+        obox[6 * o + 3 + a] = 0x007FFFFFu;  // ordered -inf
+    }
+}
+
+template <int STACK, bool F32>
+void launch_narrow(unsigned g, hipStream_t s, const uint64_t* pairs, uint64_t np, const uint64_t* task_off, SetView A,
+                   SetView B, uint32_t* found, int dir_mask, int allow_f32, unsigned long long* stats) {
+    hipLaunchKernelGGL((k_narrow<STACK, F32>), dim3(g), dim3(kBlock), 0, s, pairs, np, task_off, A, B, found, dir_mask,
+                       allow_f32, stats);
+}
+
+template <int STACK, bool F32>
+void launch_trace(int mode, unsigned g, hipStream_t s, const float* es, const float* ee, const int32_t* eo,
+                  uint64_t ne, const Node* nodes, const LeafTri* leaves, uint64_t ntri, int swap, int allow_f32,
+                  uint32_t* counts, const uint64_t* off, uint64_t* out, unsigned long long* table, uint64_t mask,
+                  uint32_t* overflow, unsigned long long* stats) {
+    if (mode == 0)
+        hipLaunchKernelGGL((k_trace_global<STACK, F32, 0>), dim3(g), dim3(kBlock), 0, s, es, ee, eo, ne, nodes, leaves,
+                           ntri, swap, allow_f32, counts, off, out, table, mask, overflow, stats);
+    else if (mode == 1)
+        hipLaunchKernelGGL((k_trace_global<STACK, F32, 1>), dim3(g), dim3(kBlock), 0, s, es, ee, eo, ne, nodes, leaves,
+                           ntri, swap, allow_f32, counts, off, out, table, mask, overflow, stats);
+    else if (mode == 2)
+        hipLaunchKernelGGL((k_trace_global<STACK, F32, 2>), dim3(g), dim3(kBlock), 0, s, es, ee, eo, ne, nodes, leaves,
+                           ntri, swap, allow_f32, counts, off, out, table, mask, overflow, stats);
+    else
+        hipLaunchKernelGGL((k_trace_global<STACK, F32, 3>), dim3(g), dim3(kBlock), 0, s, es, ee, eo, ne, nodes, leaves,
+                           ntri, swap, allow_f32, counts, off, out, table, mask, overflow, stats);
+}
+
+}  // namespace
+
+namespace pzk {
+
+int stack_for_height(uint32_t h) { return h < 32 ? 32 : (h < 64 ? 64 : 0); }
+
+void object_stats(const float* verts, uint64_t nv, const uint32_t* idx, const int32_t* t2o, uint64_t nt, int32_t nobj,
+                  uint32_t* obox, uint32_t* tri_cnt, uint32_t* first_tri, hipStream_t s) {
+    if (nobj <= 0) return;
+    hipLaunchKernelGGL(k_init_boxes, dim3(grid1((uint64_t)nobj)), dim3(kBlock), 0, s, obox, (int64_t)nobj);
+    hipLaunchKernelGGL((k_fill_any<uint32_t>), dim3(grid1((uint64_t)nobj)), dim3(kBlock), 0, s, tri_cnt, 0u,
+                       (uint64_t)nobj);
+    hipLaunchKernelGGL((k_fill_any<uint32_t>), dim3(grid1((uint64_t)nobj)), dim3(kBlock), 0, s, first_tri,
+                       0xFFFFFFFFu, (uint64_t)nobj);
+    if (nt)
+        hipLaunchKernelGGL(k_object_stats, dim3(grid1(nt)), dim3(kBlock), 0, s, verts, nv, idx, t2o, nt, nobj, obox,
+                           tri_cnt, first_tri);
+}
+
+void edge_count(const int32_t* eo, uint64_t ne, int32_t nobj, uint32_t* edge_cnt, hipStream_t s) {
+    if (nobj <= 0) return;
+    hipLaunchKernelGGL((k_fill_any<uint32_t>), dim3(grid1((uint64_t)nobj)), dim3(kBlock), 0, s, edge_cnt, 0u,
+                       (uint64_t)nobj);
+    if (ne) hipLaunchKernelGGL(k_edge_count, dim3(grid1(ne)), dim3(kBlock), 0, s, eo, ne, nobj, edge_cnt);
+}
+
+void scene_box(const uint32_t* obox, int32_t nobj, uint32_t* scene, hipStream_t s) {
+    hipLaunchKernelGGL(k_init_boxes, dim3(1), dim3(kBlock), 0, s, scene, (int64_t)1);
+    if (nobj > 0) hipLaunchKernelGGL(k_scene_box, dim3(grid1((uint64_t)nobj)), dim3(kBlock), 0, s, obox, nobj, scene);
+}
+
+void object_keys(const uint32_t* obox, int32_t nobj, const uint32_t* scene, uint64_t* keys, uint32_t* vals,
+                 hipStream_t s) {
+    if (nobj > 0)
+        hipLaunchKernelGGL(k_object_keys, dim3(grid1((uint64_t)nobj)), dim3(kBlock), 0, s, obox, nobj, scene, keys,
+                           vals);
+}
+
+void broad_count(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const Node* nodesB, const uint32_t* orderB,
+                 int32_t nB, uint32_t* cnt, hipStream_t s) {
+    if (nA > 0)
+        hipLaunchKernelGGL((k_broad<0>), dim3(grid1((uint64_t)nA)), dim3(kBlock), 0, s, oboxA, nA, oboxB, nodesB, orderB, nB,
+                           cnt, (const uint64_t*)nullptr, (uint64_t*)nullptr, (unsigned long long*)nullptr,
+                           (uint64_t)0);
+}
+void broad_write(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const Node* nodesB, const uint32_t* orderB,
+                 int32_t nB, const uint64_t* off, uint64_t* out, hipStream_t s) {
+    if (nA > 0)
+        hipLaunchKernelGGL((k_broad<1>), dim3(grid1((uint64_t)nA)), dim3(kBlock), 0, s, oboxA, nA, oboxB, nodesB, orderB, nB,
+                           (uint32_t*)nullptr, off, out, (unsigned long long*)nullptr, (uint64_t)0);
+}
+void broad_append(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const Node* nodesB,
+                  const uint32_t* orderB, int32_t nB, uint64_t* out, unsigned long long* cursor, uint64_t cap,
+                  hipStream_t s) {
+    if (nA > 0)
+        hipLaunchKernelGGL((k_broad<2>), dim3(grid1((uint64_t)nA)), dim3(kBlock), 0, s, oboxA, nA, oboxB, nodesB, orderB, nB,
+                           (uint32_t*)nullptr, (const uint64_t*)nullptr, out, cursor, cap);
+}
+
+void mark_needed(const uint64_t* pairs, uint64_t np, uint32_t* needA, uint32_t* needB, hipStream_t s) {
+    if (np) hipLaunchKernelGGL(k_mark_needed, dim3(grid1(np)), dim3(kBlock), 0, s, pairs, np, needA, needB);
+}
+
+void needed_layout(const uint32_t* need, const uint32_t* loc, int32_t nobj, const uint32_t* tri_cnt,
+                   const uint32_t* edge_cnt, int32_t* obj_of_loc, uint32_t* seg_tri_cnt, uint32_t* seg_edge_cnt,
+                   hipStream_t s) {
+    if (nobj > 0)
+        hipLaunchKernelGGL(k_needed_layout, dim3(grid1((uint64_t)nobj)), dim3(kBlock), 0, s, need, loc, nobj, tri_cnt,
+                           edge_cnt, obj_of_loc, seg_tri_cnt, seg_edge_cnt);
+}
+
+void tri_keys_two_level(const float* verts, uint64_t nv, const uint32_t* idx, const int32_t* t2o, uint64_t nt,
+                        int32_t nobj, const uint32_t* need, const uint32_t* loc, const uint32_t* obox,
+                        const uint32_t* seg_off, uint32_t* cursor, uint64_t* keys, uint32_t* vals, hipStream_t s) {
+    if (nt)
+        hipLaunchKernelGGL(k_tri_keys_two_level, dim3(grid1(nt)), dim3(kBlock), 0, s, verts, idx, t2o, nt, nobj, need,
+                           loc, obox, seg_off, cursor, keys, vals, nv);
+}
+
+void tri_keys_global(const float* verts, const uint32_t* idx, uint64_t nt, const uint32_t* scene, uint64_t* keys,
+                     uint32_t* vals, hipStream_t s) {
+    if (nt) hipLaunchKernelGGL(k_tri_keys_global, dim3(grid1(nt)), dim3(kBlock), 0, s, verts, idx, nt, scene, keys, vals);
+}
+
+void edge_scatter(const float* es, const float* ee, const int32_t* eo, uint64_t ne, int32_t nobj,
+                  const uint32_t* need, const uint32_t* loc, const uint32_t* eseg_off, uint32_t* cursor,
+                  EdgeRec* out, hipStream_t s) {
+    if (ne)
+        hipLaunchKernelGGL(k_edge_scatter, dim3(grid1(ne)), dim3(kBlock), 0, s, es, ee, eo, ne, nobj, need, loc,
+                           eseg_off, cursor, out);
+}
+
+void task_counts(const uint64_t* pairs, uint64_t np, const uint32_t* locA, const uint32_t* esegA,
+                 const uint32_t* locB, const uint32_t* esegB, int dir_mask, uint32_t* counts, hipStream_t s) {
+    if (np)
+        hipLaunchKernelGGL(k_task_counts, dim3(grid1(np)), dim3(kBlock), 0, s, pairs, np, locA, esegA, locB, esegB,
+                           dir_mask, counts);
+}
+
+bool narrow(const uint64_t* pairs, uint64_t np, const uint64_t* task_off, uint64_t ntasks, const SetView& A,
+            const SetView& B, uint32_t* found, int dir_mask, bool f32, int allow_f32, uint32_t height,
+            unsigned long long* stats, hipStream_t s) {
+    if (ntasks == 0) return true;
+    int stack = stack_for_height(height);
+    if (stack == 0) return false;
+    unsigned g = (unsigned)((ntasks + kWavesPerBlock - 1) / kWavesPerBlock);
+    if (stack == 32) {
+        if (f32)
+            launch_narrow<32, true>(g, s, pairs, np, task_off, A, B, found, dir_mask, allow_f32, stats);
+        else
+            launch_narrow<32, false>(g, s, pairs, np, task_off, A, B, found, dir_mask, allow_f32, stats);
+    } else {
+        if (f32)
+            launch_narrow<64, true>(g, s, pairs, np, task_off, A, B, found, dir_mask, allow_f32, stats);
+        else
+            launch_narrow<64, false>(g, s, pairs, np, task_off, A, B, found, dir_mask, allow_f32, stats);
+    }
+    return true;
+}
+
+bool parity(const uint64_t* pairs, uint64_t np, const uint32_t* found, const SetView& A, const SetView& B,
+            const float* vA, const uint32_t* iA, const uint32_t* ftA, const float* vB, const uint32_t* iB,
+            const uint32_t* ftB, int kind, uint32_t* inside, uint32_t height, unsigned long long* stats,
+            hipStream_t s) {
+    if (np == 0) return true;
+    int stack = stack_for_height(height);
+    if (stack == 0) return false;
+    uint64_t n = np * (kind == 1 ? 2 : 1);
+    if (stack == 32)
+        hipLaunchKernelGGL((k_parity<32>), dim3(grid1(n)), dim3(kBlock), 0, s, pairs, np, found, A, B, vA, iA, ftA, vB,
+                           iB, ftB, kind, inside, stats);
+    else
+        hipLaunchKernelGGL((k_parity<64>), dim3(grid1(n)), dim3(kBlock), 0, s, pairs, np, found, A, B, vA, iA, ftA, vB,
+                           iB, ftB, kind, inside, stats);
+    return true;
+}
+
+void select_pairs(const uint64_t* pairs, uint64_t np, const uint32_t* found, const uint32_t* inside, int predicate,
+                  int include_overlap, uint64_t* out_keys, uint32_t* out_type, unsigned long long* count,
+                  hipStream_t s) {
+    if (np)
+        hipLaunchKernelGGL(k_select_pairs, dim3(grid1(np)), dim3(kBlock), 0, s, pairs, np, found, inside, predicate,
+                           include_overlap, out_keys, out_type, count);
+}
+
+bool trace_global(int mode, const float* es, const float* ee, const int32_t* eo, uint64_t ne, const Node* nodes,
+                  const LeafTri* leaves, uint64_t ntri, uint32_t height, int swap, bool f32, int allow_f32,
+                  uint32_t* counts, const uint64_t* off, uint64_t* out, unsigned long long* table, uint64_t mask,
+                  uint32_t* overflow, unsigned long long* stats, hipStream_t s) {
+    if (ne == 0) return true;
+    int stack = stack_for_height(height);
+    if (stack == 0) return false;
+    unsigned g = grid1(ne);
+    if (stack == 32) {
+        if (f32)
+            launch_trace<32, true>(mode, g, s, es, ee, eo, ne, nodes, leaves, ntri, swap, allow_f32, counts, off, out,
+                                   table, mask, overflow, stats);
+        else
+            launch_trace<32, false>(mode, g, s, es, ee, eo, ne, nodes, leaves, ntri, swap, allow_f32, counts, off, out,
+                                    table, mask, overflow, stats);
+    } else {
+        if (f32)
+            launch_trace<64, true>(mode, g, s, es, ee, eo, ne, nodes, leaves, ntri, swap, allow_f32, counts, off, out,
+                                   table, mask, overflow, stats);
+        else
+            launch_trace<64, false>(mode, g, s, es, ee, eo, ne, nodes, leaves, ntri, swap, allow_f32, counts, off, out,
+                                    table, mask, overflow, stats);
+    }
+    return true;
+}
+
+void compact_table(const unsigned long long* table, uint64_t size, uint64_t* out, uint64_t cap,
+                   unsigned long long* count, hipStream_t s) {
+    if (size) hipLaunchKernelGGL(k_compact_table, dim3(grid1(size)), dim3(kBlock), 0, s, table, size, out, cap, count);
+}
+
+void fill_ull(unsigned long long* p, unsigned long long v, uint64_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL((k_fill_any<unsigned long long>), dim3(grid1(n)), dim3(kBlock), 0, s, p, v, n);
+}
+
+}  // namespace pzk

@@ -1,0 +1,75 @@
+// Launchers of the join kernels (join.hip).
+#pragma once
+
+#include "pz_kernels_bvh.h"
+
+namespace pzd {
+// Device view of one set's per-object acceleration data for a two-level join.
+struct SetView {
+    const Node* nodes;
+    const LeafTri* leaves;
+    const uint32_t* seg_off;   // per needed object: first sorted triangle (n + 1 entries)
+    const EdgeRec* edges;      // edges grouped per needed object
+    const uint32_t* eseg_off;  // per needed object: first edge (n + 1 entries)
+    const uint32_t* loc;       // object id -> needed-object index
+    const uint32_t* obox;      // ordered-uint boxes by object id (6 per object)
+};
+}  // namespace pzd
+
+namespace pzk {
+
+using pzd::EdgeRec;
+using pzd::LeafTri;
+using pzd::Node;
+using pzd::SetView;
+
+// Stack depth for a BVH of the given height (0: unsupported).
+int stack_for_height(uint32_t h);
+
+void object_stats(const float* verts, uint64_t nv, const uint32_t* idx, const int32_t* t2o, uint64_t nt, int32_t nobj,
+                  uint32_t* obox, uint32_t* tri_cnt, uint32_t* first_tri, hipStream_t s);
+void edge_count(const int32_t* eo, uint64_t ne, int32_t nobj, uint32_t* edge_cnt, hipStream_t s);
+void scene_box(const uint32_t* obox, int32_t nobj, uint32_t* scene, hipStream_t s);
+void object_keys(const uint32_t* obox, int32_t nobj, const uint32_t* scene, uint64_t* keys, uint32_t* vals,
+                 hipStream_t s);
+// Broad phase: mesh1 object boxes against the mesh2 object LBVH (stack of 64).
+void broad_count(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const Node* nodesB, const uint32_t* orderB,
+                 int32_t nB, uint32_t* cnt, hipStream_t s);
+void broad_write(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const Node* nodesB, const uint32_t* orderB,
+                 int32_t nB, const uint64_t* off, uint64_t* out, hipStream_t s);
+void broad_append(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const Node* nodesB,
+                  const uint32_t* orderB, int32_t nB, uint64_t* out, unsigned long long* cursor, uint64_t cap,
+                  hipStream_t s);
+void mark_needed(const uint64_t* pairs, uint64_t np, uint32_t* needA, uint32_t* needB, hipStream_t s);
+void needed_layout(const uint32_t* need, const uint32_t* loc, int32_t nobj, const uint32_t* tri_cnt,
+                   const uint32_t* edge_cnt, int32_t* obj_of_loc, uint32_t* seg_tri_cnt, uint32_t* seg_edge_cnt,
+                   hipStream_t s);
+void tri_keys_two_level(const float* verts, uint64_t nv, const uint32_t* idx, const int32_t* t2o, uint64_t nt,
+                        int32_t nobj, const uint32_t* need, const uint32_t* loc, const uint32_t* obox,
+                        const uint32_t* seg_off, uint32_t* cursor, uint64_t* keys, uint32_t* vals, hipStream_t s);
+void tri_keys_global(const float* verts, const uint32_t* idx, uint64_t nt, const uint32_t* scene, uint64_t* keys,
+                     uint32_t* vals, hipStream_t s);
+void edge_scatter(const float* es, const float* ee, const int32_t* eo, uint64_t ne, int32_t nobj,
+                  const uint32_t* need, const uint32_t* loc, const uint32_t* eseg_off, uint32_t* cursor,
+                  EdgeRec* out, hipStream_t s);
+void task_counts(const uint64_t* pairs, uint64_t np, const uint32_t* locA, const uint32_t* esegA,
+                 const uint32_t* locB, const uint32_t* esegB, int dir_mask, uint32_t* counts, hipStream_t s);
+bool narrow(const uint64_t* pairs, uint64_t np, const uint64_t* task_off, uint64_t ntasks, const SetView& A,
+            const SetView& B, uint32_t* found, int dir_mask, bool f32, int allow_f32, uint32_t height,
+            unsigned long long* stats, hipStream_t s);
+bool parity(const uint64_t* pairs, uint64_t np, const uint32_t* found, const SetView& A, const SetView& B,
+            const float* vA, const uint32_t* iA, const uint32_t* ftA, const float* vB, const uint32_t* iB,
+            const uint32_t* ftB, int kind, uint32_t* inside, uint32_t height, unsigned long long* stats,
+            hipStream_t s);
+void select_pairs(const uint64_t* pairs, uint64_t np, const uint32_t* found, const uint32_t* inside, int predicate,
+                  int include_overlap, uint64_t* out_keys, uint32_t* out_type, unsigned long long* count,
+                  hipStream_t s);
+bool trace_global(int mode, const float* es, const float* ee, const int32_t* eo, uint64_t ne, const Node* nodes,
+                  const LeafTri* leaves, uint64_t ntri, uint32_t height, int swap, bool f32, int allow_f32,
+                  uint32_t* counts, const uint64_t* off, uint64_t* out, unsigned long long* table, uint64_t mask,
+                  uint32_t* overflow, unsigned long long* stats, hipStream_t s);
+void compact_table(const unsigned long long* table, uint64_t size, uint64_t* out, uint64_t cap,
+                   unsigned long long* count, hipStream_t s);
+void fill_ull(unsigned long long* p, unsigned long long v, uint64_t n, hipStream_t s);
+
+}  // namespace pzk
