@@ -156,6 +156,9 @@ typedef struct pz_dataset pz_dataset;
 int32_t   pz_device_count(void);
 pz_status pz_ctx_create(int32_t device, pz_ctx** out);
 void      pz_ctx_destroy(pz_ctx* ctx);
+/* Free and total HBM of the context's device (sizing from free memory,
+ * pierce/query/src/applications/pierce_overlap_main.cpp:474-489). */
+pz_status pz_ctx_mem_info(pz_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
 
 /* Upload one dataset to HBM (GeometryUploader::upload + PrecomputedEdgeData).
  * A dataset without edges is rejected with PZ_E_NO_EDGES

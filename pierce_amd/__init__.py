@@ -110,6 +110,7 @@ _SIGS = [
     ("pz_device_count", C.c_int32, []),
     ("pz_ctx_create", C.c_int32, [C.c_int32, C.POINTER(C.c_void_p)]),
     ("pz_ctx_destroy", None, [C.c_void_p]),
+    ("pz_ctx_mem_info", C.c_int32, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("pz_dataset_upload", C.c_int32, [C.c_void_p, C.POINTER(MeshView), C.POINTER(C.c_void_p)]),
     ("pz_dataset_load_pre", C.c_int32, [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p)]),
     ("pz_dataset_release", None, [C.c_void_p]),

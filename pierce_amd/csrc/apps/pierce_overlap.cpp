@@ -1,0 +1,165 @@
+// pierce_overlap: estimate-sized overlap join
+// (pierce/query/src/applications/pierce_overlap_main.cpp:313-749).  The grid
+// selectivity estimate (or --hash-table-size / --hash-table-free-mem-fraction)
+// sizes the first-pass buffer; an overflow falls back to the exact count, so
+// the result never loses pairs.
+#include <cmath>
+#include <iostream>
+
+#include "app_common.hpp"
+
+int main(int argc, char** argv) {
+    app::Timer timer;
+    timer.start("Data Reading");
+    app::Common c;
+    c.output = "estimated_overlap_timing.json";
+    float gamma = 0.8f, epsilon = 0.001f, free_frac = 0.f;
+    std::string direction = "both", pairs_out;
+    bool track = false, no_alpha = false, estimate_only = false;
+    unsigned long long table_size = 0;
+    int max_iter = 100;
+    bool bad = false;
+    app::parse(argc, argv, c, false, [&](const std::string& a, int& i, int argc_, char** argv_) {
+        auto has = [&]() { return i + 1 < argc_; };
+        if (a == "--gamma" && has()) gamma = std::stof(argv_[++i]);
+        else if (a == "--epsilon" && has()) epsilon = std::stof(argv_[++i]);
+        else if (a == "--query-direction" && has()) direction = argv_[++i];
+        else if (a == "--pairs-output" && has()) pairs_out = argv_[++i];
+        else if (a == "--track-hash-contention") track = true;
+        else if (a == "--no-alpha-correction") no_alpha = true;
+        else if (a == "--hash-table-size" && has()) table_size = std::stoull(argv_[++i]);
+        else if (a == "--hash-table-free-mem-fraction" && has()) {
+            free_frac = std::stof(argv_[++i]);
+            if (!(free_frac >= 0.f && free_frac <= 1.f)) bad = true;
+        } else if (a == "--overlap-max-iterations" && has()) max_iter = std::atoi(argv_[++i]);
+        else if (a == "--estimate-only") estimate_only = true;
+        else return false;
+        return true;
+    });
+    if (c.help) {
+        auto o = app::common_help(false);
+        o.insert(o.end() - 1, {{"--gamma <float>", "Estimation gamma (default: 0.8)"},
+                               {"--epsilon <float>", "Estimation epsilon (default: 0.001)"},
+                               {"--query-direction <both|mesh1_to_mesh2|mesh2_to_mesh1>", "Query direction"},
+                               {"--pairs-output <path>", "CSV of result pairs (object_id_mesh1,object_id_mesh2)"},
+                               {"--track-hash-contention", "Accepted; the two-level join uses no hash table"},
+                               {"--no-alpha-correction", "Disable the replication correction of the estimate"},
+                               {"--hash-table-size <n>", "Fixed first-pass capacity"},
+                               {"--hash-table-free-mem-fraction <f>", "Capacity from a fraction of free HBM"},
+                               {"--overlap-max-iterations <n>", "Accepted; every hit is enumerated exactly"},
+                               {"--estimate-only", "Run only the selectivity estimation"}});
+        app::print_help(argv[0], "--mesh1 <path> --mesh2 <path> [options]",
+                        "Estimate-sized mesh overlap join.", o);
+        return 0;
+    }
+    if (bad) {
+        std::cerr << "Error: --hash-table-free-mem-fraction must be in [0, 1]" << std::endl;
+        return 1;
+    }
+    int dir = direction == "both" ? PZ_DIR_BOTH
+              : direction == "mesh1_to_mesh2" ? PZ_DIR_1TO2
+              : direction == "mesh2_to_mesh1" ? PZ_DIR_2TO1
+                                              : -1;
+    if (dir < 0) {
+        std::cerr << "Error: invalid --query-direction '" << direction
+                  << "'. Valid values are: both, mesh1_to_mesh2, mesh2_to_mesh1" << std::endl;
+        return 1;
+    }
+    (void)max_iter;
+    if (c.mesh1.empty() || c.mesh2.empty()) {
+        std::cerr << "Error: --mesh1 and --mesh2 are required" << std::endl;
+        return 1;
+    }
+    timer.next("Application Creation");
+    pz_ctx* ctx = nullptr;
+    if (pz_ctx_create(c.device, &ctx) != PZ_OK) return app::fail_join("device initialisation");
+    uint64_t nt1 = 0, nt2 = 0, no1 = 0, no2 = 0;
+    pz_dataset* d1 = app::load(ctx, c.mesh1, "Mesh1", timer, &nt1, &no1);
+    if (!d1) return 1;
+    pz_dataset* d2 = app::load(ctx, c.mesh2, "Mesh2", timer, &nt2, &no2);
+    if (!d2) return 1;
+
+    pz_join_desc d;
+    pz_join_desc_defaults(&d);
+    d.strategy = c.strategy;
+    d.precision = c.precision;
+    d.direction = dir;
+    d.gamma = gamma;
+    d.epsilon = epsilon;
+    d.no_alpha_correction = no_alpha ? 1 : 0;
+    d.sizing = PZ_SIZING_ESTIMATE;
+    const char* sizing_line = "Using Direct Estimated Hash Table Size: ";
+    if (table_size > 0) {
+        d.sizing = PZ_SIZING_FIXED;
+        d.capacity = table_size;
+        sizing_line = "Using Manual Hash Table Size: ";
+    } else if (free_frac > 0.f) {
+        uint64_t fb = 0, tb = 0;
+        if (pz_ctx_mem_info(ctx, &fb, &tb) == PZ_OK) {
+            d.sizing = PZ_SIZING_FIXED;
+            d.capacity = std::max<uint64_t>(1024, (uint64_t)(free_frac * (double)fb) / 8);
+            sizing_line = "Using Free GPU Memory Hash Table Size: ";
+        }
+    }
+    if (c.sizing_override >= 0 && c.sizing_override != PZ_SIZING_LEARNED) d.sizing = c.sizing_override;
+    if (c.strategy == PZ_STRATEGY_GLOBAL && d.sizing != PZ_SIZING_TWO_PASS) d.dedup = PZ_DEDUP_HASH;
+
+    pz_join_result r;
+    if (estimate_only) {
+        d.sizing = PZ_SIZING_ESTIMATE;
+        timer.next("Selectivity Estimation");
+        if (pz_join(ctx, d1, d2, &d, &r) != PZ_OK) return app::fail_join("estimate");
+        std::cout << "\n=== Selectivity Estimation (Overlap - Direct) ===" << std::endl;
+        std::cout << "Final Estimated Pairs:     " << r.predicted << std::endl;
+        pz_join_result_release(&r);
+        timer.finish(c.output);
+        return 0;
+    }
+    timer.next("Warmup");
+    if (c.warmup > 0) std::cout << "Running " << c.warmup << " warmup iterations (estimation + hash query)..." << std::endl;
+    for (int w = 0; w < c.warmup; ++w) {
+        if (pz_join(ctx, d1, d2, &d, &r) != PZ_OK) return app::fail_join("warmup query");
+        pz_join_result_release(&r);
+    }
+    timer.next("Query");
+    for (int run = 0; run < c.runs; ++run) {
+        if (run) pz_join_result_release(&r);
+        if (pz_join(ctx, d1, d2, &d, &r) != PZ_OK) return app::fail_join("query");
+        std::cout << "\n=== Selectivity Estimation (Overlap - Direct) ===" << std::endl;
+        std::cout << "Final Estimated Pairs:     " << r.predicted << std::endl;
+        std::cout << sizing_line << r.capacity << std::endl;
+        std::cout << "Hash Table Allocated Bytes: " << r.capacity * 8 << std::endl;
+        if (r.overflowed) std::cout << "Sized buffer overflowed; exact count used." << std::endl;
+        std::cout << "Hash Table Query found " << r.num_pairs << " unique pairs." << std::endl;
+    }
+    timer.next("Output");
+    app::add_phases(timer, r);
+    // Query-time keys the benchmark adapters sum (pierce_adapter.py:343-357).
+    timer.add("Raytrace_Hash_Mesh1ToMesh2", app::phase_ms(r, {"Broad", "Narrow", "Raytrace"}));
+    timer.add("compact_hash_table_pairs", app::phase_ms(r, {"compact"}));
+    std::cout << "\n=== Mesh Overlap Join Summary ===" << std::endl;
+    std::cout << "Mesh1 triangles: " << nt1 << std::endl;
+    std::cout << "Mesh1 objects: " << no1 << std::endl;
+    std::cout << "Mesh2 triangles: " << nt2 << std::endl;
+    std::cout << "Mesh2 objects: " << no2 << std::endl;
+    std::cout << "Query direction: " << direction << std::endl;
+    std::cout << "Hash contention tracking: " << (track ? "enabled" : "disabled") << std::endl;
+    std::cout << "Unique object pairs: " << r.num_pairs << std::endl;
+    std::cout << "Result Buffer Capacity: " << r.capacity << std::endl;
+    std::cout << "Result Buffer Allocated Bytes: " << r.capacity * sizeof(pz_pair) << std::endl;
+    std::cout << "Result Buffer Used Bytes: " << r.num_pairs * sizeof(pz_pair) << std::endl;
+    std::cout << "Hash Table Slots: " << r.capacity << std::endl;
+    std::cout << "Hash Table Allocated Bytes: " << r.capacity * 8 << std::endl;
+    if (!pairs_out.empty()) {
+        if (app::write_pairs_csv(pairs_out, "object_id_mesh1,object_id_mesh2", r))
+            std::cout << "Pair results written to: " << pairs_out << std::endl;
+    }
+    pz_join_result_release(&r);
+    pz_dataset_release(d1);
+    pz_dataset_release(d2);
+    pz_ctx_destroy(ctx);
+    timer.finish(c.output);
+    std::cout << "\nQuery completed in " << timer.total_ms() << " ms." << std::endl;
+    std::cout << "Results saved to: " << c.output << std::endl;
+    return 0;
+}

@@ -358,9 +358,11 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
     r->num_candidates = np;
 
     ph.begin("Build Mesh Index");
-    const int dir_mask = d.predicate == PZ_PRED_OVERLAP
-                             ? (d.direction == PZ_DIR_1TO2 ? 1 : (d.direction == PZ_DIR_2TO1 ? 2 : 3))
-                             : 3;
+    // bit 0: mesh1 edges vs mesh2 triangles (+ mesh1 points in mesh2 objects); bit 1: the reverse.
+    // Containment always needs both edge directions.
+    const int dir_mask = d.predicate == PZ_PRED_CONTAINMENT
+                             ? 3
+                             : (d.direction == PZ_DIR_1TO2 ? 1 : (d.direction == PZ_DIR_2TO1 ? 2 : 3));
     Needed na = needed_layout(c, A, pa, "A"), nb = needed_layout(c, B, pb, "B");
     const size_t sA = std::max(nA, 1), sB = std::max(nB, 1);
     pzk::fill_u32(na.need, 0u, sA, s);
@@ -449,7 +451,7 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         inside = buf<uint32_t>(c, "np.inside", np + 1);
         pzk::fill_u32(inside, 0u, np + 1, s);
         pzk::parity(pairs, np, found, va, vb, A->v, A->idx, pa.first_tri, B->v, B->idx, pb.first_tri,
-                    d.predicate == PZ_PRED_INTERSECTION ? 1 : 0, inside, height, stats, s);
+                    d.predicate == PZ_PRED_INTERSECTION ? 1 : 0, dir_mask, inside, height, stats, s);
     }
 
     ph.begin("Download Results");
@@ -566,7 +568,8 @@ void join_global(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const pz_j
             slots = std::max<uint64_t>(d.capacity, 1);
         else
             slots = (uint64_t)std::ceil((double)expected / lf);
-        uint64_t cap2 = 1024;
+        // power-of-two table; estimate-sized tables keep a 1024-slot floor
+        uint64_t cap2 = d.sizing == PZ_SIZING_FIXED ? 16 : 1024;
         while (cap2 < slots && cap2 < (1ull << 34)) cap2 <<= 1;
         r->capacity = cap2;
         unsigned long long* table = buf<unsigned long long>(c, "g.table", cap2);
@@ -658,6 +661,18 @@ pz_status pz_ctx_create(int32_t device, pz_ctx** out) {
         HIP_OK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
         for (auto& e : c->ev) HIP_OK(hipEventCreate(&e));
         *out = c.release();
+        return PZ_OK;
+    });
+}
+
+pz_status pz_ctx_mem_info(pz_ctx* c, uint64_t* free_bytes, uint64_t* total_bytes) {
+    if (!c || !free_bytes || !total_bytes) return fail(PZ_E_INVALID, "pz_ctx_mem_info: null argument");
+    return guarded([&]() -> pz_status {
+        HIP_OK(hipSetDevice(c->device));
+        size_t f = 0, t = 0;
+        HIP_OK(hipMemGetInfo(&f, &t));
+        *free_bytes = f;
+        *total_bytes = t;
         return PZ_OK;
     });
 }

@@ -492,7 +492,7 @@ __global__ __launch_bounds__(kBlock) void k_parity(const uint64_t* __restrict__ 
                                                    const float* __restrict__ vA, const uint32_t* __restrict__ iA,
                                                    const uint32_t* __restrict__ ftA, const float* __restrict__ vB,
                                                    const uint32_t* __restrict__ iB, const uint32_t* __restrict__ ftB,
-                                                   int kind, uint32_t* __restrict__ inside,
+                                                   int kind, int dir_mask, uint32_t* __restrict__ inside,
                                                    unsigned long long* __restrict__ stats) {
     __shared__ int stk[kWavesPerBlock * STACK * PZ_WAVE];
     const int lane = __lane_id(), w = threadIdx.x / PZ_WAVE;
@@ -501,7 +501,8 @@ __global__ __launch_bounds__(kBlock) void k_parity(const uint64_t* __restrict__ 
     const int ndir = kind == 1 ? 2 : 1;
     const uint64_t p = gid / ndir;
     const int dir = (int)(gid % ndir);  // 0: point of B in A; 1: point of A in B
-    bool act = p < np && !found[p];
+    // dir 0 (mesh2 point in mesh1 object) belongs to the mesh2->mesh1 direction, dir 1 to mesh1->mesh2
+    bool act = p < np && !found[p] && (kind == 0 || (dir_mask & (dir == 0 ? 2 : 1)));
     LaneStats st{0, 0, 0, 0, 0};
     float3 pt = make_float3(0, 0, 0);
     int cur = 0, sp = 0;
@@ -907,18 +908,18 @@ bool narrow(const uint64_t* pairs, uint64_t np, const uint64_t* task_off, uint64
 
 bool parity(const uint64_t* pairs, uint64_t np, const uint32_t* found, const SetView& A, const SetView& B,
             const float* vA, const uint32_t* iA, const uint32_t* ftA, const float* vB, const uint32_t* iB,
-            const uint32_t* ftB, int kind, uint32_t* inside, uint32_t height, unsigned long long* stats,
-            hipStream_t s) {
+            const uint32_t* ftB, int kind, int dir_mask, uint32_t* inside, uint32_t height,
+            unsigned long long* stats, hipStream_t s) {
     if (np == 0) return true;
     int stack = stack_for_height(height);
     if (stack == 0) return false;
     uint64_t n = np * (kind == 1 ? 2 : 1);
     if (stack == 32)
         hipLaunchKernelGGL((k_parity<32>), dim3(grid1(n)), dim3(kBlock), 0, s, pairs, np, found, A, B, vA, iA, ftA, vB,
-                           iB, ftB, kind, inside, stats);
+                           iB, ftB, kind, dir_mask, inside, stats);
     else
         hipLaunchKernelGGL((k_parity<64>), dim3(grid1(n)), dim3(kBlock), 0, s, pairs, np, found, A, B, vA, iA, ftA, vB,
-                           iB, ftB, kind, inside, stats);
+                           iB, ftB, kind, dir_mask, inside, stats);
     return true;
 }
 

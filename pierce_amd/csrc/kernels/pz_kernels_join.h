@@ -59,8 +59,8 @@ bool narrow(const uint64_t* pairs, uint64_t np, const uint64_t* task_off, uint64
             unsigned long long* stats, hipStream_t s);
 bool parity(const uint64_t* pairs, uint64_t np, const uint32_t* found, const SetView& A, const SetView& B,
             const float* vA, const uint32_t* iA, const uint32_t* ftA, const float* vB, const uint32_t* iB,
-            const uint32_t* ftB, int kind, uint32_t* inside, uint32_t height, unsigned long long* stats,
-            hipStream_t s);
+            const uint32_t* ftB, int kind, int dir_mask, uint32_t* inside, uint32_t height,
+            unsigned long long* stats, hipStream_t s);
 void select_pairs(const uint64_t* pairs, uint64_t np, const uint32_t* found, const uint32_t* inside, int predicate,
                   int include_overlap, uint64_t* out_keys, uint32_t* out_type, unsigned long long* count,
                   hipStream_t s);
