@@ -1,0 +1,243 @@
+#!/usr/bin/env python3
+"""Join throughput of the MI355X spatial-join engine on one GPU.
+
+Workload (default): A = 100,000 ellipsoid icospheres at subdivision 3 (1,280
+triangles each), radii U[0.5, 2], centres U[0, 1000)^3; B = 100,000 meshes of
+which 20% are copies of random A meshes scaled by U[0.3, 0.8] about their
+centroid and 80% fresh ellipsoids; fp32 vertices, seeded (A 42, B 43).
+
+One step = one overlap join, device-resident geometry -> sorted unique host
+pair set, including every index build.  Two paths run on the same GPU:
+  chosen : two-level join (object broad phase + per-object BVHs, early exit),
+           fp32-filtered exact predicates;
+  plain  : one BVH per set over all triangles, every edge traced, fp64-filtered
+           exact predicates throughout, exact two-pass count/scan/write/sort/unique.
+Both are checked against each other pair for pair; the CPU oracle (Face
+restatement) is timed and checked on a spatial sub-box of the same workload.
+
+Prints one JSON line (see the repository's measurement contract).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "join ms + Gtri-tests/s, chosen vs plain GPU path, 0 mismatched pairs"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes of one launch of each profiled kernel, from the .pre payload
+# units (12 B per vertex, 12 + 4 B per triangle, 28 B per edge) and the index
+# sizes the join reports.  Keys: kernel name -> f(counts) -> bytes per launch.
+def alg_bytes(kernel, k):
+    if kernel == "object_stats":    # indices + map per triangle, each vertex once; one launch per set
+        return (16 * k["T"] + 12 * k["V"]) / 2
+    if kernel == "edge_count":      # source-object ids; one launch per set
+        return 4 * k["E"] / 2
+    if kernel == "tri_keys":        # payload read + (key, triangle) written for indexed triangles
+        return (16 * k["T"] + 12 * k["V"] + 12 * k["IT"]) / 2
+    if kernel == "edge_scatter":    # ids of all edges + endpoints read and 32-B records written for indexed edges
+        return (4 * k["E"] + 24 * k["IE"] + 32 * k["IE"]) / 2
+    if kernel == "refit":           # per indexed triangle: order, key, indices, 3 vertices read; one node written
+        return (4 + 8 + 12 + 36 + 64) * k["IT"] / 2
+    if kernel == "gather_leaves":   # per indexed triangle: order, indices, 3 vertices, map read; 48 B written
+        return (4 + 12 + 36 + 4 + 48) * k["IT"] / 2
+    if kernel == "karras":          # keys read (~neighbour loads), node children + parents written
+        return (8 + 16 + 8) * k["IT"] / 2
+    if kernel == "narrow":          # every indexed edge record read once
+        return 32 * k["IE"]
+    return None
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def gen_sets(pz, n, with_grid=False):
+    common = dict(level_min=3, level_max=3, radius_min=0.5, radius_max=2.0, box=1000.0)
+    a = pz.generate(pz.GEN_ELLIPSOIDS, count=n, seed=42, **common)
+    b = pz.generate(pz.GEN_ELLIPSOIDS, count=n, seed=43, copy_fraction=0.2, copy_scale_min=0.3,
+                    copy_scale_max=0.8, copy_source_count=n, copy_source_seed=42, **common)
+    return a, b
+
+
+def sub_box(pz, m, side):
+    """Objects whose first vertex lies in [0, side)^3, re-numbered compactly."""
+    t2o = m.triangle_to_object
+    first = np.full(t2o.max() + 1, -1, np.int64)
+    order = np.arange(len(t2o))
+    first[t2o[::-1]] = order[::-1]
+    p0 = m.vertices[m.indices[first, 0]]
+    keep = np.all(p0 < side, axis=1)
+    new_id = np.cumsum(keep) - 1
+    tmask = keep[t2o]
+    idx = m.indices[tmask]
+    used = np.unique(idx)
+    remap = np.zeros(m.vertices.shape[0], np.uint32)
+    remap[used] = np.arange(len(used), dtype=np.uint32)
+    emask = keep[m.edge_objects]
+    return pz.Mesh(m.vertices[used], remap[idx], new_id[t2o[tmask]].astype(np.int32),
+                   m.edge_starts[emask], m.edge_ends[emask], new_id[m.edge_objects[emask]].astype(np.int32))
+
+
+def time_join(pz, ctx, da, db, steps, **desc):
+    outs = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        outs.append(pz.join(ctx, da, db, **desc))
+    dt = (time.perf_counter() - t0) / steps * 1000.0
+    return dt, outs[-1]
+
+
+def pair_mismatch(p, q):
+    a = set(map(tuple, p.tolist()))
+    b = set(map(tuple, q.tolist()))
+    return len(a ^ b)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--objects", type=int, default=100000, help="meshes per set")
+    ap.add_argument("--plain-steps", type=int, default=None, help="timed steps of the plain path (default: --steps)")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-side", type=float, default=500.0, help="side of the CPU sample sub-box")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the join runs on a single device (multi-GPU sharding is described "
+              "in DESIGN.md, not run); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import pierce_amd as pz
+
+    log(f"[bench] generating {args.objects} + {args.objects} meshes")
+    t = time.perf_counter()
+    A, B = gen_sets(pz, args.objects)
+    log(f"[bench] generated in {time.perf_counter() - t:.1f} s: T={A.indices.shape[0]}+{B.indices.shape[0]}")
+    counts = dict(V=A.vertices.shape[0] + B.vertices.shape[0], T=A.indices.shape[0] + B.indices.shape[0],
+                  E=A.edge_objects.shape[0] + B.edge_objects.shape[0])
+
+    with pz.Context(0) as ctx:
+        da, db = ctx.upload(A), ctx.upload(B)
+        log("[bench] uploaded; counting candidate tests")
+        n_cand = pz.count_candidate_tests(ctx, da, db)
+        log(f"[bench] N_cand = {n_cand}")
+
+        chosen = dict(strategy=pz.STRATEGY_TWO_LEVEL, precision=pz.PREC_FP32_FILTERED, sizing=pz.SIZING_TWO_PASS)
+        plain = dict(strategy=pz.STRATEGY_GLOBAL, precision=pz.PREC_FP64, sizing=pz.SIZING_TWO_PASS,
+                     dedup=pz.DEDUP_SORT)
+        for _ in range(args.warmup):
+            pz.join(ctx, da, db, **chosen)
+        log("[bench] chosen path warm; timing")
+        pz.profile_reset()
+        pz.profile_enable(True)
+        ms_chosen, out = time_join(pz, ctx, da, db, args.steps, **chosen)
+        pz.profile_enable(False)
+        kernels = {}
+        for k in ("object_stats", "edge_count", "tri_keys", "edge_scatter", "karras", "refit", "gather_leaves",
+                  "radix_scatter", "narrow", "parity"):
+            tot, n = pz.profile_query(k)
+            if n:
+                kernels[k] = (tot, n)
+        log(f"[bench] chosen {ms_chosen:.3f} ms/join, pairs={len(out.pairs)}; kernels "
+            + ", ".join(f"{k}={v[0] / args.steps:.2f}ms" for k, v in kernels.items()))
+
+        ms_plain, mism, plain_phases = None, None, None
+        if not args.no_plain:
+            psteps = args.plain_steps or args.steps
+            for _ in range(min(args.warmup, 1)):
+                pz.join(ctx, da, db, **plain)
+            ms_plain, pout = time_join(pz, ctx, da, db, psteps, **plain)
+            mism = pair_mismatch(out.pairs, pout.pairs)
+            plain_phases = [(n, round(m, 3)) for n, m in pout.phases]
+            log(f"[bench] plain {ms_plain:.3f} ms/join, pairs={len(pout.pairs)}, mismatches={mism}")
+
+        # Roofline of the dominant profiled kernel (largest device time in the timed steps).
+        counts.update(IT=out.index_triangles, IE=out.index_edges)
+        dom = max(kernels, key=lambda k: kernels[k][0]) if kernels else None
+        roofline = None
+        if dom:
+            tot, n = kernels[dom]
+            avg_ms = tot / n
+            b = alg_bytes(dom, counts)
+            if b is not None:
+                ach = b / (avg_ms * 1e-3) / 1e9
+                roofline = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None,
+                            "alg_bytes_per_launch": int(b), "avg_launch_ms": round(avg_ms, 4), "launches": n}
+
+        # CPU baseline: the oracle's Face restatement on a spatial sub-box of the same workload.
+        cpu = None
+        sample_mism = None
+        if not args.no_cpu:
+            from oracle import oracle as orc
+
+            threads = min(16, os.cpu_count() or 1)
+            orc.set_threads(threads)
+            sa, sb = sub_box(pz, A, args.cpu_side), sub_box(pz, B, args.cpu_side)
+            log(f"[bench] CPU sample: {sa.num_objects} + {sb.num_objects} objects, "
+                f"{sa.indices.shape[0] + sb.indices.shape[0]} triangles, {threads} threads")
+            pairs_cpu, us = orc.face_overlap(sa, sb, with_time=True)
+            n_cand_s = orc.count_candidate_tests(sa, sb)
+            dsa, dsb = ctx.upload(sa), ctx.upload(sb)
+            gs = pz.join(ctx, dsa, dsb, **chosen)
+            sample_mism = pair_mismatch(gs.pairs, pairs_cpu)
+            cpu = {"value": round(n_cand_s / (us * 1e-6) / 1e9, 6), "unit": "Gtri-tests/s", "cores": threads,
+                   "kind": "port",
+                   "sample": (f"Face restatement (oracle/oracle.cpp) on the objects of both sets whose first vertex "
+                              f"lies in [0,{args.cpu_side:g})^3: {sa.num_objects}+{sb.num_objects} objects, "
+                              f"N_cand={n_cand_s}, {len(pairs_cpu)} pairs, query {us / 1e3:.1f} ms; "
+                              f"GPU chosen path on the same sample: {sample_mism} mismatched pairs"),
+                   "query_ms": round(us / 1e3, 3), "pairs": int(len(pairs_cpu))}
+            log(f"[bench] CPU sample query {us / 1e3:.1f} ms, mismatches vs GPU {sample_mism}")
+
+    value = n_cand / (ms_chosen * 1e-3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Gtri-tests/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_chosen, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"overlap join, {args.objects} vs {args.objects} ellipsoid icospheres s=3 "
+                               f"(1280 tris), 20% of B scaled copies of A, centres in 1000^3",
+                   "objects_per_set": args.objects, "triangles_per_set": counts["T"] // 2,
+                   "edges_per_set": counts["E"] // 2, "predicate": "overlap",
+                   "chosen": "two-level, fp32-filtered exact, two-pass sizing",
+                   "plain": "set-wide BVH, fp64-filtered exact, two-pass count/scan/write/sort/unique"},
+        "join_ms_chosen": round(ms_chosen, 4),
+        "join_ms_plain": None if ms_plain is None else round(ms_plain, 4),
+        "speedup_vs_plain": None if ms_plain is None else round(ms_plain / ms_chosen, 3),
+        "mismatched_pairs": mism,
+        "pairs": int(len(out.pairs)),
+        "n_cand": int(n_cand),
+        "pairs_per_s": round(len(out.pairs) / (ms_chosen * 1e-3), 1),
+        "tests_executed": out.tests_total,
+        "escalated_fp64": out.tests_fp64,
+        "escalated_exact": out.tests_exact,
+        "fp32_only_misclassified": out.tests_fp32_misclassified,
+        "phases_chosen": [(n, round(m, 3)) for n, m in out.phases],
+        "phases_plain": plain_phases,
+        "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kernels.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -239,6 +239,8 @@ typedef struct {
     uint64_t tests_fp32_misclassified; /* escalated tests an unguarded fp32 evaluation gets wrong */
     uint64_t edges_traced;      /* edge traversals started */
     uint64_t stack_overflows;   /* must be 0: traversal stack exceeded (result invalid) */
+    uint64_t index_triangles;   /* triangles indexed by the per-object (two-level) or set-wide BVHs */
+    uint64_t index_edges;       /* edges regrouped per object (two-level) */
     double   ms_total;          /* device-resident geometry -> sorted host pair set */
     pz_phase phases[PZ_MAX_PHASES];
 } pz_join_result;
@@ -251,6 +253,15 @@ void      pz_join_result_release(pz_join_result* r);
  * boxes overlap (the BVH-independent candidate-test count). */
 pz_status pz_count_candidate_tests(pz_ctx* ctx, const pz_dataset* mesh1, const pz_dataset* mesh2,
                                    uint64_t* out);
+
+/* ------------------------------------------------------------- profiling */
+/* Per-kernel device time from HIP events recorded on the launching stream
+ * (the phase timer of pierce/query/src/timer.cpp, at kernel granularity).
+ * Kernel names: object_stats, edge_count, tri_keys, edge_scatter, karras,
+ * refit, gather_leaves, radix_scatter, narrow, parity, trace_global. */
+void      pz_profile_enable(int32_t on);
+void      pz_profile_reset(void);
+pz_status pz_profile_query(const char* kernel, double* total_ms, uint64_t* launches);
 
 /* ------------------------------------------------ reference extern "C" helpers */
 /* pierce/query/src/cuda/mesh_query_deduplication.h:7-11: device inputs, caller-sized

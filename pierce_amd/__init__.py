@@ -91,7 +91,8 @@ class JoinResult(C.Structure):
                 ("num_candidates", C.c_uint64), ("capacity", C.c_uint64), ("predicted", C.c_uint64),
                 ("overflowed", C.c_int32), ("num_phases", C.c_int32), ("tests_total", C.c_uint64),
                 ("tests_fp64", C.c_uint64), ("tests_exact", C.c_uint64), ("tests_fp32_misclassified", C.c_uint64),
-                ("edges_traced", C.c_uint64), ("stack_overflows", C.c_uint64), ("ms_total", C.c_double),
+                ("edges_traced", C.c_uint64), ("stack_overflows", C.c_uint64), ("index_triangles", C.c_uint64),
+                ("index_edges", C.c_uint64), ("ms_total", C.c_double),
                 ("phases", Phase * MAX_PHASES)]
 
 
@@ -119,6 +120,9 @@ _SIGS = [
     ("pz_join", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(JoinDesc), C.POINTER(JoinResult)]),
     ("pz_join_result_release", None, [C.POINTER(JoinResult)]),
     ("pz_count_candidate_tests", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]),
+    ("pz_profile_enable", None, [C.c_int32]),
+    ("pz_profile_reset", None, []),
+    ("pz_profile_query", C.c_int32, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     ("merge_and_deduplicate_pairs_gpu", C.c_longlong,
      [C.c_void_p, C.c_longlong, C.c_void_p, C.c_longlong, C.c_void_p]),
     ("compact_hash_table_pairs", C.c_int, [C.c_void_p, C.c_ulonglong, C.c_void_p, C.c_int]),
@@ -364,6 +368,8 @@ class JoinOutput:
     tests_exact: int
     tests_fp32_misclassified: int
     edges_traced: int
+    index_triangles: int
+    index_edges: int
     ms_total: float
     phases: list
 
@@ -396,9 +402,25 @@ def join(ctx: Context, a: Dataset, b: Dataset, **kw) -> JoinOutput:
         phases = [(r.phases[i].name.decode(), r.phases[i].ms) for i in range(r.num_phases)]
         return JoinOutput(pairs, types, int(r.num_candidates), int(r.capacity), int(r.predicted), bool(r.overflowed),
                           int(r.tests_total), int(r.tests_fp64), int(r.tests_exact),
-                          int(r.tests_fp32_misclassified), int(r.edges_traced), float(r.ms_total), phases)
+                          int(r.tests_fp32_misclassified), int(r.edges_traced), int(r.index_triangles),
+                          int(r.index_edges), float(r.ms_total), phases)
     finally:
         lib().pz_join_result_release(C.byref(r))
+
+
+def profile_enable(on: bool = True) -> None:
+    lib().pz_profile_enable(1 if on else 0)
+
+
+def profile_reset() -> None:
+    lib().pz_profile_reset()
+
+
+def profile_query(kernel: str) -> tuple:
+    """(total ms, launches) of one kernel since the last reset."""
+    ms, n = C.c_double(), C.c_uint64()
+    _check(lib().pz_profile_query(kernel.encode(), C.byref(ms), C.byref(n)))
+    return float(ms.value), int(n.value)
 
 
 def count_candidate_tests(ctx: Context, a: Dataset, b: Dataset) -> int:

@@ -424,6 +424,8 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
     };
     SetAccel xa = build(A, pa, na, ta, "A");
     SetAccel xb = build(B, pb, nb, tb, "B");
+    r->index_triangles = ta.ntri + tb.ntri;
+    r->index_edges = ta.nedge + tb.nedge;
 
     ph.begin("Narrow Phase");
     uint32_t* tcnt = buf<uint32_t>(c, "np.tcnt", np + 1);
@@ -523,6 +525,7 @@ void join_global(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const pz_j
     if (do21) ga = build_global(c, A, pa, "A");
     ph.begin("Build Mesh2 Index");
     if (do12) gb = build_global(c, B, pb, "B");
+    r->index_triangles = (do21 ? A->nt : 0) + (do12 ? B->nt : 0);
     const bool f32 = d.precision == PZ_PREC_FP32_FILTERED;
     const int allow_f32 = (std::max(A->max_abs, B->max_abs) <= 0x1p39f) ? 1 : 0;
     uint32_t* overflow = buf<uint32_t>(c, "g.overflow", 1);
@@ -844,6 +847,14 @@ pz_status pz_count_candidate_tests(pz_ctx* c, const pz_dataset* a, const pz_data
         *out = readback<uint64_t>(c, {off + n1 + n2})[0];
         return PZ_OK;
     });
+}
+
+void pz_profile_enable(int32_t on) { pzk::prof_enable(on != 0); }
+void pz_profile_reset(void) { pzk::prof_reset(); }
+pz_status pz_profile_query(const char* kernel, double* total_ms, uint64_t* launches) {
+    if (!kernel || !total_ms || !launches) return fail(PZ_E_INVALID, "pz_profile_query: null argument");
+    pzk::prof_query(kernel, total_ms, launches);
+    return PZ_OK;
 }
 
 // ----------------------------------------------- reference extern "C" helpers

@@ -79,33 +79,39 @@ __global__ void k_karras(const uint64_t* __restrict__ keys, uint64_t n, const ui
     nodes[i].d = make_int4(c0, c1, 0, 0);
 }
 
-__device__ __forceinline__ void store_child_box(Node* nd, int slot, const float* lo, const float* hi) {
-    float* f = reinterpret_cast<float*>(nd);
-    if (slot == 0) {
-        f[0] = lo[0];
-        f[1] = lo[1];
-        f[2] = lo[2];
-        f[3] = hi[0];
-        f[4] = hi[1];
-        f[5] = hi[2];
-    } else {
-        f[6] = lo[0];
-        f[7] = lo[1];
-        f[8] = lo[2];
-        f[9] = hi[0];
-        f[10] = hi[1];
-        f[11] = hi[2];
-    }
+// Child boxes cross workgroups during the refit.  They are written and read
+// with agent-scope relaxed atomics, which on gfx950 bypass the per-XCD L2
+// (sc1), so no cache write-back fence is needed; the writer waits for its
+// stores to complete (s_waitcnt) before incrementing the node counter, and
+// the reader only loads after its increment has returned.
+__device__ __forceinline__ void st_agent(float* p, float v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void load_child_box(const Node* nd, int slot, float* lo, float* hi) {
-    const volatile float* f = reinterpret_cast<const volatile float*>(nd);
-    int o = slot == 0 ? 0 : 6;
-    lo[0] = f[o + 0];
-    lo[1] = f[o + 1];
-    lo[2] = f[o + 2];
-    hi[0] = f[o + 3];
-    hi[1] = f[o + 4];
-    hi[2] = f[o + 5];
+__device__ __forceinline__ float ld_agent(const float* p) {
+    return __hip_atomic_load(const_cast<float*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void store_child_box(Node* nd, int slot, const float* lo, const float* hi, int height) {
+    float* f = reinterpret_cast<float*>(nd);
+    const int o = slot == 0 ? 0 : 6;
+    st_agent(f + o + 0, lo[0]);
+    st_agent(f + o + 1, lo[1]);
+    st_agent(f + o + 2, lo[2]);
+    st_agent(f + o + 3, hi[0]);
+    st_agent(f + o + 4, hi[1]);
+    st_agent(f + o + 5, hi[2]);
+    st_agent(f + 14 + slot, __int_as_float(height));
+}
+__device__ __forceinline__ void load_child_box(const Node* nd, int slot, float* lo, float* hi, int* height) {
+    const float* f = reinterpret_cast<const float*>(nd);
+    const int o = slot == 0 ? 0 : 6;
+    lo[0] = ld_agent(f + o + 0);
+    lo[1] = ld_agent(f + o + 1);
+    lo[2] = ld_agent(f + o + 2);
+    hi[0] = ld_agent(f + o + 3);
+    hi[1] = ld_agent(f + o + 4);
+    hi[2] = ld_agent(f + o + 5);
+    *height = __float_as_int(ld_agent(f + 14 + slot));
 }
 
 template <class BoxFn>
@@ -124,15 +130,15 @@ __device__ void refit_from_leaf(uint64_t p, const uint64_t* __restrict__ keys, u
     while (true) {
         Node* nd = nodes + node;
         int slot = (nd->d.x == child) ? 0 : 1;
-        store_child_box(nd, slot, lo, hi);
-        reinterpret_cast<int*>(nd)[14 + slot] = height;
-        __threadfence();
-        uint32_t prev = atomicAdd(&counters[node], 1u);
+        store_child_box(nd, slot, lo, hi, height);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);  // stores complete before the counter moves
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        uint32_t prev = __hip_atomic_fetch_add(&counters[node], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (prev == 0) return;  // sibling not done yet; it continues upward
-        __threadfence();
         float slo[3], shi[3];
-        load_child_box(nd, 1 - slot, slo, shi);
-        int sh = reinterpret_cast<const volatile int*>(nd)[14 + (1 - slot)];
+        int sh;
+        load_child_box(nd, 1 - slot, slo, shi, &sh);
         for (int a = 0; a < 3; ++a) {
             lo[a] = fminf(lo[a], slo[a]);
             hi[a] = fmaxf(hi[a], shi[a]);
@@ -210,6 +216,7 @@ namespace pzk {
 void lbvh_karras(const uint64_t* sorted_keys, uint64_t n, const uint32_t* seg_off, int seg_shift, pzd::Node* nodes,
                  uint32_t* parent_int, uint32_t* parent_leaf, uint32_t* counters, hipStream_t s) {
     if (n == 0) return;
+    KernelTimer kt_("karras", s);
     hipLaunchKernelGGL(k_karras, dim3(grid1(n)), dim3(256), 0, s, sorted_keys, n, seg_off, seg_shift, nodes, parent_int,
                        parent_leaf, counters);
 }
@@ -220,6 +227,7 @@ void lbvh_refit_tris(const uint64_t* sorted_keys, const uint32_t* order, uint64_
                      hipStream_t s) {
     if (n == 0) return;
     TriBoxFn fn{verts, idx, order};
+    KernelTimer kt_("refit", s);
     hipLaunchKernelGGL(k_refit_tris, dim3(grid1(n)), dim3(256), 0, s, sorted_keys, n, seg_off, seg_shift, nodes,
                        parent_int, parent_leaf, counters, max_depth, fn);
 }
@@ -236,6 +244,7 @@ void lbvh_refit_ord_boxes(const uint64_t* sorted_keys, const uint32_t* order, ui
 void gather_leaf_tris(const uint32_t* order, uint64_t n, const float* verts, const uint32_t* idx, const int32_t* t2o,
                       pzd::LeafTri* out, hipStream_t s) {
     if (n == 0) return;
+    KernelTimer kt_("gather_leaves", s);
     hipLaunchKernelGGL(k_gather_leaf_tris, dim3(grid1(n)), dim3(256), 0, s, order, n, verts, idx, t2o, out);
 }
 

@@ -763,8 +763,11 @@ __global__ void k_init_boxes(uint32_t* obox, int64_t n) {
 template <int STACK, bool F32>
 void launch_narrow(unsigned g, hipStream_t s, const uint64_t* pairs, uint64_t np, const uint64_t* task_off, SetView A,
                    SetView B, uint32_t* found, int dir_mask, int allow_f32, unsigned long long* stats) {
-    hipLaunchKernelGGL((k_narrow<STACK, F32>), dim3(g), dim3(kBlock), 0, s, pairs, np, task_off, A, B, found, dir_mask,
-                       allow_f32, stats);
+    {
+        pzk::KernelTimer kt_("narrow", s);
+        hipLaunchKernelGGL((k_narrow<STACK, F32>), dim3(g), dim3(kBlock), 0, s, pairs, np, task_off, A, B, found, dir_mask,
+                           allow_f32, stats);
+    }
 }
 
 template <int STACK, bool F32>
@@ -800,6 +803,7 @@ void object_stats(const float* verts, uint64_t nv, const uint32_t* idx, const in
                        (uint64_t)nobj);
     hipLaunchKernelGGL((k_fill_any<uint32_t>), dim3(grid1((uint64_t)nobj)), dim3(kBlock), 0, s, first_tri,
                        0xFFFFFFFFu, (uint64_t)nobj);
+    KernelTimer kt_("object_stats", s);
     if (nt)
         hipLaunchKernelGGL(k_object_stats, dim3(grid1(nt)), dim3(kBlock), 0, s, verts, nv, idx, t2o, nt, nobj, obox,
                            tri_cnt, first_tri);
@@ -809,6 +813,7 @@ void edge_count(const int32_t* eo, uint64_t ne, int32_t nobj, uint32_t* edge_cnt
     if (nobj <= 0) return;
     hipLaunchKernelGGL((k_fill_any<uint32_t>), dim3(grid1((uint64_t)nobj)), dim3(kBlock), 0, s, edge_cnt, 0u,
                        (uint64_t)nobj);
+    KernelTimer kt_("edge_count", s);
     if (ne) hipLaunchKernelGGL(k_edge_count, dim3(grid1(ne)), dim3(kBlock), 0, s, eo, ne, nobj, edge_cnt);
 }
 
@@ -860,6 +865,7 @@ void needed_layout(const uint32_t* need, const uint32_t* loc, int32_t nobj, cons
 void tri_keys_two_level(const float* verts, uint64_t nv, const uint32_t* idx, const int32_t* t2o, uint64_t nt,
                         int32_t nobj, const uint32_t* need, const uint32_t* loc, const uint32_t* obox,
                         const uint32_t* seg_off, uint32_t* cursor, uint64_t* keys, uint32_t* vals, hipStream_t s) {
+    KernelTimer kt_("tri_keys", s);
     if (nt)
         hipLaunchKernelGGL(k_tri_keys_two_level, dim3(grid1(nt)), dim3(kBlock), 0, s, verts, idx, t2o, nt, nobj, need,
                            loc, obox, seg_off, cursor, keys, vals, nv);
@@ -873,6 +879,7 @@ void tri_keys_global(const float* verts, const uint32_t* idx, uint64_t nt, const
 void edge_scatter(const float* es, const float* ee, const int32_t* eo, uint64_t ne, int32_t nobj,
                   const uint32_t* need, const uint32_t* loc, const uint32_t* eseg_off, uint32_t* cursor,
                   EdgeRec* out, hipStream_t s) {
+    KernelTimer kt_("edge_scatter", s);
     if (ne)
         hipLaunchKernelGGL(k_edge_scatter, dim3(grid1(ne)), dim3(kBlock), 0, s, es, ee, eo, ne, nobj, need, loc,
                            eseg_off, cursor, out);
@@ -914,6 +921,7 @@ bool parity(const uint64_t* pairs, uint64_t np, const uint32_t* found, const Set
     int stack = stack_for_height(height);
     if (stack == 0) return false;
     uint64_t n = np * (kind == 1 ? 2 : 1);
+    KernelTimer kt_("parity", s);
     if (stack == 32)
         hipLaunchKernelGGL((k_parity<32>), dim3(grid1(n)), dim3(kBlock), 0, s, pairs, np, found, A, B, vA, iA, ftA, vB,
                            iB, ftB, kind, dir_mask, inside, stats);
@@ -939,6 +947,7 @@ bool trace_global(int mode, const float* es, const float* ee, const int32_t* eo,
     int stack = stack_for_height(height);
     if (stack == 0) return false;
     unsigned g = grid1(ne);
+    KernelTimer kt_("trace_global", s);
     if (stack == 32) {
         if (f32)
             launch_trace<32, true>(mode, g, s, es, ee, eo, ne, nodes, leaves, ntri, swap, allow_f32, counts, off, out,

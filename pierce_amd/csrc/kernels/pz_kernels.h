@@ -33,6 +33,36 @@ void keys_to_pairs(const uint64_t* keys, uint64_t n, int32_t* pairs, hipStream_t
 void pairs_to_keys(const int32_t* pairs, uint64_t n, uint64_t* keys, hipStream_t s);
 
 void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
+
+// Optional per-kernel timing: when enabled, launches bracketed by KernelTimer
+// record a HIP event pair on their stream; totals are collected after the
+// stream is synchronised.
+void prof_enable(bool on);
+bool prof_enabled();
+void prof_record(const char* name, hipEvent_t start, hipEvent_t stop);
+hipEvent_t prof_event();
+// Sum (ms) and count of completed launches of `name` since the last reset.
+void prof_query(const char* name, double* total_ms, uint64_t* count);
+void prof_reset();
+
+struct KernelTimer {
+    const char* name;
+    hipStream_t s;
+    hipEvent_t a = nullptr, b = nullptr;
+    KernelTimer(const char* n, hipStream_t st) : name(n), s(st) {
+        if (prof_enabled()) {
+            a = prof_event();
+            b = prof_event();
+            (void)hipEventRecord(a, s);
+        }
+    }
+    ~KernelTimer() {
+        if (a) {
+            (void)hipEventRecord(b, s);
+            prof_record(name, a, b);
+        }
+    }
+};
 void fill_u64(uint64_t* p, uint64_t v, uint64_t n, hipStream_t s);
 
 }  // namespace pzk

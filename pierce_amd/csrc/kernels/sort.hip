@@ -8,6 +8,11 @@
 #include "pz_device.h"
 #include "pz_kernels.h"
 
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
 namespace {
 
 constexpr int kScanThreads = 256;
@@ -205,6 +210,7 @@ bool radix_impl(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* va
         uint32_t* vo = alt ? vals : vals_alt;
         hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortThreads), 0, s, ki, n, shift, counts, nb);
         scan_impl<uint32_t, uint32_t>(counts, offs, ncounts, scan_tmp, s);
+        pzk::KernelTimer kt_("radix_scatter", s);
         hipLaunchKernelGGL((k_radix_scatter<HasVals>), dim3(nb), dim3(kSortThreads), 0, s, ki, vi, n, shift, offs,
                            nb, ko, vo);
         alt = !alt;
@@ -246,6 +252,76 @@ inline unsigned grid1(uint64_t n, int b = 256) { return (unsigned)((n + b - 1) /
 }  // namespace
 
 namespace pzk {
+
+namespace {
+struct ProfState {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    size_t next = 0;
+    struct Rec {
+        const char* name;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> pending;
+    std::map<std::string, std::pair<double, uint64_t>> totals;
+    std::mutex mu;
+};
+ProfState& prof_state() {
+    static ProfState st;
+    return st;
+}
+void prof_drain(ProfState& st) {
+    for (auto& r : st.pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            auto& t = st.totals[r.name];
+            t.first += ms;
+            t.second += 1;
+        }
+    }
+    st.pending.clear();
+    st.next = 0;
+}
+}  // namespace
+
+void prof_enable(bool on) {
+    auto& st = prof_state();
+    std::lock_guard<std::mutex> g(st.mu);
+    st.on = on;
+}
+bool prof_enabled() { return prof_state().on; }
+hipEvent_t prof_event() {
+    auto& st = prof_state();
+    std::lock_guard<std::mutex> g(st.mu);
+    if (st.next == st.pool.size()) {
+        if (st.next >= 4096) prof_drain(st);  // bounded pool: fold completed pairs
+        if (st.next == st.pool.size()) {
+            hipEvent_t e = nullptr;
+            (void)hipEventCreate(&e);
+            st.pool.push_back(e);
+        }
+    }
+    return st.pool[st.next++];
+}
+void prof_record(const char* name, hipEvent_t a, hipEvent_t b) {
+    auto& st = prof_state();
+    std::lock_guard<std::mutex> g(st.mu);
+    st.pending.push_back({name, a, b});
+}
+void prof_query(const char* name, double* total_ms, uint64_t* count) {
+    auto& st = prof_state();
+    std::lock_guard<std::mutex> g(st.mu);
+    prof_drain(st);
+    auto it = st.totals.find(name);
+    *total_ms = it == st.totals.end() ? 0.0 : it->second.first;
+    *count = it == st.totals.end() ? 0 : it->second.second;
+}
+void prof_reset() {
+    auto& st = prof_state();
+    std::lock_guard<std::mutex> g(st.mu);
+    prof_drain(st);
+    st.totals.clear();
+}
 
 size_t scan_temp_bytes(uint64_t n) { return scan_temp_rec(n, 8) + 256; }
 
