@@ -49,6 +49,8 @@ def alg_bytes(kernel, k):
         return (4 + 12 + 36 + 4 + 48) * k["IT"] / 2
     if kernel == "karras":          # keys read (~neighbour loads), node children + parents written
         return (8 + 16 + 8) * k["IT"] / 2
+    if kernel == "build_small":     # per indexed triangle: indices + 3 vertices read, one node + one leaf written
+        return (12 + 36 + 64 + 48) * k["IT"] / 2
     if kernel == "narrow":          # every indexed edge record read once
         return 32 * k["IE"]
     return None
@@ -143,7 +145,7 @@ def main():
         pz.profile_enable(False)
         kernels = {}
         for k in ("object_stats", "edge_count", "tri_keys", "edge_scatter", "karras", "refit", "gather_leaves",
-                  "radix_scatter", "narrow", "parity"):
+                  "radix_scatter", "build_small", "narrow", "parity"):
             tot, n = pz.profile_query(k)
             if n:
                 kernels[k] = (tot, n)

@@ -148,6 +148,7 @@ struct SetPrep {  // per-object statistics of one set (device)
     uint32_t* obox;
     uint32_t* tri_cnt;
     uint32_t* first_tri;
+    uint32_t* last_tri;
     uint32_t* edge_cnt;
 };
 
@@ -157,8 +158,10 @@ SetPrep prep_set(pz_ctx* c, const pz_dataset* d, const std::string& tag) {
     p.obox = buf<uint32_t>(c, tag + ".obox", 6 * n);
     p.tri_cnt = buf<uint32_t>(c, tag + ".tri_cnt", n);
     p.first_tri = buf<uint32_t>(c, tag + ".first_tri", n);
+    p.last_tri = buf<uint32_t>(c, tag + ".last_tri", n);
     p.edge_cnt = buf<uint32_t>(c, tag + ".edge_cnt", n);
-    pzk::object_stats(d->v, d->nv, d->idx, d->t2o, d->nt, d->nobj, p.obox, p.tri_cnt, p.first_tri, c->s);
+    pzk::object_stats(d->v, d->nv, d->idx, d->t2o, d->nt, d->nobj, p.obox, p.tri_cnt, p.first_tri, p.last_tri,
+                      c->s);
     pzk::edge_count(d->eo, d->ne, d->nobj, p.edge_cnt, c->s);
     return p;
 }
@@ -356,6 +359,12 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         }
     }
     r->num_candidates = np;
+    if (np == 0) {  // no boxes meet: nothing to index or test
+        ph.begin("Download Results");
+        uint64_t* none = buf<uint64_t>(c, "sel.keys", 1);
+        finish_pairs(c, none, nullptr, 0, nA, d.predicate == PZ_PRED_CONTAINMENT && d.include_overlap_pairs, r);
+        return;
+    }
 
     ph.begin("Build Mesh Index");
     // bit 0: mesh1 edges vs mesh2 triangles (+ mesh1 points in mesh2 objects); bit 1: the reverse.
@@ -369,61 +378,82 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
     pzk::fill_u32(nb.need, 0u, sB, s);
     pzk::mark_needed(pairs, np, na.need, nb.need, s);
     struct Tot {
-        uint64_t nneed, ntri, nedge;
+        uint64_t nneed, ntri, nedge, nbig, nbig_tri;
     };
-    auto layout = [&](const pz_dataset* D, const SetPrep& P, Needed& N, const std::string& tag) {
+    // Needed objects of one set: objects built by the global pipeline ("big":
+    // more than kSmallMax triangles or not stored contiguously) take the first
+    // segment indices, LDS-built ones follow.
+    auto layout = [&](const pz_dataset* D, const SetPrep& P, Needed& N, const std::string& tag) -> uint32_t* {
         size_t n = std::max(D->nobj, 1);
+        const uint64_t no = (uint64_t)std::max(D->nobj, 0);
         uint32_t* stc = buf<uint32_t>(c, tag + ".seg_tri_cnt", n);
         uint32_t* sec = buf<uint32_t>(c, tag + ".seg_edge_cnt", n);
+        uint32_t* big = buf<uint32_t>(c, tag + ".big", n);
+        uint32_t* small = buf<uint32_t>(c, tag + ".small", n);
+        uint32_t* loc_big = buf<uint32_t>(c, tag + ".loc_big", n + 1);
+        uint32_t* loc_small = buf<uint32_t>(c, tag + ".loc_small", n + 1);
+        uint32_t* totals = buf<uint32_t>(c, tag + ".totals", 4);
         pzk::fill_u32(stc, 0u, n, s);
         pzk::fill_u32(sec, 0u, n, s);
         void* tmp = buf<char>(c, "scan.tmp", pzk::scan_temp_bytes(n + 1));
-        pzk::exclusive_scan_u32_u32(N.need, N.loc, (uint64_t)std::max(D->nobj, 0), tmp, s);
+        pzk::need_split(N.need, P.tri_cnt, P.first_tri, P.last_tri, D->nobj, big, small, s);
+        pzk::exclusive_scan_u32_u32(big, loc_big, no, tmp, s);
+        pzk::exclusive_scan_u32_u32(small, loc_small, no, tmp, s);
+        pzk::merge_locs(big, loc_big, loc_small, D->nobj, N.loc, totals, s);
         pzk::needed_layout(N.need, N.loc, D->nobj, P.tri_cnt, P.edge_cnt, N.obj_of_loc, stc, sec, s);
-        pzk::exclusive_scan_u32_u32(stc, N.seg_off, (uint64_t)std::max(D->nobj, 0), tmp, s);
-        pzk::exclusive_scan_u32_u32(sec, N.eseg_off, (uint64_t)std::max(D->nobj, 0), tmp, s);
+        pzk::exclusive_scan_u32_u32(stc, N.seg_off, no, tmp, s);
+        pzk::exclusive_scan_u32_u32(sec, N.eseg_off, no, tmp, s);
+        pzk::big_prefix(N.seg_off, totals, s);
+        return big;
     };
-    layout(A, pa, na, "A");
-    layout(B, pb, nb, "B");
+    uint32_t* bigA = layout(A, pa, na, "A");
+    uint32_t* bigB = layout(B, pb, nb, "B");
+    uint32_t* totA = buf<uint32_t>(c, "A.totals", 4);
+    uint32_t* totB = buf<uint32_t>(c, "B.totals", 4);
     auto tot = readback<uint32_t>(c, {na.loc + std::max(nA, 0), na.seg_off + std::max(nA, 0),
-                                      na.eseg_off + std::max(nA, 0), nb.loc + std::max(nB, 0),
-                                      nb.seg_off + std::max(nB, 0), nb.eseg_off + std::max(nB, 0)});
-    Tot ta{tot[0], tot[1], tot[2]}, tb{tot[3], tot[4], tot[5]};
+                                      na.eseg_off + std::max(nA, 0), totA, totA + 1, nb.loc + std::max(nB, 0),
+                                      nb.seg_off + std::max(nB, 0), nb.eseg_off + std::max(nB, 0), totB, totB + 1});
+    Tot ta{tot[0], tot[1], tot[2], tot[3], tot[4]}, tb{tot[5], tot[6], tot[7], tot[8], tot[9]};
 
-    auto build = [&](const pz_dataset* D, const SetPrep& P, const Needed& N, const Tot& T,
+    auto build = [&](const pz_dataset* D, const SetPrep& P, const Needed& N, const Tot& T, uint32_t* big,
                      const std::string& tag) -> SetAccel {
         SetAccel acc;
-        const uint64_t n = T.ntri;
-        uint64_t* keys = buf<uint64_t>(c, tag + ".tkeys", n + 1);
-        uint32_t* vals = buf<uint32_t>(c, tag + ".tvals", n + 1);
-        uint64_t* keys2 = buf<uint64_t>(c, tag + ".tkeys2", n + 1);
-        uint32_t* vals2 = buf<uint32_t>(c, tag + ".tvals2", n + 1);
-        uint32_t* cursor = buf<uint32_t>(c, tag + ".cursor", (size_t)std::max(D->nobj, 1));
-        pzk::fill_u32(cursor, 0u, (size_t)std::max(D->nobj, 1), s);
-        pzk::tri_keys_two_level(D->v, D->nv, D->idx, D->t2o, D->nt, D->nobj, N.need, N.loc, P.obox, N.seg_off,
-                                cursor, keys, vals, s);
-        void* tmp = buf<char>(c, "sort.tmp." + tag, pzk::radix_sort_temp_bytes(n));
-        bool alt = pzk::radix_sort_pairs(keys, vals, keys2, vals2, n, 0, 30 + bits_for(std::max<uint64_t>(T.nneed, 1)),
-                                         tmp, s);
-        uint64_t* sk = alt ? keys2 : keys;
-        uint32_t* order = alt ? vals2 : vals;
+        const uint64_t n = T.ntri, nb_tri = T.nbig_tri;
         acc.nodes = buf<pzd::Node>(c, tag + ".nodes", n + 1);
-        uint32_t* pint = buf<uint32_t>(c, tag + ".pint", n + 1);
-        uint32_t* pleaf = buf<uint32_t>(c, tag + ".pleaf", n + 1);
-        uint32_t* cnt = buf<uint32_t>(c, tag + ".ncnt", n + 1);
+        acc.leaves = buf<pzd::LeafTri>(c, tag + ".leaves", n + 1);
         acc.height = buf<uint32_t>(c, tag + ".height", 1);
         pzk::fill_u32(acc.height, 0u, 1, s);
-        pzk::lbvh_karras(sk, n, N.seg_off, 30, acc.nodes, pint, pleaf, cnt, s);
-        pzk::lbvh_refit_tris(sk, order, n, N.seg_off, 30, D->v, D->idx, acc.nodes, pint, pleaf, cnt, acc.height, s);
-        acc.leaves = buf<pzd::LeafTri>(c, tag + ".leaves", n + 1);
-        pzk::gather_leaf_tris(order, n, D->v, D->idx, D->t2o, acc.leaves, s);
+        uint32_t* cursor = buf<uint32_t>(c, tag + ".cursor", (size_t)std::max(D->nobj, 1));
+        if (nb_tri > 0) {  // global pipeline for the big objects (segment prefix)
+            uint64_t* keys = buf<uint64_t>(c, tag + ".tkeys", nb_tri + 1);
+            uint32_t* vals = buf<uint32_t>(c, tag + ".tvals", nb_tri + 1);
+            uint64_t* keys2 = buf<uint64_t>(c, tag + ".tkeys2", nb_tri + 1);
+            uint32_t* vals2 = buf<uint32_t>(c, tag + ".tvals2", nb_tri + 1);
+            pzk::fill_u32(cursor, 0u, (size_t)std::max(D->nobj, 1), s);
+            pzk::tri_keys_two_level(D->v, D->nv, D->idx, D->t2o, D->nt, D->nobj, big, N.loc, P.obox, N.seg_off,
+                                    cursor, keys, vals, s);
+            void* tmp = buf<char>(c, "sort.tmp." + tag, pzk::radix_sort_temp_bytes(nb_tri));
+            bool alt = pzk::radix_sort_pairs(keys, vals, keys2, vals2, nb_tri, 0,
+                                             30 + bits_for(std::max<uint64_t>(T.nbig, 1)), tmp, s);
+            uint64_t* sk = alt ? keys2 : keys;
+            uint32_t* order = alt ? vals2 : vals;
+            uint32_t* pint = buf<uint32_t>(c, tag + ".pint", nb_tri + 1);
+            uint32_t* pleaf = buf<uint32_t>(c, tag + ".pleaf", nb_tri + 1);
+            uint32_t* cnt = buf<uint32_t>(c, tag + ".ncnt", nb_tri + 1);
+            pzk::lbvh_karras(sk, nb_tri, N.seg_off, 30, acc.nodes, pint, pleaf, cnt, s);
+            pzk::lbvh_refit_tris(sk, order, nb_tri, N.seg_off, 30, D->v, D->idx, acc.nodes, pint, pleaf, cnt,
+                                 acc.height, s);
+            pzk::gather_leaf_tris(order, nb_tri, D->v, D->idx, D->t2o, acc.leaves, s);
+        }
+        pzk::build_small(D->v, D->idx, D->t2o, N.obj_of_loc, P.first_tri, P.tri_cnt, P.obox, N.seg_off,
+                         (uint32_t)T.nbig, (uint32_t)(T.nneed - T.nbig), acc.nodes, acc.leaves, acc.height, s);
         acc.edges = buf<pzd::EdgeRec>(c, tag + ".edges", T.nedge + 1);
         pzk::fill_u32(cursor, 0u, (size_t)std::max(D->nobj, 1), s);
         pzk::edge_scatter(D->es, D->ee, D->eo, D->ne, D->nobj, N.need, N.loc, N.eseg_off, cursor, acc.edges, s);
         return acc;
     };
-    SetAccel xa = build(A, pa, na, ta, "A");
-    SetAccel xb = build(B, pb, nb, tb, "B");
+    SetAccel xa = build(A, pa, na, ta, bigA, "A");
+    SetAccel xb = build(B, pb, nb, tb, bigB, "B");
     r->index_triangles = ta.ntri + tb.ntri;
     r->index_edges = ta.nedge + tb.nedge;
 

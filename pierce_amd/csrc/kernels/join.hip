@@ -35,7 +35,8 @@ __global__ __launch_bounds__(kBlock) void k_object_stats(const float* __restrict
                                                          const uint32_t* __restrict__ idx,
                                                          const int32_t* __restrict__ t2o, uint64_t nt, int32_t nobj,
                                                          uint32_t* __restrict__ obox, uint32_t* __restrict__ tri_cnt,
-                                                         uint32_t* __restrict__ first_tri) {
+                                                         uint32_t* __restrict__ first_tri,
+                                                         uint32_t* __restrict__ last_tri) {
     uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     bool valid = t < nt;
     int32_t o = -1;
@@ -70,6 +71,7 @@ __global__ __launch_bounds__(kBlock) void k_object_stats(const float* __restrict
             hi[a] = h;
         }
         uint32_t tmin = (uint32_t)__shfl((int)(uint32_t)t, leader, 64);
+        uint32_t tmax = (uint32_t)__shfl((int)(uint32_t)t, 63 - __clzll((long long)vm), 64);
         if (__lane_id() == leader) {
             for (int a = 0; a < 3; ++a) {
                 atomicMin(&obox[6ull * o0 + a], f2ord(lo[a]));
@@ -77,6 +79,7 @@ __global__ __launch_bounds__(kBlock) void k_object_stats(const float* __restrict
             }
             atomicAdd(&tri_cnt[o0], (uint32_t)__popcll(vm));
             atomicMin(&first_tri[o0], tmin);
+            atomicMax(&last_tri[o0], tmax);
         }
     } else if (valid) {
         for (int a = 0; a < 3; ++a) {
@@ -85,6 +88,7 @@ __global__ __launch_bounds__(kBlock) void k_object_stats(const float* __restrict
         }
         atomicAdd(&tri_cnt[o], 1u);
         atomicMin(&first_tri[o], (uint32_t)t);
+        atomicMax(&last_tri[o], (uint32_t)t);
     }
 }
 
@@ -374,7 +378,7 @@ __device__ __forceinline__ bool leaf_test(const LeafTri& L, const float3& p, con
 constexpr int kChunk = 256;
 
 template <int STACK, bool F32>
-__global__ __launch_bounds__(kBlock) void k_narrow(const uint64_t* __restrict__ pairs, uint64_t np,
+__global__ __launch_bounds__(kBlock, 4) void k_narrow(const uint64_t* __restrict__ pairs, uint64_t np,
                                                    const uint64_t* __restrict__ task_off, SetView A, SetView B,
                                                    uint32_t* __restrict__ found, int dir_mask, int allow_f32,
                                                    unsigned long long* __restrict__ stats) {
@@ -487,7 +491,7 @@ __global__ __launch_bounds__(kBlock) void k_narrow(const uint64_t* __restrict__ 
 // triangles.  kind 0: containment (point of mesh2 object in mesh1 object);
 // kind 1: intersection (both directions, sets inside[] bit 0 / bit 1).
 template <int STACK>
-__global__ __launch_bounds__(kBlock) void k_parity(const uint64_t* __restrict__ pairs, uint64_t np,
+__global__ __launch_bounds__(kBlock, 4) void k_parity(const uint64_t* __restrict__ pairs, uint64_t np,
                                                    const uint32_t* __restrict__ found, SetView A, SetView B,
                                                    const float* __restrict__ vA, const uint32_t* __restrict__ iA,
                                                    const uint32_t* __restrict__ ftA, const float* __restrict__ vB,
@@ -649,7 +653,7 @@ __device__ __forceinline__ bool hash_insert(unsigned long long* table, uint64_t 
 // One lane per source edge; every hit on a distinct target object kept
 // (consecutive hits on one object collapse).
 template <int STACK, bool F32, int Mode>
-__global__ __launch_bounds__(kBlock) void k_trace_global(
+__global__ __launch_bounds__(kBlock, 4) void k_trace_global(
     const float* __restrict__ es, const float* __restrict__ ee, const int32_t* __restrict__ eo, uint64_t ne,
     const Node* __restrict__ nodes, const LeafTri* __restrict__ leaves, uint64_t ntri, int swap, int allow_f32,
     uint32_t* __restrict__ counts, const uint64_t* __restrict__ off, uint64_t* __restrict__ out,
@@ -796,17 +800,19 @@ namespace pzk {
 int stack_for_height(uint32_t h) { return h < 32 ? 32 : (h < 64 ? 64 : 0); }
 
 void object_stats(const float* verts, uint64_t nv, const uint32_t* idx, const int32_t* t2o, uint64_t nt, int32_t nobj,
-                  uint32_t* obox, uint32_t* tri_cnt, uint32_t* first_tri, hipStream_t s) {
+                  uint32_t* obox, uint32_t* tri_cnt, uint32_t* first_tri, uint32_t* last_tri, hipStream_t s) {
     if (nobj <= 0) return;
     hipLaunchKernelGGL(k_init_boxes, dim3(grid1((uint64_t)nobj)), dim3(kBlock), 0, s, obox, (int64_t)nobj);
     hipLaunchKernelGGL((k_fill_any<uint32_t>), dim3(grid1((uint64_t)nobj)), dim3(kBlock), 0, s, tri_cnt, 0u,
                        (uint64_t)nobj);
     hipLaunchKernelGGL((k_fill_any<uint32_t>), dim3(grid1((uint64_t)nobj)), dim3(kBlock), 0, s, first_tri,
                        0xFFFFFFFFu, (uint64_t)nobj);
+    hipLaunchKernelGGL((k_fill_any<uint32_t>), dim3(grid1((uint64_t)nobj)), dim3(kBlock), 0, s, last_tri, 0u,
+                       (uint64_t)nobj);
     KernelTimer kt_("object_stats", s);
     if (nt)
         hipLaunchKernelGGL(k_object_stats, dim3(grid1(nt)), dim3(kBlock), 0, s, verts, nv, idx, t2o, nt, nobj, obox,
-                           tri_cnt, first_tri);
+                           tri_cnt, first_tri, last_tri);
 }
 
 void edge_count(const int32_t* eo, uint64_t ne, int32_t nobj, uint32_t* edge_cnt, hipStream_t s) {

@@ -27,7 +27,25 @@ using pzd::SetView;
 int stack_for_height(uint32_t h);
 
 void object_stats(const float* verts, uint64_t nv, const uint32_t* idx, const int32_t* t2o, uint64_t nt, int32_t nobj,
-                  uint32_t* obox, uint32_t* tri_cnt, uint32_t* first_tri, hipStream_t s);
+                  uint32_t* obox, uint32_t* tri_cnt, uint32_t* first_tri, uint32_t* last_tri, hipStream_t s);
+
+// --- small-object LDS builder (build_small.hip)
+// Objects with at most kSmallMax triangles stored contiguously are built
+// entirely in LDS by one workgroup each; the rest use the global pipeline.
+constexpr uint32_t kSmallMax = 2048;
+// big[o] / small[o] flags of the needed objects.
+void need_split(const uint32_t* need, const uint32_t* tri_cnt, const uint32_t* first_tri, const uint32_t* last_tri,
+                int32_t nobj, uint32_t* big, uint32_t* small, hipStream_t s);
+// loc[o] = big ? loc_big[o] : (number of big objects) + loc_small[o]; also
+// totals[0] = number of big objects.
+void merge_locs(const uint32_t* big, const uint32_t* loc_big, const uint32_t* loc_small, int32_t nobj, uint32_t* loc,
+                uint32_t* totals, hipStream_t s);
+// totals[1] = seg_off[totals[0]] (triangles of the big objects).
+void big_prefix(const uint32_t* seg_off, uint32_t* totals, hipStream_t s);
+// One workgroup per small object l in [l_begin, l_begin + count).
+void build_small(const float* verts, const uint32_t* idx, const int32_t* t2o, const int32_t* obj_of_loc,
+                 const uint32_t* first_tri, const uint32_t* tri_cnt, const uint32_t* obox, const uint32_t* seg_off,
+                 uint32_t l_begin, uint32_t count, Node* nodes, LeafTri* leaves, uint32_t* height, hipStream_t s);
 void edge_count(const int32_t* eo, uint64_t ne, int32_t nobj, uint32_t* edge_cnt, hipStream_t s);
 void scene_box(const uint32_t* obox, int32_t nobj, uint32_t* scene, hipStream_t s);
 void object_keys(const uint32_t* obox, int32_t nobj, const uint32_t* scene, uint64_t* keys, uint32_t* vals,

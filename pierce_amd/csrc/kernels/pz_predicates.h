@@ -107,8 +107,9 @@ __device__ __noinline__ int orient3d_exact(float3 a, float3 b, float3 c, float3 
     return A.sign();
 }
 
-__device__ __forceinline__ int orient3d_f64(const float3& a, const float3& b, const float3& c, const float3& d,
-                                            Esc& esc) {
+// Out of line: in fp32-filtered traversal this stage is rare, and keeping its
+// doubles out of the caller's register budget keeps traversal occupancy up.
+__device__ __noinline__ int orient3d_f64(float3 a, float3 b, float3 c, float3 d, Esc& esc) {
 #pragma clang fp contract(off)
     double adx = (double)a.x - d.x, bdx = (double)b.x - d.x, cdx = (double)c.x - d.x;
     double ady = (double)a.y - d.y, bdy = (double)b.y - d.y, cdy = (double)c.y - d.y;
