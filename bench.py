@@ -51,6 +51,8 @@ def alg_bytes(kernel, k):
         return (8 + 16 + 8) * k["IT"] / 2
     if kernel == "build_small":     # per indexed triangle: indices + 3 vertices read, one node + one leaf written
         return (12 + 36 + 64 + 48) * k["IT"] / 2
+    if kernel == "narrow_dual":     # every indexed triangle (48 B) and internal node (64 B) of both objects read once
+        return (48 + 64) * k["IT"]
     if kernel == "narrow":          # every indexed edge record read once
         return 32 * k["IE"]
     return None
@@ -145,7 +147,7 @@ def main():
         pz.profile_enable(False)
         kernels = {}
         for k in ("object_stats", "edge_count", "tri_keys", "edge_scatter", "karras", "refit", "gather_leaves",
-                  "radix_scatter", "build_small", "narrow", "parity"):
+                  "radix_scatter", "build_small", "narrow_dual", "narrow", "parity"):
             tot, n = pz.profile_query(k)
             if n:
                 kernels[k] = (tot, n)

@@ -211,7 +211,8 @@ typedef struct {
     int32_t  no_alpha_correction;
     int32_t  keep_device_result; /* reserved, must be 0 */
     const pz_mlp* model;        /* PZ_SIZING_LEARNED */
-    int32_t  reserved[8];
+    int32_t  narrow_mode;       /* two-level: 0 = dual-tree with per-edge fallback, 1 = per-edge only */
+    int32_t  reserved[7];
 } pz_join_desc;
 
 /* Fill *d with the defaults: overlap, both directions, TWO_LEVEL, FP32_FILTERED, TWO_PASS. */

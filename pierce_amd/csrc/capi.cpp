@@ -162,7 +162,6 @@ SetPrep prep_set(pz_ctx* c, const pz_dataset* d, const std::string& tag) {
     p.edge_cnt = buf<uint32_t>(c, tag + ".edge_cnt", n);
     pzk::object_stats(d->v, d->nv, d->idx, d->t2o, d->nt, d->nobj, p.obox, p.tri_cnt, p.first_tri, p.last_tri,
                       c->s);
-    pzk::edge_count(d->eo, d->ne, d->nobj, p.edge_cnt, c->s);
     return p;
 }
 
@@ -402,7 +401,6 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         pzk::merge_locs(big, loc_big, loc_small, D->nobj, N.loc, totals, s);
         pzk::needed_layout(N.need, N.loc, D->nobj, P.tri_cnt, P.edge_cnt, N.obj_of_loc, stc, sec, s);
         pzk::exclusive_scan_u32_u32(stc, N.seg_off, no, tmp, s);
-        pzk::exclusive_scan_u32_u32(sec, N.eseg_off, no, tmp, s);
         pzk::big_prefix(N.seg_off, totals, s);
         return big;
     };
@@ -410,10 +408,9 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
     uint32_t* bigB = layout(B, pb, nb, "B");
     uint32_t* totA = buf<uint32_t>(c, "A.totals", 4);
     uint32_t* totB = buf<uint32_t>(c, "B.totals", 4);
-    auto tot = readback<uint32_t>(c, {na.loc + std::max(nA, 0), na.seg_off + std::max(nA, 0),
-                                      na.eseg_off + std::max(nA, 0), totA, totA + 1, nb.loc + std::max(nB, 0),
-                                      nb.seg_off + std::max(nB, 0), nb.eseg_off + std::max(nB, 0), totB, totB + 1});
-    Tot ta{tot[0], tot[1], tot[2], tot[3], tot[4]}, tb{tot[5], tot[6], tot[7], tot[8], tot[9]};
+    auto tot = readback<uint32_t>(c, {na.loc + std::max(nA, 0), na.seg_off + std::max(nA, 0), totA, totA + 1,
+                                      nb.loc + std::max(nB, 0), nb.seg_off + std::max(nB, 0), totB, totB + 1});
+    Tot ta{tot[0], tot[1], 0, tot[2], tot[3]}, tb{tot[4], tot[5], 0, tot[6], tot[7]};
 
     auto build = [&](const pz_dataset* D, const SetPrep& P, const Needed& N, const Tot& T, uint32_t* big,
                      const std::string& tag) -> SetAccel {
@@ -447,35 +444,75 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         }
         pzk::build_small(D->v, D->idx, D->t2o, N.obj_of_loc, P.first_tri, P.tri_cnt, P.obox, N.seg_off,
                          (uint32_t)T.nbig, (uint32_t)(T.nneed - T.nbig), acc.nodes, acc.leaves, acc.height, s);
-        acc.edges = buf<pzd::EdgeRec>(c, tag + ".edges", T.nedge + 1);
-        pzk::fill_u32(cursor, 0u, (size_t)std::max(D->nobj, 1), s);
-        pzk::edge_scatter(D->es, D->ee, D->eo, D->ne, D->nobj, N.need, N.loc, N.eseg_off, cursor, acc.edges, s);
+        acc.edges = nullptr;
         return acc;
     };
     SetAccel xa = build(A, pa, na, ta, bigA, "A");
     SetAccel xb = build(B, pb, nb, tb, bigB, "B");
     r->index_triangles = ta.ntri + tb.ntri;
-    r->index_edges = ta.nedge + tb.nedge;
 
     ph.begin("Narrow Phase");
-    uint32_t* tcnt = buf<uint32_t>(c, "np.tcnt", np + 1);
-    uint64_t* toff = buf<uint64_t>(c, "np.toff", np + 2);
     uint32_t* found = buf<uint32_t>(c, "np.found", np + 1);
-    void* tmp = buf<char>(c, "scan.tmp.np", pzk::scan_temp_bytes(np + 1));
-    pzk::task_counts(pairs, np, na.loc, na.eseg_off, nb.loc, nb.eseg_off, dir_mask, tcnt, s);
-    pzk::exclusive_scan_u32_u64(tcnt, toff, np, tmp, s);
+    uint32_t* fallback = buf<uint32_t>(c, "np.fallback", np + 1);
     pzk::fill_u32(found, 0u, np + 1, s);
-    auto hv = readback<uint64_t>(c, {toff + np});
-    auto hh = readback<uint32_t>(c, {xa.height, xb.height});
-    const uint64_t ntasks = hv[0];
-    const uint32_t height = std::max(hh[0], hh[1]);
-    if (pzk::stack_for_height(height) == 0)
-        throw PzError(PZ_E_LIMIT, "per-object BVH height " + std::to_string(height) + " exceeds the traversal stack (63)");
-    pzd::SetView va{xa.nodes, xa.leaves, na.seg_off, xa.edges, na.eseg_off, na.loc, pa.obox};
-    pzd::SetView vb{xb.nodes, xb.leaves, nb.seg_off, xb.edges, nb.eseg_off, nb.loc, pb.obox};
+    pzk::fill_u32(fallback, 0u, np + 1, s);
+    pzd::SetView va{xa.nodes, xa.leaves, na.seg_off, nullptr, na.eseg_off, na.loc, pa.obox};
+    pzd::SetView vb{xb.nodes, xb.leaves, nb.seg_off, nullptr, nb.eseg_off, nb.loc, pb.obox};
     const bool f32 = d.precision == PZ_PREC_FP32_FILTERED;
     const int allow_f32 = (std::max(A->max_abs, B->max_abs) <= 0x1p39f) ? 1 : 0;
-    pzk::narrow(pairs, np, toff, ntasks, va, vb, found, dir_mask, f32, allow_f32, height, stats, s);
+    uint64_t nfb = np;
+    const uint64_t* fb_pairs = pairs;
+    uint32_t* fb_index = nullptr;
+    if (d.narrow_mode == 0) {
+        pzk::narrow_dual(pairs, np, va, vb, found, fallback, dir_mask, f32, allow_f32, stats, s);
+        uint64_t* fbp = buf<uint64_t>(c, "np.fb_pairs", np + 1);
+        fb_index = buf<uint32_t>(c, "np.fb_index", np + 1);
+        unsigned long long* fbc = buf<unsigned long long>(c, "np.fb_count", 1);
+        pzk::fill_ull(fbc, 0ull, 1, s);
+        pzk::compact_flagged(pairs, np, fallback, fbp, fb_index, fbc, s);
+        nfb = readback<unsigned long long>(c, {fbc})[0];
+        fb_pairs = fbp;
+    }
+    const uint32_t height = [&]() {
+        auto hh = readback<uint32_t>(c, {xa.height, xb.height});
+        return std::max(hh[0], hh[1]);
+    }();
+    if (pzk::stack_for_height(height) == 0)
+        throw PzError(PZ_E_LIMIT, "per-object BVH height " + std::to_string(height) + " exceeds the traversal stack (63)");
+    r->stack_overflows = d.narrow_mode == 0 ? nfb : 0;  // pairs re-run by the per-edge kernel
+    if (nfb > 0) {
+        // Per-edge narrow phase: every edge of the .pre payload regrouped by object.
+        auto edges_of = [&](const pz_dataset* D, const SetPrep& P, Needed& N, const std::string& tag) {
+            const size_t n = std::max(D->nobj, 1);
+            const uint64_t no = (uint64_t)std::max(D->nobj, 0);
+            uint32_t* stc = buf<uint32_t>(c, tag + ".seg_tri_cnt", n);
+            uint32_t* sec = buf<uint32_t>(c, tag + ".seg_edge_cnt", n);
+            pzk::edge_count(D->eo, D->ne, D->nobj, P.edge_cnt, s);
+            pzk::fill_u32(sec, 0u, n, s);
+            pzk::needed_layout(N.need, N.loc, D->nobj, P.tri_cnt, P.edge_cnt, N.obj_of_loc, stc, sec, s);
+            void* tmp = buf<char>(c, "scan.tmp", pzk::scan_temp_bytes(n + 1));
+            pzk::exclusive_scan_u32_u32(sec, N.eseg_off, no, tmp, s);
+            uint64_t ne = readback<uint32_t>(c, {N.eseg_off + no})[0];
+            pzd::EdgeRec* edges = buf<pzd::EdgeRec>(c, tag + ".edges", ne + 1);
+            uint32_t* cursor = buf<uint32_t>(c, tag + ".cursor", n);
+            pzk::fill_u32(cursor, 0u, n, s);
+            pzk::edge_scatter(D->es, D->ee, D->eo, D->ne, D->nobj, N.need, N.loc, N.eseg_off, cursor, edges, s);
+            r->index_edges += ne;
+            return edges;
+        };
+        va.edges = edges_of(A, pa, na, "A");
+        vb.edges = edges_of(B, pb, nb, "B");
+        uint32_t* tcnt = buf<uint32_t>(c, "np.tcnt", nfb + 1);
+        uint64_t* toff = buf<uint64_t>(c, "np.toff", nfb + 2);
+        void* tmp = buf<char>(c, "scan.tmp.np", pzk::scan_temp_bytes(nfb + 1));
+        pzk::task_counts(fb_pairs, nfb, na.loc, na.eseg_off, nb.loc, nb.eseg_off, dir_mask, tcnt, s);
+        pzk::exclusive_scan_u32_u64(tcnt, toff, nfb, tmp, s);
+        const uint64_t ntasks = readback<uint64_t>(c, {toff + nfb})[0];
+        uint32_t* sub_found = fb_index ? buf<uint32_t>(c, "np.sub_found", nfb + 1) : found;
+        if (fb_index) pzk::fill_u32(sub_found, 0u, nfb + 1, s);
+        pzk::narrow(fb_pairs, nfb, toff, ntasks, va, vb, sub_found, dir_mask, f32, allow_f32, height, stats, s);
+        if (fb_index) pzk::scatter_found(sub_found, fb_index, nfb, found, s);
+    }
 
     uint32_t* inside = nullptr;
     if (d.predicate != PZ_PRED_OVERLAP) {

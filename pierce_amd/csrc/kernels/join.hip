@@ -486,6 +486,186 @@ __global__ __launch_bounds__(kBlock, 4) void k_narrow(const uint64_t* __restrict
     flush_stats(st, stats);
 }
 
+// Dual-tree narrow phase: one wave per candidate pair walks pairs of
+// (mesh1 node, mesh2 node) whose boxes meet, from the two object roots, with
+// a LIFO queue in LDS (64 entries popped per step, children pairs pushed
+// after a wave prefix sum).  At a pair of leaves the edges of each triangle
+// are tested against the other triangle (direction bit 0: mesh1 edges,
+// bit 1: mesh2 edges), which is the edge-against-triangle predicate on the
+// triangles' own edges.  The wave stops at the first proven intersection.
+// A queue overflow marks the pair for the per-edge kernel instead of
+// dropping work.
+constexpr int kDualQ = 1536;
+constexpr int kDualWaves = 4;
+
+__device__ __forceinline__ void leaf_box(const LeafTri& L, float* lo, float* hi) {
+    lo[0] = fminf(L.v0.x, fminf(L.v1.x, L.v2.x));
+    lo[1] = fminf(L.v0.y, fminf(L.v1.y, L.v2.y));
+    lo[2] = fminf(L.v0.z, fminf(L.v1.z, L.v2.z));
+    hi[0] = fmaxf(L.v0.x, fmaxf(L.v1.x, L.v2.x));
+    hi[1] = fmaxf(L.v0.y, fmaxf(L.v1.y, L.v2.y));
+    hi[2] = fmaxf(L.v0.z, fmaxf(L.v1.z, L.v2.z));
+}
+
+__device__ __forceinline__ void child_box(const Node& nd, int slot, float* lo, float* hi) {
+    if (slot == 0) {
+        lo[0] = nd.a.x; lo[1] = nd.a.y; lo[2] = nd.a.z;
+        hi[0] = nd.a.w; hi[1] = nd.b.x; hi[2] = nd.b.y;
+    } else {
+        lo[0] = nd.b.z; lo[1] = nd.b.w; lo[2] = nd.c.x;
+        hi[0] = nd.c.y; hi[1] = nd.c.z; hi[2] = nd.c.w;
+    }
+}
+
+__device__ __forceinline__ bool box_box(const float* alo, const float* ahi, const float* blo, const float* bhi) {
+    return !(alo[0] > bhi[0] || ahi[0] < blo[0] || alo[1] > bhi[1] || ahi[1] < blo[1] || alo[2] > bhi[2] ||
+             ahi[2] < blo[2]);
+}
+
+template <bool F32>
+__device__ __forceinline__ bool tri_pair_test(const LeafTri& X, const LeafTri& Y, int dir_mask, bool allow_f32,
+                                              LaneStats& st) {
+    const float3 x0 = make_float3(X.v0.x, X.v0.y, X.v0.z), x1 = make_float3(X.v1.x, X.v1.y, X.v1.z),
+                 x2 = make_float3(X.v2.x, X.v2.y, X.v2.z);
+    const float3 y0 = make_float3(Y.v0.x, Y.v0.y, Y.v0.z), y1 = make_float3(Y.v1.x, Y.v1.y, Y.v1.z),
+                 y2 = make_float3(Y.v2.x, Y.v2.y, Y.v2.z);
+    Esc esc{0};
+    bool hit = false;
+    int tests = 0;
+    if (dir_mask & 1) {
+        ++tests;
+        hit = seg_tri<F32>(x0, x1, y0, y1, y2, allow_f32, esc);
+        if (!hit) { ++tests; hit = seg_tri<F32>(x1, x2, y0, y1, y2, allow_f32, esc); }
+        if (!hit) { ++tests; hit = seg_tri<F32>(x2, x0, y0, y1, y2, allow_f32, esc); }
+    }
+    if (!hit && (dir_mask & 2)) {
+        ++tests;
+        hit = seg_tri<F32>(y0, y1, x0, x1, x2, allow_f32, esc);
+        if (!hit) { ++tests; hit = seg_tri<F32>(y1, y2, x0, x1, x2, allow_f32, esc); }
+        if (!hit) { ++tests; hit = seg_tri<F32>(y2, y0, x0, x1, x2, allow_f32, esc); }
+    }
+    st.tests += tests;
+    if (esc.level >= 1) {
+        if (esc.level == 1)
+            ++st.f64;
+        else
+            ++st.exact;
+        if (F32) {
+            bool naive = false;
+            if (dir_mask & 1)
+                naive = seg_tri_naive(x0, x1, y0, y1, y2) || seg_tri_naive(x1, x2, y0, y1, y2) ||
+                        seg_tri_naive(x2, x0, y0, y1, y2);
+            if (!naive && (dir_mask & 2))
+                naive = seg_tri_naive(y0, y1, x0, x1, x2) || seg_tri_naive(y1, y2, x0, x1, x2) ||
+                        seg_tri_naive(y2, y0, x0, x1, x2);
+            if (naive != hit) ++st.mis;
+        }
+    }
+    return hit;
+}
+
+template <bool F32>
+__global__ __launch_bounds__(kDualWaves * PZ_WAVE, 2) void k_narrow_dual(
+    const uint64_t* __restrict__ pairs, uint64_t np, SetView A, SetView B, uint32_t* __restrict__ found,
+    uint32_t* __restrict__ fallback, int dir_mask, int allow_f32, unsigned long long* __restrict__ stats) {
+    __shared__ int2 q[kDualWaves][kDualQ];
+    const int lane = __lane_id(), w = threadIdx.x / PZ_WAVE;
+    const uint64_t p = (uint64_t)blockIdx.x * kDualWaves + w;
+    if (p >= np) return;
+    const uint64_t key = pairs[p];
+    const int32_t oa = (int32_t)(key >> 32), ob = (int32_t)(uint32_t)key;
+    const uint32_t la = A.loc[oa], lb = B.loc[ob];
+    const uint32_t a0 = A.seg_off[la], a1 = A.seg_off[la + 1], b0 = B.seg_off[lb], b1 = B.seg_off[lb + 1];
+    const int ra = (a1 - a0 >= 2) ? (int)a0 : ~(int)a0;
+    const int rb = (b1 - b0 >= 2) ? (int)b0 : ~(int)b0;
+    int2* Q = q[w];
+    if (lane == 0) Q[0] = make_int2(ra, rb);
+    int qn = 1;
+    bool hit = false, ovf = false;
+    LaneStats st{0, 0, 0, 0, 0};
+    while (qn > 0) {
+        const int take = qn < PZ_WAVE ? qn : PZ_WAVE;
+        const bool act = lane < take;
+        int2 e = act ? Q[qn - 1 - lane] : make_int2(0, 0);
+        qn -= take;
+        int2 o0 = make_int2(0, 0), o1 = o0, o2 = o0, o3 = o0;
+        int nout = 0;
+        if (act) {
+            const int x = e.x, y = e.y;
+            if (x < 0 && y < 0) {
+                hit = tri_pair_test<F32>(A.leaves[~x], B.leaves[~y], dir_mask, allow_f32 != 0, st);
+            } else if (x >= 0 && y >= 0) {
+                const Node X = A.nodes[x], Y = B.nodes[y];
+                float xl[2][3], xh[2][3], yl[2][3], yh[2][3];
+                child_box(X, 0, xl[0], xh[0]);
+                child_box(X, 1, xl[1], xh[1]);
+                child_box(Y, 0, yl[0], yh[0]);
+                child_box(Y, 1, yl[1], yh[1]);
+                const int xc[2] = {X.d.x, X.d.y}, yc[2] = {Y.d.x, Y.d.y};
+                for (int i = 0; i < 2; ++i)
+                    for (int j = 0; j < 2; ++j)
+                        if (box_box(xl[i], xh[i], yl[j], yh[j])) {
+                            int2 v = make_int2(xc[i], yc[j]);
+                            if (nout == 0) o0 = v;
+                            else if (nout == 1) o1 = v;
+                            else if (nout == 2) o2 = v;
+                            else o3 = v;
+                            ++nout;
+                        }
+            } else if (x >= 0) {  // mesh2 side is a leaf
+                const Node X = A.nodes[x];
+                float yl[3], yh[3], cl[3], chh[3];
+                leaf_box(B.leaves[~y], yl, yh);
+                child_box(X, 0, cl, chh);
+                if (box_box(cl, chh, yl, yh)) { o0 = make_int2(X.d.x, y); nout = 1; }
+                child_box(X, 1, cl, chh);
+                if (box_box(cl, chh, yl, yh)) {
+                    if (nout == 0) o0 = make_int2(X.d.y, y);
+                    else o1 = make_int2(X.d.y, y);
+                    ++nout;
+                }
+            } else {  // mesh1 side is a leaf
+                const Node Y = B.nodes[y];
+                float xl[3], xh[3], cl[3], chh[3];
+                leaf_box(A.leaves[~x], xl, xh);
+                child_box(Y, 0, cl, chh);
+                if (box_box(cl, chh, xl, xh)) { o0 = make_int2(x, Y.d.x); nout = 1; }
+                child_box(Y, 1, cl, chh);
+                if (box_box(cl, chh, xl, xh)) {
+                    if (nout == 0) o0 = make_int2(x, Y.d.y);
+                    else o1 = make_int2(x, Y.d.y);
+                    ++nout;
+                }
+            }
+        }
+        if (__ballot(hit) != 0) break;
+        // append children pairs: wave-wide exclusive prefix of nout
+        int incl = nout;
+        for (int s = 1; s < PZ_WAVE; s <<= 1) {
+            int u = __shfl_up(incl, s, PZ_WAVE);
+            if (lane >= s) incl += u;
+        }
+        const int total = __shfl(incl, PZ_WAVE - 1, PZ_WAVE);
+        if (qn + total > kDualQ) {
+            ovf = true;
+            break;
+        }
+        int pos = qn + incl - nout;
+        if (nout > 0) Q[pos] = o0;
+        if (nout > 1) Q[pos + 1] = o1;
+        if (nout > 2) Q[pos + 2] = o2;
+        if (nout > 3) Q[pos + 3] = o3;
+        qn += total;
+        __builtin_amdgcn_wave_barrier();
+    }
+    const bool any_hit = __ballot(hit) != 0;
+    if (lane == 0) {
+        if (any_hit) found[p] = 1u;
+        else if (ovf) fallback[p] = 1u;
+    }
+    flush_stats(st, stats);
+}
+
 // Parity: thread per (pair, direction).  Counts +Z crossings of the
 // perturbed ray from the first vertex of one object against the other's
 // triangles.  kind 0: containment (point of mesh2 object in mesh1 object);
@@ -749,6 +929,31 @@ __global__ __launch_bounds__(kBlock) void k_compact_table(const unsigned long lo
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_compact_flagged(const uint64_t* __restrict__ pairs, uint64_t np,
+                                                           const uint32_t* __restrict__ flag, uint64_t* __restrict__ out,
+                                                           uint32_t* __restrict__ out_index,
+                                                           unsigned long long* __restrict__ count) {
+    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    bool keep = i < np && flag[i] != 0;
+    uint64_t m = __ballot(keep);
+    if (m == 0) return;
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned long long base = 0;
+    if (__lane_id() == leader) base = atomicAdd(count, (unsigned long long)__popcll(m));
+    base = __shfl((long long)base, leader, 64);
+    if (keep) {
+        uint64_t pos = base + lanes_below(m);
+        out[pos] = pairs[i];
+        out_index[pos] = (uint32_t)i;
+    }
+}
+
+__global__ void k_scatter_found(const uint32_t* __restrict__ sub, const uint32_t* __restrict__ index, uint64_t n,
+                                uint32_t* __restrict__ found) {
+    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n && sub[i]) found[index[i]] = 1u;
+}
+
 template <class T>
 __global__ void k_fill_any(T* p, T v, uint64_t n) {
     uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -919,6 +1124,19 @@ bool narrow(const uint64_t* pairs, uint64_t np, const uint64_t* task_off, uint64
     return true;
 }
 
+void narrow_dual(const uint64_t* pairs, uint64_t np, const SetView& A, const SetView& B, uint32_t* found,
+                 uint32_t* fallback, int dir_mask, bool f32, int allow_f32, unsigned long long* stats, hipStream_t s) {
+    if (np == 0) return;
+    unsigned g = (unsigned)((np + kDualWaves - 1) / kDualWaves);
+    KernelTimer kt_("narrow_dual", s);
+    if (f32)
+        hipLaunchKernelGGL((k_narrow_dual<true>), dim3(g), dim3(kDualWaves * PZ_WAVE), 0, s, pairs, np, A, B, found,
+                           fallback, dir_mask, allow_f32, stats);
+    else
+        hipLaunchKernelGGL((k_narrow_dual<false>), dim3(g), dim3(kDualWaves * PZ_WAVE), 0, s, pairs, np, A, B, found,
+                           fallback, dir_mask, allow_f32, stats);
+}
+
 bool parity(const uint64_t* pairs, uint64_t np, const uint32_t* found, const SetView& A, const SetView& B,
             const float* vA, const uint32_t* iA, const uint32_t* ftA, const float* vB, const uint32_t* iB,
             const uint32_t* ftB, int kind, int dir_mask, uint32_t* inside, uint32_t height,
@@ -975,6 +1193,15 @@ bool trace_global(int mode, const float* es, const float* ee, const int32_t* eo,
 void compact_table(const unsigned long long* table, uint64_t size, uint64_t* out, uint64_t cap,
                    unsigned long long* count, hipStream_t s) {
     if (size) hipLaunchKernelGGL(k_compact_table, dim3(grid1(size)), dim3(kBlock), 0, s, table, size, out, cap, count);
+}
+
+void compact_flagged(const uint64_t* pairs, uint64_t np, const uint32_t* flag, uint64_t* out, uint32_t* out_index,
+                     unsigned long long* count, hipStream_t s) {
+    if (np) hipLaunchKernelGGL(k_compact_flagged, dim3(grid1(np)), dim3(kBlock), 0, s, pairs, np, flag, out, out_index, count);
+}
+
+void scatter_found(const uint32_t* sub_found, const uint32_t* index, uint64_t n, uint32_t* found, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_scatter_found, dim3(grid1(n)), dim3(kBlock), 0, s, sub_found, index, n, found);
 }
 
 void fill_ull(unsigned long long* p, unsigned long long v, uint64_t n, hipStream_t s) {

@@ -75,6 +75,15 @@ void task_counts(const uint64_t* pairs, uint64_t np, const uint32_t* locA, const
 bool narrow(const uint64_t* pairs, uint64_t np, const uint64_t* task_off, uint64_t ntasks, const SetView& A,
             const SetView& B, uint32_t* found, int dir_mask, bool f32, int allow_f32, uint32_t height,
             unsigned long long* stats, hipStream_t s);
+// Dual-tree narrow phase (one wave per pair); pairs whose LDS queue overflows
+// get fallback[p] = 1 and must be re-run with narrow().
+void narrow_dual(const uint64_t* pairs, uint64_t np, const SetView& A, const SetView& B, uint32_t* found,
+                 uint32_t* fallback, int dir_mask, bool f32, int allow_f32, unsigned long long* stats, hipStream_t s);
+// Compact pairs with flag[p] != 0 into out (count -> *count).
+void compact_flagged(const uint64_t* pairs, uint64_t np, const uint32_t* flag, uint64_t* out, uint32_t* out_index,
+                     unsigned long long* count, hipStream_t s);
+// found[index[i]] |= sub_found[i]
+void scatter_found(const uint32_t* sub_found, const uint32_t* index, uint64_t n, uint32_t* found, hipStream_t s);
 bool parity(const uint64_t* pairs, uint64_t np, const uint32_t* found, const SetView& A, const SetView& B,
             const float* vA, const uint32_t* iA, const uint32_t* ftA, const float* vB, const uint32_t* iB,
             const uint32_t* ftB, int kind, int dir_mask, uint32_t* inside, uint32_t height,

@@ -28,12 +28,13 @@ def c1(ctx):
     return a, b, ctx.upload(a), ctx.upload(b), orc.face_overlap(a, b)
 
 
+@pytest.mark.parametrize("narrow_mode", [0, 1])
 @pytest.mark.parametrize("precision", [pz.PREC_FP32_FILTERED, pz.PREC_FP64])
 @pytest.mark.parametrize("sizing", [pz.SIZING_TWO_PASS, pz.SIZING_FIXED])
-def test_two_level_overlap_c1(ctx, c1, precision, sizing):
+def test_two_level_overlap_c1(ctx, c1, precision, sizing, narrow_mode):
     a, b, da, db, expected = c1
-    print(f"[two-level c1] precision={precision} sizing={sizing}", flush=True)
-    out = pz.join(ctx, da, db, precision=precision, sizing=sizing, capacity=16)
+    print(f"[two-level c1] precision={precision} sizing={sizing} narrow={narrow_mode}", flush=True)
+    out = pz.join(ctx, da, db, precision=precision, sizing=sizing, capacity=16, narrow_mode=narrow_mode)
     assert out.overflowed == (sizing == pz.SIZING_FIXED and out.num_candidates > 16)
     np.testing.assert_array_equal(out.pairs, expected)
 
@@ -54,10 +55,11 @@ def test_hash_overflow_falls_back(ctx, c1):
     np.testing.assert_array_equal(out.pairs, expected)
 
 
+@pytest.mark.parametrize("narrow_mode", [0, 1])
 @pytest.mark.parametrize("direction", [pz.DIR_1TO2, pz.DIR_2TO1])
-def test_direction_split_c1(ctx, c1, direction):
+def test_direction_split_c1(ctx, c1, direction, narrow_mode):
     a, b, da, db, expected = c1
-    out = pz.join(ctx, da, db, direction=direction)
+    out = pz.join(ctx, da, db, direction=direction, narrow_mode=narrow_mode)
     np.testing.assert_array_equal(out.pairs, orc.edge_overlap(a, b, direction))
 
 
@@ -75,13 +77,14 @@ def containment_sets(n=300, box=60.0):
     return a, b
 
 
+@pytest.mark.parametrize("narrow_mode", [0, 1])
 @pytest.mark.parametrize("predicate", [pz.PRED_INTERSECTION, pz.PRED_CONTAINMENT])
-def test_predicates_vs_oracle(ctx, predicate):
+def test_predicates_vs_oracle(ctx, predicate, narrow_mode):
     a, b = containment_sets()
-    print(f"[predicate {predicate}]", flush=True)
+    print(f"[predicate {predicate} narrow {narrow_mode}]", flush=True)
     da, db = ctx.upload(a), ctx.upload(b)
     exp, _ = orc.predicate_join(a, b, predicate)
-    out = pz.join(ctx, da, db, predicate=predicate)
+    out = pz.join(ctx, da, db, predicate=predicate, narrow_mode=narrow_mode)
     np.testing.assert_array_equal(out.pairs, exp)
     if predicate == pz.PRED_CONTAINMENT:
         assert len(exp) > 0
