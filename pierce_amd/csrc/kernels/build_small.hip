@@ -15,7 +15,7 @@ namespace {
 
 using namespace pzd;
 
-constexpr int kThreads = 512;
+constexpr int kThreads = 1024;
 constexpr int kMax = (int)pzk::kSmallMax;
 
 __device__ __forceinline__ float3 vtx(const float* __restrict__ v, uint32_t i) {
@@ -71,6 +71,7 @@ __global__ __launch_bounds__(kThreads) void k_build_small(
     const uint32_t* __restrict__ tri_cnt, const uint32_t* __restrict__ obox, const uint32_t* __restrict__ seg_off,
     uint32_t l_begin, Node* __restrict__ nodes, LeafTri* __restrict__ leaves, uint32_t* __restrict__ height_out) {
     __shared__ uint64_t key[kMax];                 // 16 KB
+    __shared__ float lbox[kMax][6];                // 48 KB: box of the leaf at sorted position p
     __shared__ float nbox[kMax][6];                // 48 KB: box of internal node i
     __shared__ int16_t ch[kMax][2];                // children: j >= 0 internal, < 0 leaf ~j
     __shared__ uint16_t par_leaf[kMax], par_int[kMax];
@@ -107,10 +108,16 @@ __global__ __launch_bounds__(kThreads) void k_build_small(
         }
     }
     __syncthreads();
-    // 2. bitonic sort of npow keys (ascending)
+    // 2. bitonic sort of npow keys (ascending).  Thread i owns compare-exchange
+    // pair i, so for strides <= 64 every pair of a wave lies in that wave's own
+    // 128-key segment and only stages with larger strides need a block barrier.
+    static_assert(kThreads * 2 >= kMax, "one compare-exchange pair per thread");
     for (int size = 2; size <= npow; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int i = tid; i < npow / 2; i += kThreads) {
+            const bool cross_wave = stride > PZ_WAVE;
+            if (cross_wave) __syncthreads();
+            const int i = tid;
+            if (i < npow / 2) {
                 int lo_i = 2 * i - (i & (stride - 1));
                 int hi_i = lo_i + stride;
                 bool up = (lo_i & size) == 0;
@@ -120,7 +127,18 @@ __global__ __launch_bounds__(kThreads) void k_build_small(
                     key[hi_i] = x;
                 }
             }
-            __syncthreads();
+            if (cross_wave) __syncthreads();
+            else __builtin_amdgcn_wave_barrier();
+        }
+    }
+    __syncthreads();
+    // leaf boxes in sorted order (the only other global read of the triangles)
+    for (int p = tid; p < n; p += kThreads) {
+        float blo[3], bhi[3];
+        tri_box(verts, idx, t0 + (uint32_t)(key[p] & 0xFFFFFFFFu), blo, bhi);
+        for (int k = 0; k < 3; ++k) {
+            lbox[p][k] = blo[k];
+            lbox[p][3 + k] = bhi[k];
         }
     }
     // 3. Karras hierarchy over sorted positions [0, n)
@@ -164,7 +182,10 @@ __global__ __launch_bounds__(kThreads) void k_build_small(
     if (n >= 2) {
         for (int p = tid; p < n; p += kThreads) {
             float blo[3], bhi[3];
-            tri_box(verts, idx, t0 + (uint32_t)(key[p] & 0xFFFFFFFFu), blo, bhi);
+            for (int k = 0; k < 3; ++k) {
+                blo[k] = lbox[p][k];
+                bhi[k] = lbox[p][3 + k];
+            }
             int node = par_leaf[p];
             int child = ~p;
             int h = 0;
@@ -176,7 +197,10 @@ __global__ __launch_bounds__(kThreads) void k_build_small(
                 float slo[3], shi[3];
                 int sh;
                 if (sib < 0) {
-                    tri_box(verts, idx, t0 + (uint32_t)(key[~sib] & 0xFFFFFFFFu), slo, shi);
+                    for (int k = 0; k < 3; ++k) {
+                        slo[k] = lbox[~sib][k];
+                        shi[k] = lbox[~sib][3 + k];
+                    }
                     sh = 0;
                 } else {
                     for (int k = 0; k < 3; ++k) {
@@ -211,7 +235,7 @@ __global__ __launch_bounds__(kThreads) void k_build_small(
         for (int sl = 0; sl < 2; ++sl) {
             const int c = ch[i][sl];
             if (c < 0) {
-                tri_box(verts, idx, t0 + (uint32_t)(key[~c] & 0xFFFFFFFFu), &b[sl][0], &b[sl][3]);
+                for (int k = 0; k < 6; ++k) b[sl][k] = lbox[~c][k];
                 gc[sl] = ~(int)(base + (uint32_t)(~c));
             } else {
                 for (int k = 0; k < 6; ++k) b[sl][k] = nbox[c][k];
