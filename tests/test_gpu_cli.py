@@ -1,0 +1,103 @@
+"""The four query tools end to end on .pre files: CSV contents against the
+oracle, stdout lines and timing-JSON keys the reference harness parses
+(benchmarks/overlap/adapters/pierce_adapter.py:228-376,
+benchmarks/predicates/adapters/containment.py:134-172)."""
+import csv
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import pierce_amd as pz
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def run(tool, args, cwd):
+    print(f"[cli] {tool} {' '.join(args)}", flush=True)
+    r = subprocess.run([os.path.join(pz.BIN_DIR, tool)] + args, cwd=cwd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def read_pairs(path):
+    with open(path) as f:
+        rows = list(csv.reader(f))
+    return rows[0], [tuple(int(x) for x in r[:2]) for r in rows[1:]], [r[2] for r in rows[1:] if len(r) > 2]
+
+
+@pytest.fixture(scope="module")
+def pre_files(tmp_path_factory):
+    d = tmp_path_factory.mktemp("pre")
+    gen = os.path.join(pz.BIN_DIR, "pierce_generate")
+    common = ["--count", "300", "--level", "2", "--radius-min", "0.5", "--radius-max", "2.0", "--box", "60",
+              "--generate-grid"]
+    subprocess.run([gen, "--output", str(d / "a.pre"), "--seed", "7"] + common, check=True, capture_output=True)
+    subprocess.run([gen, "--output", str(d / "b.pre"), "--seed", "8", "--copy-fraction", "0.2", "--copy-scale-min",
+                    "0.3", "--copy-scale-max", "0.8", "--copy-source-count", "300", "--copy-source-seed", "7"] + common,
+                   check=True, capture_output=True)
+    a, b = pz.read_pre(str(d / "a.pre")), pz.read_pre(str(d / "b.pre"))
+    return d, a, b
+
+
+def test_two_pass_tool(pre_files, tmp_path):
+    d, a, b = pre_files
+    out = run("pierce_overlap_two_pass", ["--mesh1", str(d / "a.pre"), "--mesh2", str(d / "b.pre"), "--output",
+                                          "t.json", "--warmup-runs", "1", "--ptx", "ignored"], tmp_path)
+    hdr, pairs, _ = read_pairs(tmp_path / "mesh_overlap_results.csv")
+    assert hdr == ["object_id_mesh1", "object_id_mesh2"]
+    exp = orc.face_overlap(a, b)
+    assert pairs == [tuple(p) for p in exp.tolist()]
+    assert re.search(r"^Unique object pairs: %d$" % len(exp), out, re.M)
+    assert re.search(r"^Mesh1 objects: 300$", out, re.M)
+    phases = json.load(open(tmp_path / "t.json"))["phases"]
+    assert any(k.startswith("raytrace_") for k in phases) and "download results_1" in phases
+
+
+def test_overlap_tool_estimate_and_pairs_output(pre_files, tmp_path):
+    d, a, b = pre_files
+    out = run("pierce_overlap", ["--mesh1", str(d / "a.pre"), "--mesh2", str(d / "b.pre"), "--pairs-output",
+                                 "p.csv", "--no-export", "--query-direction", "both"], tmp_path)
+    hdr, pairs, _ = read_pairs(tmp_path / "p.csv")
+    assert pairs == [tuple(p) for p in orc.face_overlap(a, b).tolist()]
+    assert re.search(r"^Final Estimated Pairs:\s+\d+$", out, re.M)
+    assert re.search(r"^Hash Table Query found \d+ unique pairs\.$", out, re.M)
+    phases = json.load(open(tmp_path / "estimated_overlap_timing.json"))["phases"]
+    assert "selectivity estimation_1" in phases and "raytrace_hash_mesh1tomesh2_1" in phases
+
+
+def test_intersection_tool(pre_files, tmp_path):
+    d, a, b = pre_files
+    out = run("pierce_intersection", ["--mesh1", str(d / "a.pre"), "--mesh2", str(d / "b.pre")], tmp_path)
+    hdr, pairs, _ = read_pairs(tmp_path / "intersection_pairs.csv")
+    assert hdr == ["a_object_id", "b_object_id"]
+    exp, _ = orc.predicate_join(a, b, 1)
+    assert pairs == [tuple(p) for p in exp.tolist()]
+    assert re.search(r"^Actual Intersection Pairs: %d$" % len(exp), out, re.M)
+
+
+def test_containment_tool(pre_files, tmp_path):
+    d, a, b = pre_files
+    out = run("pierce_containment", ["--mesh1", str(d / "a.pre"), "--mesh2", str(d / "b.pre"),
+                                     "--include-overlap-pairs"], tmp_path)
+    hdr, pairs, types = read_pairs(tmp_path / "mesh_containment_results.csv")
+    assert hdr == ["a_object_id", "b_object_id", "pair_type"]
+    exp, et = orc.predicate_join(a, b, 2, include_overlap=True)
+    cont = [tuple(p) for p, t in zip(exp.tolist(), et) if t == 0]
+    over = [tuple(p) for p, t in zip(exp.tolist(), et) if t == 1]
+    assert pairs == cont + over and types == ["containment"] * len(cont) + ["overlap"] * len(over)
+    assert re.search(r"^Containment pairs \(B in A\): %d$" % len(cont), out, re.M)
+    assert re.search(r"^Reported pairs: %d$" % len(exp), out, re.M)
+
+
+def test_tool_rejects_pre_without_edges(tmp_path):
+    m = pz.generate(pz.GEN_ELLIPSOIDS, count=4, seed=1, edges_mode=pz.EDGES_NONE)
+    pz.write_pre(str(tmp_path / "n.pre"), m)
+    r = subprocess.run([os.path.join(pz.BIN_DIR, "pierce_overlap_two_pass"), "--mesh1", str(tmp_path / "n.pre"),
+                        "--mesh2", str(tmp_path / "n.pre")], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 1 and "precomputed edges" in r.stderr

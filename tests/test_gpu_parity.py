@@ -11,6 +11,7 @@ import pytest
 
 import pierce_amd as pz
 from oracle import oracle as orc
+from tests.meshes import containment_sets, cube_mesh, equal_cube_offsets, nested_cubes
 
 pytestmark = pytest.mark.gpu
 
@@ -68,15 +69,6 @@ def test_candidate_count_c1(ctx, c1):
     assert pz.count_candidate_tests(ctx, da, db) == orc.count_candidate_tests(a, b)
 
 
-def containment_sets(n=300, box=60.0):
-    a = pz.generate(pz.GEN_ELLIPSOIDS, count=n, seed=7, level_min=2, level_max=2, radius_min=0.5, radius_max=2.0,
-                    box=box)
-    b = pz.generate(pz.GEN_ELLIPSOIDS, count=n, seed=8, level_min=2, level_max=2, radius_min=0.5, radius_max=2.0,
-                    box=box, copy_fraction=0.2, copy_scale_min=0.3, copy_scale_max=0.8, copy_source_count=n,
-                    copy_source_seed=7)
-    return a, b
-
-
 @pytest.mark.parametrize("narrow_mode", [0, 1])
 @pytest.mark.parametrize("predicate", [pz.PRED_INTERSECTION, pz.PRED_CONTAINMENT])
 def test_predicates_vs_oracle(ctx, predicate, narrow_mode):
@@ -97,3 +89,56 @@ def test_containment_with_overlap_pairs(ctx):
     out = pz.join(ctx, da, db, predicate=pz.PRED_CONTAINMENT, include_overlap_pairs=1)
     np.testing.assert_array_equal(out.pairs, exp)
     np.testing.assert_array_equal(out.types, types)
+
+
+@pytest.mark.parametrize("precision", [pz.PREC_FP32_FILTERED, pz.PREC_FP64])
+@pytest.mark.parametrize("strategy,narrow_mode", [(pz.STRATEGY_TWO_LEVEL, 0), (pz.STRATEGY_TWO_LEVEL, 1),
+                                                  (pz.STRATEGY_GLOBAL, 0)])
+def test_touching_cubes_exact(ctx, precision, strategy, narrow_mode):
+    """Exact face/edge/vertex contact and coplanar faces: the escalation paths decide these."""
+    offs = equal_cube_offsets()
+    a = cube_mesh([np.zeros(3, np.float32)] * len(offs), [2.0] * len(offs))
+    b = cube_mesh(offs, [2.0] * len(offs))
+    exp = orc.face_overlap(a, b)
+    da, db = ctx.upload(a), ctx.upload(b)
+    out = pz.join(ctx, da, db, precision=precision, strategy=strategy, narrow_mode=narrow_mode)
+    np.testing.assert_array_equal(out.pairs, exp)
+    assert out.tests_exact > 0  # coplanar contact needs exact signs
+
+
+@pytest.mark.parametrize("predicate", [pz.PRED_OVERLAP, pz.PRED_INTERSECTION, pz.PRED_CONTAINMENT])
+def test_nested_cubes(ctx, predicate):
+    a, b = nested_cubes()
+    da, db = ctx.upload(a), ctx.upload(b)
+    exp = orc.face_overlap(a, b) if predicate == pz.PRED_OVERLAP else orc.predicate_join(a, b, predicate)[0]
+    out = pz.join(ctx, da, db, predicate=predicate)
+    np.testing.assert_array_equal(out.pairs, exp)
+
+
+def test_self_join_and_swap(ctx):
+    a, b = containment_sets()
+    da, db = ctx.upload(a), ctx.upload(b)
+    self_pairs = pz.join(ctx, da, da).pairs
+    assert set((i, i) for i in range(a.num_objects)) <= set(map(tuple, self_pairs.tolist()))
+    ab = pz.join(ctx, da, db).pairs
+    ba = pz.join(ctx, db, da).pairs
+    assert set(map(tuple, ab.tolist())) == set(map(tuple, ba[:, ::-1].tolist()))
+
+
+def test_sizing_modes_agree(ctx, c1):
+    a, b, da, db, expected = c1
+    for sizing, cap in ((pz.SIZING_TWO_PASS, 0), (pz.SIZING_FIXED, 1), (pz.SIZING_FIXED, 1 << 20),
+                        (pz.SIZING_ESTIMATE, 0)):
+        out = pz.join(ctx, da, db, sizing=sizing, capacity=cap)
+        np.testing.assert_array_equal(out.pairs, expected)
+
+
+def test_empty_result(ctx):
+    a = pz.generate(pz.GEN_ELLIPSOIDS, count=5, seed=1, box=10.0)
+    b = pz.generate(pz.GEN_ELLIPSOIDS, count=5, seed=2, box=10.0)
+    b.vertices += np.float32(1000.0)
+    b.edge_starts += np.float32(1000.0)
+    b.edge_ends += np.float32(1000.0)
+    da, db = ctx.upload(a), ctx.upload(b)
+    for pred in (pz.PRED_OVERLAP, pz.PRED_INTERSECTION, pz.PRED_CONTAINMENT):
+        assert len(pz.join(ctx, da, db, predicate=pred).pairs) == 0

@@ -21,6 +21,7 @@ import pytest
 
 import pierce_amd as pz
 from oracle import oracle as orc
+from tests.meshes import cube_mesh
 
 
 # ------------------------------------------------------------ exact reference
@@ -258,20 +259,6 @@ def test_point_in_convex_mesh_with_degenerate_rays():
 
 
 # ------------------------------------------------------------ joins
-def cube_mesh(centres, edges):
-    verts, idx, t2o = [], [], []
-    tmpl = pz.generate(pz.GEN_CUBES, count=1, seed=0, radius_min=2.0, radius_max=2.0, box=0.0)
-    unit = tmpl.vertices / 1.0  # cube of edge 2 centred at 0 -> half extents 1
-    for o, (c, e) in enumerate(zip(centres, edges)):
-        base = len(verts)
-        for v in unit:
-            verts.append(np.float32(c) + np.float32(e / 2.0) * v)
-        idx.extend((tmpl.indices + base).tolist())
-        t2o.extend([o] * len(tmpl.indices))
-    m = pz.Mesh(np.array(verts, np.float32), np.array(idx, np.uint32), np.array(t2o, np.int32))
-    return pz.extract_edges(m)
-
-
 def test_equal_cubes_closed_form():
     # two equal axis-aligned cubes of edge s overlap iff ||d||_inf <= s (face contact included)
     s = 2.0
