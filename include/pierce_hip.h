@@ -250,6 +250,22 @@ pz_status pz_join(pz_ctx* ctx, const pz_dataset* mesh1, const pz_dataset* mesh2,
                   const pz_join_desc* desc, pz_join_result* out);
 void      pz_join_result_release(pz_join_result* r);
 
+/* ------------------------------------------------------------------ batches */
+/* One dataset holding many independent sets ("parts") concatenated: part p's
+ * object ids, vertex indices and edge objects are shifted by the totals of the
+ * parts before it.  Used by pz_join_batch; no reference counterpart (the
+ * reference runs one join per process). */
+pz_status pz_dataset_upload_batch(pz_ctx* ctx, uint32_t nparts, const pz_mesh_view* views, pz_dataset** out);
+/* Batched join: query q joins part q of mesh1 with part q of mesh2 in one
+ * pass of the two-level pipeline (one object subtree per part, one set of
+ * launches for all queries).  descs[q].predicate / direction /
+ * include_overlap_pairs apply to query q; every other field is taken from
+ * descs[0] (strategy must be TWO_LEVEL).  results[q] gets query q's pairs with
+ * part-local object ids; summary (optional) gets the batch's timings and
+ * counters (no pairs). */
+pz_status pz_join_batch(pz_ctx* ctx, const pz_dataset* mesh1, const pz_dataset* mesh2, uint32_t nqueries,
+                        const pz_join_desc* descs, pz_join_result* results, pz_join_result* summary);
+
 /* Number of (edge, triangle) pairs over both directions whose closed fp32
  * boxes overlap (the BVH-independent candidate-test count). */
 pz_status pz_count_candidate_tests(pz_ctx* ctx, const pz_dataset* mesh1, const pz_dataset* mesh2,

@@ -120,6 +120,9 @@ _SIGS = [
     ("pz_join", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(JoinDesc), C.POINTER(JoinResult)]),
     ("pz_join_result_release", None, [C.POINTER(JoinResult)]),
     ("pz_count_candidate_tests", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]),
+    ("pz_dataset_upload_batch", C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(MeshView), C.POINTER(C.c_void_p)]),
+    ("pz_join_batch", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(JoinDesc),
+                                  C.POINTER(JoinResult), C.POINTER(JoinResult)]),
     ("pz_profile_enable", None, [C.c_int32]),
     ("pz_profile_reset", None, []),
     ("pz_profile_query", C.c_int32, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
@@ -327,12 +330,19 @@ class Context:
     def load_pre(self, path: str) -> "Dataset":
         return Dataset(self, None, path)
 
+    def upload_batch(self, meshes) -> "Dataset":
+        """One dataset whose part p is meshes[p] (for join_batch)."""
+        return Dataset(self, None, None, parts=list(meshes))
+
 
 class Dataset:
-    def __init__(self, ctx: Context, mesh: Optional[Mesh], path: Optional[str] = None):
+    def __init__(self, ctx: Context, mesh: Optional[Mesh], path: Optional[str] = None, parts=None):
         self.ctx = ctx
         self._h = C.c_void_p()
-        if mesh is not None:
+        if parts is not None:
+            views = (MeshView * len(parts))(*[m.view() for m in parts])
+            _check(lib().pz_dataset_upload_batch(ctx._h, len(parts), views, C.byref(self._h)))
+        elif mesh is not None:
             v = mesh.view()
             _check(lib().pz_dataset_upload(ctx._h, C.byref(v), C.byref(self._h)))
         else:
@@ -384,26 +394,52 @@ def join_desc(**kw) -> JoinDesc:
     return d
 
 
+def _output(r: JoinResult) -> JoinOutput:
+    n = int(r.num_pairs)
+    if n and r.pairs:
+        buf = (C.c_int32 * (2 * n)).from_address(r.pairs)
+        pairs = np.frombuffer(buf, np.int32).reshape(n, 2).copy()
+    else:
+        pairs = np.zeros((0, 2), np.int32)
+    types = None
+    if r.pair_types:
+        types = np.frombuffer((C.c_uint8 * max(n, 1)).from_address(r.pair_types), np.uint8)[:n].copy()
+    phases = [(r.phases[i].name.decode(), r.phases[i].ms) for i in range(r.num_phases)]
+    return JoinOutput(pairs, types, int(r.num_candidates), int(r.capacity), int(r.predicted), bool(r.overflowed),
+                      int(r.tests_total), int(r.tests_fp64), int(r.tests_exact), int(r.tests_fp32_misclassified),
+                      int(r.edges_traced), int(r.index_triangles), int(r.index_edges), float(r.ms_total), phases)
+
+
+def join_batch(ctx: Context, a: Dataset, b: Dataset, queries, **common):
+    """Batched join of part q of a with part q of b for every query q.
+
+    queries: list of dicts of per-query pz_join_desc fields (predicate, direction,
+    include_overlap_pairs); common: fields shared by the batch.  Returns
+    (list of JoinOutput with part-local ids, summary JoinOutput)."""
+    nq = len(queries)
+    descs = (JoinDesc * nq)()
+    for q, kw in enumerate(queries):
+        lib().pz_join_desc_defaults(C.byref(descs[q]))
+        for k, v in list(common.items()) + list(kw.items()):
+            setattr(descs[q], k, v)
+    res = (JoinResult * nq)()
+    summ = JoinResult()
+    _check(lib().pz_join_batch(ctx._h, a._h, b._h, nq, descs, res, C.byref(summ)))
+    try:
+        outs = [_output(res[q]) for q in range(nq)]
+        return outs, _output(summ)
+    finally:
+        for q in range(nq):
+            lib().pz_join_result_release(C.byref(res[q]))
+
+
 def join(ctx: Context, a: Dataset, b: Dataset, **kw) -> JoinOutput:
     """Run one join on the GPU; keyword arguments set pz_join_desc fields."""
     d = join_desc(**kw)
     r = JoinResult()
     _check(lib().pz_join(ctx._h, a._h, b._h, C.byref(d), C.byref(r)))
     try:
-        n = int(r.num_pairs)
-        if n:
-            buf = (C.c_int32 * (2 * n)).from_address(r.pairs)
-            pairs = np.frombuffer(buf, np.int32).reshape(n, 2).copy()
-        else:
-            pairs = np.zeros((0, 2), np.int32)
-        types = None
-        if r.pair_types:
-            types = np.frombuffer((C.c_uint8 * max(n, 1)).from_address(r.pair_types), np.uint8)[:n].copy()
-        phases = [(r.phases[i].name.decode(), r.phases[i].ms) for i in range(r.num_phases)]
-        return JoinOutput(pairs, types, int(r.num_candidates), int(r.capacity), int(r.predicted), bool(r.overflowed),
-                          int(r.tests_total), int(r.tests_fp64), int(r.tests_exact),
-                          int(r.tests_fp32_misclassified), int(r.edges_traced), int(r.index_triangles),
-                          int(r.index_edges), float(r.ms_total), phases)
+        return _output(r)
     finally:
         lib().pz_join_result_release(C.byref(r))
 

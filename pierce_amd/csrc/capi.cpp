@@ -31,6 +31,16 @@
 
 using pz::fail;
 
+namespace pz {
+// predicate | direction bits << 2 of one join description
+uint8_t pair_meta_of(const pz_join_desc& d) {
+    const int dir = d.predicate == PZ_PRED_CONTAINMENT
+                        ? 3
+                        : (d.direction == PZ_DIR_1TO2 ? 1 : (d.direction == PZ_DIR_2TO1 ? 2 : 3));
+    return (uint8_t)(d.predicate | (dir << 2));
+}
+}  // namespace pz
+
 // ------------------------------------------------------------------ context
 struct DevBuf {
     void* p = nullptr;
@@ -59,6 +69,10 @@ struct pz_dataset {
     float max_abs = 0.f;
     std::vector<pz_grid_cell> grid;
     float cell_size = 0.f;
+    // batched datasets: part p owns objects [part_obj_off[p], part_obj_off[p + 1])
+    std::vector<uint64_t> part_obj_off;
+    uint32_t* part_of = nullptr;   // device: part id per object
+    uint32_t* part_off = nullptr;  // device: part_obj_off as uint32
 };
 
 namespace {
@@ -173,6 +187,9 @@ struct ObjBvh {
 };
 
 ObjBvh build_object_bvh(pz_ctx* c, const pz_dataset* d, const SetPrep& p, const std::string& tag) {
+    // Batched sets: one subtree per part (keys carry the part above the Morton bits).
+    const bool parts = !d->part_obj_off.empty();
+    const uint64_t nparts = parts ? d->part_obj_off.size() - 1 : 1;
     const uint64_t n = std::max<int32_t>(d->nobj, 0);
     ObjBvh b;
     uint32_t* scene = buf<uint32_t>(c, tag + ".scene", 8);
@@ -187,13 +204,14 @@ ObjBvh build_object_bvh(pz_ctx* c, const pz_dataset* d, const SetPrep& p, const 
     b.height = buf<uint32_t>(c, tag + ".oheight", 1);
     void* tmp = buf<char>(c, "sort.tmp.obj", pzk::radix_sort_temp_bytes(n));
     pzk::scene_box(p.obox, d->nobj, scene, c->s);
-    pzk::object_keys(p.obox, d->nobj, scene, keys, vals, c->s);
-    bool alt = pzk::radix_sort_pairs(keys, vals, keys2, vals2, n, 0, 30, tmp, c->s);
+    pzk::object_keys(p.obox, d->nobj, scene, parts ? d->part_of : nullptr, keys, vals, c->s);
+    bool alt = pzk::radix_sort_pairs(keys, vals, keys2, vals2, n, 0, 30 + (parts ? bits_for(nparts) : 0), tmp, c->s);
     uint64_t* sk = alt ? keys2 : keys;
     b.order = alt ? vals2 : vals;
     pzk::fill_u32(b.height, 0u, 1, c->s);
-    pzk::lbvh_karras(sk, n, nullptr, 0, b.nodes, pint, pleaf, cnt, c->s);
-    pzk::lbvh_refit_ord_boxes(sk, b.order, n, nullptr, 0, p.obox, b.nodes, pint, pleaf, cnt, b.height, c->s);
+    const uint32_t* seg = parts ? d->part_off : nullptr;
+    pzk::lbvh_karras(sk, n, seg, 30, b.nodes, pint, pleaf, cnt, c->s);
+    pzk::lbvh_refit_ord_boxes(sk, b.order, n, seg, 30, p.obox, b.nodes, pint, pleaf, cnt, b.height, c->s);
     return b;
 }
 
@@ -305,8 +323,10 @@ double grid_estimate(const pz_dataset* a, const pz_dataset* b, float gamma, floa
 }
 
 // ------------------------------------------------------------- two-level
+// meta_of_part: device predicate | dir << 2 per part for batched sets (null: from d).
 void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const pz_join_desc& d, pz_join_result* r,
-                    Phases& ph) {
+                    Phases& ph, const uint8_t* meta_of_part = nullptr) {
+    const bool batched = meta_of_part != nullptr;
     hipStream_t s = c->s;
     unsigned long long* stats = buf<unsigned long long>(c, "stats", 8);
     pzk::fill_ull(stats, 0ull, 8, s);
@@ -329,11 +349,13 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         uint32_t* cnt = buf<uint32_t>(c, "bp.cnt", (size_t)std::max(nA, 1));
         uint64_t* off = buf<uint64_t>(c, "bp.off", (size_t)std::max(nA, 1) + 1);
         void* tmp = buf<char>(c, "scan.tmp", pzk::scan_temp_bytes((uint64_t)std::max(nA, 1)));
-        pzk::broad_count(pa.obox, nA, pb.obox, ob.nodes, ob.order, nB, cnt, s);
+        pzk::broad_count(pa.obox, nA, pb.obox, ob.nodes, ob.order, nB, batched ? A->part_of : nullptr,
+                         batched ? B->part_off : nullptr, cnt, s);
         pzk::exclusive_scan_u32_u64(cnt, off, (uint64_t)std::max(nA, 0), tmp, s);
         np = readback<uint64_t>(c, {off + std::max(nA, 0)})[0];
         pairs = buf<uint64_t>(c, "bp.pairs", np + 1);
-        pzk::broad_write(pa.obox, nA, pb.obox, ob.nodes, ob.order, nB, off, pairs, s);
+        pzk::broad_write(pa.obox, nA, pb.obox, ob.nodes, ob.order, nB, batched ? A->part_of : nullptr,
+                         batched ? B->part_off : nullptr, off, pairs, s);
     };
     if (d.sizing == PZ_SIZING_TWO_PASS) {
         two_pass();
@@ -350,7 +372,8 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         pairs = buf<uint64_t>(c, "bp.pairs", cap + 1);
         unsigned long long* cursor = buf<unsigned long long>(c, "bp.cursor", 1);
         pzk::fill_ull(cursor, 0ull, 1, s);
-        pzk::broad_append(pa.obox, nA, pb.obox, ob.nodes, ob.order, nB, pairs, cursor, cap, s);
+        pzk::broad_append(pa.obox, nA, pb.obox, ob.nodes, ob.order, nB, batched ? A->part_of : nullptr,
+                          batched ? B->part_off : nullptr, pairs, cursor, cap, s);
         np = readback<unsigned long long>(c, {cursor})[0];
         if (np > cap) {
             r->overflowed = 1;
@@ -361,16 +384,16 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
     if (np == 0) {  // no boxes meet: nothing to index or test
         ph.begin("Download Results");
         uint64_t* none = buf<uint64_t>(c, "sel.keys", 1);
-        finish_pairs(c, none, nullptr, 0, nA, d.predicate == PZ_PRED_CONTAINMENT && d.include_overlap_pairs, r);
+        finish_pairs(c, none, nullptr, 0, nA,
+                     batched || (d.predicate == PZ_PRED_CONTAINMENT && d.include_overlap_pairs), r);
         return;
     }
 
     ph.begin("Build Mesh Index");
-    // bit 0: mesh1 edges vs mesh2 triangles (+ mesh1 points in mesh2 objects); bit 1: the reverse.
-    // Containment always needs both edge directions.
-    const int dir_mask = d.predicate == PZ_PRED_CONTAINMENT
-                             ? 3
-                             : (d.direction == PZ_DIR_1TO2 ? 1 : (d.direction == PZ_DIR_2TO1 ? 2 : 3));
+    // Per-pair meta = predicate | direction bits << 2 (bit 0: mesh1 edges vs mesh2
+    // triangles and mesh1 points in mesh2 objects; bit 1: the reverse).
+    uint8_t* meta = buf<uint8_t>(c, "np.meta", np + 1);
+    pzk::pair_meta(pairs, np, batched ? A->part_of : nullptr, meta_of_part, pz::pair_meta_of(d), meta, s);
     Needed na = needed_layout(c, A, pa, "A"), nb = needed_layout(c, B, pb, "B");
     const size_t sA = std::max(nA, 1), sB = std::max(nB, 1);
     pzk::fill_u32(na.need, 0u, sA, s);
@@ -464,7 +487,7 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
     const uint64_t* fb_pairs = pairs;
     uint32_t* fb_index = nullptr;
     if (d.narrow_mode == 0) {
-        pzk::narrow_dual(pairs, np, va, vb, found, fallback, dir_mask, f32, allow_f32, stats, s);
+        pzk::narrow_dual(pairs, np, va, vb, found, fallback, meta, f32, allow_f32, stats, s);
         uint64_t* fbp = buf<uint64_t>(c, "np.fb_pairs", np + 1);
         fb_index = buf<uint32_t>(c, "np.fb_index", np + 1);
         unsigned long long* fbc = buf<unsigned long long>(c, "np.fb_count", 1);
@@ -505,22 +528,28 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         uint32_t* tcnt = buf<uint32_t>(c, "np.tcnt", nfb + 1);
         uint64_t* toff = buf<uint64_t>(c, "np.toff", nfb + 2);
         void* tmp = buf<char>(c, "scan.tmp.np", pzk::scan_temp_bytes(nfb + 1));
-        pzk::task_counts(fb_pairs, nfb, na.loc, na.eseg_off, nb.loc, nb.eseg_off, dir_mask, tcnt, s);
+        uint8_t* fb_meta = meta;
+        if (fb_index) {  // meta of the compacted fallback pairs
+            fb_meta = buf<uint8_t>(c, "np.fb_meta", nfb + 1);
+            pzk::pair_meta(fb_pairs, nfb, batched ? A->part_of : nullptr, meta_of_part, pz::pair_meta_of(d), fb_meta,
+                           s);
+        }
+        pzk::task_counts(fb_pairs, nfb, na.loc, na.eseg_off, nb.loc, nb.eseg_off, fb_meta, tcnt, s);
         pzk::exclusive_scan_u32_u64(tcnt, toff, nfb, tmp, s);
         const uint64_t ntasks = readback<uint64_t>(c, {toff + nfb})[0];
         uint32_t* sub_found = fb_index ? buf<uint32_t>(c, "np.sub_found", nfb + 1) : found;
         if (fb_index) pzk::fill_u32(sub_found, 0u, nfb + 1, s);
-        pzk::narrow(fb_pairs, nfb, toff, ntasks, va, vb, sub_found, dir_mask, f32, allow_f32, height, stats, s);
+        pzk::narrow(fb_pairs, nfb, toff, ntasks, va, vb, sub_found, fb_meta, f32, allow_f32, height, stats, s);
         if (fb_index) pzk::scatter_found(sub_found, fb_index, nfb, found, s);
     }
 
     uint32_t* inside = nullptr;
-    if (d.predicate != PZ_PRED_OVERLAP) {
+    if (d.predicate != PZ_PRED_OVERLAP || batched) {
         ph.begin("Containment Parity");
         inside = buf<uint32_t>(c, "np.inside", np + 1);
         pzk::fill_u32(inside, 0u, np + 1, s);
-        pzk::parity(pairs, np, found, va, vb, A->v, A->idx, pa.first_tri, B->v, B->idx, pb.first_tri,
-                    d.predicate == PZ_PRED_INTERSECTION ? 1 : 0, dir_mask, inside, height, stats, s);
+        pzk::parity(pairs, np, found, va, vb, A->v, A->idx, pa.first_tri, B->v, B->idx, pb.first_tri, meta, inside,
+                    height, stats, s);
     }
 
     ph.begin("Download Results");
@@ -528,9 +557,10 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
     uint32_t* otype = buf<uint32_t>(c, "sel.type", np + 1);
     unsigned long long* ocnt = buf<unsigned long long>(c, "sel.cnt", 1);
     pzk::fill_ull(ocnt, 0ull, 1, s);
-    pzk::select_pairs(pairs, np, found, inside, d.predicate, d.include_overlap_pairs, okeys, otype, ocnt, s);
+    // batches keep overlap pairs of containment queries (typed) and filter per query on the host
+    pzk::select_pairs(pairs, np, found, inside, meta, batched ? 1 : d.include_overlap_pairs, okeys, otype, ocnt, s);
     uint64_t nout = readback<unsigned long long>(c, {ocnt})[0];
-    const bool want_types = d.predicate == PZ_PRED_CONTAINMENT && d.include_overlap_pairs;
+    const bool want_types = batched || (d.predicate == PZ_PRED_CONTAINMENT && d.include_overlap_pairs);
     finish_pairs(c, okeys, want_types ? otype : nullptr, nout, nA, want_types, r);
     auto st = readback<unsigned long long>(c, {stats, stats + 1, stats + 2, stats + 3, stats + 4});
     r->tests_total = st[0];
@@ -759,7 +789,15 @@ void pz_ctx_destroy(pz_ctx* c) {
     delete c;
 }
 
+pz_status pz_dataset_upload_parts(pz_ctx* c, const pz_mesh_view* v, const std::vector<uint64_t>* part_obj_off,
+                                  pz_dataset** out);
+
 pz_status pz_dataset_upload(pz_ctx* c, const pz_mesh_view* v, pz_dataset** out) {
+    return pz_dataset_upload_parts(c, v, nullptr, out);
+}
+
+pz_status pz_dataset_upload_parts(pz_ctx* c, const pz_mesh_view* v, const std::vector<uint64_t>* part_obj_off,
+                                  pz_dataset** out) {
     if (!c || !v || !out) return fail(PZ_E_INVALID, "pz_dataset_upload: null argument");
     if (v->num_edges == 0 || !v->edge_starts || !v->edge_ends || !v->edge_objects)
         return fail(PZ_E_NO_EDGES,
@@ -803,6 +841,24 @@ pz_status pz_dataset_upload(pz_ctx* c, const pz_mesh_view* v, pz_dataset** out) 
         }
         if (v->grid_cells && v->num_grid_cells) d->grid.assign(v->grid_cells, v->grid_cells + v->num_grid_cells);
         d->cell_size = v->grid_cell_size;
+        if (part_obj_off) {
+            d->part_obj_off = *part_obj_off;
+            if (d->part_obj_off.back() != (uint64_t)std::max(d->nobj, 0))
+                d->nobj = (int32_t)d->part_obj_off.back();  // trailing empty parts own no objects
+            std::vector<uint32_t> part_of((size_t)std::max(d->nobj, 1), 0u), off(d->part_obj_off.size());
+            for (size_t p = 0; p + 1 < d->part_obj_off.size(); ++p)
+                for (uint64_t o = d->part_obj_off[p]; o < d->part_obj_off[p + 1]; ++o) part_of[o] = (uint32_t)p;
+            for (size_t p = 0; p < off.size(); ++p) off[p] = (uint32_t)d->part_obj_off[p];
+            try {
+                HIP_OK(hipMalloc((void**)&d->part_of, part_of.size() * 4));
+                HIP_OK(hipMemcpy(d->part_of, part_of.data(), part_of.size() * 4, hipMemcpyHostToDevice));
+                HIP_OK(hipMalloc((void**)&d->part_off, off.size() * 4));
+                HIP_OK(hipMemcpy(d->part_off, off.data(), off.size() * 4, hipMemcpyHostToDevice));
+            } catch (...) {
+                pz_dataset_release(d.release());
+                throw;
+            }
+        }
         *out = d.release();
         return PZ_OK;
     });
@@ -820,7 +876,8 @@ pz_status pz_dataset_load_pre(pz_ctx* c, const char* path, pz_dataset** out) {
 
 void pz_dataset_release(pz_dataset* d) {
     if (!d) return;
-    for (void* p : {(void*)d->v, (void*)d->idx, (void*)d->t2o, (void*)d->es, (void*)d->ee, (void*)d->eo})
+    for (void* p : {(void*)d->v, (void*)d->idx, (void*)d->t2o, (void*)d->es, (void*)d->ee, (void*)d->eo,
+                    (void*)d->part_of, (void*)d->part_off})
         if (p) (void)hipFree(p);
     delete d;
 }
@@ -912,6 +969,125 @@ pz_status pz_count_candidate_tests(pz_ctx* c, const pz_dataset* a, const pz_data
                           nullptr, nullptr, nullptr, 0, overflow, stats, s);
         pzk::exclusive_scan_u32_u64(cnt, off, n1 + n2, tmp, s);
         *out = readback<uint64_t>(c, {off + n1 + n2})[0];
+        return PZ_OK;
+    });
+}
+
+pz_status pz_dataset_upload_batch(pz_ctx* c, uint32_t nparts, const pz_mesh_view* views, pz_dataset** out) {
+    if (!c || !views || !out || nparts == 0) return fail(PZ_E_INVALID, "pz_dataset_upload_batch: bad argument");
+    return guarded([&]() -> pz_status {
+        // Concatenate: vertex indices, object ids and edge object ids are shifted
+        // by the running vertex / object totals; part p owns objects [off[p], off[p+1]).
+        pz_host_mesh m;
+        std::vector<uint64_t> part_off(nparts + 1, 0);
+        uint64_t nv = 0, nt = 0, ne = 0;
+        for (uint32_t p = 0; p < nparts; ++p) {
+            nv += views[p].num_vertices;
+            nt += views[p].num_triangles;
+            ne += views[p].num_edges;
+        }
+        if (nv >= (1ull << 32)) throw PzError(PZ_E_LIMIT, "more than 2^32 vertices in one batch side");
+        m.vertices.reserve(3 * nv);
+        m.indices.reserve(3 * nt);
+        m.tri_to_object.reserve(nt);
+        m.edge_starts.reserve(3 * ne);
+        m.edge_ends.reserve(3 * ne);
+        m.edge_objects.reserve(ne);
+        uint64_t vbase = 0, obase = 0;
+        for (uint32_t p = 0; p < nparts; ++p) {
+            const pz_mesh_view& v = views[p];
+            if (v.num_edges == 0) throw PzError(PZ_E_NO_EDGES, "batch part " + std::to_string(p) + " has no edges");
+            int32_t mx = -1;
+            for (uint64_t t = 0; t < v.num_triangles; ++t) mx = std::max(mx, v.triangle_to_object[t]);
+            for (uint64_t e = 0; e < v.num_edges; ++e) mx = std::max(mx, v.edge_objects[e]);
+            m.vertices.insert(m.vertices.end(), v.vertices, v.vertices + 3 * v.num_vertices);
+            for (uint64_t i = 0; i < 3 * v.num_triangles; ++i) {
+                if (v.indices[i] >= v.num_vertices) throw PzError(PZ_E_INVALID, "triangle index beyond the vertex array");
+                m.indices.push_back((uint32_t)(v.indices[i] + vbase));
+            }
+            for (uint64_t t = 0; t < v.num_triangles; ++t)
+                m.tri_to_object.push_back(v.triangle_to_object[t] < 0 ? -1 : (int32_t)(v.triangle_to_object[t] + obase));
+            m.edge_starts.insert(m.edge_starts.end(), v.edge_starts, v.edge_starts + 3 * v.num_edges);
+            m.edge_ends.insert(m.edge_ends.end(), v.edge_ends, v.edge_ends + 3 * v.num_edges);
+            for (uint64_t e = 0; e < v.num_edges; ++e)
+                m.edge_objects.push_back(v.edge_objects[e] < 0 ? -1 : (int32_t)(v.edge_objects[e] + obase));
+            vbase += v.num_vertices;
+            obase += (uint64_t)(mx + 1);
+            part_off[p + 1] = obase;
+        }
+        if (obase >= (1ull << 31)) throw PzError(PZ_E_LIMIT, "more than 2^31 objects in one batch side");
+        pz_mesh_view cv = m.view();
+        pz_status st = pz_dataset_upload_parts(c, &cv, &part_off, out);
+        if (st != PZ_OK) throw PzError(st, pz_last_error());
+        return PZ_OK;
+    });
+}
+
+pz_status pz_join_batch(pz_ctx* c, const pz_dataset* a, const pz_dataset* b, uint32_t nq, const pz_join_desc* descs,
+                        pz_join_result* results, pz_join_result* summary) {
+    if (!c || !a || !b || !descs || !results || nq == 0) return fail(PZ_E_INVALID, "pz_join_batch: bad argument");
+    if (a->part_obj_off.size() != (size_t)nq + 1 || b->part_obj_off.size() != (size_t)nq + 1)
+        return fail(PZ_E_INVALID, "pz_join_batch: both datasets must come from pz_dataset_upload_batch with nq parts");
+    for (uint32_t q = 0; q < nq; ++q) {
+        if (descs[q].predicate < 0 || descs[q].predicate > 2) return fail(PZ_E_INVALID, "unknown predicate");
+        init_result(&results[q]);
+    }
+    if (descs[0].strategy != PZ_STRATEGY_TWO_LEVEL)
+        return fail(PZ_E_INVALID, "batched joins use the two-level strategy");
+    return guarded([&]() -> pz_status {
+        HIP_OK(hipSetDevice(c->device));
+        pz_join_result all;
+        init_result(&all);
+        double t0 = now_ms();
+        Phases ph(c, &all);
+        pz_join_desc d = descs[0];
+        std::vector<uint8_t> hmeta(nq);
+        for (uint32_t q = 0; q < nq; ++q) hmeta[q] = pz::pair_meta_of(descs[q]);
+        uint8_t* dmeta = buf<uint8_t>(c, "batch.meta", nq);
+        HIP_OK(hipMemcpyAsync(dmeta, hmeta.data(), nq, hipMemcpyHostToDevice, c->s));
+        if (d.sizing == PZ_SIZING_LEARNED) {
+            ph.begin("Selectivity Estimation");
+            all.predicted = pz::mlp_predict_batch_total(c, d.model, a, b);
+        } else if (d.sizing == PZ_SIZING_ESTIMATE) {
+            d.sizing = PZ_SIZING_TWO_PASS;  // the grid estimate is per dataset, not per part
+        }
+        join_two_level(c, a, b, d, &all, ph, dmeta);
+        ph.finish();
+        all.ms_total = now_ms() - t0;
+        // split by part: pairs are sorted by (mesh1, mesh2) global ids, so parts come in order
+        std::vector<std::vector<pz_pair>> per(nq);
+        std::vector<std::vector<uint8_t>> pert(nq);
+        for (uint64_t i = 0; i < all.num_pairs; ++i) {
+            const pz_pair g = all.pairs[i];
+            const uint64_t ga = (uint64_t)g.object_id_mesh1, gb = (uint64_t)g.object_id_mesh2;
+            const uint32_t q = (uint32_t)(std::upper_bound(a->part_obj_off.begin(), a->part_obj_off.end(), ga) -
+                                          a->part_obj_off.begin() - 1);
+            const uint8_t type = all.pair_types ? all.pair_types[i] : 0;
+            const bool want_overlap = descs[q].predicate == PZ_PRED_CONTAINMENT && descs[q].include_overlap_pairs;
+            if (type == 1 && !want_overlap) continue;
+            per[q].push_back({(int32_t)(ga - a->part_obj_off[q]), (int32_t)(gb - b->part_obj_off[q])});
+            pert[q].push_back(type);
+        }
+        for (uint32_t q = 0; q < nq; ++q) {
+            pz_join_result& r = results[q];
+            r.num_pairs = per[q].size();
+            r.pairs = static_cast<pz_pair*>(std::malloc(std::max<size_t>(per[q].size(), 1) * sizeof(pz_pair)));
+            if (!r.pairs) throw PzError(PZ_E_NOMEM, "host result allocation failed");
+            std::copy(per[q].begin(), per[q].end(), r.pairs);
+            if (descs[q].predicate == PZ_PRED_CONTAINMENT && descs[q].include_overlap_pairs) {
+                r.pair_types = static_cast<uint8_t*>(std::malloc(std::max<size_t>(pert[q].size(), 1)));
+                if (!r.pair_types) throw PzError(PZ_E_NOMEM, "host result allocation failed");
+                std::copy(pert[q].begin(), pert[q].end(), r.pair_types);
+            }
+            r.ms_total = all.ms_total;
+        }
+        if (summary) {
+            *summary = all;
+            summary->pairs = nullptr;
+            summary->pair_types = nullptr;
+        }
+        std::free(all.pairs);
+        std::free(all.pair_types);
         return PZ_OK;
     });
 }

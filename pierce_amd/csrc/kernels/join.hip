@@ -142,10 +142,11 @@ __device__ __forceinline__ void load_scene(const uint32_t* scene, float* lo, flo
     }
 }
 
-// Object keys: Morton code of the box centre inside the scene box.
+// Object keys: (part << 30) | Morton code of the box centre inside the scene box.
 __global__ __launch_bounds__(kBlock) void k_object_keys(const uint32_t* __restrict__ obox, int32_t nobj,
-                                                        const uint32_t* __restrict__ scene, uint64_t* __restrict__ keys,
-                                                        uint32_t* __restrict__ vals) {
+                                                        const uint32_t* __restrict__ scene,
+                                                        const uint32_t* __restrict__ part_of,
+                                                        uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     int32_t o = blockIdx.x * kBlock + threadIdx.x;
     if (o >= nobj) return;
     float lo[3], inv[3];
@@ -157,7 +158,8 @@ __global__ __launch_bounds__(kBlock) void k_object_keys(const uint32_t* __restri
         empty |= !(l <= h);
         c[a] = 0.5f * l + 0.5f * h;
     }
-    keys[o] = empty ? 0ull : morton30(c[0], c[1], c[2], lo[0], lo[1], lo[2], inv[0], inv[1], inv[2]);
+    const uint64_t part = part_of ? (uint64_t)part_of[o] : 0ull;
+    keys[o] = (part << 30) | (empty ? 0ull : morton30(c[0], c[1], c[2], lo[0], lo[1], lo[2], inv[0], inv[1], inv[2]));
     vals[o] = (uint32_t)o;
 }
 
@@ -169,11 +171,20 @@ template <int Mode>
 __global__ __launch_bounds__(kBlock) void k_broad(const uint32_t* __restrict__ oboxA, int32_t nA,
                                                   const uint32_t* __restrict__ oboxB,
                                                   const Node* __restrict__ nodesB, const uint32_t* __restrict__ orderB,
-                                                  int32_t nB, uint32_t* __restrict__ cnt,
+                                                  int32_t nB, const uint32_t* __restrict__ part_of_a,
+                                                  const uint32_t* __restrict__ boff, uint32_t* __restrict__ cnt,
                                                   const uint64_t* __restrict__ off, uint64_t* __restrict__ out,
                                                   unsigned long long* __restrict__ cursor, uint64_t cap) {
     int32_t a = blockIdx.x * kBlock + threadIdx.x;
-    if (a >= nA || nB <= 0) return;
+    if (a >= nA) return;
+    // mesh2 objects of this object's part occupy sorted positions [b0, b0 + nb)
+    const uint32_t part = part_of_a ? part_of_a[a] : 0u;
+    const int32_t b0 = boff ? (int32_t)boff[part] : 0;
+    const int32_t nb = boff ? (int32_t)(boff[part + 1] - boff[part]) : nB;
+    if (nb <= 0) {
+        if (Mode == 0) cnt[a] = 0;
+        return;
+    }
     float lo[3], hi[3];
     for (int k = 0; k < 3; ++k) {
         lo[k] = ord2f(oboxA[6ll * a + k]);
@@ -193,15 +204,16 @@ __global__ __launch_bounds__(kBlock) void k_broad(const uint32_t* __restrict__ o
         }
         ++n;
     };
-    if (nB == 1) {
+    if (nb == 1) {
         // single object: no internal node; test its box directly
-        if (boxes_meet(lo, hi, ord2f(oboxB[0]), ord2f(oboxB[1]), ord2f(oboxB[2]), ord2f(oboxB[3]), ord2f(oboxB[4]),
-                       ord2f(oboxB[5])))
-            emit(0);
+        const int32_t b = (int32_t)orderB[b0];
+        const uint32_t* bb = oboxB + 6ll * b;
+        if (boxes_meet(lo, hi, ord2f(bb[0]), ord2f(bb[1]), ord2f(bb[2]), ord2f(bb[3]), ord2f(bb[4]), ord2f(bb[5])))
+            emit(b);
     } else {
         int stack[64];
         int sp = 0;
-        int cur = 0;
+        int cur = b0;
         while (true) {
             const Node nd = nodesB[cur];
             bool h0 = boxes_meet(lo, hi, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
@@ -229,6 +241,16 @@ __global__ __launch_bounds__(kBlock) void k_broad(const uint32_t* __restrict__ o
         }
     }
     if (Mode == 0) cnt[a] = n;
+}
+
+// Per-pair predicate and direction mask: meta = predicate | dir_mask << 2
+// (predicate 0 overlap, 1 intersection, 2 containment; dir bit 0: mesh1 edges
+// and points, bit 1: mesh2's).  Batches take them from the pair's part.
+__global__ void k_pair_meta(const uint64_t* __restrict__ pairs, uint64_t np, const uint32_t* __restrict__ part_of_a,
+                            const uint8_t* __restrict__ meta_of_part, uint8_t uniform, uint8_t* __restrict__ meta) {
+    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= np) return;
+    meta[i] = part_of_a ? meta_of_part[part_of_a[(uint32_t)(pairs[i] >> 32)]] : uniform;
 }
 
 // ------------------------------------------------- needed objects / layout
@@ -380,8 +402,8 @@ constexpr int kChunk = 256;
 template <int STACK, bool F32>
 __global__ __launch_bounds__(kBlock, 4) void k_narrow(const uint64_t* __restrict__ pairs, uint64_t np,
                                                    const uint64_t* __restrict__ task_off, SetView A, SetView B,
-                                                   uint32_t* __restrict__ found, int dir_mask, int allow_f32,
-                                                   unsigned long long* __restrict__ stats) {
+                                                   uint32_t* __restrict__ found, const uint8_t* __restrict__ meta,
+                                                   int allow_f32, unsigned long long* __restrict__ stats) {
     __shared__ int stk[kWavesPerBlock * STACK * PZ_WAVE];
     const int lane = __lane_id(), w = threadIdx.x / PZ_WAVE;
     int* my = stk + (w * STACK) * PZ_WAVE + lane;
@@ -399,6 +421,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_narrow(const uint64_t* __restrict
     }
     const uint64_t p = lo;
     if (found[p]) return;
+    const int dir_mask = meta[p] >> 2;
     const uint64_t key = pairs[p];
     const int32_t oa = (int32_t)(key >> 32), ob = (int32_t)(uint32_t)key;
     const uint32_t la = A.loc[oa], lb = B.loc[ob];
@@ -567,12 +590,14 @@ __device__ __forceinline__ bool tri_pair_test(const LeafTri& X, const LeafTri& Y
 template <bool F32>
 __global__ __launch_bounds__(kDualWaves * PZ_WAVE, 2) void k_narrow_dual(
     const uint64_t* __restrict__ pairs, uint64_t np, SetView A, SetView B, uint32_t* __restrict__ found,
-    uint32_t* __restrict__ fallback, int dir_mask, int allow_f32, unsigned long long* __restrict__ stats) {
+    uint32_t* __restrict__ fallback, const uint8_t* __restrict__ meta, int allow_f32,
+    unsigned long long* __restrict__ stats) {
     __shared__ int2 q[kDualWaves][kDualQ];
     const int lane = __lane_id(), w = threadIdx.x / PZ_WAVE;
     const uint64_t p = (uint64_t)blockIdx.x * kDualWaves + w;
     if (p >= np) return;
     const uint64_t key = pairs[p];
+    const int dir_mask = meta[p] >> 2;
     const int32_t oa = (int32_t)(key >> 32), ob = (int32_t)(uint32_t)key;
     const uint32_t la = A.loc[oa], lb = B.loc[ob];
     const uint32_t a0 = A.seg_off[la], a1 = A.seg_off[la + 1], b0 = B.seg_off[lb], b1 = B.seg_off[lb + 1];
@@ -676,17 +701,21 @@ __global__ __launch_bounds__(kBlock, 4) void k_parity(const uint64_t* __restrict
                                                    const float* __restrict__ vA, const uint32_t* __restrict__ iA,
                                                    const uint32_t* __restrict__ ftA, const float* __restrict__ vB,
                                                    const uint32_t* __restrict__ iB, const uint32_t* __restrict__ ftB,
-                                                   int kind, int dir_mask, uint32_t* __restrict__ inside,
+                                                   const uint8_t* __restrict__ meta, uint32_t* __restrict__ inside,
                                                    unsigned long long* __restrict__ stats) {
     __shared__ int stk[kWavesPerBlock * STACK * PZ_WAVE];
     const int lane = __lane_id(), w = threadIdx.x / PZ_WAVE;
     int* my = stk + (w * STACK) * PZ_WAVE + lane;
     const uint64_t gid = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    const int ndir = kind == 1 ? 2 : 1;
-    const uint64_t p = gid / ndir;
-    const int dir = (int)(gid % ndir);  // 0: point of B in A; 1: point of A in B
-    // dir 0 (mesh2 point in mesh1 object) belongs to the mesh2->mesh1 direction, dir 1 to mesh1->mesh2
-    bool act = p < np && !found[p] && (kind == 0 || (dir_mask & (dir == 0 ? 2 : 1)));
+    const uint64_t p = gid >> 1;
+    const int dir = (int)(gid & 1);  // 0: point of B in A; 1: point of A in B
+    // containment: dir 0 only; intersection: dir 0 belongs to the mesh2->mesh1
+    // direction (bit 1), dir 1 to mesh1->mesh2 (bit 0); overlap: none
+    bool act = false;
+    if (p < np && !found[p]) {
+        const int pred = meta[p] & 3, dm = meta[p] >> 2;
+        act = (pred == 2 && dir == 0) || (pred == 1 && (dm & (dir == 0 ? 2 : 1)));
+    }
     LaneStats st{0, 0, 0, 0, 0};
     float3 pt = make_float3(0, 0, 0);
     int cur = 0, sp = 0;
@@ -751,7 +780,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_parity(const uint64_t* __restrict
 // sel: 0 drop; else keep with type value (sel - 1) in vals.
 __global__ __launch_bounds__(kBlock) void k_select_pairs(const uint64_t* __restrict__ pairs, uint64_t np,
                                                          const uint32_t* __restrict__ found,
-                                                         const uint32_t* __restrict__ inside, int predicate,
+                                                         const uint32_t* __restrict__ inside,
+                                                         const uint8_t* __restrict__ meta,
                                                          int include_overlap, uint64_t* __restrict__ out_keys,
                                                          uint32_t* __restrict__ out_type,
                                                          unsigned long long* __restrict__ count) {
@@ -759,6 +789,7 @@ __global__ __launch_bounds__(kBlock) void k_select_pairs(const uint64_t* __restr
     uint32_t type = 0;
     bool keep = false;
     if (i < np) {
+        const int predicate = meta[i] & 3;
         bool ov = found[i] != 0;
         uint32_t in = inside ? inside[i] : 0u;
         if (predicate == 0) {
@@ -790,9 +821,11 @@ __global__ __launch_bounds__(kBlock) void k_select_pairs(const uint64_t* __restr
 // Tasks per pair: chunks of source edges in the enabled directions.
 __global__ void k_task_counts(const uint64_t* __restrict__ pairs, uint64_t np, const uint32_t* __restrict__ locA,
                               const uint32_t* __restrict__ esegA, const uint32_t* __restrict__ locB,
-                              const uint32_t* __restrict__ esegB, int dir_mask, uint32_t* __restrict__ counts) {
+                              const uint32_t* __restrict__ esegB, const uint8_t* __restrict__ meta,
+                              uint32_t* __restrict__ counts) {
     uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= np) return;
+    const int dir_mask = meta[i] >> 2;
     uint64_t key = pairs[i];
     uint32_t la = locA[(uint32_t)(key >> 32)], lb = locB[(uint32_t)key];
     uint32_t ea = esegA[la + 1] - esegA[la], eb = esegB[lb + 1] - esegB[lb];
@@ -971,10 +1004,10 @@ __global__ void k_init_boxes(uint32_t* obox, int64_t n) {
 
 template <int STACK, bool F32>
 void launch_narrow(unsigned g, hipStream_t s, const uint64_t* pairs, uint64_t np, const uint64_t* task_off, SetView A,
-                   SetView B, uint32_t* found, int dir_mask, int allow_f32, unsigned long long* stats) {
+                   SetView B, uint32_t* found, const uint8_t* meta, int allow_f32, unsigned long long* stats) {
     {
         pzk::KernelTimer kt_("narrow", s);
-        hipLaunchKernelGGL((k_narrow<STACK, F32>), dim3(g), dim3(kBlock), 0, s, pairs, np, task_off, A, B, found, dir_mask,
+        hipLaunchKernelGGL((k_narrow<STACK, F32>), dim3(g), dim3(kBlock), 0, s, pairs, np, task_off, A, B, found, meta,
                            allow_f32, stats);
     }
 }
@@ -1033,32 +1066,33 @@ void scene_box(const uint32_t* obox, int32_t nobj, uint32_t* scene, hipStream_t 
     if (nobj > 0) hipLaunchKernelGGL(k_scene_box, dim3(grid1((uint64_t)nobj)), dim3(kBlock), 0, s, obox, nobj, scene);
 }
 
-void object_keys(const uint32_t* obox, int32_t nobj, const uint32_t* scene, uint64_t* keys, uint32_t* vals,
-                 hipStream_t s) {
+void object_keys(const uint32_t* obox, int32_t nobj, const uint32_t* scene, const uint32_t* part_of, uint64_t* keys,
+                 uint32_t* vals, hipStream_t s) {
     if (nobj > 0)
-        hipLaunchKernelGGL(k_object_keys, dim3(grid1((uint64_t)nobj)), dim3(kBlock), 0, s, obox, nobj, scene, keys,
-                           vals);
+        hipLaunchKernelGGL(k_object_keys, dim3(grid1((uint64_t)nobj)), dim3(kBlock), 0, s, obox, nobj, scene, part_of,
+                           keys, vals);
 }
 
 void broad_count(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const Node* nodesB, const uint32_t* orderB,
-                 int32_t nB, uint32_t* cnt, hipStream_t s) {
+                 int32_t nB, const uint32_t* part_of_a, const uint32_t* boff, uint32_t* cnt, hipStream_t s) {
     if (nA > 0)
         hipLaunchKernelGGL((k_broad<0>), dim3(grid1((uint64_t)nA)), dim3(kBlock), 0, s, oboxA, nA, oboxB, nodesB, orderB, nB,
-                           cnt, (const uint64_t*)nullptr, (uint64_t*)nullptr, (unsigned long long*)nullptr,
+                           part_of_a, boff, cnt, (const uint64_t*)nullptr, (uint64_t*)nullptr, (unsigned long long*)nullptr,
                            (uint64_t)0);
 }
 void broad_write(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const Node* nodesB, const uint32_t* orderB,
-                 int32_t nB, const uint64_t* off, uint64_t* out, hipStream_t s) {
+                 int32_t nB, const uint32_t* part_of_a, const uint32_t* boff, const uint64_t* off, uint64_t* out,
+                 hipStream_t s) {
     if (nA > 0)
         hipLaunchKernelGGL((k_broad<1>), dim3(grid1((uint64_t)nA)), dim3(kBlock), 0, s, oboxA, nA, oboxB, nodesB, orderB, nB,
-                           (uint32_t*)nullptr, off, out, (unsigned long long*)nullptr, (uint64_t)0);
+                           part_of_a, boff, (uint32_t*)nullptr, off, out, (unsigned long long*)nullptr, (uint64_t)0);
 }
 void broad_append(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const Node* nodesB,
-                  const uint32_t* orderB, int32_t nB, uint64_t* out, unsigned long long* cursor, uint64_t cap,
-                  hipStream_t s) {
+                  const uint32_t* orderB, int32_t nB, const uint32_t* part_of_a, const uint32_t* boff, uint64_t* out,
+                  unsigned long long* cursor, uint64_t cap, hipStream_t s) {
     if (nA > 0)
         hipLaunchKernelGGL((k_broad<2>), dim3(grid1((uint64_t)nA)), dim3(kBlock), 0, s, oboxA, nA, oboxB, nodesB, orderB, nB,
-                           (uint32_t*)nullptr, (const uint64_t*)nullptr, out, cursor, cap);
+                           part_of_a, boff, (uint32_t*)nullptr, (const uint64_t*)nullptr, out, cursor, cap);
 }
 
 void mark_needed(const uint64_t* pairs, uint64_t np, uint32_t* needA, uint32_t* needB, hipStream_t s) {
@@ -1097,14 +1131,14 @@ void edge_scatter(const float* es, const float* ee, const int32_t* eo, uint64_t 
 }
 
 void task_counts(const uint64_t* pairs, uint64_t np, const uint32_t* locA, const uint32_t* esegA,
-                 const uint32_t* locB, const uint32_t* esegB, int dir_mask, uint32_t* counts, hipStream_t s) {
+                 const uint32_t* locB, const uint32_t* esegB, const uint8_t* meta, uint32_t* counts, hipStream_t s) {
     if (np)
         hipLaunchKernelGGL(k_task_counts, dim3(grid1(np)), dim3(kBlock), 0, s, pairs, np, locA, esegA, locB, esegB,
-                           dir_mask, counts);
+                           meta, counts);
 }
 
 bool narrow(const uint64_t* pairs, uint64_t np, const uint64_t* task_off, uint64_t ntasks, const SetView& A,
-            const SetView& B, uint32_t* found, int dir_mask, bool f32, int allow_f32, uint32_t height,
+            const SetView& B, uint32_t* found, const uint8_t* meta, bool f32, int allow_f32, uint32_t height,
             unsigned long long* stats, hipStream_t s) {
     if (ntasks == 0) return true;
     int stack = stack_for_height(height);
@@ -1112,54 +1146,55 @@ bool narrow(const uint64_t* pairs, uint64_t np, const uint64_t* task_off, uint64
     unsigned g = (unsigned)((ntasks + kWavesPerBlock - 1) / kWavesPerBlock);
     if (stack == 32) {
         if (f32)
-            launch_narrow<32, true>(g, s, pairs, np, task_off, A, B, found, dir_mask, allow_f32, stats);
+            launch_narrow<32, true>(g, s, pairs, np, task_off, A, B, found, meta, allow_f32, stats);
         else
-            launch_narrow<32, false>(g, s, pairs, np, task_off, A, B, found, dir_mask, allow_f32, stats);
+            launch_narrow<32, false>(g, s, pairs, np, task_off, A, B, found, meta, allow_f32, stats);
     } else {
         if (f32)
-            launch_narrow<64, true>(g, s, pairs, np, task_off, A, B, found, dir_mask, allow_f32, stats);
+            launch_narrow<64, true>(g, s, pairs, np, task_off, A, B, found, meta, allow_f32, stats);
         else
-            launch_narrow<64, false>(g, s, pairs, np, task_off, A, B, found, dir_mask, allow_f32, stats);
+            launch_narrow<64, false>(g, s, pairs, np, task_off, A, B, found, meta, allow_f32, stats);
     }
     return true;
 }
 
 void narrow_dual(const uint64_t* pairs, uint64_t np, const SetView& A, const SetView& B, uint32_t* found,
-                 uint32_t* fallback, int dir_mask, bool f32, int allow_f32, unsigned long long* stats, hipStream_t s) {
+                 uint32_t* fallback, const uint8_t* meta, bool f32, int allow_f32, unsigned long long* stats,
+                 hipStream_t s) {
     if (np == 0) return;
     unsigned g = (unsigned)((np + kDualWaves - 1) / kDualWaves);
     KernelTimer kt_("narrow_dual", s);
     if (f32)
         hipLaunchKernelGGL((k_narrow_dual<true>), dim3(g), dim3(kDualWaves * PZ_WAVE), 0, s, pairs, np, A, B, found,
-                           fallback, dir_mask, allow_f32, stats);
+                           fallback, meta, allow_f32, stats);
     else
         hipLaunchKernelGGL((k_narrow_dual<false>), dim3(g), dim3(kDualWaves * PZ_WAVE), 0, s, pairs, np, A, B, found,
-                           fallback, dir_mask, allow_f32, stats);
+                           fallback, meta, allow_f32, stats);
 }
 
 bool parity(const uint64_t* pairs, uint64_t np, const uint32_t* found, const SetView& A, const SetView& B,
             const float* vA, const uint32_t* iA, const uint32_t* ftA, const float* vB, const uint32_t* iB,
-            const uint32_t* ftB, int kind, int dir_mask, uint32_t* inside, uint32_t height,
+            const uint32_t* ftB, const uint8_t* meta, uint32_t* inside, uint32_t height,
             unsigned long long* stats, hipStream_t s) {
     if (np == 0) return true;
     int stack = stack_for_height(height);
     if (stack == 0) return false;
-    uint64_t n = np * (kind == 1 ? 2 : 1);
+    uint64_t n = np * 2;
     KernelTimer kt_("parity", s);
     if (stack == 32)
         hipLaunchKernelGGL((k_parity<32>), dim3(grid1(n)), dim3(kBlock), 0, s, pairs, np, found, A, B, vA, iA, ftA, vB,
-                           iB, ftB, kind, dir_mask, inside, stats);
+                           iB, ftB, meta, inside, stats);
     else
         hipLaunchKernelGGL((k_parity<64>), dim3(grid1(n)), dim3(kBlock), 0, s, pairs, np, found, A, B, vA, iA, ftA, vB,
-                           iB, ftB, kind, dir_mask, inside, stats);
+                           iB, ftB, meta, inside, stats);
     return true;
 }
 
-void select_pairs(const uint64_t* pairs, uint64_t np, const uint32_t* found, const uint32_t* inside, int predicate,
-                  int include_overlap, uint64_t* out_keys, uint32_t* out_type, unsigned long long* count,
-                  hipStream_t s) {
+void select_pairs(const uint64_t* pairs, uint64_t np, const uint32_t* found, const uint32_t* inside,
+                  const uint8_t* meta, int include_overlap, uint64_t* out_keys, uint32_t* out_type,
+                  unsigned long long* count, hipStream_t s) {
     if (np)
-        hipLaunchKernelGGL(k_select_pairs, dim3(grid1(np)), dim3(kBlock), 0, s, pairs, np, found, inside, predicate,
+        hipLaunchKernelGGL(k_select_pairs, dim3(grid1(np)), dim3(kBlock), 0, s, pairs, np, found, inside, meta,
                            include_overlap, out_keys, out_type, count);
 }
 
@@ -1193,6 +1228,13 @@ bool trace_global(int mode, const float* es, const float* ee, const int32_t* eo,
 void compact_table(const unsigned long long* table, uint64_t size, uint64_t* out, uint64_t cap,
                    unsigned long long* count, hipStream_t s) {
     if (size) hipLaunchKernelGGL(k_compact_table, dim3(grid1(size)), dim3(kBlock), 0, s, table, size, out, cap, count);
+}
+
+void pair_meta(const uint64_t* pairs, uint64_t np, const uint32_t* part_of_a, const uint8_t* meta_of_part,
+               uint8_t uniform, uint8_t* meta, hipStream_t s) {
+    if (np)
+        hipLaunchKernelGGL(k_pair_meta, dim3(grid1(np)), dim3(kBlock), 0, s, pairs, np, part_of_a, meta_of_part,
+                           uniform, meta);
 }
 
 void compact_flagged(const uint64_t* pairs, uint64_t np, const uint32_t* flag, uint64_t* out, uint32_t* out_index,

@@ -48,16 +48,23 @@ void build_small(const float* verts, const uint32_t* idx, const int32_t* t2o, co
                  uint32_t l_begin, uint32_t count, Node* nodes, LeafTri* leaves, uint32_t* height, hipStream_t s);
 void edge_count(const int32_t* eo, uint64_t ne, int32_t nobj, uint32_t* edge_cnt, hipStream_t s);
 void scene_box(const uint32_t* obox, int32_t nobj, uint32_t* scene, hipStream_t s);
-void object_keys(const uint32_t* obox, int32_t nobj, const uint32_t* scene, uint64_t* keys, uint32_t* vals,
-                 hipStream_t s);
+// keys = (part_of[o] << 30) | Morton of the box centre (part 0 when part_of is null)
+void object_keys(const uint32_t* obox, int32_t nobj, const uint32_t* scene, const uint32_t* part_of, uint64_t* keys,
+                 uint32_t* vals, hipStream_t s);
 // Broad phase: mesh1 object boxes against the mesh2 object LBVH (stack of 64).
+// With parts (batched joins), mesh1 object a only meets the mesh2 objects of
+// its part: sorted positions [boff[part_of_a[a]], boff[part + 1]).
 void broad_count(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const Node* nodesB, const uint32_t* orderB,
-                 int32_t nB, uint32_t* cnt, hipStream_t s);
+                 int32_t nB, const uint32_t* part_of_a, const uint32_t* boff, uint32_t* cnt, hipStream_t s);
 void broad_write(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const Node* nodesB, const uint32_t* orderB,
-                 int32_t nB, const uint64_t* off, uint64_t* out, hipStream_t s);
+                 int32_t nB, const uint32_t* part_of_a, const uint32_t* boff, const uint64_t* off, uint64_t* out,
+                 hipStream_t s);
 void broad_append(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const Node* nodesB,
-                  const uint32_t* orderB, int32_t nB, uint64_t* out, unsigned long long* cursor, uint64_t cap,
-                  hipStream_t s);
+                  const uint32_t* orderB, int32_t nB, const uint32_t* part_of_a, const uint32_t* boff, uint64_t* out,
+                  unsigned long long* cursor, uint64_t cap, hipStream_t s);
+// meta[p] = predicate | dir_mask << 2, from the pair's part or `uniform`.
+void pair_meta(const uint64_t* pairs, uint64_t np, const uint32_t* part_of_a, const uint8_t* meta_of_part,
+               uint8_t uniform, uint8_t* meta, hipStream_t s);
 void mark_needed(const uint64_t* pairs, uint64_t np, uint32_t* needA, uint32_t* needB, hipStream_t s);
 void needed_layout(const uint32_t* need, const uint32_t* loc, int32_t nobj, const uint32_t* tri_cnt,
                    const uint32_t* edge_cnt, int32_t* obj_of_loc, uint32_t* seg_tri_cnt, uint32_t* seg_edge_cnt,
@@ -71,14 +78,15 @@ void edge_scatter(const float* es, const float* ee, const int32_t* eo, uint64_t 
                   const uint32_t* need, const uint32_t* loc, const uint32_t* eseg_off, uint32_t* cursor,
                   EdgeRec* out, hipStream_t s);
 void task_counts(const uint64_t* pairs, uint64_t np, const uint32_t* locA, const uint32_t* esegA,
-                 const uint32_t* locB, const uint32_t* esegB, int dir_mask, uint32_t* counts, hipStream_t s);
+                 const uint32_t* locB, const uint32_t* esegB, const uint8_t* meta, uint32_t* counts, hipStream_t s);
 bool narrow(const uint64_t* pairs, uint64_t np, const uint64_t* task_off, uint64_t ntasks, const SetView& A,
-            const SetView& B, uint32_t* found, int dir_mask, bool f32, int allow_f32, uint32_t height,
+            const SetView& B, uint32_t* found, const uint8_t* meta, bool f32, int allow_f32, uint32_t height,
             unsigned long long* stats, hipStream_t s);
 // Dual-tree narrow phase (one wave per pair); pairs whose LDS queue overflows
 // get fallback[p] = 1 and must be re-run with narrow().
 void narrow_dual(const uint64_t* pairs, uint64_t np, const SetView& A, const SetView& B, uint32_t* found,
-                 uint32_t* fallback, int dir_mask, bool f32, int allow_f32, unsigned long long* stats, hipStream_t s);
+                 uint32_t* fallback, const uint8_t* meta, bool f32, int allow_f32, unsigned long long* stats,
+                 hipStream_t s);
 // Compact pairs with flag[p] != 0 into out (count -> *count).
 void compact_flagged(const uint64_t* pairs, uint64_t np, const uint32_t* flag, uint64_t* out, uint32_t* out_index,
                      unsigned long long* count, hipStream_t s);
@@ -86,11 +94,11 @@ void compact_flagged(const uint64_t* pairs, uint64_t np, const uint32_t* flag, u
 void scatter_found(const uint32_t* sub_found, const uint32_t* index, uint64_t n, uint32_t* found, hipStream_t s);
 bool parity(const uint64_t* pairs, uint64_t np, const uint32_t* found, const SetView& A, const SetView& B,
             const float* vA, const uint32_t* iA, const uint32_t* ftA, const float* vB, const uint32_t* iB,
-            const uint32_t* ftB, int kind, int dir_mask, uint32_t* inside, uint32_t height,
+            const uint32_t* ftB, const uint8_t* meta, uint32_t* inside, uint32_t height,
             unsigned long long* stats, hipStream_t s);
-void select_pairs(const uint64_t* pairs, uint64_t np, const uint32_t* found, const uint32_t* inside, int predicate,
-                  int include_overlap, uint64_t* out_keys, uint32_t* out_type, unsigned long long* count,
-                  hipStream_t s);
+void select_pairs(const uint64_t* pairs, uint64_t np, const uint32_t* found, const uint32_t* inside,
+                  const uint8_t* meta, int include_overlap, uint64_t* out_keys, uint32_t* out_type,
+                  unsigned long long* count, hipStream_t s);
 bool trace_global(int mode, const float* es, const float* ee, const int32_t* eo, uint64_t ne, const Node* nodes,
                   const LeafTri* leaves, uint64_t ntri, uint32_t height, int swap, bool f32, int allow_f32,
                   uint32_t* counts, const uint64_t* off, uint64_t* out, unsigned long long* table, uint64_t mask,
