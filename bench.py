@@ -58,6 +58,13 @@ def alg_bytes(kernel, k):
     return None
 
 
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
+# workload, scaled by a 1 GiB copy (fetch x 2.0, write x 1.0); kernel -> bytes.
+MEASURED_TRAFFIC = {
+    "build_small": 0.760e9 + 2.883e9,
+}
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -114,11 +121,25 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-side", type=float, default=500.0, help="side of the CPU sample sub-box")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device to device three times (scale of the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the join runs on a single device (multi-GPU sharding is described "
               "in DESIGN.md, not run); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        import torch
+
+        n = 1 << 28
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}), flush=True)
+        return
 
     import pierce_amd as pz
 
@@ -175,7 +196,8 @@ def main():
             if b is not None:
                 ach = b / (avg_ms * 1e-3) / 1e9
                 roofline = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
-                            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None,
+                            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
+                            "traffic": MEASURED_TRAFFIC.get(dom),
                             "alg_bytes_per_launch": int(b), "avg_launch_ms": round(avg_ms, 4), "launches": n}
 
         # CPU baseline: the oracle's Face restatement on a spatial sub-box of the same workload.

@@ -109,6 +109,8 @@ pz_status pz_host_mesh_compute_grid(pz_host_mesh* mesh, float cell_size);
 #define PZ_GEN_SPHERES_MIXED  1  /* spheres, level l in [level_min, level_max] with P(l) ~ 2^-l */
 #define PZ_GEN_CUBES          2  /* axis-aligned cubes, 12 triangles */
 #define PZ_GEN_TUBES          3  /* branching generalised-cylinder tubes */
+#define PZ_GEN_QUERY_MIX      4  /* per mesh: 50% ellipsoids (level uniform in [level_min, level_max]),
+                                    30% cubes (edge U[1, 10]), 20% tubes (radius U[0.5, 3]) */
 
 #define PZ_EDGES_NONE     0
 #define PZ_EDGES_GROUPED  1  /* each mesh's unique edges, meshes in id order (fast) */
@@ -119,7 +121,7 @@ typedef struct {
     int32_t  edges_mode;      /* PZ_EDGES_* */
     uint64_t count;           /* number of meshes */
     uint64_t seed;
-    int32_t  level_min;       /* icosphere subdivision */
+    int32_t  level_min;       /* icosphere subdivision (ellipsoids: uniform in [min, max]) */
     int32_t  level_max;
     float    radius_min;      /* ellipsoid radii / sphere radius / cube edge / tube radius */
     float    radius_max;
@@ -186,7 +188,8 @@ pz_status pz_dataset_info(const pz_dataset* ds, uint64_t out[5]);
 
 #define PZ_SIZING_TWO_PASS 0  /* exact count pass, scan, write pass */
 #define PZ_SIZING_ESTIMATE 1  /* grid selectivity estimate (estimated_overlap.cu:29-40) */
-#define PZ_SIZING_LEARNED  2  /* MLP estimate (needs pz_join_desc.model) */
+#define PZ_SIZING_LEARNED  2  /* MLP estimate (needs pz_join_desc.model); the buffer holds the
+                                 prediction raised by the model's calibrated margin */
 #define PZ_SIZING_FIXED    3  /* capacity given by pz_join_desc.capacity */
 
 #define PZ_DEDUP_SORT 0  /* GLOBAL strategy: raw hits sorted + uniqued */
@@ -205,7 +208,7 @@ typedef struct {
     int32_t max_hits_per_edge;  /* 0 = unlimited (the reference caps at overlap_max_iterations) */
     uint64_t capacity;          /* FIXED: buffer slots; ESTIMATE/LEARNED: 0 = computed */
     float    load_factor;       /* hash sets: slots = expected / load_factor (0 -> 0.5) */
-    float    margin;            /* estimate-sized buffers hold expected * (1 + margin) (0 -> 0.25) */
+    float    margin;            /* ESTIMATE: buffers hold expected * (1 + margin) (0 -> 0.25) */
     float    gamma;             /* grid estimator exponent (0 -> 0.8) */
     float    epsilon;           /* grid estimator size epsilon (0 -> 0.001) */
     int32_t  no_alpha_correction;
@@ -231,7 +234,9 @@ typedef struct {
     uint8_t* pair_types;        /* containment + include_overlap_pairs: 0 = containment, 1 = overlap; else NULL */
     uint64_t num_candidates;    /* TWO_LEVEL: object pairs with overlapping boxes; GLOBAL: raw edge hits */
     uint64_t capacity;          /* slots of the sized buffer actually used */
-    uint64_t predicted;         /* sizing prediction (0 for TWO_PASS) */
+    uint64_t predicted;         /* sizing prediction of the sized count (0 for TWO_PASS): candidate
+                                   object pairs (TWO_LEVEL) or unique pairs (GLOBAL) */
+    uint64_t predicted_pairs;   /* LEARNED: predicted result pairs */
     int32_t  overflowed;        /* 1: the estimate-sized buffer overflowed; the exact count was used */
     int32_t  num_phases;
     uint64_t tests_total;       /* primitive predicate evaluations (segment-triangle and parity) */
@@ -262,9 +267,71 @@ pz_status pz_dataset_upload_batch(pz_ctx* ctx, uint32_t nparts, const pz_mesh_vi
  * include_overlap_pairs apply to query q; every other field is taken from
  * descs[0] (strategy must be TWO_LEVEL).  results[q] gets query q's pairs with
  * part-local object ids; summary (optional) gets the batch's timings and
- * counters (no pairs). */
+ * counters (no pairs).  results[q].num_candidates is query q's candidate
+ * object-pair count; with LEARNED sizing results[q].predicted /
+ * predicted_pairs / capacity hold query q's prediction and its sized share. */
 pz_status pz_join_batch(pz_ctx* ctx, const pz_dataset* mesh1, const pz_dataset* mesh2, uint32_t nqueries,
                         const pz_join_desc* descs, pz_join_result* results, pz_join_result* summary);
+
+/* --------------------------------------------------- learned output sizing */
+/* A 32-128-128-2 MLP predicts, per join (or per query of a batch),
+ * y0 = log2(1 + candidate object pairs) and y1 = log2(1 + result pairs) from
+ * 32 features of the two sets' object boxes.  Inference runs in bf16 on the
+ * matrix cores; training runs on the GPU in fp32 (forward, backward, AdamW).
+ * Replaces the grid estimate of
+ * pierce/query/src/applications/pierce_overlap_main.cpp:376-501 as the sizing
+ * step; an overflow still falls back to the exact two-pass count. */
+#define PZ_MLP_FEATURES 32
+#define PZ_MLP_OUTPUTS  2
+#define PZ_MLP_PARAMS   20994  /* W1[128][32] b1[128] W2[128][128] b2[128] W3[2][128] b3[2] */
+#define PZ_MLP_NORM     64     /* feature mean[32], 1/std[32] */
+
+typedef struct {
+    int32_t  epochs;          /* passes over the samples */
+    int32_t  batch;           /* samples per step (clamped to the sample count) */
+    float    lr;              /* AdamW learning rate, cosine-decayed to lr_final */
+    float    lr_final;
+    float    weight_decay;    /* decoupled decay (0 = none) */
+    int32_t  use_graph;       /* 1: capture one epoch as a HIP graph and replay it */
+    uint64_t seed;            /* shuffling */
+    float    margin_quantile; /* quantile of (target - prediction) over the validation samples used
+                                 as the log2 sizing margin (plus 0.05) */
+    float    validation_fraction; /* samples held out of training to calibrate the margins (a seeded
+                                     random subset; with fewer than 8 the training samples are used) */
+    int32_t  reserved[4];
+} pz_mlp_train_desc;
+
+/* 150 epochs, batch 64, lr 2e-3 -> 1e-4, quantile 0.95, validation 0.2 */
+void      pz_mlp_train_desc_defaults(pz_mlp_train_desc* d);
+/* New model on the context's device: He-uniform weights from seed, identity
+ * normalisation (set from the data by the first pz_mlp_train). */
+pz_status pz_mlp_create(pz_ctx* ctx, uint64_t seed, pz_mlp** out);
+void      pz_mlp_destroy(pz_mlp* model);
+pz_status pz_mlp_save(const pz_mlp* model, const char* path);
+pz_status pz_mlp_load(pz_ctx* ctx, const char* path, pz_mlp** out);
+/* Any output may be NULL: params[PZ_MLP_PARAMS], norm[PZ_MLP_NORM], margin[PZ_MLP_OUTPUTS]. */
+pz_status pz_mlp_get_params(const pz_mlp* model, float* params, float* norm, float* margin);
+pz_status pz_mlp_set_params(pz_mlp* model, const float* params, const float* norm, const float* margin);
+
+/* Features of one join (out[32]) or of every query of a batch (out[nq][32]). */
+pz_status pz_join_features(pz_ctx* ctx, const pz_dataset* mesh1, const pz_dataset* mesh2, int32_t predicate,
+                           float* out);
+pz_status pz_batch_features(pz_ctx* ctx, const pz_dataset* mesh1, const pz_dataset* mesh2, uint32_t nqueries,
+                            const int32_t* predicates, float* out);
+
+/* Y[n][2] from X[n][32] (host arrays): bf16 matrix-core inference (the path the
+ * joins use) or the fp32 forward of the training kernels. */
+pz_status pz_mlp_predict(pz_ctx* ctx, const pz_mlp* model, const float* X, uint64_t n, float* Y);
+pz_status pz_mlp_forward(pz_ctx* ctx, const pz_mlp* model, const float* X, uint64_t n, float* Y);
+/* Loss = mean over samples of sum over outputs of (y - t)^2 and its gradient
+ * grad[PZ_MLP_PARAMS] over all n samples as one batch (fp32). */
+pz_status pz_mlp_gradients(pz_ctx* ctx, const pz_mlp* model, const float* X, const float* T, uint64_t n,
+                           float* loss, float* grad);
+/* Train on (X[n][32], T[n][2]) with T = log2(1 + exact counts) from two-pass
+ * joins; epoch_loss (optional) gets the mean step loss of each epoch.  Then
+ * calibrates the sizing margins on the validation samples. */
+pz_status pz_mlp_train(pz_ctx* ctx, pz_mlp* model, const float* X, const float* T, uint64_t n,
+                       const pz_mlp_train_desc* desc, float* epoch_loss);
 
 /* Number of (edge, triangle) pairs over both directions whose closed fp32
  * boxes overlap (the BVH-independent candidate-test count). */

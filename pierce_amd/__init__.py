@@ -27,7 +27,7 @@ PZ_OK = 0
 PZ_E_INVALID, PZ_E_HIP, PZ_E_IO, PZ_E_OVERFLOW, PZ_E_NOMEM, PZ_E_NO_EDGES, PZ_E_LIMIT, PZ_E_NODEVICE = (
     -1, -2, -3, -4, -5, -6, -7, -8)
 
-GEN_ELLIPSOIDS, GEN_SPHERES_MIXED, GEN_CUBES, GEN_TUBES = 0, 1, 2, 3
+GEN_ELLIPSOIDS, GEN_SPHERES_MIXED, GEN_CUBES, GEN_TUBES, GEN_QUERY_MIX = 0, 1, 2, 3, 4
 EDGES_NONE, EDGES_GROUPED, EDGES_PRE = 0, 1, 2
 
 PRED_OVERLAP, PRED_INTERSECTION, PRED_CONTAINMENT = 0, 1, 2
@@ -89,12 +89,23 @@ class Phase(C.Structure):
 class JoinResult(C.Structure):
     _fields_ = [("num_pairs", C.c_uint64), ("pairs", C.c_void_p), ("pair_types", C.c_void_p),
                 ("num_candidates", C.c_uint64), ("capacity", C.c_uint64), ("predicted", C.c_uint64),
-                ("overflowed", C.c_int32), ("num_phases", C.c_int32), ("tests_total", C.c_uint64),
+                ("predicted_pairs", C.c_uint64), ("overflowed", C.c_int32), ("num_phases", C.c_int32), ("tests_total", C.c_uint64),
                 ("tests_fp64", C.c_uint64), ("tests_exact", C.c_uint64), ("tests_fp32_misclassified", C.c_uint64),
                 ("edges_traced", C.c_uint64), ("stack_overflows", C.c_uint64), ("index_triangles", C.c_uint64),
                 ("index_edges", C.c_uint64), ("ms_total", C.c_double),
                 ("phases", Phase * MAX_PHASES)]
 
+
+MLP_FEATURES, MLP_OUTPUTS, MLP_PARAMS, MLP_NORM = 32, 2, 20994, 64
+
+
+class MlpTrainDesc(C.Structure):
+    _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("lr_final", C.c_float),
+                ("weight_decay", C.c_float), ("use_graph", C.c_int32), ("seed", C.c_uint64),
+                ("margin_quantile", C.c_float), ("validation_fraction", C.c_float), ("reserved", C.c_int32 * 4)]
+
+
+_FP = C.POINTER(C.c_float)
 
 # Every function declared in include/pierce_hip.h: (name, restype, argtypes).
 _SIGS = [
@@ -132,6 +143,20 @@ _SIGS = [
     ("exclusive_scan_gpu", C.c_longlong, [C.c_void_p, C.c_void_p, C.c_int]),
     ("pz_estimate_overlap_selectivity", C.c_float,
      [C.POINTER(GridCell), C.POINTER(GridCell), C.c_int32, C.c_float, C.c_float, C.c_float]),
+    ("pz_mlp_train_desc_defaults", None, [C.POINTER(MlpTrainDesc)]),
+    ("pz_mlp_create", C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
+    ("pz_mlp_destroy", None, [C.c_void_p]),
+    ("pz_mlp_save", C.c_int32, [C.c_void_p, C.c_char_p]),
+    ("pz_mlp_load", C.c_int32, [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p)]),
+    ("pz_mlp_get_params", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("pz_mlp_set_params", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("pz_join_features", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
+    ("pz_batch_features", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    ("pz_mlp_predict", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    ("pz_mlp_forward", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    ("pz_mlp_gradients", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, _FP, C.c_void_p]),
+    ("pz_mlp_train", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
+                                 C.POINTER(MlpTrainDesc), C.c_void_p]),
 ]
 
 _lib: Optional[C.CDLL] = None
@@ -382,6 +407,7 @@ class JoinOutput:
     index_edges: int
     ms_total: float
     phases: list
+    predicted_pairs: int = 0
 
 
 def join_desc(**kw) -> JoinDesc:
@@ -390,8 +416,12 @@ def join_desc(**kw) -> JoinDesc:
     for k, v in kw.items():
         if not hasattr(d, k):
             raise KeyError(k)
-        setattr(d, k, v)
+        setattr(d, k, _field(v))
     return d
+
+
+def _field(v):
+    return v._h.value if isinstance(v, Model) else v
 
 
 def _output(r: JoinResult) -> JoinOutput:
@@ -407,7 +437,8 @@ def _output(r: JoinResult) -> JoinOutput:
     phases = [(r.phases[i].name.decode(), r.phases[i].ms) for i in range(r.num_phases)]
     return JoinOutput(pairs, types, int(r.num_candidates), int(r.capacity), int(r.predicted), bool(r.overflowed),
                       int(r.tests_total), int(r.tests_fp64), int(r.tests_exact), int(r.tests_fp32_misclassified),
-                      int(r.edges_traced), int(r.index_triangles), int(r.index_edges), float(r.ms_total), phases)
+                      int(r.edges_traced), int(r.index_triangles), int(r.index_edges), float(r.ms_total), phases,
+                      int(r.predicted_pairs))
 
 
 def join_batch(ctx: Context, a: Dataset, b: Dataset, queries, **common):
@@ -421,7 +452,7 @@ def join_batch(ctx: Context, a: Dataset, b: Dataset, queries, **common):
     for q, kw in enumerate(queries):
         lib().pz_join_desc_defaults(C.byref(descs[q]))
         for k, v in list(common.items()) + list(kw.items()):
-            setattr(descs[q], k, v)
+            setattr(descs[q], k, _field(v))
     res = (JoinResult * nq)()
     summ = JoinResult()
     _check(lib().pz_join_batch(ctx._h, a._h, b._h, nq, descs, res, C.byref(summ)))
@@ -463,3 +494,110 @@ def count_candidate_tests(ctx: Context, a: Dataset, b: Dataset) -> int:
     out = C.c_uint64()
     _check(lib().pz_count_candidate_tests(ctx._h, a._h, b._h, C.byref(out)))
     return int(out.value)
+
+
+# ------------------------------------------------------------ learned sizing
+def _f32(a, shape) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.shape != shape:
+        raise ValueError(f"expected shape {shape}, got {a.shape}")
+    return a
+
+
+def train_desc(**kw) -> MlpTrainDesc:
+    d = MlpTrainDesc()
+    lib().pz_mlp_train_desc_defaults(C.byref(d))
+    for k, v in kw.items():
+        if not hasattr(d, k):
+            raise KeyError(k)
+        setattr(d, k, v)
+    return d
+
+
+class Model:
+    """Selectivity MLP on a context's device (pz_mlp)."""
+
+    def __init__(self, ctx: Context, seed: int = 1, path: Optional[str] = None):
+        self.ctx = ctx
+        self._h = C.c_void_p()
+        if path is None:
+            _check(lib().pz_mlp_create(ctx._h, seed, C.byref(self._h)))
+        else:
+            _check(lib().pz_mlp_load(ctx._h, path.encode(), C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().pz_mlp_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def save(self, path: str) -> None:
+        _check(lib().pz_mlp_save(self._h, path.encode()))
+
+    def get(self):
+        """(params[20994], norm[64], margin[2]) as float32 arrays."""
+        p = np.zeros(MLP_PARAMS, np.float32)
+        n = np.zeros(MLP_NORM, np.float32)
+        m = np.zeros(MLP_OUTPUTS, np.float32)
+        _check(lib().pz_mlp_get_params(self._h, _ptr(p), _ptr(n), _ptr(m)))
+        return p, n, m
+
+    def set(self, params=None, norm=None, margin=None) -> None:
+        p = None if params is None else _f32(params, (MLP_PARAMS,))
+        n = None if norm is None else _f32(norm, (MLP_NORM,))
+        m = None if margin is None else _f32(margin, (MLP_OUTPUTS,))
+        _check(lib().pz_mlp_set_params(self._h, _ptr(p), _ptr(n), _ptr(m)))
+
+    def _run(self, fn, X) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, MLP_FEATURES)
+        Y = np.zeros((X.shape[0], MLP_OUTPUTS), np.float32)
+        _check(fn(self.ctx._h, self._h, _ptr(X), X.shape[0], _ptr(Y)))
+        return Y
+
+    def predict(self, X) -> np.ndarray:
+        """bf16 matrix-core inference: Y[n][2] = log2(1 + counts)."""
+        return self._run(lib().pz_mlp_predict, X)
+
+    def forward(self, X) -> np.ndarray:
+        """fp32 forward of the training kernels."""
+        return self._run(lib().pz_mlp_forward, X)
+
+    def gradients(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, MLP_FEATURES)
+        T = _f32(T, (X.shape[0], MLP_OUTPUTS))
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        _check(lib().pz_mlp_gradients(self.ctx._h, self._h, _ptr(X), _ptr(T), X.shape[0], C.byref(loss), _ptr(g)))
+        return float(loss.value), g
+
+    def train(self, X, T, **kw) -> np.ndarray:
+        """Train on the GPU; returns the mean loss of each epoch."""
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, MLP_FEATURES)
+        T = _f32(T, (X.shape[0], MLP_OUTPUTS))
+        d = train_desc(**kw)
+        losses = np.zeros(max(d.epochs, 1), np.float32)
+        _check(lib().pz_mlp_train(self.ctx._h, self._h, _ptr(X), _ptr(T), X.shape[0], C.byref(d), _ptr(losses)))
+        return losses
+
+
+def join_features(ctx: Context, a: Dataset, b: Dataset, predicate: int = PRED_OVERLAP) -> np.ndarray:
+    out = np.zeros(MLP_FEATURES, np.float32)
+    _check(lib().pz_join_features(ctx._h, a._h, b._h, predicate, _ptr(out)))
+    return out
+
+
+def batch_features(ctx: Context, a: Dataset, b: Dataset, predicates) -> np.ndarray:
+    preds = np.ascontiguousarray(predicates, dtype=np.int32)
+    out = np.zeros((preds.shape[0], MLP_FEATURES), np.float32)
+    _check(lib().pz_batch_features(ctx._h, a._h, b._h, preds.shape[0], _ptr(preds), _ptr(out)))
+    return out
+
+
+def log_counts(counts) -> np.ndarray:
+    """Training target of a count: log2(1 + count)."""
+    return np.log2(1.0 + np.asarray(counts, dtype=np.float64)).astype(np.float32)

@@ -25,11 +25,12 @@
 #include <vector>
 
 #include "host/host_mesh.hpp"
+#include "kernels/pz_kernels_est.h"
 #include "kernels/pz_kernels_join.h"
 #include "pierce_hip.h"
 #include "pz_internal.h"
 
-using pz::fail;
+using namespace pz;
 
 namespace pz {
 // predicate | direction bits << 2 of one join description
@@ -39,98 +40,22 @@ uint8_t pair_meta_of(const pz_join_desc& d) {
                         : (d.direction == PZ_DIR_1TO2 ? 1 : (d.direction == PZ_DIR_2TO1 ? 2 : 3));
     return (uint8_t)(d.predicate | (dir << 2));
 }
+
+SetPrep prep_set(pz_ctx* c, const pz_dataset* d, const std::string& tag) {
+    SetPrep p;
+    size_t n = std::max<int32_t>(d->nobj, 1);
+    p.obox = buf<uint32_t>(c, tag + ".obox", 6 * n);
+    p.tri_cnt = buf<uint32_t>(c, tag + ".tri_cnt", n);
+    p.first_tri = buf<uint32_t>(c, tag + ".first_tri", n);
+    p.last_tri = buf<uint32_t>(c, tag + ".last_tri", n);
+    p.edge_cnt = buf<uint32_t>(c, tag + ".edge_cnt", n);
+    pzk::object_stats(d->v, d->nv, d->idx, d->t2o, d->nt, d->nobj, p.obox, p.tri_cnt, p.first_tri, p.last_tri,
+                      c->s);
+    return p;
+}
 }  // namespace pz
 
-// ------------------------------------------------------------------ context
-struct DevBuf {
-    void* p = nullptr;
-    size_t n = 0;
-};
-
-struct pz_ctx {
-    int device = 0;
-    hipStream_t s = nullptr;
-    std::unordered_map<std::string, DevBuf> bufs;
-    void* pinned = nullptr;  // small readbacks
-    size_t pinned_n = 0;
-    hipEvent_t ev[PZ_MAX_PHASES + 1];
-};
-
-struct pz_dataset {
-    pz_ctx* ctx = nullptr;
-    float* v = nullptr;
-    uint32_t* idx = nullptr;
-    int32_t* t2o = nullptr;
-    float* es = nullptr;
-    float* ee = nullptr;
-    int32_t* eo = nullptr;
-    uint64_t nv = 0, nt = 0, ne = 0;
-    int32_t nobj = 0;
-    float max_abs = 0.f;
-    std::vector<pz_grid_cell> grid;
-    float cell_size = 0.f;
-    // batched datasets: part p owns objects [part_obj_off[p], part_obj_off[p + 1])
-    std::vector<uint64_t> part_obj_off;
-    uint32_t* part_of = nullptr;   // device: part id per object
-    uint32_t* part_off = nullptr;  // device: part_obj_off as uint32
-};
-
 namespace {
-
-#define HIP_OK(expr)                                                                                   \
-    do {                                                                                               \
-        hipError_t e_ = (expr);                                                                        \
-        if (e_ != hipSuccess)                                                                          \
-            throw PzError(e_ == hipErrorOutOfMemory ? PZ_E_NOMEM : PZ_E_HIP,                           \
-                          std::string(#expr) + ": " + hipGetErrorString(e_));                          \
-    } while (0)
-
-struct PzError {
-    pz_status code;
-    std::string msg;
-    PzError(pz_status c, std::string m) : code(c), msg(std::move(m)) {}
-};
-
-template <class T>
-T* buf(pz_ctx* c, const std::string& name, size_t count) {
-    size_t bytes = std::max<size_t>(count * sizeof(T), 256);
-    DevBuf& b = c->bufs[name];
-    if (b.n < bytes) {
-        if (b.p) HIP_OK(hipFree(b.p));
-        b.p = nullptr;
-        b.n = 0;
-        size_t want = bytes + bytes / 4;
-        HIP_OK(hipMalloc(&b.p, want));
-        b.n = want;
-    }
-    return reinterpret_cast<T*>(b.p);
-}
-
-void* pinned(pz_ctx* c, size_t bytes) {
-    if (c->pinned_n < bytes) {
-        if (c->pinned) HIP_OK(hipHostFree(c->pinned));
-        c->pinned = nullptr;
-        HIP_OK(hipHostMalloc(&c->pinned, bytes, hipHostMallocDefault));
-        c->pinned_n = bytes;
-    }
-    return c->pinned;
-}
-
-// Read n scalars (each from a device address) in one synchronisation.
-template <class T>
-std::vector<T> readback(pz_ctx* c, std::initializer_list<const T*> srcs) {
-    T* h = reinterpret_cast<T*>(pinned(c, srcs.size() * sizeof(T)));
-    size_t i = 0;
-    for (const T* p : srcs) HIP_OK(hipMemcpyAsync(h + i++, p, sizeof(T), hipMemcpyDeviceToHost, c->s));
-    HIP_OK(hipStreamSynchronize(c->s));
-    return std::vector<T>(h, h + srcs.size());
-}
-
-int bits_for(uint64_t n) {  // bits needed to hold values < n
-    int b = 0;
-    while (b < 64 && (1ull << b) < n) ++b;
-    return std::max(b, 1);
-}
 
 // Phase timer: events recorded on the stream at phase boundaries.
 struct Phases {
@@ -158,27 +83,6 @@ struct Phases {
 };
 
 // ------------------------------------------------------------- set prep
-struct SetPrep {  // per-object statistics of one set (device)
-    uint32_t* obox;
-    uint32_t* tri_cnt;
-    uint32_t* first_tri;
-    uint32_t* last_tri;
-    uint32_t* edge_cnt;
-};
-
-SetPrep prep_set(pz_ctx* c, const pz_dataset* d, const std::string& tag) {
-    SetPrep p;
-    size_t n = std::max<int32_t>(d->nobj, 1);
-    p.obox = buf<uint32_t>(c, tag + ".obox", 6 * n);
-    p.tri_cnt = buf<uint32_t>(c, tag + ".tri_cnt", n);
-    p.first_tri = buf<uint32_t>(c, tag + ".first_tri", n);
-    p.last_tri = buf<uint32_t>(c, tag + ".last_tri", n);
-    p.edge_cnt = buf<uint32_t>(c, tag + ".edge_cnt", n);
-    pzk::object_stats(d->v, d->nv, d->idx, d->t2o, d->nt, d->nobj, p.obox, p.tri_cnt, p.first_tri, p.last_tri,
-                      c->s);
-    return p;
-}
-
 // LBVH over object boxes of one set (broad-phase target).
 struct ObjBvh {
     pzd::Node* nodes;
@@ -241,10 +145,6 @@ struct SetAccel {
     pzd::EdgeRec* edges;
     uint32_t* height;
 };
-
-double now_ms() {
-    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
 
 void init_result(pz_join_result* r) {
     std::memset(r, 0, sizeof(*r));
@@ -324,8 +224,11 @@ double grid_estimate(const pz_dataset* a, const pz_dataset* b, float gamma, floa
 
 // ------------------------------------------------------------- two-level
 // meta_of_part: device predicate | dir << 2 per part for batched sets (null: from d).
+// preds_out (optional) receives the learned per-part predictions.
 void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const pz_join_desc& d, pz_join_result* r,
-                    Phases& ph, const uint8_t* meta_of_part = nullptr) {
+                    Phases& ph, const uint8_t* meta_of_part = nullptr,
+                    std::vector<SizePrediction>* preds_out = nullptr,
+                    std::vector<uint64_t>* part_cand_out = nullptr) {
     const bool batched = meta_of_part != nullptr;
     hipStream_t s = c->s;
     unsigned long long* stats = buf<unsigned long long>(c, "stats", 8);
@@ -334,6 +237,20 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
 
     ph.begin("Object Bounds");
     SetPrep pa = prep_set(c, A, "A"), pb = prep_set(c, B, "B");
+    double learned_upper = 0;  // LEARNED: calibrated upper bound of the candidate-pair count
+    if (d.sizing == PZ_SIZING_LEARNED) {
+        ph.begin("Selectivity Estimation");
+        std::vector<SizePrediction> pr = mlp_predict(c, d.model, A, B, pa, pb, meta_of_part, pair_meta_of(d));
+        double cand = 0, pairs = 0;
+        for (const SizePrediction& p : pr) {
+            cand += p.cand;
+            pairs += p.pairs;
+            learned_upper += p.cand_upper;
+        }
+        r->predicted = (uint64_t)std::llround(cand);
+        r->predicted_pairs = (uint64_t)std::llround(pairs);
+        if (preds_out) *preds_out = std::move(pr);
+    }
 
     ph.begin("Build Object Index");
     ObjBvh ob = build_object_bvh(c, B, pb, "B");
@@ -362,10 +279,12 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         r->capacity = np;
     } else {
         uint64_t cap = d.capacity;
-        if (d.sizing != PZ_SIZING_FIXED) {
+        if (d.sizing == PZ_SIZING_ESTIMATE) {
             double pred = (double)r->predicted;
             float margin = d.margin > 0 ? d.margin : 0.25f;
             cap = (uint64_t)std::ceil(pred * (1.0 + margin)) + 64;
+        } else if (d.sizing == PZ_SIZING_LEARNED) {
+            cap = (uint64_t)std::ceil(std::min(learned_upper, 0x1p50)) + 64;
         }
         cap = std::max<uint64_t>(cap, 1);
         r->capacity = cap;
@@ -381,6 +300,14 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         }
     }
     r->num_candidates = np;
+    if (batched && part_cand_out) {  // candidate object pairs of each query
+        const uint32_t nparts = (uint32_t)(A->part_obj_off.size() - 1);
+        unsigned long long* pc = buf<unsigned long long>(c, "bp.part_cnt", nparts);
+        pzk::part_pair_counts(pairs, np, A->part_of, nparts, pc, s);
+        part_cand_out->resize(nparts);
+        HIP_OK(hipMemcpyAsync(part_cand_out->data(), pc, sizeof(uint64_t) * nparts, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+    }
     if (np == 0) {  // no boxes meet: nothing to index or test
         ph.begin("Download Results");
         uint64_t* none = buf<uint64_t>(c, "sel.keys", 1);
@@ -617,6 +544,15 @@ void join_global(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const pz_j
     pzk::fill_ull(stats, 0ull, 8, s);
     ph.begin("Build Mesh1 Index");
     SetPrep pa = prep_set(c, A, "A"), pb = prep_set(c, B, "B");
+    double learned_upper = 0;  // LEARNED: calibrated upper bound of the unique-pair count
+    if (d.sizing == PZ_SIZING_LEARNED) {
+        ph.begin("Selectivity Estimation");
+        std::vector<SizePrediction> pr = mlp_predict(c, d.model, A, B, pa, pb, nullptr, pair_meta_of(d));
+        r->predicted = (uint64_t)std::llround(pr[0].pairs);
+        r->predicted_pairs = r->predicted;
+        learned_upper = pr[0].pairs_upper;
+        ph.begin("Build Mesh1 Index");
+    }
     const bool do12 = d.direction != PZ_DIR_2TO1, do21 = d.direction != PZ_DIR_1TO2;
     GlobalBvh ga{}, gb{};
     if (do21) ga = build_global(c, A, pa, "A");
@@ -662,6 +598,7 @@ void join_global(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const pz_j
     bool done = false;
     if (d.dedup == PZ_DEDUP_HASH || d.sizing != PZ_SIZING_TWO_PASS) {
         uint64_t expected = d.sizing == PZ_SIZING_FIXED ? d.capacity : std::max<uint64_t>(r->predicted, 1);
+        if (d.sizing == PZ_SIZING_LEARNED) expected = (uint64_t)std::ceil(std::min(std::max(learned_upper, 1.0), 0x1p40));
         float lf = d.load_factor > 0 ? d.load_factor : 0.5f;
         uint64_t slots = 1024;
         if (d.sizing == PZ_SIZING_FIXED)
@@ -725,17 +662,6 @@ void join_global(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const pz_j
     r->tests_exact = st[2];
     r->edges_traced = st[4];
     r->tests_fp32_misclassified = st[3];
-}
-
-template <class F>
-pz_status guarded(F f) {
-    try {
-        return f();
-    } catch (const PzError& e) {
-        return fail(e.code, e.msg);
-    } catch (const std::bad_alloc&) {
-        return fail(PZ_E_NOMEM, "host allocation failed");
-    }
 }
 
 }  // namespace
@@ -926,9 +852,6 @@ pz_status pz_join(pz_ctx* c, const pz_dataset* a, const pz_dataset* b, const pz_
             double est = grid_estimate(a, b, d.gamma > 0 ? d.gamma : 0.8f, d.epsilon > 0 ? d.epsilon : 0.001f,
                                        !d.no_alpha_correction);
             r->predicted = est > 0 ? (uint64_t)est : 0;
-        } else if (d.sizing == PZ_SIZING_LEARNED) {
-            ph.begin("Selectivity Estimation");
-            r->predicted = pz::mlp_predict_count(c, d.model, a, b, d.predicate);
         }
         if (d.strategy == PZ_STRATEGY_GLOBAL)
             join_global(c, a, b, d, r, ph);
@@ -1045,13 +968,12 @@ pz_status pz_join_batch(pz_ctx* c, const pz_dataset* a, const pz_dataset* b, uin
         for (uint32_t q = 0; q < nq; ++q) hmeta[q] = pz::pair_meta_of(descs[q]);
         uint8_t* dmeta = buf<uint8_t>(c, "batch.meta", nq);
         HIP_OK(hipMemcpyAsync(dmeta, hmeta.data(), nq, hipMemcpyHostToDevice, c->s));
-        if (d.sizing == PZ_SIZING_LEARNED) {
-            ph.begin("Selectivity Estimation");
-            all.predicted = pz::mlp_predict_batch_total(c, d.model, a, b);
-        } else if (d.sizing == PZ_SIZING_ESTIMATE) {
-            d.sizing = PZ_SIZING_TWO_PASS;  // the grid estimate is per dataset, not per part
-        }
-        join_two_level(c, a, b, d, &all, ph, dmeta);
+        if (d.sizing == PZ_SIZING_LEARNED && !d.model)
+            throw PzError(PZ_E_INVALID, "learned sizing needs a model");
+        if (d.sizing == PZ_SIZING_ESTIMATE) d.sizing = PZ_SIZING_TWO_PASS;  // the grid estimate is per dataset
+        std::vector<SizePrediction> preds;
+        std::vector<uint64_t> all_part_cand;
+        join_two_level(c, a, b, d, &all, ph, dmeta, &preds, &all_part_cand);
         ph.finish();
         all.ms_total = now_ms() - t0;
         // split by part: pairs are sorted by (mesh1, mesh2) global ids, so parts come in order
@@ -1080,6 +1002,12 @@ pz_status pz_join_batch(pz_ctx* c, const pz_dataset* a, const pz_dataset* b, uin
                 std::copy(pert[q].begin(), pert[q].end(), r.pair_types);
             }
             r.ms_total = all.ms_total;
+            if (q < preds.size()) {
+                r.predicted = (uint64_t)std::llround(preds[q].cand);
+                r.predicted_pairs = (uint64_t)std::llround(preds[q].pairs);
+                r.capacity = (uint64_t)std::ceil(preds[q].cand_upper);
+            }
+            if (q < all_part_cand.size()) r.num_candidates = all_part_cand[q];
         }
         if (summary) {
             *summary = all;

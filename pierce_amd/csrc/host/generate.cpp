@@ -224,6 +224,12 @@ int mixed_level(Rng& r, int lmin, int lmax) {
     return lmax;
 }
 
+// Level uniform in [lmin, lmax].
+int uniform_level(Rng& r, int lmin, int lmax) {
+    const int l = lmin + (int)(r.uniform() * (double)(lmax - lmin + 1));
+    return std::min(l, lmax);
+}
+
 }  // namespace
 
 extern "C" void pz_gen_defaults(int32_t kind, pz_gen_params* p) {
@@ -251,6 +257,13 @@ extern "C" void pz_gen_defaults(int32_t kind, pz_gen_params* p) {
         p->box = 500.0f;
     } else if (kind == PZ_GEN_CUBES) {
         p->radius_max = 10.0f;
+    } else if (kind == PZ_GEN_QUERY_MIX) {
+        p->level_min = 1;
+        p->level_max = 4;
+        p->tri_median = 15000;
+        p->tri_sigma = 0.6f;
+        p->tri_min = 5000;
+        p->tri_max = 50000;
     }
 }
 
@@ -262,6 +275,8 @@ static pz_status generate_impl(const pz_gen_params& p, pz_host_mesh& m) {
         return pz::fail(PZ_E_INVALID, "invalid radius or box");
     if (p.copy_fraction > 0 && p.kind != PZ_GEN_ELLIPSOIDS)
         return pz::fail(PZ_E_INVALID, "copies are supported for ellipsoid sets only");
+    if (p.copy_fraction > 0 && p.level_max != p.level_min)
+        return pz::fail(PZ_E_INVALID, "copies need a single subdivision level");
 
     // Per-mesh template choice (the first draws of each mesh's stream).
     std::vector<MeshPlan> plan(n);
@@ -277,6 +292,10 @@ static pz_status generate_impl(const pz_gen_params& p, pz_host_mesh& m) {
         } else if (p.kind == PZ_GEN_SPHERES_MIXED) {
             Rng r(p.seed, i);
             level[i] = mixed_level(r, p.level_min, p.level_max);
+            T = &icosphere(level[i]);
+        } else if (p.level_max > p.level_min) {  // ellipsoids, level uniform in [min, max]
+            Rng r(p.seed, i);
+            level[i] = uniform_level(r, p.level_min, p.level_max);
             T = &icosphere(level[i]);
         } else {
             T = &icosphere(p.level_min);
@@ -315,6 +334,7 @@ static pz_status generate_impl(const pz_gen_params& p, pz_host_mesh& m) {
         } else if (p.kind == PZ_GEN_CUBES) {
             cube_vertices(T, r, p, V);
         } else {
+            if (p.level_max > p.level_min) (void)uniform_level(r, p.level_min, p.level_max);
             bool copied = false;
             if (p.copy_fraction > 0) {
                 double u = r.uniform();
@@ -358,10 +378,66 @@ static pz_status generate_impl(const pz_gen_params& p, pz_host_mesh& m) {
 
 pz_status pz_gen_tubes_impl(const pz_gen_params& p, pz_host_mesh& m);
 
+// Append b to a: vertex indices and object ids shifted by a's totals.
+static void append_mesh(pz_host_mesh& a, const pz_host_mesh& b, int32_t obj_base) {
+    const uint32_t vbase = (uint32_t)a.num_vertices();
+    a.vertices.insert(a.vertices.end(), b.vertices.begin(), b.vertices.end());
+    for (uint32_t i : b.indices) a.indices.push_back(i + vbase);
+    for (int32_t o : b.tri_to_object) a.tri_to_object.push_back(o + obj_base);
+    a.edge_starts.insert(a.edge_starts.end(), b.edge_starts.begin(), b.edge_starts.end());
+    a.edge_ends.insert(a.edge_ends.end(), b.edge_ends.begin(), b.edge_ends.end());
+    for (int32_t o : b.edge_objects) a.edge_objects.push_back(o + obj_base);
+}
+
+// Mixed query set: each mesh is an ellipsoid (50%, level uniform in
+// [level_min, level_max], radii [radius_min, radius_max]), an axis-aligned
+// cube (30%, edge U[1, 10]) or a branching tube (20%, radius U[0.5, 3],
+// lognormal triangle counts from the tube fields).  Object ids: ellipsoids,
+// then cubes, then tubes.
+static pz_status generate_mix(const pz_gen_params& p, pz_host_mesh& m) {
+    uint64_t n_kind[3] = {0, 0, 0};
+    for (uint64_t i = 0; i < p.count; ++i) {
+        Rng r(p.seed ^ 0x6d69787175657279ull, i);
+        const double u = r.uniform();
+        ++n_kind[u < 0.5 ? 0 : (u < 0.8 ? 1 : 2)];
+    }
+    const int32_t sub_kind[3] = {PZ_GEN_ELLIPSOIDS, PZ_GEN_CUBES, PZ_GEN_TUBES};
+    int32_t obj_base = 0;
+    for (int k = 0; k < 3; ++k) {
+        if (n_kind[k] == 0) continue;
+        pz_gen_params q = p;
+        q.kind = sub_kind[k];
+        q.count = n_kind[k];
+        q.seed = p.seed * 3 + 1 + (uint64_t)k;
+        q.edges_mode = p.edges_mode == PZ_EDGES_PRE ? PZ_EDGES_NONE : p.edges_mode;
+        q.copy_fraction = 0.f;
+        q.degenerate_fraction = 0.f;
+        if (k == 1) {
+            q.radius_min = 1.f;
+            q.radius_max = 10.f;
+        } else if (k == 2) {
+            q.radius_min = 0.5f;
+            q.radius_max = 3.f;
+        }
+        pz_host_mesh sub;
+        pz_status st = k == 2 ? pz_gen_tubes_impl(q, sub) : generate_impl(q, sub);
+        if (st != PZ_OK) return st;
+        append_mesh(m, sub, obj_base);
+        obj_base += (int32_t)n_kind[k];
+    }
+    if (m.num_vertices() >= (1ull << 32)) return pz::fail(PZ_E_LIMIT, "more than 2^32 vertices in one set");
+    if (p.edges_mode == PZ_EDGES_PRE) pz::extract_edges_pre_order(m.view(), m.edge_starts, m.edge_ends, m.edge_objects);
+    return PZ_OK;
+}
+
 extern "C" pz_status pz_generate(const pz_gen_params* params, pz_host_mesh** out) {
     if (!params || !out) return pz::fail(PZ_E_INVALID, "pz_generate: null argument");
+    if (params->kind < PZ_GEN_ELLIPSOIDS || params->kind > PZ_GEN_QUERY_MIX)
+        return pz::fail(PZ_E_INVALID, "unknown generator kind");
     auto* m = new pz_host_mesh();
-    pz_status st = params->kind == PZ_GEN_TUBES ? pz_gen_tubes_impl(*params, *m) : generate_impl(*params, *m);
+    pz_status st = params->kind == PZ_GEN_TUBES       ? pz_gen_tubes_impl(*params, *m)
+                   : params->kind == PZ_GEN_QUERY_MIX ? generate_mix(*params, *m)
+                                                      : generate_impl(*params, *m);
     if (st != PZ_OK) {
         delete m;
         return st;

@@ -1,15 +1,151 @@
-// Internal declarations shared by the host translation units.
+// Internal declarations shared by the host translation units (compiled by
+// hipcc): context and dataset state, device buffer pool, error plumbing, and
+// the per-set statistics both the joins and the size estimator start from.
 #pragma once
 
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <chrono>
+#include <initializer_list>
+#include <new>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "host/host_mesh.hpp"
 #include "pierce_hip.h"
 
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+};
+
+struct pz_ctx {
+    int device = 0;
+    hipStream_t s = nullptr;
+    std::unordered_map<std::string, DevBuf> bufs;
+    void* pinned = nullptr;  // small readbacks
+    size_t pinned_n = 0;
+    hipEvent_t ev[PZ_MAX_PHASES + 1];
+};
+
+struct pz_dataset {
+    pz_ctx* ctx = nullptr;
+    float* v = nullptr;
+    uint32_t* idx = nullptr;
+    int32_t* t2o = nullptr;
+    float* es = nullptr;
+    float* ee = nullptr;
+    int32_t* eo = nullptr;
+    uint64_t nv = 0, nt = 0, ne = 0;
+    int32_t nobj = 0;
+    float max_abs = 0.f;
+    std::vector<pz_grid_cell> grid;
+    float cell_size = 0.f;
+    // batched datasets: part p owns objects [part_obj_off[p], part_obj_off[p + 1])
+    std::vector<uint64_t> part_obj_off;
+    uint32_t* part_of = nullptr;   // device: part id per object
+    uint32_t* part_off = nullptr;  // device: part_obj_off as uint32
+};
+
 namespace pz {
-// Learned output-size prediction for a join (mlp.cpp).
-uint64_t mlp_predict_count(pz_ctx* ctx, const pz_mlp* model, const pz_dataset* a, const pz_dataset* b,
-                           int predicate);
-// Sum over the parts of a batch of the predicted candidate-pair counts.
-uint64_t mlp_predict_batch_total(pz_ctx* ctx, const pz_mlp* model, const pz_dataset* a, const pz_dataset* b);
+
+#define HIP_OK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            throw PzError(e_ == hipErrorOutOfMemory ? PZ_E_NOMEM : PZ_E_HIP,                           \
+                          std::string(#expr) + ": " + hipGetErrorString(e_));                          \
+    } while (0)
+
+struct PzError {
+    pz_status code;
+    std::string msg;
+    PzError(pz_status c, std::string m) : code(c), msg(std::move(m)) {}
+};
+
+template <class T>
+inline T* buf(pz_ctx* c, const std::string& name, size_t count) {
+    size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+    DevBuf& b = c->bufs[name];
+    if (b.n < bytes) {
+        if (b.p) HIP_OK(hipFree(b.p));
+        b.p = nullptr;
+        b.n = 0;
+        size_t want = bytes + bytes / 4;
+        HIP_OK(hipMalloc(&b.p, want));
+        b.n = want;
+    }
+    return reinterpret_cast<T*>(b.p);
+}
+
+inline void* pinned(pz_ctx* c, size_t bytes) {
+    if (c->pinned_n < bytes) {
+        if (c->pinned) HIP_OK(hipHostFree(c->pinned));
+        c->pinned = nullptr;
+        HIP_OK(hipHostMalloc(&c->pinned, bytes, hipHostMallocDefault));
+        c->pinned_n = bytes;
+    }
+    return c->pinned;
+}
+
+// Read n scalars (each from a device address) in one synchronisation.
+template <class T>
+std::vector<T> readback(pz_ctx* c, std::initializer_list<const T*> srcs) {
+    T* h = reinterpret_cast<T*>(pinned(c, srcs.size() * sizeof(T)));
+    size_t i = 0;
+    for (const T* p : srcs) HIP_OK(hipMemcpyAsync(h + i++, p, sizeof(T), hipMemcpyDeviceToHost, c->s));
+    HIP_OK(hipStreamSynchronize(c->s));
+    return std::vector<T>(h, h + srcs.size());
+}
+
+inline int bits_for(uint64_t n) {  // bits needed to hold values < n
+    int b = 0;
+    while (b < 64 && (1ull << b) < n) ++b;
+    return std::max(b, 1);
+}
+
+inline double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <class F>
+pz_status guarded(F f) {
+    try {
+        return f();
+    } catch (const PzError& e) {
+        return fail(e.code, e.msg);
+    } catch (const std::bad_alloc&) {
+        return fail(PZ_E_NOMEM, "host allocation failed");
+    }
+}
+
+// per-object statistics of one set (device): ordered-uint boxes (lo xyz, hi xyz),
+// triangle counts, first / last triangle index, edge counts
+struct SetPrep {
+    uint32_t* obox;
+    uint32_t* tri_cnt;
+    uint32_t* first_tri;
+    uint32_t* last_tri;
+    uint32_t* edge_cnt;
+};
+SetPrep prep_set(pz_ctx* c, const pz_dataset* d, const std::string& tag);
+
+// predicate | direction bits << 2 of one join description
 uint8_t pair_meta_of(const pz_join_desc& d);
+
+// Learned output sizing (mlp.cpp).  One prediction per part (a plain dataset
+// is one part): point estimates and calibrated upper bounds of the
+// candidate-pair count (object pairs with overlapping boxes) and of the
+// result-pair count.
+struct SizePrediction {
+    double cand, pairs;
+    double cand_upper, pairs_upper;
+};
+// meta_of_part: device predicate | dir << 2 per part (batched) or null (meta for every part).
+std::vector<SizePrediction> mlp_predict(pz_ctx* c, const pz_mlp* model, const pz_dataset* a, const pz_dataset* b,
+                                        const SetPrep& pa, const SetPrep& pb, const uint8_t* meta_of_part,
+                                        uint8_t meta);
 }  // namespace pz
