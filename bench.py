@@ -105,6 +105,38 @@ def time_join(pz, ctx, da, db, steps, **desc):
     return dt, outs[-1]
 
 
+def stream_sizing(pz, ctx, model, n, batch=48):
+    """Held-out mixed-kind query stream: learned vs two-pass sizing of the batched join."""
+    from pierce_amd.stream import make_stream, run_stream
+
+    held = make_stream(n, seed0=50000)
+    run_stream(ctx, held[:batch], batch=batch, sizing=pz.SIZING_LEARNED, model=model)  # warm-up
+    exact, s_exact, _ = run_stream(ctx, held, batch=batch, sizing=pz.SIZING_TWO_PASS)
+    got, s_learned, _ = run_stream(ctx, held, batch=batch, sizing=pz.SIZING_LEARNED, model=model)
+    mism = sum(pair_mismatch(e.pairs, g.pairs) for e, g in zip(exact, got))
+    cand = np.array([e.num_candidates for e in exact], np.float64)
+    pairs = np.array([len(e.pairs) for e in exact], np.float64)
+    pc = np.array([g.predicted for g in got], np.float64)
+    pp = np.array([g.predicted_pairs for g in got], np.float64)
+    ms_exact = sum(s.ms_total for s in s_exact)
+    ms_learned = sum(s.ms_total for s in s_learned)
+    res = {"queries": n, "batch": batch,
+           "workload": "per query: set sizes lognormal(median 40, sigma 0.9) clipped to [2, 400]; 50% ellipsoids "
+                       "(subdivision 1-4), 30% cubes, 20% tubes (500-4000 tris); predicate uniform; density and "
+                       "B offset drawn per query",
+           "mismatched_pairs": int(mism),
+           "median_abs_log2_err_candidates": round(float(np.median(np.abs(np.log2(1 + pc) - np.log2(1 + cand)))), 3),
+           "median_abs_log2_err_pairs": round(float(np.median(np.abs(np.log2(1 + pp) - np.log2(1 + pairs)))), 3),
+           "query_capacity_covers_exact": round(float(np.mean([g.capacity >= e.num_candidates
+                                                               for e, g in zip(exact, got)])), 4),
+           "batch_overflow_rate": round(float(np.mean([s.overflowed for s in s_learned])), 4),
+           "device_ms_two_pass": round(ms_exact, 3), "device_ms_learned": round(ms_learned, 3),
+           "queries_per_s_two_pass": round(n / (ms_exact * 1e-3), 1),
+           "queries_per_s_learned": round(n / (ms_learned * 1e-3), 1)}
+    log(f"[bench] stream of {n}: learned {ms_learned:.1f} ms vs two-pass {ms_exact:.1f} ms, mismatches {mism}")
+    return res
+
+
 def pair_mismatch(p, q):
     a = set(map(tuple, p.tolist()))
     b = set(map(tuple, q.tolist()))
@@ -121,6 +153,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-side", type=float, default=500.0, help="side of the CPU sample sub-box")
+    ap.add_argument("--no-learned", action="store_true", help="two-pass sizing only (no estimator training)")
+    ap.add_argument("--train-queries", type=int, default=768, help="queries of the estimator's training stream")
+    ap.add_argument("--stream-queries", type=int, default=480, help="held-out queries of the sizing stream (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device three times (scale of the HBM byte counters)")
     args = ap.parse_args()
@@ -151,12 +186,32 @@ def main():
                   E=A.edge_objects.shape[0] + B.edge_objects.shape[0])
 
     with pz.Context(0) as ctx:
+        model, learned = None, None
+        if not args.no_learned:
+            # Learned sizing: the MLP is trained on the GPU on (features, exact two-pass counts)
+            # of a mixed-kind query stream, before and outside every timed region.
+            from pierce_amd.stream import make_stream, training_data
+
+            t = time.perf_counter()
+            X, T, _ = training_data(ctx, make_stream(args.train_queries, seed0=1000), batch=48)
+            t1 = time.perf_counter()
+            model = pz.Model(ctx, seed=4)
+            losses = model.train(X, T)
+            margins = model.get()[2]
+            learned = {"train_queries": args.train_queries, "stream_and_exact_joins_s": round(t1 - t, 2),
+                       "train_s": round(time.perf_counter() - t1, 3), "epochs": int(len(losses)),
+                       "loss_first_epoch": round(float(losses[0]), 4), "loss_last_epoch": round(float(losses[-1]), 4),
+                       "log2_margins": [round(float(x), 3) for x in margins]}
+            log(f"[bench] estimator trained on {args.train_queries} queries: loss {losses[0]:.3f} -> {losses[-1]:.4f}")
         da, db = ctx.upload(A), ctx.upload(B)
         log("[bench] uploaded; counting candidate tests")
         n_cand = pz.count_candidate_tests(ctx, da, db)
         log(f"[bench] N_cand = {n_cand}")
 
-        chosen = dict(strategy=pz.STRATEGY_TWO_LEVEL, precision=pz.PREC_FP32_FILTERED, sizing=pz.SIZING_TWO_PASS)
+        chosen = dict(strategy=pz.STRATEGY_TWO_LEVEL, precision=pz.PREC_FP32_FILTERED,
+                      sizing=pz.SIZING_LEARNED if model else pz.SIZING_TWO_PASS)
+        if model:
+            chosen["model"] = model
         plain = dict(strategy=pz.STRATEGY_GLOBAL, precision=pz.PREC_FP64, sizing=pz.SIZING_TWO_PASS,
                      dedup=pz.DEDUP_SORT)
         for _ in range(args.warmup):
@@ -184,6 +239,27 @@ def main():
             mism = pair_mismatch(out.pairs, pout.pairs)
             plain_phases = [(n, round(m, 3)) for n, m in pout.phases]
             log(f"[bench] plain {ms_plain:.3f} ms/join, pairs={len(pout.pairs)}, mismatches={mism}")
+
+        if model:
+            learned["c2_candidates_predicted"] = out.predicted
+            learned["c2_candidates_exact"] = out.num_candidates
+            learned["c2_capacity"] = out.capacity
+            learned["c2_overflowed"] = out.overflowed
+            learned["c2_pairs_predicted"] = out.predicted_pairs
+            if not args.no_plain:
+                # the reference-shaped path with the learned hash-set sizing instead of two passes
+                glearn = dict(strategy=pz.STRATEGY_GLOBAL, precision=pz.PREC_FP64, sizing=pz.SIZING_LEARNED,
+                              dedup=pz.DEDUP_HASH, model=model)
+                pz.join(ctx, da, db, **glearn)
+                ms_gl, gout = time_join(pz, ctx, da, db, args.plain_steps or args.steps, **glearn)
+                learned["global_fp64_learned_hash_ms"] = round(ms_gl, 3)
+                learned["global_fp64_two_pass_ms"] = round(ms_plain, 3)
+                learned["global_learned_speedup"] = round(ms_plain / ms_gl, 3)
+                learned["global_learned_mismatches"] = pair_mismatch(out.pairs, gout.pairs)
+                learned["global_learned_overflowed"] = gout.overflowed
+                log(f"[bench] global fp64 with learned hash sizing {ms_gl:.3f} ms/join (two-pass {ms_plain:.3f})")
+            if args.stream_queries > 0:
+                learned["stream"] = stream_sizing(pz, ctx, model, args.stream_queries)
 
         # Roofline of the dominant profiled kernel (largest device time in the timed steps).
         counts.update(IT=out.index_triangles, IE=out.index_edges)
@@ -243,7 +319,8 @@ def main():
                                f"(1280 tris), 20% of B scaled copies of A, centres in 1000^3",
                    "objects_per_set": args.objects, "triangles_per_set": counts["T"] // 2,
                    "edges_per_set": counts["E"] // 2, "predicate": "overlap",
-                   "chosen": "two-level, fp32-filtered exact, two-pass sizing",
+                   "chosen": "two-level, fp32-filtered exact, " + (
+                       "learned sizing (bf16 MLP) with two-pass fallback" if model else "two-pass sizing"),
                    "plain": "set-wide BVH, fp64-filtered exact, two-pass count/scan/write/sort/unique"},
         "join_ms_chosen": round(ms_chosen, 4),
         "join_ms_plain": None if ms_plain is None else round(ms_plain, 4),
@@ -261,6 +338,7 @@ def main():
         "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kernels.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "learned_sizing": learned,
     }
     print(json.dumps(line), flush=True)
 

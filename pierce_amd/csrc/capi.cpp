@@ -284,8 +284,11 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
             float margin = d.margin > 0 ? d.margin : 0.25f;
             cap = (uint64_t)std::ceil(pred * (1.0 + margin)) + 64;
         } else if (d.sizing == PZ_SIZING_LEARNED) {
-            cap = (uint64_t)std::ceil(std::min(learned_upper, 0x1p50)) + 64;
+            cap = (uint64_t)std::ceil(std::min(learned_upper, 0x1p40)) + 64;
         }
+        // never more slots than object pairs (or 2^32): a larger count overflows into the exact recount
+        cap = std::min<uint64_t>(cap, std::min<uint64_t>((uint64_t)std::max(nA, 1) * (uint64_t)std::max(nB, 1) + 64,
+                                                         1ull << 32));
         cap = std::max<uint64_t>(cap, 1);
         r->capacity = cap;
         pairs = buf<uint64_t>(c, "bp.pairs", cap + 1);
@@ -607,7 +610,7 @@ void join_global(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const pz_j
             slots = (uint64_t)std::ceil((double)expected / lf);
         // power-of-two table; estimate-sized tables keep a 1024-slot floor
         uint64_t cap2 = d.sizing == PZ_SIZING_FIXED ? 16 : 1024;
-        while (cap2 < slots && cap2 < (1ull << 34)) cap2 <<= 1;
+        while (cap2 < slots && cap2 < (1ull << 31)) cap2 <<= 1;  // at most 2^31 slots (16 GiB)
         r->capacity = cap2;
         unsigned long long* table = buf<unsigned long long>(c, "g.table", cap2);
         pzk::fill_ull(table, ~0ull, cap2, s);

@@ -105,52 +105,87 @@ __device__ inline uint32_t wave_max_u32(uint32_t v) {
     return v;
 }
 
-// Box, counts and triangle totals of one set per part (wave-aggregated atomics).
+// Each wavefront walks a contiguous chunk of object ids.  Objects of one part
+// are contiguous, so a wave keeps lane-local running sums for the current
+// part and flushes them (wave reduction + one atomic per field) only when the
+// part changes and at the end: a few atomics per wave instead of one per
+// object on the same per-part address.
+constexpr int kFeatChunk = 256;
+inline unsigned feat_grid(int32_t nobj) {
+    const uint64_t waves = ((uint64_t)nobj + kFeatChunk - 1) / kFeatChunk;
+    return (unsigned)((waves * 64 + kBlock - 1) / kBlock);
+}
+
+// Box, counts and triangle totals of one set per part.
 __global__ __launch_bounds__(kBlock) void k_feat_bounds(const uint32_t* __restrict__ obox,
                                                         const uint32_t* __restrict__ tri, int32_t nobj,
                                                         const uint32_t* __restrict__ part_of, int set,
                                                         PartAcc* __restrict__ acc) {
-    const int32_t o = blockIdx.x * kBlock + threadIdx.x;
-    const bool valid = o < nobj && tri[o] > 0;
-    const uint32_t p = (valid && part_of) ? part_of[o] : 0u;
-    uint32_t lo[3], hi[3];
-    for (int k = 0; k < 3; ++k) {
-        lo[k] = valid ? obox[6ll * o + k] : 0xFFFFFFFFu;
-        hi[k] = valid ? obox[6ll * o + 3 + k] : 0u;
-    }
-    const uint32_t t = valid ? tri[o] : 0u;
-    const uint64_t vm = __ballot(valid);
-    if (vm == 0) return;
-    const int leader = __ffsll((long long)vm) - 1;
-    const uint32_t p0 = (uint32_t)__shfl((int)p, leader, 64);
-    if (__ballot(valid && p == p0) == vm) {
+    const int lane = __lane_id();
+    const int64_t begin = (int64_t)((blockIdx.x * kBlock + threadIdx.x) / 64) * kFeatChunk;
+    const int64_t end = min(begin + (int64_t)kFeatChunk, (int64_t)nobj);
+    uint32_t run_p = 0xFFFFFFFFu;  // wave-uniform part of the running sums
+    uint32_t lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u}, cnt = 0u, tmax = 0u;
+    unsigned long long tsum = 0ull;
+    auto flush = [&]() {  // called by the whole wave
+        if (run_p == 0xFFFFFFFFu) return;
         uint32_t l[3], h[3];
         for (int k = 0; k < 3; ++k) {
             l[k] = wave_min_u32(lo[k]);
             h[k] = wave_max_u32(hi[k]);
+            lo[k] = 0xFFFFFFFFu;
+            hi[k] = 0u;
         }
-        const unsigned long long ts = wave_sum_u64(t);
-        const uint32_t tm = wave_max_u32(t);
-        if (__lane_id() == leader) {
-            PartAcc& A = acc[2 * p0 + set];
+        const unsigned long long n = wave_sum_u64(cnt), ts = wave_sum_u64(tsum);
+        const uint32_t tm = wave_max_u32(tmax);
+        if (lane == 0 && n > 0) {
+            PartAcc& A = acc[2 * run_p + set];
             for (int k = 0; k < 3; ++k) {
                 atomicMin(&A.lo[k], l[k]);
                 atomicMax(&A.hi[k], h[k]);
             }
-            atomicAdd(&A.n, (uint32_t)__popcll(vm));
+            atomicAdd(&A.n, (uint32_t)n);
             atomicAdd(&A.tris, ts);
             atomicMax(&A.max_tris, tm);
         }
-    } else if (valid) {
-        PartAcc& A = acc[2 * p + set];
-        for (int k = 0; k < 3; ++k) {
-            atomicMin(&A.lo[k], lo[k]);
-            atomicMax(&A.hi[k], hi[k]);
+        cnt = tmax = 0u;
+        tsum = 0ull;
+        run_p = 0xFFFFFFFFu;
+    };
+    for (int64_t base = begin; base < end; base += 64) {
+        const int64_t o = base + lane;
+        const bool valid = o < end && tri[o] > 0;
+        const uint32_t p = (valid && part_of) ? part_of[o] : 0u;
+        const uint64_t vm = __ballot(valid);
+        if (vm == 0) continue;
+        const uint32_t p0 = (uint32_t)__shfl((int)p, __ffsll((long long)vm) - 1, 64);
+        const bool uniform = __ballot(valid && p == p0) == vm;
+        if (!uniform || p0 != run_p) {
+            flush();
+            if (uniform) run_p = p0;
         }
-        atomicAdd(&A.n, 1u);
-        atomicAdd(&A.tris, (unsigned long long)t);
-        atomicMax(&A.max_tris, t);
+        if (!valid) continue;
+        const uint32_t t = tri[o];
+        if (uniform) {
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = min(lo[k], obox[6ll * o + k]);
+                hi[k] = max(hi[k], obox[6ll * o + 3 + k]);
+            }
+            ++cnt;
+            tsum += t;
+            tmax = max(tmax, t);
+        } else {  // a part boundary inside these 64 objects
+            PartAcc& A = acc[2 * p + set];
+            for (int k = 0; k < 3; ++k) {
+                atomicMin(&A.lo[k], obox[6ll * o + k]);
+                atomicMax(&A.hi[k], obox[6ll * o + 3 + k]);
+            }
+            atomicAdd(&A.n, 1u);
+            atomicAdd(&A.tris, (unsigned long long)t);
+            atomicMax(&A.max_tris, t);
+        }
     }
+    flush();
 }
 
 // Per-cell counts and extents (cell of the box centre), per-part extent sums
@@ -159,52 +194,76 @@ __global__ __launch_bounds__(kBlock) void k_feat_cells(const uint32_t* __restric
                                                        const uint32_t* __restrict__ tri, int32_t nobj,
                                                        const uint32_t* __restrict__ part_of, int set,
                                                        PartAcc* __restrict__ acc, CellAcc* __restrict__ cells) {
-    const int32_t o = blockIdx.x * kBlock + threadIdx.x;
-    bool valid = o < nobj && tri[o] > 0;
-    const uint32_t p = (valid && part_of) ? part_of[o] : 0u;
-    unsigned long long q[3] = {0ull, 0ull, 0ull};
-    uint32_t in_r = 0;
-    if (valid) {
-        const PartAcc& A = acc[2 * p];
-        const PartAcc& B = acc[2 * p + 1];
-        valid = A.n > 0 && B.n > 0;
-        if (valid) {
-            const Frame f = part_frame(A, B);
-            int cell[3];
-            bool meets = !f.r_empty;
-            for (int k = 0; k < 3; ++k) {
-                const float lo = ord2f(obox[6ll * o + k]), hi = ord2f(obox[6ll * o + 3 + k]);
-                const float c = (lo + hi) * 0.5f;
-                const float u = (c - f.ulo[k]) / f.cs[k];
-                cell[k] = min(max((int)floorf(u), 0), kFeatGrid - 1);
-                const float e = fminf((hi - lo) / f.cs[k], kFix);
-                q[k] = (unsigned long long)(e * kFix + 0.5f);
-                if (lo > f.rhi[k] || hi < f.rlo[k]) meets = false;
-            }
-            in_r = meets ? 1u : 0u;
-            CellAcc& C = cells[(2ull * p + set) * kFeatCells + (cell[2] * kFeatGrid + cell[1]) * kFeatGrid + cell[0]];
-            atomicAdd(&C.cnt, 1u);
-            for (int k = 0; k < 3; ++k) atomicAdd(&C.ext[k], q[k]);
+    const int lane = __lane_id();
+    const int64_t begin = (int64_t)((blockIdx.x * kBlock + threadIdx.x) / 64) * kFeatChunk;
+    const int64_t end = min(begin + (int64_t)kFeatChunk, (int64_t)nobj);
+    uint32_t run_p = 0xFFFFFFFFu;
+    unsigned long long rq[3] = {0ull, 0ull, 0ull}, rin = 0ull;
+    auto flush = [&]() {
+        if (run_p == 0xFFFFFFFFu) return;
+        unsigned long long sq[3];
+        for (int k = 0; k < 3; ++k) {
+            sq[k] = wave_sum_u64(rq[k]);
+            rq[k] = 0ull;
         }
-    }
-    const uint64_t vm = __ballot(valid);
-    if (vm == 0) return;
-    const int leader = __ffsll((long long)vm) - 1;
-    const uint32_t p0 = (uint32_t)__shfl((int)p, leader, 64);
-    if (__ballot(valid && p == p0) == vm) {
-        unsigned long long s[3];
-        for (int k = 0; k < 3; ++k) s[k] = wave_sum_u64(q[k]);
-        const unsigned long long r = wave_sum_u64(in_r);
-        if (__lane_id() == leader) {
-            PartAcc& A = acc[2 * p0 + set];
-            for (int k = 0; k < 3; ++k) atomicAdd(&A.ext[k], s[k]);
+        const unsigned long long r = wave_sum_u64(rin);
+        rin = 0ull;
+        if (lane == 0) {
+            PartAcc& A = acc[2 * run_p + set];
+            for (int k = 0; k < 3; ++k) atomicAdd(&A.ext[k], sq[k]);
             atomicAdd(&A.n_in_r, r);
         }
-    } else if (valid) {
-        PartAcc& A = acc[2 * p + set];
-        for (int k = 0; k < 3; ++k) atomicAdd(&A.ext[k], q[k]);
-        atomicAdd(&A.n_in_r, (unsigned long long)in_r);
+        run_p = 0xFFFFFFFFu;
+    };
+    for (int64_t base = begin; base < end; base += 64) {
+        const int64_t o = base + lane;
+        bool valid = o < end && tri[o] > 0;
+        const uint32_t p = (valid && part_of) ? part_of[o] : 0u;
+        unsigned long long q[3] = {0ull, 0ull, 0ull};
+        uint32_t in_r = 0;
+        if (valid) {
+            const PartAcc& A = acc[2 * p];
+            const PartAcc& B = acc[2 * p + 1];
+            valid = A.n > 0 && B.n > 0;
+            if (valid) {
+                const Frame f = part_frame(A, B);
+                int cell[3];
+                bool meets = !f.r_empty;
+                for (int k = 0; k < 3; ++k) {
+                    const float lo = ord2f(obox[6ll * o + k]), hi = ord2f(obox[6ll * o + 3 + k]);
+                    const float c = (lo + hi) * 0.5f;
+                    const float u = (c - f.ulo[k]) / f.cs[k];
+                    cell[k] = min(max((int)floorf(u), 0), kFeatGrid - 1);
+                    const float e = fminf((hi - lo) / f.cs[k], kFix);
+                    q[k] = (unsigned long long)(e * kFix + 0.5f);
+                    if (lo > f.rhi[k] || hi < f.rlo[k]) meets = false;
+                }
+                in_r = meets ? 1u : 0u;
+                CellAcc& C =
+                    cells[(2ull * p + set) * kFeatCells + (cell[2] * kFeatGrid + cell[1]) * kFeatGrid + cell[0]];
+                atomicAdd(&C.cnt, 1u);
+                for (int k = 0; k < 3; ++k) atomicAdd(&C.ext[k], q[k]);
+            }
+        }
+        const uint64_t vm = __ballot(valid);
+        if (vm == 0) continue;
+        const uint32_t p0 = (uint32_t)__shfl((int)p, __ffsll((long long)vm) - 1, 64);
+        const bool uniform = __ballot(valid && p == p0) == vm;
+        if (!uniform || p0 != run_p) {
+            flush();
+            if (uniform) run_p = p0;
+        }
+        if (!valid) continue;
+        if (uniform) {
+            for (int k = 0; k < 3; ++k) rq[k] += q[k];
+            rin += in_r;
+        } else {
+            PartAcc& A = acc[2 * p + set];
+            for (int k = 0; k < 3; ++k) atomicAdd(&A.ext[k], q[k]);
+            atomicAdd(&A.n_in_r, (unsigned long long)in_r);
+        }
     }
+    flush();
 }
 
 // Probability that two intervals of half-width-sum d (in cell units) whose
@@ -549,16 +608,16 @@ void join_features(const uint32_t* oboxA, const uint32_t* triA, int32_t nA, cons
     hipLaunchKernelGGL(k_zero_u64, dim3(grid1(zero_n)), dim3(kBlock), 0, s,
                        reinterpret_cast<unsigned long long*>(sc.cells), zero_n);
     if (nA > 0)
-        hipLaunchKernelGGL(k_feat_bounds, dim3(grid1((uint64_t)nA)), dim3(kBlock), 0, s, oboxA, triA, nA, partA, 0,
+        hipLaunchKernelGGL(k_feat_bounds, dim3(feat_grid(nA)), dim3(kBlock), 0, s, oboxA, triA, nA, partA, 0,
                            sc.acc);
     if (nB > 0)
-        hipLaunchKernelGGL(k_feat_bounds, dim3(grid1((uint64_t)nB)), dim3(kBlock), 0, s, oboxB, triB, nB, partB, 1,
+        hipLaunchKernelGGL(k_feat_bounds, dim3(feat_grid(nB)), dim3(kBlock), 0, s, oboxB, triB, nB, partB, 1,
                            sc.acc);
     if (nA > 0)
-        hipLaunchKernelGGL(k_feat_cells, dim3(grid1((uint64_t)nA)), dim3(kBlock), 0, s, oboxA, triA, nA, partA, 0,
+        hipLaunchKernelGGL(k_feat_cells, dim3(feat_grid(nA)), dim3(kBlock), 0, s, oboxA, triA, nA, partA, 0,
                            sc.acc, sc.cells);
     if (nB > 0)
-        hipLaunchKernelGGL(k_feat_cells, dim3(grid1((uint64_t)nB)), dim3(kBlock), 0, s, oboxB, triB, nB, partB, 1,
+        hipLaunchKernelGGL(k_feat_cells, dim3(feat_grid(nB)), dim3(kBlock), 0, s, oboxB, triB, nB, partB, 1,
                            sc.acc, sc.cells);
     hipLaunchKernelGGL(k_feat_hist, dim3(grid1((uint64_t)nparts * kFeatCells)), dim3(kBlock), 0, s, sc.cells, nparts,
                        sc.hist);

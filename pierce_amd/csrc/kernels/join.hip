@@ -167,6 +167,9 @@ __global__ __launch_bounds__(kBlock) void k_object_keys(const uint32_t* __restri
 // Thread per mesh1 object: walk the mesh2 object BVH with its box.
 // mode 0: count into cnt[a]; mode 1: write at off[a]; mode 2: append with
 // capacity (cursor counts every pair, writes only below cap).
+// Mode 0: count pairs per mesh1 object; 1: write them at off[a]; 2: append
+// into a capacity-sized buffer (one cursor reservation per wavefront: each
+// lane counts first, the wave reserves its total, then lanes write).
 template <int Mode>
 __global__ __launch_bounds__(kBlock) void k_broad(const uint32_t* __restrict__ oboxA, int32_t nA,
                                                   const uint32_t* __restrict__ oboxB,
@@ -175,42 +178,31 @@ __global__ __launch_bounds__(kBlock) void k_broad(const uint32_t* __restrict__ o
                                                   const uint32_t* __restrict__ boff, uint32_t* __restrict__ cnt,
                                                   const uint64_t* __restrict__ off, uint64_t* __restrict__ out,
                                                   unsigned long long* __restrict__ cursor, uint64_t cap) {
-    int32_t a = blockIdx.x * kBlock + threadIdx.x;
-    if (a >= nA) return;
+    const int32_t a = blockIdx.x * kBlock + threadIdx.x;
+    if (Mode != 2 && a >= nA) return;  // append keeps every lane for the wave-wide reservation
+    bool active = a < nA;
     // mesh2 objects of this object's part occupy sorted positions [b0, b0 + nb)
-    const uint32_t part = part_of_a ? part_of_a[a] : 0u;
+    const uint32_t part = (active && part_of_a) ? part_of_a[a] : 0u;
     const int32_t b0 = boff ? (int32_t)boff[part] : 0;
     const int32_t nb = boff ? (int32_t)(boff[part + 1] - boff[part]) : nB;
-    if (nb <= 0) {
-        if (Mode == 0) cnt[a] = 0;
-        return;
-    }
-    float lo[3], hi[3];
-    for (int k = 0; k < 3; ++k) {
-        lo[k] = ord2f(oboxA[6ll * a + k]);
-        hi[k] = ord2f(oboxA[6ll * a + 3 + k]);
-    }
-    if (!(lo[0] <= hi[0] && lo[1] <= hi[1] && lo[2] <= hi[2])) {
-        if (Mode == 0) cnt[a] = 0;
-        return;
-    }
-    uint32_t n = 0;
-    uint64_t base = Mode == 1 ? off[a] : 0;
-    auto emit = [&](int32_t b) {
-        if (Mode == 1) out[base + n] = pack_pair(a, b);
-        if (Mode == 2) {
-            unsigned long long slot = atomicAdd(cursor, 1ull);
-            if (slot < cap) out[slot] = pack_pair(a, b);
+    float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {-1.f, -1.f, -1.f};
+    if (active) {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = ord2f(oboxA[6ll * a + k]);
+            hi[k] = ord2f(oboxA[6ll * a + 3 + k]);
         }
-        ++n;
-    };
-    if (nb == 1) {
-        // single object: no internal node; test its box directly
-        const int32_t b = (int32_t)orderB[b0];
-        const uint32_t* bb = oboxB + 6ll * b;
-        if (boxes_meet(lo, hi, ord2f(bb[0]), ord2f(bb[1]), ord2f(bb[2]), ord2f(bb[3]), ord2f(bb[4]), ord2f(bb[5])))
-            emit(b);
-    } else {
+    }
+    active = active && nb > 0 && lo[0] <= hi[0] && lo[1] <= hi[1] && lo[2] <= hi[2];
+    // Visit every mesh2 object whose box meets this object's box.
+    auto traverse = [&](auto&& emit) {
+        if (nb == 1) {  // single object: no internal node; test its box directly
+            const int32_t b = (int32_t)orderB[b0];
+            const uint32_t* bb = oboxB + 6ll * b;
+            if (boxes_meet(lo, hi, ord2f(bb[0]), ord2f(bb[1]), ord2f(bb[2]), ord2f(bb[3]), ord2f(bb[4]),
+                           ord2f(bb[5])))
+                emit(b);
+            return;
+        }
         int stack[64];
         int sp = 0;
         int cur = b0;
@@ -218,7 +210,7 @@ __global__ __launch_bounds__(kBlock) void k_broad(const uint32_t* __restrict__ o
             const Node nd = nodesB[cur];
             bool h0 = boxes_meet(lo, hi, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
             bool h1 = boxes_meet(lo, hi, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w);
-            int c0 = nd.d.x, c1 = nd.d.y;
+            const int c0 = nd.d.x, c1 = nd.d.y;
             if (h0 && c0 < 0) {
                 emit((int32_t)orderB[~c0]);
                 h0 = false;
@@ -239,8 +231,37 @@ __global__ __launch_bounds__(kBlock) void k_broad(const uint32_t* __restrict__ o
                 cur = stack[--sp];
             }
         }
+    };
+    uint32_t n = 0;
+    if (Mode == 0) {
+        if (active) traverse([&](int32_t) { ++n; });
+        cnt[a] = n;
+    } else if (Mode == 1) {
+        if (active) {
+            const uint64_t base = off[a];
+            traverse([&](int32_t b) { out[base + n++] = pack_pair(a, b); });
+        }
+    } else {
+        if (active) traverse([&](int32_t) { ++n; });
+        const int lane = __lane_id();
+        uint32_t incl = n;  // inclusive prefix over the wave
+        for (int sft = 1; sft < PZ_WAVE; sft <<= 1) {
+            const uint32_t u = (uint32_t)__shfl_up((int)incl, sft, PZ_WAVE);
+            if (lane >= sft) incl += u;
+        }
+        const uint32_t total = (uint32_t)__shfl((int)incl, PZ_WAVE - 1, PZ_WAVE);
+        unsigned long long wbase = 0ull;
+        if (lane == 0 && total > 0) wbase = atomicAdd(cursor, (unsigned long long)total);
+        wbase = __shfl(wbase, 0, PZ_WAVE);
+        const uint64_t base = wbase + incl - n;
+        if (active && n > 0 && base < cap) {
+            uint32_t i = 0;
+            traverse([&](int32_t b) {
+                if (base + i < cap) out[base + i] = pack_pair(a, b);
+                ++i;
+            });
+        }
     }
-    if (Mode == 0) cnt[a] = n;
 }
 
 // Per-pair predicate and direction mask: meta = predicate | dir_mask << 2
