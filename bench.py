@@ -31,26 +31,27 @@ sys.path.insert(0, ROOT)
 METRIC = "join ms + Gtri-tests/s, chosen vs plain GPU path, 0 mismatched pairs"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes of one launch of each profiled kernel, from the .pre payload
-# units (12 B per vertex, 12 + 4 B per triangle, 28 B per edge) and the index
-# sizes the join reports.  Keys: kernel name -> f(counts) -> bytes per launch.
+# Algorithmic bytes of each profiled kernel in one join (all its launches: one
+# per set, or one per size class and set), from the .pre payload units (12 B
+# per vertex, 12 + 4 B per triangle, 28 B per edge) and the index sizes the
+# join reports.  Keys: kernel name -> f(counts) -> bytes per join.
 def alg_bytes(kernel, k):
-    if kernel == "object_stats":    # indices + map per triangle, each vertex once; one launch per set
-        return (16 * k["T"] + 12 * k["V"]) / 2
-    if kernel == "edge_count":      # source-object ids; one launch per set
-        return 4 * k["E"] / 2
+    if kernel == "object_stats":    # indices + map per triangle, each vertex once
+        return 16 * k["T"] + 12 * k["V"]
+    if kernel == "edge_count":      # source-object ids
+        return 4 * k["E"]
     if kernel == "tri_keys":        # payload read + (key, triangle) written for indexed triangles
-        return (16 * k["T"] + 12 * k["V"] + 12 * k["IT"]) / 2
+        return 16 * k["T"] + 12 * k["V"] + 12 * k["IT"]
     if kernel == "edge_scatter":    # ids of all edges + endpoints read and 32-B records written for indexed edges
-        return (4 * k["E"] + 24 * k["IE"] + 32 * k["IE"]) / 2
+        return 4 * k["E"] + 24 * k["IE"] + 32 * k["IE"]
     if kernel == "refit":           # per indexed triangle: order, key, indices, 3 vertices read; one node written
-        return (4 + 8 + 12 + 36 + 64) * k["IT"] / 2
+        return (4 + 8 + 12 + 36 + 64) * k["IT"]
     if kernel == "gather_leaves":   # per indexed triangle: order, indices, 3 vertices, map read; 48 B written
-        return (4 + 12 + 36 + 4 + 48) * k["IT"] / 2
+        return (4 + 12 + 36 + 4 + 48) * k["IT"]
     if kernel == "karras":          # keys read (~neighbour loads), node children + parents written
-        return (8 + 16 + 8) * k["IT"] / 2
+        return (8 + 16 + 8) * k["IT"]
     if kernel == "build_small":     # per indexed triangle: indices + 3 vertices read, one node + one leaf written
-        return (12 + 36 + 64 + 48) * k["IT"] / 2
+        return (12 + 36 + 64 + 48) * k["IT"]
     if kernel == "narrow_dual":     # every indexed triangle (48 B) and internal node (64 B) of both objects read once
         return (48 + 64) * k["IT"]
     if kernel == "narrow":          # every indexed edge record read once
@@ -61,7 +62,7 @@ def alg_bytes(kernel, k):
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
 # workload, scaled by a 1 GiB copy (fetch x 2.0, write x 1.0); kernel -> bytes.
 MEASURED_TRAFFIC = {
-    "build_small": 0.760e9 + 2.883e9,
+    "build_small": 1.095e9 + 3.041e9,
 }
 
 
@@ -216,6 +217,9 @@ def main():
                      dedup=pz.DEDUP_SORT)
         for _ in range(args.warmup):
             pz.join(ctx, da, db, **chosen)
+        # autotuner: workgroup size of the LDS index builder per size class, on this workload (untimed)
+        tune = pz.autotune(ctx, da, db, **chosen)
+        log(f"[bench] autotune: {tune['build_threads']} ({tune['joins_run']} joins)")
         log("[bench] chosen path warm; timing")
         pz.profile_reset()
         pz.profile_enable(True)
@@ -223,7 +227,7 @@ def main():
         pz.profile_enable(False)
         kernels = {}
         for k in ("object_stats", "edge_count", "tri_keys", "edge_scatter", "karras", "refit", "gather_leaves",
-                  "radix_scatter", "build_small", "narrow_dual", "narrow", "parity"):
+                  "radix_scatter", "build_small", "narrow_dual", "narrow", "parity", "features", "mlp_infer"):
             tot, n = pz.profile_query(k)
             if n:
                 kernels[k] = (tot, n)
@@ -270,6 +274,7 @@ def main():
             avg_ms = tot / n
             b = alg_bytes(dom, counts)
             if b is not None:
+                b = b * args.steps / n  # per launch
                 ach = b / (avg_ms * 1e-3) / 1e9
                 roofline = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
                             "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
@@ -339,6 +344,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "learned_sizing": learned,
+        "autotune": tune,
     }
     print(json.dumps(line), flush=True)
 

@@ -333,6 +333,38 @@ pz_status pz_mlp_gradients(pz_ctx* ctx, const pz_mlp* model, const float* X, con
 pz_status pz_mlp_train(pz_ctx* ctx, pz_mlp* model, const float* X, const float* T, uint64_t n,
                        const pz_mlp_train_desc* desc, float* epoch_loss);
 
+/* ---------------------------------------------------------------- tuning */
+/* Launch parameters picked per size class.  The per-object LDS index builder
+ * runs one launch per size class of objects (at most 128, 512 and 2048
+ * triangles); each class has its own workgroup size.  No reference
+ * counterpart (OptiX picks its own build parameters). */
+typedef struct {
+    int32_t build_threads[3];   /* 128, 256, 512 or 1024 */
+    int32_t narrow_queue[2];    /* dual-tree narrow phase: node pairs queued in LDS per wave
+                                   (256, 512, 1024 or 1536; overflowing pairs go to the per-edge kernel),
+                                   [0] for joins whose objects all have <= 2048 triangles, [1] otherwise */
+    int32_t reserved[11];
+} pz_tune;
+
+typedef struct {
+    pz_tune chosen;
+    double  build_ms[3][4];     /* class x candidate {128, 256, 512, 1024}: device ms of that class's
+                                   launches in one join; < 0 when the workload has no object of the class */
+    double  narrow_ms[4];       /* queue {256, 512, 1024, 1536}: device ms of the narrow phase kernels */
+    int32_t joins_run;
+    int32_t queue_class;        /* which narrow_queue entry this workload tuned */
+    int32_t reserved[6];
+} pz_tune_report;
+
+pz_status pz_tune_get(const pz_ctx* ctx, pz_tune* out);
+pz_status pz_tune_set(pz_ctx* ctx, const pz_tune* in);
+/* Autotuner: after one warm-up join, runs the join once per candidate
+ * workgroup size and times every size class's builder launches with HIP
+ * events, keeping the fastest per class; then once per queue length, keeping
+ * the fastest narrow phase.  The choices stay in the context. */
+pz_status pz_autotune(pz_ctx* ctx, const pz_dataset* mesh1, const pz_dataset* mesh2, const pz_join_desc* desc,
+                      pz_tune_report* report);
+
 /* Number of (edge, triangle) pairs over both directions whose closed fp32
  * boxes overlap (the BVH-independent candidate-test count). */
 pz_status pz_count_candidate_tests(pz_ctx* ctx, const pz_dataset* mesh1, const pz_dataset* mesh2,

@@ -107,6 +107,19 @@ class MlpTrainDesc(C.Structure):
 
 _FP = C.POINTER(C.c_float)
 
+
+class Tune(C.Structure):
+    _fields_ = [("build_threads", C.c_int32 * 3), ("narrow_queue", C.c_int32 * 2), ("reserved", C.c_int32 * 11)]
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("chosen", Tune), ("build_ms", (C.c_double * 4) * 3), ("narrow_ms", C.c_double * 4),
+                ("joins_run", C.c_int32), ("queue_class", C.c_int32), ("reserved", C.c_int32 * 6)]
+
+
+TUNE_CANDIDATES = (128, 256, 512, 1024)
+QUEUE_CANDIDATES = (256, 512, 1024, 1536)
+
 # Every function declared in include/pierce_hip.h: (name, restype, argtypes).
 _SIGS = [
     ("pz_last_error", C.c_char_p, []),
@@ -143,6 +156,9 @@ _SIGS = [
     ("exclusive_scan_gpu", C.c_longlong, [C.c_void_p, C.c_void_p, C.c_int]),
     ("pz_estimate_overlap_selectivity", C.c_float,
      [C.POINTER(GridCell), C.POINTER(GridCell), C.c_int32, C.c_float, C.c_float, C.c_float]),
+    ("pz_tune_get", C.c_int32, [C.c_void_p, C.POINTER(Tune)]),
+    ("pz_tune_set", C.c_int32, [C.c_void_p, C.POINTER(Tune)]),
+    ("pz_autotune", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(JoinDesc), C.POINTER(TuneReport)]),
     ("pz_mlp_train_desc_defaults", None, [C.POINTER(MlpTrainDesc)]),
     ("pz_mlp_create", C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
     ("pz_mlp_destroy", None, [C.c_void_p]),
@@ -601,3 +617,31 @@ def batch_features(ctx: Context, a: Dataset, b: Dataset, predicates) -> np.ndarr
 def log_counts(counts) -> np.ndarray:
     """Training target of a count: log2(1 + count)."""
     return np.log2(1.0 + np.asarray(counts, dtype=np.float64)).astype(np.float32)
+
+
+# ------------------------------------------------------------------- tuning
+def tune_get(ctx: Context) -> dict:
+    t = Tune()
+    _check(lib().pz_tune_get(ctx._h, C.byref(t)))
+    return {"build_threads": list(t.build_threads), "narrow_queue": list(t.narrow_queue)}
+
+
+def tune_set(ctx: Context, build_threads, narrow_queue=(1536, 1536)) -> None:
+    t = Tune()
+    t.build_threads[:] = list(build_threads)
+    t.narrow_queue[:] = list(narrow_queue)
+    _check(lib().pz_tune_set(ctx._h, C.byref(t)))
+
+
+def autotune(ctx: Context, a: Dataset, b: Dataset, **kw) -> dict:
+    """Pick the LDS builder's workgroup size per size class on this workload."""
+    d = join_desc(**kw)
+    r = TuneReport()
+    _check(lib().pz_autotune(ctx._h, a._h, b._h, C.byref(d), C.byref(r)))
+    return {"build_threads": list(r.chosen.build_threads), "narrow_queue": list(r.chosen.narrow_queue),
+            "queue_class": int(r.queue_class),
+            "joins_run": int(r.joins_run),
+            "narrow_ms": {str(QUEUE_CANDIDATES[i]): round(r.narrow_ms[i], 4) for i in range(4)},
+            "build_ms": {f"class{k}": {str(TUNE_CANDIDATES[i]): (None if r.build_ms[k][i] < 0
+                                                              else round(r.build_ms[k][i], 4))
+                                      for i in range(4)} for k in range(3)}}

@@ -331,6 +331,7 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
     pzk::mark_needed(pairs, np, na.need, nb.need, s);
     struct Tot {
         uint64_t nneed, ntri, nedge, nbig, nbig_tri;
+        uint32_t ncls[pzk::kSmallClasses], cls_max[pzk::kSmallClasses];  // LDS-built objects per size class
     };
     // Needed objects of one set: objects built by the global pipeline ("big":
     // more than kSmallMax triangles or not stored contiguously) take the first
@@ -341,17 +342,19 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         uint32_t* stc = buf<uint32_t>(c, tag + ".seg_tri_cnt", n);
         uint32_t* sec = buf<uint32_t>(c, tag + ".seg_edge_cnt", n);
         uint32_t* big = buf<uint32_t>(c, tag + ".big", n);
-        uint32_t* small = buf<uint32_t>(c, tag + ".small", n);
+        uint32_t* cls = buf<uint32_t>(c, tag + ".cls", pzk::kSmallClasses * (n + 1));
         uint32_t* loc_big = buf<uint32_t>(c, tag + ".loc_big", n + 1);
-        uint32_t* loc_small = buf<uint32_t>(c, tag + ".loc_small", n + 1);
-        uint32_t* totals = buf<uint32_t>(c, tag + ".totals", 4);
+        uint32_t* loc_cls = buf<uint32_t>(c, tag + ".loc_cls", pzk::kSmallClasses * (n + 1));
+        uint32_t* totals = buf<uint32_t>(c, tag + ".totals", 8);
         pzk::fill_u32(stc, 0u, n, s);
         pzk::fill_u32(sec, 0u, n, s);
+        pzk::fill_u32(totals, 0u, 8, s);
         void* tmp = buf<char>(c, "scan.tmp", pzk::scan_temp_bytes(n + 1));
-        pzk::need_split(N.need, P.tri_cnt, P.first_tri, P.last_tri, D->nobj, big, small, s);
+        pzk::need_split(N.need, P.tri_cnt, P.first_tri, P.last_tri, D->nobj, big, cls, totals, s);
         pzk::exclusive_scan_u32_u32(big, loc_big, no, tmp, s);
-        pzk::exclusive_scan_u32_u32(small, loc_small, no, tmp, s);
-        pzk::merge_locs(big, loc_big, loc_small, D->nobj, N.loc, totals, s);
+        for (int k = 0; k < pzk::kSmallClasses; ++k)
+            pzk::exclusive_scan_u32_u32(cls + (size_t)k * (no + 1), loc_cls + (size_t)k * (no + 1), no, tmp, s);
+        pzk::merge_locs(big, loc_big, loc_cls, D->nobj, N.loc, totals, s);
         pzk::needed_layout(N.need, N.loc, D->nobj, P.tri_cnt, P.edge_cnt, N.obj_of_loc, stc, sec, s);
         pzk::exclusive_scan_u32_u32(stc, N.seg_off, no, tmp, s);
         pzk::big_prefix(N.seg_off, totals, s);
@@ -359,11 +362,26 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
     };
     uint32_t* bigA = layout(A, pa, na, "A");
     uint32_t* bigB = layout(B, pb, nb, "B");
-    uint32_t* totA = buf<uint32_t>(c, "A.totals", 4);
-    uint32_t* totB = buf<uint32_t>(c, "B.totals", 4);
-    auto tot = readback<uint32_t>(c, {na.loc + std::max(nA, 0), na.seg_off + std::max(nA, 0), totA, totA + 1,
-                                      nb.loc + std::max(nB, 0), nb.seg_off + std::max(nB, 0), totB, totB + 1});
-    Tot ta{tot[0], tot[1], 0, tot[2], tot[3]}, tb{tot[4], tot[5], 0, tot[6], tot[7]};
+    uint32_t* totA = buf<uint32_t>(c, "A.totals", 8);
+    uint32_t* totB = buf<uint32_t>(c, "B.totals", 8);
+    // needed objects and their triangles, then totals[0..7] of each set, in one synchronisation
+    uint32_t* hb = static_cast<uint32_t*>(pinned(c, 20 * sizeof(uint32_t)));
+    HIP_OK(hipMemcpyAsync(hb + 0, na.loc + std::max(nA, 0), 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(hb + 1, na.seg_off + std::max(nA, 0), 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(hb + 2, totA, 32, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(hb + 10, nb.loc + std::max(nB, 0), 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(hb + 11, nb.seg_off + std::max(nB, 0), 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(hb + 12, totB, 32, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    auto tot_of = [&](const uint32_t* h) {
+        Tot t{h[0], h[1], 0, h[2], h[3], {}, {}};
+        for (int k = 0; k < pzk::kSmallClasses; ++k) {
+            t.ncls[k] = h[4 + k];
+            t.cls_max[k] = h[7 + k];
+        }
+        return t;
+    };
+    const Tot ta = tot_of(hb), tb = tot_of(hb + 10);
 
     auto build = [&](const pz_dataset* D, const SetPrep& P, const Needed& N, const Tot& T, uint32_t* big,
                      const std::string& tag) -> SetAccel {
@@ -395,8 +413,16 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
                                  acc.height, s);
             pzk::gather_leaf_tris(order, nb_tri, D->v, D->idx, D->t2o, acc.leaves, s);
         }
-        pzk::build_small(D->v, D->idx, D->t2o, N.obj_of_loc, P.first_tri, P.tri_cnt, P.obox, N.seg_off,
-                         (uint32_t)T.nbig, (uint32_t)(T.nneed - T.nbig), acc.nodes, acc.leaves, acc.height, s);
+        static const char* const kTimer[pzk::kSmallClasses] = {"build_small.c0", "build_small.c1",
+                                                                "build_small.c2"};
+        uint32_t l0 = (uint32_t)T.nbig;
+        for (int k = 0; k < pzk::kSmallClasses; ++k) {  // one launch per size class
+            if (!pzk::build_small(D->v, D->idx, D->t2o, N.obj_of_loc, P.first_tri, P.tri_cnt, P.obox, N.seg_off, l0,
+                                  T.ncls[k], T.cls_max[k], c->tune.build_threads[k], acc.nodes, acc.leaves,
+                                  acc.height, kTimer[k], s))
+                throw PzError(PZ_E_INVALID, "invalid LDS-builder configuration");
+            l0 += T.ncls[k];
+        }
         acc.edges = nullptr;
         return acc;
     };
@@ -417,7 +443,11 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
     const uint64_t* fb_pairs = pairs;
     uint32_t* fb_index = nullptr;
     if (d.narrow_mode == 0) {
-        pzk::narrow_dual(pairs, np, va, vb, found, fallback, meta, f32, allow_f32, stats, s);
+        const int qclass = (ta.nbig + tb.nbig > 0) ? 1 : 0;  // pairs with objects beyond the LDS builder
+        c->last_queue_class = qclass;
+        if (!pzk::narrow_dual(pairs, np, va, vb, found, fallback, meta, f32, allow_f32,
+                              c->tune.narrow_queue[qclass], stats, s))
+            throw PzError(PZ_E_INVALID, "invalid narrow-phase queue length");
         uint64_t* fbp = buf<uint64_t>(c, "np.fb_pairs", np + 1);
         fb_index = buf<uint32_t>(c, "np.fb_index", np + 1);
         unsigned long long* fbc = buf<unsigned long long>(c, "np.fb_count", 1);
@@ -1021,6 +1051,101 @@ pz_status pz_join_batch(pz_ctx* c, const pz_dataset* a, const pz_dataset* b, uin
         std::free(all.pair_types);
         return PZ_OK;
     });
+}
+
+pz_status pz_tune_get(const pz_ctx* c, pz_tune* out) {
+    if (!c || !out) return fail(PZ_E_INVALID, "pz_tune_get: null argument");
+    std::memset(out, 0, sizeof(*out));
+    for (int k = 0; k < pzk::kSmallClasses; ++k) out->build_threads[k] = c->tune.build_threads[k];
+    out->narrow_queue[0] = c->tune.narrow_queue[0];
+    out->narrow_queue[1] = c->tune.narrow_queue[1];
+    return PZ_OK;
+}
+
+static bool valid_queue(int q) { return q == 256 || q == 512 || q == 1024 || q == 1536; }
+
+static bool valid_threads(int t) { return t == 128 || t == 256 || t == 512 || t == 1024; }
+
+pz_status pz_tune_set(pz_ctx* c, const pz_tune* in) {
+    if (!c || !in) return fail(PZ_E_INVALID, "pz_tune_set: null argument");
+    for (int k = 0; k < pzk::kSmallClasses; ++k)
+        if (!valid_threads(in->build_threads[k])) return fail(PZ_E_INVALID, "build_threads must be 128, 256, 512 or 1024");
+    if (!valid_queue(in->narrow_queue[0]) || !valid_queue(in->narrow_queue[1]))
+        return fail(PZ_E_INVALID, "narrow_queue must be 256, 512, 1024 or 1536");
+    for (int k = 0; k < pzk::kSmallClasses; ++k) c->tune.build_threads[k] = in->build_threads[k];
+    c->tune.narrow_queue[0] = in->narrow_queue[0];
+    c->tune.narrow_queue[1] = in->narrow_queue[1];
+    return PZ_OK;
+}
+
+pz_status pz_autotune(pz_ctx* c, const pz_dataset* a, const pz_dataset* b, const pz_join_desc* desc,
+                      pz_tune_report* report) {
+    if (!c || !a || !b || !desc || !report) return fail(PZ_E_INVALID, "pz_autotune: null argument");
+    if (desc->strategy != PZ_STRATEGY_TWO_LEVEL) return fail(PZ_E_INVALID, "pz_autotune tunes the two-level strategy");
+    std::memset(report, 0, sizeof(*report));
+    static const int kCand[4] = {128, 256, 512, 1024};
+    static const char* const kTimer[pzk::kSmallClasses] = {"build_small.c0", "build_small.c1", "build_small.c2"};
+    const TuneTable saved = c->tune;
+    const bool was_on = pzk::prof_enabled();
+    pz_join_result r;
+    auto run = [&]() -> pz_status {
+        pz_status st = pz_join(c, a, b, desc, &r);
+        if (st == PZ_OK) pz_join_result_release(&r);
+        ++report->joins_run;
+        return st;
+    };
+    pz_status st = run();  // warm-up: buffers grown, code loaded
+    for (int i = 0; i < 4 && st == PZ_OK; ++i) {  // LDS builder workgroup size, all classes at once
+        for (int k = 0; k < pzk::kSmallClasses; ++k) c->tune.build_threads[k] = kCand[i];
+        pzk::prof_reset();
+        pzk::prof_enable(true);
+        st = run();
+        pzk::prof_enable(false);
+        for (int k = 0; k < pzk::kSmallClasses; ++k) {
+            double ms = 0;
+            uint64_t n = 0;
+            pzk::prof_query(kTimer[k], &ms, &n);
+            report->build_ms[k][i] = n ? ms : -1.0;
+        }
+    }
+    c->tune = saved;
+    if (st == PZ_OK)
+        for (int k = 0; k < pzk::kSmallClasses; ++k) {
+            int best = -1;
+            for (int i = 0; i < 4; ++i)
+                if (report->build_ms[k][i] >= 0 && (best < 0 || report->build_ms[k][i] < report->build_ms[k][best]))
+                    best = i;
+            if (best >= 0) c->tune.build_threads[k] = kCand[best];
+            report->chosen.build_threads[k] = c->tune.build_threads[k];
+        }
+    // narrow phase: the dual-tree kernel plus the per-edge kernel its queue overflows go to
+    static const int kQueue[4] = {256, 512, 1024, 1536};
+    const TuneTable base = c->tune;
+    const int qc = c->last_queue_class;  // the warm-up join's class
+    for (int i = 0; i < 4 && st == PZ_OK; ++i) {
+        c->tune.narrow_queue[qc] = kQueue[i];
+        pzk::prof_reset();
+        pzk::prof_enable(true);
+        st = run();
+        pzk::prof_enable(false);
+        double ms = 0, ms2 = 0;
+        uint64_t n = 0;
+        pzk::prof_query("narrow_dual", &ms, &n);
+        pzk::prof_query("narrow", &ms2, &n);
+        report->narrow_ms[i] = ms + ms2;
+    }
+    pzk::prof_reset();
+    pzk::prof_enable(was_on);
+    c->tune = base;
+    if (st != PZ_OK) return st;
+    int best = 0;
+    for (int i = 1; i < 4; ++i)
+        if (report->narrow_ms[i] < report->narrow_ms[best]) best = i;
+    c->tune.narrow_queue[qc] = kQueue[best];
+    report->chosen.narrow_queue[0] = c->tune.narrow_queue[0];
+    report->chosen.narrow_queue[1] = c->tune.narrow_queue[1];
+    report->queue_class = qc;
+    return PZ_OK;
 }
 
 void pz_profile_enable(int32_t on) { pzk::prof_enable(on != 0); }

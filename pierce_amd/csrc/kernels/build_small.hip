@@ -1,13 +1,25 @@
 // Per-object LBVH built entirely in LDS, one workgroup per object.
 //
-// For an object of n <= kSmallMax triangles stored contiguously from
-// first_tri: 30-bit Morton codes of the centroids inside the object's box,
-// packed with the local index into distinct 64-bit keys; bitonic sort in LDS;
-// Karras hierarchy over the sorted keys; bottom-up refit with LDS counters;
-// then one coalesced write of the internal nodes and leaf triangles at the
-// object's segment [seg_off[l], seg_off[l] + n).  Compared with the global
-// pipeline (radix sort + Karras + atomic refit + gather), the triangles are
-// read from HBM once and every intermediate stays on chip.
+// For an object of n <= 2048 triangles stored contiguously from first_tri:
+//   1. 32-bit keys: a 21-bit Morton code of the centroid on a 127^3 grid over
+//      the object's own box (never all ones) above the 11-bit local index, so
+//      keys are distinct and the padding key 0xFFFFFFFF sorts last;
+//   2. bitonic sort: each wavefront keeps a 128-key chunk in registers
+//      (two keys per lane) and runs every stride <= 64 with lane shuffles;
+//      only strides >= 128 go through LDS with a block barrier (10 of the 66
+//      stages for 2048 keys);
+//   3. agglomerative bottom-up build (Apetrei 2014, "Fast and simple
+//      agglomerative LBVH construction"): every leaf climbs; a node covering
+//      sorted leaves [l, r] joins the neighbouring split with the smaller
+//      key XOR (split r, or l - 1); the first child to arrive at a split
+//      stops, the second merges both boxes, writes the finished node record
+//      straight to HBM and climbs on.  This builds the radix tree and refits
+//      it in one pass, with no per-leaf box array in LDS.
+//   4. the root must sit at the segment's first node (the traversals start
+//      there): it is written at index 0, and internal node 0's record moves
+//      to the root's split index with the one link into it patched.
+// LDS is dynamic, sized by the largest object of the launch: 4 B per key
+// (next power of two) + 33 B per internal node.
 #include "pz_device.h"
 #include "pz_kernels_join.h"
 
@@ -15,249 +27,284 @@ namespace {
 
 using namespace pzd;
 
-constexpr int kThreads = 1024;
 constexpr int kMax = (int)pzk::kSmallMax;
 
 __device__ __forceinline__ float3 vtx(const float* __restrict__ v, uint32_t i) {
     return make_float3(v[3ull * i], v[3ull * i + 1], v[3ull * i + 2]);
 }
 
-__device__ __forceinline__ void tri_box(const float* __restrict__ v, const uint32_t* __restrict__ idx, uint32_t t,
-                                        float* lo, float* hi) {
-    float3 a = vtx(v, idx[3ull * t]), b = vtx(v, idx[3ull * t + 1]), c = vtx(v, idx[3ull * t + 2]);
-    lo[0] = fminf(a.x, fminf(b.x, c.x));
-    lo[1] = fminf(a.y, fminf(b.y, c.y));
-    lo[2] = fminf(a.z, fminf(b.z, c.z));
-    hi[0] = fmaxf(a.x, fmaxf(b.x, c.x));
-    hi[1] = fmaxf(a.y, fmaxf(b.y, c.y));
-    hi[2] = fmaxf(a.z, fmaxf(b.z, c.z));
+// Size class of a needed object: 0..2 for the LDS builder, -1 for the global pipeline.
+__device__ __forceinline__ int small_class(uint32_t n) {
+    return n <= pzk::kClassMax[0] ? 0 : (n <= pzk::kClassMax[1] ? 1 : (n <= pzk::kClassMax[2] ? 2 : -1));
 }
 
 __global__ void k_need_split(const uint32_t* __restrict__ need, const uint32_t* __restrict__ tri_cnt,
                              const uint32_t* __restrict__ first_tri, const uint32_t* __restrict__ last_tri,
-                             int32_t nobj, uint32_t* __restrict__ big, uint32_t* __restrict__ small) {
+                             int32_t nobj, uint32_t* __restrict__ big, uint32_t* __restrict__ cls,
+                             uint32_t* __restrict__ totals) {
     int32_t o = blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= nobj) return;
-    uint32_t n = tri_cnt[o];
-    bool contiguous = n > 0 && last_tri[o] - first_tri[o] + 1 == n;
-    bool sm = need[o] && contiguous && n <= (uint32_t)kMax;
-    big[o] = (need[o] && !sm) ? 1u : 0u;
-    small[o] = sm ? 1u : 0u;
+    const uint32_t n = tri_cnt[o];
+    const bool contiguous = n > 0 && last_tri[o] - first_tri[o] + 1 == n;
+    const int c = (need[o] && contiguous) ? small_class(n) : -1;
+    big[o] = (need[o] && c < 0) ? 1u : 0u;
+    for (int k = 0; k < pzk::kSmallClasses; ++k) cls[(uint64_t)k * (nobj + 1) + o] = (c == k) ? 1u : 0u;
+    if (c >= 0) atomicMax(&totals[5 + c], n);
 }
 
+// loc = big ? loc_big : nbig + (objects of lower classes) + loc within the class.
 __global__ void k_merge_locs(const uint32_t* __restrict__ big, const uint32_t* __restrict__ loc_big,
-                             const uint32_t* __restrict__ loc_small, int32_t nobj, uint32_t* __restrict__ loc,
+                             const uint32_t* __restrict__ loc_cls, int32_t nobj, uint32_t* __restrict__ loc,
                              uint32_t* __restrict__ totals) {
     int32_t o = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nbig = loc_big[nobj];
-    if (o == 0) totals[0] = nbig;
+    uint32_t off[pzk::kSmallClasses + 1];
+    off[0] = nbig;
+    for (int k = 0; k < pzk::kSmallClasses; ++k) off[k + 1] = off[k] + loc_cls[(uint64_t)k * (nobj + 1) + nobj];
+    if (o == 0) {
+        totals[0] = nbig;
+        for (int k = 0; k < pzk::kSmallClasses; ++k) totals[2 + k] = off[k + 1] - off[k];
+    }
     if (o >= nobj) return;
-    loc[o] = big[o] ? loc_big[o] : nbig + loc_small[o];
-    if (o == nobj - 1) loc[nobj] = nbig + loc_small[nobj];
+    uint32_t v = loc_big[o];
+    if (!big[o])
+        for (int k = 0; k < pzk::kSmallClasses; ++k)
+            if (loc_cls[(uint64_t)k * (nobj + 1) + o + 1] != loc_cls[(uint64_t)k * (nobj + 1) + o])
+                v = off[k] + loc_cls[(uint64_t)k * (nobj + 1) + o];
+    loc[o] = v;
+    if (o == nobj - 1) loc[nobj] = off[pzk::kSmallClasses];
 }
 
 __global__ void k_big_prefix(const uint32_t* __restrict__ seg_off, uint32_t* __restrict__ totals) {
     if (threadIdx.x == 0 && blockIdx.x == 0) totals[1] = seg_off[totals[0]];
 }
 
-__device__ __forceinline__ int delta64(const uint64_t* key, int i, int j, int n) {
-    if (j < 0 || j >= n) return -1;
-    return __clzll((long long)(key[i] ^ key[j]));  // keys are distinct
-}
-
-__global__ __launch_bounds__(kThreads) void k_build_small(
+template <int T>
+__global__ __launch_bounds__(T) void k_build_small(
     const float* __restrict__ verts, const uint32_t* __restrict__ idx, const int32_t* __restrict__ t2o,
     const int32_t* __restrict__ obj_of_loc, const uint32_t* __restrict__ first_tri,
     const uint32_t* __restrict__ tri_cnt, const uint32_t* __restrict__ obox, const uint32_t* __restrict__ seg_off,
-    uint32_t l_begin, Node* __restrict__ nodes, LeafTri* __restrict__ leaves, uint32_t* __restrict__ height_out) {
-    __shared__ uint64_t key[kMax];                 // 16 KB
-    __shared__ float lbox[kMax][6];                // 48 KB: box of the leaf at sorted position p
-    __shared__ float nbox[kMax][6];                // 48 KB: box of internal node i
-    __shared__ int16_t ch[kMax][2];                // children: j >= 0 internal, < 0 leaf ~j
-    __shared__ uint16_t par_leaf[kMax], par_int[kMax];
-    __shared__ uint32_t cnt[kMax];
-    __shared__ uint8_t hgt[kMax];
-
+    uint32_t l_begin, int cap, Node* __restrict__ nodes, LeafTri* __restrict__ leaves, uint32_t* __restrict__ height_out
+) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    int cpow = 1;
+    while (cpow < cap) cpow <<= 1;
+    uint32_t* key = reinterpret_cast<uint32_t*>(lds);                       // cpow
+    float* nbox = reinterpret_cast<float*>(key + cpow);                    // (cap - 1) x 6
+    int* bound = reinterpret_cast<int*>(nbox + 6 * (cap > 1 ? cap - 1 : 1)); // cap - 1
+    int16_t* cid = reinterpret_cast<int16_t*>(bound + (cap > 1 ? cap - 1 : 1)); // (cap - 1) x 2
+    uint8_t* hgt = reinterpret_cast<uint8_t*>(cid + 2 * (cap > 1 ? cap - 1 : 1));   // cap - 1
+    __shared__ Node rec0;       // record of internal node 0 (moved to the root's index at the end)
+    __shared__ int p0_info;     // parent of node 0 << 1 | side, or -1
+    __shared__ int root_idx;
     const uint32_t l = l_begin + blockIdx.x;
     const int32_t o = obj_of_loc[l];
     const int n = (int)tri_cnt[o];
     const uint32_t t0 = first_tri[o];
     const uint32_t base = seg_off[l];
-    const int tid = threadIdx.x;
-
-    float lo[3], inv[3];
+    const int tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+    float lo[3], sc[3];
     for (int k = 0; k < 3; ++k) {
-        float bl = ord2f(obox[6ll * o + k]), bh = ord2f(obox[6ll * o + 3 + k]);
+        const float bl = ord2f(obox[6ll * o + k]), bh = ord2f(obox[6ll * o + 3 + k]);
         lo[k] = bl;
-        inv[k] = bh > bl ? 1.0f / (bh - bl) : 0.0f;
+        sc[k] = bh > bl ? 127.0f / (bh - bl) : 0.0f;
     }
     int npow = 1;
     while (npow < n) npow <<= 1;
-
-    // 1. keys
-    for (int p = tid; p < npow; p += kThreads) {
+    // 1. keys: Morton code of the centroid on a 127^3 grid (never all ones), local index below
+    for (int p = tid; p < npow; p += T) {
+        uint32_t kv = 0xFFFFFFFFu;
         if (p < n) {
-            uint32_t t = t0 + (uint32_t)p;
-            float3 a = vtx(verts, idx[3ull * t]), b = vtx(verts, idx[3ull * t + 1]), c = vtx(verts, idx[3ull * t + 2]);
-            float cx = (a.x + b.x + c.x) * (1.0f / 3.0f), cy = (a.y + b.y + c.y) * (1.0f / 3.0f),
-                  cz = (a.z + b.z + c.z) * (1.0f / 3.0f);
-            uint32_t m = morton30(cx, cy, cz, lo[0], lo[1], lo[2], inv[0], inv[1], inv[2]);
-            key[p] = ((uint64_t)m << 32) | (uint32_t)p;
-        } else {
-            key[p] = ~0ull;
-        }
-    }
-    __syncthreads();
-    // 2. bitonic sort of npow keys (ascending).  Thread i owns compare-exchange
-    // pair i, so for strides <= 64 every pair of a wave lies in that wave's own
-    // 128-key segment and only stages with larger strides need a block barrier.
-    static_assert(kThreads * 2 >= kMax, "one compare-exchange pair per thread");
-    for (int size = 2; size <= npow; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            const bool cross_wave = stride > PZ_WAVE;
-            if (cross_wave) __syncthreads();
-            const int i = tid;
-            if (i < npow / 2) {
-                int lo_i = 2 * i - (i & (stride - 1));
-                int hi_i = lo_i + stride;
-                bool up = (lo_i & size) == 0;
-                uint64_t x = key[lo_i], y = key[hi_i];
-                if ((x > y) == up) {
-                    key[lo_i] = y;
-                    key[hi_i] = x;
-                }
-            }
-            if (cross_wave) __syncthreads();
-            else __builtin_amdgcn_wave_barrier();
-        }
-    }
-    __syncthreads();
-    // leaf boxes in sorted order (the only other global read of the triangles)
-    for (int p = tid; p < n; p += kThreads) {
-        float blo[3], bhi[3];
-        tri_box(verts, idx, t0 + (uint32_t)(key[p] & 0xFFFFFFFFu), blo, bhi);
-        for (int k = 0; k < 3; ++k) {
-            lbox[p][k] = blo[k];
-            lbox[p][3 + k] = bhi[k];
-        }
-    }
-    // 3. Karras hierarchy over sorted positions [0, n)
-    for (int i = tid; i < n - 1; i += kThreads) {
-        const int d = (delta64(key, i, i + 1, n) - delta64(key, i, i - 1, n)) >= 0 ? 1 : -1;
-        const int dmin = delta64(key, i, i - d, n);
-        int lmax = 2;
-        while (delta64(key, i, i + lmax * d, n) > dmin) lmax <<= 1;
-        int len = 0;
-        for (int t = lmax >> 1; t >= 1; t >>= 1)
-            if (delta64(key, i, i + (len + t) * d, n) > dmin) len += t;
-        const int j = i + len * d;
-        const int dnode = delta64(key, i, j, n);
-        int s = 0;
-        for (int div = 2;; div <<= 1) {
-            int t = (len + div - 1) / div;
-            if (delta64(key, i, i + (s + t) * d, n) > dnode) s += t;
-            if (t <= 1) break;
-        }
-        const int g = i + s * d + (d < 0 ? -1 : 0);
-        const int lo_j = min(i, j), hi_j = max(i, j);
-        if (lo_j == g) {
-            ch[i][0] = (int16_t)~g;
-            par_leaf[g] = (uint16_t)i;
-        } else {
-            ch[i][0] = (int16_t)g;
-            par_int[g] = (uint16_t)i;
-        }
-        if (hi_j == g + 1) {
-            ch[i][1] = (int16_t)~(g + 1);
-            par_leaf[g + 1] = (uint16_t)i;
-        } else {
-            ch[i][1] = (int16_t)(g + 1);
-            par_int[g + 1] = (uint16_t)i;
-        }
-        cnt[i] = 0;
-    }
-    __syncthreads();
-    // 4. refit: climb from every leaf; the second arrival at a node computes it.
-    uint32_t my_height = 0;
-    if (n >= 2) {
-        for (int p = tid; p < n; p += kThreads) {
-            float blo[3], bhi[3];
+            const uint32_t t = t0 + (uint32_t)p;
+            const uint32_t i0 = idx[3ull * t], i1 = idx[3ull * t + 1], i2 = idx[3ull * t + 2];
+            uint32_t q[3];
             for (int k = 0; k < 3; ++k) {
-                blo[k] = lbox[p][k];
-                bhi[k] = lbox[p][3 + k];
+                const float c = (verts[3ull * i0 + k] + verts[3ull * i1 + k] + verts[3ull * i2 + k]) * (1.0f / 3.0f);
+                const float f = fminf(fmaxf((c - lo[k]) * sc[k], 0.0f), 126.0f);
+                q[k] = (uint32_t)f;
             }
-            int node = par_leaf[p];
-            int child = ~p;
-            int h = 0;
-            while (true) {
-                __threadfence_block();
-                if (atomicAdd(&cnt[node], 1u) == 0) break;  // sibling still climbing
-                __threadfence_block();
-                const int sib = ch[node][0] == child ? ch[node][1] : ch[node][0];
-                float slo[3], shi[3];
-                int sh;
-                if (sib < 0) {
-                    for (int k = 0; k < 3; ++k) {
-                        slo[k] = lbox[~sib][k];
-                        shi[k] = lbox[~sib][3 + k];
-                    }
-                    sh = 0;
-                } else {
-                    for (int k = 0; k < 3; ++k) {
-                        slo[k] = nbox[sib][k];
-                        shi[k] = nbox[sib][3 + k];
-                    }
-                    sh = hgt[sib];
-                }
-                for (int k = 0; k < 3; ++k) {
-                    blo[k] = fminf(blo[k], slo[k]);
-                    bhi[k] = fmaxf(bhi[k], shi[k]);
-                    nbox[node][k] = blo[k];
-                    nbox[node][3 + k] = bhi[k];
-                }
-                h = 1 + max(h, sh);
-                hgt[node] = (uint8_t)h;
-                if (node == 0) {
-                    my_height = (uint32_t)h;
-                    break;
-                }
-                child = node;
-                node = par_int[node];
-            }
+            uint32_t m = 0;
+            for (int b = 6; b >= 0; --b)
+                m = (m << 3) | (((q[0] >> b) & 1u) << 2) | (((q[1] >> b) & 1u) << 1) | ((q[2] >> b) & 1u);
+            kv = (m << 11) | (uint32_t)p;
         }
+        key[p] = kv;
     }
     __syncthreads();
-    if (my_height) atomicMax(height_out, my_height);
-    // 5. write internal nodes and leaves at the object's segment
-    for (int i = tid; i < n - 1; i += kThreads) {
-        float b[2][6];
-        int gc[2];
-        for (int sl = 0; sl < 2; ++sl) {
-            const int c = ch[i][sl];
-            if (c < 0) {
-                for (int k = 0; k < 6; ++k) b[sl][k] = lbox[~c][k];
-                gc[sl] = ~(int)(base + (uint32_t)(~c));
-            } else {
-                for (int k = 0; k < 6; ++k) b[sl][k] = nbox[c][k];
-                gc[sl] = (int)(base + (uint32_t)c);
+    // 2. bitonic sort, ascending.  A wave holds a 128-key chunk in registers
+    // (lane L: positions c*128 + L and c*128 + 64 + L); strides <= 64 run with
+    // lane shuffles, strides >= 128 in LDS with block barriers.
+    auto reg_stages = [&](int size, int smax) {
+        for (int c = wave; c * 128 < npow; c += T / 64) {
+            const int p0 = c * 128 + lane, p1 = p0 + 64;
+            uint32_t k0 = key[p0], k1 = key[p1];
+            for (int sz = (smax ? size : 2); sz <= size; sz <<= 1) {
+                for (int st = min(sz >> 1, 64); st > 0; st >>= 1) {
+                    if (st == 64) {
+                        const bool up = (p0 & sz) == 0;
+                        const uint32_t a = min(k0, k1), b = max(k0, k1);
+                        k0 = up ? a : b;
+                        k1 = up ? b : a;
+                    } else {
+                        const bool lower = (lane & st) == 0;
+                        const uint32_t o0 = (uint32_t)__shfl_xor((int)k0, st, 64);
+                        const uint32_t o1 = (uint32_t)__shfl_xor((int)k1, st, 64);
+                        const bool up0 = (p0 & sz) == 0, up1 = (p1 & sz) == 0;
+                        k0 = (lower == up0) ? min(k0, o0) : max(k0, o0);
+                        k1 = (lower == up1) ? min(k1, o1) : max(k1, o1);
+                    }
+                }
             }
+            key[p0] = k0;
+            key[p1] = k1;
         }
-        Node nd;
-        nd.a = make_float4(b[0][0], b[0][1], b[0][2], b[0][3]);
-        nd.b = make_float4(b[0][4], b[0][5], b[1][0], b[1][1]);
-        nd.c = make_float4(b[1][2], b[1][3], b[1][4], b[1][5]);
-        nd.d = make_int4(gc[0], gc[1], 0, 0);
-        nodes[base + i] = nd;
+    };
+    if (npow >= 128) {
+        reg_stages(128, 0);  // sizes 2 .. 128 entirely in registers
+        for (int size = 256; size <= npow; size <<= 1) {
+            for (int st = size >> 1; st >= 128; st >>= 1) {
+                __syncthreads();
+                for (int i = tid; i < npow / 2; i += T) {
+                    const int a = 2 * i - (i & (st - 1)), b = a + st;
+                    const bool up = (a & size) == 0;
+                    const uint32_t x = key[a], y = key[b];
+                    if ((x > y) == up) {
+                        key[a] = y;
+                        key[b] = x;
+                    }
+                }
+            }
+            __syncthreads();
+            reg_stages(size, 1);  // strides 64 .. 1 of this size
+        }
+    } else {
+        for (int size = 2; size <= npow; size <<= 1)
+            for (int st = size >> 1; st > 0; st >>= 1) {
+                __syncthreads();
+                for (int i = tid; i < npow / 2; i += T) {
+                    const int a = 2 * i - (i & (st - 1)), b = a + st;
+                    const bool up = (a & size) == 0;
+                    const uint32_t x = key[a], y = key[b];
+                    if ((x > y) == up) {
+                        key[a] = y;
+                        key[b] = x;
+                    }
+                }
+            }
     }
-    for (int p = tid; p < n; p += kThreads) {
-        const uint32_t t = t0 + (uint32_t)(key[p] & 0xFFFFFFFFu);
-        float3 a = vtx(verts, idx[3ull * t]), b = vtx(verts, idx[3ull * t + 1]), c = vtx(verts, idx[3ull * t + 2]);
+    for (int i = tid; i < n - 1; i += T) bound[i] = -1;
+    if (tid == 0) {
+        p0_info = -1;
+        root_idx = 0;
+    }
+    __syncthreads();
+    // 3. agglomerative build: every leaf climbs; at each node the first child
+    // to arrive stops, the second merges both boxes, writes the node and goes on.
+    uint32_t my_h = 0;
+    auto leaf_box = [&](int p, float* blo, float* bhi) {
+        const uint32_t t = t0 + (key[p] & 0x7FFu);
+        const uint32_t i0 = idx[3ull * t], i1 = idx[3ull * t + 1], i2 = idx[3ull * t + 2];
+        for (int k = 0; k < 3; ++k) {
+            const float a = verts[3ull * i0 + k], b = verts[3ull * i1 + k], c = verts[3ull * i2 + k];
+            blo[k] = fminf(a, fminf(b, c));
+            bhi[k] = fmaxf(a, fmaxf(b, c));
+        }
+    };
+    auto delta = [&](int i) { return key[i] ^ key[i + 1]; };  // split i between sorted keys i and i+1
+    for (int p = tid; p < n; p += T) {
+        const uint32_t t = t0 + (key[p] & 0x7FFu);
+        const uint32_t i0 = idx[3ull * t], i1 = idx[3ull * t + 1], i2 = idx[3ull * t + 2];
+        const float3 a = vtx(verts, i0), b = vtx(verts, i1), c = vtx(verts, i2);
         LeafTri L;
         L.v0 = make_float4(a.x, a.y, a.z, __uint_as_float(t));
         L.v1 = make_float4(b.x, b.y, b.z, __int_as_float(t2o[t]));
         L.v2 = make_float4(c.x, c.y, c.z, 0.0f);
         leaves[base + p] = L;
+        if (n < 2) continue;
+        float blo[3] = {fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)), fminf(a.z, fminf(b.z, c.z))};
+        float bhi[3] = {fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)), fmaxf(a.z, fmaxf(b.z, c.z))};
+        int lft = p, rgt = p;
+        int me = ~p;  // leaf: ~position; internal: split index
+        int h = 0;
+        for (int guard = 0; guard < 64; ++guard) {  // every climb ends at the root within 64 levels
+            int parent, side;
+            if (lft == 0 || (rgt != n - 1 && delta(rgt) < delta(lft - 1))) {
+                parent = rgt;
+                side = 0;
+            } else {
+                parent = lft - 1;
+                side = 1;
+            }
+            cid[2 * parent + side] = (int16_t)me;
+            if (me >= 0) hgt[me] = (uint8_t)h;
+            // LDS-only ordering: the node records stored to HBM need no wait here
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+            const int other = atomicExch(&bound[parent], side == 0 ? lft : rgt);
+            if (other < 0) break;  // first to arrive: the sibling finishes this node
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+            const int sib = cid[2 * parent + (1 - side)];
+            float slo[3], shi[3];
+            int sh = 0;
+            if (sib < 0) {
+                leaf_box(~sib, slo, shi);
+            } else {
+                for (int k = 0; k < 3; ++k) {
+                    slo[k] = nbox[6 * sib + k];
+                    shi[k] = nbox[6 * sib + 3 + k];
+                }
+                sh = hgt[sib];
+            }
+            if (side == 0) rgt = other; else lft = other;
+            // node record: child 0 = left (side 0), child 1 = right
+            const float* L0 = side == 0 ? blo : slo; const float* H0 = side == 0 ? bhi : shi;
+            const float* L1 = side == 0 ? slo : blo; const float* H1 = side == 0 ? shi : bhi;
+            const int c0 = side == 0 ? me : sib, c1 = side == 0 ? sib : me;
+            Node nd;
+            nd.a = make_float4(L0[0], L0[1], L0[2], H0[0]);
+            nd.b = make_float4(H0[1], H0[2], L1[0], L1[1]);
+            nd.c = make_float4(L1[2], H1[0], H1[1], H1[2]);
+            nd.d = make_int4(c0, c1, 0, 0);  // local links, rebased below
+            for (int k = 0; k < 3; ++k) {
+                blo[k] = fminf(blo[k], slo[k]);
+                bhi[k] = fmaxf(bhi[k], shi[k]);
+                nbox[6 * parent + k] = blo[k];
+                nbox[6 * parent + 3 + k] = bhi[k];
+            }
+            h = 1 + max(h, sh);
+            // rebase links: leaves ~(base + pos), internal base + index (0 and the root swap places)
+            int g0 = c0 < 0 ? ~(int)(base + (uint32_t)(~c0)) : (int)(base + (uint32_t)c0);
+            int g1 = c1 < 0 ? ~(int)(base + (uint32_t)(~c1)) : (int)(base + (uint32_t)c1);
+            nd.d = make_int4(g0, g1, 0, 0);
+            if (c0 == 0) p0_info = (parent << 1) | 0;
+            if (c1 == 0) p0_info = (parent << 1) | 1;
+            me = parent;
+            const bool root = (lft == 0 && rgt == n - 1);
+            if (parent == 0) rec0 = nd;
+            else if (root) nodes[base] = nd;  // the root lives at the segment's first node
+            else nodes[base + parent] = nd;
+            if (root) {
+                root_idx = parent;
+                my_h = (uint32_t)h;
+                break;
+            }
+        }
     }
+    __syncthreads();
+    // 4. node 0's record goes to the root's index (unless node 0 is the root),
+    // and the link to node 0 in its parent is redirected there.
+    if (tid == 0 && n >= 2) {
+        const int r = root_idx;
+        if (r == 0) {
+            nodes[base] = rec0;
+        } else {
+            nodes[base + r] = rec0;
+            const int par = p0_info >> 1, sd = p0_info & 1;
+            // links into internal node 0 were written as base + 0: point them at base + r
+            int* link = reinterpret_cast<int*>(&nodes[par == r ? base : base + par].d) + sd;
+            *link = (int)(base + (uint32_t)r);
+        }
+    }
+    if (my_h) atomicMax(height_out, my_h);
 }
 
 inline unsigned grid1(uint64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
@@ -267,29 +314,49 @@ inline unsigned grid1(uint64_t n, int b = 256) { return (unsigned)((n + b - 1) /
 namespace pzk {
 
 void need_split(const uint32_t* need, const uint32_t* tri_cnt, const uint32_t* first_tri, const uint32_t* last_tri,
-                int32_t nobj, uint32_t* big, uint32_t* small, hipStream_t s) {
+                int32_t nobj, uint32_t* big, uint32_t* cls, uint32_t* totals, hipStream_t s) {
     if (nobj > 0)
         hipLaunchKernelGGL(k_need_split, dim3(grid1((uint64_t)nobj)), dim3(256), 0, s, need, tri_cnt, first_tri,
-                           last_tri, nobj, big, small);
+                           last_tri, nobj, big, cls, totals);
 }
 
-void merge_locs(const uint32_t* big, const uint32_t* loc_big, const uint32_t* loc_small, int32_t nobj, uint32_t* loc,
+void merge_locs(const uint32_t* big, const uint32_t* loc_big, const uint32_t* loc_cls, int32_t nobj, uint32_t* loc,
                 uint32_t* totals, hipStream_t s) {
     hipLaunchKernelGGL(k_merge_locs, dim3(grid1((uint64_t)std::max(nobj, 1))), dim3(256), 0, s, big, loc_big,
-                       loc_small, nobj, loc, totals);
+                       loc_cls, nobj, loc, totals);
 }
 
 void big_prefix(const uint32_t* seg_off, uint32_t* totals, hipStream_t s) {
     hipLaunchKernelGGL(k_big_prefix, dim3(1), dim3(64), 0, s, seg_off, totals);
 }
 
-void build_small(const float* verts, const uint32_t* idx, const int32_t* t2o, const int32_t* obj_of_loc,
+size_t build_small_lds(uint32_t cap) {
+    uint32_t cpow = 1;
+    while (cpow < cap) cpow <<= 1;
+    return 4ull * cpow + 33ull * std::max<uint32_t>(cap, 2) + 16;
+}
+
+bool build_small(const float* verts, const uint32_t* idx, const int32_t* t2o, const int32_t* obj_of_loc,
                  const uint32_t* first_tri, const uint32_t* tri_cnt, const uint32_t* obox, const uint32_t* seg_off,
-                 uint32_t l_begin, uint32_t count, Node* nodes, LeafTri* leaves, uint32_t* height, hipStream_t s) {
-    if (count == 0) return;
-    KernelTimer kt_("build_small", s);
-    hipLaunchKernelGGL(k_build_small, dim3(count), dim3(kThreads), 0, s, verts, idx, t2o, obj_of_loc, first_tri,
-                       tri_cnt, obox, seg_off, l_begin, nodes, leaves, height);
+                 uint32_t l_begin, uint32_t count, uint32_t cap, int threads, Node* nodes, LeafTri* leaves,
+                 uint32_t* height, const char* timer, hipStream_t s) {
+    if (count == 0) return true;
+    if (cap == 0 || cap > pzk::kSmallMax) return false;
+    const size_t lds = build_small_lds(cap);
+    KernelTimer kt_(timer ? timer : "build_small", s);
+    KernelTimer kt_all_("build_small", s);
+#define PZ_LAUNCH_BS(T)                                                                                          \
+    hipLaunchKernelGGL(k_build_small<T>, dim3(count), dim3(T), lds, s, verts, idx, t2o, obj_of_loc, first_tri, \
+                       tri_cnt, obox, seg_off, l_begin, (int)cap, nodes, leaves, height)
+    switch (threads) {
+        case 128: PZ_LAUNCH_BS(128); break;
+        case 256: PZ_LAUNCH_BS(256); break;
+        case 512: PZ_LAUNCH_BS(512); break;
+        case 1024: PZ_LAUNCH_BS(1024); break;
+        default: return false;
+    }
+#undef PZ_LAUNCH_BS
+    return true;
 }
 
 }  // namespace pzk

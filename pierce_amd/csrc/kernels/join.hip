@@ -29,67 +29,108 @@ __device__ __forceinline__ float3 vtx(const float* __restrict__ v, uint32_t i) {
 }
 
 // ------------------------------------------------------------ object stats
-// Per triangle: object box (ordered-uint atomics), triangle count and first
-// triangle per object; lanes of a wave that share one object reduce first.
+// Per object: box (ordered-uint atomics), triangle count, first and last
+// triangle.  Each wavefront walks a contiguous run of kStatsRun triangles and
+// keeps lane-local running sums for the object it is on; when the 64
+// triangles of a step all belong to that object they just accumulate, and the
+// sums are flushed (wave reduction + one atomic per field) only when the
+// object changes.  Objects' triangles are usually contiguous, so a run of
+// 1,024 triangles costs a couple of flushes instead of one set of atomics
+// per 64 triangles.
+constexpr int kStatsRun = 1024;
+
 __global__ __launch_bounds__(kBlock) void k_object_stats(const float* __restrict__ verts, uint64_t nv,
                                                          const uint32_t* __restrict__ idx,
                                                          const int32_t* __restrict__ t2o, uint64_t nt, int32_t nobj,
                                                          uint32_t* __restrict__ obox, uint32_t* __restrict__ tri_cnt,
                                                          uint32_t* __restrict__ first_tri,
                                                          uint32_t* __restrict__ last_tri) {
-    uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    bool valid = t < nt;
-    int32_t o = -1;
+    const int lane = __lane_id();
+    const uint64_t begin = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) / PZ_WAVE * kStatsRun;
+    const uint64_t end = begin + kStatsRun < nt ? begin + kStatsRun : nt;
+    int32_t run_o = -1;  // wave-uniform object of the running sums
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    if (valid) {
-        o = t2o[t];
-        uint32_t i0 = idx[3 * t], i1 = idx[3 * t + 1], i2 = idx[3 * t + 2];
-        valid = o >= 0 && o < nobj && i0 < nv && i1 < nv && i2 < nv;
-        if (valid) {
-            float3 a = vtx(verts, i0), b = vtx(verts, i1), c = vtx(verts, i2);
-            lo[0] = fminf(a.x, fminf(b.x, c.x));
-            lo[1] = fminf(a.y, fminf(b.y, c.y));
-            lo[2] = fminf(a.z, fminf(b.z, c.z));
-            hi[0] = fmaxf(a.x, fmaxf(b.x, c.x));
-            hi[1] = fmaxf(a.y, fmaxf(b.y, c.y));
-            hi[2] = fmaxf(a.z, fmaxf(b.z, c.z));
-        }
-    }
-    uint64_t vm = __ballot(valid);
-    if (vm == 0) return;
-    int leader = __ffsll((long long)vm) - 1;
-    int32_t o0 = __shfl(o, leader, 64);
-    bool uniform = __ballot(valid && o == o0) == vm;
-    if (uniform) {
+    uint32_t cnt = 0, tmin = 0xFFFFFFFFu, tmax = 0u;
+    auto flush = [&]() {  // whole wave
+        if (run_o < 0) return;
+        float l[3], h[3];
         for (int a = 0; a < 3; ++a) {
-            float l = lo[a], h = hi[a];
-            for (int s = 32; s > 0; s >>= 1) {
-                l = fminf(l, __shfl_xor(l, s, 64));
-                h = fmaxf(h, __shfl_xor(h, s, 64));
+            l[a] = lo[a];
+            h[a] = hi[a];
+            for (int sft = 32; sft > 0; sft >>= 1) {
+                l[a] = fminf(l[a], __shfl_xor(l[a], sft, 64));
+                h[a] = fmaxf(h[a], __shfl_xor(h[a], sft, 64));
             }
-            lo[a] = l;
-            hi[a] = h;
+            lo[a] = INFINITY;
+            hi[a] = -INFINITY;
         }
-        uint32_t tmin = (uint32_t)__shfl((int)(uint32_t)t, leader, 64);
-        uint32_t tmax = (uint32_t)__shfl((int)(uint32_t)t, 63 - __clzll((long long)vm), 64);
-        if (__lane_id() == leader) {
+        uint32_t c = cnt, mn = tmin, mx = tmax;
+        for (int sft = 32; sft > 0; sft >>= 1) {
+            c += (uint32_t)__shfl_xor((int)c, sft, 64);
+            mn = min(mn, (uint32_t)__shfl_xor((int)mn, sft, 64));
+            mx = max(mx, (uint32_t)__shfl_xor((int)mx, sft, 64));
+        }
+        if (lane == 0) {
             for (int a = 0; a < 3; ++a) {
-                atomicMin(&obox[6ull * o0 + a], f2ord(lo[a]));
-                atomicMax(&obox[6ull * o0 + 3 + a], f2ord(hi[a]));
+                atomicMin(&obox[6ull * run_o + a], f2ord(l[a]));
+                atomicMax(&obox[6ull * run_o + 3 + a], f2ord(h[a]));
             }
-            atomicAdd(&tri_cnt[o0], (uint32_t)__popcll(vm));
-            atomicMin(&first_tri[o0], tmin);
-            atomicMax(&last_tri[o0], tmax);
+            atomicAdd(&tri_cnt[run_o], c);
+            atomicMin(&first_tri[run_o], mn);
+            atomicMax(&last_tri[run_o], mx);
         }
-    } else if (valid) {
-        for (int a = 0; a < 3; ++a) {
-            atomicMin(&obox[6ull * o + a], f2ord(lo[a]));
-            atomicMax(&obox[6ull * o + 3 + a], f2ord(hi[a]));
+        cnt = 0;
+        tmin = 0xFFFFFFFFu;
+        tmax = 0u;
+        run_o = -1;
+    };
+    for (uint64_t step = begin; step < end; step += PZ_WAVE) {
+        const uint64_t t = step + lane;
+        bool valid = t < end;
+        int32_t o = -1;
+        float tl[3], th[3];
+        if (valid) {
+            o = t2o[t];
+            const uint32_t i0 = idx[3 * t], i1 = idx[3 * t + 1], i2 = idx[3 * t + 2];
+            valid = o >= 0 && o < nobj && i0 < nv && i1 < nv && i2 < nv;
+            if (valid) {
+                const float3 a = vtx(verts, i0), b = vtx(verts, i1), c = vtx(verts, i2);
+                tl[0] = fminf(a.x, fminf(b.x, c.x));
+                tl[1] = fminf(a.y, fminf(b.y, c.y));
+                tl[2] = fminf(a.z, fminf(b.z, c.z));
+                th[0] = fmaxf(a.x, fmaxf(b.x, c.x));
+                th[1] = fmaxf(a.y, fmaxf(b.y, c.y));
+                th[2] = fmaxf(a.z, fmaxf(b.z, c.z));
+            }
         }
-        atomicAdd(&tri_cnt[o], 1u);
-        atomicMin(&first_tri[o], (uint32_t)t);
-        atomicMax(&last_tri[o], (uint32_t)t);
+        const uint64_t vm = __ballot(valid);
+        if (vm == 0) continue;
+        const int32_t o0 = __shfl(o, __ffsll((long long)vm) - 1, 64);
+        const bool uniform = __ballot(valid && o == o0) == vm;
+        if (!uniform || o0 != run_o) {
+            flush();
+            if (uniform) run_o = o0;
+        }
+        if (!valid) continue;
+        if (uniform) {
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = fminf(lo[a], tl[a]);
+                hi[a] = fmaxf(hi[a], th[a]);
+            }
+            ++cnt;
+            tmin = min(tmin, (uint32_t)t);
+            tmax = max(tmax, (uint32_t)t);
+        } else {  // several objects in these 64 triangles
+            for (int a = 0; a < 3; ++a) {
+                atomicMin(&obox[6ull * o + a], f2ord(tl[a]));
+                atomicMax(&obox[6ull * o + 3 + a], f2ord(th[a]));
+            }
+            atomicAdd(&tri_cnt[o], 1u);
+            atomicMin(&first_tri[o], (uint32_t)t);
+            atomicMax(&last_tri[o], (uint32_t)t);
+        }
     }
+    flush();
 }
 
 __global__ __launch_bounds__(kBlock) void k_edge_count(const int32_t* __restrict__ eo, uint64_t ne, int32_t nobj,
@@ -538,8 +579,9 @@ __global__ __launch_bounds__(kBlock, 4) void k_narrow(const uint64_t* __restrict
 // bit 1: mesh2 edges), which is the edge-against-triangle predicate on the
 // triangles' own edges.  The wave stops at the first proven intersection.
 // A queue overflow marks the pair for the per-edge kernel instead of
-// dropping work.
-constexpr int kDualQ = 1536;
+// dropping work.  The queue length Q (node pairs per wave) is a launch
+// parameter: a shorter queue leaves LDS for more resident waves, a longer one
+// sends fewer pairs to the per-edge kernel.
 constexpr int kDualWaves = 4;
 
 __device__ __forceinline__ void leaf_box(const LeafTri& L, float* lo, float* hi) {
@@ -608,12 +650,12 @@ __device__ __forceinline__ bool tri_pair_test(const LeafTri& X, const LeafTri& Y
     return hit;
 }
 
-template <bool F32>
+template <bool F32, int QLEN>
 __global__ __launch_bounds__(kDualWaves * PZ_WAVE, 2) void k_narrow_dual(
     const uint64_t* __restrict__ pairs, uint64_t np, SetView A, SetView B, uint32_t* __restrict__ found,
     uint32_t* __restrict__ fallback, const uint8_t* __restrict__ meta, int allow_f32,
     unsigned long long* __restrict__ stats) {
-    __shared__ int2 q[kDualWaves][kDualQ];
+    __shared__ int2 q[kDualWaves][QLEN];
     const int lane = __lane_id(), w = threadIdx.x / PZ_WAVE;
     const uint64_t p = (uint64_t)blockIdx.x * kDualWaves + w;
     if (p >= np) return;
@@ -692,7 +734,7 @@ __global__ __launch_bounds__(kDualWaves * PZ_WAVE, 2) void k_narrow_dual(
             if (lane >= s) incl += u;
         }
         const int total = __shfl(incl, PZ_WAVE - 1, PZ_WAVE);
-        if (qn + total > kDualQ) {
+        if (qn + total > QLEN) {
             ovf = true;
             break;
         }
@@ -1069,8 +1111,10 @@ void object_stats(const float* verts, uint64_t nv, const uint32_t* idx, const in
     hipLaunchKernelGGL((k_fill_any<uint32_t>), dim3(grid1((uint64_t)nobj)), dim3(kBlock), 0, s, last_tri, 0u,
                        (uint64_t)nobj);
     KernelTimer kt_("object_stats", s);
+    const uint64_t waves = (nt + kStatsRun - 1) / kStatsRun;
     if (nt)
-        hipLaunchKernelGGL(k_object_stats, dim3(grid1(nt)), dim3(kBlock), 0, s, verts, nv, idx, t2o, nt, nobj, obox,
+        hipLaunchKernelGGL(k_object_stats, dim3(grid1(waves * PZ_WAVE)), dim3(kBlock), 0, s, verts, nv, idx, t2o, nt,
+                           nobj, obox,
                            tri_cnt, first_tri, last_tri);
 }
 
@@ -1179,18 +1223,33 @@ bool narrow(const uint64_t* pairs, uint64_t np, const uint64_t* task_off, uint64
     return true;
 }
 
-void narrow_dual(const uint64_t* pairs, uint64_t np, const SetView& A, const SetView& B, uint32_t* found,
-                 uint32_t* fallback, const uint8_t* meta, bool f32, int allow_f32, unsigned long long* stats,
-                 hipStream_t s) {
-    if (np == 0) return;
+bool narrow_dual(const uint64_t* pairs, uint64_t np, const SetView& A, const SetView& B, uint32_t* found,
+                 uint32_t* fallback, const uint8_t* meta, bool f32, int allow_f32, int queue,
+                 unsigned long long* stats, hipStream_t s) {
+    if (np == 0) return true;
     unsigned g = (unsigned)((np + kDualWaves - 1) / kDualWaves);
     KernelTimer kt_("narrow_dual", s);
-    if (f32)
-        hipLaunchKernelGGL((k_narrow_dual<true>), dim3(g), dim3(kDualWaves * PZ_WAVE), 0, s, pairs, np, A, B, found,
-                           fallback, meta, allow_f32, stats);
-    else
-        hipLaunchKernelGGL((k_narrow_dual<false>), dim3(g), dim3(kDualWaves * PZ_WAVE), 0, s, pairs, np, A, B, found,
-                           fallback, meta, allow_f32, stats);
+#define PZ_LAUNCH_DUAL(F, Qn)                                                                                 \
+    hipLaunchKernelGGL((k_narrow_dual<F, Qn>), dim3(g), dim3(kDualWaves * PZ_WAVE), 0, s, pairs, np, A, B, found, \
+                       fallback, meta, allow_f32, stats)
+#define PZ_DUAL_Q(Qn)                 \
+    case Qn:                          \
+        if (f32)                      \
+            PZ_LAUNCH_DUAL(true, Qn);  \
+        else                          \
+            PZ_LAUNCH_DUAL(false, Qn); \
+        break;
+    switch (queue) {
+        PZ_DUAL_Q(256)
+        PZ_DUAL_Q(512)
+        PZ_DUAL_Q(1024)
+        PZ_DUAL_Q(1536)
+        default:
+            return false;
+    }
+#undef PZ_DUAL_Q
+#undef PZ_LAUNCH_DUAL
+    return true;
 }
 
 bool parity(const uint64_t* pairs, uint64_t np, const uint32_t* found, const SetView& A, const SetView& B,

@@ -32,20 +32,30 @@ void object_stats(const float* verts, uint64_t nv, const uint32_t* idx, const in
 // --- small-object LDS builder (build_small.hip)
 // Objects with at most kSmallMax triangles stored contiguously are built
 // entirely in LDS by one workgroup each; the rest use the global pipeline.
+// The LDS builder runs one launch per size class so each launch sizes its
+// LDS (and picks its workgroup size) for the class.
 constexpr uint32_t kSmallMax = 2048;
-// big[o] / small[o] flags of the needed objects.
+constexpr int kSmallClasses = 3;
+constexpr uint32_t kClassMax[kSmallClasses] = {128, 512, 2048};
+// big[o] flags; cls[k * (nobj + 1) + o] flags per class; totals[5 + k] = max
+// triangle count of class k (atomicMax: zero totals first).
 void need_split(const uint32_t* need, const uint32_t* tri_cnt, const uint32_t* first_tri, const uint32_t* last_tri,
-                int32_t nobj, uint32_t* big, uint32_t* small, hipStream_t s);
-// loc[o] = big ? loc_big[o] : (number of big objects) + loc_small[o]; also
-// totals[0] = number of big objects.
-void merge_locs(const uint32_t* big, const uint32_t* loc_big, const uint32_t* loc_small, int32_t nobj, uint32_t* loc,
+                int32_t nobj, uint32_t* big, uint32_t* cls, uint32_t* totals, hipStream_t s);
+// loc[o]: big objects first (loc_big), then class 0, 1, 2 in id order
+// (loc_cls: exclusive scans of the class flags, nobj + 1 entries each);
+// totals[0] = big objects, totals[2 + k] = objects of class k.
+void merge_locs(const uint32_t* big, const uint32_t* loc_big, const uint32_t* loc_cls, int32_t nobj, uint32_t* loc,
                 uint32_t* totals, hipStream_t s);
 // totals[1] = seg_off[totals[0]] (triangles of the big objects).
 void big_prefix(const uint32_t* seg_off, uint32_t* totals, hipStream_t s);
-// One workgroup per small object l in [l_begin, l_begin + count).
-void build_small(const float* verts, const uint32_t* idx, const int32_t* t2o, const int32_t* obj_of_loc,
+// Dynamic LDS bytes of one workgroup for objects of at most cap triangles.
+size_t build_small_lds(uint32_t cap);
+// One workgroup of `threads` (128, 256, 512 or 1024) per object l in
+// [l_begin, l_begin + count), all of at most cap triangles.  false: bad cap or threads.
+bool build_small(const float* verts, const uint32_t* idx, const int32_t* t2o, const int32_t* obj_of_loc,
                  const uint32_t* first_tri, const uint32_t* tri_cnt, const uint32_t* obox, const uint32_t* seg_off,
-                 uint32_t l_begin, uint32_t count, Node* nodes, LeafTri* leaves, uint32_t* height, hipStream_t s);
+                 uint32_t l_begin, uint32_t count, uint32_t cap, int threads, pzd::Node* nodes, pzd::LeafTri* leaves,
+                 uint32_t* height, const char* timer, hipStream_t s);
 void edge_count(const int32_t* eo, uint64_t ne, int32_t nobj, uint32_t* edge_cnt, hipStream_t s);
 void scene_box(const uint32_t* obox, int32_t nobj, uint32_t* scene, hipStream_t s);
 // keys = (part_of[o] << 30) | Morton of the box centre (part 0 when part_of is null)
@@ -82,11 +92,12 @@ void task_counts(const uint64_t* pairs, uint64_t np, const uint32_t* locA, const
 bool narrow(const uint64_t* pairs, uint64_t np, const uint64_t* task_off, uint64_t ntasks, const SetView& A,
             const SetView& B, uint32_t* found, const uint8_t* meta, bool f32, int allow_f32, uint32_t height,
             unsigned long long* stats, hipStream_t s);
-// Dual-tree narrow phase (one wave per pair); pairs whose LDS queue overflows
-// get fallback[p] = 1 and must be re-run with narrow().
-void narrow_dual(const uint64_t* pairs, uint64_t np, const SetView& A, const SetView& B, uint32_t* found,
-                 uint32_t* fallback, const uint8_t* meta, bool f32, int allow_f32, unsigned long long* stats,
-                 hipStream_t s);
+// Dual-tree narrow phase (one wave per pair, an LDS queue of `queue` node
+// pairs per wave: 256, 512, 1024 or 1536); pairs whose queue overflows get
+// fallback[p] = 1 and must be re-run with narrow().  false: bad queue length.
+bool narrow_dual(const uint64_t* pairs, uint64_t np, const SetView& A, const SetView& B, uint32_t* found,
+                 uint32_t* fallback, const uint8_t* meta, bool f32, int allow_f32, int queue,
+                 unsigned long long* stats, hipStream_t s);
 // Compact pairs with flag[p] != 0 into out (count -> *count).
 void compact_flagged(const uint64_t* pairs, uint64_t np, const uint32_t* flag, uint64_t* out, uint32_t* out_index,
                      unsigned long long* count, hipStream_t s);

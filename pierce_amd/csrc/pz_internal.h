@@ -22,8 +22,18 @@ struct DevBuf {
     size_t n = 0;
 };
 
+// Launch parameters picked per size class (defaults, or measured by pz_autotune).
+struct TuneTable {
+    int build_threads[3] = {128, 256, 512};  // LDS builder workgroup size for objects <= 128 / 512 / 2048 triangles
+    // dual-tree narrow phase: LDS queue entries per wave, for joins whose objects
+    // are all LDS-built (<= 2048 triangles) [0] or that index larger ones [1]
+    int narrow_queue[2] = {1536, 1536};
+};
+
 struct pz_ctx {
     int device = 0;
+    TuneTable tune;
+    int last_queue_class = 0;  // narrow-queue class of the last two-level join (what pz_autotune tunes)
     hipStream_t s = nullptr;
     std::unordered_map<std::string, DevBuf> bufs;
     void* pinned = nullptr;  // small readbacks
