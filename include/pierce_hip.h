@@ -247,6 +247,9 @@ typedef struct {
     uint64_t stack_overflows;   /* must be 0: traversal stack exceeded (result invalid) */
     uint64_t index_triangles;   /* triangles indexed by the per-object (two-level) or set-wide BVHs */
     uint64_t index_edges;       /* edges regrouped per object (two-level) */
+    uint64_t tasks_spilled;     /* narrow-phase subtree tasks handed from one work-queue round to the next */
+    int32_t  narrow_rounds;     /* work-queue rounds of the narrow phase */
+    int32_t  reserved_r;
     double   ms_total;          /* device-resident geometry -> sorted host pair set */
     pz_phase phases[PZ_MAX_PHASES];
 } pz_join_result;
@@ -343,7 +346,8 @@ typedef struct {
     int32_t narrow_queue[2];    /* dual-tree narrow phase: node pairs queued in LDS per wave
                                    (256, 512, 1024 or 1536; overflowing pairs go to the per-edge kernel),
                                    [0] for joins whose objects all have <= 2048 triangles, [1] otherwise */
-    int32_t reserved[11];
+    int32_t narrow_chunk[2];    /* work-queue tasks a wave takes at a time (1..64), same two classes */
+    int32_t reserved[9];
 } pz_tune;
 
 typedef struct {
@@ -351,6 +355,7 @@ typedef struct {
     double  build_ms[3][4];     /* class x candidate {128, 256, 512, 1024}: device ms of that class's
                                    launches in one join; < 0 when the workload has no object of the class */
     double  narrow_ms[4];       /* queue {256, 512, 1024, 1536}: device ms of the narrow phase kernels */
+    double  chunk_ms[4];        /* chunk {1, 2, 4, 8} at the chosen queue: device ms of the narrow phase */
     int32_t joins_run;
     int32_t queue_class;        /* which narrow_queue entry this workload tuned */
     int32_t reserved[6];
@@ -360,8 +365,9 @@ pz_status pz_tune_get(const pz_ctx* ctx, pz_tune* out);
 pz_status pz_tune_set(pz_ctx* ctx, const pz_tune* in);
 /* Autotuner: after one warm-up join, runs the join once per candidate
  * workgroup size and times every size class's builder launches with HIP
- * events, keeping the fastest per class; then once per queue length, keeping
- * the fastest narrow phase.  The choices stay in the context. */
+ * events, keeping the fastest per class; then once per queue length and once
+ * per work-queue chunk, keeping the fastest narrow phase for the workload's
+ * class.  The choices stay in the context (13 joins in all). */
 pz_status pz_autotune(pz_ctx* ctx, const pz_dataset* mesh1, const pz_dataset* mesh2, const pz_join_desc* desc,
                       pz_tune_report* report);
 

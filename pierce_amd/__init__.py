@@ -92,7 +92,8 @@ class JoinResult(C.Structure):
                 ("predicted_pairs", C.c_uint64), ("overflowed", C.c_int32), ("num_phases", C.c_int32), ("tests_total", C.c_uint64),
                 ("tests_fp64", C.c_uint64), ("tests_exact", C.c_uint64), ("tests_fp32_misclassified", C.c_uint64),
                 ("edges_traced", C.c_uint64), ("stack_overflows", C.c_uint64), ("index_triangles", C.c_uint64),
-                ("index_edges", C.c_uint64), ("ms_total", C.c_double),
+                ("index_edges", C.c_uint64), ("tasks_spilled", C.c_uint64), ("narrow_rounds", C.c_int32),
+                ("reserved_r", C.c_int32), ("ms_total", C.c_double),
                 ("phases", Phase * MAX_PHASES)]
 
 
@@ -109,16 +110,19 @@ _FP = C.POINTER(C.c_float)
 
 
 class Tune(C.Structure):
-    _fields_ = [("build_threads", C.c_int32 * 3), ("narrow_queue", C.c_int32 * 2), ("reserved", C.c_int32 * 11)]
+    _fields_ = [("build_threads", C.c_int32 * 3), ("narrow_queue", C.c_int32 * 2), ("narrow_chunk", C.c_int32 * 2),
+                ("reserved", C.c_int32 * 9)]
 
 
 class TuneReport(C.Structure):
     _fields_ = [("chosen", Tune), ("build_ms", (C.c_double * 4) * 3), ("narrow_ms", C.c_double * 4),
+                ("chunk_ms", C.c_double * 4),
                 ("joins_run", C.c_int32), ("queue_class", C.c_int32), ("reserved", C.c_int32 * 6)]
 
 
 TUNE_CANDIDATES = (128, 256, 512, 1024)
 QUEUE_CANDIDATES = (256, 512, 1024, 1536)
+CHUNK_CANDIDATES = (1, 2, 4, 8)
 
 # Every function declared in include/pierce_hip.h: (name, restype, argtypes).
 _SIGS = [
@@ -424,6 +428,8 @@ class JoinOutput:
     ms_total: float
     phases: list
     predicted_pairs: int = 0
+    tasks_spilled: int = 0
+    narrow_rounds: int = 0
 
 
 def join_desc(**kw) -> JoinDesc:
@@ -454,7 +460,7 @@ def _output(r: JoinResult) -> JoinOutput:
     return JoinOutput(pairs, types, int(r.num_candidates), int(r.capacity), int(r.predicted), bool(r.overflowed),
                       int(r.tests_total), int(r.tests_fp64), int(r.tests_exact), int(r.tests_fp32_misclassified),
                       int(r.edges_traced), int(r.index_triangles), int(r.index_edges), float(r.ms_total), phases,
-                      int(r.predicted_pairs))
+                      int(r.predicted_pairs), int(r.tasks_spilled), int(r.narrow_rounds))
 
 
 def join_batch(ctx: Context, a: Dataset, b: Dataset, queries, **common):
@@ -623,13 +629,15 @@ def log_counts(counts) -> np.ndarray:
 def tune_get(ctx: Context) -> dict:
     t = Tune()
     _check(lib().pz_tune_get(ctx._h, C.byref(t)))
-    return {"build_threads": list(t.build_threads), "narrow_queue": list(t.narrow_queue)}
+    return {"build_threads": list(t.build_threads), "narrow_queue": list(t.narrow_queue),
+            "narrow_chunk": list(t.narrow_chunk)}
 
 
-def tune_set(ctx: Context, build_threads, narrow_queue=(1536, 1536)) -> None:
+def tune_set(ctx: Context, build_threads, narrow_queue=(1536, 1536), narrow_chunk=(1, 1)) -> None:
     t = Tune()
     t.build_threads[:] = list(build_threads)
     t.narrow_queue[:] = list(narrow_queue)
+    t.narrow_chunk[:] = list(narrow_chunk)
     _check(lib().pz_tune_set(ctx._h, C.byref(t)))
 
 
@@ -639,7 +647,8 @@ def autotune(ctx: Context, a: Dataset, b: Dataset, **kw) -> dict:
     r = TuneReport()
     _check(lib().pz_autotune(ctx._h, a._h, b._h, C.byref(d), C.byref(r)))
     return {"build_threads": list(r.chosen.build_threads), "narrow_queue": list(r.chosen.narrow_queue),
-            "queue_class": int(r.queue_class),
+            "narrow_chunk": list(r.chosen.narrow_chunk), "queue_class": int(r.queue_class),
+            "chunk_ms": {str(CHUNK_CANDIDATES[i]): round(r.chunk_ms[i], 4) for i in range(4)},
             "joins_run": int(r.joins_run),
             "narrow_ms": {str(QUEUE_CANDIDATES[i]): round(r.narrow_ms[i], 4) for i in range(4)},
             "build_ms": {f"class{k}": {str(TUNE_CANDIDATES[i]): (None if r.build_ms[k][i] < 0

@@ -445,9 +445,41 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
     if (d.narrow_mode == 0) {
         const int qclass = (ta.nbig + tb.nbig > 0) ? 1 : 0;  // pairs with objects beyond the LDS builder
         c->last_queue_class = qclass;
-        if (!pzk::narrow_dual(pairs, np, va, vb, found, fallback, meta, f32, allow_f32,
-                              c->tune.narrow_queue[qclass], stats, s))
-            throw PzError(PZ_E_INVALID, "invalid narrow-phase queue length");
+        // Work queue of (pair, subtree pair) tasks, run in rounds: round 0 takes
+        // every candidate pair from its roots; a wave whose LDS queue fills up
+        // hands its shallowest subtree pairs to the next round.
+        const uint64_t cap_tasks = std::min<uint64_t>(std::max<uint64_t>(8 * np, 1 << 16), 1ull << 24);
+        pzd::DualTask* tin = buf<pzd::DualTask>(c, "np.tasks0", cap_tasks);
+        pzd::DualTask* tout = buf<pzd::DualTask>(c, "np.tasks1", cap_tasks);
+        unsigned long long* ctr = buf<unsigned long long>(c, "np.task_ctr", 2);
+        if (c->persistent_blocks == 0) {
+            int cus = 0;
+            HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+            c->persistent_blocks = (unsigned)std::max(cus, 1) * 8u;
+        }
+        const pzd::DualTask* cur = nullptr;
+        uint64_t ntasks = np, spilled_total = 0;
+        int rounds = 0;
+        while (ntasks > 0) {
+            pzk::fill_ull(ctr, 0ull, 2, s);
+            if (!pzk::narrow_dual(pairs, va, vb, found, fallback, meta, f32, allow_f32, c->tune.narrow_queue[qclass],
+                                  cur, ntasks, ctr, c->tune.narrow_chunk[qclass], tout, ctr + 1, cap_tasks,
+                                  c->persistent_blocks, stats, s))
+                throw PzError(PZ_E_INVALID, "invalid narrow-phase queue length");
+            ++rounds;
+            const uint64_t spilled = std::min<uint64_t>(readback<unsigned long long>(c, {ctr + 1})[0], cap_tasks);
+            spilled_total += spilled;
+            if (spilled == 0) break;
+            if (rounds >= 32) {  // leave what is left to the per-edge kernel
+                pzk::mark_task_pairs(tout, spilled, fallback, s);
+                break;
+            }
+            std::swap(tin, tout);
+            cur = tin;
+            ntasks = spilled;
+        }
+        r->narrow_rounds = rounds;
+        r->tasks_spilled = spilled_total;
         uint64_t* fbp = buf<uint64_t>(c, "np.fb_pairs", np + 1);
         fb_index = buf<uint32_t>(c, "np.fb_index", np + 1);
         unsigned long long* fbc = buf<unsigned long long>(c, "np.fb_count", 1);
@@ -1057,8 +1089,10 @@ pz_status pz_tune_get(const pz_ctx* c, pz_tune* out) {
     if (!c || !out) return fail(PZ_E_INVALID, "pz_tune_get: null argument");
     std::memset(out, 0, sizeof(*out));
     for (int k = 0; k < pzk::kSmallClasses; ++k) out->build_threads[k] = c->tune.build_threads[k];
-    out->narrow_queue[0] = c->tune.narrow_queue[0];
-    out->narrow_queue[1] = c->tune.narrow_queue[1];
+    for (int k = 0; k < 2; ++k) {
+        out->narrow_queue[k] = c->tune.narrow_queue[k];
+        out->narrow_chunk[k] = c->tune.narrow_chunk[k];
+    }
     return PZ_OK;
 }
 
@@ -1072,9 +1106,14 @@ pz_status pz_tune_set(pz_ctx* c, const pz_tune* in) {
         if (!valid_threads(in->build_threads[k])) return fail(PZ_E_INVALID, "build_threads must be 128, 256, 512 or 1024");
     if (!valid_queue(in->narrow_queue[0]) || !valid_queue(in->narrow_queue[1]))
         return fail(PZ_E_INVALID, "narrow_queue must be 256, 512, 1024 or 1536");
+    for (int k = 0; k < 2; ++k)
+        if (in->narrow_chunk[k] < 1 || in->narrow_chunk[k] > 64)
+            return fail(PZ_E_INVALID, "narrow_chunk must be in [1, 64]");
     for (int k = 0; k < pzk::kSmallClasses; ++k) c->tune.build_threads[k] = in->build_threads[k];
-    c->tune.narrow_queue[0] = in->narrow_queue[0];
-    c->tune.narrow_queue[1] = in->narrow_queue[1];
+    for (int k = 0; k < 2; ++k) {
+        c->tune.narrow_queue[k] = in->narrow_queue[k];
+        c->tune.narrow_chunk[k] = in->narrow_chunk[k];
+    }
     return PZ_OK;
 }
 
@@ -1142,8 +1181,32 @@ pz_status pz_autotune(pz_ctx* c, const pz_dataset* a, const pz_dataset* b, const
     for (int i = 1; i < 4; ++i)
         if (report->narrow_ms[i] < report->narrow_ms[best]) best = i;
     c->tune.narrow_queue[qc] = kQueue[best];
-    report->chosen.narrow_queue[0] = c->tune.narrow_queue[0];
-    report->chosen.narrow_queue[1] = c->tune.narrow_queue[1];
+    // work-queue chunk: tasks a wave takes at a time
+    static const int kChunk[4] = {1, 2, 4, 8};
+    const TuneTable base2 = c->tune;
+    pzk::prof_enable(true);
+    for (int i = 0; i < 4 && st == PZ_OK; ++i) {
+        c->tune.narrow_chunk[qc] = kChunk[i];
+        pzk::prof_reset();
+        st = run();
+        double ms = 0, ms2 = 0;
+        uint64_t n = 0;
+        pzk::prof_query("narrow_dual", &ms, &n);
+        pzk::prof_query("narrow", &ms2, &n);
+        report->chunk_ms[i] = ms + ms2;
+    }
+    pzk::prof_reset();
+    pzk::prof_enable(was_on);
+    c->tune = base2;
+    if (st != PZ_OK) return st;
+    best = 0;
+    for (int i = 1; i < 4; ++i)
+        if (report->chunk_ms[i] < report->chunk_ms[best]) best = i;
+    c->tune.narrow_chunk[qc] = kChunk[best];
+    for (int k = 0; k < 2; ++k) {
+        report->chosen.narrow_queue[k] = c->tune.narrow_queue[k];
+        report->chosen.narrow_chunk[k] = c->tune.narrow_chunk[k];
+    }
     report->queue_class = qc;
     return PZ_OK;
 }

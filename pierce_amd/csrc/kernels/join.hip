@@ -652,106 +652,152 @@ __device__ __forceinline__ bool tri_pair_test(const LeafTri& X, const LeafTri& Y
 
 template <bool F32, int QLEN>
 __global__ __launch_bounds__(kDualWaves * PZ_WAVE, 2) void k_narrow_dual(
-    const uint64_t* __restrict__ pairs, uint64_t np, SetView A, SetView B, uint32_t* __restrict__ found,
+    const uint64_t* __restrict__ pairs, SetView A, SetView B, uint32_t* __restrict__ found,
     uint32_t* __restrict__ fallback, const uint8_t* __restrict__ meta, int allow_f32,
+    const DualTask* __restrict__ tasks_in, uint64_t ntasks, unsigned long long* __restrict__ next_task, int chunk,
+    DualTask* __restrict__ tasks_out, unsigned long long* __restrict__ nout, uint64_t out_cap,
     unsigned long long* __restrict__ stats) {
     __shared__ int2 q[kDualWaves][QLEN];
     const int lane = __lane_id(), w = threadIdx.x / PZ_WAVE;
-    const uint64_t p = (uint64_t)blockIdx.x * kDualWaves + w;
-    if (p >= np) return;
-    const uint64_t key = pairs[p];
-    const int dir_mask = meta[p] >> 2;
-    const int32_t oa = (int32_t)(key >> 32), ob = (int32_t)(uint32_t)key;
-    const uint32_t la = A.loc[oa], lb = B.loc[ob];
-    const uint32_t a0 = A.seg_off[la], a1 = A.seg_off[la + 1], b0 = B.seg_off[lb], b1 = B.seg_off[lb + 1];
-    const int ra = (a1 - a0 >= 2) ? (int)a0 : ~(int)a0;
-    const int rb = (b1 - b0 >= 2) ? (int)b0 : ~(int)b0;
-    int2* Q = q[w];
-    if (lane == 0) Q[0] = make_int2(ra, rb);
-    int qn = 1;
-    bool hit = false, ovf = false;
+    int2* Q = q[w];  // ring buffer: entries [bot, top) modulo QLEN, LIFO at the top
     LaneStats st{0, 0, 0, 0, 0};
-    while (qn > 0) {
-        const int take = qn < PZ_WAVE ? qn : PZ_WAVE;
-        const bool act = lane < take;
-        int2 e = act ? Q[qn - 1 - lane] : make_int2(0, 0);
-        qn -= take;
-        int2 o0 = make_int2(0, 0), o1 = o0, o2 = o0, o3 = o0;
-        int nout = 0;
-        if (act) {
-            const int x = e.x, y = e.y;
-            if (x < 0 && y < 0) {
-                hit = tri_pair_test<F32>(A.leaves[~x], B.leaves[~y], dir_mask, allow_f32 != 0, st);
-            } else if (x >= 0 && y >= 0) {
-                const Node X = A.nodes[x], Y = B.nodes[y];
-                float xl[2][3], xh[2][3], yl[2][3], yh[2][3];
-                child_box(X, 0, xl[0], xh[0]);
-                child_box(X, 1, xl[1], xh[1]);
-                child_box(Y, 0, yl[0], yh[0]);
-                child_box(Y, 1, yl[1], yh[1]);
-                const int xc[2] = {X.d.x, X.d.y}, yc[2] = {Y.d.x, Y.d.y};
-                for (int i = 0; i < 2; ++i)
-                    for (int j = 0; j < 2; ++j)
-                        if (box_box(xl[i], xh[i], yl[j], yh[j])) {
-                            int2 v = make_int2(xc[i], yc[j]);
-                            if (nout == 0) o0 = v;
-                            else if (nout == 1) o1 = v;
-                            else if (nout == 2) o2 = v;
-                            else o3 = v;
-                            ++nout;
-                        }
-            } else if (x >= 0) {  // mesh2 side is a leaf
-                const Node X = A.nodes[x];
-                float yl[3], yh[3], cl[3], chh[3];
-                leaf_box(B.leaves[~y], yl, yh);
-                child_box(X, 0, cl, chh);
-                if (box_box(cl, chh, yl, yh)) { o0 = make_int2(X.d.x, y); nout = 1; }
-                child_box(X, 1, cl, chh);
-                if (box_box(cl, chh, yl, yh)) {
-                    if (nout == 0) o0 = make_int2(X.d.y, y);
-                    else o1 = make_int2(X.d.y, y);
-                    ++nout;
-                }
-            } else {  // mesh1 side is a leaf
-                const Node Y = B.nodes[y];
-                float xl[3], xh[3], cl[3], chh[3];
-                leaf_box(A.leaves[~x], xl, xh);
-                child_box(Y, 0, cl, chh);
-                if (box_box(cl, chh, xl, xh)) { o0 = make_int2(x, Y.d.x); nout = 1; }
-                child_box(Y, 1, cl, chh);
-                if (box_box(cl, chh, xl, xh)) {
-                    if (nout == 0) o0 = make_int2(x, Y.d.y);
-                    else o1 = make_int2(x, Y.d.y);
-                    ++nout;
-                }
+    while (true) {
+        unsigned long long t0 = 0;
+        if (lane == 0) t0 = atomicAdd(next_task, (unsigned long long)chunk);
+        t0 = __shfl(t0, 0, PZ_WAVE);
+        if (t0 >= ntasks) break;
+        const uint64_t t1 = t0 + (uint64_t)chunk < ntasks ? t0 + (uint64_t)chunk : ntasks;
+        for (uint64_t t = t0; t < t1; ++t) {
+            uint32_t p;
+            int2 root;
+            if (tasks_in) {
+                const DualTask tk = tasks_in[t];
+                p = tk.pair;
+                root = make_int2(tk.x, tk.y);
+            } else {
+                p = (uint32_t)t;
+                const uint64_t key = pairs[p];
+                const int32_t oa = (int32_t)(key >> 32), ob = (int32_t)(uint32_t)key;
+                const uint32_t la = A.loc[oa], lb = B.loc[ob];
+                const uint32_t a0 = A.seg_off[la], a1 = A.seg_off[la + 1], b0 = B.seg_off[lb], b1 = B.seg_off[lb + 1];
+                root = make_int2((a1 - a0 >= 2) ? (int)a0 : ~(int)a0, (b1 - b0 >= 2) ? (int)b0 : ~(int)b0);
             }
+            if (found[p]) continue;  // another task of this pair already proved a hit
+            const int dir_mask = meta[p] >> 2;
+            if (lane == 0) Q[0] = root;
+            __builtin_amdgcn_wave_barrier();
+            int bot = 0, top = 1;
+            bool hit = false, ovf = false;
+            int iter = 0;
+            while (top > bot) {
+                if ((++iter & 15) == 0 && found[p]) break;
+                const int qn = top - bot;
+                const int take = qn < PZ_WAVE ? qn : PZ_WAVE;
+                const bool act = lane < take;
+                int2 e = act ? Q[(top - 1 - lane) % QLEN] : make_int2(0, 0);
+                top -= take;
+                int2 o0 = make_int2(0, 0), o1 = o0, o2 = o0, o3 = o0;
+                int nout_l = 0;
+                if (act) {
+                    const int x = e.x, y = e.y;
+                    if (x < 0 && y < 0) {
+                        hit = tri_pair_test<F32>(A.leaves[~x], B.leaves[~y], dir_mask, allow_f32 != 0, st);
+                    } else if (x >= 0 && y >= 0) {
+                        const Node X = A.nodes[x], Y = B.nodes[y];
+                        float xl[2][3], xh[2][3], yl[2][3], yh[2][3];
+                        child_box(X, 0, xl[0], xh[0]);
+                        child_box(X, 1, xl[1], xh[1]);
+                        child_box(Y, 0, yl[0], yh[0]);
+                        child_box(Y, 1, yl[1], yh[1]);
+                        const int xc[2] = {X.d.x, X.d.y}, yc[2] = {Y.d.x, Y.d.y};
+                        for (int i = 0; i < 2; ++i)
+                            for (int j = 0; j < 2; ++j)
+                                if (box_box(xl[i], xh[i], yl[j], yh[j])) {
+                                    int2 v = make_int2(xc[i], yc[j]);
+                                    if (nout_l == 0) o0 = v;
+                                    else if (nout_l == 1) o1 = v;
+                                    else if (nout_l == 2) o2 = v;
+                                    else o3 = v;
+                                    ++nout_l;
+                                }
+                    } else if (x >= 0) {  // mesh2 side is a leaf
+                        const Node X = A.nodes[x];
+                        float yl[3], yh[3], cl[3], chh[3];
+                        leaf_box(B.leaves[~y], yl, yh);
+                        child_box(X, 0, cl, chh);
+                        if (box_box(cl, chh, yl, yh)) { o0 = make_int2(X.d.x, y); nout_l = 1; }
+                        child_box(X, 1, cl, chh);
+                        if (box_box(cl, chh, yl, yh)) {
+                            if (nout_l == 0) o0 = make_int2(X.d.y, y);
+                            else o1 = make_int2(X.d.y, y);
+                            ++nout_l;
+                        }
+                    } else {  // mesh1 side is a leaf
+                        const Node Y = B.nodes[y];
+                        float xl[3], xh[3], cl[3], chh[3];
+                        leaf_box(A.leaves[~x], xl, xh);
+                        child_box(Y, 0, cl, chh);
+                        if (box_box(cl, chh, xl, xh)) { o0 = make_int2(x, Y.d.x); nout_l = 1; }
+                        child_box(Y, 1, cl, chh);
+                        if (box_box(cl, chh, xl, xh)) {
+                            if (nout_l == 0) o0 = make_int2(x, Y.d.y);
+                            else o1 = make_int2(x, Y.d.y);
+                            ++nout_l;
+                        }
+                    }
+                }
+                if (__ballot(hit) != 0) break;
+                // append children pairs: wave-wide exclusive prefix of nout_l
+                int incl = nout_l;
+                for (int sft = 1; sft < PZ_WAVE; sft <<= 1) {
+                    int u = __shfl_up(incl, sft, PZ_WAVE);
+                    if (lane >= sft) incl += u;
+                }
+                const int total = __shfl(incl, PZ_WAVE - 1, PZ_WAVE);
+                if (top - bot + total > QLEN) {
+                    // hand the oldest half (the shallowest subtrees) to the next round
+                    const int k = (top - bot) / 2;
+                    unsigned long long base = 0;
+                    if (lane == 0 && k > 0) base = atomicAdd(nout, (unsigned long long)k);
+                    base = __shfl(base, 0, PZ_WAVE);
+                    const int kk = base < out_cap ? (int)((out_cap - base) < (uint64_t)k ? out_cap - base : k) : 0;
+                    for (int i = lane; i < kk; i += PZ_WAVE) {
+                        const int2 v = Q[(bot + i) % QLEN];
+                        DualTask tk;
+                        tk.pair = p;
+                        tk.x = v.x;
+                        tk.y = v.y;
+                        tk.pad = 0;
+                        tasks_out[base + i] = tk;
+                    }
+                    bot += kk;
+                    if (top - bot + total > QLEN) {
+                        ovf = true;
+                        break;
+                    }
+                }
+                const int pos = top + incl - nout_l;
+                if (nout_l > 0) Q[pos % QLEN] = o0;
+                if (nout_l > 1) Q[(pos + 1) % QLEN] = o1;
+                if (nout_l > 2) Q[(pos + 2) % QLEN] = o2;
+                if (nout_l > 3) Q[(pos + 3) % QLEN] = o3;
+                top += total;
+                __builtin_amdgcn_wave_barrier();
+            }
+            const bool any_hit = __ballot(hit) != 0;
+            if (lane == 0) {
+                if (any_hit) found[p] = 1u;
+                else if (ovf) fallback[p] = 1u;
+            }
+            __builtin_amdgcn_wave_barrier();
         }
-        if (__ballot(hit) != 0) break;
-        // append children pairs: wave-wide exclusive prefix of nout
-        int incl = nout;
-        for (int s = 1; s < PZ_WAVE; s <<= 1) {
-            int u = __shfl_up(incl, s, PZ_WAVE);
-            if (lane >= s) incl += u;
-        }
-        const int total = __shfl(incl, PZ_WAVE - 1, PZ_WAVE);
-        if (qn + total > QLEN) {
-            ovf = true;
-            break;
-        }
-        int pos = qn + incl - nout;
-        if (nout > 0) Q[pos] = o0;
-        if (nout > 1) Q[pos + 1] = o1;
-        if (nout > 2) Q[pos + 2] = o2;
-        if (nout > 3) Q[pos + 3] = o3;
-        qn += total;
-        __builtin_amdgcn_wave_barrier();
-    }
-    const bool any_hit = __ballot(hit) != 0;
-    if (lane == 0) {
-        if (any_hit) found[p] = 1u;
-        else if (ovf) fallback[p] = 1u;
     }
     flush_stats(st, stats);
+}
+
+__global__ void k_mark_task_pairs(const DualTask* __restrict__ tasks, uint64_t n, uint32_t* __restrict__ fallback) {
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) fallback[tasks[i].pair] = 1u;
 }
 
 // Parity: thread per (pair, direction).  Counts +Z crossings of the
@@ -1223,15 +1269,19 @@ bool narrow(const uint64_t* pairs, uint64_t np, const uint64_t* task_off, uint64
     return true;
 }
 
-bool narrow_dual(const uint64_t* pairs, uint64_t np, const SetView& A, const SetView& B, uint32_t* found,
-                 uint32_t* fallback, const uint8_t* meta, bool f32, int allow_f32, int queue,
-                 unsigned long long* stats, hipStream_t s) {
-    if (np == 0) return true;
-    unsigned g = (unsigned)((np + kDualWaves - 1) / kDualWaves);
+bool narrow_dual(const uint64_t* pairs, const SetView& A, const SetView& B, uint32_t* found, uint32_t* fallback,
+                 const uint8_t* meta, bool f32, int allow_f32, int queue, const DualTask* tasks_in, uint64_t ntasks,
+                 unsigned long long* next_task, int chunk, DualTask* tasks_out, unsigned long long* nout,
+                 uint64_t out_cap, unsigned blocks, unsigned long long* stats, hipStream_t s) {
+    if (ntasks == 0) return true;
+    if (chunk < 1) chunk = 1;
+    const uint64_t need = (ntasks + (uint64_t)chunk * kDualWaves - 1) / ((uint64_t)chunk * kDualWaves);
+    const unsigned g = (unsigned)(need < blocks ? need : blocks);
     KernelTimer kt_("narrow_dual", s);
-#define PZ_LAUNCH_DUAL(F, Qn)                                                                                 \
-    hipLaunchKernelGGL((k_narrow_dual<F, Qn>), dim3(g), dim3(kDualWaves * PZ_WAVE), 0, s, pairs, np, A, B, found, \
-                       fallback, meta, allow_f32, stats)
+#define PZ_LAUNCH_DUAL(F, Qn)                                                                                  \
+    hipLaunchKernelGGL((k_narrow_dual<F, Qn>), dim3(g), dim3(kDualWaves * PZ_WAVE), 0, s, pairs, A, B, found,   \
+                       fallback, meta, allow_f32, tasks_in, ntasks, next_task, chunk, tasks_out, nout, out_cap, \
+                       stats)
 #define PZ_DUAL_Q(Qn)                 \
     case Qn:                          \
         if (f32)                      \
@@ -1251,6 +1301,12 @@ bool narrow_dual(const uint64_t* pairs, uint64_t np, const SetView& A, const Set
 #undef PZ_LAUNCH_DUAL
     return true;
 }
+
+void mark_task_pairs(const DualTask* tasks, uint64_t n, uint32_t* fallback, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_mark_task_pairs, dim3(grid1(n)), dim3(kBlock), 0, s, tasks, n, fallback);
+}
+
+int narrow_dual_waves() { return kDualWaves; }
 
 bool parity(const uint64_t* pairs, uint64_t np, const uint32_t* found, const SetView& A, const SetView& B,
             const float* vA, const uint32_t* iA, const uint32_t* ftA, const float* vB, const uint32_t* iB,

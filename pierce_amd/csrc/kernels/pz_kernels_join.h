@@ -14,10 +14,18 @@ struct SetView {
     const uint32_t* loc;       // object id -> needed-object index
     const uint32_t* obox;      // ordered-uint boxes by object id (6 per object)
 };
+// One task of the narrow phase's work queue: a (mesh1 node, mesh2 node)
+// subtree pair of candidate pair `pair` still to be walked (links as in Node).
+struct DualTask {
+    uint32_t pair;
+    int32_t x, y;
+    uint32_t pad;
+};
 }  // namespace pzd
 
 namespace pzk {
 
+using pzd::DualTask;
 using pzd::EdgeRec;
 using pzd::LeafTri;
 using pzd::Node;
@@ -92,12 +100,22 @@ void task_counts(const uint64_t* pairs, uint64_t np, const uint32_t* locA, const
 bool narrow(const uint64_t* pairs, uint64_t np, const uint64_t* task_off, uint64_t ntasks, const SetView& A,
             const SetView& B, uint32_t* found, const uint8_t* meta, bool f32, int allow_f32, uint32_t height,
             unsigned long long* stats, hipStream_t s);
-// Dual-tree narrow phase (one wave per pair, an LDS queue of `queue` node
-// pairs per wave: 256, 512, 1024 or 1536); pairs whose queue overflows get
-// fallback[p] = 1 and must be re-run with narrow().  false: bad queue length.
-bool narrow_dual(const uint64_t* pairs, uint64_t np, const SetView& A, const SetView& B, uint32_t* found,
-                 uint32_t* fallback, const uint8_t* meta, bool f32, int allow_f32, int queue,
-                 unsigned long long* stats, hipStream_t s);
+// One round of the dual-tree narrow phase as a persistent work queue.
+// Waves fetch `chunk` tasks at a time from *next_task (zeroed by the caller)
+// until ntasks are taken.  tasks_in == nullptr: task i is candidate pair i
+// from the two object roots.  A wave whose LDS queue (`queue` node pairs:
+// 256, 512, 1024 or 1536) would overflow moves its oldest half to tasks_out
+// (reserved through *nout, at most out_cap) for the next round; if that is
+// full too, fallback[p] = 1 and the pair must be re-run with narrow().
+// `blocks` = persistent grid size.  false: bad queue length.
+bool narrow_dual(const uint64_t* pairs, const SetView& A, const SetView& B, uint32_t* found, uint32_t* fallback,
+                 const uint8_t* meta, bool f32, int allow_f32, int queue, const DualTask* tasks_in, uint64_t ntasks,
+                 unsigned long long* next_task, int chunk, DualTask* tasks_out, unsigned long long* nout,
+                 uint64_t out_cap, unsigned blocks, unsigned long long* stats, hipStream_t s);
+// fallback[tasks[i].pair] = 1 for every task (work the rounds did not finish).
+void mark_task_pairs(const DualTask* tasks, uint64_t n, uint32_t* fallback, hipStream_t s);
+// Waves per block of narrow_dual.
+int narrow_dual_waves();
 // Compact pairs with flag[p] != 0 into out (count -> *count).
 void compact_flagged(const uint64_t* pairs, uint64_t np, const uint32_t* flag, uint64_t* out, uint32_t* out_index,
                      unsigned long long* count, hipStream_t s);

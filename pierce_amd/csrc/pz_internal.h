@@ -28,12 +28,14 @@ struct TuneTable {
     // dual-tree narrow phase: LDS queue entries per wave, for joins whose objects
     // are all LDS-built (<= 2048 triangles) [0] or that index larger ones [1]
     int narrow_queue[2] = {1536, 1536};
+    int narrow_chunk[2] = {1, 1};  // tasks a wave takes from the narrow-phase work queue at a time
 };
 
 struct pz_ctx {
     int device = 0;
     TuneTable tune;
     int last_queue_class = 0;  // narrow-queue class of the last two-level join (what pz_autotune tunes)
+    unsigned persistent_blocks = 0;  // grid of the persistent narrow kernel: 8 blocks per CU
     hipStream_t s = nullptr;
     std::unordered_map<std::string, DevBuf> bufs;
     void* pinned = nullptr;  // small readbacks

@@ -34,7 +34,7 @@ def test_every_workgroup_size_gives_the_oracle_pairs(ctx, mixed, threads):
         inter = pz.join(ctx, da, db, predicate=pz.PRED_INTERSECTION)
         np.testing.assert_array_equal(inter.pairs, orc.predicate_join(a, b, pz.PRED_INTERSECTION)[0])
     finally:
-        pz.tune_set(ctx, saved["build_threads"], saved["narrow_queue"])
+        pz.tune_set(ctx, saved["build_threads"], saved["narrow_queue"], saved["narrow_chunk"])
 
 
 def test_autotune_times_each_class(ctx, mixed):
@@ -43,7 +43,8 @@ def test_autotune_times_each_class(ctx, mixed):
     try:
         rep = pz.autotune(ctx, da, db)
         print(f"[tune] {rep}", flush=True)
-        assert rep["joins_run"] == 9
+        assert rep["joins_run"] == 13
+        assert rep["narrow_chunk"][rep["queue_class"]] in pz.CHUNK_CANDIDATES
         q = rep["narrow_queue"][rep["queue_class"]]
         assert rep["queue_class"] == 1  # level-4 spheres (5,120 triangles) exceed the LDS builder
         assert q in pz.QUEUE_CANDIDATES
@@ -62,7 +63,7 @@ def test_autotune_times_each_class(ctx, mixed):
         assert pz.tune_get(ctx)["narrow_queue"] == rep["narrow_queue"]
         np.testing.assert_array_equal(pz.join(ctx, da, db).pairs, exp)
     finally:
-        pz.tune_set(ctx, saved["build_threads"], saved["narrow_queue"])
+        pz.tune_set(ctx, saved["build_threads"], saved["narrow_queue"], saved["narrow_chunk"])
 
 
 def test_tune_set_rejects_bad_sizes(ctx):

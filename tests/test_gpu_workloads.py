@@ -89,3 +89,21 @@ def test_mixed_stream_batch(ctx):
     outs, _ = pz.join_batch(ctx, da, db, [dict(predicate=q[2]) for q in queries])
     for i, ((a, b, p), o) in enumerate(zip(queries, outs)):
         np.testing.assert_array_equal(o.pairs, oracle_pairs(a, b, p), err_msg=f"query {i} predicate {p}")
+
+
+@pytest.mark.parametrize("queue,chunk", [(256, 1), (256, 8), (512, 2)])
+def test_tubes_work_queue_rounds(ctx, tubes, queue, chunk):
+    """Short LDS queues make waves hand subtree tasks to later work-queue rounds."""
+    a, b, da, db, exp = tubes
+    saved = pz.tune_get(ctx)
+    try:
+        pz.tune_set(ctx, saved["build_threads"], narrow_queue=(queue, queue), narrow_chunk=(chunk, chunk))
+        for pred in PREDS:
+            out = pz.join(ctx, da, db, predicate=pred)
+            np.testing.assert_array_equal(out.pairs, exp[pred], err_msg=f"predicate {pred}")
+            print(f"[tubes] queue={queue} chunk={chunk} predicate={pred}: {out.narrow_rounds} rounds, "
+                  f"{out.tasks_spilled} tasks spilled", flush=True)
+        if queue == 256:
+            assert out.tasks_spilled > 0 and out.narrow_rounds > 1
+    finally:
+        pz.tune_set(ctx, saved["build_threads"], saved["narrow_queue"], saved["narrow_chunk"])
