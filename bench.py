@@ -221,9 +221,11 @@ def main():
         tune = pz.autotune(ctx, da, db, **chosen)
         log(f"[bench] autotune: {tune['build_threads']} ({tune['joins_run']} joins)")
         log("[bench] chosen path warm; timing")
+        ms_chosen, out = time_join(pz, ctx, da, db, args.steps, **chosen)
+        # per-kernel device times from HIP events, in separate steps (timers disable graph replay)
         pz.profile_reset()
         pz.profile_enable(True)
-        ms_chosen, out = time_join(pz, ctx, da, db, args.steps, **chosen)
+        time_join(pz, ctx, da, db, args.steps, **chosen)
         pz.profile_enable(False)
         kernels = {}
         for k in ("object_stats", "edge_count", "tri_keys", "edge_scatter", "karras", "refit", "gather_leaves",
@@ -338,6 +340,9 @@ def main():
         "escalated_fp64": out.tests_fp64,
         "escalated_exact": out.tests_exact,
         "fp32_only_misclassified": out.tests_fp32_misclassified,
+        "graph_launches_per_join": out.graph_launches,
+        "narrow_rounds": out.narrow_rounds,
+        "narrow_tasks_spilled": out.tasks_spilled,
         "phases_chosen": [(n, round(m, 3)) for n, m in out.phases],
         "phases_plain": plain_phases,
         "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kernels.items()},

@@ -158,6 +158,10 @@ typedef struct pz_dataset pz_dataset;
 int32_t   pz_device_count(void);
 pz_status pz_ctx_create(int32_t device, pz_ctx** out);
 void      pz_ctx_destroy(pz_ctx* ctx);
+/* Repeated joins of the same datasets replay their fixed-shape segments
+ * (object statistics, object index) from HIP graphs captured on the second
+ * join; on by default.  No reference counterpart. */
+void      pz_ctx_set_graphs(pz_ctx* ctx, int32_t on);
 /* Free and total HBM of the context's device (sizing from free memory,
  * pierce/query/src/applications/pierce_overlap_main.cpp:474-489). */
 pz_status pz_ctx_mem_info(pz_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
@@ -249,7 +253,7 @@ typedef struct {
     uint64_t index_edges;       /* edges regrouped per object (two-level) */
     uint64_t tasks_spilled;     /* narrow-phase subtree tasks handed from one work-queue round to the next */
     int32_t  narrow_rounds;     /* work-queue rounds of the narrow phase */
-    int32_t  reserved_r;
+    int32_t  graph_launches;    /* fixed-shape segments replayed from captured HIP graphs */
     double   ms_total;          /* device-resident geometry -> sorted host pair set */
     pz_phase phases[PZ_MAX_PHASES];
 } pz_join_result;

@@ -93,7 +93,7 @@ class JoinResult(C.Structure):
                 ("tests_fp64", C.c_uint64), ("tests_exact", C.c_uint64), ("tests_fp32_misclassified", C.c_uint64),
                 ("edges_traced", C.c_uint64), ("stack_overflows", C.c_uint64), ("index_triangles", C.c_uint64),
                 ("index_edges", C.c_uint64), ("tasks_spilled", C.c_uint64), ("narrow_rounds", C.c_int32),
-                ("reserved_r", C.c_int32), ("ms_total", C.c_double),
+                ("graph_launches", C.c_int32), ("ms_total", C.c_double),
                 ("phases", Phase * MAX_PHASES)]
 
 
@@ -139,6 +139,7 @@ _SIGS = [
     ("pz_device_count", C.c_int32, []),
     ("pz_ctx_create", C.c_int32, [C.c_int32, C.POINTER(C.c_void_p)]),
     ("pz_ctx_destroy", None, [C.c_void_p]),
+    ("pz_ctx_set_graphs", None, [C.c_void_p, C.c_int32]),
     ("pz_ctx_mem_info", C.c_int32, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("pz_dataset_upload", C.c_int32, [C.c_void_p, C.POINTER(MeshView), C.POINTER(C.c_void_p)]),
     ("pz_dataset_load_pre", C.c_int32, [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p)]),
@@ -369,6 +370,9 @@ class Context:
         except Exception:
             pass
 
+    def set_graphs(self, on: bool) -> None:
+        lib().pz_ctx_set_graphs(self._h, 1 if on else 0)
+
     def upload(self, mesh: Mesh) -> "Dataset":
         return Dataset(self, mesh)
 
@@ -430,6 +434,7 @@ class JoinOutput:
     predicted_pairs: int = 0
     tasks_spilled: int = 0
     narrow_rounds: int = 0
+    graph_launches: int = 0
 
 
 def join_desc(**kw) -> JoinDesc:
@@ -460,7 +465,7 @@ def _output(r: JoinResult) -> JoinOutput:
     return JoinOutput(pairs, types, int(r.num_candidates), int(r.capacity), int(r.predicted), bool(r.overflowed),
                       int(r.tests_total), int(r.tests_fp64), int(r.tests_exact), int(r.tests_fp32_misclassified),
                       int(r.edges_traced), int(r.index_triangles), int(r.index_edges), float(r.ms_total), phases,
-                      int(r.predicted_pairs), int(r.tasks_spilled), int(r.narrow_rounds))
+                      int(r.predicted_pairs), int(r.tasks_spilled), int(r.narrow_rounds), int(r.graph_launches))
 
 
 def join_batch(ctx: Context, a: Dataset, b: Dataset, queries, **common):

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -83,12 +84,6 @@ struct Phases {
 };
 
 // ------------------------------------------------------------- set prep
-// LBVH over object boxes of one set (broad-phase target).
-struct ObjBvh {
-    pzd::Node* nodes;
-    uint32_t* order;
-    uint32_t* height;
-};
 
 ObjBvh build_object_bvh(pz_ctx* c, const pz_dataset* d, const SetPrep& p, const std::string& tag) {
     // Batched sets: one subtree per part (keys carry the part above the Morton bits).
@@ -224,6 +219,80 @@ double grid_estimate(const pz_dataset* a, const pz_dataset* b, float gamma, floa
 
 // ------------------------------------------------------------- two-level
 // meta_of_part: device predicate | dir << 2 per part for batched sets (null: from d).
+// ------------------------------------------------------------ HIP graphs
+// A fixed-shape segment of device work (its launches depend only on the
+// datasets) is run eagerly the first time its key is seen, captured as a HIP
+// graph the second time, and replayed after that.  The pool pointers the
+// segment uses are stored with the graph; any buffer move (pool epoch change)
+// drops the graph and starts over.  Only kernels are captured (no memset or
+// copy nodes), and nothing is captured while per-kernel profiling is on.
+}  // namespace
+struct GraphSeg {
+    uint64_t k1 = 0, k2 = 0;
+    int tag = 0;
+    uint64_t epoch = 0;
+    hipGraphExec_t exec = nullptr;
+    SegOut out;
+};
+namespace {
+
+void drop_graph(GraphSeg* g) {
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    g->exec = nullptr;
+}
+
+template <class F>
+SegOut run_segment(pz_ctx* c, uint64_t k1, uint64_t k2, int tag, pz_join_result* r, F&& enqueue) {
+    GraphSeg* g = nullptr;
+    for (GraphSeg* e : c->segs)
+        if (e->k1 == k1 && e->k2 == k2 && e->tag == tag) g = e;
+    if (!c->graphs || pzk::prof_enabled()) return enqueue();  // plain launches, nothing recorded
+    if (g && g->epoch == c->pool_epoch) {
+        if (!g->exec) {  // second use with unchanged buffers: capture (no allocation can happen)
+            hipGraph_t graph = nullptr;
+            HIP_OK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+            SegOut o;
+            try {
+                o = enqueue();
+            } catch (...) {
+                (void)hipStreamEndCapture(c->s, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            HIP_OK(hipStreamEndCapture(c->s, &graph));
+            const hipError_t e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess || g->epoch != c->pool_epoch) {
+                drop_graph(g);
+                throw PzError(PZ_E_HIP, "graph capture of a fixed-shape segment failed");
+            }
+            g->out = o;
+        }
+        HIP_OK(hipGraphLaunch(g->exec, c->s));
+        if (r) ++r->graph_launches;
+        return g->out;
+    }
+    SegOut o = enqueue();
+    if (!g) {
+        if (c->segs.size() >= 16) {  // keep the most recent keys
+            drop_graph(c->segs.front());
+            delete c->segs.front();
+            c->segs.erase(c->segs.begin());
+        }
+        g = new GraphSeg();
+        g->k1 = k1;
+        g->k2 = k2;
+        g->tag = tag;
+        c->segs.push_back(g);
+    }
+    drop_graph(g);
+    g->epoch = c->pool_epoch;
+    g->out = o;
+    return o;
+}
+
+enum { kSegPrep = 1, kSegObjBvh = 2 };
+
 // preds_out (optional) receives the learned per-part predictions.
 void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const pz_join_desc& d, pz_join_result* r,
                     Phases& ph, const uint8_t* meta_of_part = nullptr,
@@ -236,11 +305,31 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
     const int32_t nA = A->nobj, nB = B->nobj;
 
     ph.begin("Object Bounds");
-    SetPrep pa = prep_set(c, A, "A"), pb = prep_set(c, B, "B");
+    const SegOut prep = run_segment(c, A->uid, B->uid, kSegPrep, r, [&]() {
+        SegOut o;
+        o.pa = prep_set(c, A, "A");
+        o.pb = prep_set(c, B, "B");
+        return o;
+    });
+    const SetPrep pa = prep.pa, pb = prep.pb;
+    // LEARNED: features and inference run on the side stream while the main
+    // stream builds the object index (both only read the object statistics).
+    uint32_t npred = 0;
+    if (d.sizing == PZ_SIZING_LEARNED) npred = mlp_predict_async(c, d.model, A, B, pa, pb, meta_of_part, pair_meta_of(d));
+
+    ph.begin("Build Object Index");
+    const ObjBvh ob = run_segment(c, B->uid, 0, kSegObjBvh, r, [&]() {
+                          SegOut o;
+                          o.ob = build_object_bvh(c, B, pb, "B");
+                          return o;
+                      }).ob;
+
+    ph.begin("Broad Phase");
+    // The broad-phase walk keeps a 64-entry stack: the object LBVH must be shallower.
+    const uint32_t oheight = readback<uint32_t>(c, {ob.height})[0];
     double learned_upper = 0;  // LEARNED: calibrated upper bound of the candidate-pair count
     if (d.sizing == PZ_SIZING_LEARNED) {
-        ph.begin("Selectivity Estimation");
-        std::vector<SizePrediction> pr = mlp_predict(c, d.model, A, B, pa, pb, meta_of_part, pair_meta_of(d));
+        std::vector<SizePrediction> pr = mlp_predict_wait(c, d.model, npred);
         double cand = 0, pairs = 0;
         for (const SizePrediction& p : pr) {
             cand += p.cand;
@@ -251,13 +340,6 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         r->predicted_pairs = (uint64_t)std::llround(pairs);
         if (preds_out) *preds_out = std::move(pr);
     }
-
-    ph.begin("Build Object Index");
-    ObjBvh ob = build_object_bvh(c, B, pb, "B");
-
-    ph.begin("Broad Phase");
-    // The broad-phase walk keeps a 64-entry stack: the object LBVH must be shallower.
-    const uint32_t oheight = readback<uint32_t>(c, {ob.height})[0];
     if (oheight >= 64)
         throw PzError(PZ_E_LIMIT, "object BVH height " + std::to_string(oheight) + " exceeds the broad-phase stack (63)");
     uint64_t np = 0;
@@ -609,15 +691,9 @@ void join_global(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const pz_j
     pzk::fill_ull(stats, 0ull, 8, s);
     ph.begin("Build Mesh1 Index");
     SetPrep pa = prep_set(c, A, "A"), pb = prep_set(c, B, "B");
-    double learned_upper = 0;  // LEARNED: calibrated upper bound of the unique-pair count
-    if (d.sizing == PZ_SIZING_LEARNED) {
-        ph.begin("Selectivity Estimation");
-        std::vector<SizePrediction> pr = mlp_predict(c, d.model, A, B, pa, pb, nullptr, pair_meta_of(d));
-        r->predicted = (uint64_t)std::llround(pr[0].pairs);
-        r->predicted_pairs = r->predicted;
-        learned_upper = pr[0].pairs_upper;
-        ph.begin("Build Mesh1 Index");
-    }
+    // LEARNED: features and inference overlap the set-wide index builds on the side stream
+    uint32_t npred = 0;
+    if (d.sizing == PZ_SIZING_LEARNED) npred = mlp_predict_async(c, d.model, A, B, pa, pb, nullptr, pair_meta_of(d));
     const bool do12 = d.direction != PZ_DIR_2TO1, do21 = d.direction != PZ_DIR_1TO2;
     GlobalBvh ga{}, gb{};
     if (do21) ga = build_global(c, A, pa, "A");
@@ -661,6 +737,13 @@ void join_global(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const pz_j
     uint64_t nuniq = 0;
     uint64_t* uniq = nullptr;
     bool done = false;
+    double learned_upper = 0;  // LEARNED: calibrated upper bound of the unique-pair count
+    if (d.sizing == PZ_SIZING_LEARNED) {
+        std::vector<SizePrediction> pr = mlp_predict_wait(c, d.model, npred);
+        r->predicted = (uint64_t)std::llround(pr[0].pairs);
+        r->predicted_pairs = r->predicted;
+        learned_upper = pr[0].pairs_upper;
+    }
     if (d.dedup == PZ_DEDUP_HASH || d.sizing != PZ_SIZING_TWO_PASS) {
         uint64_t expected = d.sizing == PZ_SIZING_FIXED ? d.capacity : std::max<uint64_t>(r->predicted, 1);
         if (d.sizing == PZ_SIZING_LEARNED) expected = (uint64_t)std::ceil(std::min(std::max(learned_upper, 1.0), 0x1p40));
@@ -750,6 +833,9 @@ pz_status pz_ctx_create(int32_t device, pz_ctx** out) {
         c->device = device;
         HIP_OK(hipSetDevice(device));
         HIP_OK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+        HIP_OK(hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking));
+        HIP_OK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        HIP_OK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
         for (auto& e : c->ev) HIP_OK(hipEventCreate(&e));
         *out = c.release();
         return PZ_OK;
@@ -772,10 +858,19 @@ void pz_ctx_destroy(pz_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->s);
+    (void)hipStreamSynchronize(c->s2);
+    for (GraphSeg* g : c->segs) {
+        drop_graph(g);
+        delete g;
+    }
     for (auto& kv : c->bufs)
         if (kv.second.p) (void)hipFree(kv.second.p);
     if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->pinned_est) (void)hipHostFree(c->pinned_est);
     for (auto& e : c->ev) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(c->ev_fork);
+    (void)hipEventDestroy(c->ev_join);
+    (void)hipStreamDestroy(c->s2);
     (void)hipStreamDestroy(c->s);
     delete c;
 }
@@ -799,8 +894,10 @@ pz_status pz_dataset_upload_parts(pz_ctx* c, const pz_mesh_view* v, const std::v
         return fail(PZ_E_INVALID, "null geometry array");
     return guarded([&]() -> pz_status {
         HIP_OK(hipSetDevice(c->device));
+        static std::atomic<uint64_t> next_uid{1};
         std::unique_ptr<pz_dataset> d(new pz_dataset());
         d->ctx = c;
+        d->uid = next_uid++;
         d->nv = v->num_vertices;
         d->nt = v->num_triangles;
         d->ne = v->num_edges;
@@ -1209,6 +1306,10 @@ pz_status pz_autotune(pz_ctx* c, const pz_dataset* a, const pz_dataset* b, const
     }
     report->queue_class = qc;
     return PZ_OK;
+}
+
+void pz_ctx_set_graphs(pz_ctx* c, int32_t on) {
+    if (c) c->graphs = on != 0;
 }
 
 void pz_profile_enable(int32_t on) { pzk::prof_enable(on != 0); }

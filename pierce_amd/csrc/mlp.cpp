@@ -98,7 +98,7 @@ pz_mlp* new_model(pz_ctx* c) {
 
 // Features of every part of a join (device X, nparts x 32).
 float* features_device(pz_ctx* c, const pz_dataset* a, const pz_dataset* b, const SetPrep& pa, const SetPrep& pb,
-                       const uint8_t* meta_of_part, uint8_t meta, uint32_t* nparts_out) {
+                       const uint8_t* meta_of_part, uint8_t meta, uint32_t* nparts_out, hipStream_t stream) {
     const bool parts = !a->part_obj_off.empty();
     if (parts != !b->part_obj_off.empty() || (parts && a->part_obj_off.size() != b->part_obj_off.size()))
         throw PzError(PZ_E_INVALID, "both datasets must be plain or batched with the same number of parts");
@@ -106,7 +106,7 @@ float* features_device(pz_ctx* c, const pz_dataset* a, const pz_dataset* b, cons
     void* scratch = buf<char>(c, "est.scratch", pzk::features_scratch_bytes(nparts));
     float* X = buf<float>(c, "est.X", (size_t)nparts * pzk::kMlpIn);
     pzk::join_features(pa.obox, pa.tri_cnt, a->nobj, parts ? a->part_of : nullptr, pb.obox, pb.tri_cnt, b->nobj,
-                       parts ? b->part_of : nullptr, nparts, meta_of_part, meta, scratch, X, c->s);
+                       parts ? b->part_of : nullptr, nparts, meta_of_part, meta, scratch, X, stream);
     *nparts_out = nparts;
     return X;
 }
@@ -135,24 +135,40 @@ float quantile(std::vector<float> v, float q) {
 
 }  // namespace
 
-std::vector<SizePrediction> pz::mlp_predict(pz_ctx* c, const pz_mlp* model, const pz_dataset* a,
-                                            const pz_dataset* b, const SetPrep& pa, const SetPrep& pb,
-                                            const uint8_t* meta_of_part, uint8_t meta) {
+uint32_t pz::mlp_predict_async(pz_ctx* c, const pz_mlp* model, const pz_dataset* a, const pz_dataset* b,
+                               const SetPrep& pa, const SetPrep& pb, const uint8_t* meta_of_part, uint8_t meta) {
     check_model(c, model);
     pz_mlp* m = const_cast<pz_mlp*>(model);
+    HIP_OK(hipEventRecord(c->ev_fork, c->s));
+    HIP_OK(hipStreamWaitEvent(c->s2, c->ev_fork, 0));
     uint32_t np = 0;
-    float* X = features_device(c, a, b, pa, pb, meta_of_part, meta, &np);
+    float* X = features_device(c, a, b, pa, pb, meta_of_part, meta, &np, c->s2);
     float* Y = buf<float>(c, "est.Y", (size_t)np * pzk::kMlpOut);
-    ensure_packed(m);
-    pzk::mlp_infer_bf16(X, np, m->d_norm, m->d_w16, m->d_params, Y, c->s);
-    float* h = static_cast<float*>(pinned(c, sizeof(float) * pzk::kMlpOut * np));
-    HIP_OK(hipMemcpyAsync(h, Y, sizeof(float) * pzk::kMlpOut * np, hipMemcpyDeviceToHost, c->s));
-    HIP_OK(hipStreamSynchronize(c->s));
+    if (!m->packed) {
+        pzk::mlp_pack_bf16(m->d_params, m->d_w16, c->s2);
+        m->packed = true;
+    }
+    pzk::mlp_infer_bf16(X, np, m->d_norm, m->d_w16, m->d_params, Y, c->s2);
+    const size_t bytes = sizeof(float) * pzk::kMlpOut * np;
+    if (c->pinned_est_n < bytes) {
+        if (c->pinned_est) HIP_OK(hipHostFree(c->pinned_est));
+        c->pinned_est = nullptr;
+        HIP_OK(hipHostMalloc(&c->pinned_est, bytes, hipHostMallocDefault));
+        c->pinned_est_n = bytes;
+    }
+    HIP_OK(hipMemcpyAsync(c->pinned_est, Y, bytes, hipMemcpyDeviceToHost, c->s2));
+    HIP_OK(hipEventRecord(c->ev_join, c->s2));
+    return np;
+}
+
+std::vector<SizePrediction> pz::mlp_predict_wait(pz_ctx* c, const pz_mlp* model, uint32_t np) {
+    HIP_OK(hipEventSynchronize(c->ev_join));
+    const float* h = static_cast<const float*>(c->pinned_est);
     std::vector<SizePrediction> out(np);
     for (uint32_t p = 0; p < np; ++p) {
         auto val = [](float y) { return std::exp2(std::min(std::max((double)y, 0.0), 50.0)) - 1.0; };
         const float y0 = h[2 * p], y1 = h[2 * p + 1];
-        out[p] = {val(y0), val(y1), val(y0 + m->margin[0]), val(y1 + m->margin[1])};
+        out[p] = {val(y0), val(y1), val(y0 + model->margin[0]), val(y1 + model->margin[1])};
     }
     return out;
 }
@@ -277,7 +293,7 @@ pz_status pz_join_features(pz_ctx* c, const pz_dataset* a, const pz_dataset* b, 
         HIP_OK(hipSetDevice(c->device));
         SetPrep pa = prep_set(c, a, "A"), pb = prep_set(c, b, "B");
         uint32_t np = 0;
-        float* X = features_device(c, a, b, pa, pb, nullptr, meta_of_predicate(predicate), &np);
+        float* X = features_device(c, a, b, pa, pb, nullptr, meta_of_predicate(predicate), &np, c->s);
         download(c, out, X, sizeof(float) * pzk::kMlpIn);
         return PZ_OK;
     });
@@ -298,7 +314,7 @@ pz_status pz_batch_features(pz_ctx* c, const pz_dataset* a, const pz_dataset* b,
         upload(c, dm, hm.data(), nq);
         SetPrep pa = prep_set(c, a, "A"), pb = prep_set(c, b, "B");
         uint32_t np = 0;
-        float* X = features_device(c, a, b, pa, pb, dm, 0, &np);
+        float* X = features_device(c, a, b, pa, pb, dm, 0, &np, c->s);
         download(c, out, X, sizeof(float) * pzk::kMlpIn * nq);
         return PZ_OK;
     });

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "host/host_mesh.hpp"
+#include "kernels/pz_device.h"
 #include "pierce_hip.h"
 
 struct DevBuf {
@@ -33,9 +34,16 @@ struct TuneTable {
 
 struct pz_ctx {
     int device = 0;
+    hipStream_t s2 = nullptr;          // side stream (default priority) for work that overlaps the main stream
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    void* pinned_est = nullptr;        // host landing area of the side stream's predictions
+    size_t pinned_est_n = 0;
     TuneTable tune;
     int last_queue_class = 0;  // narrow-queue class of the last two-level join (what pz_autotune tunes)
     unsigned persistent_blocks = 0;  // grid of the persistent narrow kernel: 8 blocks per CU
+    uint64_t pool_epoch = 0;         // bumped whenever a pool buffer moves (captured graphs go stale)
+    bool graphs = true;              // capture repeated fixed-shape segments as HIP graphs
+    std::vector<struct GraphSeg*> segs;
     hipStream_t s = nullptr;
     std::unordered_map<std::string, DevBuf> bufs;
     void* pinned = nullptr;  // small readbacks
@@ -45,6 +53,7 @@ struct pz_ctx {
 
 struct pz_dataset {
     pz_ctx* ctx = nullptr;
+    uint64_t uid = 0;  // unique per upload (keys the captured HIP graphs)
     float* v = nullptr;
     uint32_t* idx = nullptr;
     int32_t* t2o = nullptr;
@@ -89,6 +98,7 @@ inline T* buf(pz_ctx* c, const std::string& name, size_t count) {
         size_t want = bytes + bytes / 4;
         HIP_OK(hipMalloc(&b.p, want));
         b.n = want;
+        ++c->pool_epoch;
     }
     return reinterpret_cast<T*>(b.p);
 }
@@ -145,6 +155,19 @@ struct SetPrep {
 };
 SetPrep prep_set(pz_ctx* c, const pz_dataset* d, const std::string& tag);
 
+// LBVH over the object boxes of one set (broad-phase target).
+struct ObjBvh {
+    pzd::Node* nodes;
+    uint32_t* order;
+    uint32_t* height;
+};
+
+// Host-side results of a segment of device work (pool pointers), kept with its graph.
+struct SegOut {
+    SetPrep pa{}, pb{};
+    ObjBvh ob{};
+};
+
 // predicate | direction bits << 2 of one join description
 uint8_t pair_meta_of(const pz_join_desc& d);
 
@@ -157,7 +180,10 @@ struct SizePrediction {
     double cand_upper, pairs_upper;
 };
 // meta_of_part: device predicate | dir << 2 per part (batched) or null (meta for every part).
-std::vector<SizePrediction> mlp_predict(pz_ctx* c, const pz_mlp* model, const pz_dataset* a, const pz_dataset* b,
-                                        const SetPrep& pa, const SetPrep& pb, const uint8_t* meta_of_part,
-                                        uint8_t meta);
+// Enqueues features, inference and the copy of the outputs to the host on the
+// side stream after everything enqueued so far on the main stream; collect
+// with mlp_predict_wait.  Returns the number of parts.
+uint32_t mlp_predict_async(pz_ctx* c, const pz_mlp* model, const pz_dataset* a, const pz_dataset* b,
+                           const SetPrep& pa, const SetPrep& pb, const uint8_t* meta_of_part, uint8_t meta);
+std::vector<SizePrediction> mlp_predict_wait(pz_ctx* c, const pz_mlp* model, uint32_t nparts);
 }  // namespace pz

@@ -109,4 +109,3 @@ def test_learned_single_join(ctx, trained, strategy):
     got = pz.join(ctx, da, db, strategy=strategy, sizing=pz.SIZING_LEARNED, model=m)
     np.testing.assert_array_equal(got.pairs, want.pairs)
     assert got.predicted > 0 and got.predicted_pairs > 0
-    assert any(name == "Selectivity Estimation" for name, _ in got.phases)
