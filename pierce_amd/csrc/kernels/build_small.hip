@@ -42,14 +42,22 @@ __global__ void k_need_split(const uint32_t* __restrict__ need, const uint32_t* 
                              const uint32_t* __restrict__ first_tri, const uint32_t* __restrict__ last_tri,
                              int32_t nobj, uint32_t* __restrict__ big, uint32_t* __restrict__ cls,
                              uint32_t* __restrict__ totals) {
-    int32_t o = blockIdx.x * blockDim.x + threadIdx.x;
-    if (o >= nobj) return;
-    const uint32_t n = tri_cnt[o];
-    const bool contiguous = n > 0 && last_tri[o] - first_tri[o] + 1 == n;
-    const int c = (need[o] && contiguous) ? small_class(n) : -1;
-    big[o] = (need[o] && c < 0) ? 1u : 0u;
-    for (int k = 0; k < pzk::kSmallClasses; ++k) cls[(uint64_t)k * (nobj + 1) + o] = (c == k) ? 1u : 0u;
-    if (c >= 0) atomicMax(&totals[5 + c], n);
+    const int32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = -1;
+    uint32_t n = 0;
+    if (o < nobj) {
+        n = tri_cnt[o];
+        const bool contiguous = n > 0 && last_tri[o] - first_tri[o] + 1 == n;
+        c = (need[o] && contiguous) ? small_class(n) : -1;
+        big[o] = (need[o] && c < 0) ? 1u : 0u;
+        for (int k = 0; k < pzk::kSmallClasses; ++k) cls[(uint64_t)k * (nobj + 1) + o] = (c == k) ? 1u : 0u;
+    }
+    // largest object per class: wave maximum first, one atomic per wave and class
+    for (int k = 0; k < pzk::kSmallClasses; ++k) {
+        uint32_t m = c == k ? n : 0u;
+        for (int sft = 32; sft > 0; sft >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, sft, 64));
+        if (__lane_id() == 0 && m > 0) atomicMax(&totals[5 + k], m);
+    }
 }
 
 // loc = big ? loc_big : nbig + (objects of lower classes) + loc within the class.

@@ -270,28 +270,39 @@ __global__ __launch_bounds__(kBlock) void k_feat_cells(const uint32_t* __restric
 // centres fall uniformly in one unit cell overlap: P(|x - y| <= d) = d(2 - d).
 __device__ inline double overlap_prob(double d) { return d >= 1.0 ? 1.0 : d * (2.0 - d); }
 
-// Expected overlapping box pairs per cell; histogram over log2 bins.
+// Expected overlapping box pairs per cell; histogram over log2 bins.  A block
+// covers 256 cells of one part (4096 cells per part): it sums its bins in LDS
+// and adds each non-zero bin to the part's histogram once.
 __global__ __launch_bounds__(kBlock) void k_feat_hist(const CellAcc* __restrict__ cells, uint32_t nparts,
                                                       unsigned long long* __restrict__ hist) {
+    static_assert(kFeatCells % kBlock == 0, "a block stays inside one part");
+    __shared__ unsigned long long bins[kHistSlots];
+    if (threadIdx.x < kHistSlots) bins[threadIdx.x] = 0ull;
+    __syncthreads();
     const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= (uint64_t)nparts * kFeatCells) return;
-    const uint64_t p = i / kFeatCells, c = i % kFeatCells;
-    const CellAcc& a = cells[(2 * p) * kFeatCells + c];
-    const CellAcc& b = cells[(2 * p + 1) * kFeatCells + c];
-    if (a.cnt == 0 || b.cnt == 0) return;
-    double est = (double)a.cnt * (double)b.cnt;
-    for (int k = 0; k < 3; ++k) {
-        const double ea = (double)a.ext[k] / a.cnt / kFix, eb = (double)b.ext[k] / b.cnt / kFix;
-        est *= overlap_prob(0.5 * (ea + eb));
+    const uint64_t p = ((uint64_t)blockIdx.x * kBlock) / kFeatCells;
+    if (i < (uint64_t)nparts * kFeatCells) {
+        const uint64_t c = i % kFeatCells;
+        const CellAcc& a = cells[(2 * p) * kFeatCells + c];
+        const CellAcc& b = cells[(2 * p + 1) * kFeatCells + c];
+        if (a.cnt != 0 && b.cnt != 0) {
+            double est = (double)a.cnt * (double)b.cnt;
+            for (int k = 0; k < 3; ++k) {
+                const double ea = (double)a.ext[k] / a.cnt / kFix, eb = (double)b.ext[k] / b.cnt / kFix;
+                est *= overlap_prob(0.5 * (ea + eb));
+            }
+            if (est > 0) {
+                const int bin = min(max((int)floor(log2(est)) + 4, 0), kFeatBins - 1);
+                const unsigned long long qv = (unsigned long long)(fmin(est, 0x1p40) * kFix + 0.5);
+                atomicAdd(&bins[bin], qv);
+                atomicAdd(&bins[kFeatBins], qv);
+            }
+            atomicAdd(&bins[kFeatBins + 1], 1ull);
+        }
     }
-    unsigned long long* h = hist + p * kHistSlots;
-    if (est > 0) {
-        const int bin = min(max((int)floor(log2(est)) + 4, 0), kFeatBins - 1);
-        const unsigned long long qv = (unsigned long long)(fmin(est, 0x1p40) * kFix + 0.5);
-        atomicAdd(&h[bin], qv);
-        atomicAdd(&h[kFeatBins], qv);
-    }
-    atomicAdd(&h[kFeatBins + 1], 1ull);
+    __syncthreads();
+    if (threadIdx.x < kHistSlots && p < nparts && bins[threadIdx.x] != 0ull)
+        atomicAdd(&hist[p * kHistSlots + threadIdx.x], bins[threadIdx.x]);
 }
 
 __global__ void k_feat_finalize(const PartAcc* __restrict__ acc, const unsigned long long* __restrict__ hist,

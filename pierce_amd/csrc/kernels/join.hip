@@ -150,24 +150,38 @@ __global__ __launch_bounds__(kBlock) void k_edge_count(const int32_t* __restrict
 }
 
 // Union of object boxes (ordered uints) into scene[6].
+// Scene box of the object boxes: a grid-stride loop over objects, then wave
+// and block reductions, so each block issues one atomic per bound.
+constexpr unsigned kSceneBlocks = 64;
 __global__ __launch_bounds__(kBlock) void k_scene_box(const uint32_t* __restrict__ obox, int32_t nobj,
                                                       uint32_t* __restrict__ scene) {
-    int32_t o = blockIdx.x * kBlock + threadIdx.x;
-    uint32_t v[6];
-    for (int a = 0; a < 3; ++a) {
-        v[a] = o < nobj ? obox[6ll * o + a] : 0xFFFFFFFFu;
-        v[3 + a] = o < nobj ? obox[6ll * o + 3 + a] : 0u;
-    }
-    for (int a = 0; a < 3; ++a)
-        for (int s = 32; s > 0; s >>= 1) {
-            v[a] = min(v[a], (uint32_t)__shfl_xor((int)v[a], s, 64));
-            v[3 + a] = max(v[3 + a], (uint32_t)__shfl_xor((int)v[3 + a], s, 64));
+    __shared__ uint32_t part[kBlock / PZ_WAVE][6];
+    uint32_t v[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+    for (int32_t o = blockIdx.x * kBlock + threadIdx.x; o < nobj; o += gridDim.x * kBlock)
+        for (int a = 0; a < 3; ++a) {
+            v[a] = min(v[a], obox[6ll * o + a]);
+            v[3 + a] = max(v[3 + a], obox[6ll * o + 3 + a]);
         }
+    for (int a = 0; a < 3; ++a)
+        for (int sft = 32; sft > 0; sft >>= 1) {
+            v[a] = min(v[a], (uint32_t)__shfl_xor((int)v[a], sft, 64));
+            v[3 + a] = max(v[3 + a], (uint32_t)__shfl_xor((int)v[3 + a], sft, 64));
+        }
+    const int w = threadIdx.x / PZ_WAVE;
     if (__lane_id() == 0)
+        for (int a = 0; a < 6; ++a) part[w][a] = v[a];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBlock / PZ_WAVE; ++k)
+            for (int a = 0; a < 3; ++a) {
+                v[a] = min(v[a], part[k][a]);
+                v[3 + a] = max(v[3 + a], part[k][3 + a]);
+            }
         for (int a = 0; a < 3; ++a) {
             atomicMin(&scene[a], v[a]);
             atomicMax(&scene[3 + a], v[3 + a]);
         }
+    }
 }
 
 __device__ __forceinline__ void load_scene(const uint32_t* scene, float* lo, float* inv) {
@@ -1174,7 +1188,9 @@ void edge_count(const int32_t* eo, uint64_t ne, int32_t nobj, uint32_t* edge_cnt
 
 void scene_box(const uint32_t* obox, int32_t nobj, uint32_t* scene, hipStream_t s) {
     hipLaunchKernelGGL(k_init_boxes, dim3(1), dim3(kBlock), 0, s, scene, (int64_t)1);
-    if (nobj > 0) hipLaunchKernelGGL(k_scene_box, dim3(grid1((uint64_t)nobj)), dim3(kBlock), 0, s, obox, nobj, scene);
+    if (nobj > 0)
+        hipLaunchKernelGGL(k_scene_box, dim3(std::min(grid1((uint64_t)nobj), kSceneBlocks)), dim3(kBlock), 0, s, obox,
+                           nobj, scene);
 }
 
 void object_keys(const uint32_t* obox, int32_t nobj, const uint32_t* scene, const uint32_t* part_of, uint64_t* keys,
