@@ -138,6 +138,97 @@ def stream_sizing(pz, ctx, model, n, batch=48):
     return res
 
 
+def extra_configs(pz, ctx, model, steps):
+    """The other configurations: chosen path vs the plain GPU path on the same GPU.
+
+    C1 (1000 vs 1000 ellipsoids, s=2) is also checked against the CPU oracle in full.
+    C3 (10k vs 10k spheres, s in 1..6) runs its three predicates as one batch.
+    C4 is reduced to 200 vs 200 tubes (same triangle-count law, density kept)."""
+    from oracle import oracle as orc
+
+    learned = dict(sizing=pz.SIZING_LEARNED, model=model) if model else dict(sizing=pz.SIZING_TWO_PASS)
+    chosen = dict(strategy=pz.STRATEGY_TWO_LEVEL, precision=pz.PREC_FP32_FILTERED, **learned)
+    plain = dict(strategy=pz.STRATEGY_GLOBAL, precision=pz.PREC_FP64, sizing=pz.SIZING_TWO_PASS, dedup=pz.DEDUP_SORT)
+    plain_tl = dict(strategy=pz.STRATEGY_TWO_LEVEL, precision=pz.PREC_FP64, sizing=pz.SIZING_TWO_PASS)
+    res = {}
+
+    def single(name, a, b, oracle=False, plain_steps=1):
+        da, db = ctx.upload(a), ctx.upload(b)
+        pz.join(ctx, da, db, **chosen)
+        ms_c, out = time_join(pz, ctx, da, db, steps, **chosen)
+        pz.join(ctx, da, db, **plain)
+        ms_p, pout = time_join(pz, ctx, da, db, plain_steps, **plain)
+        r = {"objects": [a.num_objects, b.num_objects], "triangles": [int(a.indices.shape[0]), int(b.indices.shape[0])],
+             "join_ms_chosen": round(ms_c, 3), "join_ms_plain": round(ms_p, 3), "speedup_vs_plain": round(ms_p / ms_c, 2),
+             "mismatched_pairs": pair_mismatch(out.pairs, pout.pairs), "pairs": int(len(out.pairs)),
+             "tests": out.tests_total, "escalated_fp64": out.tests_fp64, "escalated_exact": out.tests_exact,
+             "fp32_only_misclassified": out.tests_fp32_misclassified, "sizing_overflowed": out.overflowed,
+             "narrow_rounds": out.narrow_rounds}
+        if oracle:
+            exp = orc.face_overlap(a, b)
+            r["oracle_mismatched_pairs"] = pair_mismatch(out.pairs, exp)
+        da.close()
+        db.close()
+        log(f"[bench] {name}: chosen {ms_c:.3f} ms, plain {ms_p:.3f} ms, mismatches {r['mismatched_pairs']}")
+        return r
+
+    try:
+        a = pz.generate(pz.GEN_ELLIPSOIDS, count=1000, seed=42, box=100.0, jitter=0.02)
+        b = pz.generate(pz.GEN_ELLIPSOIDS, count=1000, seed=43, box=100.0, jitter=0.02)
+        res["c1"] = dict(workload="overlap, 1000 vs 1000 ellipsoid icospheres s=2 (320 tris), radii U[1,3], "
+                                  "centres in 100^3, jitter 0.02 r", **single("C1", a, b, oracle=True, plain_steps=steps))
+    except Exception as e:  # keep the headline line even if a side configuration fails
+        res["c1"] = {"error": str(e)}
+    try:
+        kw = dict(level_min=1, level_max=6, radius_min=1.0, radius_max=4.0, box=500.0)
+        a = pz.generate(pz.GEN_SPHERES_MIXED, count=10000, seed=42, **kw)
+        b = pz.generate(pz.GEN_SPHERES_MIXED, count=10000, seed=43, **kw)
+        preds = [pz.PRED_OVERLAP, pz.PRED_INTERSECTION, pz.PRED_CONTAINMENT]
+        da3, db3 = ctx.upload_batch([a] * 3), ctx.upload_batch([b] * 3)
+        q = [dict(predicate=p) for p in preds]
+        pz.join_batch(ctx, da3, db3, q, **chosen)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            outs, summ = pz.join_batch(ctx, da3, db3, q, **chosen)
+        ms_c = (time.perf_counter() - t0) / steps * 1e3
+        da, db = ctx.upload(a), ctx.upload(b)
+        for d in (plain, plain_tl):
+            pz.join(ctx, da, db, **d)  # warm
+        t0 = time.perf_counter()
+        pouts = [pz.join(ctx, da, db, **plain)]
+        pouts += [pz.join(ctx, da, db, predicate=p, **plain_tl) for p in preds[1:]]
+        ms_p = (time.perf_counter() - t0) * 1e3
+        mism = sum(pair_mismatch(o.pairs, po.pairs) for o, po in zip(outs, pouts))
+        res["c3"] = {"workload": "overlap + intersection + containment, 10000 vs 10000 spheres, subdivision 1-6 with "
+                                 "P ~ 2^-l (80-81920 tris), radius U[1,4], centres in 500^3",
+                     "chosen": "the three predicates as one batched join (fp32-filtered, learned sizing)",
+                     "plain": "one join per launch: overlap on the set-wide path, intersection and containment on "
+                              "the two-level path, fp64 filter, two-pass sizing",
+                     "objects": [a.num_objects, b.num_objects],
+                     "triangles": [int(a.indices.shape[0]), int(b.indices.shape[0])],
+                     "join_ms_chosen": round(ms_c, 3), "join_ms_plain": round(ms_p, 3),
+                     "speedup_vs_plain": round(ms_p / ms_c, 2), "mismatched_pairs": int(mism),
+                     "pairs": [int(len(o.pairs)) for o in outs], "tests": summ.tests_total,
+                     "escalated_fp64": summ.tests_fp64, "escalated_exact": summ.tests_exact,
+                     "fp32_only_misclassified": summ.tests_fp32_misclassified, "narrow_rounds": summ.narrow_rounds}
+        log(f"[bench] C3: batch {ms_c:.3f} ms, plain {ms_p:.3f} ms, mismatches {mism}")
+        for d in (da3, db3, da, db):
+            d.close()
+    except Exception as e:
+        res["c3"] = {"error": str(e)}
+    try:
+        kw = dict(box=464.0)
+        a = pz.generate(pz.GEN_TUBES, count=200, seed=42, **kw)
+        b = pz.generate(pz.GEN_TUBES, count=200, seed=43, degenerate_fraction=0.01, copy_source_count=200,
+                        copy_source_seed=42, **kw)
+        res["c4_reduced"] = dict(workload="overlap, 200 vs 200 branching tubes (lognormal median 200k tris, "
+                                          "clipped [50k, 2M]), centres in 464^3 (the 2000-mesh density), 1% "
+                                          "near-degenerate copies", **single("C4 (reduced)", a, b))
+    except Exception as e:
+        res["c4_reduced"] = {"error": str(e)}
+    return res
+
+
 def pair_mismatch(p, q):
     a = set(map(tuple, p.tolist()))
     b = set(map(tuple, q.tolist()))
@@ -155,6 +246,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-side", type=float, default=500.0, help="side of the CPU sample sub-box")
     ap.add_argument("--no-learned", action="store_true", help="two-pass sizing only (no estimator training)")
+    ap.add_argument("--no-extra", action="store_true", help="skip the C1 / C3 / reduced C4 comparison lines")
     ap.add_argument("--train-queries", type=int, default=768, help="queries of the estimator's training stream")
     ap.add_argument("--stream-queries", type=int, default=480, help="held-out queries of the sizing stream (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -266,6 +358,7 @@ def main():
                 log(f"[bench] global fp64 with learned hash sizing {ms_gl:.3f} ms/join (two-pass {ms_plain:.3f})")
             if args.stream_queries > 0:
                 learned["stream"] = stream_sizing(pz, ctx, model, args.stream_queries)
+        other = None if args.no_extra else extra_configs(pz, ctx, model, args.steps)
 
         # Roofline of the dominant profiled kernel (largest device time in the timed steps).
         counts.update(IT=out.index_triangles, IE=out.index_edges)
@@ -350,6 +443,7 @@ def main():
         "cpu_baseline": cpu,
         "learned_sizing": learned,
         "autotune": tune,
+        "other_configs": other,
     }
     print(json.dumps(line), flush=True)
 
