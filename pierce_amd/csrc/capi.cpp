@@ -109,8 +109,9 @@ ObjBvh build_object_bvh(pz_ctx* c, const pz_dataset* d, const SetPrep& p, const 
     b.order = alt ? vals2 : vals;
     pzk::fill_u32(b.height, 0u, 1, c->s);
     const uint32_t* seg = parts ? d->part_off : nullptr;
-    pzk::lbvh_karras(sk, n, seg, 30, b.nodes, pint, pleaf, cnt, c->s);
-    pzk::lbvh_refit_ord_boxes(sk, b.order, n, seg, 30, p.obox, b.nodes, pint, pleaf, cnt, b.height, c->s);
+    uint32_t* rng = buf<uint32_t>(c, tag + ".orange", n + 1);
+    pzk::lbvh_karras(sk, n, seg, 30, b.nodes, pint, pleaf, cnt, rng, c->s);
+    pzk::lbvh_refit_ord_boxes(sk, b.order, n, seg, 30, p.obox, b.nodes, pint, pleaf, cnt, rng, b.height, c->s);
     return b;
 }
 
@@ -481,7 +482,7 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
             uint32_t* vals2 = buf<uint32_t>(c, tag + ".tvals2", nb_tri + 1);
             pzk::fill_u32(cursor, 0u, (size_t)std::max(D->nobj, 1), s);
             pzk::tri_keys_two_level(D->v, D->nv, D->idx, D->t2o, D->nt, D->nobj, big, N.loc, P.obox, N.seg_off,
-                                    cursor, keys, vals, s);
+                                    P.first_tri, P.last_tri, P.tri_cnt, cursor, keys, vals, s);
             void* tmp = buf<char>(c, "sort.tmp." + tag, pzk::radix_sort_temp_bytes(nb_tri));
             bool alt = pzk::radix_sort_pairs(keys, vals, keys2, vals2, nb_tri, 0,
                                              30 + bits_for(std::max<uint64_t>(T.nbig, 1)), tmp, s);
@@ -490,8 +491,9 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
             uint32_t* pint = buf<uint32_t>(c, tag + ".pint", nb_tri + 1);
             uint32_t* pleaf = buf<uint32_t>(c, tag + ".pleaf", nb_tri + 1);
             uint32_t* cnt = buf<uint32_t>(c, tag + ".ncnt", nb_tri + 1);
-            pzk::lbvh_karras(sk, nb_tri, N.seg_off, 30, acc.nodes, pint, pleaf, cnt, s);
-            pzk::lbvh_refit_tris(sk, order, nb_tri, N.seg_off, 30, D->v, D->idx, acc.nodes, pint, pleaf, cnt,
+            uint32_t* rng = buf<uint32_t>(c, tag + ".trange", nb_tri + 1);
+            pzk::lbvh_karras(sk, nb_tri, N.seg_off, 30, acc.nodes, pint, pleaf, cnt, rng, s);
+            pzk::lbvh_refit_tris(sk, order, nb_tri, N.seg_off, 30, D->v, D->idx, acc.nodes, pint, pleaf, cnt, rng,
                                  acc.height, s);
             pzk::gather_leaf_tris(order, nb_tri, D->v, D->idx, D->t2o, acc.leaves, s);
         }
@@ -672,8 +674,9 @@ GlobalBvh build_global(pz_ctx* c, const pz_dataset* D, const SetPrep& P, const s
     uint32_t* cnt = buf<uint32_t>(c, tag + ".gcnt", n + 1);
     uint32_t* h = buf<uint32_t>(c, tag + ".gheight", 1);
     pzk::fill_u32(h, 0u, 1, s);
-    pzk::lbvh_karras(sk, n, nullptr, 0, g.nodes, pint, pleaf, cnt, s);
-    pzk::lbvh_refit_tris(sk, order, n, nullptr, 0, D->v, D->idx, g.nodes, pint, pleaf, cnt, h, s);
+    uint32_t* rng = buf<uint32_t>(c, tag + ".grange", n + 1);
+    pzk::lbvh_karras(sk, n, nullptr, 0, g.nodes, pint, pleaf, cnt, rng, s);
+    pzk::lbvh_refit_tris(sk, order, n, nullptr, 0, D->v, D->idx, g.nodes, pint, pleaf, cnt, rng, h, s);
     g.leaves = buf<pzd::LeafTri>(c, tag + ".gleaves", n + 1);
     pzk::gather_leaf_tris(order, n, D->v, D->idx, D->t2o, g.leaves, s);
     g.height = readback<uint32_t>(c, {h})[0];

@@ -5,7 +5,9 @@
 // its internal nodes occupy slots [begin, end - 1) and its root is slot
 // `begin` (or the single leaf when end - begin == 1).  Keys tie-break on the
 // sorted position, so duplicate Morton codes still give a valid radix tree.
-// Boxes are filled bottom-up with one atomic counter per internal node.
+// Boxes are filled bottom-up with one counter per internal node: in LDS for
+// nodes whose leaves all belong to one workgroup's block of 2,048 sorted
+// positions, in HBM (agent-scope atomics) for the few that span blocks.
 #include "pz_device.h"
 #include "pz_kernels_bvh.h"
 
@@ -34,7 +36,8 @@ __device__ __forceinline__ void seg_range(const uint32_t* __restrict__ seg_off, 
 
 __global__ void k_karras(const uint64_t* __restrict__ keys, uint64_t n, const uint32_t* __restrict__ seg_off,
                          int seg_shift, Node* __restrict__ nodes, uint32_t* __restrict__ parent_int,
-                         uint32_t* __restrict__ parent_leaf, uint32_t* __restrict__ counters) {
+                         uint32_t* __restrict__ parent_leaf, uint32_t* __restrict__ counters,
+                         uint32_t* __restrict__ range_end) {
     uint64_t gi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gi >= n) return;
     int64_t b, e;
@@ -50,6 +53,7 @@ __global__ void k_karras(const uint64_t* __restrict__ keys, uint64_t n, const ui
     for (int64_t t = lmax >> 1; t >= 1; t >>= 1)
         if (delta(keys, i, i + (l + t) * d, b, e) > dmin) l += t;
     const int64_t j = i + l * d;
+    range_end[i] = (uint32_t)j;  // node i covers sorted positions [min(i, j), max(i, j)]
     const int dnode = delta(keys, i, j, b, e);
     int64_t s = 0;
     int64_t div = 2;
@@ -114,41 +118,89 @@ __device__ __forceinline__ void load_child_box(const Node* nd, int slot, float* 
     *height = __float_as_int(ld_agent(f + 14 + slot));
 }
 
-template <class BoxFn>
-__device__ void refit_from_leaf(uint64_t p, const uint64_t* __restrict__ keys, uint64_t n,
-                                const uint32_t* __restrict__ seg_off, int seg_shift, Node* nodes,
-                                const uint32_t* __restrict__ parent_int, const uint32_t* __restrict__ parent_leaf,
-                                uint32_t* counters, uint32_t* max_depth, BoxFn boxfn) {
-    int64_t b, e;
-    seg_range(seg_off, keys, seg_shift, n, p, b, e);
-    float lo[3], hi[3];
-    boxfn(p, lo, hi);
-    if (e - b == 1) return;
-    uint32_t node = parent_leaf[p];
-    int child = ~(int)p;
-    int height = 0;  // height of the subtree carried upward (leaf = 0)
-    while (true) {
-        Node* nd = nodes + node;
-        int slot = (nd->d.x == child) ? 0 : 1;
-        store_child_box(nd, slot, lo, hi, height);
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        __builtin_amdgcn_s_waitcnt(0);  // stores complete before the counter moves
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        uint32_t prev = __hip_atomic_fetch_add(&counters[node], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev == 0) return;  // sibling not done yet; it continues upward
-        float slo[3], shi[3];
-        int sh;
-        load_child_box(nd, 1 - slot, slo, shi, &sh);
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = fminf(lo[a], slo[a]);
-            hi[a] = fmaxf(hi[a], shi[a]);
-        }
-        height = 1 + (height > sh ? height : sh);
-        if ((int64_t)node == b) break;  // segment root
-        child = (int)node;
-        node = parent_int[node];
+// Child boxes of nodes whose whole leaf range lies in one workgroup's block
+// never leave the CU: plain stores, a workgroup-scope fence and an LDS
+// counter per node.  Only the few nodes that span blocks (the top levels of
+// each segment) use the agent-scope protocol above.
+constexpr int kRefitBlock = 2048;
+constexpr int kRefitThreads = 256;
+
+__device__ __forceinline__ void store_child_box_local(Node* nd, int slot, const float* lo, const float* hi,
+                                                      int height) {
+    float* f = reinterpret_cast<float*>(nd);
+    const int o = slot == 0 ? 0 : 6;
+    for (int a = 0; a < 3; ++a) {
+        f[o + a] = lo[a];
+        f[o + 3 + a] = hi[a];
     }
-    atomicMax(max_depth, (uint32_t)height);
+    f[14 + slot] = __int_as_float(height);
+}
+__device__ __forceinline__ void load_child_box_local(const Node* nd, int slot, float* lo, float* hi, int* height) {
+    const float* f = reinterpret_cast<const float*>(nd);
+    const int o = slot == 0 ? 0 : 6;
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = f[o + a];
+        hi[a] = f[o + 3 + a];
+    }
+    *height = __float_as_int(f[14 + slot]);
+}
+
+template <class BoxFn>
+__device__ void refit_blocked(const uint64_t* __restrict__ keys, uint64_t n, const uint32_t* __restrict__ seg_off,
+                              int seg_shift, Node* nodes, const uint32_t* __restrict__ parent_int,
+                              const uint32_t* __restrict__ parent_leaf, uint32_t* counters,
+                              const uint32_t* __restrict__ range_end, uint32_t* max_depth, BoxFn boxfn) {
+    __shared__ uint32_t lcnt[kRefitBlock];
+    const uint64_t B = (uint64_t)blockIdx.x * kRefitBlock;
+    const uint64_t E = B + kRefitBlock < n ? B + kRefitBlock : n;
+    for (int i = threadIdx.x; i < kRefitBlock; i += kRefitThreads) lcnt[i] = 0u;
+    __syncthreads();
+    for (uint64_t p = B + threadIdx.x; p < E; p += kRefitThreads) {
+        int64_t b, e;
+        seg_range(seg_off, keys, seg_shift, n, p, b, e);
+        float lo[3], hi[3];
+        boxfn(p, lo, hi);
+        if (e - b == 1) continue;
+        uint32_t node = parent_leaf[p];
+        int child = ~(int)p;
+        int height = 0;
+        while (true) {
+            Node* nd = nodes + node;
+            const int slot = (nd->d.x == child) ? 0 : 1;
+            const uint64_t j = range_end[node];
+            const uint64_t rlo = node < j ? node : j, rhi = node < j ? j : node;
+            float slo[3], shi[3];
+            int sh;
+            if (rlo >= B && rhi < E) {  // both subtrees were finished by this workgroup
+                store_child_box_local(nd, slot, lo, hi, height);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                const uint32_t prev = atomicAdd(&lcnt[node - B], 1u);
+                if (prev == 0) break;
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                load_child_box_local(nd, 1 - slot, slo, shi, &sh);
+            } else {
+                store_child_box(nd, slot, lo, hi, height);
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                __builtin_amdgcn_s_waitcnt(0);
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                const uint32_t prev =
+                    __hip_atomic_fetch_add(&counters[node], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (prev == 0) break;
+                load_child_box(nd, 1 - slot, slo, shi, &sh);
+            }
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = fminf(lo[a], slo[a]);
+                hi[a] = fmaxf(hi[a], shi[a]);
+            }
+            height = 1 + (height > sh ? height : sh);
+            if ((int64_t)node == b) {  // segment root
+                atomicMax(max_depth, (uint32_t)height);
+                break;
+            }
+            child = (int)node;
+            node = parent_int[node];
+        }
+    }
 }
 
 struct TriBoxFn {
@@ -178,17 +230,21 @@ struct OrdBoxFn {  // boxes stored as ordered uints: 6 per primitive
     }
 };
 
-__global__ void k_refit_tris(const uint64_t* keys, uint64_t n, const uint32_t* seg_off, int seg_shift, Node* nodes,
-                             const uint32_t* parent_int, const uint32_t* parent_leaf, uint32_t* counters,
-                             uint32_t* max_depth, TriBoxFn fn) {
-    uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n) refit_from_leaf(p, keys, n, seg_off, seg_shift, nodes, parent_int, parent_leaf, counters, max_depth, fn);
+__global__ __launch_bounds__(kRefitThreads) void k_refit_tris(const uint64_t* keys, uint64_t n,
+                                                               const uint32_t* seg_off, int seg_shift, Node* nodes,
+                                                               const uint32_t* parent_int,
+                                                               const uint32_t* parent_leaf, uint32_t* counters,
+                                                               const uint32_t* range_end, uint32_t* max_depth,
+                                                               TriBoxFn fn) {
+    refit_blocked(keys, n, seg_off, seg_shift, nodes, parent_int, parent_leaf, counters, range_end, max_depth, fn);
 }
-__global__ void k_refit_boxes(const uint64_t* keys, uint64_t n, const uint32_t* seg_off, int seg_shift, Node* nodes,
-                              const uint32_t* parent_int, const uint32_t* parent_leaf, uint32_t* counters,
-                              uint32_t* max_depth, OrdBoxFn fn) {
-    uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n) refit_from_leaf(p, keys, n, seg_off, seg_shift, nodes, parent_int, parent_leaf, counters, max_depth, fn);
+__global__ __launch_bounds__(kRefitThreads) void k_refit_boxes(const uint64_t* keys, uint64_t n,
+                                                                const uint32_t* seg_off, int seg_shift, Node* nodes,
+                                                                const uint32_t* parent_int,
+                                                                const uint32_t* parent_leaf, uint32_t* counters,
+                                                                const uint32_t* range_end, uint32_t* max_depth,
+                                                                OrdBoxFn fn) {
+    refit_blocked(keys, n, seg_off, seg_shift, nodes, parent_int, parent_leaf, counters, range_end, max_depth, fn);
 }
 
 __global__ void k_gather_leaf_tris(const uint32_t* __restrict__ order, uint64_t n, const float* __restrict__ verts,
@@ -214,31 +270,33 @@ inline unsigned grid1(uint64_t n, int b = 256) { return (unsigned)((n + b - 1) /
 namespace pzk {
 
 void lbvh_karras(const uint64_t* sorted_keys, uint64_t n, const uint32_t* seg_off, int seg_shift, pzd::Node* nodes,
-                 uint32_t* parent_int, uint32_t* parent_leaf, uint32_t* counters, hipStream_t s) {
+                 uint32_t* parent_int, uint32_t* parent_leaf, uint32_t* counters, uint32_t* range_end,
+                 hipStream_t s) {
     if (n == 0) return;
     KernelTimer kt_("karras", s);
     hipLaunchKernelGGL(k_karras, dim3(grid1(n)), dim3(256), 0, s, sorted_keys, n, seg_off, seg_shift, nodes, parent_int,
-                       parent_leaf, counters);
+                       parent_leaf, counters, range_end);
 }
 
 void lbvh_refit_tris(const uint64_t* sorted_keys, const uint32_t* order, uint64_t n, const uint32_t* seg_off,
                      int seg_shift, const float* verts, const uint32_t* idx, pzd::Node* nodes,
-                     const uint32_t* parent_int, const uint32_t* parent_leaf, uint32_t* counters, uint32_t* max_depth,
-                     hipStream_t s) {
+                     const uint32_t* parent_int, const uint32_t* parent_leaf, uint32_t* counters,
+                     const uint32_t* range_end, uint32_t* max_depth, hipStream_t s) {
     if (n == 0) return;
     TriBoxFn fn{verts, idx, order};
     KernelTimer kt_("refit", s);
-    hipLaunchKernelGGL(k_refit_tris, dim3(grid1(n)), dim3(256), 0, s, sorted_keys, n, seg_off, seg_shift, nodes,
-                       parent_int, parent_leaf, counters, max_depth, fn);
+    hipLaunchKernelGGL(k_refit_tris, dim3(grid1(n, kRefitBlock)), dim3(kRefitThreads), 0, s, sorted_keys, n, seg_off,
+                       seg_shift, nodes, parent_int, parent_leaf, counters, range_end, max_depth, fn);
 }
 
 void lbvh_refit_ord_boxes(const uint64_t* sorted_keys, const uint32_t* order, uint64_t n, const uint32_t* seg_off,
                           int seg_shift, const uint32_t* obox, pzd::Node* nodes, const uint32_t* parent_int,
-                          const uint32_t* parent_leaf, uint32_t* counters, uint32_t* max_depth, hipStream_t s) {
+                          const uint32_t* parent_leaf, uint32_t* counters, const uint32_t* range_end,
+                          uint32_t* max_depth, hipStream_t s) {
     if (n == 0) return;
     OrdBoxFn fn{obox, order};
-    hipLaunchKernelGGL(k_refit_boxes, dim3(grid1(n)), dim3(256), 0, s, sorted_keys, n, seg_off, seg_shift, nodes,
-                       parent_int, parent_leaf, counters, max_depth, fn);
+    hipLaunchKernelGGL(k_refit_boxes, dim3(grid1(n, kRefitBlock)), dim3(kRefitThreads), 0, s, sorted_keys, n, seg_off,
+                       seg_shift, nodes, parent_int, parent_leaf, counters, range_end, max_depth, fn);
 }
 
 void gather_leaf_tris(const uint32_t* order, uint64_t n, const float* verts, const uint32_t* idx, const int32_t* t2o,

@@ -371,10 +371,13 @@ __device__ __forceinline__ uint32_t reserve_slot(bool valid, uint32_t l, uint32_
 }
 
 // Triangles of needed objects -> (loc << 30 | Morton in object box, triangle).
+// Objects whose triangles are contiguous place triangle t at t - first_tri;
+// the others reserve slots through per-object cursors.
 __global__ __launch_bounds__(kBlock) void k_tri_keys_two_level(
     const float* __restrict__ verts, const uint32_t* __restrict__ idx, const int32_t* __restrict__ t2o, uint64_t nt,
     int32_t nobj, const uint32_t* __restrict__ need, const uint32_t* __restrict__ loc,
-    const uint32_t* __restrict__ obox, const uint32_t* __restrict__ seg_off, uint32_t* __restrict__ cursor,
+    const uint32_t* __restrict__ obox, const uint32_t* __restrict__ seg_off, const uint32_t* __restrict__ first_tri,
+    const uint32_t* __restrict__ last_tri, const uint32_t* __restrict__ tri_cnt, uint32_t* __restrict__ cursor,
     uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, uint64_t nv) {
     uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     int32_t o = t < nt ? t2o[t] : -1;
@@ -386,9 +389,11 @@ __global__ __launch_bounds__(kBlock) void k_tri_keys_two_level(
         i2 = idx[3 * t + 2];
         valid = i0 < nv && i1 < nv && i2 < nv;
     }
+    const bool direct = valid && last_tri[o] - first_tri[o] + 1 == tri_cnt[o];
     uint32_t l = valid ? loc[o] : 0u;
-    uint32_t slot = reserve_slot(valid, l, cursor);
+    uint32_t slot = reserve_slot(valid && !direct, l, cursor);
     if (!valid) return;
+    if (direct) slot = (uint32_t)t - first_tri[o];
     float3 a = vtx(verts, i0), b = vtx(verts, i1), c = vtx(verts, i2);
     float lo[3], inv[3];
     for (int k = 0; k < 3; ++k) {
@@ -1236,11 +1241,12 @@ void needed_layout(const uint32_t* need, const uint32_t* loc, int32_t nobj, cons
 
 void tri_keys_two_level(const float* verts, uint64_t nv, const uint32_t* idx, const int32_t* t2o, uint64_t nt,
                         int32_t nobj, const uint32_t* need, const uint32_t* loc, const uint32_t* obox,
-                        const uint32_t* seg_off, uint32_t* cursor, uint64_t* keys, uint32_t* vals, hipStream_t s) {
+                        const uint32_t* seg_off, const uint32_t* first_tri, const uint32_t* last_tri,
+                        const uint32_t* tri_cnt, uint32_t* cursor, uint64_t* keys, uint32_t* vals, hipStream_t s) {
     KernelTimer kt_("tri_keys", s);
     if (nt)
         hipLaunchKernelGGL(k_tri_keys_two_level, dim3(grid1(nt)), dim3(kBlock), 0, s, verts, idx, t2o, nt, nobj, need,
-                           loc, obox, seg_off, cursor, keys, vals, nv);
+                           loc, obox, seg_off, first_tri, last_tri, tri_cnt, cursor, keys, vals, nv);
 }
 
 void tri_keys_global(const float* verts, const uint32_t* idx, uint64_t nt, const uint32_t* scene, uint64_t* keys,

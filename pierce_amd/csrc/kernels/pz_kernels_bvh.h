@@ -7,15 +7,18 @@
 namespace pzk {
 
 // --- LBVH (bvh.hip)
+// range_end[i] (n entries): the other end of internal node i's sorted range.
 void lbvh_karras(const uint64_t* sorted_keys, uint64_t n, const uint32_t* seg_off, int seg_shift, pzd::Node* nodes,
-                 uint32_t* parent_int, uint32_t* parent_leaf, uint32_t* counters, hipStream_t s);
+                 uint32_t* parent_int, uint32_t* parent_leaf, uint32_t* counters, uint32_t* range_end,
+                 hipStream_t s);
 void lbvh_refit_tris(const uint64_t* sorted_keys, const uint32_t* order, uint64_t n, const uint32_t* seg_off,
                      int seg_shift, const float* verts, const uint32_t* idx, pzd::Node* nodes,
-                     const uint32_t* parent_int, const uint32_t* parent_leaf, uint32_t* counters, uint32_t* max_depth,
-                     hipStream_t s);
+                     const uint32_t* parent_int, const uint32_t* parent_leaf, uint32_t* counters,
+                     const uint32_t* range_end, uint32_t* max_depth, hipStream_t s);
 void lbvh_refit_ord_boxes(const uint64_t* sorted_keys, const uint32_t* order, uint64_t n, const uint32_t* seg_off,
                           int seg_shift, const uint32_t* obox, pzd::Node* nodes, const uint32_t* parent_int,
-                          const uint32_t* parent_leaf, uint32_t* counters, uint32_t* max_depth, hipStream_t s);
+                          const uint32_t* parent_leaf, uint32_t* counters, const uint32_t* range_end,
+                          uint32_t* max_depth, hipStream_t s);
 void gather_leaf_tris(const uint32_t* order, uint64_t n, const float* verts, const uint32_t* idx, const int32_t* t2o,
                       pzd::LeafTri* out, hipStream_t s);
 

@@ -89,7 +89,8 @@ void needed_layout(const uint32_t* need, const uint32_t* loc, int32_t nobj, cons
                    hipStream_t s);
 void tri_keys_two_level(const float* verts, uint64_t nv, const uint32_t* idx, const int32_t* t2o, uint64_t nt,
                         int32_t nobj, const uint32_t* need, const uint32_t* loc, const uint32_t* obox,
-                        const uint32_t* seg_off, uint32_t* cursor, uint64_t* keys, uint32_t* vals, hipStream_t s);
+                        const uint32_t* seg_off, const uint32_t* first_tri, const uint32_t* last_tri,
+                        const uint32_t* tri_cnt, uint32_t* cursor, uint64_t* keys, uint32_t* vals, hipStream_t s);
 void tri_keys_global(const float* verts, const uint32_t* idx, uint64_t nt, const uint32_t* scene, uint64_t* keys,
                      uint32_t* vals, hipStream_t s);
 void edge_scatter(const float* es, const float* ee, const int32_t* eo, uint64_t ne, int32_t nobj,
