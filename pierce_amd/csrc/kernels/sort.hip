@@ -192,10 +192,54 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t* 
     }
 }
 
+// Up to one radix tile of keys: a bitonic sort in the LDS of one workgroup
+// (one launch instead of three per 8-bit pass).  Keys are below 2^63, so the
+// padding key ~0 sorts last; equal keys may leave in any order.
+constexpr int kSmallSortThreads = 1024;
+template <bool HasVals>
+__global__ __launch_bounds__(kSmallSortThreads) void k_sort_small(uint64_t* __restrict__ keys,
+                                                                  uint32_t* __restrict__ vals, uint32_t n) {
+    __shared__ uint64_t K[kSortTile];
+    __shared__ uint32_t V[HasVals ? kSortTile : 1];
+    uint32_t npow = 2;
+    while (npow < n) npow <<= 1;
+    for (uint32_t i = threadIdx.x; i < npow; i += kSmallSortThreads) {
+        K[i] = i < n ? keys[i] : ~0ull;
+        if (HasVals) V[i] = i < n ? vals[i] : 0u;
+    }
+    __syncthreads();
+    for (uint32_t size = 2; size <= npow; size <<= 1)
+        for (uint32_t st = size >> 1; st > 0; st >>= 1) {
+            for (uint32_t i = threadIdx.x; i < npow / 2; i += kSmallSortThreads) {
+                const uint32_t a = 2 * i - (i & (st - 1)), b = a + st;
+                const bool up = (a & size) == 0;
+                const uint64_t x = K[a], y = K[b];
+                if ((x > y) == up) {
+                    K[a] = y;
+                    K[b] = x;
+                    if (HasVals) {
+                        const uint32_t t = V[a];
+                        V[a] = V[b];
+                        V[b] = t;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (uint32_t i = threadIdx.x; i < n; i += kSmallSortThreads) {
+        keys[i] = K[i];
+        if (HasVals) vals[i] = V[i];
+    }
+}
+
 template <bool HasVals>
 bool radix_impl(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, uint64_t n, int begin_bit,
                 int end_bit, void* temp, hipStream_t s) {
     if (n <= 1 || end_bit <= begin_bit) return false;
+    if (n <= (uint64_t)kSortTile && begin_bit == 0 && end_bit < 64) {  // keys < 2^end_bit: one workgroup
+        hipLaunchKernelGGL((k_sort_small<HasVals>), dim3(1), dim3(kSmallSortThreads), 0, s, keys, vals, (uint32_t)n);
+        return false;
+    }
     uint32_t nb = (uint32_t)((n + kSortTile - 1) / kSortTile);
     uint64_t ncounts = (uint64_t)kRadix * nb;
     uint32_t* counts = reinterpret_cast<uint32_t*>(temp);
