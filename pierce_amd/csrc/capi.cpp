@@ -501,7 +501,7 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
                                                                 "build_small.c2"};
         uint32_t l0 = (uint32_t)T.nbig;
         for (int k = 0; k < pzk::kSmallClasses; ++k) {  // one launch per size class
-            if (!pzk::build_small(D->v, D->idx, D->t2o, N.obj_of_loc, P.first_tri, P.tri_cnt, P.obox, N.seg_off, l0,
+            if (!pzk::build_small(D->v, D->idx, N.obj_of_loc, P.first_tri, P.tri_cnt, P.obox, N.seg_off, l0,
                                   T.ncls[k], T.cls_max[k], c->tune.build_threads[k], acc.nodes, acc.leaves,
                                   acc.height, kTimer[k], s))
                 throw PzError(PZ_E_INVALID, "invalid LDS-builder configuration");

@@ -60,7 +60,7 @@ void big_prefix(const uint32_t* seg_off, uint32_t* totals, hipStream_t s);
 size_t build_small_lds(uint32_t cap);
 // One workgroup of `threads` (128, 256, 512 or 1024) per object l in
 // [l_begin, l_begin + count), all of at most cap triangles.  false: bad cap or threads.
-bool build_small(const float* verts, const uint32_t* idx, const int32_t* t2o, const int32_t* obj_of_loc,
+bool build_small(const float* verts, const uint32_t* idx, const int32_t* obj_of_loc,
                  const uint32_t* first_tri, const uint32_t* tri_cnt, const uint32_t* obox, const uint32_t* seg_off,
                  uint32_t l_begin, uint32_t count, uint32_t cap, int threads, pzd::Node* nodes, pzd::LeafTri* leaves,
                  uint32_t* height, const char* timer, hipStream_t s);
