@@ -92,6 +92,21 @@ pz_status pz_pre_read(const char* path, pz_host_mesh** out);
 /* Write a .pre v4 file byte-compatible with BinaryIO.h:32-82. */
 pz_status pz_pre_write(const char* path, const pz_mesh_view* mesh);
 
+/* Read a Wavefront .obj file into a host mesh, one object per shape (`o` /
+ * `g` group with faces), as pierce/preprocess/src/ObjMeshDatasetLoader.cpp:14-168
+ * loads it through tinyobjloader with triangulation: polygon faces become
+ * triangle fans, negative face indices count back from the last vertex,
+ * shapes without faces are dropped, vertices outside the vertex list drop the
+ * triangles that use them.  An object's vertices are stored in ascending file
+ * order (the reference's order is std::unordered_set iteration order).
+ * No edges and no grid are attached.  Not a .obj path / unreadable /
+ * malformed `v` or `f` line / no triangles -> PZ_E_IO. */
+pz_status pz_obj_read(const char* path, pz_host_mesh** out);
+/* Translate every object by its own offset drawn uniformly from [lo, hi]^3
+ * (pierce/preprocess/src/pierce_preprocess.cpp:243-283, which uses [-15, 15]
+ * and an unseeded generator); deterministic for a given seed. */
+pz_status pz_host_mesh_shuffle(pz_host_mesh* mesh, uint64_t seed, float lo, float hi);
+
 /* Copy a view into a new host mesh (owned arrays). */
 pz_status pz_host_mesh_from_view(const pz_mesh_view* view, pz_host_mesh** out);
 

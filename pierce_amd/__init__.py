@@ -130,6 +130,8 @@ _SIGS = [
     ("pz_host_mesh_view", C.c_int32, [C.c_void_p, C.POINTER(MeshView)]),
     ("pz_host_mesh_free", None, [C.c_void_p]),
     ("pz_pre_read", C.c_int32, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    ("pz_obj_read", C.c_int32, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    ("pz_host_mesh_shuffle", C.c_int32, [C.c_void_p, C.c_uint64, C.c_float, C.c_float]),
     ("pz_pre_write", C.c_int32, [C.c_char_p, C.POINTER(MeshView)]),
     ("pz_host_mesh_from_view", C.c_int32, [C.POINTER(MeshView), C.POINTER(C.c_void_p)]),
     ("pz_host_mesh_extract_edges", C.c_int32, [C.c_void_p]),
@@ -305,6 +307,39 @@ def read_pre(path: str) -> Mesh:
     h = C.c_void_p()
     _check(lib().pz_pre_read(path.encode(), C.byref(h)))
     return _mesh_from_handle(h)
+
+
+def read_obj(path: str) -> Mesh:
+    """A Wavefront .obj file, one object per shape with faces
+    (pierce/preprocess/src/ObjMeshDatasetLoader.cpp:14-168); no edges, no grid."""
+    h = C.c_void_p()
+    _check(lib().pz_obj_read(path.encode(), C.byref(h)))
+    return _mesh_from_handle(h)
+
+
+def write_obj(path: str, mesh: Mesh) -> None:
+    """Writes one `o object<i>` shape per object (triangles of object i,
+    1-based indices into the shared vertex list)."""
+    tri = mesh.indices
+    t2o = mesh.triangle_to_object
+    with open(path, "w") as f:
+        for x, y, z in mesh.vertices.tolist():
+            f.write(f"v {x!r} {y!r} {z!r}\n")
+        order = np.argsort(t2o, kind="stable")
+        cur = None
+        for t in order.tolist():
+            o = int(t2o[t])
+            if o != cur:
+                f.write(f"o object{o}\n")
+                cur = o
+            a, b, c = (int(i) + 1 for i in tri[t])
+            f.write(f"f {a} {b} {c}\n")
+
+
+def shuffle(mesh: Mesh, seed: int, lo: float = -15.0, hi: float = 15.0) -> Mesh:
+    """Every object moved by its own uniform offset in [lo, hi]^3
+    (pierce/preprocess/src/pierce_preprocess.cpp:243-283)."""
+    return _with_handle(mesh, lambda h: lib().pz_host_mesh_shuffle(h, seed, lo, hi))
 
 
 def write_pre(path: str, mesh: Mesh) -> None:
