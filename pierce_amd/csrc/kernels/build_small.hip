@@ -2,34 +2,39 @@
 //
 // For an object of n <= 2048 triangles stored contiguously from first_tri:
 //   1. one gather: each thread loads its local triangles q = tid + k * T
-//      (the first 2048 / T of them stay in registers for the whole build) and
+//      (the first 2-4 of them stay in registers for the whole build) and
 //      writes a 32-bit key: a 21-bit Morton code of the centroid on a 127^3
 //      grid over the object's own box (never all ones) above the 11-bit local
-//      index, so keys are distinct and the padding key 0xFFFFFFFF sorts last;
-//   2. bitonic sort: a wavefront holds 256 keys in registers (four per lane);
-//      strides 128 and 64 compare registers, strides 32 .. 4 swap lanes with
-//      ds_bpermute and strides 2 and 1 with DPP quad permutes; only strides
-//      >= 256 go through LDS with a block barrier (6 of the 66 stages for
-//      2048 keys);
-//   3. leaf records in sorted order, staged through LDS in two halves so the
+//      index, so keys are distinct;
+//   2. bitonic sort in which every comparator puts the smaller key first: a
+//      size-s block starts by pairing position i with its mirror
+//      i ^ (s - 1), then halves with strides s/4 .. 1.  The virtual keys at
+//      positions >= n are +infinity and never move, so they take no LDS.  A
+//      wavefront holds 256 keys in registers (four per lane): register pairs
+//      for strides 128 and 64, ds_bpermute for lane strides 32 .. 4 and the
+//      mirrors, DPP quad permutes for strides 2 and 1; only strides >= 256
+//      go through LDS with a block barrier;
+//   3. the sorted position of each local triangle, kept in registers; leaf
+//      records in sorted order, staged through LDS in two halves so the
 //      48-byte records are written to HBM coalesced;
 //   4. agglomerative bottom-up build (Apetrei 2014, "Fast and simple
 //      agglomerative LBVH construction"), LDS only: each thread climbs from
 //      the sorted positions of its own triangles.  A node covering sorted
 //      leaves [l, r] joins the neighbouring split with the smaller key XOR
-//      (split r, or l - 1).  The first child to arrive parks and stops; the
-//      exchange on bound[split] hands its far range end, id and height to
-//      the second, which merges both boxes and climbs on.  Boxes live in one
-//      LDS array: leaf p's box at p until node p completes, node i's box at
-//      i afterwards (every node containing leaf p completes after leaf p's
-//      parent, which consumed it);
+//      (split r, or l - 1).  The first child to arrive stops; the exchange
+//      on bound[split] hands its far range end, id and height to the second,
+//      which merges both boxes, packs the two child ids into bound[split]
+//      and climbs on.  Boxes live in one LDS array: leaf p's box at p until
+//      node p completes, node i's box at i afterwards (every node containing
+//      leaf p completes after leaf p's parent, which consumed it);
 //   5. node records written in index order from LDS (coalesced); a leaf
 //      child's box is re-read from its just-written leaf record.  The root
 //      must sit at the segment's first node (the traversals start there): it
 //      is written at index 0 and internal node 0's record at the root's
 //      index, with the one link into node 0 redirected.
-// LDS is dynamic, sized by the largest object of the launch: 4 B per key
-// (next power of two, at least 2) + 34 B per triangle + 4 B.
+// LDS is dynamic, 32 B per triangle of the launch's largest object: keys
+// (4 B), boxes (24 B) and the exchange words (4 B), which hold the sorted
+// positions until the climb starts.
 #include <type_traits>
 
 #include "pz_device.h"
@@ -116,18 +121,15 @@ __global__ __launch_bounds__(T) void k_build_small(
     const float* __restrict__ verts, const uint32_t* __restrict__ idx, const int32_t* __restrict__ obj_of_loc,
     const uint32_t* __restrict__ first_tri, const uint32_t* __restrict__ tri_cnt, const uint32_t* __restrict__ obox, const uint32_t* __restrict__ seg_off,
     uint32_t l_begin, int cap, Node* __restrict__ nodes, LeafTri* __restrict__ leaves, uint32_t* __restrict__ height_out) {
-    constexpr int PR = T >= 512 ? kMax / T : 4;  // local triangles per thread held in registers
+    constexpr int PR = T == 1024 ? 2 : (T == 512 ? 3 : 4);  // local triangles per thread kept in registers
+    constexpr int NP = (kMax / T + 1) / 2;                 // sorted positions per thread, two per register
     extern __shared__ __align__(16) unsigned char lds[];
-    int cpow = 2;
-    while (cpow < cap) cpow <<= 1;
-    uint32_t* key = reinterpret_cast<uint32_t*>(lds);        // cpow
-    float* box = reinterpret_cast<float*>(key + cpow);       // 6 x cap: leaf p's box at p, then node i's at i
-    int* bound = reinterpret_cast<int*>(box + 6 * cap);      // cap
-    int16_t* cid = reinterpret_cast<int16_t*>(bound + cap);  // 2 x cap + 2: children of node i
-    uint16_t* inv = reinterpret_cast<uint16_t*>(cid + 2 * cap + 2);  // cap: local triangle -> sorted position
-    float* stage = box;  // before the climb: 9 floats per triangle of one half (36 x ceil(cap / 2) <= 32 x cap + 4 B)
-    __shared__ int root_idx;
-    __shared__ uint32_t wg_h;
+    const int capk = (cap + 3) & ~3;
+    uint32_t* key = reinterpret_cast<uint32_t*>(lds);      // capk
+    float* box = reinterpret_cast<float*>(key + capk);     // 6 x cap (staging before the climb)
+    int* bound = reinterpret_cast<int*>(box + 6 * cap);    // cap: sorted positions, then exchange / child ids
+    uint16_t* inv = reinterpret_cast<uint16_t*>(bound);
+    float* stage = box;
     const uint32_t l = l_begin + blockIdx.x;
     const int32_t o = obj_of_loc[l];
     const int n = (int)tri_cnt[o];
@@ -159,8 +161,8 @@ __global__ __launch_bounds__(T) void k_build_small(
         b = vtx(verts, i1);
         c = vtx(verts, i2);
     };
-    // 1. one gather; local triangle q = tid + k * T, the first PR in registers
-    float3 ra[PR], rb[PR], rc[PR];
+    // 1. one gather
+    float3 ra[PR > 0 ? PR : 1], rb[PR > 0 ? PR : 1], rc[PR > 0 ? PR : 1];
 #pragma unroll
     for (int k = 0; k < PR; ++k) {
         const int q = tid + k * T;
@@ -169,107 +171,150 @@ __global__ __launch_bounds__(T) void k_build_small(
 #pragma unroll
     for (int k = 0; k < PR; ++k) {
         const int q = tid + k * T;
-        if (q < npow) key[q] = q < n ? morton(ra[k], rb[k], rc[k], q) : 0xFFFFFFFFu;
+        if (q < n) key[q] = morton(ra[k], rb[k], rc[k], q);
     }
-    for (int q = tid + PR * T; q < npow; q += T) {
-        uint32_t kv = 0xFFFFFFFFu;
-        if (q < n) {
-            float3 a, b, c;
-            tri(q, a, b, c);
-            kv = morton(a, b, c, q);
-        }
-        key[q] = kv;
+    for (int q = tid + PR * T; q < n; q += T) {
+        float3 a, b, c;
+        tri(q, a, b, c);
+        key[q] = morton(a, b, c, q);
     }
     __syncthreads();
-    // 2. bitonic sort, ascending
-    auto cx = [](uint32_t& x, uint32_t& y, bool up) {
-        const uint32_t a = min(x, y), b = max(x, y);
-        x = up ? a : b;
-        y = up ? b : a;
-    };
-    auto lds_stage = [&](int size, int st) {
-        for (int i = tid; i < npow / 2; i += T) {
-            const int a = 2 * i - (i & (st - 1)), b = a + st;
-            const bool up = (a & size) == 0;
-            const uint32_t x = key[a], y = key[b];
-            if ((x > y) == up) {
-                key[a] = y;
-                key[b] = x;
+    // 2. bitonic sort, every comparator ascending (a block's first step pairs
+    // i with its mirror i ^ (size - 1)), so the virtual keys at positions >= n
+    // (+inf) never move and need no storage.
+    auto lds_mirror = [&](int s) {
+        const int hs = s >> 1;
+        for (int t = tid; t < npow / 2; t += T) {
+            const int blk = t / hs, off = t - blk * hs;
+            const int a = blk * s + off, b = blk * s + s - 1 - off;
+            if (b < n) {
+                const uint32_t x = key[a], y = key[b];
+                if (x > y) {
+                    key[a] = y;
+                    key[b] = x;
+                }
             }
         }
     };
-    if (npow >= 256) {
-        // a wave holds 256 keys: lane L has positions c*256 + j*64 + L, j = 0..3;
-        // strides 128 and 64 compare registers, strides <= 32 swap lanes
-        auto reg4 = [&](int s_lo, int s_hi) {
-            for (int c = wave; c * 256 < npow; c += T / 64) {
-                const int pb = c * 256 + lane;
-                uint32_t k0 = key[pb], k1 = key[pb + 64], k2 = key[pb + 128], k3 = key[pb + 192];
-                for (int sz = s_lo; sz <= s_hi; sz <<= 1) {
-                    if (sz >= 256) {
-                        cx(k0, k2, (pb & sz) == 0);
-                        cx(k1, k3, ((pb + 64) & sz) == 0);
-                    }
-                    if (sz >= 128) {
-                        cx(k0, k1, (pb & sz) == 0);
-                        cx(k2, k3, ((pb + 128) & sz) == 0);
-                    }
-                    const bool d0 = (pb & sz) == 0, d1 = ((pb + 64) & sz) == 0;
-                    const bool d2 = ((pb + 128) & sz) == 0, d3 = ((pb + 192) & sz) == 0;
-                    auto stage = [&](auto stc) {
-                        constexpr int st = decltype(stc)::value;
-                        if (2 * st > sz) return;
-                        const bool lower = (lane & st) == 0;
-                        const uint32_t o0 = lane_swap<st>(k0), o1 = lane_swap<st>(k1);
-                        const uint32_t o2 = lane_swap<st>(k2), o3 = lane_swap<st>(k3);
-                        k0 = (lower == d0) ? min(k0, o0) : max(k0, o0);
-                        k1 = (lower == d1) ? min(k1, o1) : max(k1, o1);
-                        k2 = (lower == d2) ? min(k2, o2) : max(k2, o2);
-                        k3 = (lower == d3) ? min(k3, o3) : max(k3, o3);
-                    };
-                    stage(std::integral_constant<int, 32>{});
-                    stage(std::integral_constant<int, 16>{});
-                    stage(std::integral_constant<int, 8>{});
-                    stage(std::integral_constant<int, 4>{});
-                    stage(std::integral_constant<int, 2>{});
-                    stage(std::integral_constant<int, 1>{});
+    auto lds_half = [&](int st) {
+        for (int t = tid; t < npow / 2; t += T) {
+            const int a = 2 * t - (t & (st - 1)), b = a + st;
+            if (b < n) {
+                const uint32_t x = key[a], y = key[b];
+                if (x > y) {
+                    key[a] = y;
+                    key[b] = x;
                 }
-                key[pb] = k0;
-                key[pb + 64] = k1;
-                key[pb + 128] = k2;
-                key[pb + 192] = k3;
             }
-        };
-        reg4(2, 256);
-        for (int size = 512; size <= npow; size <<= 1) {
-            for (int st = size >> 1; st >= 256; st >>= 1) {
+        }
+    };
+    // a wave holds 256 positions: lane L has c*256 + j*64 + L, j = 0..3
+    auto lohi = [](uint32_t& x, uint32_t& y) {
+        const uint32_t a = min(x, y), b = max(x, y);
+        x = a;
+        y = b;
+    };
+    auto lane_half = [&](auto stc, uint32_t& k0, uint32_t& k1, uint32_t& k2, uint32_t& k3) {
+        constexpr int st = decltype(stc)::value;
+        const bool lower = (lane & st) == 0;
+        const uint32_t o0 = lane_swap<st>(k0), o1 = lane_swap<st>(k1), o2 = lane_swap<st>(k2), o3 = lane_swap<st>(k3);
+        k0 = lower ? min(k0, o0) : max(k0, o0);
+        k1 = lower ? min(k1, o1) : max(k1, o1);
+        k2 = lower ? min(k2, o2) : max(k2, o2);
+        k3 = lower ? min(k3, o3) : max(k3, o3);
+    };
+    auto bperm = [&](uint32_t v, int m) {  // value of lane (lane ^ m)
+        return (uint32_t)__builtin_amdgcn_ds_bpermute((int)((lane ^ m) << 2), (int)v);
+    };
+    auto reg_halves = [&](int s, uint32_t& k0, uint32_t& k1, uint32_t& k2, uint32_t& k3) {
+        // half-cleaner strides min(s/4, 128) .. 1
+        if (s >= 512) {
+            lohi(k0, k2);
+            lohi(k1, k3);
+        }
+        if (s >= 256) {
+            lohi(k0, k1);
+            lohi(k2, k3);
+        }
+        if (s >= 128) lane_half(std::integral_constant<int, 32>{}, k0, k1, k2, k3);
+        if (s >= 64) lane_half(std::integral_constant<int, 16>{}, k0, k1, k2, k3);
+        if (s >= 32) lane_half(std::integral_constant<int, 8>{}, k0, k1, k2, k3);
+        if (s >= 16) lane_half(std::integral_constant<int, 4>{}, k0, k1, k2, k3);
+        if (s >= 8) lane_half(std::integral_constant<int, 2>{}, k0, k1, k2, k3);
+        if (s >= 4) lane_half(std::integral_constant<int, 1>{}, k0, k1, k2, k3);
+    };
+    auto reg_pass = [&](int s_lo, int s_hi) {  // s_lo == s_hi >= 512: half-cleaners only
+        for (int c = wave; c * 256 < n; c += T / 64) {
+            const int pb = c * 256 + lane;
+            uint32_t k0 = pb < n ? key[pb] : ~0u, k1 = pb + 64 < n ? key[pb + 64] : ~0u;
+            uint32_t k2 = pb + 128 < n ? key[pb + 128] : ~0u, k3 = pb + 192 < n ? key[pb + 192] : ~0u;
+            if (s_lo >= 512) {
+                reg_halves(s_lo, k0, k1, k2, k3);
+            } else {
+                for (int s = s_lo; s <= s_hi; s <<= 1) {
+                    // mirror step: partner of position i is i ^ (s - 1)
+                    if (s <= 64) {
+                        const int m = s - 1;
+                        const bool lower = (lane & (s >> 1)) == 0;
+                        const uint32_t o0 = bperm(k0, m), o1 = bperm(k1, m), o2 = bperm(k2, m), o3 = bperm(k3, m);
+                        k0 = lower ? min(k0, o0) : max(k0, o0);
+                        k1 = lower ? min(k1, o1) : max(k1, o1);
+                        k2 = lower ? min(k2, o2) : max(k2, o2);
+                        k3 = lower ? min(k3, o3) : max(k3, o3);
+                    } else if (s == 128) {  // j ^ 1, lane ^ 63
+                        const uint32_t o0 = bperm(k1, 63), o1 = bperm(k0, 63), o2 = bperm(k3, 63), o3 = bperm(k2, 63);
+                        k0 = min(k0, o0);
+                        k1 = max(k1, o1);
+                        k2 = min(k2, o2);
+                        k3 = max(k3, o3);
+                    } else {  // s == 256: j ^ 3, lane ^ 63
+                        const uint32_t o0 = bperm(k3, 63), o1 = bperm(k2, 63), o2 = bperm(k1, 63), o3 = bperm(k0, 63);
+                        k0 = min(k0, o0);
+                        k1 = min(k1, o1);
+                        k2 = max(k2, o2);
+                        k3 = max(k3, o3);
+                    }
+                    reg_halves(s, k0, k1, k2, k3);
+                }
+            }
+            if (pb < n) key[pb] = k0;
+            if (pb + 64 < n) key[pb + 64] = k1;
+            if (pb + 128 < n) key[pb + 128] = k2;
+            if (pb + 192 < n) key[pb + 192] = k3;
+        }
+    };
+    if (n > 1) {
+        reg_pass(2, min(npow, 256));
+        for (int s = 512; s <= npow; s <<= 1) {
+            __syncthreads();
+            lds_mirror(s);
+            for (int st = s >> 2; st >= 256; st >>= 1) {
                 __syncthreads();
-                lds_stage(size, st);
+                lds_half(st);
             }
             __syncthreads();
-            reg4(size, size);
+            reg_pass(s, s);
         }
-    } else {
-        for (int size = 2; size <= npow; size <<= 1)
-            for (int st = size >> 1; st > 0; st >>= 1) {
-                __syncthreads();
-                lds_stage(size, st);
-            }
     }
     __syncthreads();
-    // 3. sorted position of every local triangle
+    // 3. sorted positions (kept in registers for the climb), leaf records
+    // staged through LDS in two halves
     for (int p = tid; p < n; p += T) inv[key[p] & 0x7FFu] = (uint16_t)p;
-    if (tid == 0) {
-        root_idx = 0;
-        wg_h = 0;
-    }
     __syncthreads();
-    // 3b. leaf records in sorted order, staged through LDS in two halves
+    uint32_t ppos[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) ppos[k] = 0;
+#pragma unroll
+    for (int k = 0; k < 2 * NP; ++k) {
+        const int q = tid + k * T;
+        if (q < n) ppos[k >> 1] |= (uint32_t)inv[q] << (16 * (k & 1));
+    }
+    auto pos_of = [&](int k) { return (int)((ppos[k >> 1] >> (16 * (k & 1))) & 0xFFFFu); };
     const int half = (n + 1) >> 1;
     for (int r = 0; r < 2; ++r) {
         const int h0 = r * half, h1 = min(n, h0 + half);
-        auto put = [&](int q, const float3& a, const float3& b, const float3& c) {
-            const int p = (int)inv[q];
+        auto put = [&](int k, const float3& a, const float3& b, const float3& c) {
+            const int p = pos_of(k);
             if (p < h0 || p >= h1) return;
             float* st = stage + 9 * (p - h0);
             st[0] = a.x; st[1] = a.y; st[2] = a.z;
@@ -279,12 +324,14 @@ __global__ __launch_bounds__(T) void k_build_small(
 #pragma unroll
         for (int k = 0; k < PR; ++k) {
             const int q = tid + k * T;
-            if (q < n) put(q, ra[k], rb[k], rc[k]);
+            if (q < n) put(k, ra[k], rb[k], rc[k]);
         }
-        for (int q = tid + PR * T; q < n; q += T) {
+        for (int k = PR; k < 2 * NP; ++k) {
+            const int q = tid + k * T;
+            if (q >= n) break;
             float3 a, b, c;
             tri(q, a, b, c);
-            put(q, a, b, c);
+            put(k, a, b, c);
         }
         __syncthreads();
         for (int p = h0 + tid; p < h1; p += T) {
@@ -299,12 +346,8 @@ __global__ __launch_bounds__(T) void k_build_small(
     }
     for (int i = tid; i < n - 1; i += T) bound[i] = -1;
     __syncthreads();
-    // 4. climb: a leaf's box sits at its position p; a finished node's box
-    // moves to its own index i (leaf i's box is consumed by then, as every
-    // node containing leaf i completes after leaf i's parent).
-    auto climb = [&](int q, const float3& a, const float3& b, const float3& c) {
-        if (n < 2) return;
-        const int p = min((int)inv[q], n - 1);
+    // 4. climb (as v8); the second arrival packs the children into bound[split]
+    auto climb = [&](int p, const float3& a, const float3& b, const float3& c) {
         float blo[3] = {fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)), fminf(a.z, fminf(b.z, c.z))};
         float bhi[3] = {fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)), fmaxf(a.z, fmaxf(b.z, c.z))};
         {
@@ -339,8 +382,8 @@ __global__ __launch_bounds__(T) void k_build_small(
                 lft = oend;
                 dL = kx ^ ky;
             }
-            cid[2 * parent] = (int16_t)(s0 ? me : sib);
-            cid[2 * parent + 1] = (int16_t)(s0 ? sib : me);
+            const int c0 = s0 ? me : sib, c1 = s0 ? sib : me;
+            bound[parent] = (int)(((uint32_t)c0 & 0xFFFFu) | ((uint32_t)c1 << 16));
             float* sb = box + 6 * parent;
             for (int k = 0; k < 3; ++k) {
                 sb[k] = blo[k];
@@ -349,58 +392,62 @@ __global__ __launch_bounds__(T) void k_build_small(
             h = 1 + max(h, sh);
             me = parent;
             if (lft == 0 && rgt == n - 1) {
-                root_idx = parent;
-                wg_h = (uint32_t)h;
+                box[6 * (n - 1)] = __int_as_float(parent);  // root index: leaf n-1's slot is free by now
+                atomicMax(height_out, (uint32_t)h);
                 break;
             }
         }
     };
+    if (n >= 2) {
 #pragma unroll
-    for (int k = 0; k < PR; ++k) {
-        const int q = tid + k * T;
-        if (q < n) climb(q, ra[k], rb[k], rc[k]);
-    }
-    for (int q = tid + PR * T; q < n; q += T) {
-        float3 a, b, c;
-        tri(q, a, b, c);
-        climb(q, a, b, c);
+        for (int k = 0; k < PR; ++k) {
+            const int q = tid + k * T;
+            if (q < n) climb(pos_of(k), ra[k], rb[k], rc[k]);
+        }
+        for (int k = PR; k < 2 * NP; ++k) {
+            const int q = tid + k * T;
+            if (q >= n) break;
+            float3 a, b, c;
+            tri(q, a, b, c);
+            climb(pos_of(k), a, b, c);
+        }
     }
     __syncthreads();
     // 5. node records in index order (root at 0, node 0 at the root's index)
-    const int rt = root_idx;
-    auto child_box = [&](int c, float* bx) {
-        if (c >= 0) {
-            for (int k = 0; k < 6; ++k) bx[k] = box[6 * c + k];
-        } else {
-            const LeafTri& L = leaves[base + (uint32_t)(~c)];
-            const float4 v0 = L.v0, v1 = L.v1, v2 = L.v2;
-            bx[0] = fminf(v0.x, fminf(v1.x, v2.x));
-            bx[1] = fminf(v0.y, fminf(v1.y, v2.y));
-            bx[2] = fminf(v0.z, fminf(v1.z, v2.z));
-            bx[3] = fmaxf(v0.x, fmaxf(v1.x, v2.x));
-            bx[4] = fmaxf(v0.y, fmaxf(v1.y, v2.y));
-            bx[5] = fmaxf(v0.z, fmaxf(v1.z, v2.z));
+    if (n >= 2) {
+        const int rt = __float_as_int(box[6 * (n - 1)]);
+        auto child_box = [&](int c, float* bx) {
+            if (c >= 0) {
+                for (int k = 0; k < 6; ++k) bx[k] = box[6 * c + k];
+            } else {
+                const LeafTri& L = leaves[base + (uint32_t)(~c)];
+                const float4 v0 = L.v0, v1 = L.v1, v2 = L.v2;
+                bx[0] = fminf(v0.x, fminf(v1.x, v2.x));
+                bx[1] = fminf(v0.y, fminf(v1.y, v2.y));
+                bx[2] = fminf(v0.z, fminf(v1.z, v2.z));
+                bx[3] = fmaxf(v0.x, fmaxf(v1.x, v2.x));
+                bx[4] = fmaxf(v0.y, fmaxf(v1.y, v2.y));
+                bx[5] = fmaxf(v0.z, fmaxf(v1.z, v2.z));
+            }
+        };
+        auto link = [&](int c) {
+            if (c < 0) return ~(int)(base + (uint32_t)(~c));
+            return (int)(base + (uint32_t)(c == 0 ? rt : c));
+        };
+        for (int i = tid; i < n - 1; i += T) {
+            const uint32_t v = (uint32_t)bound[i];
+            const int c0 = (int16_t)(v & 0xFFFFu), c1 = (int16_t)(v >> 16);
+            float bl[6], br[6];
+            child_box(c0, bl);
+            child_box(c1, br);
+            Node nd;
+            nd.a = make_float4(bl[0], bl[1], bl[2], bl[3]);
+            nd.b = make_float4(bl[4], bl[5], br[0], br[1]);
+            nd.c = make_float4(br[2], br[3], br[4], br[5]);
+            nd.d = make_int4(link(c0), link(c1), 0, 0);
+            nodes[base + (i == rt ? 0 : (i == 0 ? rt : i))] = nd;
         }
-    };
-    auto link = [&](int c) {
-        if (c < 0) return ~(int)(base + (uint32_t)(~c));
-        const int m = c == 0 ? rt : c;  // the root is no one's child
-        return (int)(base + (uint32_t)m);
-    };
-    for (int i = tid; i < n - 1; i += T) {
-        const int c0 = cid[2 * i], c1 = cid[2 * i + 1];
-        float bl[6], br[6];
-        child_box(c0, bl);
-        child_box(c1, br);
-        Node nd;
-        nd.a = make_float4(bl[0], bl[1], bl[2], bl[3]);
-        nd.b = make_float4(bl[4], bl[5], br[0], br[1]);
-        nd.c = make_float4(br[2], br[3], br[4], br[5]);
-        nd.d = make_int4(link(c0), link(c1), 0, 0);
-        const int at = i == rt ? 0 : (i == 0 ? rt : i);
-        nodes[base + at] = nd;
     }
-    if (tid == 0 && wg_h) atomicMax(height_out, wg_h);
 }
 
 inline unsigned grid1(uint64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
@@ -426,11 +473,7 @@ void big_prefix(const uint32_t* seg_off, uint32_t* totals, hipStream_t s) {
     hipLaunchKernelGGL(k_big_prefix, dim3(1), dim3(64), 0, s, seg_off, totals);
 }
 
-size_t build_small_lds(uint32_t cap) {
-    uint32_t cpow = 2;
-    while (cpow < cap) cpow <<= 1;
-    return 4ull * cpow + 34ull * cap + 4;
-}
+size_t build_small_lds(uint32_t cap) { return 32ull * std::max<uint32_t>(cap, 2); }
 
 bool build_small(const float* verts, const uint32_t* idx, const int32_t* obj_of_loc,
                  const uint32_t* first_tri, const uint32_t* tri_cnt, const uint32_t* obox, const uint32_t* seg_off,

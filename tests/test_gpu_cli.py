@@ -101,3 +101,21 @@ def test_tool_rejects_pre_without_edges(tmp_path):
                         "--mesh2", str(tmp_path / "n.pre")], cwd=tmp_path, capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 1 and "precomputed edges" in r.stderr
+
+
+def test_obj_through_preprocess_to_two_pass(tmp_path):
+    """OBJ datasets -> pierce_preprocess (.pre v4 with edges and grid) -> the
+    two-pass tool, against the oracle on the meshes as read from the OBJ."""
+    a = pz.generate(pz.GEN_ELLIPSOIDS, count=60, seed=21, level_min=1, level_max=2, box=25.0)
+    b = pz.generate(pz.GEN_ELLIPSOIDS, count=60, seed=22, level_min=1, level_max=2, box=25.0)
+    for name, m in (("a", a), ("b", b)):
+        pz.write_obj(str(tmp_path / f"{name}.obj"), m)
+        subprocess.run([os.path.join(pz.BIN_DIR, "pierce_preprocess"), "--mode", "mesh", "--dataset",
+                        str(tmp_path / f"{name}.obj"), "--output-geometry", str(tmp_path / f"{name}.pre"),
+                        "--output-timing", str(tmp_path / f"{name}_timing.json"), "--generate-grid",
+                        "--grid-cell-size", "5.0"], check=True, capture_output=True, timeout=120)
+    run("pierce_overlap_two_pass", ["--mesh1", "a.pre", "--mesh2", "b.pre", "--output", "t.json"], tmp_path)
+    _, pairs, _ = read_pairs(tmp_path / "mesh_overlap_results.csv")
+    exp = orc.face_overlap(pz.read_obj(str(tmp_path / "a.obj")), pz.read_obj(str(tmp_path / "b.obj")))
+    assert len(exp) > 0
+    assert pairs == [tuple(p) for p in exp.tolist()]
