@@ -535,7 +535,11 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         const uint64_t cap_tasks = std::min<uint64_t>(std::max<uint64_t>(8 * np, 1 << 16), 1ull << 24);
         pzd::DualTask* tin = buf<pzd::DualTask>(c, "np.tasks0", cap_tasks);
         pzd::DualTask* tout = buf<pzd::DualTask>(c, "np.tasks1", cap_tasks);
-        unsigned long long* ctr = buf<unsigned long long>(c, "np.task_ctr", 2);
+        unsigned long long* ctr = buf<unsigned long long>(c, "np.task_ctr", 3);  // next task, spilled, open leaf pairs
+        // leaf pairs the fp32 walk leaves open, decided after the rounds
+        const uint64_t cap_esc = std::min<uint64_t>(std::max<uint64_t>(np, 1 << 16), 1ull << 22);
+        uint4* esc = buf<uint4>(c, "np.esc", cap_esc);
+        pzk::fill_ull(ctr + 2, 0ull, 1, s);
         if (c->persistent_blocks == 0) {
             int cus = 0;
             HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
@@ -547,8 +551,8 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         while (ntasks > 0) {
             pzk::fill_ull(ctr, 0ull, 2, s);
             if (!pzk::narrow_dual(pairs, va, vb, found, fallback, meta, f32, allow_f32, c->tune.narrow_queue[qclass],
-                                  cur, ntasks, ctr, c->tune.narrow_chunk[qclass], tout, ctr + 1, cap_tasks,
-                                  c->persistent_blocks, stats, s))
+                                  cur, ntasks, ctr, c->tune.narrow_chunk[qclass], tout, ctr + 1, cap_tasks, esc,
+                                  ctr + 2, cap_esc, c->persistent_blocks, stats, s))
                 throw PzError(PZ_E_INVALID, "invalid narrow-phase queue length");
             ++rounds;
             const uint64_t spilled = std::min<uint64_t>(readback<unsigned long long>(c, {ctr + 1})[0], cap_tasks);
@@ -562,6 +566,7 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
             cur = tin;
             ntasks = spilled;
         }
+        pzk::escalate(esc, ctr + 2, cap_esc, va, vb, meta, found, stats, s);
         r->narrow_rounds = rounds;
         r->tasks_spilled = spilled_total;
         uint64_t* fbp = buf<uint64_t>(c, "np.fb_pairs", np + 1);

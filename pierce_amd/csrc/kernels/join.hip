@@ -669,12 +669,51 @@ __device__ __forceinline__ bool tri_pair_test(const LeafTri& X, const LeafTri& Y
     return hit;
 }
 
+// tri_pair_test decided by the fp32 filter alone: 1 = hit, 0 = miss,
+// -1 = undecided (some edge unproven and none proven to hit).
+__device__ __forceinline__ int tri_pair_f32(const LeafTri& X, const LeafTri& Y, int dir_mask, LaneStats& st) {
+    const float3 x0 = make_float3(X.v0.x, X.v0.y, X.v0.z), x1 = make_float3(X.v1.x, X.v1.y, X.v1.z),
+                 x2 = make_float3(X.v2.x, X.v2.y, X.v2.z);
+    const float3 y0 = make_float3(Y.v0.x, Y.v0.y, Y.v0.z), y1 = make_float3(Y.v1.x, Y.v1.y, Y.v1.z),
+                 y2 = make_float3(Y.v2.x, Y.v2.y, Y.v2.z);
+    bool open = false;
+    int r;
+    if (dir_mask & 1) {
+        ++st.tests;
+        if ((r = seg_tri_f32(x0, x1, y0, y1, y2)) > 0) return 1;
+        open |= r < 0;
+        ++st.tests;
+        if ((r = seg_tri_f32(x1, x2, y0, y1, y2)) > 0) return 1;
+        open |= r < 0;
+        ++st.tests;
+        if ((r = seg_tri_f32(x2, x0, y0, y1, y2)) > 0) return 1;
+        open |= r < 0;
+    }
+    if (dir_mask & 2) {
+        ++st.tests;
+        if ((r = seg_tri_f32(y0, y1, x0, x1, x2)) > 0) return 1;
+        open |= r < 0;
+        ++st.tests;
+        if ((r = seg_tri_f32(y1, y2, x0, x1, x2)) > 0) return 1;
+        open |= r < 0;
+        ++st.tests;
+        if ((r = seg_tri_f32(y2, y0, x0, x1, x2)) > 0) return 1;
+        open |= r < 0;
+    }
+    return open ? -1 : 0;
+}
+
+// With F32 the walk runs the fp32 filter only: a leaf pair it cannot decide
+// goes to esc_list (pair, mesh1 leaf, mesh2 leaf) for k_escalate, so the
+// fp64 and exact stages (and their registers) stay out of this kernel; if
+// esc_list is full the pair is sent to the per-edge kernel instead.
 template <bool F32, int QLEN>
 __global__ __launch_bounds__(kDualWaves * PZ_WAVE, 2) void k_narrow_dual(
     const uint64_t* __restrict__ pairs, SetView A, SetView B, uint32_t* __restrict__ found,
     uint32_t* __restrict__ fallback, const uint8_t* __restrict__ meta, int allow_f32,
     const DualTask* __restrict__ tasks_in, uint64_t ntasks, unsigned long long* __restrict__ next_task, int chunk,
     DualTask* __restrict__ tasks_out, unsigned long long* __restrict__ nout, uint64_t out_cap,
+    uint4* __restrict__ esc_list, unsigned long long* __restrict__ esc_count, uint64_t esc_cap,
     unsigned long long* __restrict__ stats) {
     __shared__ int2 q[kDualWaves][QLEN];
     const int lane = __lane_id(), w = threadIdx.x / PZ_WAVE;
@@ -706,7 +745,7 @@ __global__ __launch_bounds__(kDualWaves * PZ_WAVE, 2) void k_narrow_dual(
             if (lane == 0) Q[0] = root;
             __builtin_amdgcn_wave_barrier();
             int bot = 0, top = 1;
-            bool hit = false, ovf = false;
+            bool hit = false, ovf = false, esc_full = false;
             int iter = 0;
             while (top > bot) {
                 if ((++iter & 15) == 0 && found[p]) break;
@@ -720,7 +759,17 @@ __global__ __launch_bounds__(kDualWaves * PZ_WAVE, 2) void k_narrow_dual(
                 if (act) {
                     const int x = e.x, y = e.y;
                     if (x < 0 && y < 0) {
-                        hit = tri_pair_test<F32>(A.leaves[~x], B.leaves[~y], dir_mask, allow_f32 != 0, st);
+                        if (F32) {
+                            const int r = tri_pair_f32(A.leaves[~x], B.leaves[~y], dir_mask, st);
+                            hit = r > 0;
+                            if (r < 0) {
+                                const unsigned long long slot = atomicAdd(esc_count, 1ull);
+                                if (slot < esc_cap) esc_list[slot] = make_uint4(p, (uint32_t)~x, (uint32_t)~y, 0u);
+                                else esc_full = true;
+                            }
+                        } else {
+                            hit = tri_pair_test<false>(A.leaves[~x], B.leaves[~y], dir_mask, false, st);
+                        }
                     } else if (x >= 0 && y >= 0) {
                         const Node X = A.nodes[x], Y = B.nodes[y];
                         float xl[2][3], xh[2][3], yl[2][3], yh[2][3];
@@ -804,9 +853,10 @@ __global__ __launch_bounds__(kDualWaves * PZ_WAVE, 2) void k_narrow_dual(
                 __builtin_amdgcn_wave_barrier();
             }
             const bool any_hit = __ballot(hit) != 0;
+            const bool any_full = __ballot(esc_full) != 0;
             if (lane == 0) {
                 if (any_hit) found[p] = 1u;
-                else if (ovf) fallback[p] = 1u;
+                else if (ovf || any_full) fallback[p] = 1u;
             }
             __builtin_amdgcn_wave_barrier();
         }
@@ -817,6 +867,48 @@ __global__ __launch_bounds__(kDualWaves * PZ_WAVE, 2) void k_narrow_dual(
 __global__ void k_mark_task_pairs(const DualTask* __restrict__ tasks, uint64_t n, uint32_t* __restrict__ fallback) {
     const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i < n) fallback[tasks[i].pair] = 1u;
+}
+
+// The leaf pairs the fp32 filter left open, decided with the fp64 and exact
+// stages (grid-stride over min(*count, cap) entries, so no host read-back).
+// Counters as in tri_pair_test: one fp64 or exact escalation per entry, and
+// a misclassification when unguarded fp32 would have answered otherwise.
+__global__ __launch_bounds__(kBlock) void k_escalate(const uint4* __restrict__ list,
+                                                     const unsigned long long* __restrict__ count, uint64_t cap,
+                                                     SetView A, SetView B, const uint8_t* __restrict__ meta,
+                                                     uint32_t* __restrict__ found, unsigned long long* __restrict__ stats) {
+    const uint64_t n = *count < cap ? *count : cap;
+    LaneStats st{0, 0, 0, 0, 0};
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
+        const uint4 e = list[i];
+        if (found[e.x]) continue;
+        const LeafTri X = A.leaves[e.y], Y = B.leaves[e.z];
+        const int dm = meta[e.x] >> 2;
+        const float3 x0 = make_float3(X.v0.x, X.v0.y, X.v0.z), x1 = make_float3(X.v1.x, X.v1.y, X.v1.z),
+                     x2 = make_float3(X.v2.x, X.v2.y, X.v2.z);
+        const float3 y0 = make_float3(Y.v0.x, Y.v0.y, Y.v0.z), y1 = make_float3(Y.v1.x, Y.v1.y, Y.v1.z),
+                     y2 = make_float3(Y.v2.x, Y.v2.y, Y.v2.z);
+        Esc esc{0};
+        bool hit = false;
+        if (dm & 1)
+            hit = seg_tri<false>(x0, x1, y0, y1, y2, false, esc) || seg_tri<false>(x1, x2, y0, y1, y2, false, esc) ||
+                  seg_tri<false>(x2, x0, y0, y1, y2, false, esc);
+        if (!hit && (dm & 2))
+            hit = seg_tri<false>(y0, y1, x0, x1, x2, false, esc) || seg_tri<false>(y1, y2, x0, x1, x2, false, esc) ||
+                  seg_tri<false>(y2, y0, x0, x1, x2, false, esc);
+        if (esc.level == 2) ++st.exact;
+        else ++st.f64;
+        bool naive = false;
+        if (dm & 1)
+            naive = seg_tri_naive(x0, x1, y0, y1, y2) || seg_tri_naive(x1, x2, y0, y1, y2) ||
+                    seg_tri_naive(x2, x0, y0, y1, y2);
+        if (!naive && (dm & 2))
+            naive = seg_tri_naive(y0, y1, x0, x1, x2) || seg_tri_naive(y1, y2, x0, x1, x2) ||
+                    seg_tri_naive(y2, y0, x0, x1, x2);
+        if (naive != hit) ++st.mis;
+        if (hit) found[e.x] = 1u;
+    }
+    flush_stats(st, stats);
 }
 
 // Parity: thread per (pair, direction).  Counts +Z crossings of the
@@ -1294,7 +1386,9 @@ bool narrow(const uint64_t* pairs, uint64_t np, const uint64_t* task_off, uint64
 bool narrow_dual(const uint64_t* pairs, const SetView& A, const SetView& B, uint32_t* found, uint32_t* fallback,
                  const uint8_t* meta, bool f32, int allow_f32, int queue, const DualTask* tasks_in, uint64_t ntasks,
                  unsigned long long* next_task, int chunk, DualTask* tasks_out, unsigned long long* nout,
-                 uint64_t out_cap, unsigned blocks, unsigned long long* stats, hipStream_t s) {
+                 uint64_t out_cap, uint4* esc_list, unsigned long long* esc_count, uint64_t esc_cap, unsigned blocks,
+                 unsigned long long* stats, hipStream_t s) {
+    f32 = f32 && allow_f32 != 0;  // the fp32-only walk needs the filter's input range
     if (ntasks == 0) return true;
     if (chunk < 1) chunk = 1;
     const uint64_t need = (ntasks + (uint64_t)chunk * kDualWaves - 1) / ((uint64_t)chunk * kDualWaves);
@@ -1303,7 +1397,7 @@ bool narrow_dual(const uint64_t* pairs, const SetView& A, const SetView& B, uint
 #define PZ_LAUNCH_DUAL(F, Qn)                                                                                  \
     hipLaunchKernelGGL((k_narrow_dual<F, Qn>), dim3(g), dim3(kDualWaves * PZ_WAVE), 0, s, pairs, A, B, found,   \
                        fallback, meta, allow_f32, tasks_in, ntasks, next_task, chunk, tasks_out, nout, out_cap, \
-                       stats)
+                       esc_list, esc_count, esc_cap, stats)
 #define PZ_DUAL_Q(Qn)                 \
     case Qn:                          \
         if (f32)                      \
@@ -1326,6 +1420,13 @@ bool narrow_dual(const uint64_t* pairs, const SetView& A, const SetView& B, uint
 
 void mark_task_pairs(const DualTask* tasks, uint64_t n, uint32_t* fallback, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_mark_task_pairs, dim3(grid1(n)), dim3(kBlock), 0, s, tasks, n, fallback);
+}
+
+void escalate(const uint4* list, const unsigned long long* count, uint64_t cap, const SetView& A, const SetView& B,
+              const uint8_t* meta, uint32_t* found, unsigned long long* stats, hipStream_t s) {
+    if (cap == 0) return;
+    const uint64_t g = std::min<uint64_t>(grid1(cap), 1024);
+    hipLaunchKernelGGL(k_escalate, dim3((unsigned)g), dim3(kBlock), 0, s, list, count, cap, A, B, meta, found, stats);
 }
 
 int narrow_dual_waves() { return kDualWaves; }

@@ -108,11 +108,20 @@ bool narrow(const uint64_t* pairs, uint64_t np, const uint64_t* task_off, uint64
 // 256, 512, 1024 or 1536) would overflow moves its oldest half to tasks_out
 // (reserved through *nout, at most out_cap) for the next round; if that is
 // full too, fallback[p] = 1 and the pair must be re-run with narrow().
+// With f32 (and allow_f32) the walk uses the fp32 filter alone: leaf pairs
+// it cannot decide are appended to esc_list (pair, mesh1 leaf, mesh2 leaf;
+// *esc_count zeroed by the caller, at most esc_cap kept, a pair that finds
+// the list full goes to fallback) for escalate().
 // `blocks` = persistent grid size.  false: bad queue length.
 bool narrow_dual(const uint64_t* pairs, const SetView& A, const SetView& B, uint32_t* found, uint32_t* fallback,
                  const uint8_t* meta, bool f32, int allow_f32, int queue, const DualTask* tasks_in, uint64_t ntasks,
                  unsigned long long* next_task, int chunk, DualTask* tasks_out, unsigned long long* nout,
-                 uint64_t out_cap, unsigned blocks, unsigned long long* stats, hipStream_t s);
+                 uint64_t out_cap, uint4* esc_list, unsigned long long* esc_count, uint64_t esc_cap, unsigned blocks,
+                 unsigned long long* stats, hipStream_t s);
+// Decides the leaf pairs narrow_dual left open with the fp64 and exact
+// predicates: found[pair] = 1 on a hit (entries min(*count, cap)).
+void escalate(const uint4* list, const unsigned long long* count, uint64_t cap, const SetView& A, const SetView& B,
+              const uint8_t* meta, uint32_t* found, unsigned long long* stats, hipStream_t s);
 // fallback[tasks[i].pair] = 1 for every task (work the rounds did not finish).
 void mark_task_pairs(const DualTask* tasks, uint64_t n, uint32_t* fallback, hipStream_t s);
 // Waves per block of narrow_dual.

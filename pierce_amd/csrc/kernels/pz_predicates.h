@@ -149,6 +149,23 @@ __device__ __forceinline__ int orient3d(const float3& a, const float3& b, const 
     return orient3d_f64(a, b, c, d, esc);
 }
 
+// fp32 stage alone: the sign when the filter proves it, 0 when it cannot
+// (the caller decides such cases with the exact predicates).  Valid only
+// where orient3d's allow_f32 holds.
+__device__ __forceinline__ int orient3d_f32(const float3& a, const float3& b, const float3& c, const float3& d) {
+#pragma clang fp contract(off)
+    float adx = a.x - d.x, bdx = b.x - d.x, cdx = c.x - d.x;
+    float ady = a.y - d.y, bdy = b.y - d.y, cdy = c.y - d.y;
+    float adz = a.z - d.z, bdz = b.z - d.z, cdz = c.z - d.z;
+    float bdxcdy = bdx * cdy, cdxbdy = cdx * bdy, cdxady = cdx * ady, adxcdy = adx * cdy;
+    float adxbdy = adx * bdy, bdxady = bdx * ady;
+    float det = adz * (bdxcdy - cdxbdy) + bdz * (cdxady - adxcdy) + cdz * (adxbdy - bdxady);
+    float perm = (fabsf(bdxcdy) + fabsf(cdxbdy)) * fabsf(adz) + (fabsf(cdxady) + fabsf(adxcdy)) * fabsf(bdz) +
+                 (fabsf(adxbdy) + fabsf(bdxady)) * fabsf(cdz);
+    float err = PZ_O3D_A32 * perm + 0x1p-100f;
+    return det > err ? 1 : (-det > err ? -1 : 0);
+}
+
 // Naive fp32 sign (no filter): what an unguarded fp32 test would decide.
 __device__ __forceinline__ int orient3d_naive(const float3& a, const float3& b, const float3& c, const float3& d) {
 #pragma clang fp contract(off)
@@ -259,6 +276,22 @@ __device__ __forceinline__ bool seg_tri(const float3& p, const float3& q, const 
     int s3 = orient3d<AllowF32>(p, q, c, a, allow_f32, esc);
     if (s1 * s3 < 0 || s2 * s3 < 0) return false;
     return !(s1 == 0 && s2 == 0 && s3 == 0);
+}
+
+// seg_tri decided by the fp32 filter alone: 1 = intersects, 0 = does not,
+// -1 = undecided.  A certain answer agrees with seg_tri: a miss needs two
+// proven opposite signs (or both endpoints proven on one side), which decide
+// seg_tri whatever the unproven signs are; a hit needs every sign proven.
+__device__ __forceinline__ int seg_tri_f32(const float3& p, const float3& q, const float3& a, const float3& b,
+                                           const float3& c) {
+    const int op = orient3d_f32(a, b, c, p), oq = orient3d_f32(a, b, c, q);
+    if (op != 0 && op == oq) return 0;
+    if (op == 0 || oq == 0) return -1;
+    const int s1 = orient3d_f32(p, q, a, b), s2 = orient3d_f32(p, q, b, c);
+    if (s1 != 0 && s2 != 0 && s1 != s2) return 0;
+    const int s3 = orient3d_f32(p, q, c, a);
+    if (s3 != 0 && ((s1 != 0 && s1 != s3) || (s2 != 0 && s2 != s3))) return 0;
+    return (s1 != 0 && s2 != 0 && s3 != 0) ? 1 : -1;
 }
 
 // What unguarded fp32 arithmetic would decide (for the misclassification count).
