@@ -234,7 +234,9 @@ typedef struct {
     int32_t  keep_device_result; /* reserved, must be 0 */
     const pz_mlp* model;        /* PZ_SIZING_LEARNED */
     int32_t  narrow_mode;       /* two-level: 0 = dual-tree with per-edge fallback, 1 = per-edge only */
-    int32_t  reserved[7];
+    int32_t  broad_mode;        /* two-level: 0 = auto (box scan for <= 4096 mesh2 objects), 1 = object
+                                   LBVH walk, 2 = box scan (every mesh1 box against every mesh2 box) */
+    int32_t  reserved[6];
 } pz_join_desc;
 
 /* Fill *d with the defaults: overlap, both directions, TWO_LEVEL, FP32_FILTERED, TWO_PASS. */

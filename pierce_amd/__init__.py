@@ -79,7 +79,7 @@ class JoinDesc(C.Structure):
                 ("include_overlap_pairs", C.c_int32), ("max_hits_per_edge", C.c_int32), ("capacity", C.c_uint64),
                 ("load_factor", C.c_float), ("margin", C.c_float), ("gamma", C.c_float), ("epsilon", C.c_float),
                 ("no_alpha_correction", C.c_int32), ("keep_device_result", C.c_int32), ("model", C.c_void_p),
-                ("narrow_mode", C.c_int32), ("reserved", C.c_int32 * 7)]
+                ("narrow_mode", C.c_int32), ("broad_mode", C.c_int32), ("reserved", C.c_int32 * 6)]
 
 
 class Phase(C.Structure):

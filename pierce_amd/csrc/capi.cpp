@@ -293,6 +293,10 @@ SegOut run_segment(pz_ctx* c, uint64_t k1, uint64_t k2, int tag, pz_join_result*
 }
 
 enum { kSegPrep = 1, kSegObjBvh = 2 };
+// Largest mesh2 object count whose broad phase tests every box instead of
+// walking an object LBVH (a walk costs tens of microseconds of dependent
+// loads whatever the set size; a full scan of this many boxes costs less).
+constexpr int32_t kBruteMaxObjects = 4096;
 
 // preds_out (optional) receives the learned per-part predictions.
 void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const pz_join_desc& d, pz_join_result* r,
@@ -318,16 +322,21 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
     uint32_t npred = 0;
     if (d.sizing == PZ_SIZING_LEARNED) npred = mlp_predict_async(c, d.model, A, B, pa, pb, meta_of_part, pair_meta_of(d));
 
+    // Few mesh2 objects: the broad phase tests every box (k_broad_brute) and
+    // no object index is built; otherwise mesh1 boxes walk an object LBVH.
+    const bool brute = d.broad_mode == 2 || (d.broad_mode == 0 && nB <= kBruteMaxObjects);
     ph.begin("Build Object Index");
-    const ObjBvh ob = run_segment(c, B->uid, 0, kSegObjBvh, r, [&]() {
-                          SegOut o;
-                          o.ob = build_object_bvh(c, B, pb, "B");
-                          return o;
-                      }).ob;
+    ObjBvh ob;
+    if (!brute)
+        ob = run_segment(c, B->uid, 0, kSegObjBvh, r, [&]() {
+                 SegOut o;
+                 o.ob = build_object_bvh(c, B, pb, "B");
+                 return o;
+             }).ob;
 
     ph.begin("Broad Phase");
     // The broad-phase walk keeps a 64-entry stack: the object LBVH must be shallower.
-    const uint32_t oheight = readback<uint32_t>(c, {ob.height})[0];
+    const uint32_t oheight = brute ? 0u : readback<uint32_t>(c, {ob.height})[0];
     double learned_upper = 0;  // LEARNED: calibrated upper bound of the candidate-pair count
     if (d.sizing == PZ_SIZING_LEARNED) {
         std::vector<SizePrediction> pr = mlp_predict_wait(c, d.model, npred);
@@ -1012,6 +1021,7 @@ pz_status pz_join(pz_ctx* c, const pz_dataset* a, const pz_dataset* b, const pz_
     if (d.load_factor < 0 || d.load_factor > 1) return fail(PZ_E_INVALID, "load factor must be in (0, 1]");
     if (d.sizing == PZ_SIZING_LEARNED && !d.model) return fail(PZ_E_INVALID, "learned sizing needs a model");
     if (d.keep_device_result) return fail(PZ_E_INVALID, "keep_device_result is reserved");
+    if (d.broad_mode < 0 || d.broad_mode > 2) return fail(PZ_E_INVALID, "unknown broad-phase mode");
     init_result(r);
     return guarded([&]() -> pz_status {
         HIP_OK(hipSetDevice(c->device));
@@ -1127,6 +1137,7 @@ pz_status pz_join_batch(pz_ctx* c, const pz_dataset* a, const pz_dataset* b, uin
     }
     if (descs[0].strategy != PZ_STRATEGY_TWO_LEVEL)
         return fail(PZ_E_INVALID, "batched joins use the two-level strategy");
+    if (descs[0].broad_mode < 0 || descs[0].broad_mode > 2) return fail(PZ_E_INVALID, "unknown broad-phase mode");
     return guarded([&]() -> pz_status {
         HIP_OK(hipSetDevice(c->device));
         pz_join_result all;

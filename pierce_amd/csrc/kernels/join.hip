@@ -319,6 +319,98 @@ __global__ __launch_bounds__(kBlock) void k_broad(const uint32_t* __restrict__ o
     }
 }
 
+// Broad phase without an object index, for small mesh2 sets: each mesh1
+// object tests the box of every mesh2 object of its part (the part's objects
+// have ids [boff[part], boff[part + 1])).  Same modes and output as k_broad,
+// except that one object's pairs are grouped by lane, not by id.
+// - S lanes (a power of two dividing the wave) share one mesh1 object; lane j
+//   of the group tests tile entries j, j + S, ...
+// - The block stages the mesh2 boxes its objects need (a contiguous id range:
+//   parts are contiguous in both sets) through LDS, 256 at a time; lanes read
+//   them as broadcasts.  All lanes stay for the barriers.
+// - Count: the group's sum.  Write: lane j writes at off[a] plus the counts
+//   of lanes < j of its group.  Append: one reservation per wave.
+template <int Mode, bool Parts, int S>
+__global__ __launch_bounds__(kBlock) void k_broad_brute(const uint32_t* __restrict__ oboxA, int32_t nA,
+                                                        const uint32_t* __restrict__ oboxB, int32_t nB,
+                                                        const uint32_t* __restrict__ part_of_a,
+                                                        const uint32_t* __restrict__ boff, uint32_t* __restrict__ cnt,
+                                                        const uint64_t* __restrict__ off, uint64_t* __restrict__ out,
+                                                        unsigned long long* __restrict__ cursor, uint64_t cap) {
+    constexpr int kPer = kBlock / S;  // mesh1 objects per block
+    __shared__ float tile[kBlock][6];
+    const int lane = __lane_id();
+    const int j = (int)threadIdx.x % S;
+    const int32_t first = blockIdx.x * kPer;
+    const int32_t a = first + (int32_t)threadIdx.x / S;
+    const bool in = a < nA;
+    int32_t b0 = 0, b1 = nB, bb0 = 0, bb1 = nB;  // this object's and the block's mesh2 ids
+    if (Parts) {
+        const int32_t last = min(nA, first + kPer) - 1;
+        bb0 = (int32_t)boff[part_of_a[first]];
+        bb1 = (int32_t)boff[part_of_a[last] + 1];
+        const uint32_t part = part_of_a[in ? a : first];
+        b0 = (int32_t)boff[part];
+        b1 = (int32_t)boff[part + 1];
+    }
+    float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {-1.f, -1.f, -1.f};
+    if (in) {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = ord2f(oboxA[6ll * a + k]);
+            hi[k] = ord2f(oboxA[6ll * a + 3 + k]);
+        }
+    }
+    const bool active = in && b1 > b0 && lo[0] <= hi[0] && lo[1] <= hi[1] && lo[2] <= hi[2];
+    auto visit = [&](auto&& emit) {
+        for (int32_t t0 = bb0; t0 < bb1; t0 += kBlock) {
+            __syncthreads();
+            const int32_t bl = t0 + (int32_t)threadIdx.x;
+            if (bl < bb1)
+                for (int k = 0; k < 6; ++k) tile[threadIdx.x][k] = ord2f(oboxB[6ll * bl + k]);
+            __syncthreads();
+            if (active) {
+                const int32_t i0 = max(b0 - t0, 0), i1 = min(b1 - t0, min(kBlock, bb1 - t0));
+                for (int32_t i = i0 + j; i < i1; i += S)
+                    if (boxes_meet(lo, hi, tile[i][0], tile[i][1], tile[i][2], tile[i][3], tile[i][4], tile[i][5]))
+                        emit(t0 + i);
+            }
+        }
+    };
+    uint32_t n = 0;
+    visit([&](int32_t) { ++n; });
+    if (Mode == 0) {
+        uint32_t sum = n;
+        for (int sft = S / 2; sft > 0; sft >>= 1) sum += (uint32_t)__shfl_xor((int)sum, sft, PZ_WAVE);
+        if (in && j == 0) cnt[a] = sum;
+        return;
+    }
+    uint64_t base;
+    if (Mode == 1) {
+        uint32_t incl = n;  // inclusive prefix within the group
+        for (int sft = 1; sft < S; sft <<= 1) {
+            const uint32_t u = (uint32_t)__shfl_up((int)incl, sft, PZ_WAVE);
+            if (j >= sft) incl += u;
+        }
+        base = (in ? off[a] : 0ull) + incl - n;
+    } else {
+        uint32_t incl = n;  // inclusive prefix over the wave
+        for (int sft = 1; sft < PZ_WAVE; sft <<= 1) {
+            const uint32_t u = (uint32_t)__shfl_up((int)incl, sft, PZ_WAVE);
+            if (lane >= sft) incl += u;
+        }
+        const uint32_t total = (uint32_t)__shfl((int)incl, PZ_WAVE - 1, PZ_WAVE);
+        unsigned long long wbase = 0ull;
+        if (lane == 0 && total > 0) wbase = atomicAdd(cursor, (unsigned long long)total);
+        wbase = __shfl(wbase, 0, PZ_WAVE);
+        base = wbase + incl - n;
+    }
+    uint32_t i = 0;
+    visit([&](int32_t b) {
+        if (Mode == 1 || base + i < cap) out[base + i] = pack_pair(a, b);
+        ++i;
+    });
+}
+
 // Per-pair predicate and direction mask: meta = predicate | dir_mask << 2
 // (predicate 0 overlap, 1 intersection, 2 containment; dir bit 0: mesh1 edges
 // and points, bit 1: mesh2's).  Batches take them from the pair's part.
@@ -1297,9 +1389,32 @@ void object_keys(const uint32_t* obox, int32_t nobj, const uint32_t* scene, cons
                            keys, vals);
 }
 
+// Lanes per mesh1 object of the box scan: enough lanes to fill the GPU.
+inline int brute_lanes(int32_t nA) { return nA <= 4096 ? 64 : (nA <= 16384 ? 16 : (nA <= 65536 ? 4 : 1)); }
+#define PZ_BRUTE_S(M, P, Sn, ...)                                                                          \
+    hipLaunchKernelGGL((k_broad_brute<M, P, Sn>), dim3((unsigned)(((uint64_t)nA * Sn + kBlock - 1) / kBlock)), \
+                       dim3(kBlock), 0, s, oboxA, nA, oboxB, nB, part_of_a, boff, __VA_ARGS__)
+#define PZ_BRUTE_P(M, P, ...)                                   \
+    switch (brute_lanes(nA)) {                                  \
+        case 64: PZ_BRUTE_S(M, P, 64, __VA_ARGS__); break;      \
+        case 16: PZ_BRUTE_S(M, P, 16, __VA_ARGS__); break;      \
+        case 4: PZ_BRUTE_S(M, P, 4, __VA_ARGS__); break;        \
+        default: PZ_BRUTE_S(M, P, 1, __VA_ARGS__); break;       \
+    }
+#define PZ_BRUTE(M, ...)                          \
+    do {                                          \
+        if (boff) {                               \
+            PZ_BRUTE_P(M, true, __VA_ARGS__)      \
+        } else {                                  \
+            PZ_BRUTE_P(M, false, __VA_ARGS__)     \
+        }                                         \
+    } while (0)
+
 void broad_count(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const Node* nodesB, const uint32_t* orderB,
                  int32_t nB, const uint32_t* part_of_a, const uint32_t* boff, uint32_t* cnt, hipStream_t s) {
-    if (nA > 0)
+    if (nA > 0 && !nodesB)
+        PZ_BRUTE(0, cnt, (const uint64_t*)nullptr, (uint64_t*)nullptr, (unsigned long long*)nullptr, (uint64_t)0);
+    else if (nA > 0)
         hipLaunchKernelGGL((k_broad<0>), dim3(grid1((uint64_t)nA)), dim3(kBlock), 0, s, oboxA, nA, oboxB, nodesB, orderB, nB,
                            part_of_a, boff, cnt, (const uint64_t*)nullptr, (uint64_t*)nullptr, (unsigned long long*)nullptr,
                            (uint64_t)0);
@@ -1307,17 +1422,25 @@ void broad_count(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const
 void broad_write(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const Node* nodesB, const uint32_t* orderB,
                  int32_t nB, const uint32_t* part_of_a, const uint32_t* boff, const uint64_t* off, uint64_t* out,
                  hipStream_t s) {
-    if (nA > 0)
+    if (nA > 0 && !nodesB)
+        PZ_BRUTE(1, (uint32_t*)nullptr, off, out, (unsigned long long*)nullptr, (uint64_t)0);
+    else if (nA > 0)
         hipLaunchKernelGGL((k_broad<1>), dim3(grid1((uint64_t)nA)), dim3(kBlock), 0, s, oboxA, nA, oboxB, nodesB, orderB, nB,
                            part_of_a, boff, (uint32_t*)nullptr, off, out, (unsigned long long*)nullptr, (uint64_t)0);
 }
 void broad_append(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const Node* nodesB,
                   const uint32_t* orderB, int32_t nB, const uint32_t* part_of_a, const uint32_t* boff, uint64_t* out,
                   unsigned long long* cursor, uint64_t cap, hipStream_t s) {
-    if (nA > 0)
+    if (nA > 0 && !nodesB)
+        PZ_BRUTE(2, (uint32_t*)nullptr, (const uint64_t*)nullptr, out, cursor, cap);
+    else if (nA > 0)
         hipLaunchKernelGGL((k_broad<2>), dim3(grid1((uint64_t)nA)), dim3(kBlock), 0, s, oboxA, nA, oboxB, nodesB, orderB, nB,
                            part_of_a, boff, (uint32_t*)nullptr, (const uint64_t*)nullptr, out, cursor, cap);
 }
+
+#undef PZ_BRUTE
+#undef PZ_BRUTE_P
+#undef PZ_BRUTE_S
 
 void mark_needed(const uint64_t* pairs, uint64_t np, uint32_t* needA, uint32_t* needB, hipStream_t s) {
     if (np) hipLaunchKernelGGL(k_mark_needed, dim3(grid1(np)), dim3(kBlock), 0, s, pairs, np, needA, needB);

@@ -72,6 +72,8 @@ void object_keys(const uint32_t* obox, int32_t nobj, const uint32_t* scene, cons
 // Broad phase: mesh1 object boxes against the mesh2 object LBVH (stack of 64).
 // With parts (batched joins), mesh1 object a only meets the mesh2 objects of
 // its part: sorted positions [boff[part_of_a[a]], boff[part + 1]).
+// nodesB == nullptr: no LBVH; every mesh2 object of the part is tested
+// (object ids [boff[part], boff[part + 1]), or all nB without parts).
 void broad_count(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const Node* nodesB, const uint32_t* orderB,
                  int32_t nB, const uint32_t* part_of_a, const uint32_t* boff, uint32_t* cnt, hipStream_t s);
 void broad_write(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const Node* nodesB, const uint32_t* orderB,

@@ -157,9 +157,9 @@ SetPrep prep_set(pz_ctx* c, const pz_dataset* d, const std::string& tag);
 
 // LBVH over the object boxes of one set (broad-phase target).
 struct ObjBvh {
-    pzd::Node* nodes;
-    uint32_t* order;
-    uint32_t* height;
+    pzd::Node* nodes = nullptr;  // nullptr: no object index (the broad phase tests every box)
+    uint32_t* order = nullptr;
+    uint32_t* height = nullptr;
 };
 
 // Host-side results of a segment of device work (pool pointers), kept with its graph.

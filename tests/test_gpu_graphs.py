@@ -3,6 +3,9 @@
 Each replay is checked against the eager result, with other joins (eager
 work and synchronisation) between the replays, and again after a larger join
 has moved the context's buffers (the graphs must be dropped and re-captured).
+The segments are the object statistics of both sets and, when the broad
+phase walks one (WALK), the mesh2 object index: two graph launches per join.
+With the box scan there is no object index and one launch.
 """
 import numpy as np
 import pytest
@@ -11,6 +14,9 @@ import pierce_amd as pz
 from oracle import oracle as orc
 
 pytestmark = pytest.mark.gpu
+
+
+WALK = dict(broad_mode=1)
 
 
 def _sets(count, seed, box):
@@ -24,13 +30,13 @@ def test_replays_match_eager_with_interleaved_work(ctx):
     c, d = _sets(300, 7, 40.0)
     da, db, dc, dd = ctx.upload(a), ctx.upload(b), ctx.upload(c), ctx.upload(d)
     ctx.set_graphs(False)
-    want_ab = pz.join(ctx, da, db)
+    want_ab = pz.join(ctx, da, db, **WALK)
     want_cd = pz.join(ctx, dc, dd, predicate=pz.PRED_INTERSECTION)
     np.testing.assert_array_equal(want_ab.pairs, orc.face_overlap(a, b))
     ctx.set_graphs(True)
     launches = []
     for i in range(5):
-        out = pz.join(ctx, da, db)
+        out = pz.join(ctx, da, db, **WALK)
         np.testing.assert_array_equal(out.pairs, want_ab.pairs, err_msg=f"replay {i}")
         assert out.num_candidates == want_ab.num_candidates
         launches.append(out.graph_launches)
@@ -46,16 +52,20 @@ def test_buffer_growth_drops_graphs(ctx):
     da, db = ctx.upload(a), ctx.upload(b)
     want = orc.face_overlap(a, b)
     for _ in range(3):
-        np.testing.assert_array_equal(pz.join(ctx, da, db).pairs, want)
+        np.testing.assert_array_equal(pz.join(ctx, da, db, **WALK).pairs, want)
     dba, dbb = ctx.upload(big_a), ctx.upload(big_b)
-    np.testing.assert_array_equal(pz.join(ctx, dba, dbb).pairs, orc.face_overlap(big_a, big_b))
-    first = pz.join(ctx, da, db)  # buffers moved: runs eagerly and re-records
+    np.testing.assert_array_equal(pz.join(ctx, dba, dbb, **WALK).pairs, orc.face_overlap(big_a, big_b))
+    first = pz.join(ctx, da, db, **WALK)  # buffers moved: runs eagerly and re-records
     np.testing.assert_array_equal(first.pairs, want)
     assert first.graph_launches == 0
     for _ in range(3):
-        out = pz.join(ctx, da, db)
+        out = pz.join(ctx, da, db, **WALK)
         np.testing.assert_array_equal(out.pairs, want)
     assert out.graph_launches == 2
+    for _ in range(3):  # box scan: the statistics segment only
+        out = pz.join(ctx, da, db, broad_mode=2)
+        np.testing.assert_array_equal(out.pairs, want)
+    assert out.graph_launches == 1
 
 
 def test_graphs_with_learned_sizing_side_stream(ctx):
@@ -67,7 +77,7 @@ def test_graphs_with_learned_sizing_side_stream(ctx):
     X = rng.normal(0, 1, (64, pz.MLP_FEATURES)).astype(np.float32)
     m.train(X, np.abs(rng.normal(8, 2, (64, 2))).astype(np.float32), epochs=5)
     for _ in range(4):
-        out = pz.join(ctx, da, db, sizing=pz.SIZING_LEARNED, model=m)
+        out = pz.join(ctx, da, db, sizing=pz.SIZING_LEARNED, model=m, **WALK)
         np.testing.assert_array_equal(out.pairs, want)
     assert out.graph_launches == 2
     m.close()

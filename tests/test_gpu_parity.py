@@ -125,12 +125,23 @@ def test_self_join_and_swap(ctx):
     assert set(map(tuple, ab.tolist())) == set(map(tuple, ba[:, ::-1].tolist()))
 
 
-def test_sizing_modes_agree(ctx, c1):
+@pytest.mark.parametrize("broad_mode", [1, 2])  # object LBVH walk, box scan
+def test_sizing_modes_agree(ctx, c1, broad_mode):
     a, b, da, db, expected = c1
     for sizing, cap in ((pz.SIZING_TWO_PASS, 0), (pz.SIZING_FIXED, 1), (pz.SIZING_FIXED, 1 << 20),
                         (pz.SIZING_ESTIMATE, 0)):
-        out = pz.join(ctx, da, db, sizing=sizing, capacity=cap)
+        out = pz.join(ctx, da, db, sizing=sizing, capacity=cap, broad_mode=broad_mode)
         np.testing.assert_array_equal(out.pairs, expected)
+
+
+def test_broad_modes_give_the_same_candidates(ctx, c1):
+    a, b, da, db, expected = c1
+    walk = pz.join(ctx, da, db, broad_mode=1)
+    scan = pz.join(ctx, da, db, broad_mode=2)
+    assert walk.num_candidates == scan.num_candidates > 0
+    np.testing.assert_array_equal(walk.pairs, scan.pairs)
+    with pytest.raises(pz.PierceError):
+        pz.join(ctx, da, db, broad_mode=3)
 
 
 def test_empty_result(ctx):

@@ -51,11 +51,12 @@ def test_sweep_single(ctx, sweep, pred, precision):
     np.testing.assert_array_equal(out.pairs, exp[pred])
 
 
-def test_sweep_three_predicates_as_one_batch(ctx, sweep):
+@pytest.mark.parametrize("broad_mode", [0, 1, 2])  # auto, object LBVH walk, box scan (per query part)
+def test_sweep_three_predicates_as_one_batch(ctx, sweep, broad_mode):
     a, b, _, _, exp = sweep
     da = ctx.upload_batch([a, a, a])
     db = ctx.upload_batch([b, b, b])
-    outs, summary = pz.join_batch(ctx, da, db, [dict(predicate=p) for p in PREDS])
+    outs, summary = pz.join_batch(ctx, da, db, [dict(predicate=p, broad_mode=broad_mode) for p in PREDS])
     for p, o in zip(PREDS, outs):
         np.testing.assert_array_equal(o.pairs, exp[p], err_msg=f"predicate {p}")
 
