@@ -162,16 +162,17 @@ void finish_pairs(pz_ctx* c, uint64_t* keys, uint32_t* types, uint64_t n, int32_
     r->num_pairs = n;
     r->pairs = static_cast<pz_pair*>(std::malloc(std::max<uint64_t>(n, 1) * sizeof(pz_pair)));
     if (!r->pairs) throw PzError(PZ_E_NOMEM, "host result allocation failed");
-    if (n) HIP_OK(hipMemcpyAsync(r->pairs, dpairs, n * sizeof(pz_pair), hipMemcpyDeviceToHost, c->s));
-    std::vector<uint32_t> ht;
-    if (want_types && n) {
-        ht.resize(n);
-        HIP_OK(hipMemcpyAsync(ht.data(), st, n * 4, hipMemcpyDeviceToHost, c->s));
-    }
+    // through the pinned landing buffer: device-to-pinned copies, then host copies
+    const size_t pb = n * sizeof(pz_pair), tb = want_types ? n * 4 : 0;
+    char* h = static_cast<char*>(pinned(c, std::max<size_t>(pb + tb, 8)));
+    if (n) HIP_OK(hipMemcpyAsync(h, dpairs, pb, hipMemcpyDeviceToHost, c->s));
+    if (want_types && n) HIP_OK(hipMemcpyAsync(h + pb, st, tb, hipMemcpyDeviceToHost, c->s));
     HIP_OK(hipStreamSynchronize(c->s));
+    if (n) std::memcpy(r->pairs, h, pb);
     if (want_types) {
         r->pair_types = static_cast<uint8_t*>(std::malloc(std::max<uint64_t>(n, 1)));
         if (!r->pair_types) throw PzError(PZ_E_NOMEM, "host result allocation failed");
+        const uint32_t* ht = reinterpret_cast<const uint32_t*>(h + pb);
         for (uint64_t i = 0; i < n; ++i) r->pair_types[i] = static_cast<uint8_t>(ht[i]);
     }
 }
@@ -458,12 +459,11 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
     uint32_t* totB = buf<uint32_t>(c, "B.totals", 8);
     // needed objects and their triangles, then totals[0..7] of each set, in one synchronisation
     uint32_t* hb = static_cast<uint32_t*>(pinned(c, 20 * sizeof(uint32_t)));
-    HIP_OK(hipMemcpyAsync(hb + 0, na.loc + std::max(nA, 0), 4, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipMemcpyAsync(hb + 1, na.seg_off + std::max(nA, 0), 4, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipMemcpyAsync(hb + 2, totA, 32, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipMemcpyAsync(hb + 10, nb.loc + std::max(nB, 0), 4, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipMemcpyAsync(hb + 11, nb.seg_off + std::max(nB, 0), 4, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipMemcpyAsync(hb + 12, totB, 32, hipMemcpyDeviceToHost, s));
+    pzk::HostGather g{{na.loc + std::max(nA, 0), na.seg_off + std::max(nA, 0), totA, nb.loc + std::max(nB, 0),
+                       nb.seg_off + std::max(nB, 0), totB},
+                      {1, 1, 8, 1, 1, 8},
+                      6};
+    pzk::gather_to_host(g, hb, s);
     HIP_OK(hipStreamSynchronize(s));
     auto tot_of = [&](const uint32_t* h) {
         Tot t{h[0], h[1], 0, h[2], h[3], {}, {}};
@@ -535,6 +535,7 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
     uint64_t nfb = np;
     const uint64_t* fb_pairs = pairs;
     uint32_t* fb_index = nullptr;
+    uint32_t height = 0;  // of the per-object BVHs of both sets
     if (d.narrow_mode == 0) {
         const int qclass = (ta.nbig + tb.nbig > 0) ? 1 : 0;  // pairs with objects beyond the LDS builder
         c->last_queue_class = qclass;
@@ -557,6 +558,15 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         const pzd::DualTask* cur = nullptr;
         uint64_t ntasks = np, spilled_total = 0;
         int rounds = 0;
+        uint64_t* fbp = buf<uint64_t>(c, "np.fb_pairs", np + 1);
+        fb_index = buf<uint32_t>(c, "np.fb_index", np + 1);
+        unsigned long long* fbc = buf<unsigned long long>(c, "np.fb_count", 1);
+        // after the last round: decide the open leaf pairs, list the fallback pairs
+        auto tail = [&]() {
+            pzk::escalate(esc, ctr + 2, cap_esc, va, vb, meta, found, stats, s);
+            pzk::fill_ull(fbc, 0ull, 1, s);
+            pzk::compact_flagged(pairs, np, fallback, fbp, fb_index, fbc, s);
+        };
         while (ntasks > 0) {
             pzk::fill_ull(ctr, 0ull, 2, s);
             if (!pzk::narrow_dual(pairs, va, vb, found, fallback, meta, f32, allow_f32, c->tune.narrow_queue[qclass],
@@ -564,32 +574,38 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
                                   ctr + 2, cap_esc, c->persistent_blocks, stats, s))
                 throw PzError(PZ_E_INVALID, "invalid narrow-phase queue length");
             ++rounds;
-            const uint64_t spilled = std::min<uint64_t>(readback<unsigned long long>(c, {ctr + 1})[0], cap_tasks);
+            // The tail runs as if this were the last round (it usually is), so one
+            // read-back brings the spill count, the fallback count and both heights.
+            tail();
+            uint32_t* hb = static_cast<uint32_t*>(pinned(c, 6 * sizeof(uint32_t)));
+            pzk::HostGather g{{reinterpret_cast<const uint32_t*>(ctr + 1), reinterpret_cast<const uint32_t*>(fbc),
+                               xa.height, xb.height},
+                              {2, 2, 1, 1},
+                              4};
+            pzk::gather_to_host(g, hb, s);
+            HIP_OK(hipStreamSynchronize(s));
+            const uint64_t spilled = std::min<uint64_t>(hb[0] | ((uint64_t)hb[1] << 32), cap_tasks);
+            nfb = hb[2] | ((uint64_t)hb[3] << 32);
+            height = std::max(hb[4], hb[5]);
             spilled_total += spilled;
             if (spilled == 0) break;
             if (rounds >= 32) {  // leave what is left to the per-edge kernel
                 pzk::mark_task_pairs(tout, spilled, fallback, s);
+                tail();
+                nfb = readback<unsigned long long>(c, {fbc})[0];
                 break;
             }
             std::swap(tin, tout);
             cur = tin;
             ntasks = spilled;
         }
-        pzk::escalate(esc, ctr + 2, cap_esc, va, vb, meta, found, stats, s);
         r->narrow_rounds = rounds;
         r->tasks_spilled = spilled_total;
-        uint64_t* fbp = buf<uint64_t>(c, "np.fb_pairs", np + 1);
-        fb_index = buf<uint32_t>(c, "np.fb_index", np + 1);
-        unsigned long long* fbc = buf<unsigned long long>(c, "np.fb_count", 1);
-        pzk::fill_ull(fbc, 0ull, 1, s);
-        pzk::compact_flagged(pairs, np, fallback, fbp, fb_index, fbc, s);
-        nfb = readback<unsigned long long>(c, {fbc})[0];
         fb_pairs = fbp;
-    }
-    const uint32_t height = [&]() {
+    } else {
         auto hh = readback<uint32_t>(c, {xa.height, xb.height});
-        return std::max(hh[0], hh[1]);
-    }();
+        height = std::max(hh[0], hh[1]);
+    }
     if (pzk::stack_for_height(height) == 0)
         throw PzError(PZ_E_LIMIT, "per-object BVH height " + std::to_string(height) + " exceeds the traversal stack (63)");
     r->stack_overflows = d.narrow_mode == 0 ? nfb : 0;  // pairs re-run by the per-edge kernel
@@ -649,10 +665,12 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
     pzk::fill_ull(ocnt, 0ull, 1, s);
     // batches keep overlap pairs of containment queries (typed) and filter per query on the host
     pzk::select_pairs(pairs, np, found, inside, meta, batched ? 1 : d.include_overlap_pairs, okeys, otype, ocnt, s);
-    uint64_t nout = readback<unsigned long long>(c, {ocnt})[0];
+    // the result count and the test counters in one read-back
+    auto st = readback<unsigned long long>(c, {ocnt, stats, stats + 1, stats + 2, stats + 3, stats + 4});
+    const uint64_t nout = st[0];
+    st.erase(st.begin());
     const bool want_types = batched || (d.predicate == PZ_PRED_CONTAINMENT && d.include_overlap_pairs);
     finish_pairs(c, okeys, want_types ? otype : nullptr, nout, nA, want_types, r);
-    auto st = readback<unsigned long long>(c, {stats, stats + 1, stats + 2, stats + 3, stats + 4});
     r->tests_total = st[0];
     r->tests_fp64 = st[1];
     r->tests_exact = st[2];

@@ -33,6 +33,15 @@ void keys_to_pairs(const uint64_t* keys, uint64_t n, int32_t* pairs, hipStream_t
 void pairs_to_keys(const int32_t* pairs, uint64_t n, uint64_t* keys, hipStream_t s);
 
 void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
+// Up to 8 device ranges of 32-bit words, copied back to back into host-pinned
+// memory by one small kernel (one launch instead of one copy per range).
+struct HostGather {
+    static constexpr int kMax = 8;
+    const uint32_t* src[kMax];
+    uint32_t len[kMax];  // words
+    uint32_t n;
+};
+void gather_to_host(const HostGather& g, uint32_t* host_dst, hipStream_t s);
 
 // Optional per-kernel timing: when enabled, launches bracketed by KernelTimer
 // record a HIP event pair on their stream; totals are collected after the

@@ -419,6 +419,16 @@ void pairs_to_keys(const int32_t* pairs, uint64_t n, uint64_t* keys, hipStream_t
         hipLaunchKernelGGL(k_pairs_to_keys, dim3(grid1(n)), dim3(256), 0, s, reinterpret_cast<const int2*>(pairs), n,
                            keys);
 }
+__global__ void k_gather_to_host(HostGather g, uint32_t* __restrict__ dst) {
+    uint32_t at = 0;
+    for (uint32_t k = 0; k < g.n; ++k) {
+        for (uint32_t i = threadIdx.x; i < g.len[k]; i += blockDim.x) dst[at + i] = g.src[k][i];
+        at += g.len[k];
+    }
+}
+void gather_to_host(const HostGather& g, uint32_t* host_dst, hipStream_t s) {
+    if (g.n) hipLaunchKernelGGL(k_gather_to_host, dim3(1), dim3(64), 0, s, g, host_dst);
+}
 void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
     if (n) hipLaunchKernelGGL((k_fill<uint32_t>), dim3(grid1(n)), dim3(256), 0, s, p, v, n);
 }

@@ -16,6 +16,7 @@
 
 #include "host/host_mesh.hpp"
 #include "kernels/pz_device.h"
+#include "kernels/pz_kernels.h"
 #include "pierce_hip.h"
 
 struct DevBuf {
@@ -113,12 +114,19 @@ inline void* pinned(pz_ctx* c, size_t bytes) {
     return c->pinned;
 }
 
-// Read n scalars (each from a device address) in one synchronisation.
+// Read up to 8 scalars (each from a device address) with one gather kernel
+// writing into pinned memory and one synchronisation.
 template <class T>
 std::vector<T> readback(pz_ctx* c, std::initializer_list<const T*> srcs) {
+    static_assert(sizeof(T) % 4 == 0, "32-bit words");
+    if (srcs.size() > (size_t)pzk::HostGather::kMax) throw PzError(PZ_E_INVALID, "readback of too many values");
     T* h = reinterpret_cast<T*>(pinned(c, srcs.size() * sizeof(T)));
-    size_t i = 0;
-    for (const T* p : srcs) HIP_OK(hipMemcpyAsync(h + i++, p, sizeof(T), hipMemcpyDeviceToHost, c->s));
+    pzk::HostGather g{};
+    for (const T* p : srcs) {
+        g.src[g.n] = reinterpret_cast<const uint32_t*>(p);
+        g.len[g.n++] = sizeof(T) / 4;
+    }
+    pzk::gather_to_host(g, reinterpret_cast<uint32_t*>(h), c->s);
     HIP_OK(hipStreamSynchronize(c->s));
     return std::vector<T>(h, h + srcs.size());
 }
