@@ -271,6 +271,9 @@ typedef struct {
     uint64_t tasks_spilled;     /* narrow-phase subtree tasks handed from one work-queue round to the next */
     int32_t  narrow_rounds;     /* work-queue rounds of the narrow phase */
     int32_t  graph_launches;    /* fixed-shape segments replayed from captured HIP graphs */
+    uint64_t hash_accesses;     /* GLOBAL with a hash set: slots probed (0 elsewhere) */
+    uint64_t hash_contentions;  /* probed slots already holding another pair
+                                   (pierce_overlap_main.cpp:676-686 reports both per run) */
     double   ms_total;          /* device-resident geometry -> sorted host pair set */
     pz_phase phases[PZ_MAX_PHASES];
 } pz_join_result;

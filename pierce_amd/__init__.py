@@ -93,7 +93,8 @@ class JoinResult(C.Structure):
                 ("tests_fp64", C.c_uint64), ("tests_exact", C.c_uint64), ("tests_fp32_misclassified", C.c_uint64),
                 ("edges_traced", C.c_uint64), ("stack_overflows", C.c_uint64), ("index_triangles", C.c_uint64),
                 ("index_edges", C.c_uint64), ("tasks_spilled", C.c_uint64), ("narrow_rounds", C.c_int32),
-                ("graph_launches", C.c_int32), ("ms_total", C.c_double),
+                ("graph_launches", C.c_int32), ("hash_accesses", C.c_uint64), ("hash_contentions", C.c_uint64),
+                ("ms_total", C.c_double),
                 ("phases", Phase * MAX_PHASES)]
 
 
@@ -470,6 +471,8 @@ class JoinOutput:
     tasks_spilled: int = 0
     narrow_rounds: int = 0
     graph_launches: int = 0
+    hash_accesses: int = 0
+    hash_contentions: int = 0
 
 
 def join_desc(**kw) -> JoinDesc:
@@ -500,7 +503,8 @@ def _output(r: JoinResult) -> JoinOutput:
     return JoinOutput(pairs, types, int(r.num_candidates), int(r.capacity), int(r.predicted), bool(r.overflowed),
                       int(r.tests_total), int(r.tests_fp64), int(r.tests_exact), int(r.tests_fp32_misclassified),
                       int(r.edges_traced), int(r.index_triangles), int(r.index_edges), float(r.ms_total), phases,
-                      int(r.predicted_pairs), int(r.tasks_spilled), int(r.narrow_rounds), int(r.graph_launches))
+                      int(r.predicted_pairs), int(r.tasks_spilled), int(r.narrow_rounds), int(r.graph_launches),
+                      int(r.hash_accesses), int(r.hash_contentions))
 
 
 def join_batch(ctx: Context, a: Dataset, b: Dataset, queries, **common):

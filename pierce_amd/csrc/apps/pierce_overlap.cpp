@@ -4,9 +4,18 @@
 // sizes the first-pass buffer; an overflow falls back to the exact count, so
 // the result never loses pairs.
 #include <cmath>
+#include <iomanip>
 #include <iostream>
 
 #include "app_common.hpp"
+
+// "<label>: contended/accesses accesses (pct%)", as pierce_overlap_main.cpp:676-686 prints it.
+static void contention_line(const std::string& label, const pz_join_result& r) {
+    const double pct = r.hash_accesses ? 100.0 * (double)r.hash_contentions / (double)r.hash_accesses : 0.0;
+    std::cout << std::fixed << std::setprecision(2) << label << ": " << r.hash_contentions << "/" << r.hash_accesses
+              << " accesses (" << pct << "%)" << std::endl;
+    std::cout.unsetf(std::ios::floatfield);
+}
 
 int main(int argc, char** argv) {
     app::Timer timer;
@@ -42,7 +51,7 @@ int main(int argc, char** argv) {
                                {"--epsilon <float>", "Estimation epsilon (default: 0.001)"},
                                {"--query-direction <both|mesh1_to_mesh2|mesh2_to_mesh1>", "Query direction"},
                                {"--pairs-output <path>", "CSV of result pairs (object_id_mesh1,object_id_mesh2)"},
-                               {"--track-hash-contention", "Accepted; the two-level join uses no hash table"},
+                               {"--track-hash-contention", "Report hash-set probes and contention (the global strategy's hash set; 0/0 for two-level)"},
                                {"--no-alpha-correction", "Disable the replication correction of the estimate"},
                                {"--hash-table-size <n>", "Fixed first-pass capacity"},
                                {"--hash-table-free-mem-fraction <f>", "Capacity from a fraction of free HBM"},
@@ -131,6 +140,7 @@ int main(int argc, char** argv) {
         std::cout << "Hash Table Allocated Bytes: " << r.capacity * 8 << std::endl;
         if (r.overflowed) std::cout << "Sized buffer overflowed; exact count used." << std::endl;
         std::cout << "Hash Table Query found " << r.num_pairs << " unique pairs." << std::endl;
+        if (track) contention_line("Hash contention (run " + std::to_string(run + 1) + ")", r);
     }
     timer.next("Output");
     app::add_phases(timer, r);
@@ -144,6 +154,7 @@ int main(int argc, char** argv) {
     std::cout << "Mesh2 objects: " << no2 << std::endl;
     std::cout << "Query direction: " << direction << std::endl;
     std::cout << "Hash contention tracking: " << (track ? "enabled" : "disabled") << std::endl;
+    if (track) contention_line("Hash contention (last run)", r);
     std::cout << "Unique object pairs: " << r.num_pairs << std::endl;
     std::cout << "Result Buffer Capacity: " << r.capacity << std::endl;
     std::cout << "Result Buffer Allocated Bytes: " << r.capacity * sizeof(pz_pair) << std::endl;

@@ -839,12 +839,15 @@ void join_global(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const pz_j
     }
     ph.begin("Download Results");
     finish_pairs(c, uniq, nullptr, nuniq, A->nobj, false, r);
-    auto st = readback<unsigned long long>(c, {stats, stats + 1, stats + 2, stats + 3, stats + 4});
+    auto st = readback<unsigned long long>(c, {stats, stats + 1, stats + 2, stats + 3, stats + 4, stats + 5,
+                                                stats + 6});
     r->tests_total = st[0];
     r->tests_fp64 = st[1];
     r->tests_exact = st[2];
     r->edges_traced = st[4];
     r->tests_fp32_misclassified = st[3];
+    r->hash_accesses = st[5];
+    r->hash_contentions = st[6];
 }
 
 }  // namespace

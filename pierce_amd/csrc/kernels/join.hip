@@ -540,11 +540,12 @@ __global__ __launch_bounds__(kBlock) void k_edge_scatter(const float* __restrict
 // ------------------------------------------------------------- traversal
 struct LaneStats {
     uint32_t tests, f64, exact, mis, traced;
+    uint32_t probes = 0, contended = 0;  // hash-set slots probed / found holding another key
 };
 
 __device__ __forceinline__ void flush_stats(LaneStats& st, unsigned long long* stats) {
-    uint32_t v[5] = {st.tests, st.f64, st.exact, st.mis, st.traced};
-    for (int k = 0; k < 5; ++k) {
+    uint32_t v[7] = {st.tests, st.f64, st.exact, st.mis, st.traced, st.probes, st.contended};
+    for (int k = 0; k < 7; ++k) {
         uint32_t s = wave_sum(v[k]);
         if (__lane_id() == 0 && s) atomicAdd(&stats[k], (unsigned long long)s);
     }
@@ -1159,15 +1160,17 @@ __device__ __forceinline__ uint64_t fmix64(uint64_t k) {
 
 // Insert into a power-of-two open-addressing set; false when the probe
 // budget runs out (the caller flags overflow and re-runs exactly).
-__device__ __forceinline__ bool hash_insert(unsigned long long* table, uint64_t mask, uint64_t key) {
+__device__ __forceinline__ bool hash_insert(unsigned long long* table, uint64_t mask, uint64_t key, LaneStats& st) {
     uint64_t h = fmix64(key) & mask;
     for (uint32_t probe = 0; probe < 4096; ++probe) {
+        ++st.probes;
         unsigned long long cur = table[h];
         if (cur == (unsigned long long)key) return true;
         if (cur == ~0ull) {
             unsigned long long prev = atomicCAS(&table[h], ~0ull, (unsigned long long)key);
             if (prev == ~0ull || prev == (unsigned long long)key) return true;
         }
+        ++st.contended;
         h = (h + 1) & mask;
     }
     return false;
@@ -1245,7 +1248,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_trace_global(
                 last = tgt;
                 uint64_t key = swap ? pack_pair(tgt, src) : pack_pair(src, tgt);
                 if (Mode == 1) out[base + n] = key;
-                if (Mode == 2 && !hash_insert(table, mask, key)) *overflow = 1u;
+                if (Mode == 2 && !hash_insert(table, mask, key, st)) *overflow = 1u;
                 ++n;
             }
         }

@@ -119,3 +119,30 @@ def test_obj_through_preprocess_to_two_pass(tmp_path):
     exp = orc.face_overlap(pz.read_obj(str(tmp_path / "a.obj")), pz.read_obj(str(tmp_path / "b.obj")))
     assert len(exp) > 0
     assert pairs == [tuple(p) for p in exp.tolist()]
+
+
+def test_hash_contention_lines(pre_files, tmp_path):
+    """--track-hash-contention prints the lines pierce_adapter.py:290-304 parses;
+    probes minus contended probes = successful inserts >= unique pairs."""
+    d, a, b = pre_files
+    out = run("pierce_overlap", ["--mesh1", str(d / "a.pre"), "--mesh2", str(d / "b.pre"), "--strategy", "global",
+                                 "--track-hash-contention", "--no-export", "--runs", "2"], tmp_path)
+    pat = r"^Hash contention \((run \d+|last run)\): (\d+)/(\d+) accesses \((\d+\.\d\d)%\)$"
+    lines = re.findall(pat, out, re.M)
+    assert [l[0] for l in lines] == ["run 1", "run 2", "last run"], out
+    n = len(orc.face_overlap(a, b))
+    for _, cont, acc, pct in lines:
+        cont, acc = int(cont), int(acc)
+        assert 0 <= cont <= acc and acc - cont >= n > 0
+        assert abs(float(pct) - 100.0 * cont / acc) < 0.006
+    assert re.search(r"^Hash contention tracking: enabled$", out, re.M)
+
+
+def test_hash_counters_in_the_result(ctx, pre_files):
+    _, a, b = pre_files
+    da, db = ctx.upload(a), ctx.upload(b)
+    g = pz.join(ctx, da, db, strategy=pz.STRATEGY_GLOBAL, sizing=pz.SIZING_ESTIMATE)
+    assert 0 <= g.hash_contentions <= g.hash_accesses and g.hash_accesses - g.hash_contentions >= len(g.pairs) > 0
+    t = pz.join(ctx, da, db)  # two-level: no hash set
+    assert t.hash_accesses == 0 and t.hash_contentions == 0
+    np.testing.assert_array_equal(g.pairs, t.pairs)
