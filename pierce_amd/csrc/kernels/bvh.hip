@@ -122,8 +122,12 @@ __device__ __forceinline__ void load_child_box(const Node* nd, int slot, float* 
 // never leave the CU: plain stores, a workgroup-scope fence and an LDS
 // counter per node.  Only the few nodes that span blocks (the top levels of
 // each segment) use the agent-scope protocol above.
+// Leaves per workgroup: 2048 for large sets; 256 below 2M leaves so that a
+// small set (an object index of 100k boxes) still spreads over every CU.
 constexpr int kRefitBlock = 2048;
+constexpr int kRefitBlockSmall = 256;
 constexpr int kRefitThreads = 256;
+inline int refit_block_for(uint64_t n) { return n >= (uint64_t)kRefitBlock * 1024 ? kRefitBlock : kRefitBlockSmall; }
 
 __device__ __forceinline__ void store_child_box_local(Node* nd, int slot, const float* lo, const float* hi,
                                                       int height) {
@@ -145,15 +149,15 @@ __device__ __forceinline__ void load_child_box_local(const Node* nd, int slot, f
     *height = __float_as_int(f[14 + slot]);
 }
 
-template <class BoxFn>
+template <int BLK, class BoxFn>
 __device__ void refit_blocked(const uint64_t* __restrict__ keys, uint64_t n, const uint32_t* __restrict__ seg_off,
                               int seg_shift, Node* nodes, const uint32_t* __restrict__ parent_int,
                               const uint32_t* __restrict__ parent_leaf, uint32_t* counters,
                               const uint32_t* __restrict__ range_end, uint32_t* max_depth, BoxFn boxfn) {
-    __shared__ uint32_t lcnt[kRefitBlock];
-    const uint64_t B = (uint64_t)blockIdx.x * kRefitBlock;
-    const uint64_t E = B + kRefitBlock < n ? B + kRefitBlock : n;
-    for (int i = threadIdx.x; i < kRefitBlock; i += kRefitThreads) lcnt[i] = 0u;
+    __shared__ uint32_t lcnt[BLK];
+    const uint64_t B = (uint64_t)blockIdx.x * BLK;
+    const uint64_t E = B + BLK < n ? B + BLK : n;
+    for (int i = threadIdx.x; i < BLK; i += kRefitThreads) lcnt[i] = 0u;
     __syncthreads();
     for (uint64_t p = B + threadIdx.x; p < E; p += kRefitThreads) {
         int64_t b, e;
@@ -230,21 +234,25 @@ struct OrdBoxFn {  // boxes stored as ordered uints: 6 per primitive
     }
 };
 
+template <int BLK>
 __global__ __launch_bounds__(kRefitThreads) void k_refit_tris(const uint64_t* keys, uint64_t n,
                                                                const uint32_t* seg_off, int seg_shift, Node* nodes,
                                                                const uint32_t* parent_int,
                                                                const uint32_t* parent_leaf, uint32_t* counters,
                                                                const uint32_t* range_end, uint32_t* max_depth,
                                                                TriBoxFn fn) {
-    refit_blocked(keys, n, seg_off, seg_shift, nodes, parent_int, parent_leaf, counters, range_end, max_depth, fn);
+    refit_blocked<BLK>(keys, n, seg_off, seg_shift, nodes, parent_int, parent_leaf, counters, range_end, max_depth,
+                       fn);
 }
+template <int BLK>
 __global__ __launch_bounds__(kRefitThreads) void k_refit_boxes(const uint64_t* keys, uint64_t n,
                                                                 const uint32_t* seg_off, int seg_shift, Node* nodes,
                                                                 const uint32_t* parent_int,
                                                                 const uint32_t* parent_leaf, uint32_t* counters,
                                                                 const uint32_t* range_end, uint32_t* max_depth,
                                                                 OrdBoxFn fn) {
-    refit_blocked(keys, n, seg_off, seg_shift, nodes, parent_int, parent_leaf, counters, range_end, max_depth, fn);
+    refit_blocked<BLK>(keys, n, seg_off, seg_shift, nodes, parent_int, parent_leaf, counters, range_end, max_depth,
+                       fn);
 }
 
 __global__ void k_gather_leaf_tris(const uint32_t* __restrict__ order, uint64_t n, const float* __restrict__ verts,
@@ -285,8 +293,14 @@ void lbvh_refit_tris(const uint64_t* sorted_keys, const uint32_t* order, uint64_
     if (n == 0) return;
     TriBoxFn fn{verts, idx, order};
     KernelTimer kt_("refit", s);
-    hipLaunchKernelGGL(k_refit_tris, dim3(grid1(n, kRefitBlock)), dim3(kRefitThreads), 0, s, sorted_keys, n, seg_off,
-                       seg_shift, nodes, parent_int, parent_leaf, counters, range_end, max_depth, fn);
+    if (refit_block_for(n) == kRefitBlock)
+        hipLaunchKernelGGL(k_refit_tris<kRefitBlock>, dim3(grid1(n, kRefitBlock)), dim3(kRefitThreads), 0, s,
+                           sorted_keys, n, seg_off, seg_shift, nodes, parent_int, parent_leaf, counters, range_end,
+                           max_depth, fn);
+    else
+        hipLaunchKernelGGL(k_refit_tris<kRefitBlockSmall>, dim3(grid1(n, kRefitBlockSmall)), dim3(kRefitThreads), 0,
+                           s, sorted_keys, n, seg_off, seg_shift, nodes, parent_int, parent_leaf, counters, range_end,
+                           max_depth, fn);
 }
 
 void lbvh_refit_ord_boxes(const uint64_t* sorted_keys, const uint32_t* order, uint64_t n, const uint32_t* seg_off,
@@ -295,8 +309,14 @@ void lbvh_refit_ord_boxes(const uint64_t* sorted_keys, const uint32_t* order, ui
                           uint32_t* max_depth, hipStream_t s) {
     if (n == 0) return;
     OrdBoxFn fn{obox, order};
-    hipLaunchKernelGGL(k_refit_boxes, dim3(grid1(n, kRefitBlock)), dim3(kRefitThreads), 0, s, sorted_keys, n, seg_off,
-                       seg_shift, nodes, parent_int, parent_leaf, counters, range_end, max_depth, fn);
+    if (refit_block_for(n) == kRefitBlock)
+        hipLaunchKernelGGL(k_refit_boxes<kRefitBlock>, dim3(grid1(n, kRefitBlock)), dim3(kRefitThreads), 0, s,
+                           sorted_keys, n, seg_off, seg_shift, nodes, parent_int, parent_leaf, counters, range_end,
+                           max_depth, fn);
+    else
+        hipLaunchKernelGGL(k_refit_boxes<kRefitBlockSmall>, dim3(grid1(n, kRefitBlockSmall)), dim3(kRefitThreads), 0,
+                           s, sorted_keys, n, seg_off, seg_shift, nodes, parent_int, parent_leaf, counters, range_end,
+                           max_depth, fn);
 }
 
 void gather_leaf_tris(const uint32_t* order, uint64_t n, const float* verts, const uint32_t* idx, const int32_t* t2o,
