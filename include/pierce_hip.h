@@ -400,6 +400,26 @@ pz_status pz_autotune(pz_ctx* ctx, const pz_dataset* mesh1, const pz_dataset* me
 pz_status pz_count_candidate_tests(pz_ctx* ctx, const pz_dataset* mesh1, const pz_dataset* mesh2,
                                    uint64_t* out);
 
+/* Containment-ray tracking of pierce_intersection --enable-tracking.
+ * Replaces the per-source tracking buffers that the any-hit program fills
+ * (pierce/query/src/cuda/mesh_intersection_anyhit.cu:46-240) and that
+ * pierce/query/src/utilities/PairHitTracking.h:30-117 and
+ * ContainmentTracking.h:22-100 copy out.  For each object s of `source` with
+ * triangles, the +Z ray from the first vertex of its first triangle is cast
+ * through every triangle of `target`; a triangle it crosses is one hit on
+ * that triangle's object.  Host outputs (all sized by source's object count):
+ *   target_ids/target_hits[s * max_targets + i]: the objects hit an odd
+ *     number of times (those that contain the point), ascending ids, with
+ *     their hit counts; unused slots -1 / 0;
+ *   candidate_count[s]: distinct objects hit (at most max_targets);
+ *   candidate_overflow[s]: hits on objects beyond the first max_targets met;
+ *   iterations[s]: 1 if a ray was cast, 0 for an object without triangles.
+ * 1 <= max_targets <= 4096 (the reference tracks 256). */
+pz_status pz_track_containment_rays(pz_ctx* ctx, const pz_dataset* source, const pz_dataset* target,
+                                    int32_t max_targets, int32_t* target_ids, uint32_t* target_hits,
+                                    uint32_t* candidate_count, uint32_t* candidate_overflow,
+                                    uint32_t* iterations);
+
 /* ------------------------------------------------------------- profiling */
 /* Per-kernel device time from HIP events recorded on the launching stream
  * (the phase timer of pierce/query/src/timer.cpp, at kernel granularity).

@@ -892,6 +892,27 @@ int or_point_in_object(const or_mesh_view* m, int32_t obj, const float* p) {
     return point_in_object(M, bvh, obj, p) ? 1 : 0;
 }
 
+// Containment-ray tracking (pierce/query/src/cuda/mesh_intersection_anyhit.cu:46-240,
+// read out by pierce/query/src/utilities/PairHitTracking.h:30-117): for every
+// source object with triangles, the +Z ray from the first vertex of its first
+// triangle against every triangle of T, brute force; hits[s * T.nobj + t] =
+// triangles of object t crossed.  Returns 0.
+int or_ray_target_hits(const or_mesh_view* s, const or_mesh_view* t, uint32_t* hits) {
+    Mesh S = make_mesh(s), T = make_mesh(t);
+    const uint64_t nt = (uint64_t)std::max<int32_t>(T.nobj, 0);
+    std::fill(hits, hits + (uint64_t)std::max<int32_t>(S.nobj, 0) * nt, 0u);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int32_t o = 0; o < S.nobj; ++o) {
+        if (S.first_tri[o] < 0) continue;
+        const float* p = first_vertex(S, o);
+        for (uint64_t k = 0; k < T.nt; ++k) {
+            if (!T.valid(k)) continue;
+            if (ray_cross(p, T.vert(k, 0), T.vert(k, 1), T.vert(k, 2))) ++hits[(uint64_t)o * nt + T.t2o[k]];
+        }
+    }
+    return 0;
+}
+
 // Brute force: every triangle pair (tiny inputs only).
 or_pairs* or_bruteforce_overlap(const or_mesh_view* a, const or_mesh_view* b) {
     Mesh A = make_mesh(a), B = make_mesh(b);

@@ -69,6 +69,8 @@ def lib():
         L.or_bruteforce_overlap.argtypes = [mv, mv]
         L.or_point_in_object.restype = C.c_int
         L.or_point_in_object.argtypes = [mv, C.c_int32, fp]
+        L.or_ray_target_hits.restype = C.c_int
+        L.or_ray_target_hits.argtypes = [mv, mv, C.POINTER(C.c_uint32)]
         L.or_count_candidate_tests.restype = C.c_uint64
         L.or_count_candidate_tests.argtypes = [mv, mv]
         L.or_set_threads.restype = None
@@ -176,6 +178,16 @@ def point_in_object(m, obj: int, p) -> bool:
     v = _View(m)
     pp, _k = _f(p)
     return bool(lib().or_point_in_object(C.byref(v.v), obj, pp))
+
+
+def ray_target_hits(src, tgt) -> np.ndarray:
+    """(src objects, tgt objects) uint32: triangles of each target object that
+    the +Z ray from the first vertex of each source object crosses (brute
+    force; pierce/query/src/cuda/mesh_intersection_anyhit.cu:46-240)."""
+    vs, vt = _View(src), _View(tgt)
+    out = np.zeros((src.num_objects, tgt.num_objects), np.uint32)
+    lib().or_ray_target_hits(C.byref(vs.v), C.byref(vt.v), out.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return out
 
 
 def count_candidate_tests(m1, m2) -> int:

@@ -152,6 +152,8 @@ _SIGS = [
     ("pz_join", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(JoinDesc), C.POINTER(JoinResult)]),
     ("pz_join_result_release", None, [C.POINTER(JoinResult)]),
     ("pz_count_candidate_tests", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]),
+    ("pz_track_containment_rays", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
+                                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("pz_dataset_upload_batch", C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(MeshView), C.POINTER(C.c_void_p)]),
     ("pz_join_batch", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(JoinDesc),
                                   C.POINTER(JoinResult), C.POINTER(JoinResult)]),
@@ -560,6 +562,24 @@ def count_candidate_tests(ctx: Context, a: Dataset, b: Dataset) -> int:
     out = C.c_uint64()
     _check(lib().pz_count_candidate_tests(ctx._h, a._h, b._h, C.byref(out)))
     return int(out.value)
+
+
+def track_containment_rays(ctx: Context, source: Dataset, target: Dataset, max_targets: int = 256) -> dict:
+    """Per-source containment-ray tracking (pierce_intersection --enable-tracking;
+    pierce/query/src/cuda/mesh_intersection_anyhit.cu:46-240).
+
+    Returns arrays over source objects: ``target_ids`` / ``target_hits``
+    (n, max_targets): objects crossed an odd number of times, ascending, -1 / 0
+    padded; ``candidate_count``: distinct objects crossed; ``candidate_overflow``:
+    hits beyond the first max_targets objects; ``iterations``: 1 if a ray was cast.
+    """
+    n = int(source.info()["objects"])
+    ids = np.empty((n, max_targets), np.int32)
+    hits = np.empty((n, max_targets), np.uint32)
+    cnt, ovf, it = (np.empty(n, np.uint32) for _ in range(3))
+    _check(lib().pz_track_containment_rays(ctx._h, source._h, target._h, int(max_targets), _ptr(ids), _ptr(hits),
+                                           _ptr(cnt), _ptr(ovf), _ptr(it)))
+    return dict(target_ids=ids, target_hits=hits, candidate_count=cnt, candidate_overflow=ovf, iterations=it)
 
 
 # ------------------------------------------------------------ learned sizing

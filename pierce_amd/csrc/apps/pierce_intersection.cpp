@@ -1,9 +1,103 @@
 // pierce_intersection: surface overlap or one solid containing the other
 // (pierce/query/src/applications/pierce_intersection_main.cpp:541-978).
-// The pairs CSV is always written (default intersection_pairs.csv).
+// The pairs CSV is always written (default intersection_pairs.csv).  With
+// --enable-tracking the containment rays' per-source tracking is written too,
+// in the two CSV layouts of pierce_intersection_main.cpp:425-539.
+#include <algorithm>
+#include <cstdio>
 #include <iostream>
+#include <vector>
 
 #include "app_common.hpp"
+
+namespace {
+
+constexpr int kTrackedTargets = 256;  // PairHitTracking.h:31
+
+// One direction's tracking over its source objects.
+struct Tracking {
+    std::vector<int32_t> ids;
+    std::vector<uint32_t> hits, candidates, overflow, iterations;
+    void init(uint64_t n) {
+        ids.assign(n * kTrackedTargets, -1);
+        hits.assign(n * kTrackedTargets, 0u);
+        candidates.assign(n, 0u);
+        overflow.assign(n, 0u);
+        iterations.assign(n, 0u);
+    }
+    bool run(pz_ctx* ctx, const pz_dataset* src, const pz_dataset* tgt) {
+        return pz_track_containment_rays(ctx, src, tgt, kTrackedTargets, ids.data(), hits.data(), candidates.data(),
+                                         overflow.data(), iterations.data()) == PZ_OK;
+    }
+};
+
+// Is (mesh1 a, mesh2 b) among the result pairs (sorted by (mesh1, mesh2))?
+bool final_pair(const pz_join_result& r, int32_t a, int32_t b) {
+    const pz_pair* b0 = r.pairs;
+    const pz_pair* e = b0 + r.num_pairs;
+    const pz_pair* it = std::lower_bound(b0, e, std::make_pair(a, b), [](const pz_pair& x, std::pair<int32_t, int32_t> k) {
+        return x.object_id_mesh1 != k.first ? x.object_id_mesh1 < k.first : x.object_id_mesh2 < k.second;
+    });
+    return it != e && it->object_id_mesh1 == a && it->object_id_mesh2 == b;
+}
+
+// direction,source_object_id,target_object_id,target_ray_hits,final_pair
+bool write_pair_hits(const std::string& path, const Tracking& t12, const Tracking& t21, const pz_join_result& r) {
+    FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) {
+        std::cerr << "Error: Failed to open intersection tracking output: " << path << std::endl;
+        return false;
+    }
+    std::fprintf(f, "direction,source_object_id,target_object_id,target_ray_hits,final_pair\n");
+    for (int d = 0; d < 2; ++d) {
+        const Tracking& t = d == 0 ? t12 : t21;
+        const uint64_t n = t.candidates.size();
+        for (uint64_t s = 0; s < n; ++s)
+            for (int i = 0; i < kTrackedTargets; ++i) {
+                const int32_t tgt = t.ids[s * kTrackedTargets + i];
+                if (tgt < 0) continue;
+                const bool fin = d == 0 ? final_pair(r, (int32_t)s, tgt) : final_pair(r, tgt, (int32_t)s);
+                std::fprintf(f, "%s,%llu,%d,%u,%d\n", d == 0 ? "mesh1_to_mesh2" : "mesh2_to_mesh1",
+                             (unsigned long long)s, tgt, t.hits[s * kTrackedTargets + i], fin ? 1 : 0);
+            }
+    }
+    std::fclose(f);
+    return true;
+}
+
+// direction,source_object_id,iterations,candidate_count,tracked_target_count,
+// tracked_hit_total,candidate_overflow_events,final_pair_count,tracked_final_pair_count
+bool write_summary(const std::string& path, const Tracking& t12, const Tracking& t21, const pz_join_result& r) {
+    FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) {
+        std::cerr << "Error: Failed to open containment tracking output: " << path << std::endl;
+        return false;
+    }
+    std::fprintf(f, "direction,source_object_id,iterations,candidate_count,tracked_target_count,tracked_hit_total,"
+                    "candidate_overflow_events,final_pair_count,tracked_final_pair_count\n");
+    for (int d = 0; d < 2; ++d) {
+        const Tracking& t = d == 0 ? t12 : t21;
+        const uint64_t n = t.candidates.size();
+        for (uint64_t s = 0; s < n; ++s) {
+            int tracked = 0, fin = 0;
+            unsigned long long total = 0;
+            for (int i = 0; i < kTrackedTargets; ++i) {
+                const int32_t tgt = t.ids[s * kTrackedTargets + i];
+                if (tgt < 0) continue;
+                ++tracked;
+                total += t.hits[s * kTrackedTargets + i];
+                fin += (d == 0 ? final_pair(r, (int32_t)s, tgt) : final_pair(r, tgt, (int32_t)s)) ? 1 : 0;
+            }
+            std::fprintf(f, "%s,%llu,%u,%u,%d,%llu,%u,%d,%d\n", d == 0 ? "mesh1_to_mesh2" : "mesh2_to_mesh1",
+                         (unsigned long long)s, t.iterations[s], t.candidates[s], tracked, total, t.overflow[s], fin,
+                         fin);
+        }
+    }
+    std::fclose(f);
+    return true;
+}
+
+}  // namespace
 
 int main(int argc, char** argv) {
     app::Timer timer;
@@ -12,6 +106,7 @@ int main(int argc, char** argv) {
     c.output = "estimated_intersection_timing.json";
     float gamma = 0.8f, epsilon = 0.001f, load_factor = 0.5f;
     std::string direction = "both", pairs_out = "intersection_pairs.csv";
+    std::string hits_out = "intersection_pair_hits.csv", tracking_out = "intersection_containment_tracking.csv";
     bool estimate_only = false, tracking = false, profiling = false;
     int max_iter = 100;
     app::parse(argc, argv, c, false, [&](const std::string& a, int& i, int argc_, char** argv_) {
@@ -25,7 +120,8 @@ int main(int argc, char** argv) {
         else if (a == "--enable-tracking") tracking = true;
         else if (a == "--enable-profiling-stats") profiling = true;
         else if (a == "--pairs-output" && has()) pairs_out = argv_[++i];
-        else if ((a == "--pair-hits-output" || a == "--containment-tracking-output") && has()) ++i;
+        else if (a == "--pair-hits-output" && has()) hits_out = argv_[++i];
+        else if (a == "--containment-tracking-output" && has()) tracking_out = argv_[++i];
         else return false;
         return true;
     });
@@ -38,9 +134,12 @@ int main(int argc, char** argv) {
                   {"--query-direction <both|mesh1_to_mesh2|mesh2_to_mesh1>", "Control query direction (default: both)"},
                   {"--overlap-max-iterations <int>", "Accepted (> 0); every hit is enumerated exactly"},
                   {"--hash-load-factor <float>", "Load factor in (0,1] for sized buffers (default: 0.5)"},
-                  {"--enable-tracking", "Accepted; per-source tracking CSVs are not produced"},
+                  {"--enable-tracking", "Enable optional containment candidate tracking and CSV exports"},
                   {"--enable-profiling-stats", "Write device test counters into the timing JSON"},
-                  {"--pairs-output <path>", "Intersection pairs CSV path (default: intersection_pairs.csv)"}});
+                  {"--pairs-output <path>", "Intersection pairs CSV path (default: intersection_pairs.csv)"},
+                  {"--pair-hits-output <path>", "Pair hit tracking CSV path (default: intersection_pair_hits.csv)"},
+                  {"--containment-tracking-output <path>",
+                   "Containment tracking CSV path (default: intersection_containment_tracking.csv)"}});
         app::print_help(argv[0], "--mesh1 <path> --mesh2 <path> [options]",
                         "Intersection query: overlapping surfaces or one object inside the other.", o);
         return 0;
@@ -66,7 +165,6 @@ int main(int argc, char** argv) {
         std::cerr << "Usage: " << argv[0] << " --mesh1 <path> --mesh2 <path> [options]" << std::endl;
         return 1;
     }
-    if (tracking) std::cout << "Note: --enable-tracking is accepted; tracking CSVs are not produced." << std::endl;
     timer.next("Application Creation");
     pz_ctx* ctx = nullptr;
     if (pz_ctx_create(c.device, &ctx) != PZ_OK) return app::fail_join("device initialisation");
@@ -134,7 +232,21 @@ int main(int argc, char** argv) {
         timer.counter("candidate_pairs", r.num_candidates);
     }
     app::write_pairs_csv(pairs_out, "a_object_id,b_object_id", r);
+    if (tracking) {
+        // directions that were not queried keep empty tracking (no rows in the
+        // pair-hit CSV, zero rows in the summary), as the reference's buffers do
+        Tracking t12, t21;
+        t12.init(no1);
+        t21.init(no2);
+        if (dir != PZ_DIR_2TO1 && !t12.run(ctx, d1, d2)) return app::fail_join("tracking mesh1_to_mesh2");
+        if (dir != PZ_DIR_1TO2 && !t21.run(ctx, d2, d1)) return app::fail_join("tracking mesh2_to_mesh1");
+        if (!write_pair_hits(hits_out, t12, t21, r) || !write_summary(tracking_out, t12, t21, r)) return 1;
+    }
     std::cout << "Intersection pairs CSV: " << pairs_out << std::endl;
+    if (tracking) {
+        std::cout << "Pair hit tracking CSV: " << hits_out << std::endl;
+        std::cout << "Containment tracking CSV: " << tracking_out << std::endl;
+    }
     pz_join_result_release(&r);
     pz_dataset_release(d1);
     pz_dataset_release(d2);

@@ -1097,6 +1097,63 @@ pz_status pz_count_candidate_tests(pz_ctx* c, const pz_dataset* a, const pz_data
     });
 }
 
+pz_status pz_track_containment_rays(pz_ctx* c, const pz_dataset* src, const pz_dataset* tgt, int32_t max_targets,
+                                    int32_t* target_ids, uint32_t* target_hits, uint32_t* candidate_count,
+                                    uint32_t* candidate_overflow, uint32_t* iterations) {
+    if (!c || !src || !tgt) return fail(PZ_E_INVALID, "pz_track_containment_rays: null argument");
+    if (max_targets < 1 || max_targets > 4096)
+        return fail(PZ_E_INVALID, "pz_track_containment_rays: max_targets must be in [1, 4096]");
+    const uint64_t ns = (uint64_t)std::max<int32_t>(src->nobj, 0);
+    if (ns && (!target_ids || !target_hits || !candidate_count || !candidate_overflow || !iterations))
+        return fail(PZ_E_INVALID, "pz_track_containment_rays: null output array");
+    return guarded([&]() -> pz_status {
+        HIP_OK(hipSetDevice(c->device));
+        hipStream_t s = c->s;
+        const uint64_t slots = ns * (uint64_t)max_targets;
+        std::fill(target_ids, target_ids + slots, -1);
+        std::fill(target_hits, target_hits + slots, 0u);
+        std::fill(candidate_count, candidate_count + ns, 0u);
+        std::fill(candidate_overflow, candidate_overflow + ns, 0u);
+        std::fill(iterations, iterations + ns, 0u);
+        if (ns == 0) return PZ_OK;
+        unsigned long long* stats = buf<unsigned long long>(c, "stats", 8);
+        pzk::fill_ull(stats, 0ull, 8, s);
+        SetPrep ps = prep_set(c, src, "A");
+        SetPrep pt = prep_set(c, tgt, "B");
+        GlobalBvh gt{};
+        if (tgt->nt) gt = build_global(c, tgt, pt, "B");
+        int32_t* ids = buf<int32_t>(c, "trk.ids", slots);
+        uint32_t* hits = buf<uint32_t>(c, "trk.hits", slots);
+        uint32_t* cnt = buf<uint32_t>(c, "trk.cnt", 3 * ns);
+        if (!pzk::track_rays(src->v, src->idx, ps.first_tri, ps.tri_cnt, src->nobj, gt.nodes, gt.leaves, tgt->nt,
+                             gt.height, max_targets, ids, hits, cnt, cnt + ns, cnt + 2 * ns, stats, s))
+            throw PzError(PZ_E_LIMIT, "set-wide BVH too tall for the tracking ray's stack");
+        std::vector<int32_t> h_ids(slots);
+        std::vector<uint32_t> h_hits(slots), h_cnt(3 * ns);
+        HIP_OK(hipMemcpyAsync(h_ids.data(), ids, slots * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(h_hits.data(), hits, slots * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(h_cnt.data(), cnt, 3 * ns * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        std::vector<std::pair<int32_t, uint32_t>> odd;
+        for (uint64_t o = 0; o < ns; ++o) {
+            const uint64_t base = o * (uint64_t)max_targets;
+            const uint32_t n = h_cnt[o];
+            odd.clear();
+            for (uint32_t i = 0; i < n; ++i)
+                if (h_hits[base + i] & 1u) odd.emplace_back(h_ids[base + i], h_hits[base + i]);
+            std::sort(odd.begin(), odd.end());
+            for (size_t i = 0; i < odd.size(); ++i) {
+                target_ids[base + i] = odd[i].first;
+                target_hits[base + i] = odd[i].second;
+            }
+            candidate_count[o] = n;
+            candidate_overflow[o] = h_cnt[ns + o];
+            iterations[o] = h_cnt[2 * ns + o];
+        }
+        return PZ_OK;
+    });
+}
+
 pz_status pz_dataset_upload_batch(pz_ctx* c, uint32_t nparts, const pz_mesh_view* views, pz_dataset** out) {
     if (!c || !views || !out || nparts == 0) return fail(PZ_E_INVALID, "pz_dataset_upload_batch: bad argument");
     return guarded([&]() -> pz_status {

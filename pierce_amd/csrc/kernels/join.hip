@@ -1089,6 +1089,89 @@ __global__ __launch_bounds__(kBlock, 4) void k_parity(const uint64_t* __restrict
     flush_stats(st, stats);
 }
 
+// Containment-ray tracking (pierce_intersection --enable-tracking,
+// pierce/query/src/cuda/mesh_intersection_anyhit.cu:46-240): thread per
+// source object.  The +Z ray from the first vertex of the object's first
+// triangle walks the set-wide BVH of the other set; every triangle it crosses
+// (the perturbed crossing of k_parity) adds a hit to that triangle's object
+// in the source's list.  The list keeps objects in the order the walk meets
+// them, at most max_targets; hits on objects beyond that count as overflow.
+// Sources without triangles cast no ray (iterations 0).
+template <int STACK>
+__global__ __launch_bounds__(kBlock, 4) void k_track_rays(const float* __restrict__ vS, const uint32_t* __restrict__ iS,
+                                                       const uint32_t* __restrict__ ftS,
+                                                       const uint32_t* __restrict__ tcS, int32_t nsrc,
+                                                       const Node* __restrict__ nodes,
+                                                       const LeafTri* __restrict__ leaves, uint64_t ntri,
+                                                       int max_targets, int32_t* __restrict__ ids,
+                                                       uint32_t* __restrict__ hits, uint32_t* __restrict__ count,
+                                                       uint32_t* __restrict__ overflow, uint32_t* __restrict__ iters,
+                                                       unsigned long long* __restrict__ stats) {
+    __shared__ int stk[kWavesPerBlock * STACK * PZ_WAVE];
+    const int lane = __lane_id(), w = threadIdx.x / PZ_WAVE;
+    int* my = stk + (w * STACK) * PZ_WAVE + lane;
+    const int64_t src = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    LaneStats st{0, 0, 0, 0, 0};
+    bool act = src < nsrc && tcS[src] > 0;
+    float3 pt = make_float3(0, 0, 0);
+    if (act) pt = vtx(vS, iS[3ull * ftS[src]]);
+    int32_t* my_ids = ids + src * (int64_t)max_targets;
+    uint32_t* my_hits = hits + src * (int64_t)max_targets;
+    uint32_t n = 0, ovf = 0;
+    const float rlo[3] = {pt.x, pt.y, pt.z}, rhi[3] = {pt.x, pt.y, INFINITY};
+    int cur = ntri >= 2 ? 0 : ~0, sp = 0;
+    bool walk = act && ntri > 0;
+    while (walk) {
+        if (cur >= 0) {
+            const Node nd = nodes[cur];
+            const bool h0 = boxes_meet(rlo, rhi, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
+            const bool h1 = boxes_meet(rlo, rhi, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w);
+            if (h0 && h1) {
+                my[sp * PZ_WAVE] = nd.d.y;
+                ++sp;
+                cur = nd.d.x;
+                continue;
+            } else if (h0) {
+                cur = nd.d.x;
+                continue;
+            } else if (h1) {
+                cur = nd.d.y;
+                continue;
+            }
+        } else {
+            const LeafTri L = leaves[~cur];
+            Esc esc{0};
+            const float3 a = make_float3(L.v0.x, L.v0.y, L.v0.z), b = make_float3(L.v1.x, L.v1.y, L.v1.z),
+                         c = make_float3(L.v2.x, L.v2.y, L.v2.z);
+            ++st.tests;
+            if (ray_cross(pt, a, b, c, esc)) {
+                const int32_t obj = (int32_t)__float_as_uint(L.v1.w);
+                uint32_t i = 0;
+                while (i < n && my_ids[i] != obj) ++i;
+                if (i < n) {
+                    ++my_hits[i];
+                } else if (n < (uint32_t)max_targets) {
+                    my_ids[n] = obj;
+                    my_hits[n] = 1u;
+                    ++n;
+                } else {
+                    ++ovf;
+                }
+            }
+            if (esc.level == 2) ++st.exact;
+        }
+        if (sp == 0) break;
+        --sp;
+        cur = my[sp * PZ_WAVE];
+    }
+    if (src < nsrc) {
+        count[src] = n;
+        overflow[src] = ovf;
+        iters[src] = act ? 1u : 0u;
+    }
+    flush_stats(st, stats);
+}
+
 // Ballot + prefix-sum compaction of selected candidate pairs.
 // sel: 0 drop; else keep with type value (sel - 1) in vals.
 __global__ __launch_bounds__(kBlock) void k_select_pairs(const uint64_t* __restrict__ pairs, uint64_t np,
@@ -1572,6 +1655,23 @@ bool parity(const uint64_t* pairs, uint64_t np, const uint32_t* found, const Set
     else
         hipLaunchKernelGGL((k_parity<64>), dim3(grid1(n)), dim3(kBlock), 0, s, pairs, np, found, A, B, vA, iA, ftA, vB,
                            iB, ftB, meta, inside, stats);
+    return true;
+}
+
+bool track_rays(const float* vS, const uint32_t* iS, const uint32_t* ftS, const uint32_t* tcS, int32_t nsrc,
+                const Node* nodes, const LeafTri* leaves, uint64_t ntri, uint32_t height, int max_targets,
+                int32_t* ids, uint32_t* hits, uint32_t* count, uint32_t* overflow, uint32_t* iters,
+                unsigned long long* stats, hipStream_t s) {
+    if (nsrc <= 0) return true;
+    const int stack = stack_for_height(height);
+    if (stack == 0) return false;
+    KernelTimer kt_("track_rays", s);
+    if (stack == 32)
+        hipLaunchKernelGGL((k_track_rays<32>), dim3(grid1((uint64_t)nsrc)), dim3(kBlock), 0, s, vS, iS, ftS, tcS, nsrc,
+                           nodes, leaves, ntri, max_targets, ids, hits, count, overflow, iters, stats);
+    else
+        hipLaunchKernelGGL((k_track_rays<64>), dim3(grid1((uint64_t)nsrc)), dim3(kBlock), 0, s, vS, iS, ftS, tcS, nsrc,
+                           nodes, leaves, ntri, max_targets, ids, hits, count, overflow, iters, stats);
     return true;
 }
 

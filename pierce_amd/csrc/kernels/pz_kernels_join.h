@@ -137,6 +137,16 @@ bool parity(const uint64_t* pairs, uint64_t np, const uint32_t* found, const Set
             const float* vA, const uint32_t* iA, const uint32_t* ftA, const float* vB, const uint32_t* iB,
             const uint32_t* ftB, const uint8_t* meta, uint32_t* inside, uint32_t height,
             unsigned long long* stats, hipStream_t s);
+// Containment-ray tracking: for each source object with triangles, the +Z
+// ray from its first vertex through the set-wide BVH (nodes, leaves, ntri
+// triangles, height) of the other set.  Per source: ids/hits[src *
+// max_targets + i] for the first `count` objects met, `overflow` hits on
+// objects beyond max_targets, iters = 1 if a ray was cast.  false: the BVH is
+// too tall for the traversal stack.
+bool track_rays(const float* vS, const uint32_t* iS, const uint32_t* ftS, const uint32_t* tcS, int32_t nsrc,
+                const Node* nodes, const LeafTri* leaves, uint64_t ntri, uint32_t height, int max_targets,
+                int32_t* ids, uint32_t* hits, uint32_t* count, uint32_t* overflow, uint32_t* iters,
+                unsigned long long* stats, hipStream_t s);
 void select_pairs(const uint64_t* pairs, uint64_t np, const uint32_t* found, const uint32_t* inside,
                   const uint8_t* meta, int include_overlap, uint64_t* out_keys, uint32_t* out_type,
                   unsigned long long* count, hipStream_t s);

@@ -357,3 +357,34 @@ def test_candidate_count_bruteforce():
         return int(m.sum())
 
     assert orc.count_candidate_tests(a, b) == count(a, b) + count(b, a)
+
+
+def test_ray_target_hits_closed_form():
+    # +Z ray from a point p against an axis-aligned cube [lo, hi] whose footprint
+    # holds p strictly: one crossing per horizontal face above p, i.e.
+    # (p.z < lo.z) + (p.z < hi.z); nothing when p's footprint is outside.  Target
+    # faces sit on multiples of 0.5 and source corners never do, so no ray meets
+    # a vertical face or a footprint boundary.
+    rng = random.Random(21)
+    tc = [np.array([rng.choice([0, 1, 2, 3]) for _ in range(3)], np.float32) for _ in range(25)]
+    te = [rng.choice([1.0, 2.0, 4.0]) for _ in range(25)]
+    sc = [np.array([rng.choice(np.arange(-1.0, 4.5, 0.5)) + 0.25 for _ in range(3)], np.float32) for _ in range(40)]
+    T = cube_mesh(tc, te)
+    S = cube_mesh(sc, [0.1] * len(sc))
+    hits = orc.ray_target_hits(S, T)
+    assert hits.shape == (40, 25)
+    first = {}
+    for t, o in enumerate(S.triangle_to_object.tolist()):
+        first.setdefault(o, t)
+    n_nonzero = 0
+    for s in range(40):
+        p = S.vertices[S.indices[first[s]][0]].astype(np.float64)
+        for t in range(25):
+            lo, hi = tc[t] - te[t] / 2, tc[t] + te[t] / 2
+            inside_xy = lo[0] < p[0] < hi[0] and lo[1] < p[1] < hi[1]
+            exp = int(inside_xy) * (int(p[2] < lo[2]) + int(p[2] < hi[2]))
+            assert hits[s, t] == exp, (s, t)
+            n_nonzero += exp > 0
+            # odd count <=> the point is inside (the oracle's own containment test)
+            assert (hits[s, t] % 2 == 1) == orc.point_in_object(T, t, p.astype(np.float32)), (s, t)
+    assert n_nonzero > 20
