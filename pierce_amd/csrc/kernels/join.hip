@@ -908,13 +908,14 @@ __global__ __launch_bounds__(kDualWaves * PZ_WAVE, 2) void k_narrow_dual(
                     }
                 }
                 if (__ballot(hit) != 0) break;
-                // append children pairs: wave-wide exclusive prefix of nout_l
-                int incl = nout_l;
-                for (int sft = 1; sft < PZ_WAVE; sft <<= 1) {
-                    int u = __shfl_up(incl, sft, PZ_WAVE);
-                    if (lane >= sft) incl += u;
-                }
-                const int total = __shfl(incl, PZ_WAVE - 1, PZ_WAVE);
+                // append children pairs: nout_l is 0..4, so its wave-wide
+                // prefix is the bit-weighted count of lower lanes in three
+                // ballots (no cross-lane data movement)
+                const uint64_t b0 = __ballot(nout_l & 1), b1 = __ballot(nout_l & 2), b2 = __ballot(nout_l & 4);
+                const uint64_t below = (1ull << lane) - 1ull;
+                const int excl = __popcll(b0 & below) + 2 * __popcll(b1 & below) + 4 * __popcll(b2 & below);
+                const int incl = excl + nout_l;
+                const int total = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2);
                 if (top - bot + total > QLEN) {
                     // hand the oldest half (the shallowest subtrees) to the next round
                     const int k = (top - bot) / 2;
