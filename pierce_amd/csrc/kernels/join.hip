@@ -851,9 +851,22 @@ __global__ __launch_bounds__(kDualWaves * PZ_WAVE, 2) void k_narrow_dual(
                 int nout_l = 0;
                 if (act) {
                     const int x = e.x, y = e.y;
+                    // Both records in one batch of loads, whatever their kind, so
+                    // a step that mixes node and leaf pairs waits for memory once:
+                    // a node's boxes are its first 48 bytes, a leaf's corners its
+                    // 48 bytes, and a node's child links the next 16.
+                    const float4* ra = x >= 0 ? reinterpret_cast<const float4*>(A.nodes + x)
+                                              : reinterpret_cast<const float4*>(A.leaves + ~x);
+                    const float4* rb = y >= 0 ? reinterpret_cast<const float4*>(B.nodes + y)
+                                              : reinterpret_cast<const float4*>(B.leaves + ~y);
+                    const float4 a0 = ra[0], a1 = ra[1], a2 = ra[2], b0 = rb[0], b1 = rb[1], b2 = rb[2];
+                    int4 ac = make_int4(0, 0, 0, 0), bc = ac;
+                    if (x >= 0) ac = reinterpret_cast<const int4*>(ra)[3];
+                    if (y >= 0) bc = reinterpret_cast<const int4*>(rb)[3];
                     if (x < 0 && y < 0) {
+                        const LeafTri LA{a0, a1, a2}, LB{b0, b1, b2};
                         if (F32) {
-                            const int r = tri_pair_f32(A.leaves[~x], B.leaves[~y], dir_mask, st);
+                            const int r = tri_pair_f32(LA, LB, dir_mask, st);
                             hit = r > 0;
                             if (r < 0) {
                                 const unsigned long long slot = atomicAdd(esc_count, 1ull);
@@ -861,50 +874,43 @@ __global__ __launch_bounds__(kDualWaves * PZ_WAVE, 2) void k_narrow_dual(
                                 else esc_full = true;
                             }
                         } else {
-                            hit = tri_pair_test<false>(A.leaves[~x], B.leaves[~y], dir_mask, false, st);
+                            hit = tri_pair_test<false>(LA, LB, dir_mask, false, st);
                         }
-                    } else if (x >= 0 && y >= 0) {
-                        const Node X = A.nodes[x], Y = B.nodes[y];
+                    } else {
+                        // each side offers its two children (a node) or itself (a leaf)
                         float xl[2][3], xh[2][3], yl[2][3], yh[2][3];
-                        child_box(X, 0, xl[0], xh[0]);
-                        child_box(X, 1, xl[1], xh[1]);
-                        child_box(Y, 0, yl[0], yh[0]);
-                        child_box(Y, 1, yl[1], yh[1]);
-                        const int xc[2] = {X.d.x, X.d.y}, yc[2] = {Y.d.x, Y.d.y};
+                        int xc[2], yc[2];
+                        const int nx = x >= 0 ? 2 : 1, ny = y >= 0 ? 2 : 1;
+                        if (x >= 0) {
+                            const Node X{a0, a1, a2, ac};
+                            child_box(X, 0, xl[0], xh[0]);
+                            child_box(X, 1, xl[1], xh[1]);
+                            xc[0] = ac.x;
+                            xc[1] = ac.y;
+                        } else {
+                            leaf_box(LeafTri{a0, a1, a2}, xl[0], xh[0]);
+                            xc[0] = x;
+                        }
+                        if (y >= 0) {
+                            const Node Y{b0, b1, b2, bc};
+                            child_box(Y, 0, yl[0], yh[0]);
+                            child_box(Y, 1, yl[1], yh[1]);
+                            yc[0] = bc.x;
+                            yc[1] = bc.y;
+                        } else {
+                            leaf_box(LeafTri{b0, b1, b2}, yl[0], yh[0]);
+                            yc[0] = y;
+                        }
                         for (int i = 0; i < 2; ++i)
                             for (int j = 0; j < 2; ++j)
-                                if (box_box(xl[i], xh[i], yl[j], yh[j])) {
-                                    int2 v = make_int2(xc[i], yc[j]);
+                                if (i < nx && j < ny && box_box(xl[i], xh[i], yl[j], yh[j])) {
+                                    const int2 v = make_int2(xc[i], yc[j]);
                                     if (nout_l == 0) o0 = v;
                                     else if (nout_l == 1) o1 = v;
                                     else if (nout_l == 2) o2 = v;
                                     else o3 = v;
                                     ++nout_l;
                                 }
-                    } else if (x >= 0) {  // mesh2 side is a leaf
-                        const Node X = A.nodes[x];
-                        float yl[3], yh[3], cl[3], chh[3];
-                        leaf_box(B.leaves[~y], yl, yh);
-                        child_box(X, 0, cl, chh);
-                        if (box_box(cl, chh, yl, yh)) { o0 = make_int2(X.d.x, y); nout_l = 1; }
-                        child_box(X, 1, cl, chh);
-                        if (box_box(cl, chh, yl, yh)) {
-                            if (nout_l == 0) o0 = make_int2(X.d.y, y);
-                            else o1 = make_int2(X.d.y, y);
-                            ++nout_l;
-                        }
-                    } else {  // mesh1 side is a leaf
-                        const Node Y = B.nodes[y];
-                        float xl[3], xh[3], cl[3], chh[3];
-                        leaf_box(A.leaves[~x], xl, xh);
-                        child_box(Y, 0, cl, chh);
-                        if (box_box(cl, chh, xl, xh)) { o0 = make_int2(x, Y.d.x); nout_l = 1; }
-                        child_box(Y, 1, cl, chh);
-                        if (box_box(cl, chh, xl, xh)) {
-                            if (nout_l == 0) o0 = make_int2(x, Y.d.y);
-                            else o1 = make_int2(x, Y.d.y);
-                            ++nout_l;
-                        }
                     }
                 }
                 if (__ballot(hit) != 0) break;
