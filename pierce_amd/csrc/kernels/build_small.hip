@@ -334,13 +334,17 @@ __global__ __launch_bounds__(T) void k_build_small(
             put(k, a, b, c);
         }
         __syncthreads();
-        for (int p = h0 + tid; p < h1; p += T) {
-            const float* st = stage + 9 * (p - h0);
-            LeafTri L;
-            L.v0 = make_float4(st[0], st[1], st[2], __uint_as_float(t0 + (key[p] & 0x7FFu)));
-            L.v1 = make_float4(st[3], st[4], st[5], __int_as_float(o));
-            L.v2 = make_float4(st[6], st[7], st[8], 0.0f);
-            leaves[base + p] = L;
+        // four lanes per record, lane c < 3 writing corner c, so that each
+        // store instruction covers 16 consecutive records
+        {
+            const int c = tid & 3;
+            for (int p = h0 + (tid >> 2); p < h1; p += T / 4) {
+                if (c < 3) {
+                    const float* st = stage + 9 * (p - h0) + 3 * c;
+                    const float w = c == 0 ? __uint_as_float(t0 + (key[p] & 0x7FFu)) : (c == 1 ? __int_as_float(o) : 0.0f);
+                    reinterpret_cast<float4*>(leaves + base + p)[c] = make_float4(st[0], st[1], st[2], w);
+                }
+            }
         }
         __syncthreads();
     }
@@ -434,18 +438,25 @@ __global__ __launch_bounds__(T) void k_build_small(
             if (c < 0) return ~(int)(base + (uint32_t)(~c));
             return (int)(base + (uint32_t)(c == 0 ? rt : c));
         };
-        for (int i = tid; i < n - 1; i += T) {
+        // four lanes per record, lane c writing its 16-byte quarter, so that
+        // each store instruction covers 16 consecutive records
+        const int c = tid & 3;
+        for (int i = tid >> 2; i < n - 1; i += T / 4) {
             const uint32_t v = (uint32_t)bound[i];
             const int c0 = (int16_t)(v & 0xFFFFu), c1 = (int16_t)(v >> 16);
-            float bl[6], br[6];
-            child_box(c0, bl);
-            child_box(c1, br);
-            Node nd;
-            nd.a = make_float4(bl[0], bl[1], bl[2], bl[3]);
-            nd.b = make_float4(bl[4], bl[5], br[0], br[1]);
-            nd.c = make_float4(br[2], br[3], br[4], br[5]);
-            nd.d = make_int4(link(c0), link(c1), 0, 0);
-            nodes[base + (i == rt ? 0 : (i == 0 ? rt : i))] = nd;
+            float4 q;
+            if (c == 3) {
+                q = make_float4(__int_as_float(link(c0)), __int_as_float(link(c1)), 0.0f, 0.0f);
+            } else {
+                // quarter 0: child 0's lo, hi.x; 1: child 0's hi.yz, child 1's lo.xy;
+                // 2: child 1's lo.z, hi
+                float bl[6], br[6];
+                if (c < 2) child_box(c0, bl);
+                if (c > 0) child_box(c1, br);
+                q = c == 0 ? make_float4(bl[0], bl[1], bl[2], bl[3])
+                           : (c == 1 ? make_float4(bl[4], bl[5], br[0], br[1]) : make_float4(br[2], br[3], br[4], br[5]));
+            }
+            reinterpret_cast<float4*>(nodes + base + (i == rt ? 0 : (i == 0 ? rt : i)))[c] = q;
         }
     }
 }
