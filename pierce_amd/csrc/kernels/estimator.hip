@@ -105,122 +105,143 @@ __device__ inline uint32_t wave_max_u32(uint32_t v) {
     return v;
 }
 
-// Each wavefront walks a contiguous chunk of object ids.  Objects of one part
-// are contiguous, so a wave keeps lane-local running sums for the current
-// part and flushes them (wave reduction + one atomic per field) only when the
-// part changes and at the end: a few atomics per wave instead of one per
-// object on the same per-part address.
-constexpr int kFeatChunk = 256;
+// One object per lane.  The objects of one part are contiguous ids, so the
+// lanes of a wave holding one part form one run: a segmented wave scan
+// (combining only lanes of the same part) leaves each run's totals in its
+// last lane, which adds them to the part with one atomic per field.  A part
+// of tens of objects costs a few atomics instead of one per object on one
+// address.  Lanes past the last object carry no part.
+constexpr uint32_t kNoPart = 0xFFFFFFFFu;
+
+template <class T, class Op>
+__device__ inline T seg_scan(T v, uint32_t part, Op op) {
+    const int lane = __lane_id();
+    for (int off = 1; off < 64; off <<= 1) {
+        const T u = __shfl_up(v, off, 64);
+        const uint32_t pu = (uint32_t)__shfl_up((int)part, off, 64);
+        if (lane >= off && pu == part) v = op(v, u);
+    }
+    return v;
+}
+__device__ inline bool seg_last(uint32_t part) {
+    const int lane = __lane_id();
+    const uint32_t next = (uint32_t)__shfl_down((int)part, 1, 64);
+    return part != kNoPart && (lane == 63 || next != part);
+}
+struct MinOp {
+    __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return min(a, b); }
+};
+struct MaxOp {
+    __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return max(a, b); }
+};
+struct SumOp {
+    template <class T>
+    __device__ T operator()(T a, T b) const { return a + b; }
+};
+
+// A wave walks `chunk` objects, 64 per step.  Runs that end inside a step
+// are added to their part at once; the run reaching lane 63 may go on in the
+// next step, so its totals are carried (wave-uniform) and seeded into lane 0
+// of the next step when that step starts with the same part.
+inline int feat_chunk(int32_t nobj) { return nobj > (1 << 16) ? 1024 : 64; }
 inline unsigned feat_grid(int32_t nobj) {
-    const uint64_t waves = ((uint64_t)nobj + kFeatChunk - 1) / kFeatChunk;
+    const uint64_t waves = ((uint64_t)nobj + feat_chunk(nobj) - 1) / feat_chunk(nobj);
     return (unsigned)((waves * 64 + kBlock - 1) / kBlock);
 }
 
 // Box, counts and triangle totals of one set per part.
 __global__ __launch_bounds__(kBlock) void k_feat_bounds(const uint32_t* __restrict__ obox,
                                                         const uint32_t* __restrict__ tri, int32_t nobj,
-                                                        const uint32_t* __restrict__ part_of, int set,
+                                                        const uint32_t* __restrict__ part_of, int set, int chunk,
                                                         PartAcc* __restrict__ acc) {
     const int lane = __lane_id();
-    const int64_t begin = (int64_t)((blockIdx.x * kBlock + threadIdx.x) / 64) * kFeatChunk;
-    const int64_t end = min(begin + (int64_t)kFeatChunk, (int64_t)nobj);
-    uint32_t run_p = 0xFFFFFFFFu;  // wave-uniform part of the running sums
-    uint32_t lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u}, cnt = 0u, tmax = 0u;
-    unsigned long long tsum = 0ull;
-    auto flush = [&]() {  // called by the whole wave
-        if (run_p == 0xFFFFFFFFu) return;
-        uint32_t l[3], h[3];
+    const int64_t begin = (int64_t)((blockIdx.x * kBlock + threadIdx.x) / 64) * chunk;
+    const int64_t end = min(begin + (int64_t)chunk, (int64_t)nobj);
+    uint32_t cp = kNoPart, clo[3], chi[3], cn = 0, ctm = 0;  // carried run (wave-uniform)
+    unsigned long long cts = 0;
+    auto add = [&](uint32_t p, const uint32_t* l, const uint32_t* h, uint32_t n, unsigned long long ts, uint32_t tm) {
+        PartAcc& A = acc[2 * p + set];
         for (int k = 0; k < 3; ++k) {
-            l[k] = wave_min_u32(lo[k]);
-            h[k] = wave_max_u32(hi[k]);
-            lo[k] = 0xFFFFFFFFu;
-            hi[k] = 0u;
+            atomicMin(&A.lo[k], l[k]);
+            atomicMax(&A.hi[k], h[k]);
         }
-        const unsigned long long n = wave_sum_u64(cnt), ts = wave_sum_u64(tsum);
-        const uint32_t tm = wave_max_u32(tmax);
-        if (lane == 0 && n > 0) {
-            PartAcc& A = acc[2 * run_p + set];
-            for (int k = 0; k < 3; ++k) {
-                atomicMin(&A.lo[k], l[k]);
-                atomicMax(&A.hi[k], h[k]);
-            }
-            atomicAdd(&A.n, (uint32_t)n);
-            atomicAdd(&A.tris, ts);
-            atomicMax(&A.max_tris, tm);
-        }
-        cnt = tmax = 0u;
-        tsum = 0ull;
-        run_p = 0xFFFFFFFFu;
+        atomicAdd(&A.n, n);
+        atomicAdd(&A.tris, ts);
+        atomicMax(&A.max_tris, tm);
     };
     for (int64_t base = begin; base < end; base += 64) {
         const int64_t o = base + lane;
-        const bool valid = o < end && tri[o] > 0;
-        const uint32_t p = (valid && part_of) ? part_of[o] : 0u;
-        const uint64_t vm = __ballot(valid);
-        if (vm == 0) continue;
-        const uint32_t p0 = (uint32_t)__shfl((int)p, __ffsll((long long)vm) - 1, 64);
-        const bool uniform = __ballot(valid && p == p0) == vm;
-        if (!uniform || p0 != run_p) {
-            flush();
-            if (uniform) run_p = p0;
+        const uint32_t t = o < end ? tri[o] : 0u;
+        const bool valid = t > 0;
+        // every object keeps its part (objects without triangles add
+        // identities), so a part is one run of lanes
+        const uint32_t p = o < end ? (part_of ? part_of[o] : 0u) : kNoPart;
+        uint32_t lo[3], hi[3], n = valid ? 1u : 0u, tm = t;
+        unsigned long long ts = t;
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = valid ? obox[6ll * o + k] : 0xFFFFFFFFu;
+            hi[k] = valid ? obox[6ll * o + 3 + k] : 0u;
         }
-        if (!valid) continue;
-        const uint32_t t = tri[o];
-        if (uniform) {
-            for (int k = 0; k < 3; ++k) {
-                lo[k] = min(lo[k], obox[6ll * o + k]);
-                hi[k] = max(hi[k], obox[6ll * o + 3 + k]);
+        const uint32_t p0 = (uint32_t)__shfl((int)p, 0, 64);
+        if (cp != kNoPart) {
+            if (cp == p0) {
+                if (lane == 0) {
+                    for (int k = 0; k < 3; ++k) {
+                        lo[k] = min(lo[k], clo[k]);
+                        hi[k] = max(hi[k], chi[k]);
+                    }
+                    n += cn;
+                    ts += cts;
+                    tm = max(tm, ctm);
+                }
+            } else if (lane == 0) {
+                add(cp, clo, chi, cn, cts, ctm);
             }
-            ++cnt;
-            tsum += t;
-            tmax = max(tmax, t);
-        } else {  // a part boundary inside these 64 objects
-            PartAcc& A = acc[2 * p + set];
+            cp = kNoPart;
+        }
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = seg_scan(lo[k], p, MinOp{});
+            hi[k] = seg_scan(hi[k], p, MaxOp{});
+        }
+        n = seg_scan(n, p, SumOp{});
+        ts = seg_scan(ts, p, SumOp{});
+        tm = seg_scan(tm, p, MaxOp{});
+        if (seg_last(p) && lane != 63) add(p, lo, hi, n, ts, tm);
+        cp = (uint32_t)__shfl((int)p, 63, 64);
+        if (cp != kNoPart) {
             for (int k = 0; k < 3; ++k) {
-                atomicMin(&A.lo[k], obox[6ll * o + k]);
-                atomicMax(&A.hi[k], obox[6ll * o + 3 + k]);
+                clo[k] = (uint32_t)__shfl((int)lo[k], 63, 64);
+                chi[k] = (uint32_t)__shfl((int)hi[k], 63, 64);
             }
-            atomicAdd(&A.n, 1u);
-            atomicAdd(&A.tris, (unsigned long long)t);
-            atomicMax(&A.max_tris, t);
+            cn = (uint32_t)__shfl((int)n, 63, 64);
+            cts = __shfl(ts, 63, 64);
+            ctm = (uint32_t)__shfl((int)tm, 63, 64);
         }
     }
-    flush();
+    if (cp != kNoPart && lane == 0) add(cp, clo, chi, cn, cts, ctm);
 }
 
 // Per-cell counts and extents (cell of the box centre), per-part extent sums
 // and objects meeting the other set's box.
 __global__ __launch_bounds__(kBlock) void k_feat_cells(const uint32_t* __restrict__ obox,
                                                        const uint32_t* __restrict__ tri, int32_t nobj,
-                                                       const uint32_t* __restrict__ part_of, int set,
+                                                       const uint32_t* __restrict__ part_of, int set, int chunk,
                                                        PartAcc* __restrict__ acc, CellAcc* __restrict__ cells) {
     const int lane = __lane_id();
-    const int64_t begin = (int64_t)((blockIdx.x * kBlock + threadIdx.x) / 64) * kFeatChunk;
-    const int64_t end = min(begin + (int64_t)kFeatChunk, (int64_t)nobj);
-    uint32_t run_p = 0xFFFFFFFFu;
-    unsigned long long rq[3] = {0ull, 0ull, 0ull}, rin = 0ull;
-    auto flush = [&]() {
-        if (run_p == 0xFFFFFFFFu) return;
-        unsigned long long sq[3];
-        for (int k = 0; k < 3; ++k) {
-            sq[k] = wave_sum_u64(rq[k]);
-            rq[k] = 0ull;
-        }
-        const unsigned long long r = wave_sum_u64(rin);
-        rin = 0ull;
-        if (lane == 0) {
-            PartAcc& A = acc[2 * run_p + set];
-            for (int k = 0; k < 3; ++k) atomicAdd(&A.ext[k], sq[k]);
-            atomicAdd(&A.n_in_r, r);
-        }
-        run_p = 0xFFFFFFFFu;
+    const int64_t begin = (int64_t)((blockIdx.x * kBlock + threadIdx.x) / 64) * chunk;
+    const int64_t end = min(begin + (int64_t)chunk, (int64_t)nobj);
+    uint32_t cp = kNoPart;  // carried run (wave-uniform)
+    unsigned long long cq[3] = {0ull, 0ull, 0ull}, cr = 0ull;
+    auto add = [&](uint32_t p, const unsigned long long* q, unsigned long long r) {
+        PartAcc& A = acc[2 * p + set];
+        for (int k = 0; k < 3; ++k) atomicAdd(&A.ext[k], q[k]);
+        atomicAdd(&A.n_in_r, r);
     };
     for (int64_t base = begin; base < end; base += 64) {
         const int64_t o = base + lane;
         bool valid = o < end && tri[o] > 0;
-        const uint32_t p = (valid && part_of) ? part_of[o] : 0u;
-        unsigned long long q[3] = {0ull, 0ull, 0ull};
-        uint32_t in_r = 0;
+        const uint32_t p = o < end ? (part_of ? part_of[o] : 0u) : kNoPart;
+        unsigned long long q[3] = {0ull, 0ull, 0ull}, r = 0ull;
         if (valid) {
             const PartAcc& A = acc[2 * p];
             const PartAcc& B = acc[2 * p + 1];
@@ -238,32 +259,35 @@ __global__ __launch_bounds__(kBlock) void k_feat_cells(const uint32_t* __restric
                     q[k] = (unsigned long long)(e * kFix + 0.5f);
                     if (lo > f.rhi[k] || hi < f.rlo[k]) meets = false;
                 }
-                in_r = meets ? 1u : 0u;
+                r = meets ? 1ull : 0ull;
                 CellAcc& C =
                     cells[(2ull * p + set) * kFeatCells + (cell[2] * kFeatGrid + cell[1]) * kFeatGrid + cell[0]];
                 atomicAdd(&C.cnt, 1u);
                 for (int k = 0; k < 3; ++k) atomicAdd(&C.ext[k], q[k]);
             }
         }
-        const uint64_t vm = __ballot(valid);
-        if (vm == 0) continue;
-        const uint32_t p0 = (uint32_t)__shfl((int)p, __ffsll((long long)vm) - 1, 64);
-        const bool uniform = __ballot(valid && p == p0) == vm;
-        if (!uniform || p0 != run_p) {
-            flush();
-            if (uniform) run_p = p0;
+        const uint32_t p0 = (uint32_t)__shfl((int)p, 0, 64);
+        if (cp != kNoPart) {
+            if (cp == p0) {
+                if (lane == 0) {
+                    for (int k = 0; k < 3; ++k) q[k] += cq[k];
+                    r += cr;
+                }
+            } else if (lane == 0) {
+                add(cp, cq, cr);
+            }
+            cp = kNoPart;
         }
-        if (!valid) continue;
-        if (uniform) {
-            for (int k = 0; k < 3; ++k) rq[k] += q[k];
-            rin += in_r;
-        } else {
-            PartAcc& A = acc[2 * p + set];
-            for (int k = 0; k < 3; ++k) atomicAdd(&A.ext[k], q[k]);
-            atomicAdd(&A.n_in_r, (unsigned long long)in_r);
+        for (int k = 0; k < 3; ++k) q[k] = seg_scan(q[k], p, SumOp{});
+        r = seg_scan(r, p, SumOp{});
+        if (seg_last(p) && lane != 63) add(p, q, r);
+        cp = (uint32_t)__shfl((int)p, 63, 64);
+        if (cp != kNoPart) {
+            for (int k = 0; k < 3; ++k) cq[k] = __shfl(q[k], 63, 64);
+            cr = __shfl(r, 63, 64);
         }
     }
-    flush();
+    if (cp != kNoPart && lane == 0) add(cp, cq, cr);
 }
 
 // Probability that two intervals of half-width-sum d (in cell units) whose
@@ -620,16 +644,16 @@ void join_features(const uint32_t* oboxA, const uint32_t* triA, int32_t nA, cons
                        reinterpret_cast<unsigned long long*>(sc.cells), zero_n);
     if (nA > 0)
         hipLaunchKernelGGL(k_feat_bounds, dim3(feat_grid(nA)), dim3(kBlock), 0, s, oboxA, triA, nA, partA, 0,
-                           sc.acc);
+                           feat_chunk(nA), sc.acc);
     if (nB > 0)
         hipLaunchKernelGGL(k_feat_bounds, dim3(feat_grid(nB)), dim3(kBlock), 0, s, oboxB, triB, nB, partB, 1,
-                           sc.acc);
+                           feat_chunk(nB), sc.acc);
     if (nA > 0)
         hipLaunchKernelGGL(k_feat_cells, dim3(feat_grid(nA)), dim3(kBlock), 0, s, oboxA, triA, nA, partA, 0,
-                           sc.acc, sc.cells);
+                           feat_chunk(nA), sc.acc, sc.cells);
     if (nB > 0)
         hipLaunchKernelGGL(k_feat_cells, dim3(feat_grid(nB)), dim3(kBlock), 0, s, oboxB, triB, nB, partB, 1,
-                           sc.acc, sc.cells);
+                           feat_chunk(nB), sc.acc, sc.cells);
     hipLaunchKernelGGL(k_feat_hist, dim3(grid1((uint64_t)nparts * kFeatCells)), dim3(kBlock), 0, s, sc.cells, nparts,
                        sc.hist);
     hipLaunchKernelGGL(k_feat_finalize, dim3(grid1(nparts)), dim3(kBlock), 0, s, sc.acc, sc.hist, nparts,

@@ -53,6 +53,10 @@ def test_join_features_match_reference(ctx):
 def test_batch_features_match_reference(ctx):
     parts = [_sets(20 + 2 * q, nA=10 + 17 * q, nB=5 + 23 * q, shift=(3.0 * q, 0.0, -2.0)) for q in range(6)]
     parts.append((parts[0][0], translate(parts[0][1], (1e4, 0, 0))))  # disjoint query
+    # objects without triangles inside a query (ids 2 and 5 of mesh1 empty)
+    A, B = parts[3]
+    keep = (A.triangle_to_object != 2) & (A.triangle_to_object != 5)
+    parts.append((pz.extract_edges(pz.Mesh(A.vertices, A.indices[keep], A.triangle_to_object[keep])), B))
     preds = [q % 3 for q in range(len(parts))]
     da = ctx.upload_batch([p[0] for p in parts])
     db = ctx.upload_batch([p[1] for p in parts])
@@ -62,7 +66,7 @@ def test_batch_features_match_reference(ctx):
         bB, tB = er.object_boxes(B)
         want = er.features(bA, tA, bB, tB, [preds[q]])[0]
         np.testing.assert_allclose(got[q], want, rtol=2e-6, atol=2e-6, err_msg=f"query {q}")
-    assert got[-1][11] == 0 and got[-1][12] == 0
+    assert got[6][11] == 0 and got[6][12] == 0  # the disjoint query
 
 
 @pytest.mark.parametrize("n", [1, 16, 37, 300])
