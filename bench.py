@@ -62,7 +62,7 @@ def alg_bytes(kernel, k):
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
 # workload, scaled by a 1 GiB copy (fetch x 2.0, write x 1.0); kernel -> bytes.
 MEASURED_TRAFFIC = {
-    "build_small": 1.715e9 + 2.889e9,
+    "build_small": 1.709e9 + 2.877e9,
 }
 
 
