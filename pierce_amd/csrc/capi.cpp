@@ -502,9 +502,9 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
             uint32_t* cnt = buf<uint32_t>(c, tag + ".ncnt", nb_tri + 1);
             uint32_t* rng = buf<uint32_t>(c, tag + ".trange", nb_tri + 1);
             pzk::lbvh_karras(sk, nb_tri, N.seg_off, 30, acc.nodes, pint, pleaf, cnt, rng, s);
-            pzk::lbvh_refit_tris(sk, order, nb_tri, N.seg_off, 30, D->v, D->idx, acc.nodes, pint, pleaf, cnt, rng,
-                                 acc.height, s);
             pzk::gather_leaf_tris(order, nb_tri, D->v, D->idx, D->t2o, acc.leaves, s);
+            pzk::lbvh_refit_leaves(sk, nb_tri, N.seg_off, 30, acc.leaves, acc.nodes, pint, pleaf, cnt, rng,
+                                   acc.height, s);
         }
         static const char* const kTimer[pzk::kSmallClasses] = {"build_small.c0", "build_small.c1",
                                                                 "build_small.c2"};
@@ -708,9 +708,11 @@ GlobalBvh build_global(pz_ctx* c, const pz_dataset* D, const SetPrep& P, const s
     pzk::fill_u32(h, 0u, 1, s);
     uint32_t* rng = buf<uint32_t>(c, tag + ".grange", n + 1);
     pzk::lbvh_karras(sk, n, nullptr, 0, g.nodes, pint, pleaf, cnt, rng, s);
-    pzk::lbvh_refit_tris(sk, order, n, nullptr, 0, D->v, D->idx, g.nodes, pint, pleaf, cnt, rng, h, s);
+    // leaf records first: the refit then reads its boxes from them in sorted
+    // order instead of gathering every triangle a second time
     g.leaves = buf<pzd::LeafTri>(c, tag + ".gleaves", n + 1);
     pzk::gather_leaf_tris(order, n, D->v, D->idx, D->t2o, g.leaves, s);
+    pzk::lbvh_refit_leaves(sk, n, nullptr, 0, g.leaves, g.nodes, pint, pleaf, cnt, rng, h, s);
     g.height = readback<uint32_t>(c, {h})[0];
     if (pzk::stack_for_height(g.height) == 0)
         throw PzError(PZ_E_LIMIT, "set-wide BVH height " + std::to_string(g.height) + " exceeds the traversal stack (63)");

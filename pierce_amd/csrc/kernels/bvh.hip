@@ -207,19 +207,16 @@ __device__ void refit_blocked(const uint64_t* __restrict__ keys, uint64_t n, con
     }
 }
 
-struct TriBoxFn {
-    const float* verts;
-    const uint32_t* idx;
-    const uint32_t* order;
+struct LeafBoxFn {  // leaf records already in sorted order: coalesced reads
+    const pzd::LeafTri* leaves;
     __device__ void operator()(uint64_t p, float* lo, float* hi) const {
-        uint32_t t = order[p];
-        const float* v0 = verts + 3ull * idx[3ull * t];
-        const float* v1 = verts + 3ull * idx[3ull * t + 1];
-        const float* v2 = verts + 3ull * idx[3ull * t + 2];
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = fminf(v0[a], fminf(v1[a], v2[a]));
-            hi[a] = fmaxf(v0[a], fmaxf(v1[a], v2[a]));
-        }
+        const pzd::LeafTri L = leaves[p];
+        lo[0] = fminf(L.v0.x, fminf(L.v1.x, L.v2.x));
+        lo[1] = fminf(L.v0.y, fminf(L.v1.y, L.v2.y));
+        lo[2] = fminf(L.v0.z, fminf(L.v1.z, L.v2.z));
+        hi[0] = fmaxf(L.v0.x, fmaxf(L.v1.x, L.v2.x));
+        hi[1] = fmaxf(L.v0.y, fmaxf(L.v1.y, L.v2.y));
+        hi[2] = fmaxf(L.v0.z, fmaxf(L.v1.z, L.v2.z));
     }
 };
 struct OrdBoxFn {  // boxes stored as ordered uints: 6 per primitive
@@ -235,12 +232,12 @@ struct OrdBoxFn {  // boxes stored as ordered uints: 6 per primitive
 };
 
 template <int BLK>
-__global__ __launch_bounds__(kRefitThreads) void k_refit_tris(const uint64_t* keys, uint64_t n,
-                                                               const uint32_t* seg_off, int seg_shift, Node* nodes,
-                                                               const uint32_t* parent_int,
-                                                               const uint32_t* parent_leaf, uint32_t* counters,
-                                                               const uint32_t* range_end, uint32_t* max_depth,
-                                                               TriBoxFn fn) {
+__global__ __launch_bounds__(kRefitThreads) void k_refit_leaves(const uint64_t* keys, uint64_t n,
+                                                                 const uint32_t* seg_off, int seg_shift, Node* nodes,
+                                                                 const uint32_t* parent_int,
+                                                                 const uint32_t* parent_leaf, uint32_t* counters,
+                                                                 const uint32_t* range_end, uint32_t* max_depth,
+                                                                 LeafBoxFn fn) {
     refit_blocked<BLK>(keys, n, seg_off, seg_shift, nodes, parent_int, parent_leaf, counters, range_end, max_depth,
                        fn);
 }
@@ -286,21 +283,21 @@ void lbvh_karras(const uint64_t* sorted_keys, uint64_t n, const uint32_t* seg_of
                        parent_leaf, counters, range_end);
 }
 
-void lbvh_refit_tris(const uint64_t* sorted_keys, const uint32_t* order, uint64_t n, const uint32_t* seg_off,
-                     int seg_shift, const float* verts, const uint32_t* idx, pzd::Node* nodes,
-                     const uint32_t* parent_int, const uint32_t* parent_leaf, uint32_t* counters,
-                     const uint32_t* range_end, uint32_t* max_depth, hipStream_t s) {
+void lbvh_refit_leaves(const uint64_t* sorted_keys, uint64_t n, const uint32_t* seg_off, int seg_shift,
+                       const pzd::LeafTri* leaves, pzd::Node* nodes, const uint32_t* parent_int,
+                       const uint32_t* parent_leaf, uint32_t* counters, const uint32_t* range_end,
+                       uint32_t* max_depth, hipStream_t s) {
     if (n == 0) return;
-    TriBoxFn fn{verts, idx, order};
+    LeafBoxFn fn{leaves};
     KernelTimer kt_("refit", s);
     if (refit_block_for(n) == kRefitBlock)
-        hipLaunchKernelGGL(k_refit_tris<kRefitBlock>, dim3(grid1(n, kRefitBlock)), dim3(kRefitThreads), 0, s,
+        hipLaunchKernelGGL(k_refit_leaves<kRefitBlock>, dim3(grid1(n, kRefitBlock)), dim3(kRefitThreads), 0, s,
                            sorted_keys, n, seg_off, seg_shift, nodes, parent_int, parent_leaf, counters, range_end,
                            max_depth, fn);
     else
-        hipLaunchKernelGGL(k_refit_tris<kRefitBlockSmall>, dim3(grid1(n, kRefitBlockSmall)), dim3(kRefitThreads), 0,
-                           s, sorted_keys, n, seg_off, seg_shift, nodes, parent_int, parent_leaf, counters, range_end,
-                           max_depth, fn);
+        hipLaunchKernelGGL(k_refit_leaves<kRefitBlockSmall>, dim3(grid1(n, kRefitBlockSmall)), dim3(kRefitThreads),
+                           0, s, sorted_keys, n, seg_off, seg_shift, nodes, parent_int, parent_leaf, counters,
+                           range_end, max_depth, fn);
 }
 
 void lbvh_refit_ord_boxes(const uint64_t* sorted_keys, const uint32_t* order, uint64_t n, const uint32_t* seg_off,

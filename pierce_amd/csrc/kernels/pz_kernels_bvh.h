@@ -11,10 +11,12 @@ namespace pzk {
 void lbvh_karras(const uint64_t* sorted_keys, uint64_t n, const uint32_t* seg_off, int seg_shift, pzd::Node* nodes,
                  uint32_t* parent_int, uint32_t* parent_leaf, uint32_t* counters, uint32_t* range_end,
                  hipStream_t s);
-void lbvh_refit_tris(const uint64_t* sorted_keys, const uint32_t* order, uint64_t n, const uint32_t* seg_off,
-                     int seg_shift, const float* verts, const uint32_t* idx, pzd::Node* nodes,
-                     const uint32_t* parent_int, const uint32_t* parent_leaf, uint32_t* counters,
-                     const uint32_t* range_end, uint32_t* max_depth, hipStream_t s);
+// Boxes bottom-up from the leaf records in sorted order (gather_leaf_tris
+// runs first); max_depth gets the largest segment height.
+void lbvh_refit_leaves(const uint64_t* sorted_keys, uint64_t n, const uint32_t* seg_off, int seg_shift,
+                       const pzd::LeafTri* leaves, pzd::Node* nodes, const uint32_t* parent_int,
+                       const uint32_t* parent_leaf, uint32_t* counters, const uint32_t* range_end,
+                       uint32_t* max_depth, hipStream_t s);
 void lbvh_refit_ord_boxes(const uint64_t* sorted_keys, const uint32_t* order, uint64_t n, const uint32_t* seg_off,
                           int seg_shift, const uint32_t* obox, pzd::Node* nodes, const uint32_t* parent_int,
                           const uint32_t* parent_leaf, uint32_t* counters, const uint32_t* range_end,
