@@ -143,7 +143,7 @@ struct SumOp {
 // are added to their part at once; the run reaching lane 63 may go on in the
 // next step, so its totals are carried (wave-uniform) and seeded into lane 0
 // of the next step when that step starts with the same part.
-inline int feat_chunk(int32_t nobj) { return nobj > (1 << 16) ? 1024 : 64; }
+inline int feat_chunk(int32_t nobj) { return nobj > (1 << 16) ? 256 : 64; }
 inline unsigned feat_grid(int32_t nobj) {
     const uint64_t waves = ((uint64_t)nobj + feat_chunk(nobj) - 1) / feat_chunk(nobj);
     return (unsigned)((waves * 64 + kBlock - 1) / kBlock);
