@@ -153,3 +153,36 @@ def test_empty_result(ctx):
     da, db = ctx.upload(a), ctx.upload(b)
     for pred in (pz.PRED_OVERLAP, pz.PRED_INTERSECTION, pz.PRED_CONTAINMENT):
         assert len(pz.join(ctx, da, db, predicate=pred).pairs) == 0
+
+
+def _ragged(seed, copies=False):
+    """Ellipsoids where some object ids have no triangles and some objects are
+    single triangles (cut from a neighbouring mesh); with `copies`, scaled
+    copies of the seed-31 set's meshes sit inside them."""
+    kw = dict(copy_fraction=0.4, copy_scale_min=0.3, copy_scale_max=0.8, copy_source_count=40,
+              copy_source_seed=31) if copies else {}
+    m = pz.generate(pz.GEN_ELLIPSOIDS, count=40, seed=seed, level_min=2, level_max=2, box=25.0, **kw)
+    t2o = m.triangle_to_object.copy()
+    keep = (t2o % 7) != 3  # ids 3, 10, 17, ... lose every triangle
+    idx, t2o = m.indices[keep], t2o[keep]
+    # the first triangle of every fifth object becomes an object of its own
+    n = int(m.triangle_to_object.max()) + 1
+    first = {}
+    for i, o in enumerate(t2o.tolist()):
+        first.setdefault(o, i)
+    for k, o in enumerate(sorted(o for o in first if o % 5 == 0)):
+        t2o[first[o]] = n + 2 * k + 1  # odd ids above n: the even ones stay empty
+    return pz.extract_edges(pz.Mesh(m.vertices, idx, t2o.astype(np.int32)))
+
+
+@pytest.mark.parametrize("predicate", [pz.PRED_OVERLAP, pz.PRED_INTERSECTION, pz.PRED_CONTAINMENT])
+def test_ragged_object_ids_and_single_triangles(ctx, predicate):
+    a, b = _ragged(31), _ragged(32, copies=True)
+    exp = orc.face_overlap(a, b) if predicate == pz.PRED_OVERLAP else orc.predicate_join(a, b, predicate)[0]
+    assert len(exp) > 0
+    da, db = ctx.upload(a), ctx.upload(b)
+    for kw in (dict(), dict(broad_mode=1), dict(narrow_mode=1), dict(precision=pz.PREC_FP64)):
+        print(f"[ragged] predicate={predicate} {kw}", flush=True)
+        np.testing.assert_array_equal(pz.join(ctx, da, db, predicate=predicate, **kw).pairs, exp)
+    if predicate == pz.PRED_OVERLAP:
+        np.testing.assert_array_equal(pz.join(ctx, da, db, strategy=pz.STRATEGY_GLOBAL).pairs, exp)
