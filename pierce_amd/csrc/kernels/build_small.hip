@@ -310,6 +310,9 @@ __global__ __launch_bounds__(T) void k_build_small(
         if (q < n) ppos[k >> 1] |= (uint32_t)inv[q] << (16 * (k & 1));
     }
     auto pos_of = [&](int k) { return (int)((ppos[k >> 1] >> (16 * (k & 1))) & 0xFFFFu); };
+    // inv lives in bound, which is reset before the climb: every thread must
+    // hold its positions first (the staging below also reuses the box area)
+    __syncthreads();
     const int half = (n + 1) >> 1;
     for (int r = 0; r < 2; ++r) {
         const int h0 = r * half, h1 = min(n, h0 + half);
