@@ -146,3 +146,27 @@ def test_intersection_tool_tracking_csvs(sets, tmp_path):
         h = (h12 if d == "mesh1_to_mesh2" else h21)[s]
         assert (it, cand, ovf) == (1, int((h > 0).sum()), 0), row
         assert tracked == int((h % 2 == 1).sum()) and total == int(h[h % 2 == 1].sum()) and fin == tfin == tracked
+
+
+def test_tracking_follows_the_query_direction(sets, tmp_path):
+    # only mesh1 -> mesh2 queried: mesh2's sources keep empty tracking (no pair
+    # rows, zero summary rows), as the reference's untouched buffers do
+    a, b = sets
+    pz.write_pre(str(tmp_path / "a.pre"), a)
+    pz.write_pre(str(tmp_path / "b.pre"), b)
+    tool = os.path.join(pz.BIN_DIR, "pierce_intersection")
+    print("[tracking] --query-direction mesh1_to_mesh2", flush=True)
+    r = subprocess.run([tool, "--mesh1", "a.pre", "--mesh2", "b.pre", "--enable-tracking", "--query-direction",
+                        "mesh1_to_mesh2", "--containment-tracking-output", "summary.csv"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    with open(tmp_path / "intersection_pair_hits.csv") as f:
+        rows = list(csv.reader(f))[1:]
+    h12 = orc.ray_target_hits(a, b)
+    assert {(d, int(s), int(t)) for d, s, t, _, _ in rows} == \
+        {("mesh1_to_mesh2", s, t) for s, t in zip(*np.nonzero(h12 % 2))}
+    with open(tmp_path / "summary.csv") as f:
+        summ = list(csv.reader(f))[1:]
+    m2 = [row for row in summ if row[0] == "mesh2_to_mesh1"]
+    assert len(m2) == b.num_objects and all(row[2:] == ["0"] * 7 for row in m2)
+    assert all(row[2] == "1" for row in summ if row[0] == "mesh1_to_mesh2")
