@@ -448,6 +448,28 @@ long long exclusive_scan_gpu(const int* d_input, long long* d_output, int num_el
 float pz_estimate_overlap_selectivity(const pz_grid_cell* cells_a, const pz_grid_cell* cells_b,
                                       int32_t num_cells, float cell_volume, float epsilon, float gamma);
 
+/* Whole-set grid estimate with its intermediates: the values the reference's
+ * verbose estimation block prints (pierce/query/src/applications/pierce_overlap_main.cpp:376-471,
+ * pierce_intersection_main.cpp:213-311, pierce_containment_main.cpp:65-135). */
+typedef struct {
+    int32_t has_grid;          /* 0 when either side has no cells or cell_size <= 0 */
+    int32_t matched_cells;     /* cells of b whose index is also a cell of a */
+    double  raw_pairs;         /* per-cell estimate summed over the matches */
+    float   avg_size[2];       /* touch-weighted mean AvgSizeMean of a, b (0 if untouched) */
+    float   avg_vol_ratio[2];  /* touch-weighted mean VolRatio of a, b (1 if untouched) */
+    float   effective_size[2]; /* avg_size * cbrt(avg_vol_ratio) */
+    float   alpha;             /* max(1, (e_a + e_b)^3 / cell_size^3); 1 without correction */
+    int64_t estimated_pairs;   /* (int64)(raw_pairs / alpha) */
+} pz_grid_estimate;
+
+/* Host-only: needs no device.  cell_size is a's (the reference takes mesh1's). */
+pz_status pz_estimate_grid(const pz_grid_cell* cells_a, uint64_t num_a, const pz_grid_cell* cells_b,
+                           uint64_t num_b, float cell_size, float epsilon, float gamma, int32_t alpha_correction,
+                           pz_grid_estimate* out);
+/* The same on two loaded datasets' grids; this is the estimate PZ_SIZING_ESTIMATE sizes with. */
+pz_status pz_dataset_estimate_grid(const pz_dataset* a, const pz_dataset* b, float epsilon, float gamma,
+                                   int32_t alpha_correction, pz_grid_estimate* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -190,6 +190,56 @@ def ray_target_hits(src, tgt) -> np.ndarray:
     return out
 
 
+def grid_estimate(cells_a, cells_b, cell_size: float, epsilon: float = 0.001, gamma: float = 0.8,
+                  alpha_correction: bool = True) -> dict:
+    """Whole-set selectivity estimate over two sparse grids (structured arrays
+    with ix, iy, iz, touch_count, avg_size_mean, vol_ratio), in float32 as the
+    reference computes it.  Cell matching by index, a's last entry winning:
+    pierce/query/src/applications/pierce_overlap_main.cpp:399-414; per-cell term
+    pierce/query/src/cuda/estimated_overlap.cu:8-41; touch-weighted global means
+    pierce/query/src/applications/pierce_containment_main.cpp:34-63; effective
+    sizes, alpha and the final division pierce_overlap_main.cpp:430-448."""
+    f = np.float32
+    if len(cells_a) == 0 or len(cells_b) == 0 or not cell_size > 0:
+        return dict(has_grid=False, estimated_pairs=0)
+    ma = {}
+    for c in cells_a:
+        ma[(int(c["ix"]), int(c["iy"]), int(c["iz"]))] = c
+    cv = f(cell_size) * f(cell_size) * f(cell_size)
+    raw, matched = 0.0, 0
+    for c in cells_b:
+        a = ma.get((int(c["ix"]), int(c["iy"]), int(c["iz"])))
+        if a is None:
+            continue
+        matched += 1
+        if a["touch_count"] == 0 or c["touch_count"] == 0:
+            continue
+        s = f(a["avg_size_mean"]) + f(c["avg_size_mean"]) + f(epsilon)
+        prob = f(s * s * s / cv)
+        prob = f(prob * f(np.power(np.sqrt(f(a["vol_ratio"]) * f(c["vol_ratio"])), f(gamma))))
+        raw += float(a["touch_count"]) * float(c["touch_count"]) * float(min(prob, f(1.0)))
+
+    def mean(cells, key, empty):
+        t = w = 0
+        for c in cells:
+            if c["touch_count"] > 0:
+                t += float(c[key]) * int(c["touch_count"])
+                w += int(c["touch_count"])
+        return f(t / w) if w else f(empty)
+
+    size = [mean(cells_a, "avg_size_mean", 0.0), mean(cells_b, "avg_size_mean", 0.0)]
+    ratio = [mean(cells_a, "vol_ratio", 1.0), mean(cells_b, "vol_ratio", 1.0)]
+    eff = [f(size[i] * f(np.cbrt(ratio[i]))) for i in range(2)]
+    comb = f(eff[0] + eff[1])
+    alpha = f(1.0)
+    if alpha_correction:
+        alpha = max(f(1.0), f(comb * comb * comb / max(cv, f(1e-9))))
+    raw32 = f(raw)
+    return dict(has_grid=True, matched_cells=matched, raw_pairs=float(raw32), avg_size=tuple(size),
+                avg_vol_ratio=tuple(ratio), effective_size=tuple(eff), alpha=float(alpha),
+                estimated_pairs=int(f(raw32 / alpha)))
+
+
 def count_candidate_tests(m1, m2) -> int:
     v1, v2 = _View(m1), _View(m2)
     return int(lib().or_count_candidate_tests(C.byref(v1.v), C.byref(v2.v)))

@@ -55,6 +55,18 @@ GRID_DTYPE = np.dtype([("ix", "<i4"), ("iy", "<i4"), ("iz", "<i4"), ("center_cou
                        ("touch_count", "<u4"), ("avg_size_mean", "<f4"), ("vol_ratio", "<f4")])
 
 
+class GridEstimate(C.Structure):
+    _fields_ = [("has_grid", C.c_int32), ("matched_cells", C.c_int32), ("raw_pairs", C.c_double),
+                ("avg_size", C.c_float * 2), ("avg_vol_ratio", C.c_float * 2), ("effective_size", C.c_float * 2),
+                ("alpha", C.c_float), ("estimated_pairs", C.c_int64)]
+
+    def as_dict(self) -> dict:
+        return dict(has_grid=bool(self.has_grid), matched_cells=self.matched_cells, raw_pairs=self.raw_pairs,
+                    avg_size=tuple(self.avg_size), avg_vol_ratio=tuple(self.avg_vol_ratio),
+                    effective_size=tuple(self.effective_size), alpha=self.alpha,
+                    estimated_pairs=self.estimated_pairs)
+
+
 class MeshView(C.Structure):
     _fields_ = [("vertices", C.c_void_p), ("num_vertices", C.c_uint64), ("indices", C.c_void_p),
                 ("num_triangles", C.c_uint64), ("triangle_to_object", C.c_void_p), ("edge_starts", C.c_void_p),
@@ -166,6 +178,11 @@ _SIGS = [
     ("exclusive_scan_gpu", C.c_longlong, [C.c_void_p, C.c_void_p, C.c_int]),
     ("pz_estimate_overlap_selectivity", C.c_float,
      [C.POINTER(GridCell), C.POINTER(GridCell), C.c_int32, C.c_float, C.c_float, C.c_float]),
+    ("pz_estimate_grid", C.c_int32,
+     [C.POINTER(GridCell), C.c_uint64, C.POINTER(GridCell), C.c_uint64, C.c_float, C.c_float, C.c_float, C.c_int32,
+      C.POINTER(GridEstimate)]),
+    ("pz_dataset_estimate_grid", C.c_int32,
+     [C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.c_int32, C.POINTER(GridEstimate)]),
     ("pz_tune_get", C.c_int32, [C.c_void_p, C.POINTER(Tune)]),
     ("pz_tune_set", C.c_int32, [C.c_void_p, C.POINTER(Tune)]),
     ("pz_autotune", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(JoinDesc), C.POINTER(TuneReport)]),
@@ -381,6 +398,19 @@ def estimate_overlap_selectivity(cells_a: np.ndarray, cells_b: np.ndarray, cell_
         cell_volume, epsilon, gamma))
 
 
+def estimate_grid(cells_a: np.ndarray, cells_b: np.ndarray, cell_size: float, epsilon: float = 0.001,
+                  gamma: float = 0.8, alpha_correction: bool = True) -> dict:
+    """Whole-set grid estimate and the intermediates the reference's verbose
+    block prints (pierce/query/src/applications/pierce_overlap_main.cpp:376-471)."""
+    a = np.ascontiguousarray(cells_a, GRID_DTYPE)
+    b = np.ascontiguousarray(cells_b, GRID_DTYPE)
+    out = GridEstimate()
+    _check(lib().pz_estimate_grid(a.ctypes.data_as(C.POINTER(GridCell)), len(a),
+                                  b.ctypes.data_as(C.POINTER(GridCell)), len(b), cell_size, epsilon, gamma,
+                                  1 if alpha_correction else 0, C.byref(out)))
+    return out.as_dict()
+
+
 # ---------------------------------------------------------------- device
 def device_count() -> int:
     return int(lib().pz_device_count())
@@ -439,6 +469,14 @@ class Dataset:
         out = (C.c_uint64 * 5)()
         _check(lib().pz_dataset_info(self._h, out))
         return dict(vertices=out[0], triangles=out[1], edges=out[2], objects=out[3], grid_cells=out[4])
+
+    def estimate_grid(self, other: "Dataset", epsilon: float = 0.001, gamma: float = 0.8,
+                      alpha_correction: bool = True) -> dict:
+        """estimate_grid over this dataset's grid (a) and other's (b)."""
+        out = GridEstimate()
+        _check(lib().pz_dataset_estimate_grid(self._h, other._h, epsilon, gamma, 1 if alpha_correction else 0,
+                                              C.byref(out)))
+        return out.as_dict()
 
     def close(self):
         if self._h:

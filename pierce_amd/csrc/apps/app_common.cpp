@@ -112,6 +112,7 @@ std::vector<std::pair<std::string, std::string>> common_help(bool no_export) {
         {"--precision <fp32|fp64>", "fp32-filtered exact tests or fp64-filtered exact tests (default: fp32)"},
         {"--sizing <two-pass|estimate|learned|fixed>", "Output sizing (default depends on the tool)"},
         {"--model <path>", "Selectivity model weights for --sizing learned"},
+        {"--verbose-estimate", "Print the estimate's matched cells, raw pairs, sizes and alpha"},
     };
     if (no_export) o.push_back({"--no-export", "Do not export results to CSV"});
     o.push_back({"--help, -h", "Show this help message"});
@@ -184,6 +185,36 @@ bool write_pairs_csv(const std::string& path, const char* header, const pz_join_
         std::fprintf(f, "%d,%d\n", r.pairs[i].object_id_mesh1, r.pairs[i].object_id_mesh2);
     std::fclose(f);
     return true;
+}
+
+void print_estimate(const char* title, const char* rule, const pz_dataset* a, const pz_dataset* b, float epsilon,
+                    float gamma, bool alpha_correction, bool full, bool alpha_line) {
+    pz_grid_estimate e;
+    if (pz_dataset_estimate_grid(a, b, epsilon, gamma, alpha_correction ? 1 : 0, &e) != PZ_OK) {
+        std::cerr << "Error: estimate: " << pz_last_error() << std::endl;
+        return;
+    }
+    if (!e.has_grid) {
+        std::cout << "Skipping estimation: Grid data not found in one or both datasets." << std::endl;
+        std::cout << "Run pierce_preprocess with --generate-grid to enable estimation." << std::endl;
+        return;
+    }
+    std::cout << "\n" << title << std::endl;
+    std::cout << "Matched Sparse Cells:      " << e.matched_cells << std::endl;
+    std::cout << "Raw Potential Pairs:       " << (long long)e.raw_pairs << std::endl;
+    if (full) {
+        std::cout << "Avg Object Size (Mesh1):   " << e.avg_size[0] << std::endl;
+        std::cout << "Avg Object Size (Mesh2):   " << e.avg_size[1] << std::endl;
+        std::cout << "Avg VolRatio (Mesh1):      " << e.avg_vol_ratio[0] << std::endl;
+        std::cout << "Avg VolRatio (Mesh2):      " << e.avg_vol_ratio[1] << std::endl;
+        std::cout << "Effective Size (Mesh1):    " << e.effective_size[0] << std::endl;
+        std::cout << "Effective Size (Mesh2):    " << e.effective_size[1] << std::endl;
+        if (alpha_line)
+            std::cout << "Alpha Correction Enabled:  " << (alpha_correction ? "yes" : "no") << std::endl;
+        std::cout << "Replication Factor (alpha):" << e.alpha << std::endl;
+    }
+    std::cout << "Final Estimated Pairs:     " << e.estimated_pairs << std::endl;
+    std::cout << rule << "\n" << std::endl;
 }
 
 int fail_join(const char* what) {

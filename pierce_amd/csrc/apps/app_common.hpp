@@ -49,6 +49,7 @@ struct Common {
     int precision = PZ_PREC_FP32_FILTERED;
     int sizing_override = -1;  // --sizing
     std::string model_path;
+    bool verbose_estimate = false;  // --verbose-estimate
 };
 
 // Scan argv; `app` handles application flags and returns true when it consumed
@@ -107,6 +108,10 @@ void parse(int argc, char** argv, Common& c, bool allow_no_export, F app) {
             continue;
         }
         if (a == "--model" && val(c.model_path)) continue;
+        if (a == "--verbose-estimate") {
+            c.verbose_estimate = true;
+            continue;
+        }
         app(a, i, argc, argv);
     }
     if (c.runs < 1) c.runs = 1;
@@ -121,6 +126,15 @@ std::vector<std::pair<std::string, std::string>> common_help(bool no_export);
 // Returns nullptr (after printing the error) on failure.
 pz_dataset* load(pz_ctx* ctx, const std::string& path, const char* label, Timer& t, uint64_t* ntri,
                  uint64_t* nobj);
+
+// The reference's verbose selectivity-estimation block (the estimate functions
+// with verbose set: pierce/query/src/applications/pierce_overlap_main.cpp:450-466,
+// pierce_intersection_main.cpp:291-311, pierce_containment_main.cpp:127-133).  The
+// reference's tools pass verbose = false at every call, so this prints only
+// under --verbose-estimate.  `full` selects the overlap/intersection layout
+// (sizes and alpha); without it only cells, raw and final are printed.
+void print_estimate(const char* title, const char* rule, const pz_dataset* a, const pz_dataset* b, float epsilon,
+                    float gamma, bool alpha_correction, bool full, bool alpha_line);
 
 // Library phases into the timer (as manual measurements).
 void add_phases(Timer& t, const pz_join_result& r);

@@ -58,6 +58,9 @@ int main(int argc, char** argv) {
     d.load_factor = load_factor;
     d.sizing = c.sizing_override >= 0 && c.sizing_override != PZ_SIZING_LEARNED ? c.sizing_override
                                                                                  : PZ_SIZING_ESTIMATE;
+    if (c.verbose_estimate)
+        app::print_estimate("=== Containment Selectivity Estimation ===", "==========================================",
+                            d1, d2, epsilon, gamma, true, false, false);
     pz_join_result r;
     timer.next("Warmup");
     for (int w = 0; w < c.warmup; ++w) {

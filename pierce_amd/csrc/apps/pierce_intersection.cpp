@@ -189,6 +189,9 @@ int main(int argc, char** argv) {
     d.load_factor = load_factor;
     d.sizing = c.sizing_override >= 0 && c.sizing_override != PZ_SIZING_LEARNED ? c.sizing_override
                                                                                  : PZ_SIZING_ESTIMATE;
+    if (c.verbose_estimate)
+        app::print_estimate("=== Selectivity Estimation ===", "==============================", d1, d2, epsilon, gamma,
+                            true, true, false);
     pz_join_result r;
     if (estimate_only) {
         d.sizing = PZ_SIZING_ESTIMATE;

@@ -113,6 +113,9 @@ int main(int argc, char** argv) {
     if (c.sizing_override >= 0 && c.sizing_override != PZ_SIZING_LEARNED) d.sizing = c.sizing_override;
     if (c.strategy == PZ_STRATEGY_GLOBAL && d.sizing != PZ_SIZING_TWO_PASS) d.dedup = PZ_DEDUP_HASH;
 
+    if (c.verbose_estimate)
+        app::print_estimate("=== Selectivity Estimation (Overlap - Direct) ===", "==============================", d1,
+                            d2, epsilon, gamma, !no_alpha, true, true);
     pz_join_result r;
     if (estimate_only) {
         d.sizing = PZ_SIZING_ESTIMATE;

@@ -178,45 +178,66 @@ void finish_pairs(pz_ctx* c, uint64_t* keys, uint32_t* types, uint64_t n, int32_
 }
 
 // ------------------------------------------------------------- estimates
-// Grid estimate of pierce/query/src/applications/pierce_overlap_main.cpp:376-471 with the
-// per-cell formula of pierce/query/src/cuda/estimated_overlap.cu:29-40, summed in double.
-double grid_estimate(const pz_dataset* a, const pz_dataset* b, float gamma, float eps, bool alpha_corr) {
-    if (a->grid.empty() || b->grid.empty() || !(a->cell_size > 0)) return -1.0;
+// Whole-set grid estimate of pierce/query/src/applications/pierce_overlap_main.cpp:376-471
+// (the same steps as pierce_intersection_main.cpp:213-311 and
+// pierce_containment_main.cpp:65-135): cells of b are matched to a's by index
+// (a's last entry wins for a repeated index), the per-cell formula of
+// pierce/query/src/cuda/estimated_overlap.cu:8-41 is summed over the matches, and
+// the sum is divided by the replication factor alpha.
+void estimate_grid(const pz_grid_cell* ga, uint64_t na, const pz_grid_cell* gb, uint64_t nb, float cell_size,
+                   float eps, float gamma, bool alpha_corr, pz_grid_estimate* o) {
+    *o = pz_grid_estimate{};
+    o->alpha = 1.0f;
+    o->avg_vol_ratio[0] = o->avg_vol_ratio[1] = 1.0f;
+    if (!na || !nb || !(cell_size > 0)) return;
+    o->has_grid = 1;
     struct K {
         int32_t x, y, z;
-        bool operator<(const K& o) const { return std::tie(x, y, z) < std::tie(o.x, o.y, o.z); }
+        bool operator<(const K& k) const { return std::tie(x, y, z) < std::tie(k.x, k.y, k.z); }
     };
     std::map<K, const pz_grid_cell*> ma;
-    for (const auto& g : a->grid) ma[K{g.ix, g.iy, g.iz}] = &g;
+    for (uint64_t i = 0; i < na; ++i) ma[K{ga[i].ix, ga[i].iy, ga[i].iz}] = &ga[i];
     std::vector<pz_grid_cell> A, B;
-    for (const auto& g : b->grid) {
-        auto it = ma.find(K{g.ix, g.iy, g.iz});
+    for (uint64_t i = 0; i < nb; ++i) {
+        auto it = ma.find(K{gb[i].ix, gb[i].iy, gb[i].iz});
         if (it != ma.end()) {
             A.push_back(*it->second);
-            B.push_back(g);
+            B.push_back(gb[i]);
         }
     }
-    float cs = a->cell_size;
-    float cell_vol = cs * cs * cs;
-    double raw = pz_estimate_overlap_selectivity(A.data(), B.data(), (int32_t)A.size(), cell_vol, eps, gamma);
-    auto avg = [](const std::vector<pz_grid_cell>& g, bool size) {
+    float cell_vol = cell_size * cell_size * cell_size;
+    o->matched_cells = (int32_t)A.size();
+    float raw = A.empty() ? 0.0f
+                          : pz_estimate_overlap_selectivity(A.data(), B.data(), (int32_t)A.size(), cell_vol, eps, gamma);
+    o->raw_pairs = raw;
+    // touch-weighted means over every cell of one side (an empty side: size 0, ratio 1)
+    auto avg = [](const pz_grid_cell* g, uint64_t n, bool size) {
         double t = 0;
-        long long n = 0;
-        for (const auto& c : g)
-            if (c.touch_count > 0) {
-                t += (double)(size ? c.avg_size_mean : c.vol_ratio) * c.touch_count;
-                n += c.touch_count;
+        long long w = 0;
+        for (uint64_t i = 0; i < n; ++i)
+            if (g[i].touch_count > 0) {
+                t += (double)(size ? g[i].avg_size_mean : g[i].vol_ratio) * g[i].touch_count;
+                w += g[i].touch_count;
             }
-        return n ? (float)(t / n) : (size ? 0.0f : 1.0f);
+        return w ? (float)(t / w) : (size ? 0.0f : 1.0f);
     };
-    float e1 = avg(a->grid, true) * std::cbrt(avg(a->grid, false));
-    float e2 = avg(b->grid, true) * std::cbrt(avg(b->grid, false));
-    float comb = e1 + e2;
+    o->avg_size[0] = avg(ga, na, true);
+    o->avg_size[1] = avg(gb, nb, true);
+    o->avg_vol_ratio[0] = avg(ga, na, false);
+    o->avg_vol_ratio[1] = avg(gb, nb, false);
+    for (int s = 0; s < 2; ++s) o->effective_size[s] = o->avg_size[s] * std::cbrt(o->avg_vol_ratio[s]);
+    float comb = o->effective_size[0] + o->effective_size[1];
     float mink = comb * comb * comb;
     if (cell_vol < 1e-9f) cell_vol = 1e-9f;
-    float alpha = 1.0f;
-    if (alpha_corr) alpha = std::max(1.0f, mink / cell_vol);
-    return raw / alpha;
+    if (alpha_corr) o->alpha = std::max(1.0f, mink / cell_vol);
+    o->estimated_pairs = (int64_t)(raw / o->alpha);
+}
+
+double grid_estimate(const pz_dataset* a, const pz_dataset* b, float gamma, float eps, bool alpha_corr) {
+    pz_grid_estimate e;
+    estimate_grid(a->grid.data(), a->grid.size(), b->grid.data(), b->grid.size(), a->cell_size, eps, gamma,
+                  alpha_corr, &e);
+    return e.has_grid ? (double)e.estimated_pairs : -1.0;
 }
 
 // ------------------------------------------------------------- two-level
@@ -1508,6 +1529,23 @@ float pz_estimate_overlap_selectivity(const pz_grid_cell* a, const pz_grid_cell*
         total += (double)a[i].touch_count * (double)b[i].touch_count * prob;
     }
     return (float)total;
+}
+
+pz_status pz_estimate_grid(const pz_grid_cell* cells_a, uint64_t num_a, const pz_grid_cell* cells_b,
+                           uint64_t num_b, float cell_size, float epsilon, float gamma, int32_t alpha_correction,
+                           pz_grid_estimate* out) {
+    if (!out || (num_a && !cells_a) || (num_b && !cells_b)) return fail(PZ_E_INVALID, "null argument");
+    return guarded([&]() -> pz_status {
+        estimate_grid(cells_a, num_a, cells_b, num_b, cell_size, epsilon, gamma, alpha_correction != 0, out);
+        return PZ_OK;
+    });
+}
+
+pz_status pz_dataset_estimate_grid(const pz_dataset* a, const pz_dataset* b, float epsilon, float gamma,
+                                   int32_t alpha_correction, pz_grid_estimate* out) {
+    if (!a || !b || !out) return fail(PZ_E_INVALID, "null argument");
+    return pz_estimate_grid(a->grid.data(), a->grid.size(), b->grid.data(), b->grid.size(), a->cell_size, epsilon,
+                            gamma, alpha_correction, out);
 }
 
 }  // extern "C"

@@ -70,6 +70,38 @@ def test_overlap_tool_estimate_and_pairs_output(pre_files, tmp_path):
     assert "selectivity estimation_1" in phases and "raytrace_hash_mesh1tomesh2_1" in phases
 
 
+def _line(out, label):
+    m = re.findall(r"^%s:\s+(-?\d+)$" % re.escape(label), out, re.M)
+    assert m, (label, out)
+    return [int(x) for x in m]
+
+
+@pytest.mark.parametrize("tool,title,full", [
+    ("pierce_overlap", "=== Selectivity Estimation (Overlap - Direct) ===", True),
+    ("pierce_intersection", "=== Selectivity Estimation ===", True),
+    ("pierce_containment", "=== Containment Selectivity Estimation ===", False),
+])
+def test_verbose_estimate_block(pre_files, tmp_path, tool, title, full):
+    # the estimation block of pierce_overlap_main.cpp:450-466 (and its
+    # intersection / containment forms) against the oracle's whole-set estimate
+    d, a, b = pre_files
+    args = ["--mesh1", str(d / "a.pre"), "--mesh2", str(d / "b.pre"), "--verbose-estimate", "--warmup-runs", "0",
+            "--no-export"]
+    if tool != "pierce_containment":
+        args.append("--estimate-only")
+    out = run(tool, args, tmp_path)
+    exp = orc.grid_estimate(a.grid, b.grid, a.grid_cell_size)
+    assert exp["has_grid"] and exp["matched_cells"] > 0 and exp["estimated_pairs"] > 0
+    assert title in out
+    assert _line(out, "Matched Sparse Cells") == [exp["matched_cells"]]
+    assert abs(_line(out, "Raw Potential Pairs")[0] - int(exp["raw_pairs"])) <= 1
+    final = _line(out, "Final Estimated Pairs")
+    assert abs(final[0] - exp["estimated_pairs"]) <= 1
+    assert all(f == final[0] for f in final)  # the join sizes with the same estimate
+    assert ("Replication Factor (alpha):" in out) == full
+    assert ("Alpha Correction Enabled:" in out) == (tool == "pierce_overlap")
+
+
 def test_intersection_tool(pre_files, tmp_path):
     d, a, b = pre_files
     out = run("pierce_intersection", ["--mesh1", str(d / "a.pre"), "--mesh2", str(d / "b.pre")], tmp_path)

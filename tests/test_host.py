@@ -196,3 +196,34 @@ def test_generate_tool_writes_pre(tmp_path):
     assert r.returncode == 0, r.stderr
     m = pz.read_pre(out)
     assert m.indices.shape[0] == 12 * 80 and m.edge_objects.shape[0] == 12 * 120 and m.grid.shape[0] > 0
+
+
+def test_grid_estimate_closed_form_and_oracle():
+    from oracle import oracle as orc
+    from tests.test_oracle import GRID_A, GRID_B, GRID_CASES
+    for (cs, eps, gamma, alpha), (raw, a, final) in GRID_CASES:
+        got = pz.estimate_grid(GRID_A, GRID_B, cs, epsilon=eps, gamma=gamma, alpha_correction=alpha)
+        assert got["has_grid"] and got["matched_cells"] == 2
+        assert (got["raw_pairs"], got["alpha"], got["estimated_pairs"]) == (raw, a, final)
+        assert got["effective_size"] == (1.0, 1.0)
+    assert not pz.estimate_grid(GRID_A, GRID_B[:0], 4.0)["has_grid"]
+    assert not pz.estimate_grid(GRID_A, GRID_B, 0.0)["has_grid"]
+    # random grids with repeated indices and untouched cells against the oracle
+    rng = np.random.default_rng(5)
+    for trial in range(20):
+        g = []
+        for n in (int(rng.integers(1, 300)), int(rng.integers(1, 300))):
+            c = np.zeros(n, pz.GRID_DTYPE)
+            c["ix"], c["iy"], c["iz"] = (rng.integers(0, 6, n) for _ in range(3))
+            c["touch_count"] = rng.integers(0, 50, n) * (rng.random(n) > 0.1)
+            c["avg_size_mean"] = rng.uniform(0.1, 6.0, n)
+            c["vol_ratio"] = rng.uniform(0.05, 1.0, n)
+            g.append(c)
+        cs = float(rng.choice([0.5, 2.0, 5.0]))
+        got = pz.estimate_grid(g[0], g[1], cs, alpha_correction=trial % 3 != 0)
+        exp = orc.grid_estimate(g[0], g[1], cs, alpha_correction=trial % 3 != 0)
+        assert got["matched_cells"] == exp["matched_cells"]
+        assert abs(got["raw_pairs"] - exp["raw_pairs"]) <= 1e-6 * max(1.0, exp["raw_pairs"])
+        assert got["alpha"] == pytest.approx(exp["alpha"], rel=1e-6)
+        assert got["avg_size"] == pytest.approx(exp["avg_size"], rel=1e-6)
+        assert abs(got["estimated_pairs"] - exp["estimated_pairs"]) <= 1

@@ -388,3 +388,41 @@ def test_ray_target_hits_closed_form():
             # odd count <=> the point is inside (the oracle's own containment test)
             assert (hits[s, t] % 2 == 1) == orc.point_in_object(T, t, p.astype(np.float32)), (s, t)
     assert n_nonzero > 20
+
+
+def _grid(rows):
+    g = np.zeros(len(rows), pz.GRID_DTYPE)
+    for i, (ix, iy, iz, t, s, r) in enumerate(rows):
+        g[i]["ix"], g[i]["iy"], g[i]["iz"] = ix, iy, iz
+        g[i]["touch_count"], g[i]["avg_size_mean"], g[i]["vol_ratio"] = t, s, r
+    return g
+
+
+# Two grids whose whole-set estimate is exact in binary arithmetic.  a repeats
+# cell (0,0,0): the first entry (touch 0) must lose to the second.  Means:
+# size_a = (64*1 + 16*3 + 16*5) / 96 = 2, ratio_a = (64/16 + 16/4 + 16/4) / 96 =
+# 1/8, so effective_a = 2 * cbrt(1/8) = 1; b is all size 1, ratio 1.
+GRID_A = _grid([(0, 0, 0, 0, 100.0, 0.125), (0, 0, 0, 64, 1.0, 0.0625), (1, 0, 0, 16, 3.0, 0.25),
+                (7, 7, 7, 16, 5.0, 0.25)])
+GRID_B = _grid([(0, 0, 0, 8, 1.0, 1.0), (1, 0, 0, 16, 1.0, 1.0), (9, 9, 9, 5, 1.0, 1.0)])
+# (cell size, epsilon, gamma, alpha on) -> (raw, alpha, final), by hand: cell
+# (0,0,0) gives 64*8*min(1, (1+1+eps)^3/cs^3 * (sqrt(1/16))^gamma), cell (1,0,0)
+# 16*16*min(1, (3+1+eps)^3/cs^3 * (sqrt(1/4))^gamma); alpha = max(1, (1+1)^3/cs^3).
+GRID_CASES = [
+    ((4.0, 0.0, 2.0, True), (4.0 + 64.0, 1.0, 68)),
+    ((4.0, 0.0, 1.0, True), (16.0 + 128.0, 1.0, 144)),
+    ((4.0, 2.0, 2.0, True), (32.0 + 216.0, 1.0, 248)),
+    ((1.0, 0.0, 2.0, True), (256.0 + 256.0, 8.0, 64)),
+    ((1.0, 0.0, 2.0, False), (256.0 + 256.0, 1.0, 512)),
+]
+
+
+@pytest.mark.parametrize("args,exp", GRID_CASES)
+def test_grid_estimate_closed_form(args, exp):
+    cs, eps, gamma, alpha = args
+    got = orc.grid_estimate(GRID_A, GRID_B, cs, epsilon=eps, gamma=gamma, alpha_correction=alpha)
+    assert got["has_grid"] and got["matched_cells"] == 2
+    assert got["raw_pairs"] == exp[0] and got["alpha"] == exp[1] and got["estimated_pairs"] == exp[2]
+    assert got["avg_size"] == (2.0, 1.0) and got["avg_vol_ratio"] == (0.125, 1.0)
+    assert got["effective_size"] == (1.0, 1.0)
+    assert not orc.grid_estimate(GRID_A, GRID_B[:0], cs)["has_grid"]
