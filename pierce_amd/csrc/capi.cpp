@@ -518,14 +518,15 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
                                              30 + bits_for(std::max<uint64_t>(T.nbig, 1)), tmp, s);
             uint64_t* sk = alt ? keys2 : keys;
             uint32_t* order = alt ? vals2 : vals;
-            uint32_t* pint = buf<uint32_t>(c, tag + ".pint", nb_tri + 1);
-            uint32_t* pleaf = buf<uint32_t>(c, tag + ".pleaf", nb_tri + 1);
-            uint32_t* cnt = buf<uint32_t>(c, tag + ".ncnt", nb_tri + 1);
-            uint32_t* rng = buf<uint32_t>(c, tag + ".trange", nb_tri + 1);
-            pzk::lbvh_karras(sk, nb_tri, N.seg_off, 30, acc.nodes, pint, pleaf, cnt, rng, s);
-            pzk::gather_leaf_tris(order, nb_tri, D->v, D->idx, D->t2o, acc.leaves, s);
-            pzk::lbvh_refit_leaves(sk, nb_tri, N.seg_off, 30, acc.leaves, acc.nodes, pint, pleaf, cnt, rng,
-                                   acc.height, s);
+            // runs of the sorted order through the LDS builder, then a top tree per object
+            const uint64_t nch = pzk::big_chunk_bound(T.nbig, nb_tri);
+            uint32_t* ccnt = buf<uint32_t>(c, tag + ".chunk_cnt", T.nbig + 1);
+            uint32_t* coff = buf<uint32_t>(c, tag + ".chunk_off", T.nbig + 1);
+            auto* e0 = buf<pzk::ChunkRoot>(c, tag + ".chunk_ent0", nch);
+            auto* e1 = buf<pzk::ChunkRoot>(c, tag + ".chunk_ent1", nch);
+            void* stmp = buf<char>(c, "scan.tmp", pzk::scan_temp_bytes(T.nbig + 1));
+            pzk::build_big_chunked(D->v, D->idx, N.obj_of_loc, N.seg_off, (uint32_t)T.nbig, nb_tri, sk, order, ccnt,
+                                   coff, stmp, e0, e1, acc.nodes, acc.leaves, acc.height, s);
         }
         static const char* const kTimer[pzk::kSmallClasses] = {"build_small.c0", "build_small.c1",
                                                                 "build_small.c2"};

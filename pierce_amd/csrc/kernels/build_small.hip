@@ -116,11 +116,35 @@ __device__ __forceinline__ uint32_t lane_swap(uint32_t v) {
     }
 }
 
-template <int T>
+// Chunk mode of the builder: the workgroup's triangles are a run of the big
+// objects' Morton-sorted order instead of an object's contiguous range.
+struct ChunkSrc {
+    const uint64_t* keys;       // (segment << 30) | 30-bit Morton code, ascending
+    const uint32_t* order;      // triangle id at each sorted position
+    const uint32_t* chunk_off;  // first chunk of each big segment (nbig + 1 entries)
+    uint32_t nbig;
+    pzk::ChunkRoot* ent;        // per chunk: root link and box
+};
+
+constexpr uint32_t kM30 = (1u << 30) - 1u;
+
+// Largest l in [0, n) with off[l] <= g, for off[0] = 0 <= g < off[n].
+__device__ __forceinline__ uint32_t seg_of_chunk(const uint32_t* __restrict__ off, uint32_t n, uint32_t g) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <int T, bool CHUNK>
 __global__ __launch_bounds__(T) void k_build_small(
     const float* __restrict__ verts, const uint32_t* __restrict__ idx, const int32_t* __restrict__ obj_of_loc,
     const uint32_t* __restrict__ first_tri, const uint32_t* __restrict__ tri_cnt, const uint32_t* __restrict__ obox, const uint32_t* __restrict__ seg_off,
-    uint32_t l_begin, int cap, Node* __restrict__ nodes, LeafTri* __restrict__ leaves, uint32_t* __restrict__ height_out) {
+    uint32_t l_begin, int cap, Node* __restrict__ nodes, LeafTri* __restrict__ leaves, uint32_t* __restrict__ height_out,
+    ChunkSrc ch) {
     constexpr int PR = T == 1024 ? 2 : (T == 512 ? 3 : 4);  // local triangles per thread kept in registers
     constexpr int NP = (kMax / T + 1) / 2;                 // sorted positions per thread, two per register
     extern __shared__ __align__(16) unsigned char lds[];
@@ -130,21 +154,46 @@ __global__ __launch_bounds__(T) void k_build_small(
     int* bound = reinterpret_cast<int*>(box + 6 * cap);    // cap: sorted positions, then exchange / child ids
     uint16_t* inv = reinterpret_cast<uint16_t*>(bound);
     float* stage = box;
-    const uint32_t l = l_begin + blockIdx.x;
-    const int32_t o = obj_of_loc[l];
-    const int n = (int)tri_cnt[o];
-    const uint32_t t0 = first_tri[o];
-    const uint32_t base = seg_off[l];
-    const int tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
-    float lo[3], sc[3];
-    for (int k = 0; k < 3; ++k) {
-        const float bl = ord2f(obox[6ll * o + k]), bh = ord2f(obox[6ll * o + 3 + k]);
-        lo[k] = bl;
-        sc[k] = bh > bl ? 127.0f / (bh - bl) : 0.0f;
+    // leaves at lbase + sorted position, nodes at nbase + node index (root at nbase)
+    uint32_t l, t0, lbase, nbase, g = 0, cpl = 0;
+    int32_t o;
+    int n;
+    if constexpr (CHUNK) {
+        g = blockIdx.x;
+        if (g >= ch.chunk_off[ch.nbig]) return;  // the grid is an upper bound on the chunk count
+        l = seg_of_chunk(ch.chunk_off, ch.nbig, g);
+        const uint32_t sb = seg_off[l], m = seg_off[l + 1] - sb;
+        const uint32_t j = g - ch.chunk_off[l], nch = ch.chunk_off[l + 1] - ch.chunk_off[l];
+        t0 = sb + j * (uint32_t)kMax;  // first sorted position of the chunk
+        n = (int)min((uint32_t)kMax, m - j * (uint32_t)kMax);
+        lbase = t0;
+        nbase = nch == 1 ? t0 : t0 + 1;  // slot t0 belongs to the top tree
+        o = obj_of_loc[l];
+        // codes of a sorted run share their leading bits: keep the 21 below them
+        const uint32_t x = (uint32_t)(ch.keys[t0] ^ ch.keys[t0 + n - 1]) & kM30;
+        cpl = x ? (uint32_t)__clz((int)x) - 2u : 30u;
+    } else {
+        l = l_begin + blockIdx.x;
+        o = obj_of_loc[l];
+        n = (int)tri_cnt[o];
+        t0 = first_tri[o];
+        lbase = nbase = seg_off[l];
     }
+    const int tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+    float lo[3] = {0.0f, 0.0f, 0.0f}, sc[3] = {0.0f, 0.0f, 0.0f};
+    if constexpr (!CHUNK)
+        for (int k = 0; k < 3; ++k) {
+            const float bl = ord2f(obox[6ll * o + k]), bh = ord2f(obox[6ll * o + 3 + k]);
+            lo[k] = bl;
+            sc[k] = bh > bl ? 127.0f / (bh - bl) : 0.0f;
+        }
     int npow = 1;
     while (npow < n) npow <<= 1;
     auto morton = [&](const float3& a, const float3& b, const float3& c, int q) {
+        if constexpr (CHUNK) {  // ascending in q: the run is sorted already
+            const uint32_t m = ((((uint32_t)ch.keys[t0 + q] & kM30) << cpl) & kM30) >> 9;
+            return (m << 11) | (uint32_t)q;
+        }
         const float cx[3] = {(a.x + b.x + c.x) * (1.0f / 3.0f), (a.y + b.y + c.y) * (1.0f / 3.0f),
                              (a.z + b.z + c.z) * (1.0f / 3.0f)};
         uint32_t qq[3];
@@ -154,8 +203,9 @@ __global__ __launch_bounds__(T) void k_build_small(
             m = (m << 3) | (((qq[0] >> b2) & 1u) << 2) | (((qq[1] >> b2) & 1u) << 1) | ((qq[2] >> b2) & 1u);
         return (m << 11) | (uint32_t)q;
     };
+    auto tri_id = [&](int q) { return CHUNK ? ch.order[t0 + (uint32_t)q] : t0 + (uint32_t)q; };
     auto tri = [&](int q, float3& a, float3& b, float3& c) {
-        const uint32_t t = t0 + (uint32_t)q;
+        const uint32_t t = tri_id(q);
         const uint32_t i0 = idx[3ull * t], i1 = idx[3ull * t + 1], i2 = idx[3ull * t + 2];
         a = vtx(verts, i0);
         b = vtx(verts, i1);
@@ -283,7 +333,7 @@ __global__ __launch_bounds__(T) void k_build_small(
             if (pb + 192 < n) key[pb + 192] = k3;
         }
     };
-    if (n > 1) {
+    if (!CHUNK && n > 1) {
         reg_pass(2, min(npow, 256));
         for (int s = 512; s <= npow; s <<= 1) {
             __syncthreads();
@@ -344,8 +394,9 @@ __global__ __launch_bounds__(T) void k_build_small(
             for (int p = h0 + (tid >> 2); p < h1; p += T / 4) {
                 if (c < 3) {
                     const float* st = stage + 9 * (p - h0) + 3 * c;
-                    const float w = c == 0 ? __uint_as_float(t0 + (key[p] & 0x7FFu)) : (c == 1 ? __int_as_float(o) : 0.0f);
-                    reinterpret_cast<float4*>(leaves + base + p)[c] = make_float4(st[0], st[1], st[2], w);
+                    const float w = c == 0 ? __uint_as_float(tri_id((int)(key[p] & 0x7FFu)))
+                                           : (c == 1 ? __int_as_float(o) : 0.0f);
+                    reinterpret_cast<float4*>(leaves + lbase + p)[c] = make_float4(st[0], st[1], st[2], w);
                 }
             }
         }
@@ -427,7 +478,7 @@ __global__ __launch_bounds__(T) void k_build_small(
             if (c >= 0) {
                 for (int k = 0; k < 6; ++k) bx[k] = box[6 * c + k];
             } else {
-                const LeafTri& L = leaves[base + (uint32_t)(~c)];
+                const LeafTri& L = leaves[lbase + (uint32_t)(~c)];
                 const float4 v0 = L.v0, v1 = L.v1, v2 = L.v2;
                 bx[0] = fminf(v0.x, fminf(v1.x, v2.x));
                 bx[1] = fminf(v0.y, fminf(v1.y, v2.y));
@@ -438,8 +489,8 @@ __global__ __launch_bounds__(T) void k_build_small(
             }
         };
         auto link = [&](int c) {
-            if (c < 0) return ~(int)(base + (uint32_t)(~c));
-            return (int)(base + (uint32_t)(c == 0 ? rt : c));
+            if (c < 0) return ~(int)(lbase + (uint32_t)(~c));
+            return (int)(nbase + (uint32_t)(c == 0 ? rt : c));
         };
         // four lanes per record, lane c writing its 16-byte quarter, so that
         // each store instruction covers 16 consecutive records
@@ -459,9 +510,83 @@ __global__ __launch_bounds__(T) void k_build_small(
                 q = c == 0 ? make_float4(bl[0], bl[1], bl[2], bl[3])
                            : (c == 1 ? make_float4(bl[4], bl[5], br[0], br[1]) : make_float4(br[2], br[3], br[4], br[5]));
             }
-            reinterpret_cast<float4*>(nodes + base + (i == rt ? 0 : (i == 0 ? rt : i)))[c] = q;
+            reinterpret_cast<float4*>(nodes + nbase + (i == rt ? 0 : (i == 0 ? rt : i)))[c] = q;
         }
     }
+    if constexpr (CHUNK) {  // the chunk's root for the top tree
+        if (tid == 0) {
+            pzk::ChunkRoot e;
+            if (n >= 2) {
+                const int rt = __float_as_int(box[6 * (n - 1)]);
+                for (int k = 0; k < 3; ++k) {
+                    e.lo[k] = box[6 * rt + k];
+                    e.hi[k] = box[6 * rt + 3 + k];
+                }
+                e.link = (int32_t)nbase;
+            } else {
+                const float3 a = ra[0], b = rb[0], c = rc[0];
+                e.lo[0] = fminf(a.x, fminf(b.x, c.x));
+                e.lo[1] = fminf(a.y, fminf(b.y, c.y));
+                e.lo[2] = fminf(a.z, fminf(b.z, c.z));
+                e.hi[0] = fmaxf(a.x, fmaxf(b.x, c.x));
+                e.hi[1] = fmaxf(a.y, fmaxf(b.y, c.y));
+                e.hi[2] = fmaxf(a.z, fmaxf(b.z, c.z));
+                e.link = ~(int32_t)lbase;
+            }
+            e.pad = 0;
+            ch.ent[g] = e;
+        }
+    }
+}
+
+// Number of chunks of each big segment.
+__global__ void k_chunk_count(const uint32_t* __restrict__ seg_off, uint32_t nbig, uint32_t* __restrict__ cnt) {
+    const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < nbig) cnt[l] = (seg_off[l + 1] - seg_off[l] + (uint32_t)kMax - 1u) / (uint32_t)kMax;
+}
+
+// One level of the top trees: an object with np >= 2 entries left (np =
+// ceil(K / 2^(level-1))) pairs entries (2i, 2i+1) under a new node and moves
+// an odd last entry up.  Nodes are numbered from the root down: the nodes
+// made at this level follow those of every level above, so the root (the
+// last level's single node) is top node 0, which is slot seg_off[l].
+__global__ void k_top_level(const uint32_t* __restrict__ seg_off, const uint32_t* __restrict__ chunk_off,
+                            uint32_t nbig, uint32_t level, const pzk::ChunkRoot* __restrict__ in,
+                            pzk::ChunkRoot* __restrict__ out, Node* __restrict__ nodes) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= chunk_off[nbig]) return;
+    const uint32_t l = seg_of_chunk(chunk_off, nbig, g);
+    const uint32_t cb = chunk_off[l], K = chunk_off[l + 1] - cb, i = g - cb;
+    const uint32_t np = (uint32_t)(((uint64_t)K + (1ull << (level - 1)) - 1) >> (level - 1));
+    if (np < 2) return;
+    const uint32_t made = np / 2;
+    if (i < made) {
+        const pzk::ChunkRoot a = in[cb + 2 * i], b = in[cb + 2 * i + 1];
+        uint32_t slot = i;
+        for (uint32_t nl = (np + 1) / 2; nl >= 2; nl = (nl + 1) / 2) slot += nl / 2;
+        const uint32_t node = seg_off[l] + slot * (uint32_t)kMax;
+        Node nd;
+        nd.a = make_float4(a.lo[0], a.lo[1], a.lo[2], a.hi[0]);
+        nd.b = make_float4(a.hi[1], a.hi[2], b.lo[0], b.lo[1]);
+        nd.c = make_float4(b.lo[2], b.hi[0], b.hi[1], b.hi[2]);
+        nd.d = make_int4(a.link, b.link, 0, 0);
+        nodes[node] = nd;
+        pzk::ChunkRoot e;
+        for (int k = 0; k < 3; ++k) {
+            e.lo[k] = fminf(a.lo[k], b.lo[k]);
+            e.hi[k] = fmaxf(a.hi[k], b.hi[k]);
+        }
+        e.link = (int32_t)node;
+        e.pad = 0;
+        out[cb + i] = e;
+    } else if ((np & 1u) && i == made) {
+        out[cb + i] = in[cb + np - 1];
+    }
+}
+
+// The top levels add at most `levels` to the chunks' height.
+__global__ void k_add_height(uint32_t* height, uint32_t levels) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *height += levels;
 }
 
 inline unsigned grid1(uint64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
@@ -498,9 +623,9 @@ bool build_small(const float* verts, const uint32_t* idx, const int32_t* obj_of_
     const size_t lds = build_small_lds(cap);
     KernelTimer kt_(timer ? timer : "build_small", s);
     KernelTimer kt_all_("build_small", s);
-#define PZ_LAUNCH_BS(T)                                                                                          \
-    hipLaunchKernelGGL(k_build_small<T>, dim3(count), dim3(T), lds, s, verts, idx, obj_of_loc, first_tri, tri_cnt, \
-                       obox, seg_off, l_begin, (int)cap, nodes, leaves, height)
+#define PZ_LAUNCH_BS(T)                                                                                         \
+    hipLaunchKernelGGL((k_build_small<T, false>), dim3(count), dim3(T), lds, s, verts, idx, obj_of_loc, first_tri, \
+                       tri_cnt, obox, seg_off, l_begin, (int)cap, nodes, leaves, height, ChunkSrc{})
     switch (threads) {
         case 128: PZ_LAUNCH_BS(128); break;
         case 256: PZ_LAUNCH_BS(256); break;
@@ -510,6 +635,33 @@ bool build_small(const float* verts, const uint32_t* idx, const int32_t* obj_of_
     }
 #undef PZ_LAUNCH_BS
     return true;
+}
+
+uint64_t big_chunk_bound(uint64_t nbig, uint64_t nbig_tri) { return nbig_tri / kSmallMax + nbig; }
+
+void build_big_chunked(const float* verts, const uint32_t* idx, const int32_t* obj_of_loc, const uint32_t* seg_off,
+                       uint32_t nbig, uint64_t nbig_tri, const uint64_t* sorted_keys, const uint32_t* order,
+                       uint32_t* chunk_cnt, uint32_t* chunk_off, void* scan_tmp, ChunkRoot* ent0, ChunkRoot* ent1,
+                       Node* nodes, LeafTri* leaves, uint32_t* height, hipStream_t s) {
+    if (nbig == 0 || nbig_tri == 0) return;
+    const uint64_t bound = big_chunk_bound(nbig, nbig_tri);
+    hipLaunchKernelGGL(k_chunk_count, dim3(grid1(nbig)), dim3(256), 0, s, seg_off, nbig, chunk_cnt);
+    exclusive_scan_u32_u32(chunk_cnt, chunk_off, nbig, scan_tmp, s);
+    {
+        KernelTimer kt_("build_chunks", s);
+        hipLaunchKernelGGL((k_build_small<1024, true>), dim3((unsigned)bound), dim3(1024), build_small_lds(kSmallMax),
+                           s, verts, idx, obj_of_loc, nullptr, nullptr, nullptr, seg_off, 0u, (int)kSmallMax, nodes,
+                           leaves, height, ChunkSrc{sorted_keys, order, chunk_off, nbig, ent0});
+    }
+    // an object has at most ceil(nbig_tri / kSmallMax) chunks
+    const uint64_t kmax = (nbig_tri + kSmallMax - 1) / kSmallMax;
+    uint32_t levels = 0;
+    while ((1ull << levels) < kmax) ++levels;
+    KernelTimer kt_("build_top", s);
+    for (uint32_t lv = 1; lv <= levels; ++lv)
+        hipLaunchKernelGGL(k_top_level, dim3(grid1(bound)), dim3(256), 0, s, seg_off, chunk_off, nbig, lv,
+                           (lv & 1) ? ent0 : ent1, (lv & 1) ? ent1 : ent0, nodes);
+    if (levels) hipLaunchKernelGGL(k_add_height, dim3(1), dim3(64), 0, s, height, levels);
 }
 
 }  // namespace pzk

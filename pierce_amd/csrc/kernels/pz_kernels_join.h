@@ -64,6 +64,23 @@ bool build_small(const float* verts, const uint32_t* idx, const int32_t* obj_of_
                  const uint32_t* first_tri, const uint32_t* tri_cnt, const uint32_t* obox, const uint32_t* seg_off,
                  uint32_t l_begin, uint32_t count, uint32_t cap, int threads, pzd::Node* nodes, pzd::LeafTri* leaves,
                  uint32_t* height, const char* timer, hipStream_t s);
+// Big objects (segments [0, nbig), triangles Morton-sorted per segment into
+// keys / order by the global pipeline's sort) are built in chunks of
+// kSmallMax consecutive sorted triangles, each chunk by the LDS builder, and
+// joined by a balanced top tree per object whose K - 1 nodes take the node
+// slots seg_off[l] + j * kSmallMax (j < K - 1; the chunks' own nodes start one
+// slot later), so the object's root stays at its segment's first node.
+struct ChunkRoot {
+    float lo[3], hi[3];
+    int32_t link, pad;  // as in Node: >= 0 node, < 0 leaf ~index
+};
+// Scratch of build_big_chunked: chunk counts and offsets (nbig + 1 each) and
+// two ChunkRoot arrays of big_chunk_bound entries.
+uint64_t big_chunk_bound(uint64_t nbig, uint64_t nbig_tri);
+void build_big_chunked(const float* verts, const uint32_t* idx, const int32_t* obj_of_loc, const uint32_t* seg_off,
+                       uint32_t nbig, uint64_t nbig_tri, const uint64_t* sorted_keys, const uint32_t* order,
+                       uint32_t* chunk_cnt, uint32_t* chunk_off, void* scan_tmp, ChunkRoot* ent0, ChunkRoot* ent1,
+                       pzd::Node* nodes, pzd::LeafTri* leaves, uint32_t* height, hipStream_t s);
 void edge_count(const int32_t* eo, uint64_t ne, int32_t nobj, uint32_t* edge_cnt, hipStream_t s);
 void scene_box(const uint32_t* obox, int32_t nobj, uint32_t* scene, hipStream_t s);
 // keys = (part_of[o] << 30) | Morton of the box centre (part 0 when part_of is null)
