@@ -511,11 +511,17 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
             uint64_t* keys2 = buf<uint64_t>(c, tag + ".tkeys2", nb_tri + 1);
             uint32_t* vals2 = buf<uint32_t>(c, tag + ".tvals2", nb_tri + 1);
             pzk::fill_u32(cursor, 0u, (size_t)std::max(D->nobj, 1), s);
+            // Morton bits of the keys: of 30, 27, 24 and 21, the most that need the
+            // fewest 8-bit radix passes beside the segment bits (chunks only need
+            // a coherent order; each chunk's tree uses the bits below its common prefix)
+            const int sb = bits_for(std::max<uint64_t>(T.nbig, 1));
+            int mbits = 30;
+            for (int m : {27, 24, 21})
+                if ((m + sb + 7) / 8 < (mbits + sb + 7) / 8) mbits = m;
             pzk::tri_keys_two_level(D->v, D->nv, D->idx, D->t2o, D->nt, D->nobj, big, N.loc, P.obox, N.seg_off,
-                                    P.first_tri, P.last_tri, P.tri_cnt, cursor, keys, vals, s);
+                                    P.first_tri, P.last_tri, P.tri_cnt, cursor, keys, vals, mbits, s);
             void* tmp = buf<char>(c, "sort.tmp." + tag, pzk::radix_sort_temp_bytes(nb_tri));
-            bool alt = pzk::radix_sort_pairs(keys, vals, keys2, vals2, nb_tri, 0,
-                                             30 + bits_for(std::max<uint64_t>(T.nbig, 1)), tmp, s);
+            bool alt = pzk::radix_sort_pairs(keys, vals, keys2, vals2, nb_tri, 0, mbits + sb, tmp, s);
             uint64_t* sk = alt ? keys2 : keys;
             uint32_t* order = alt ? vals2 : vals;
             // runs of the sorted order through the LDS builder, then a top tree per object
@@ -525,7 +531,7 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
             auto* e0 = buf<pzk::ChunkRoot>(c, tag + ".chunk_ent0", nch);
             auto* e1 = buf<pzk::ChunkRoot>(c, tag + ".chunk_ent1", nch);
             void* stmp = buf<char>(c, "scan.tmp", pzk::scan_temp_bytes(T.nbig + 1));
-            pzk::build_big_chunked(D->v, D->idx, N.obj_of_loc, N.seg_off, (uint32_t)T.nbig, nb_tri, sk, order, ccnt,
+            pzk::build_big_chunked(D->v, D->idx, N.obj_of_loc, N.seg_off, (uint32_t)T.nbig, nb_tri, sk, mbits, order, ccnt,
                                    coff, stmp, e0, e1, acc.nodes, acc.leaves, acc.height, s);
         }
         static const char* const kTimer[pzk::kSmallClasses] = {"build_small.c0", "build_small.c1",

@@ -119,10 +119,11 @@ __device__ __forceinline__ uint32_t lane_swap(uint32_t v) {
 // Chunk mode of the builder: the workgroup's triangles are a run of the big
 // objects' Morton-sorted order instead of an object's contiguous range.
 struct ChunkSrc {
-    const uint64_t* keys;       // (segment << 30) | 30-bit Morton code, ascending
+    const uint64_t* keys;       // (segment << mbits) | mbits-bit Morton code, ascending
     const uint32_t* order;      // triangle id at each sorted position
     const uint32_t* chunk_off;  // first chunk of each big segment (nbig + 1 entries)
     uint32_t nbig;
+    int mbits;
     pzk::ChunkRoot* ent;        // per chunk: root link and box
 };
 
@@ -154,6 +155,8 @@ __global__ __launch_bounds__(T) void k_build_small(
     int* bound = reinterpret_cast<int*>(box + 6 * cap);    // cap: sorted positions, then exchange / child ids
     uint16_t* inv = reinterpret_cast<uint16_t*>(bound);
     float* stage = box;
+    // a sort key's Morton part, top-aligned in 30 bits
+    auto code30 = [&](uint64_t k) { return ((uint32_t)k & ((1u << ch.mbits) - 1u)) << (30 - ch.mbits); };
     // leaves at lbase + sorted position, nodes at nbase + node index (root at nbase)
     uint32_t l, t0, lbase, nbase, g = 0, cpl = 0;
     int32_t o;
@@ -170,7 +173,7 @@ __global__ __launch_bounds__(T) void k_build_small(
         nbase = nch == 1 ? t0 : t0 + 1;  // slot t0 belongs to the top tree
         o = obj_of_loc[l];
         // codes of a sorted run share their leading bits: keep the 21 below them
-        const uint32_t x = (uint32_t)(ch.keys[t0] ^ ch.keys[t0 + n - 1]) & kM30;
+        const uint32_t x = code30(ch.keys[t0] ^ ch.keys[t0 + n - 1]);
         cpl = x ? (uint32_t)__clz((int)x) - 2u : 30u;
     } else {
         l = l_begin + blockIdx.x;
@@ -180,6 +183,7 @@ __global__ __launch_bounds__(T) void k_build_small(
         lbase = nbase = seg_off[l];
     }
     const int tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+    (void)code30;
     float lo[3] = {0.0f, 0.0f, 0.0f}, sc[3] = {0.0f, 0.0f, 0.0f};
     if constexpr (!CHUNK)
         for (int k = 0; k < 3; ++k) {
@@ -191,7 +195,7 @@ __global__ __launch_bounds__(T) void k_build_small(
     while (npow < n) npow <<= 1;
     auto morton = [&](const float3& a, const float3& b, const float3& c, int q) {
         if constexpr (CHUNK) {  // ascending in q: the run is sorted already
-            const uint32_t m = ((((uint32_t)ch.keys[t0 + q] & kM30) << cpl) & kM30) >> 9;
+            const uint32_t m = ((code30(ch.keys[t0 + q]) << cpl) & kM30) >> 9;
             return (m << 11) | (uint32_t)q;
         }
         const float cx[3] = {(a.x + b.x + c.x) * (1.0f / 3.0f), (a.y + b.y + c.y) * (1.0f / 3.0f),
@@ -640,7 +644,7 @@ bool build_small(const float* verts, const uint32_t* idx, const int32_t* obj_of_
 uint64_t big_chunk_bound(uint64_t nbig, uint64_t nbig_tri) { return nbig_tri / kSmallMax + nbig; }
 
 void build_big_chunked(const float* verts, const uint32_t* idx, const int32_t* obj_of_loc, const uint32_t* seg_off,
-                       uint32_t nbig, uint64_t nbig_tri, const uint64_t* sorted_keys, const uint32_t* order,
+                       uint32_t nbig, uint64_t nbig_tri, const uint64_t* sorted_keys, int mbits, const uint32_t* order,
                        uint32_t* chunk_cnt, uint32_t* chunk_off, void* scan_tmp, ChunkRoot* ent0, ChunkRoot* ent1,
                        Node* nodes, LeafTri* leaves, uint32_t* height, hipStream_t s) {
     if (nbig == 0 || nbig_tri == 0) return;
@@ -651,7 +655,7 @@ void build_big_chunked(const float* verts, const uint32_t* idx, const int32_t* o
         KernelTimer kt_("build_chunks", s);
         hipLaunchKernelGGL((k_build_small<1024, true>), dim3((unsigned)bound), dim3(1024), build_small_lds(kSmallMax),
                            s, verts, idx, obj_of_loc, nullptr, nullptr, nullptr, seg_off, 0u, (int)kSmallMax, nodes,
-                           leaves, height, ChunkSrc{sorted_keys, order, chunk_off, nbig, ent0});
+                           leaves, height, ChunkSrc{sorted_keys, order, chunk_off, nbig, mbits, ent0});
     }
     // an object has at most ceil(nbig_tri / kSmallMax) chunks
     const uint64_t kmax = (nbig_tri + kSmallMax - 1) / kSmallMax;

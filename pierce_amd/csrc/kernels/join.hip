@@ -470,7 +470,7 @@ __global__ __launch_bounds__(kBlock) void k_tri_keys_two_level(
     int32_t nobj, const uint32_t* __restrict__ need, const uint32_t* __restrict__ loc,
     const uint32_t* __restrict__ obox, const uint32_t* __restrict__ seg_off, const uint32_t* __restrict__ first_tri,
     const uint32_t* __restrict__ last_tri, const uint32_t* __restrict__ tri_cnt, uint32_t* __restrict__ cursor,
-    uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, uint64_t nv) {
+    uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, uint64_t nv, int mbits) {
     uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     int32_t o = t < nt ? t2o[t] : -1;
     bool valid = o >= 0 && o < nobj && need[o];
@@ -496,7 +496,8 @@ __global__ __launch_bounds__(kBlock) void k_tri_keys_two_level(
     float cx = (a.x + b.x + c.x) * (1.0f / 3.0f), cy = (a.y + b.y + c.y) * (1.0f / 3.0f),
           cz = (a.z + b.z + c.z) * (1.0f / 3.0f);
     uint64_t pos = (uint64_t)seg_off[l] + slot;
-    keys[pos] = ((uint64_t)l << 30) | morton30(cx, cy, cz, lo[0], lo[1], lo[2], inv[0], inv[1], inv[2]);
+    keys[pos] = ((uint64_t)l << mbits) |
+                (morton30(cx, cy, cz, lo[0], lo[1], lo[2], inv[0], inv[1], inv[2]) >> (30 - mbits));
     vals[pos] = (uint32_t)t;
 }
 
@@ -1550,11 +1551,12 @@ void needed_layout(const uint32_t* need, const uint32_t* loc, int32_t nobj, cons
 void tri_keys_two_level(const float* verts, uint64_t nv, const uint32_t* idx, const int32_t* t2o, uint64_t nt,
                         int32_t nobj, const uint32_t* need, const uint32_t* loc, const uint32_t* obox,
                         const uint32_t* seg_off, const uint32_t* first_tri, const uint32_t* last_tri,
-                        const uint32_t* tri_cnt, uint32_t* cursor, uint64_t* keys, uint32_t* vals, hipStream_t s) {
+                        const uint32_t* tri_cnt, uint32_t* cursor, uint64_t* keys, uint32_t* vals, int mbits,
+                        hipStream_t s) {
     KernelTimer kt_("tri_keys", s);
     if (nt)
         hipLaunchKernelGGL(k_tri_keys_two_level, dim3(grid1(nt)), dim3(kBlock), 0, s, verts, idx, t2o, nt, nobj, need,
-                           loc, obox, seg_off, first_tri, last_tri, tri_cnt, cursor, keys, vals, nv);
+                           loc, obox, seg_off, first_tri, last_tri, tri_cnt, cursor, keys, vals, nv, mbits);
 }
 
 void tri_keys_global(const float* verts, const uint32_t* idx, uint64_t nt, const uint32_t* scene, uint64_t* keys,

@@ -77,8 +77,9 @@ struct ChunkRoot {
 // Scratch of build_big_chunked: chunk counts and offsets (nbig + 1 each) and
 // two ChunkRoot arrays of big_chunk_bound entries.
 uint64_t big_chunk_bound(uint64_t nbig, uint64_t nbig_tri);
+// sorted_keys = (segment << mbits) | the top mbits of the 30-bit Morton code.
 void build_big_chunked(const float* verts, const uint32_t* idx, const int32_t* obj_of_loc, const uint32_t* seg_off,
-                       uint32_t nbig, uint64_t nbig_tri, const uint64_t* sorted_keys, const uint32_t* order,
+                       uint32_t nbig, uint64_t nbig_tri, const uint64_t* sorted_keys, int mbits, const uint32_t* order,
                        uint32_t* chunk_cnt, uint32_t* chunk_off, void* scan_tmp, ChunkRoot* ent0, ChunkRoot* ent1,
                        pzd::Node* nodes, pzd::LeafTri* leaves, uint32_t* height, hipStream_t s);
 void edge_count(const int32_t* eo, uint64_t ne, int32_t nobj, uint32_t* edge_cnt, hipStream_t s);
@@ -109,7 +110,8 @@ void needed_layout(const uint32_t* need, const uint32_t* loc, int32_t nobj, cons
 void tri_keys_two_level(const float* verts, uint64_t nv, const uint32_t* idx, const int32_t* t2o, uint64_t nt,
                         int32_t nobj, const uint32_t* need, const uint32_t* loc, const uint32_t* obox,
                         const uint32_t* seg_off, const uint32_t* first_tri, const uint32_t* last_tri,
-                        const uint32_t* tri_cnt, uint32_t* cursor, uint64_t* keys, uint32_t* vals, hipStream_t s);
+                        const uint32_t* tri_cnt, uint32_t* cursor, uint64_t* keys, uint32_t* vals, int mbits,
+                        hipStream_t s);
 void tri_keys_global(const float* verts, const uint32_t* idx, uint64_t nt, const uint32_t* scene, uint64_t* keys,
                      uint32_t* vals, hipStream_t s);
 void edge_scatter(const float* es, const float* ee, const int32_t* eo, uint64_t ne, int32_t nobj,
