@@ -36,7 +36,8 @@ __device__ __forceinline__ float3 vtx(const float* __restrict__ v, uint32_t i) {
 // sums are flushed (wave reduction + one atomic per field) only when the
 // object changes.  Objects' triangles are usually contiguous, so a run of
 // 1,024 triangles costs a couple of flushes instead of one set of atomics
-// per 64 triangles.
+// per 64 triangles.  A step that crosses a boundary also costs one flush: the
+// ending object's lanes join the sums before it, the next object's after it.
 constexpr int kStatsRun = 1024;
 
 __global__ __launch_bounds__(kBlock) void k_object_stats(const float* __restrict__ verts, uint64_t nv,
@@ -84,11 +85,21 @@ __global__ __launch_bounds__(kBlock) void k_object_stats(const float* __restrict
         tmax = 0u;
         run_o = -1;
     };
+    auto add = [&](bool mine, uint64_t t, const float* tl, const float* th) {  // into the running sums
+        if (!mine) return;
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(lo[a], tl[a]);
+            hi[a] = fmaxf(hi[a], th[a]);
+        }
+        ++cnt;
+        tmin = min(tmin, (uint32_t)t);
+        tmax = max(tmax, (uint32_t)t);
+    };
     for (uint64_t step = begin; step < end; step += PZ_WAVE) {
         const uint64_t t = step + lane;
         bool valid = t < end;
         int32_t o = -1;
-        float tl[3], th[3];
+        float tl[3] = {INFINITY, INFINITY, INFINITY}, th[3] = {-INFINITY, -INFINITY, -INFINITY};
         if (valid) {
             o = t2o[t];
             const uint32_t i0 = idx[3 * t], i1 = idx[3 * t + 1], i2 = idx[3 * t + 2];
@@ -105,22 +116,25 @@ __global__ __launch_bounds__(kBlock) void k_object_stats(const float* __restrict
         }
         const uint64_t vm = __ballot(valid);
         if (vm == 0) continue;
+        // objects of the first and the last valid lane (wave-uniform)
         const int32_t o0 = __shfl(o, __ffsll((long long)vm) - 1, 64);
-        const bool uniform = __ballot(valid && o == o0) == vm;
-        if (!uniform || o0 != run_o) {
+        const int32_t o1 = __shfl(o, 63 - __clzll((long long)vm), 64);
+        if (o0 != run_o) {
             flush();
-            if (uniform) run_o = o0;
+            run_o = o0;
         }
-        if (!valid) continue;
-        if (uniform) {
-            for (int a = 0; a < 3; ++a) {
-                lo[a] = fminf(lo[a], tl[a]);
-                hi[a] = fmaxf(hi[a], th[a]);
-            }
-            ++cnt;
-            tmin = min(tmin, (uint32_t)t);
-            tmax = max(tmax, (uint32_t)t);
-        } else {  // several objects in these 64 triangles
+        add(valid && o == o0, t, tl, th);
+        if (__ballot(valid && o != o0) == 0) continue;  // one object
+        // An object boundary: the first object is complete once its lanes are
+        // in the running sums, the last one starts them anew, and lanes of
+        // objects wholly inside these 64 triangles use per-lane atomics.
+        flush();
+        const bool second = o1 != o0;
+        if (second) {
+            run_o = o1;
+            add(valid && o == o1, t, tl, th);
+        }
+        if (valid && o != o0 && (!second || o != o1)) {
             for (int a = 0; a < 3; ++a) {
                 atomicMin(&obox[6ull * o + a], f2ord(tl[a]));
                 atomicMax(&obox[6ull * o + 3 + a], f2ord(th[a]));

@@ -186,3 +186,25 @@ def test_ragged_object_ids_and_single_triangles(ctx, predicate):
         np.testing.assert_array_equal(pz.join(ctx, da, db, predicate=predicate, **kw).pairs, exp)
     if predicate == pz.PRED_OVERLAP:
         np.testing.assert_array_equal(pz.join(ctx, da, db, strategy=pz.STRATEGY_GLOBAL).pairs, exp)
+
+
+def test_shuffled_triangle_order(ctx):
+    # Every triangle in random order: each 64-triangle step of the object
+    # bounds mixes objects (segmented wave reduction), and no object is stored
+    # contiguously, so all of them go through the sort-and-chunk builder (the
+    # 5,120- and 20,480-triangle spheres in several chunks under a top tree).
+    rng = np.random.default_rng(5)
+    sets = []
+    for seed in (51, 52):
+        m = pz.generate(pz.GEN_SPHERES_MIXED, count=60, seed=seed, level_min=1, level_max=5, radius_min=1.0,
+                        radius_max=4.0, box=30.0)
+        p = rng.permutation(m.indices.shape[0])
+        sets.append(pz.extract_edges(pz.Mesh(m.vertices, m.indices[p], m.triangle_to_object[p])))
+    a, b = sets
+    assert np.bincount(a.triangle_to_object).max() > 4096
+    da, db = ctx.upload(a), ctx.upload(b)
+    for predicate in (pz.PRED_OVERLAP, pz.PRED_INTERSECTION, pz.PRED_CONTAINMENT):
+        exp = orc.face_overlap(a, b) if predicate == pz.PRED_OVERLAP else orc.predicate_join(a, b, predicate)[0]
+        print(f"[shuffled] predicate={predicate} expected={len(exp)}", flush=True)
+        assert len(exp) > 0 or predicate == pz.PRED_CONTAINMENT
+        np.testing.assert_array_equal(pz.join(ctx, da, db, predicate=predicate).pairs, exp)
