@@ -314,7 +314,7 @@ SegOut run_segment(pz_ctx* c, uint64_t k1, uint64_t k2, int tag, pz_join_result*
     return o;
 }
 
-enum { kSegPrep = 1, kSegObjBvh = 2 };
+enum { kSegPrep = 1, kSegObjBvh = 2, kSegLayout = 3 };
 // Largest mesh2 object count whose broad phase tests every box instead of
 // walking an object LBVH (a walk costs tens of microseconds of dependent
 // loads whatever the set size; a full scan of this many boxes costs less).
@@ -474,8 +474,16 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         pzk::big_prefix(N.seg_off, totals, s);
         return big;
     };
-    uint32_t* bigA = layout(A, pa, na, "A");
-    uint32_t* bigB = layout(B, pb, nb, "B");
+    // about 40 small launches whose shapes depend only on both sets' object
+    // counts: one graph from the second join of the same datasets on
+    const SegOut lay = run_segment(c, A->uid, B->uid, kSegLayout, r, [&]() {
+        SegOut o;
+        o.big[0] = layout(A, pa, na, "A");
+        o.big[1] = layout(B, pb, nb, "B");
+        return o;
+    });
+    uint32_t* bigA = lay.big[0];
+    uint32_t* bigB = lay.big[1];
     uint32_t* totA = buf<uint32_t>(c, "A.totals", 8);
     uint32_t* totB = buf<uint32_t>(c, "B.totals", 8);
     // needed objects and their triangles, then totals[0..7] of each set, in one synchronisation

@@ -174,6 +174,7 @@ struct ObjBvh {
 struct SegOut {
     SetPrep pa{}, pb{};
     ObjBvh ob{};
+    uint32_t* big[2] = {nullptr, nullptr};  // per-object "built by the big-object path" flags of both sets
 };
 
 // predicate | direction bits << 2 of one join description
