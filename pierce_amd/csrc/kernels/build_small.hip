@@ -35,6 +35,13 @@
 // LDS is dynamic, 32 B per triangle of the launch's largest object: keys
 // (4 B), boxes (24 B) and the exchange words (4 B), which hold the sorted
 // positions until the climb starts.
+//
+// Chunk mode (objects over 2,048 triangles or not contiguous): the big
+// objects' triangles are Morton-sorted once over the whole set; each
+// workgroup takes a run of up to 2,048 sorted positions of one object, whose
+// keys are already ascending (step 2 is skipped), writes its root for the
+// per-object top tree (k_top_level), and leaves the top tree one node slot
+// per chunk.
 #include <type_traits>
 
 #include "pz_device.h"
