@@ -149,13 +149,27 @@ inline unsigned feat_grid(int32_t nobj) {
     return (unsigned)((waves * 64 + kBlock - 1) / kBlock);
 }
 
-// Box, counts and triangle totals of one set per part.
-__global__ __launch_bounds__(kBlock) void k_feat_bounds(const uint32_t* __restrict__ obox,
-                                                        const uint32_t* __restrict__ tri, int32_t nobj,
-                                                        const uint32_t* __restrict__ part_of, int set, int chunk,
-                                                        PartAcc* __restrict__ acc) {
+// Both sets in one launch: blocks [0, blocks0) walk set 0, the rest set 1.
+struct FeatSets {
+    const uint32_t* obox[2];
+    const uint32_t* tri[2];
+    const uint32_t* part_of[2];
+    int32_t nobj[2];
+    int chunk[2];
+    unsigned blocks0;
+};
+
+// Box, counts and triangle totals of each set per part.
+__global__ __launch_bounds__(kBlock) void k_feat_bounds(FeatSets fs, PartAcc* __restrict__ acc) {
+    const int set = blockIdx.x < fs.blocks0 ? 0 : 1;
+    const uint32_t* __restrict__ obox = fs.obox[set];
+    const uint32_t* __restrict__ tri = fs.tri[set];
+    const uint32_t* __restrict__ part_of = fs.part_of[set];
+    const int32_t nobj = fs.nobj[set];
+    const int chunk = fs.chunk[set];
+    const unsigned blk = blockIdx.x - (set ? fs.blocks0 : 0u);
     const int lane = __lane_id();
-    const int64_t begin = (int64_t)((blockIdx.x * kBlock + threadIdx.x) / 64) * chunk;
+    const int64_t begin = (int64_t)((blk * kBlock + threadIdx.x) / 64) * chunk;
     const int64_t end = min(begin + (int64_t)chunk, (int64_t)nobj);
     uint32_t cp = kNoPart, clo[3], chi[3], cn = 0, ctm = 0;  // carried run (wave-uniform)
     unsigned long long cts = 0;
@@ -222,13 +236,18 @@ __global__ __launch_bounds__(kBlock) void k_feat_bounds(const uint32_t* __restri
 }
 
 // Per-cell counts and extents (cell of the box centre), per-part extent sums
-// and objects meeting the other set's box.
-__global__ __launch_bounds__(kBlock) void k_feat_cells(const uint32_t* __restrict__ obox,
-                                                       const uint32_t* __restrict__ tri, int32_t nobj,
-                                                       const uint32_t* __restrict__ part_of, int set, int chunk,
-                                                       PartAcc* __restrict__ acc, CellAcc* __restrict__ cells) {
+// and objects meeting the other set's box, for both sets.
+__global__ __launch_bounds__(kBlock) void k_feat_cells(FeatSets fs, PartAcc* __restrict__ acc,
+                                                       CellAcc* __restrict__ cells) {
+    const int set = blockIdx.x < fs.blocks0 ? 0 : 1;
+    const uint32_t* __restrict__ obox = fs.obox[set];
+    const uint32_t* __restrict__ tri = fs.tri[set];
+    const uint32_t* __restrict__ part_of = fs.part_of[set];
+    const int32_t nobj = fs.nobj[set];
+    const int chunk = fs.chunk[set];
+    const unsigned blk = blockIdx.x - (set ? fs.blocks0 : 0u);
     const int lane = __lane_id();
-    const int64_t begin = (int64_t)((blockIdx.x * kBlock + threadIdx.x) / 64) * chunk;
+    const int64_t begin = (int64_t)((blk * kBlock + threadIdx.x) / 64) * chunk;
     const int64_t end = min(begin + (int64_t)chunk, (int64_t)nobj);
     uint32_t cp = kNoPart;  // carried run (wave-uniform)
     unsigned long long cq[3] = {0ull, 0ull, 0ull}, cr = 0ull;
@@ -642,18 +661,13 @@ void join_features(const uint32_t* oboxA, const uint32_t* triA, int32_t nA, cons
     const uint64_t zero_n = (sizeof(CellAcc) * 2 * (uint64_t)kFeatCells * nparts) / 8 + (uint64_t)kHistSlots * nparts;
     hipLaunchKernelGGL(k_zero_u64, dim3(grid1(zero_n)), dim3(kBlock), 0, s,
                        reinterpret_cast<unsigned long long*>(sc.cells), zero_n);
-    if (nA > 0)
-        hipLaunchKernelGGL(k_feat_bounds, dim3(feat_grid(nA)), dim3(kBlock), 0, s, oboxA, triA, nA, partA, 0,
-                           feat_chunk(nA), sc.acc);
-    if (nB > 0)
-        hipLaunchKernelGGL(k_feat_bounds, dim3(feat_grid(nB)), dim3(kBlock), 0, s, oboxB, triB, nB, partB, 1,
-                           feat_chunk(nB), sc.acc);
-    if (nA > 0)
-        hipLaunchKernelGGL(k_feat_cells, dim3(feat_grid(nA)), dim3(kBlock), 0, s, oboxA, triA, nA, partA, 0,
-                           feat_chunk(nA), sc.acc, sc.cells);
-    if (nB > 0)
-        hipLaunchKernelGGL(k_feat_cells, dim3(feat_grid(nB)), dim3(kBlock), 0, s, oboxB, triB, nB, partB, 1,
-                           feat_chunk(nB), sc.acc, sc.cells);
+    const unsigned gA = nA > 0 ? feat_grid(nA) : 0u, gB = nB > 0 ? feat_grid(nB) : 0u;
+    const FeatSets fs{{oboxA, oboxB}, {triA, triB}, {partA, partB}, {std::max(nA, 0), std::max(nB, 0)},
+                      {feat_chunk(nA), feat_chunk(nB)}, gA};
+    if (gA + gB > 0) {
+        hipLaunchKernelGGL(k_feat_bounds, dim3(gA + gB), dim3(kBlock), 0, s, fs, sc.acc);
+        hipLaunchKernelGGL(k_feat_cells, dim3(gA + gB), dim3(kBlock), 0, s, fs, sc.acc, sc.cells);
+    }
     hipLaunchKernelGGL(k_feat_hist, dim3(grid1((uint64_t)nparts * kFeatCells)), dim3(kBlock), 0, s, sc.cells, nparts,
                        sc.hist);
     hipLaunchKernelGGL(k_feat_finalize, dim3(grid1(nparts)), dim3(kBlock), 0, s, sc.acc, sc.hist, nparts,
