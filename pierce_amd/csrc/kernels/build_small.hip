@@ -52,6 +52,8 @@ namespace {
 using namespace pzd;
 
 constexpr int kMax = (int)pzk::kSmallMax;
+constexpr uint32_t kCh = pzk::kChunk;           // triangles per chunk of a big object
+constexpr int kChThreads = pzk::kChunkThreads;  // workgroup size of the chunk builds
 
 __device__ __forceinline__ float3 vtx(const float* __restrict__ v, uint32_t i) {
     return make_float3(v[3ull * i], v[3ull * i + 1], v[3ull * i + 2]);
@@ -174,8 +176,8 @@ __global__ __launch_bounds__(T) void k_build_small(
         l = seg_of_chunk(ch.chunk_off, ch.nbig, g);
         const uint32_t sb = seg_off[l], m = seg_off[l + 1] - sb;
         const uint32_t j = g - ch.chunk_off[l], nch = ch.chunk_off[l + 1] - ch.chunk_off[l];
-        t0 = sb + j * (uint32_t)kMax;  // first sorted position of the chunk
-        n = (int)min((uint32_t)kMax, m - j * (uint32_t)kMax);
+        t0 = sb + j * kCh;  // first sorted position of the chunk
+        n = (int)min(kCh, m - j * kCh);
         lbase = t0;
         nbase = nch == 1 ? t0 : t0 + 1;  // slot t0 belongs to the top tree
         o = obj_of_loc[l];
@@ -553,7 +555,7 @@ __global__ __launch_bounds__(T) void k_build_small(
 // Number of chunks of each big segment.
 __global__ void k_chunk_count(const uint32_t* __restrict__ seg_off, uint32_t nbig, uint32_t* __restrict__ cnt) {
     const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l < nbig) cnt[l] = (seg_off[l + 1] - seg_off[l] + (uint32_t)kMax - 1u) / (uint32_t)kMax;
+    if (l < nbig) cnt[l] = (seg_off[l + 1] - seg_off[l] + kCh - 1u) / kCh;
 }
 
 // One level of the top trees: an object with np >= 2 entries left (np =
@@ -575,7 +577,7 @@ __global__ void k_top_level(const uint32_t* __restrict__ seg_off, const uint32_t
         const pzk::ChunkRoot a = in[cb + 2 * i], b = in[cb + 2 * i + 1];
         uint32_t slot = i;
         for (uint32_t nl = (np + 1) / 2; nl >= 2; nl = (nl + 1) / 2) slot += nl / 2;
-        const uint32_t node = seg_off[l] + slot * (uint32_t)kMax;
+        const uint32_t node = seg_off[l] + slot * kCh;
         Node nd;
         nd.a = make_float4(a.lo[0], a.lo[1], a.lo[2], a.hi[0]);
         nd.b = make_float4(a.hi[1], a.hi[2], b.lo[0], b.lo[1]);
@@ -648,7 +650,7 @@ bool build_small(const float* verts, const uint32_t* idx, const int32_t* obj_of_
     return true;
 }
 
-uint64_t big_chunk_bound(uint64_t nbig, uint64_t nbig_tri) { return nbig_tri / kSmallMax + nbig; }
+uint64_t big_chunk_bound(uint64_t nbig, uint64_t nbig_tri) { return nbig_tri / kChunk + nbig; }
 
 void build_big_chunked(const float* verts, const uint32_t* idx, const int32_t* obj_of_loc, const uint32_t* seg_off,
                        uint32_t nbig, uint64_t nbig_tri, const uint64_t* sorted_keys, int mbits, const uint32_t* order,
@@ -660,12 +662,13 @@ void build_big_chunked(const float* verts, const uint32_t* idx, const int32_t* o
     exclusive_scan_u32_u32(chunk_cnt, chunk_off, nbig, scan_tmp, s);
     {
         KernelTimer kt_("build_chunks", s);
-        hipLaunchKernelGGL((k_build_small<1024, true>), dim3((unsigned)bound), dim3(1024), build_small_lds(kSmallMax),
-                           s, verts, idx, obj_of_loc, nullptr, nullptr, nullptr, seg_off, 0u, (int)kSmallMax, nodes,
+        hipLaunchKernelGGL((k_build_small<kChThreads, true>), dim3((unsigned)bound), dim3(kChThreads),
+                           build_small_lds(kChunk), s, verts, idx, obj_of_loc, nullptr, nullptr, nullptr, seg_off, 0u,
+                           (int)kChunk, nodes,
                            leaves, height, ChunkSrc{sorted_keys, order, chunk_off, nbig, mbits, ent0});
     }
-    // an object has at most ceil(nbig_tri / kSmallMax) chunks
-    const uint64_t kmax = (nbig_tri + kSmallMax - 1) / kSmallMax;
+    // an object has at most ceil(nbig_tri / kChunk) chunks
+    const uint64_t kmax = (nbig_tri + kChunk - 1) / kChunk;
     uint32_t levels = 0;
     while ((1ull << levels) < kmax) ++levels;
     KernelTimer kt_("build_top", s);

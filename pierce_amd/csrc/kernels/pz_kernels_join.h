@@ -66,10 +66,14 @@ bool build_small(const float* verts, const uint32_t* idx, const int32_t* obj_of_
                  uint32_t* height, const char* timer, hipStream_t s);
 // Big objects (segments [0, nbig), triangles Morton-sorted per segment into
 // keys / order by the global pipeline's sort) are built in chunks of
-// kSmallMax consecutive sorted triangles, each chunk by the LDS builder, and
+// kChunk consecutive sorted triangles, each chunk by the LDS builder, and
 // joined by a balanced top tree per object whose K - 1 nodes take the node
-// slots seg_off[l] + j * kSmallMax (j < K - 1; the chunks' own nodes start one
+// slots seg_off[l] + j * kChunk (j < K - 1; the chunks' own nodes start one
 // slot later), so the object's root stays at its segment's first node.
+// 2,048 / 1,024 measured faster than 1,024 / 512 (reduced C4: chunk builds
+// 3.98 against 4.07 ms, narrow phase 0.72 against 0.77 ms).
+constexpr uint32_t kChunk = 2048;    // sorted triangles per chunk (<= kSmallMax)
+constexpr int kChunkThreads = 1024;  // one chunk build's workgroup
 struct ChunkRoot {
     float lo[3], hi[3];
     int32_t link, pad;  // as in Node: >= 0 node, < 0 leaf ~index
