@@ -143,7 +143,8 @@ def extra_configs(pz, ctx, model, steps):
 
     C1 (1000 vs 1000 ellipsoids, s=2) is also checked against the CPU oracle in full.
     C3 (10k vs 10k spheres, s in 1..6) runs its three predicates as one batch.
-    C4 is reduced to 200 vs 200 tubes (same triangle-count law, density kept)."""
+    C4 is reduced to 200 vs 200 tubes (same triangle-count law, density kept).
+    Each configuration is autotuned on its own workload before it is timed."""
     from oracle import oracle as orc
 
     learned = dict(sizing=pz.SIZING_LEARNED, model=model) if model else dict(sizing=pz.SIZING_TWO_PASS)
@@ -154,6 +155,7 @@ def extra_configs(pz, ctx, model, steps):
 
     def single(name, a, b, oracle=False, plain_steps=1):
         da, db = ctx.upload(a), ctx.upload(b)
+        tune = pz.autotune(ctx, da, db, **chosen)  # on this workload, untimed
         pz.join(ctx, da, db, **chosen)
         ms_c, out = time_join(pz, ctx, da, db, steps, **chosen)
         pz.join(ctx, da, db, **plain)
@@ -163,7 +165,8 @@ def extra_configs(pz, ctx, model, steps):
              "mismatched_pairs": pair_mismatch(out.pairs, pout.pairs), "pairs": int(len(out.pairs)),
              "tests": out.tests_total, "escalated_fp64": out.tests_fp64, "escalated_exact": out.tests_exact,
              "fp32_only_misclassified": out.tests_fp32_misclassified, "sizing_overflowed": out.overflowed,
-             "narrow_rounds": out.narrow_rounds}
+             "narrow_rounds": out.narrow_rounds, "autotune": {k: tune[k] for k in ("build_threads", "narrow_queue",
+                                                                                  "narrow_chunk")}}
         if oracle:
             exp = orc.face_overlap(a, b)
             r["oracle_mismatched_pairs"] = pair_mismatch(out.pairs, exp)
@@ -186,6 +189,7 @@ def extra_configs(pz, ctx, model, steps):
         preds = [pz.PRED_OVERLAP, pz.PRED_INTERSECTION, pz.PRED_CONTAINMENT]
         da3, db3 = ctx.upload_batch([a] * 3), ctx.upload_batch([b] * 3)
         q = [dict(predicate=p) for p in preds]
+        tune = pz.autotune(ctx, da3, db3, **chosen)  # on this workload, untimed
         pz.join_batch(ctx, da3, db3, q, **chosen)
         t0 = time.perf_counter()
         for _ in range(steps):
@@ -210,7 +214,8 @@ def extra_configs(pz, ctx, model, steps):
                      "speedup_vs_plain": round(ms_p / ms_c, 2), "mismatched_pairs": int(mism),
                      "pairs": [int(len(o.pairs)) for o in outs], "tests": summ.tests_total,
                      "escalated_fp64": summ.tests_fp64, "escalated_exact": summ.tests_exact,
-                     "fp32_only_misclassified": summ.tests_fp32_misclassified, "narrow_rounds": summ.narrow_rounds}
+                     "fp32_only_misclassified": summ.tests_fp32_misclassified, "narrow_rounds": summ.narrow_rounds,
+                     "autotune": {k: tune[k] for k in ("build_threads", "narrow_queue", "narrow_chunk")}}
         log(f"[bench] C3: batch {ms_c:.3f} ms, plain {ms_p:.3f} ms, mismatches {mism}")
         for d in (da3, db3, da, db):
             d.close()
