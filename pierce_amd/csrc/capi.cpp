@@ -578,7 +578,9 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         // Work queue of (pair, subtree pair) tasks, run in rounds: round 0 takes
         // every candidate pair from its roots; a wave whose LDS queue fills up
         // hands its shallowest subtree pairs to the next round.
-        const uint64_t cap_tasks = std::min<uint64_t>(std::max<uint64_t>(8 * np, 1 << 16), 1ull << 24);
+        // spill capacity per round (16 B a task): tasks beyond it send their pair to
+        // the per-edge kernel, which is slow on big objects, so keep it generous
+        const uint64_t cap_tasks = std::min<uint64_t>(std::max<uint64_t>(64 * np, 1ull << 20), 1ull << 24);
         pzd::DualTask* tin = buf<pzd::DualTask>(c, "np.tasks0", cap_tasks);
         pzd::DualTask* tout = buf<pzd::DualTask>(c, "np.tasks1", cap_tasks);
         unsigned long long* ctr = buf<unsigned long long>(c, "np.task_ctr", 3);  // next task, spilled, open leaf pairs
