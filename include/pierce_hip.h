@@ -180,6 +180,11 @@ void      pz_ctx_set_graphs(pz_ctx* ctx, int32_t on);
 /* Free and total HBM of the context's device (sizing from free memory,
  * pierce/query/src/applications/pierce_overlap_main.cpp:474-489). */
 pz_status pz_ctx_mem_info(pz_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
+/* Return the context's cached device work buffers (they grow to the largest
+ * join seen and are kept for the next one) and drop its captured graphs.
+ * Datasets are not touched.  For alternating very large joins whose work
+ * buffers do not fit in HBM together.  No reference counterpart. */
+pz_status pz_ctx_trim(pz_ctx* ctx);
 
 /* Upload one dataset to HBM (GeometryUploader::upload + PrecomputedEdgeData).
  * A dataset without edges is rejected with PZ_E_NO_EDGES
@@ -265,7 +270,8 @@ typedef struct {
     uint64_t tests_exact;       /* decided by exact arithmetic */
     uint64_t tests_fp32_misclassified; /* escalated tests an unguarded fp32 evaluation gets wrong */
     uint64_t edges_traced;      /* edge traversals started */
-    uint64_t stack_overflows;   /* must be 0: traversal stack exceeded (result invalid) */
+    uint64_t stack_overflows;   /* always 0: a BVH too tall for the traversal stack fails the join with
+                                   PZ_E_LIMIT before any traversal runs */
     uint64_t index_triangles;   /* triangles indexed by the per-object (two-level) or set-wide BVHs */
     uint64_t index_edges;       /* edges regrouped per object (two-level) */
     uint64_t tasks_spilled;     /* narrow-phase subtree tasks handed from one work-queue round to the next */
@@ -274,6 +280,17 @@ typedef struct {
     uint64_t hash_accesses;     /* GLOBAL with a hash set: slots probed (0 elsewhere) */
     uint64_t hash_contentions;  /* probed slots already holding another pair
                                    (pierce_overlap_main.cpp:676-686 reports both per run) */
+    /* Walk counters (the quantities pierce_intersection --enable-profiling-stats reports,
+     * pierce/query/src/cuda/mesh_intersection.h:7-22): */
+    uint64_t narrow_fallback_pairs; /* two-level: candidate pairs the dual-tree walk handed to the
+                                       per-edge kernel (queue or list full); a normal, exact outcome */
+    uint64_t walks_started;     /* narrow-phase walks: dual-tree tasks taken + per-edge traversals */
+    uint64_t pairs_touching;    /* candidate pairs proven to have touching surfaces */
+    uint64_t parity_rays;       /* +Z parity rays cast (one per pair and direction tested) */
+    uint64_t parity_crossings;  /* triangles crossed by those rays */
+    uint64_t parity_rays_crossing; /* rays that crossed at least one triangle */
+    uint64_t parity_inside;     /* rays that crossed an odd number of triangles (point inside) */
+    uint64_t parity_tests;      /* ray-triangle tests of those rays (part of tests_total) */
     double   ms_total;          /* device-resident geometry -> sorted host pair set */
     pz_phase phases[PZ_MAX_PHASES];
 } pz_join_result;
@@ -430,9 +447,20 @@ void      pz_profile_reset(void);
 pz_status pz_profile_query(const char* kernel, double* total_ms, uint64_t* launches);
 
 /* ------------------------------------------------ reference extern "C" helpers */
-/* pierce/query/src/cuda/mesh_query_deduplication.h:7-11: device inputs, caller-sized
- * device output (>= n1 + n2); returns the number of unique pairs, sorted by
- * (mesh1, mesh2).  Returns -1 on failure. */
+/* These keep the reference's conventions: the current HIP device, the null
+ * stream, caller-owned device buffers, counts returned by value, and a block
+ * until the result is on the device.  Their scratch comes from one workspace
+ * per device that grows on demand and is kept for later calls (freed when
+ * the library is unloaded); calls from several host threads are serialised. */
+/* pierce/query/src/cuda/mesh_query_deduplication.h:7-11
+ * (mesh_query_deduplication.cu:85-111): the pairs of d_results1 followed by
+ * those of d_results2 are written to d_merged_output (sized >= n1 + n2) as the
+ * sorted unique set.  A null d_results1 (d_results2) with a non-zero count
+ * means those pairs are already at d_merged_output (d_merged_output + n1),
+ * the in-place form pierce_overlap_two_pass_main.cpp:137 uses.  Returns the
+ * number of unique pairs (0 for no input), sorted by (mesh1, mesh2) (the
+ * reference's u64 reinterpretation orders by (mesh2, mesh1); the set is the
+ * same), or -1 on failure. */
 long long merge_and_deduplicate_pairs_gpu(const pz_pair* d_results1, long long num_results1,
                                           const pz_pair* d_results2, long long num_results2,
                                           pz_pair* d_merged_output);
@@ -444,9 +472,27 @@ int compact_hash_table_pairs(const unsigned long long* d_hash_table, unsigned lo
 long long exclusive_scan_gpu(const int* d_input, long long* d_output, int num_elements);
 
 /* estimated_overlap.h:6-13 (host inputs): sum over matched cells of the grid
- * estimate; -1.0f on failure. */
+ * estimate, evaluated on the host; -1.0f on failure.  Takes the .pre file's
+ * 28-byte cells (index + GridCell). */
 float pz_estimate_overlap_selectivity(const pz_grid_cell* cells_a, const pz_grid_cell* cells_b,
                                       int32_t num_cells, float cell_volume, float epsilon, float gamma);
+
+/* The 16-byte GridCell of pierce/common/include/GridCell.h:5-10, field for field. */
+typedef struct {
+    uint32_t center_count;   /* CenterCount */
+    uint32_t touch_count;    /* TouchCount */
+    float    avg_size_mean;  /* AvgSizeMean */
+    float    vol_ratio;      /* VolRatio */
+} pz_cell_stats;
+
+/* estimateOverlapSelectivity of pierce/query/src/cuda/estimated_overlap.h:6-13
+ * (estimated_overlap.cu:8-77) on GridCell-layout host arrays: the per-cell
+ * terms and their sum are evaluated on the current HIP device (the sum in
+ * double, in a fixed order), and the total is returned as float; -1.0f on
+ * failure, as the reference returns.  A C++ definition of the reference's
+ * own declaration that forwards here is shown in INTEGRATION.md. */
+float pz_estimate_overlap_selectivity_cells(const pz_cell_stats* h_grid_a, const pz_cell_stats* h_grid_b,
+                                            int32_t num_cells, float cell_volume, float epsilon, float gamma);
 
 /* Whole-set grid estimate with its intermediates: the values the reference's
  * verbose estimation block prints (pierce/query/src/applications/pierce_overlap_main.cpp:376-471,

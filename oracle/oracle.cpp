@@ -841,18 +841,24 @@ or_pairs* or_edge_overlap(const or_mesh_view* a, const or_mesh_view* b, int dire
 }
 
 // predicate 1: intersection, 2: containment (B inside A); include_overlap adds
-// overlap pairs with type 1 for containment.
-or_pairs* or_predicate_join(const or_mesh_view* a, const or_mesh_view* b, int predicate, int include_overlap) {
+// overlap pairs with type 1 for containment.  direction (intersection only,
+// pierce_intersection_main.cpp:103-155): 0 both, 1 A's edges against B's
+// triangles and A's first vertices in B's objects, 2 the reverse; containment
+// always uses both edge directions (pierce_containment_main.cpp:385-550).
+or_pairs* or_predicate_join_dir(const or_mesh_view* a, const or_mesh_view* b, int predicate, int include_overlap,
+                                int direction) {
     Mesh A = make_mesh(a), B = make_mesh(b);
     Bvh bvhA, bvhB;
     build_tri_bvh(A, bvhA);
     build_tri_bvh(B, bvhB);
     auto t0 = std::chrono::steady_clock::now();
+    if (predicate != 1) direction = 0;
+    const bool d12 = direction != 2, d21 = direction != 1;
     PairSet ov;
     {
         std::vector<std::vector<std::pair<int32_t, int32_t>>> per(nthreads());
-        edge_dir(A, B, bvhB, false, per);
-        edge_dir(B, A, bvhA, true, per);
+        if (d12) edge_dir(A, B, bvhB, false, per);
+        if (d21) edge_dir(B, A, bvhA, true, per);
         finish_pairs(per, ov);
     }
     auto cands = box_pairs(A, B);
@@ -862,8 +868,8 @@ or_pairs* or_predicate_join(const or_mesh_view* a, const or_mesh_view* b, int pr
         int32_t x = cands[i].first, y = cands[i].second;
         bool o = has_pair(ov.data, x, y);
         if (predicate == 1) {
-            keep[i] = o || point_in_object(B, bvhB, y, first_vertex(A, x)) ||
-                      point_in_object(A, bvhA, x, first_vertex(B, y));
+            keep[i] = o || (d12 && point_in_object(B, bvhB, y, first_vertex(A, x))) ||
+                      (d21 && point_in_object(A, bvhA, x, first_vertex(B, y)));
         } else {
             if (o) {
                 keep[i] = include_overlap ? 1 : 0;
@@ -882,6 +888,10 @@ or_pairs* or_predicate_join(const or_mesh_view* a, const or_mesh_view* b, int pr
         }
     r->query_us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
     return r;
+}
+
+or_pairs* or_predicate_join(const or_mesh_view* a, const or_mesh_view* b, int predicate, int include_overlap) {
+    return or_predicate_join_dir(a, b, predicate, include_overlap, 0);
 }
 
 // Point-in-object parity for one point (tests).

@@ -65,6 +65,8 @@ def lib():
         L.or_edge_overlap.argtypes = [mv, mv, C.c_int]
         L.or_predicate_join.restype = C.c_void_p
         L.or_predicate_join.argtypes = [mv, mv, C.c_int, C.c_int]
+        L.or_predicate_join_dir.restype = C.c_void_p
+        L.or_predicate_join_dir.argtypes = [mv, mv, C.c_int, C.c_int, C.c_int]
         L.or_bruteforce_overlap.restype = C.c_void_p
         L.or_bruteforce_overlap.argtypes = [mv, mv]
         L.or_point_in_object.restype = C.c_int
@@ -161,10 +163,13 @@ def edge_overlap(m1, m2, direction=0, with_time=False):
     return (pairs, us) if with_time else pairs
 
 
-def predicate_join(m1, m2, predicate, include_overlap=False):
-    """predicate 1 = intersection, 2 = containment; returns (pairs, types)."""
+def predicate_join(m1, m2, predicate, include_overlap=False, direction=0):
+    """predicate 1 = intersection, 2 = containment; returns (pairs, types).
+    direction (intersection only): 0 both, 1 mesh1 edges and points against
+    mesh2, 2 the reverse."""
     v1, v2 = _View(m1), _View(m2)
-    pairs, types, _ = _take(lib().or_predicate_join(C.byref(v1.v), C.byref(v2.v), predicate, int(include_overlap)))
+    pairs, types, _ = _take(lib().or_predicate_join_dir(C.byref(v1.v), C.byref(v2.v), predicate,
+                                                        int(include_overlap), int(direction)))
     return pairs, types
 
 

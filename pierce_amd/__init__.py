@@ -55,6 +55,16 @@ GRID_DTYPE = np.dtype([("ix", "<i4"), ("iy", "<i4"), ("iz", "<i4"), ("center_cou
                        ("touch_count", "<u4"), ("avg_size_mean", "<f4"), ("vol_ratio", "<f4")])
 
 
+class CellStats(C.Structure):
+    """GridCell of pierce/common/include/GridCell.h:5-10 (16 bytes)."""
+    _fields_ = [("center_count", C.c_uint32), ("touch_count", C.c_uint32), ("avg_size_mean", C.c_float),
+                ("vol_ratio", C.c_float)]
+
+
+CELL_DTYPE = np.dtype([("center_count", "<u4"), ("touch_count", "<u4"), ("avg_size_mean", "<f4"),
+                       ("vol_ratio", "<f4")])
+
+
 class GridEstimate(C.Structure):
     _fields_ = [("has_grid", C.c_int32), ("matched_cells", C.c_int32), ("raw_pairs", C.c_double),
                 ("avg_size", C.c_float * 2), ("avg_vol_ratio", C.c_float * 2), ("effective_size", C.c_float * 2),
@@ -106,7 +116,9 @@ class JoinResult(C.Structure):
                 ("edges_traced", C.c_uint64), ("stack_overflows", C.c_uint64), ("index_triangles", C.c_uint64),
                 ("index_edges", C.c_uint64), ("tasks_spilled", C.c_uint64), ("narrow_rounds", C.c_int32),
                 ("graph_launches", C.c_int32), ("hash_accesses", C.c_uint64), ("hash_contentions", C.c_uint64),
-                ("ms_total", C.c_double),
+                ("narrow_fallback_pairs", C.c_uint64), ("walks_started", C.c_uint64), ("pairs_touching", C.c_uint64),
+                ("parity_rays", C.c_uint64), ("parity_crossings", C.c_uint64), ("parity_rays_crossing", C.c_uint64),
+                ("parity_inside", C.c_uint64), ("parity_tests", C.c_uint64), ("ms_total", C.c_double),
                 ("phases", Phase * MAX_PHASES)]
 
 
@@ -156,6 +168,7 @@ _SIGS = [
     ("pz_ctx_destroy", None, [C.c_void_p]),
     ("pz_ctx_set_graphs", None, [C.c_void_p, C.c_int32]),
     ("pz_ctx_mem_info", C.c_int32, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    ("pz_ctx_trim", C.c_int32, [C.c_void_p]),
     ("pz_dataset_upload", C.c_int32, [C.c_void_p, C.POINTER(MeshView), C.POINTER(C.c_void_p)]),
     ("pz_dataset_load_pre", C.c_int32, [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p)]),
     ("pz_dataset_release", None, [C.c_void_p]),
@@ -178,6 +191,8 @@ _SIGS = [
     ("exclusive_scan_gpu", C.c_longlong, [C.c_void_p, C.c_void_p, C.c_int]),
     ("pz_estimate_overlap_selectivity", C.c_float,
      [C.POINTER(GridCell), C.POINTER(GridCell), C.c_int32, C.c_float, C.c_float, C.c_float]),
+    ("pz_estimate_overlap_selectivity_cells", C.c_float,
+     [C.POINTER(CellStats), C.POINTER(CellStats), C.c_int32, C.c_float, C.c_float, C.c_float]),
     ("pz_estimate_grid", C.c_int32,
      [C.POINTER(GridCell), C.c_uint64, C.POINTER(GridCell), C.c_uint64, C.c_float, C.c_float, C.c_float, C.c_int32,
       C.POINTER(GridEstimate)]),
@@ -398,6 +413,17 @@ def estimate_overlap_selectivity(cells_a: np.ndarray, cells_b: np.ndarray, cell_
         cell_volume, epsilon, gamma))
 
 
+def estimate_overlap_selectivity_cells(cells_a: np.ndarray, cells_b: np.ndarray, cell_volume: float,
+                                       epsilon: float = 0.001, gamma: float = 0.8) -> float:
+    """estimateOverlapSelectivity (pierce/query/src/cuda/estimated_overlap.h:6-13) on
+    16-byte GridCell arrays, evaluated on the current HIP device."""
+    a = np.ascontiguousarray(cells_a, CELL_DTYPE)
+    b = np.ascontiguousarray(cells_b, CELL_DTYPE)
+    return float(lib().pz_estimate_overlap_selectivity_cells(
+        a.ctypes.data_as(C.POINTER(CellStats)), b.ctypes.data_as(C.POINTER(CellStats)), len(a), cell_volume,
+        epsilon, gamma))
+
+
 def estimate_grid(cells_a: np.ndarray, cells_b: np.ndarray, cell_size: float, epsilon: float = 0.001,
                   gamma: float = 0.8, alpha_correction: bool = True) -> dict:
     """Whole-set grid estimate and the intermediates the reference's verbose
@@ -440,6 +466,10 @@ class Context:
 
     def set_graphs(self, on: bool) -> None:
         lib().pz_ctx_set_graphs(self._h, 1 if on else 0)
+
+    def trim(self) -> None:
+        """Free the cached device work buffers (pz_ctx_trim)."""
+        _check(lib().pz_ctx_trim(self._h))
 
     def upload(self, mesh: Mesh) -> "Dataset":
         return Dataset(self, mesh)
@@ -513,6 +543,15 @@ class JoinOutput:
     graph_launches: int = 0
     hash_accesses: int = 0
     hash_contentions: int = 0
+    narrow_fallback_pairs: int = 0
+    walks_started: int = 0
+    pairs_touching: int = 0
+    parity_rays: int = 0
+    parity_crossings: int = 0
+    parity_rays_crossing: int = 0
+    parity_inside: int = 0
+    stack_overflows: int = 0
+    parity_tests: int = 0
 
 
 def join_desc(**kw) -> JoinDesc:
@@ -544,7 +583,10 @@ def _output(r: JoinResult) -> JoinOutput:
                       int(r.tests_total), int(r.tests_fp64), int(r.tests_exact), int(r.tests_fp32_misclassified),
                       int(r.edges_traced), int(r.index_triangles), int(r.index_edges), float(r.ms_total), phases,
                       int(r.predicted_pairs), int(r.tasks_spilled), int(r.narrow_rounds), int(r.graph_launches),
-                      int(r.hash_accesses), int(r.hash_contentions))
+                      int(r.hash_accesses), int(r.hash_contentions), int(r.narrow_fallback_pairs),
+                      int(r.walks_started), int(r.pairs_touching), int(r.parity_rays), int(r.parity_crossings),
+                      int(r.parity_rays_crossing), int(r.parity_inside), int(r.stack_overflows),
+                      int(r.parity_tests))
 
 
 def join_batch(ctx: Context, a: Dataset, b: Dataset, queries, **common):

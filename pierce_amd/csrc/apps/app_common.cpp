@@ -174,6 +174,36 @@ double phase_ms(const pz_join_result& r, std::initializer_list<const char*> keys
     return s;
 }
 
+double uncovered_ms(const pz_join_result& r, bool exact, std::initializer_list<const char*> except) {
+    static const char* const kEstimateKeys[] = {
+        "selectivity estimation", "raytrace_hash_mesh1tomesh2", "raytrace_hash_mesh2tomesh1",
+        "raytrace_overlap_hash_mesh1tomesh2", "raytrace_overlap_hash_mesh2tomesh1",
+        "raytrace_containment_hash_mesh1tomesh2", "raytrace_containment_hash_mesh2tomesh1",
+        "compact_hash_table_pairs", "compact_hash_table_pairs (containment)", "compact_hash_table_pairs (overlap)",
+        "download results"};
+    double s = 0;
+    for (int i = 0; i < r.num_phases; ++i) {
+        std::string k;
+        for (const char* ch = r.phases[i].name; *ch; ++ch)
+            k.push_back(static_cast<char>(std::tolower(static_cast<unsigned char>(*ch))));
+        bool covered = false;
+        if (exact) {
+            covered = k.rfind("raytrace_", 0) == 0 || k == "download results" || k == "gpu deduplication";
+        } else {
+            for (const char* e : kEstimateKeys) covered = covered || k == e;
+        }
+        bool skip = covered;
+        for (const char* e : except)
+            if (std::strstr(r.phases[i].name, e)) skip = true;
+        if (!skip) s += r.phases[i].ms;
+    }
+    return s;
+}
+
+void print_join_time(const pz_join_result& r) {
+    std::cout << "Join time (device-resident geometry to sorted host pairs): " << r.ms_total << " ms" << std::endl;
+}
+
 bool write_pairs_csv(const std::string& path, const char* header, const pz_join_result& r) {
     FILE* f = std::fopen(path.c_str(), "w");
     if (!f) {

@@ -141,6 +141,17 @@ void add_phases(Timer& t, const pz_join_result& r);
 
 // Sum of library phases whose names contain any of `keys`.
 double phase_ms(const pz_join_result& r, std::initializer_list<const char*> keys);
+// The join's phases that no query-time key of the tool's benchmark adapter
+// covers, minus those whose name contains one of `except`.  exact == true: the
+// adapter's exact mode, which sums every "raytrace_*" key, "gpu deduplication"
+// and "download results" (pierce_adapter.py:338-342); otherwise the estimate
+// modes' fixed key lists (pierce_adapter.py:343-357,
+// predicates/adapters/intersection.py:207-221, containment.py:163-172).  A tool
+// reports this sum under one of its adapter's raytrace keys, so the adapter's
+// query time covers every device phase of the join.
+double uncovered_ms(const pz_join_result& r, bool exact, std::initializer_list<const char*> except);
+// "Join time (device-resident geometry to sorted host pairs): <ms> ms"
+void print_join_time(const pz_join_result& r);
 
 bool write_pairs_csv(const std::string& path, const char* header, const pz_join_result& r);
 

@@ -75,7 +75,8 @@ int main(int argc, char** argv) {
     }
     timer.next("Output");
     app::add_phases(timer, r);
-    timer.add("Raytrace_Overlap_Hash_Mesh2ToMesh1", app::phase_ms(r, {"Broad", "Narrow"}));
+    // query-time keys of predicates/adapters/containment.py:163-172
+    timer.add("Raytrace_Overlap_Hash_Mesh2ToMesh1", app::uncovered_ms(r, false, {"Parity"}));
     timer.add("Raytrace_Containment_Hash_Mesh2ToMesh1", app::phase_ms(r, {"Parity"}));
     uint64_t ncont = 0, nover = 0;
     for (uint64_t i = 0; i < r.num_pairs; ++i) {
@@ -88,6 +89,7 @@ int main(int argc, char** argv) {
     std::cout << "A objects: " << no1 << std::endl;
     std::cout << "B objects: " << no2 << std::endl;
     std::cout << "Containment pairs (B in A): " << ncont << std::endl;
+    app::print_join_time(r);
     if (include_overlap) {
         std::cout << "Overlap/Touch pairs (A vs B): " << nover << std::endl;
         std::cout << "Reported pairs: " << r.num_pairs << std::endl;

@@ -196,7 +196,7 @@ int main(int argc, char** argv) {
     if (estimate_only) {
         d.sizing = PZ_SIZING_ESTIMATE;
         if (pz_join(ctx, d1, d2, &d, &r) != PZ_OK) return app::fail_join("estimate");
-        std::cout << "Final Estimated Pairs:     " << r.predicted << std::endl;
+        std::cout << "Final Estimated Pairs:     " << r.predicted_pairs << std::endl;
         pz_join_result_release(&r);
         timer.finish(c.output);
         return 0;
@@ -220,13 +220,38 @@ int main(int argc, char** argv) {
     }
     timer.next("Output");
     app::add_phases(timer, r);
-    timer.add("Raytrace_Overlap_Hash_Mesh1ToMesh2", app::phase_ms(r, {"Broad", "Narrow"}));
+    // query-time keys of predicates/adapters/intersection.py:207-221
+    timer.add("Raytrace_Overlap_Hash_Mesh1ToMesh2", app::uncovered_ms(r, false, {"Parity"}));
     timer.add("Raytrace_Containment_Hash_Mesh1ToMesh2", app::phase_ms(r, {"Parity"}));
     std::cout << "Mesh1 objects: " << no1 << std::endl;
     std::cout << "Mesh2 objects: " << no2 << std::endl;
-    std::cout << "Final Estimated Pairs:     " << r.predicted << std::endl;
+    std::cout << "Final Estimated Pairs:     " << r.predicted_pairs << std::endl;
     std::cout << "Actual Intersection Pairs: " << r.num_pairs << std::endl;
+    app::print_join_time(r);
     if (profiling) {
+        // The counters of pierce_intersection_main.cpp:924-948
+        // (MeshIntersectionProfilingStats, mesh_intersection.h:7-22), from this
+        // engine's walk counters.  The reference's iterations are repeated
+        // closest-hit traces per edge; here a walk tests segment-triangle pairs
+        // until the first proven hit, and a parity ray is one traversal.  Caps
+        // and same-hit suppression do not exist here (0).
+        const unsigned long long ov_tests = r.tests_total - r.parity_tests;
+        const unsigned long long walks = r.walks_started, rays = r.parity_rays;
+        timer.counter("Profile_Overlap_Trace_Calls", walks);
+        timer.counter("Profile_Overlap_Iterations_Total", ov_tests);
+        timer.counter("Profile_Overlap_Hits_Total", r.pairs_touching);
+        timer.counter("Profile_Overlap_Max_Iterations_Per_Trace", 0);
+        timer.counter("Profile_Containment_Rays_Total", rays);
+        timer.counter("Profile_Containment_Iterations_Total", rays);
+        timer.counter("Profile_Containment_Hits_Total", r.parity_crossings);
+        timer.counter("Profile_Containment_Max_Iterations_Per_Ray", rays ? 1 : 0);
+        timer.counter("Profile_Containment_Same_Hit_Suppressed", 0);
+        timer.counter("Profile_Containment_Candidate_Additions", r.parity_rays_crossing);
+        timer.counter("Profile_Containment_Candidate_Toggles", r.parity_crossings - r.parity_rays_crossing);
+        timer.counter("Profile_Containment_Candidate_Overflow", 0);
+        timer.counter("Profile_Containment_Targets_Total", r.parity_inside);
+        timer.counter("Profile_Overlap_Avg_Iterations_x1000", walks ? ov_tests * 1000ull / walks : 0ull);
+        timer.counter("Profile_Containment_Avg_Iterations_x1000", rays ? 1000ull : 0ull);
         timer.counter("tests_total", r.tests_total);
         timer.counter("tests_fp64", r.tests_fp64);
         timer.counter("tests_exact", r.tests_exact);

@@ -21,6 +21,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -144,6 +145,24 @@ struct SetAccel {
 
 void init_result(pz_join_result* r) {
     std::memset(r, 0, sizeof(*r));
+}
+
+// Result counters from the device counter block (pzk::kStatSlots values).
+void set_counters(pz_join_result* r, const std::vector<unsigned long long>& st) {
+    r->tests_total = st[0];
+    r->tests_fp64 = st[1];
+    r->tests_exact = st[2];
+    r->tests_fp32_misclassified = st[3];
+    r->edges_traced = st[4];
+    r->hash_accesses = st[5];
+    r->hash_contentions = st[6];
+    r->parity_rays = st[pzk::kStatParityRays];
+    r->parity_crossings = st[pzk::kStatParityRays + 1];
+    r->parity_rays_crossing = st[pzk::kStatParityRays + 2];
+    r->parity_inside = st[pzk::kStatParityRays + 3];
+    r->parity_tests = st[pzk::kStatParityRays + 4];
+    r->walks_started = st[pzk::kStatWalks] + st[4];
+    r->pairs_touching = st[pzk::kStatTouching];
 }
 
 // Final stage shared by both strategies: sort unique keys (+types), copy to host.
@@ -327,8 +346,8 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
                     std::vector<uint64_t>* part_cand_out = nullptr) {
     const bool batched = meta_of_part != nullptr;
     hipStream_t s = c->s;
-    unsigned long long* stats = buf<unsigned long long>(c, "stats", 8);
-    pzk::fill_ull(stats, 0ull, 8, s);
+    unsigned long long* stats = buf<unsigned long long>(c, "stats", pzk::kStatSlots);
+    pzk::fill_ull(stats, 0ull, pzk::kStatSlots, s);
     const int32_t nA = A->nobj, nB = B->nobj;
 
     ph.begin("Object Bounds");
@@ -394,9 +413,13 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
     } else {
         uint64_t cap = d.capacity;
         if (d.sizing == PZ_SIZING_ESTIMATE) {
-            double pred = (double)r->predicted;
-            float margin = d.margin > 0 ? d.margin : 0.25f;
-            cap = (uint64_t)std::ceil(pred * (1.0 + margin)) + 64;
+            // The buffer holds candidate object pairs, not result pairs: size it from
+            // the expected number of box pairs that meet (a 16^3 grid over both sets'
+            // object boxes, the estimator's feature 12), with room for twice the margin.
+            const double pred = candidate_estimate(c, A, B, pa, pb);
+            r->predicted = (uint64_t)std::llround(pred);
+            const float margin = d.margin > 0 ? d.margin : 0.25f;
+            cap = (uint64_t)std::ceil(std::min(pred, 0x1p40) * (1.0 + 2.0 * margin)) + 1024;
         } else if (d.sizing == PZ_SIZING_LEARNED) {
             cap = (uint64_t)std::ceil(std::min(learned_upper, 0x1p40)) + 64;
         }
@@ -600,8 +623,11 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         fb_index = buf<uint32_t>(c, "np.fb_index", np + 1);
         unsigned long long* fbc = buf<unsigned long long>(c, "np.fb_count", 1);
         // after the last round: decide the open leaf pairs, list the fallback pairs
+        // The escalation list then starts empty again, so each open leaf pair is
+        // decided (and counted) once, whatever the number of rounds.
         auto tail = [&]() {
             pzk::escalate(esc, ctr + 2, cap_esc, va, vb, meta, found, stats, s);
+            pzk::fill_ull(ctr + 2, 0ull, 1, s);
             pzk::fill_ull(fbc, 0ull, 1, s);
             pzk::compact_flagged(pairs, np, fallback, fbp, fb_index, fbc, s);
         };
@@ -646,7 +672,7 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
     }
     if (pzk::stack_for_height(height) == 0)
         throw PzError(PZ_E_LIMIT, "per-object BVH height " + std::to_string(height) + " exceeds the traversal stack (63)");
-    r->stack_overflows = d.narrow_mode == 0 ? nfb : 0;  // pairs re-run by the per-edge kernel
+    r->narrow_fallback_pairs = d.narrow_mode == 0 ? nfb : 0;  // pairs re-run by the per-edge kernel
     if (nfb > 0) {
         // Per-edge narrow phase: every edge of the .pre payload regrouped by object.
         auto edges_of = [&](const pz_dataset* D, const SetPrep& P, Needed& N, const std::string& tag) {
@@ -702,18 +728,20 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
     unsigned long long* ocnt = buf<unsigned long long>(c, "sel.cnt", 1);
     pzk::fill_ull(ocnt, 0ull, 1, s);
     // batches keep overlap pairs of containment queries (typed) and filter per query on the host
-    pzk::select_pairs(pairs, np, found, inside, meta, batched ? 1 : d.include_overlap_pairs, okeys, otype, ocnt, s);
-    // the result count and the test counters in one read-back
-    auto st = readback<unsigned long long>(c, {ocnt, stats, stats + 1, stats + 2, stats + 3, stats + 4});
-    const uint64_t nout = st[0];
-    st.erase(st.begin());
+    pzk::select_pairs(pairs, np, found, inside, meta, batched ? 1 : d.include_overlap_pairs, okeys, otype, ocnt,
+                      stats + pzk::kStatTouching, s);
+    // the result count and the counters in one read-back
+    unsigned long long* hs = static_cast<unsigned long long*>(pinned(c, (1 + pzk::kStatSlots) * 8));
+    pzk::HostGather gs{{reinterpret_cast<const uint32_t*>(ocnt), reinterpret_cast<const uint32_t*>(stats)},
+                       {2, 2 * pzk::kStatSlots},
+                       2};
+    pzk::gather_to_host(gs, reinterpret_cast<uint32_t*>(hs), s);
+    HIP_OK(hipStreamSynchronize(s));
+    const std::vector<unsigned long long> st(hs + 1, hs + 1 + pzk::kStatSlots);
+    const uint64_t nout = hs[0];
     const bool want_types = batched || (d.predicate == PZ_PRED_CONTAINMENT && d.include_overlap_pairs);
     finish_pairs(c, okeys, want_types ? otype : nullptr, nout, nA, want_types, r);
-    r->tests_total = st[0];
-    r->tests_fp64 = st[1];
-    r->tests_exact = st[2];
-    r->edges_traced = st[4];
-    r->tests_fp32_misclassified = st[3];
+    set_counters(r, st);
 }
 
 // ------------------------------------------------------------- global
@@ -762,8 +790,8 @@ void join_global(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const pz_j
     if (d.predicate != PZ_PRED_OVERLAP)
         throw PzError(PZ_E_INVALID, "the global strategy evaluates the overlap predicate only");
     hipStream_t s = c->s;
-    unsigned long long* stats = buf<unsigned long long>(c, "stats", 8);
-    pzk::fill_ull(stats, 0ull, 8, s);
+    unsigned long long* stats = buf<unsigned long long>(c, "stats", pzk::kStatSlots);
+    pzk::fill_ull(stats, 0ull, pzk::kStatSlots, s);
     ph.begin("Build Mesh1 Index");
     SetPrep pa = prep_set(c, A, "A"), pb = prep_set(c, B, "B");
     // LEARNED: features and inference overlap the set-wide index builds on the side stream
@@ -879,15 +907,11 @@ void join_global(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const pz_j
     }
     ph.begin("Download Results");
     finish_pairs(c, uniq, nullptr, nuniq, A->nobj, false, r);
-    auto st = readback<unsigned long long>(c, {stats, stats + 1, stats + 2, stats + 3, stats + 4, stats + 5,
-                                                stats + 6});
-    r->tests_total = st[0];
-    r->tests_fp64 = st[1];
-    r->tests_exact = st[2];
-    r->edges_traced = st[4];
-    r->tests_fp32_misclassified = st[3];
-    r->hash_accesses = st[5];
-    r->hash_contentions = st[6];
+    std::vector<unsigned long long> st(pzk::kStatSlots);
+    HIP_OK(hipMemcpyAsync(st.data(), stats, 8 * pzk::kStatSlots, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    set_counters(r, st);
+    r->pairs_touching = nuniq;
 }
 
 }  // namespace
@@ -932,6 +956,25 @@ pz_status pz_ctx_mem_info(pz_ctx* c, uint64_t* free_bytes, uint64_t* total_bytes
     });
 }
 
+pz_status pz_ctx_trim(pz_ctx* c) {
+    if (!c) return fail(PZ_E_INVALID, "pz_ctx_trim: null context");
+    return guarded([&]() -> pz_status {
+        HIP_OK(hipSetDevice(c->device));
+        HIP_OK(hipStreamSynchronize(c->s));
+        HIP_OK(hipStreamSynchronize(c->s2));
+        for (GraphSeg* g : c->segs) {
+            drop_graph(g);
+            delete g;
+        }
+        c->segs.clear();
+        for (auto& kv : c->bufs)
+            if (kv.second.p) HIP_OK(hipFree(kv.second.p));
+        c->bufs.clear();
+        ++c->pool_epoch;
+        return PZ_OK;
+    });
+}
+
 void pz_ctx_destroy(pz_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
@@ -943,6 +986,7 @@ void pz_ctx_destroy(pz_ctx* c) {
     }
     for (auto& kv : c->bufs)
         if (kv.second.p) (void)hipFree(kv.second.p);
+    c->bufs.clear();
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->pinned_est) (void)hipHostFree(c->pinned_est);
     for (auto& e : c->ev) (void)hipEventDestroy(e);
@@ -1089,10 +1133,14 @@ pz_status pz_join(pz_ctx* c, const pz_dataset* a, const pz_dataset* b, const pz_
         double t0 = now_ms();
         Phases ph(c, r);
         if (d.sizing == PZ_SIZING_ESTIMATE) {
+            // the reference's grid estimate of result pairs; the global strategy
+            // sizes its hash set with it, the two-level one sizes its candidate
+            // buffer from a candidate estimate of its own
             ph.begin("Selectivity Estimation");
             double est = grid_estimate(a, b, d.gamma > 0 ? d.gamma : 0.8f, d.epsilon > 0 ? d.epsilon : 0.001f,
                                        !d.no_alpha_correction);
-            r->predicted = est > 0 ? (uint64_t)est : 0;
+            r->predicted_pairs = est > 0 ? (uint64_t)est : 0;
+            r->predicted = r->predicted_pairs;
         }
         if (d.strategy == PZ_STRATEGY_GLOBAL)
             join_global(c, a, b, d, r, ph);
@@ -1118,7 +1166,7 @@ pz_status pz_count_candidate_tests(pz_ctx* c, const pz_dataset* a, const pz_data
     return guarded([&]() -> pz_status {
         HIP_OK(hipSetDevice(c->device));
         hipStream_t s = c->s;
-        unsigned long long* stats = buf<unsigned long long>(c, "stats", 8);
+        unsigned long long* stats = buf<unsigned long long>(c, "stats", pzk::kStatSlots);
         SetPrep pa = prep_set(c, a, "A"), pb = prep_set(c, b, "B");
         GlobalBvh ga = build_global(c, a, pa, "A");
         GlobalBvh gb = build_global(c, b, pb, "B");
@@ -1156,8 +1204,8 @@ pz_status pz_track_containment_rays(pz_ctx* c, const pz_dataset* src, const pz_d
         std::fill(candidate_overflow, candidate_overflow + ns, 0u);
         std::fill(iterations, iterations + ns, 0u);
         if (ns == 0) return PZ_OK;
-        unsigned long long* stats = buf<unsigned long long>(c, "stats", 8);
-        pzk::fill_ull(stats, 0ull, 8, s);
+        unsigned long long* stats = buf<unsigned long long>(c, "stats", pzk::kStatSlots);
+        pzk::fill_ull(stats, 0ull, pzk::kStatSlots, s);
         SetPrep ps = prep_set(c, src, "A");
         SetPrep pt = prep_set(c, tgt, "B");
         GlobalBvh gt{};
@@ -1458,79 +1506,154 @@ pz_status pz_profile_query(const char* kernel, double* total_ms, uint64_t* launc
 }
 
 // ----------------------------------------------- reference extern "C" helpers
-long long merge_and_deduplicate_pairs_gpu(const pz_pair* d1, long long n1, const pz_pair* d2, long long n2,
-                                          pz_pair* d_out) {
-    if (n1 < 0 || n2 < 0 || (n1 && !d1) || (n2 && !d2) || !d_out) return -1;
-    uint64_t n = (uint64_t)(n1 + n2);
-    void* mem = nullptr;
-    size_t bytes = 3 * (n + 1) * 8 + pzk::radix_sort_temp_bytes(n) + pzk::unique_temp_bytes(n) + 1024;
-    if (hipMalloc(&mem, bytes) != hipSuccess) return -1;
-    char* p = reinterpret_cast<char*>(mem);
-    uint64_t* k = reinterpret_cast<uint64_t*>(p);
+}  // extern "C"
+
+namespace {
+// One device workspace per HIP device for the reference-named helpers: grown
+// on demand, kept for later calls, returned when the library is unloaded.
+// The helpers run on the current device and the null stream and block until
+// done, so one mutex serialises them.
+struct HelperSpace {
+    std::mutex mu;
+    std::vector<DevBuf> dev;
+    ~HelperSpace() {
+        for (DevBuf& b : dev)
+            if (b.p) (void)hipFree(b.p);
+    }
+};
+HelperSpace& helper_space() {
+    static HelperSpace h;
+    return h;
+}
+// Workspace of at least `bytes` on the current device (caller holds the mutex).
+char* helper_ws(size_t bytes) {
+    HelperSpace& h = helper_space();
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return nullptr;
+    if ((size_t)dev >= h.dev.size()) h.dev.resize((size_t)dev + 1);
+    DevBuf& b = h.dev[(size_t)dev];
+    if (b.n < bytes) {
+        if (b.p) (void)hipFree(b.p);
+        b.p = nullptr;
+        b.n = 0;
+        const size_t want = std::max<size_t>(bytes + bytes / 4, 1 << 20);
+        if (hipMalloc(&b.p, want) != hipSuccess) return nullptr;
+        b.n = want;
+    }
+    return static_cast<char*>(b.p);
+}
+inline size_t up256(size_t x) { return (x + 255) & ~size_t(255); }
+}  // namespace
+
+namespace pz {
+// Scratch layout of the merge: keys, sorted keys, unique keys (n + 1 u64
+// each), then the radix sort's and the unique pass's temporaries.
+size_t merge_scratch_bytes(uint64_t n) {
+    return up256(3 * (n + 1) * 8) + up256(pzk::radix_sort_temp_bytes(n)) + up256(pzk::unique_temp_bytes(n));
+}
+// Sorted unique merge of s1[0..n1) and s2[0..n2) into out through `scratch`
+// (merge_scratch_bytes(n1 + n2) bytes of device memory) on stream s; the
+// sources may alias out (their keys are copied into scratch first).
+// Returns the unique count, or -1 on a HIP failure.
+long long merge_pairs_into(const pz_pair* s1, uint64_t n1, const pz_pair* s2, uint64_t n2, pz_pair* out,
+                           char* scratch, hipStream_t s) {
+    const uint64_t n = n1 + n2;
+    uint64_t* k = reinterpret_cast<uint64_t*>(scratch);
     uint64_t* k2 = k + (n + 1);
     uint64_t* u = k2 + (n + 1);
-    char* tmp = reinterpret_cast<char*>(u + (n + 1));
-    tmp = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(tmp) + 255) & ~uintptr_t(255));
-    char* utmp = tmp + pzk::radix_sort_temp_bytes(n);
-    utmp = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(utmp) + 255) & ~uintptr_t(255));
-    hipStream_t s = nullptr;
-    pzk::pairs_to_keys(reinterpret_cast<const int32_t*>(d1), (uint64_t)n1, k, s);
-    pzk::pairs_to_keys(reinterpret_cast<const int32_t*>(d2), (uint64_t)n2, k + n1, s);
-    bool alt = pzk::radix_sort_keys(k, k2, n, 0, 64, tmp, s);
+    char* tmp = scratch + up256(3 * (n + 1) * 8);
+    char* utmp = tmp + up256(pzk::radix_sort_temp_bytes(n));
+    pzk::pairs_to_keys(reinterpret_cast<const int32_t*>(s1), n1, k, s);
+    pzk::pairs_to_keys(reinterpret_cast<const int32_t*>(s2), n2, k + n1, s);
+    const bool alt = pzk::radix_sort_keys(k, k2, n, 0, 64, tmp, s);
     uint64_t* sk = alt ? k2 : k;
-    uint64_t* dcount = reinterpret_cast<uint64_t*>(alt ? k : k2);  // free buffer slot for the count
+    uint64_t* dcount = alt ? k : k2;  // the sort's other buffer is free again: its first slot holds the count
     pzk::unique_u64(sk, n, u, dcount, utmp, s);
     uint64_t cnt = 0;
-    long long result = -1;
-    if (hipMemcpy(&cnt, dcount, 8, hipMemcpyDeviceToHost) == hipSuccess) {
-        pzk::keys_to_pairs(u, cnt, reinterpret_cast<int32_t*>(d_out), s);
-        if (hipDeviceSynchronize() == hipSuccess) result = (long long)cnt;
-    }
-    (void)hipFree(mem);
-    return result;
+    if (hipMemcpyAsync(&cnt, dcount, 8, hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
+    if (hipStreamSynchronize(s) != hipSuccess) return -1;
+    pzk::keys_to_pairs(u, cnt, reinterpret_cast<int32_t*>(out), s);
+    if (hipStreamSynchronize(s) != hipSuccess) return -1;
+    return (long long)cnt;
+}
+}  // namespace pz
+
+extern "C" {
+
+long long merge_and_deduplicate_pairs_gpu(const pz_pair* d1, long long n1, const pz_pair* d2, long long n2,
+                                          pz_pair* d_out) {
+    if (n1 < 0 || n2 < 0) return -1;
+    const uint64_t n = (uint64_t)n1 + (uint64_t)n2;
+    if (n == 0) return 0;
+    if (!d_out) return -1;
+    // a null source with a count: those pairs already sit in the output
+    // (mesh_query_deduplication.cu:95-106 copies only non-null sources)
+    const pz_pair* s1 = d1 ? d1 : d_out;
+    const pz_pair* s2 = d2 ? d2 : d_out + n1;
+    std::lock_guard<std::mutex> lk(helper_space().mu);
+    char* ws = helper_ws(pz::merge_scratch_bytes(n));
+    if (!ws) return -1;
+    return pz::merge_pairs_into(s1, (uint64_t)n1, s2, (uint64_t)n2, d_out, ws, nullptr);
 }
 
 int compact_hash_table_pairs(const unsigned long long* d_table, unsigned long long size, pz_pair* d_output,
                              int max_output_size) {
     if (!d_table || !d_output || max_output_size < 0) return -1;
-    void* mem = nullptr;
-    uint64_t cap = (uint64_t)max_output_size;
-    if (hipMalloc(&mem, (cap + 2) * 8) != hipSuccess) return -1;
-    unsigned long long* cnt = reinterpret_cast<unsigned long long*>(mem);
+    const uint64_t cap = (uint64_t)max_output_size;
+    std::lock_guard<std::mutex> lk(helper_space().mu);
+    char* ws = helper_ws((cap + 2) * 8);
+    if (!ws) return -1;
+    unsigned long long* cnt = reinterpret_cast<unsigned long long*>(ws);
     uint64_t* keys = reinterpret_cast<uint64_t*>(cnt + 1);
     hipStream_t s = nullptr;
     pzk::fill_ull(cnt, 0ull, 1, s);
     pzk::compact_table(d_table, size, keys, cap, cnt, s);
     unsigned long long total = 0;
-    int result = -1;
-    if (hipMemcpy(&total, cnt, 8, hipMemcpyDeviceToHost) == hipSuccess) {
-        uint64_t n = std::min<uint64_t>(total, cap);
-        pzk::keys_to_pairs(keys, n, reinterpret_cast<int32_t*>(d_output), s);
-        if (hipDeviceSynchronize() == hipSuccess) result = (int)n;
-    }
-    (void)hipFree(mem);
-    return result;
+    if (hipMemcpyAsync(&total, cnt, 8, hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
+    if (hipStreamSynchronize(s) != hipSuccess) return -1;
+    const uint64_t n = std::min<uint64_t>(total, cap);
+    pzk::keys_to_pairs(keys, n, reinterpret_cast<int32_t*>(d_output), s);
+    if (hipStreamSynchronize(s) != hipSuccess) return -1;
+    return (int)n;
 }
 
 long long exclusive_scan_gpu(const int* d_input, long long* d_output, int num_elements) {
     if (num_elements < 0 || (num_elements && (!d_input || !d_output))) return -1;
-    void* mem = nullptr;
-    uint64_t n = (uint64_t)num_elements;
-    // out needs n + 1 slots for the total; the caller's array holds n.
-    size_t bytes = (n + 1) * 8 + pzk::scan_temp_bytes(n) + 512;
-    if (hipMalloc(&mem, bytes) != hipSuccess) return -1;
-    int64_t* out = reinterpret_cast<int64_t*>(mem);
-    char* tmp = reinterpret_cast<char*>(out + n + 1);
-    tmp = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(tmp) + 255) & ~uintptr_t(255));
-    pzk::exclusive_scan_i32_i64(reinterpret_cast<const int32_t*>(d_input), out, n, tmp, nullptr);
+    const uint64_t n = (uint64_t)num_elements;
+    std::lock_guard<std::mutex> lk(helper_space().mu);
+    // the scan writes n + 1 entries (the total last); the caller's array holds n
+    char* ws = helper_ws(up256((n + 1) * 8) + pzk::scan_temp_bytes(n));
+    if (!ws) return -1;
+    int64_t* out = reinterpret_cast<int64_t*>(ws);
+    char* tmp = ws + up256((n + 1) * 8);
+    hipStream_t s = nullptr;
+    pzk::exclusive_scan_i32_i64(reinterpret_cast<const int32_t*>(d_input), out, n, tmp, s);
+    if (n && hipMemcpyAsync(d_output, out, n * 8, hipMemcpyDeviceToDevice, s) != hipSuccess) return -1;
     long long total = -1;
-    if (n && hipMemcpy(d_output, out, n * 8, hipMemcpyDeviceToDevice) != hipSuccess) {
-        (void)hipFree(mem);
-        return -1;
-    }
-    if (hipMemcpy(&total, out + n, 8, hipMemcpyDeviceToHost) != hipSuccess) total = -1;
-    (void)hipFree(mem);
+    if (hipMemcpyAsync(&total, out + n, 8, hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
+    if (hipStreamSynchronize(s) != hipSuccess) return -1;
     return total;
+}
+
+float pz_estimate_overlap_selectivity_cells(const pz_cell_stats* a, const pz_cell_stats* b, int32_t n,
+                                            float cell_volume, float epsilon, float gamma) {
+    if (n < 0 || (n && (!a || !b))) return -1.0f;
+    if (n == 0) return 0.0f;
+    const size_t cb = up256((size_t)n * sizeof(pz_cell_stats));
+    std::lock_guard<std::mutex> lk(helper_space().mu);
+    char* ws = helper_ws(2 * cb + 256);
+    if (!ws) return -1.0f;
+    hipStream_t s = nullptr;
+    double* total = reinterpret_cast<double*>(ws + 2 * cb);
+    if (hipMemcpyAsync(ws, a, (size_t)n * sizeof(pz_cell_stats), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(ws + cb, b, (size_t)n * sizeof(pz_cell_stats), hipMemcpyHostToDevice, s) != hipSuccess)
+        return -1.0f;
+    pzk::cell_selectivity(reinterpret_cast<const uint32_t*>(ws), reinterpret_cast<const uint32_t*>(ws + cb),
+                          (uint32_t)n, cell_volume, epsilon, gamma, total, s);
+    double h = 0;
+    if (hipMemcpyAsync(&h, total, 8, hipMemcpyDeviceToHost, s) != hipSuccess) return -1.0f;
+    if (hipStreamSynchronize(s) != hipSuccess) return -1.0f;
+    return (float)h;
 }
 
 float pz_estimate_overlap_selectivity(const pz_grid_cell* a, const pz_grid_cell* b, int32_t n, float cell_volume,

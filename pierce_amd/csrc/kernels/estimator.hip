@@ -642,9 +642,52 @@ __global__ void k_mlp_step_end(const float* __restrict__ work, uint32_t B, float
     ctr[1] += 1u;
 }
 
+// ------------------------------------------------------ grid-cell estimate
+// Per matched cell (pierce/query/src/cuda/estimated_overlap.cu:8-41, in the
+// same float operations): TA * TB * min(1, (sA + sB + eps)^3 / V * sqrt(rA rB)^gamma),
+// 0 when either touch count is 0.  Cells are GridCell words {CenterCount,
+// TouchCount, AvgSizeMean, VolRatio}.  One block sums the terms in double:
+// each thread over a fixed stride, then a fixed tree, so the total does not
+// depend on scheduling.
+#pragma clang fp contract(off)
+constexpr int kCellBlock = 1024;
+__global__ __launch_bounds__(kCellBlock) void k_cell_selectivity(const uint32_t* __restrict__ ga,
+                                                                 const uint32_t* __restrict__ gb, uint32_t n,
+                                                                 float cell_volume, float eps, float gamma,
+                                                                 double* __restrict__ total) {
+    __shared__ double part[kCellBlock];
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += kCellBlock) {
+        const uint32_t ta = ga[4ull * i + 1], tb = gb[4ull * i + 1];
+        if (ta == 0u || tb == 0u) continue;
+        const float sa = __uint_as_float(ga[4ull * i + 2]), sb = __uint_as_float(gb[4ull * i + 2]);
+        const float ra = __uint_as_float(ga[4ull * i + 3]), rb = __uint_as_float(gb[4ull * i + 3]);
+        const float comb = sa + sb + eps;
+        float prob = comb * comb * comb / cell_volume;
+        prob *= powf(sqrtf(ra * rb), gamma);
+        prob = fminf(prob, 1.0f);
+        acc += (double)((float)ta * (float)tb * prob);
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kCellBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = part[0];
+}
+#pragma clang fp contract(on)
+
 }  // namespace
 
 namespace pzk {
+
+void cell_selectivity(const uint32_t* cells_a, const uint32_t* cells_b, uint32_t n, float cell_volume, float eps,
+                      float gamma, double* total, hipStream_t s) {
+    hipLaunchKernelGGL(k_cell_selectivity, dim3(1), dim3(kCellBlock), 0, s, cells_a, cells_b, n, cell_volume, eps,
+                       gamma, total);
+}
+
 
 size_t features_scratch_bytes(uint32_t nparts) {
     return ((sizeof(PartAcc) * 2 * nparts + 255) / 256) * 256 + sizeof(CellAcc) * 2 * (size_t)kFeatCells * nparts +

@@ -811,6 +811,12 @@ __device__ __forceinline__ int tri_pair_f32(const LeafTri& X, const LeafTri& Y, 
     return open ? -1 : 0;
 }
 
+// found[] is written by other waves while a walk runs: read it with a relaxed
+// device-scope atomic load, so every check fetches the current value.
+__device__ __forceinline__ bool found_now(const uint32_t* found, uint32_t p) {
+    return __hip_atomic_load(found + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+}
+
 // With F32 the walk runs the fp32 filter only: a leaf pair it cannot decide
 // goes to esc_list (pair, mesh1 leaf, mesh2 leaf) for k_escalate, so the
 // fp64 and exact stages (and their registers) stay out of this kernel; if
@@ -827,6 +833,7 @@ __global__ __launch_bounds__(kDualWaves * PZ_WAVE, 2) void k_narrow_dual(
     const int lane = __lane_id(), w = threadIdx.x / PZ_WAVE;
     int2* Q = q[w];  // ring buffer: entries [bot, top) modulo QLEN, LIFO at the top
     LaneStats st{0, 0, 0, 0, 0};
+    uint32_t walks = 0;  // tasks this wave walked
     while (true) {
         unsigned long long t0 = 0;
         if (lane == 0) t0 = atomicAdd(next_task, (unsigned long long)chunk);
@@ -848,7 +855,8 @@ __global__ __launch_bounds__(kDualWaves * PZ_WAVE, 2) void k_narrow_dual(
                 const uint32_t a0 = A.seg_off[la], a1 = A.seg_off[la + 1], b0 = B.seg_off[lb], b1 = B.seg_off[lb + 1];
                 root = make_int2((a1 - a0 >= 2) ? (int)a0 : ~(int)a0, (b1 - b0 >= 2) ? (int)b0 : ~(int)b0);
             }
-            if (found[p]) continue;  // another task of this pair already proved a hit
+            if (found_now(found, p)) continue;  // another task of this pair already proved a hit
+            ++walks;
             const int dir_mask = meta[p] >> 2;
             if (lane == 0) Q[0] = root;
             __builtin_amdgcn_wave_barrier();
@@ -856,7 +864,7 @@ __global__ __launch_bounds__(kDualWaves * PZ_WAVE, 2) void k_narrow_dual(
             bool hit = false, ovf = false, esc_full = false;
             int iter = 0;
             while (top > bot) {
-                if ((++iter & 15) == 0 && found[p]) break;
+                if ((++iter & 15) == 0 && found_now(found, p)) break;
                 const int qn = top - bot;
                 const int take = qn < PZ_WAVE ? qn : PZ_WAVE;
                 const bool act = lane < take;
@@ -977,6 +985,7 @@ __global__ __launch_bounds__(kDualWaves * PZ_WAVE, 2) void k_narrow_dual(
         }
     }
     flush_stats(st, stats);
+    if (lane == 0 && walks) atomicAdd(&stats[pzk::kStatWalks], (unsigned long long)walks);
 }
 
 __global__ void k_mark_task_pairs(const DualTask* __restrict__ tasks, uint64_t n, uint32_t* __restrict__ fallback) {
@@ -1055,6 +1064,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_parity(const uint64_t* __restrict
     float3 pt = make_float3(0, 0, 0);
     int cur = 0, sp = 0;
     bool odd = false;
+    const bool cast = act;
+    uint32_t crossings = 0;
     const SetView* T = dir == 0 ? &A : &B;
     int32_t ot = 0;
     if (act) {
@@ -1099,7 +1110,10 @@ __global__ __launch_bounds__(kBlock, 4) void k_parity(const uint64_t* __restrict
             Esc esc{0};
             float3 a = make_float3(L.v0.x, L.v0.y, L.v0.z), b = make_float3(L.v1.x, L.v1.y, L.v1.z),
                    c = make_float3(L.v2.x, L.v2.y, L.v2.z);
-            if (ray_cross(pt, a, b, c, esc)) odd = !odd;
+            if (ray_cross(pt, a, b, c, esc)) {
+                odd = !odd;
+                ++crossings;
+            }
             ++st.tests;
             if (esc.level == 2) ++st.exact;
         }
@@ -1109,6 +1123,12 @@ __global__ __launch_bounds__(kBlock, 4) void k_parity(const uint64_t* __restrict
     }
     if (p < np && odd) atomicOr(&inside[p], 1u << dir);
     flush_stats(st, stats);
+    // rays cast, triangles crossed, rays crossing something, points inside, tests
+    const uint32_t pv[5] = {cast ? 1u : 0u, crossings, crossings ? 1u : 0u, odd ? 1u : 0u, st.tests};
+    for (int k = 0; k < 5; ++k) {
+        const uint32_t sum = wave_sum(pv[k]);
+        if (lane == 0 && sum) atomicAdd(&stats[pzk::kStatParityRays + k], (unsigned long long)sum);
+    }
 }
 
 // Containment-ray tracking (pierce_intersection --enable-tracking,
@@ -1202,10 +1222,13 @@ __global__ __launch_bounds__(kBlock) void k_select_pairs(const uint64_t* __restr
                                                          const uint8_t* __restrict__ meta,
                                                          int include_overlap, uint64_t* __restrict__ out_keys,
                                                          uint32_t* __restrict__ out_type,
-                                                         unsigned long long* __restrict__ count) {
+                                                         unsigned long long* __restrict__ count,
+                                                         unsigned long long* __restrict__ touching) {
     uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     uint32_t type = 0;
     bool keep = false;
+    const uint64_t mt = __ballot(i < np && found[i] != 0);  // pairs proven touching
+    if (mt && __lane_id() == 0) atomicAdd(touching, (unsigned long long)__popcll(mt));
     if (i < np) {
         const int predicate = meta[i] & 3;
         bool ov = found[i] != 0;
@@ -1700,10 +1723,10 @@ bool track_rays(const float* vS, const uint32_t* iS, const uint32_t* ftS, const 
 
 void select_pairs(const uint64_t* pairs, uint64_t np, const uint32_t* found, const uint32_t* inside,
                   const uint8_t* meta, int include_overlap, uint64_t* out_keys, uint32_t* out_type,
-                  unsigned long long* count, hipStream_t s) {
+                  unsigned long long* count, unsigned long long* touching, hipStream_t s) {
     if (np)
         hipLaunchKernelGGL(k_select_pairs, dim3(grid1(np)), dim3(kBlock), 0, s, pairs, np, found, inside, meta,
-                           include_overlap, out_keys, out_type, count);
+                           include_overlap, out_keys, out_type, count, touching);
 }
 
 bool trace_global(int mode, const float* es, const float* ee, const int32_t* eo, uint64_t ne, const Node* nodes,

@@ -38,6 +38,12 @@ void join_features(const uint32_t* oboxA, const uint32_t* triA, int32_t nA, cons
                    const uint32_t* oboxB, const uint32_t* triB, int32_t nB, const uint32_t* partB, uint32_t nparts,
                    const uint8_t* meta_of_part, uint8_t meta, void* scratch, float* X, hipStream_t s);
 
+// Grid-cell estimate of pierce/query/src/cuda/estimated_overlap.cu:8-77 over n
+// matched cells (GridCell layout, 4 words each): *total (device) = the sum of
+// the per-cell terms in double, in a fixed order.
+void cell_selectivity(const uint32_t* cells_a, const uint32_t* cells_b, uint32_t n, float cell_volume, float eps,
+                      float gamma, double* total, hipStream_t s);
+
 // Candidate pairs per part of mesh1 (pairs packed mesh1 << 32 | mesh2); counts zeroed here.
 void part_pair_counts(const uint64_t* pairs, uint64_t np, const uint32_t* part_of_a, uint32_t nparts,
                       unsigned long long* counts, hipStream_t s);

@@ -25,6 +25,14 @@ struct DualTask {
 
 namespace pzk {
 
+// Device counters of one join (u64 each): 0 tests, 1 decided in fp64,
+// 2 decided exactly, 3 fp32 misclassified, 4 edges traced, 5 hash-set slots
+// probed, 6 probed slots holding another pair, 7 parity rays cast, 8 triangles
+// they crossed, 9 rays crossing at least one, 10 rays ending inside, 11 their
+// ray-triangle tests, 12 dual-tree walks, 13 candidate pairs proven touching.
+constexpr int kStatSlots = 16;
+constexpr int kStatParityRays = 7, kStatWalks = 12, kStatTouching = 13;
+
 using pzd::DualTask;
 using pzd::EdgeRec;
 using pzd::LeafTri;
@@ -172,7 +180,7 @@ bool track_rays(const float* vS, const uint32_t* iS, const uint32_t* ftS, const 
                 unsigned long long* stats, hipStream_t s);
 void select_pairs(const uint64_t* pairs, uint64_t np, const uint32_t* found, const uint32_t* inside,
                   const uint8_t* meta, int include_overlap, uint64_t* out_keys, uint32_t* out_type,
-                  unsigned long long* count, hipStream_t s);
+                  unsigned long long* count, unsigned long long* touching, hipStream_t s);
 bool trace_global(int mode, const float* es, const float* ee, const int32_t* eo, uint64_t ne, const Node* nodes,
                   const LeafTri* leaves, uint64_t ntri, uint32_t height, int swap, bool f32, int allow_f32,
                   uint32_t* counts, const uint64_t* off, uint64_t* out, unsigned long long* table, uint64_t mask,

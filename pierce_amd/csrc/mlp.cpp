@@ -161,6 +161,14 @@ uint32_t pz::mlp_predict_async(pz_ctx* c, const pz_mlp* model, const pz_dataset*
     return np;
 }
 
+double pz::candidate_estimate(pz_ctx* c, const pz_dataset* a, const pz_dataset* b, const SetPrep& pa,
+                              const SetPrep& pb) {
+    uint32_t np = 0;
+    const float* X = features_device(c, a, b, pa, pb, nullptr, 0, &np, c->s);
+    const float f = readback<float>(c, {X + 12})[0];  // log2(1 + expected box pairs)
+    return std::exp2(std::min(std::max((double)f, 0.0), 60.0)) - 1.0;
+}
+
 std::vector<SizePrediction> pz::mlp_predict_wait(pz_ctx* c, const pz_mlp* model, uint32_t np) {
     HIP_OK(hipEventSynchronize(c->ev_join));
     const float* h = static_cast<const float*>(c->pinned_est);

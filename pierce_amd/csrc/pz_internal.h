@@ -96,7 +96,8 @@ inline T* buf(pz_ctx* c, const std::string& name, size_t count) {
         if (b.p) HIP_OK(hipFree(b.p));
         b.p = nullptr;
         b.n = 0;
-        size_t want = bytes + bytes / 4;
+        // headroom for growth, at most 1 GiB (large joins would otherwise waste a quarter of HBM)
+        size_t want = bytes + std::min<size_t>(bytes / 4, size_t(1) << 30);
         HIP_OK(hipMalloc(&b.p, want));
         b.n = want;
         ++c->pool_epoch;
@@ -195,4 +196,15 @@ struct SizePrediction {
 uint32_t mlp_predict_async(pz_ctx* c, const pz_mlp* model, const pz_dataset* a, const pz_dataset* b,
                            const SetPrep& pa, const SetPrep& pb, const uint8_t* meta_of_part, uint8_t meta);
 std::vector<SizePrediction> mlp_predict_wait(pz_ctx* c, const pz_mlp* model, uint32_t nparts);
+// Expected number of object pairs whose boxes meet (plain datasets): the
+// 16^3-grid box-pair expectation of the estimator's features, on the main stream.
+double candidate_estimate(pz_ctx* c, const pz_dataset* a, const pz_dataset* b, const SetPrep& pa, const SetPrep& pb);
+
+// Sorted unique merge of the reference's merge_and_deduplicate_pairs_gpu
+// (capi.cpp): scratch bytes for n input pairs, and the merge of s1[0..n1) and
+// s2[0..n2) into out through that scratch on stream s (sources may alias out).
+// Returns the unique count, or -1 on a HIP failure.
+size_t merge_scratch_bytes(uint64_t n);
+long long merge_pairs_into(const pz_pair* s1, uint64_t n1, const pz_pair* s2, uint64_t n2, pz_pair* out,
+                           char* scratch, hipStream_t s);
 }  // namespace pz

@@ -178,3 +178,67 @@ def test_hash_counters_in_the_result(ctx, pre_files):
     t = pz.join(ctx, da, db)  # two-level: no hash set
     assert t.hash_accesses == 0 and t.hash_contentions == 0
     np.testing.assert_array_equal(g.pairs, t.pairs)
+
+
+# Query-time sums of the reference adapters, per tool: the phase keys each one
+# adds up (pierce_adapter.py:338-376 for the exact and direct_estimation modes,
+# predicates/adapters/intersection.py:207-221, containment.py:163-172).
+ADAPTER_KEYS = {
+    "pierce_overlap": ["selectivity estimation", "raytrace_hash_mesh1tomesh2", "raytrace_hash_mesh2tomesh1",
+                       "raytrace_overlap_hash_mesh1tomesh2", "raytrace_overlap_hash_mesh2tomesh1",
+                       "compact_hash_table_pairs", "download results"],
+    "pierce_overlap_two_pass": None,  # every raytrace_* key + gpu deduplication + download results
+    "pierce_intersection": ["selectivity estimation", "raytrace_hash_mesh1tomesh2", "raytrace_hash_mesh2tomesh1",
+                            "raytrace_overlap_hash_mesh1tomesh2", "raytrace_overlap_hash_mesh2tomesh1",
+                            "raytrace_containment_hash_mesh1tomesh2", "raytrace_containment_hash_mesh2tomesh1",
+                            "compact_hash_table_pairs", "download results"],
+    "pierce_containment": ["selectivity estimation", "raytrace_overlap_hash_mesh1tomesh2",
+                           "raytrace_overlap_hash_mesh2tomesh1", "raytrace_containment_hash_mesh1tomesh2",
+                           "raytrace_containment_hash_mesh2tomesh1", "compact_hash_table_pairs (containment)",
+                           "compact_hash_table_pairs (overlap)", "download results"],
+}
+
+
+def adapter_query_ms(tool, phases):
+    """The adapter's query time from a timing JSON (keys lower-cased with a
+    _<n> suffix, summed per name), using duration_us for precision."""
+    vals = {}
+    for k, v in phases.items():
+        name = re.sub(r"_\d+$", "", k.lower())
+        vals[name] = vals.get(name, 0.0) + v["duration_us"] / 1000.0
+    keys = ADAPTER_KEYS[tool]
+    if keys is None:
+        return sum(v for k, v in vals.items() if k.startswith("raytrace_")) + vals.get("gpu deduplication", 0.0) + \
+            vals.get("download results", 0.0)
+    return sum(vals.get(k, 0.0) for k in keys)
+
+
+@pytest.fixture(scope="module")
+def larger_pre(tmp_path_factory):
+    d = tmp_path_factory.mktemp("pre_large")
+    gen = os.path.join(pz.BIN_DIR, "pierce_generate")
+    common = ["--count", "4000", "--level", "2", "--radius-min", "0.5", "--radius-max", "2.0", "--box", "150",
+              "--generate-grid"]
+    subprocess.run([gen, "--output", str(d / "a.pre"), "--seed", "17"] + common, check=True, capture_output=True)
+    subprocess.run([gen, "--output", str(d / "b.pre"), "--seed", "18", "--copy-fraction", "0.2", "--copy-scale-min",
+                    "0.3", "--copy-scale-max", "0.8", "--copy-source-count", "4000", "--copy-source-seed", "17"]
+                   + common, check=True, capture_output=True)
+    return d
+
+
+@pytest.mark.parametrize("tool", list(ADAPTER_KEYS))
+@pytest.mark.parametrize("strategy", ["two-level", "global"])
+def test_adapter_query_time_covers_the_join(larger_pre, tmp_path, tool, strategy):
+    """Every device phase of the join sits under a key the tool's adapter sums,
+    so the adapter's query time is at least 95% of the join's own time."""
+    if strategy == "global" and tool not in ("pierce_overlap", "pierce_overlap_two_pass"):
+        pytest.skip("the global strategy evaluates overlap only")
+    d = larger_pre
+    out = run(tool, ["--mesh1", str(d / "a.pre"), "--mesh2", str(d / "b.pre"), "--output", "t.json",
+                     "--warmup-runs", "2", "--no-export", "--strategy", strategy], tmp_path)
+    join_ms = float(re.search(r"^Join time \(device-resident geometry to sorted host pairs\): ([\d.e+-]+) ms$",
+                              out, re.M).group(1))
+    q = adapter_query_ms(tool, json.load(open(tmp_path / "t.json"))["phases"])
+    print(f"[adapter] {tool} {strategy}: adapter query {q:.3f} ms, join {join_ms:.3f} ms", flush=True)
+    assert q >= 0.95 * join_ms
+    assert q <= 1.05 * join_ms + 0.05  # nothing counted twice

@@ -52,7 +52,7 @@ def test_gain_over_the_grid_estimate(ctx, trained):
     for A, B, _ in held:
         da = ctx.upload(pz.compute_grid(A, 5.0))
         db = ctx.upload(pz.compute_grid(B, 5.0))
-        est.append(pz.join(ctx, da, db, sizing=pz.SIZING_ESTIMATE).predicted)
+        est.append(pz.join(ctx, da, db, sizing=pz.SIZING_ESTIMATE).predicted_pairs)
         da.close()
         db.close()
     err_grid = np.abs(pz.log_counts(est) - T[:, 1])

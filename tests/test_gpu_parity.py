@@ -64,6 +64,41 @@ def test_direction_split_c1(ctx, c1, direction, narrow_mode):
     np.testing.assert_array_equal(out.pairs, orc.edge_overlap(a, b, direction))
 
 
+@pytest.mark.parametrize("narrow_mode", [0, 1])
+@pytest.mark.parametrize("direction", [pz.DIR_1TO2, pz.DIR_2TO1])
+def test_intersection_direction(ctx, direction, narrow_mode):
+    """--query-direction of pierce_intersection (pierce_intersection_main.cpp:103-155):
+    one edge direction and the parity rays from that side's first vertices."""
+    for name, (a, b) in (("ellipsoids", containment_sets()), ("nested cubes", nested_cubes())):
+        exp, _ = orc.predicate_join(a, b, pz.PRED_INTERSECTION, direction=direction)
+        both, _ = orc.predicate_join(a, b, pz.PRED_INTERSECTION)
+        assert 0 < len(exp) <= len(both)
+        da, db = ctx.upload(a), ctx.upload(b)
+        for kw in (dict(), dict(precision=pz.PREC_FP64), dict(broad_mode=1)):
+            print(f"[intersection dir {direction}] {name} narrow={narrow_mode} {kw}", flush=True)
+            out = pz.join(ctx, da, db, predicate=pz.PRED_INTERSECTION, direction=direction, narrow_mode=narrow_mode,
+                          **kw)
+            np.testing.assert_array_equal(out.pairs, exp)
+        # batched: each query carries its own direction
+        outs, _ = pz.join_batch(ctx, ctx.upload_batch([a, a]), ctx.upload_batch([b, b]),
+                                [dict(predicate=pz.PRED_INTERSECTION, direction=direction),
+                                 dict(predicate=pz.PRED_INTERSECTION)], narrow_mode=narrow_mode)
+        np.testing.assert_array_equal(outs[0].pairs, exp)
+        np.testing.assert_array_equal(outs[1].pairs, both)
+
+
+@pytest.mark.parametrize("direction", [pz.DIR_1TO2, pz.DIR_2TO1])
+def test_containment_uses_both_edge_directions(ctx, direction):
+    """pierce_containment has no direction: its edge phase always runs both ways
+    (pierce_containment_main.cpp:385-550), so a direction in the descriptor
+    leaves the result unchanged."""
+    a, b = containment_sets()
+    exp, _ = orc.predicate_join(a, b, pz.PRED_CONTAINMENT)
+    da, db = ctx.upload(a), ctx.upload(b)
+    out = pz.join(ctx, da, db, predicate=pz.PRED_CONTAINMENT, direction=direction)
+    np.testing.assert_array_equal(out.pairs, exp)
+
+
 def test_candidate_count_c1(ctx, c1):
     a, b, da, db, _ = c1
     assert pz.count_candidate_tests(ctx, da, db) == orc.count_candidate_tests(a, b)

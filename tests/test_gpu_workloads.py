@@ -108,3 +108,65 @@ def test_tubes_work_queue_rounds(ctx, tubes, queue, chunk):
             assert out.tasks_spilled > 0 and out.narrow_rounds > 1
     finally:
         pz.tune_set(ctx, saved["build_threads"], saved["narrow_queue"], saved["narrow_chunk"])
+
+
+def _diamonds(centres):
+    """Edge-2 cubes turned 45 degrees about z, bottom at z = 0 and top at z = 2
+    (exact), one object each."""
+    from tests.meshes import cube_mesh
+
+    c = np.float32(np.sqrt(0.5))
+    m = cube_mesh([np.array([0, 0, 1], np.float32)] * len(centres), [2.0] * len(centres))
+    v = m.vertices.copy()
+    x, y = v[:, 0].copy(), v[:, 1].copy()
+    v[:, 0], v[:, 1] = (x - y) * c, (x + y) * c
+    for o, (cx, cy) in enumerate(centres):
+        sel = np.unique(m.indices[m.triangle_to_object == o])
+        v[sel, 0] += np.float32(cx)
+        v[sel, 1] += np.float32(cy)
+    return pz.Mesh(v, m.indices, m.triangle_to_object)
+
+
+def test_escalations_counted_once_over_rounds(ctx):
+    """Escalated leaf pairs are decided (and counted) once, however many
+    work-queue rounds the narrow phase takes.
+
+    Object 0 of each set: a sphere and its copy scaled by 1 + 2^-21 about the
+    centre (disjoint surfaces), whose walk spills into later rounds under a
+    short LDS queue.  The other objects: two interleaved grids of turned cubes
+    whose top and bottom faces are coplanar with gaps of 0.83 between them, so
+    their triangle boxes meet but their surfaces do not.  The coplanar leaf
+    pairs cannot be decided in fp32 and go to the escalation list.  No pair
+    touches, so every walk visits every leaf pair whose boxes meet, and the
+    escalation count is the same whatever the split into rounds."""
+    s = pz.generate(pz.GEN_ELLIPSOIDS, count=1, seed=3, level_min=3, level_max=3, radius_min=1.0, radius_max=1.0,
+                    box=0.0)
+    f = np.float32(1 + 2.0 ** -21)
+    grid = [(20.0 + 4 * i, 4.0 * j) for i in range(8) for j in range(8)]
+    da_m, db_m = _diamonds(grid), _diamonds([(x + 2, y + 2) for x, y in grid])
+
+    def join_meshes(m0, m1):
+        n0 = m0.vertices.shape[0]
+        return pz.extract_edges(pz.Mesh(np.concatenate([m0.vertices, m1.vertices]),
+                                        np.concatenate([m0.indices, m1.indices + n0]),
+                                        np.concatenate([m0.triangle_to_object, m1.triangle_to_object + 1])))
+
+    a = join_meshes(s, da_m)
+    b = join_meshes(pz.Mesh(s.vertices * f, s.indices, s.triangle_to_object), db_m)
+    assert len(orc.face_overlap(a, b)) == 0
+    da, db = ctx.upload(a), ctx.upload(b)
+    saved = pz.tune_get(ctx)
+    got = {}
+    try:
+        for queue in (1536, 256):
+            pz.tune_set(ctx, saved["build_threads"], narrow_queue=(queue, queue), narrow_chunk=(1, 1))
+            out = pz.join(ctx, da, db)
+            assert len(out.pairs) == 0
+            got[queue] = (out.narrow_rounds, out.tests_fp64 + out.tests_exact, out.narrow_fallback_pairs)
+            print(f"[escalation] queue {queue}: rounds {out.narrow_rounds}, escalated {got[queue][1]}, "
+                  f"fallback pairs {out.narrow_fallback_pairs}", flush=True)
+    finally:
+        pz.tune_set(ctx, saved["build_threads"], saved["narrow_queue"], saved["narrow_chunk"])
+    assert got[1536][0] == 1 and got[256][0] > 1
+    assert got[1536][2] == 0 and got[256][2] == 0
+    assert got[1536][1] > 0 and got[256][1] == got[1536][1]

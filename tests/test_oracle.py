@@ -21,7 +21,7 @@ import pytest
 
 import pierce_amd as pz
 from oracle import oracle as orc
-from tests.meshes import cube_mesh
+from tests.meshes import cube_mesh, nested_cubes
 
 
 # ------------------------------------------------------------ exact reference
@@ -313,6 +313,45 @@ def test_nested_cubes_closed_form():
             if np.all(lb > la) and np.all(hb < ha):
                 exp_c.add((i, j))
     assert set(map(tuple, cont.tolist())) == exp_c
+
+
+def test_directional_intersection_closed_form_and_invariants():
+    """Intersection in one direction (pierce_intersection_main.cpp:103-155):
+    edges of one side against the other's triangles, plus the first vertex of
+    each object of that side strictly inside an object of the other.  On
+    axis-aligned cubes the point test is a closed form (point strictly inside a
+    box); the one-direction edge set is pinned by the union invariant of
+    test_invariants_swap_self_and_predicates.  Both directions together give
+    the undirected result, and swapping the sets swaps the directions."""
+    A, B = nested_cubes()
+
+    def first_vertices(m):
+        out = {}
+        for t, o in enumerate(m.triangle_to_object.tolist()):
+            out.setdefault(o, m.vertices[m.indices[t, 0]])
+        return out
+
+    def boxes(m):
+        res = {}
+        for o in range(m.num_objects):
+            v = m.vertices[np.unique(m.indices[m.triangle_to_object == o])]
+            res[o] = (v.min(axis=0), v.max(axis=0))
+        return res
+
+    fa, fb, ba, bb = first_vertices(A), first_vertices(B), boxes(A), boxes(B)
+    s = lambda x: set(map(tuple, x.tolist()))
+    inside = lambda p, bx: bool(np.all(p > bx[0]) and np.all(p < bx[1]))
+    d12 = s(orc.predicate_join(A, B, 1, direction=1)[0])
+    d21 = s(orc.predicate_join(A, B, 1, direction=2)[0])
+    exp12 = s(orc.edge_overlap(A, B, 1)) | {(i, j) for i in fa for j in bb if inside(fa[i], bb[j])}
+    exp21 = s(orc.edge_overlap(A, B, 2)) | {(i, j) for j in fb for i in ba if inside(fb[j], ba[i])}
+    assert d12 == exp12 and d21 == exp21
+    assert d12 != d21 and len(d12) > 0 and len(d21) > 0
+    assert d12 | d21 == s(orc.predicate_join(A, B, 1)[0])
+    swapped = s(orc.predicate_join(B, A, 1, direction=2)[0])
+    assert {(j, i) for i, j in swapped} == d12
+    # containment ignores the direction
+    np.testing.assert_array_equal(orc.predicate_join(A, B, 2, direction=1)[0], orc.predicate_join(A, B, 2)[0])
 
 
 def test_bruteforce_matches_face_and_touch():
