@@ -464,6 +464,19 @@ pz_status pz_profile_query(const char* kernel, double* total_ms, uint64_t* launc
 long long merge_and_deduplicate_pairs_gpu(const pz_pair* d_results1, long long num_results1,
                                           const pz_pair* d_results2, long long num_results2,
                                           pz_pair* d_merged_output);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): merge_and_deduplicate_pairs_gpu with the same arguments,
+ * in-place forms and result, whose scratch is a fixed device array of the
+ * library (40 MiB per device, held from load to process exit) instead of the
+ * growable workspace.  It calls no allocating routine.  More than
+ * PZ_NOALLOC_MAX_PAIRS input pairs (n1 + n2) -> -1 and pz_last_error() says
+ * why.  Every call on one device uses that device's one array: calls that
+ * overlap on one device race (serialise them). */
+#define PZ_NOALLOC_MAX_PAIRS 1048576
+long long merge_and_deduplicate_pairs_gpu_noalloc(const pz_pair* d_results1, long long num_results1,
+                                                  const pz_pair* d_results2, long long num_results2,
+                                                  pz_pair* d_merged_output);
 /* mesh_query_deduplication.h:13-16: device hash table of (id1 << 32 | id2) keys,
  * empty = ~0ull; writes at most max_output_size pairs; returns the count written. */
 int compact_hash_table_pairs(const unsigned long long* d_hash_table, unsigned long long table_size,

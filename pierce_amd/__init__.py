@@ -37,6 +37,7 @@ PREC_FP32_FILTERED, PREC_FP64 = 0, 1
 SIZING_TWO_PASS, SIZING_ESTIMATE, SIZING_LEARNED, SIZING_FIXED = 0, 1, 2, 3
 DEDUP_SORT, DEDUP_HASH = 0, 1
 MAX_PHASES = 24
+NOALLOC_MAX_PAIRS = 1 << 20  # PZ_NOALLOC_MAX_PAIRS
 
 
 class PierceError(RuntimeError):
@@ -186,6 +187,8 @@ _SIGS = [
     ("pz_profile_reset", None, []),
     ("pz_profile_query", C.c_int32, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     ("merge_and_deduplicate_pairs_gpu", C.c_longlong,
+     [C.c_void_p, C.c_longlong, C.c_void_p, C.c_longlong, C.c_void_p]),
+    ("merge_and_deduplicate_pairs_gpu_noalloc", C.c_longlong,
      [C.c_void_p, C.c_longlong, C.c_void_p, C.c_longlong, C.c_void_p]),
     ("compact_hash_table_pairs", C.c_int, [C.c_void_p, C.c_ulonglong, C.c_void_p, C.c_int]),
     ("exclusive_scan_gpu", C.c_longlong, [C.c_void_p, C.c_void_p, C.c_int]),
