@@ -182,7 +182,8 @@ void      pz_ctx_set_graphs(pz_ctx* ctx, int32_t on);
 pz_status pz_ctx_mem_info(pz_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
 /* Return the context's cached device work buffers (they grow to the largest
  * join seen and are kept for the next one) and drop its captured graphs.
- * Datasets are not touched.  For alternating very large joins whose work
+ * Uploaded datasets are not touched; datasets instanced into a slot of the
+ * context (pz_library_instantiate) lose their arrays.  For alternating very large joins whose work
  * buffers do not fit in HBM together.  No reference counterpart. */
 pz_status pz_ctx_trim(pz_ctx* ctx);
 
@@ -291,6 +292,8 @@ typedef struct {
     uint64_t parity_rays_crossing; /* rays that crossed at least one triangle */
     uint64_t parity_inside;     /* rays that crossed an odd number of triangles (point inside) */
     uint64_t parity_tests;      /* ray-triangle tests of those rays (part of tests_total) */
+    uint32_t index_height;      /* tallest triangle BVH the join built (at most 63, else PZ_E_LIMIT) */
+    uint32_t reserved_u32;
     double   ms_total;          /* device-resident geometry -> sorted host pair set */
     pz_phase phases[PZ_MAX_PHASES];
 } pz_join_result;
@@ -298,6 +301,10 @@ typedef struct {
 pz_status pz_join(pz_ctx* ctx, const pz_dataset* mesh1, const pz_dataset* mesh2,
                   const pz_join_desc* desc, pz_join_result* out);
 void      pz_join_result_release(pz_join_result* r);
+
+/* Read a device-resident dataset back into a new host mesh (its vertices,
+ * triangles, object map, edges and grid). */
+pz_status pz_dataset_download(const pz_dataset* ds, pz_host_mesh** out);
 
 /* ------------------------------------------------------------------ batches */
 /* One dataset holding many independent sets ("parts") concatenated: part p's
@@ -316,6 +323,38 @@ pz_status pz_dataset_upload_batch(pz_ctx* ctx, uint32_t nparts, const pz_mesh_vi
  * predicted_pairs / capacity hold query q's prediction and its sized share. */
 pz_status pz_join_batch(pz_ctx* ctx, const pz_dataset* mesh1, const pz_dataset* mesh2, uint32_t nqueries,
                         const pz_join_desc* descs, pz_join_result* results, pz_join_result* summary);
+
+/* ------------------------------------------------------------ mesh library */
+/* A catalogue of meshes resident in HBM from which query sets are assembled
+ * by instancing: a set is a list of (library mesh, affine map) instances.
+ * Streams of queries over sets that reuse a bounded set of shapes move each
+ * set device to device instead of across PCIe.  No reference counterpart
+ * (the reference loads every set from its own .pre file). */
+typedef struct pz_library pz_library;
+typedef struct {
+    uint32_t mesh;       /* library object id */
+    uint32_t pad;
+    float    xform[12];  /* x' = A x + t: A row-major in xform[0..8], t in xform[9..11] */
+} pz_instance;
+/* Every object of the view must own one contiguous run of triangles and one
+ * of vertices (as pz_generate and .pre files from the reference's
+ * preprocessor store them); edges are regrouped per object.  Object ids >= 0. */
+pz_status pz_library_create(pz_ctx* ctx, const pz_mesh_view* view, pz_library** out);
+void      pz_library_release(pz_library* lib);
+/* out[0..3] = vertices, triangles, edges, objects */
+pz_status pz_library_info(const pz_library* lib, uint64_t out[4]);
+/* A batched dataset of nparts parts, as pz_dataset_upload_batch makes: part p
+ * holds instances [counts[0] + .. + counts[p-1], .. + counts[p]) in order as
+ * its objects.  Vertices and edge endpoints are mapped in fp32 as
+ * ((a0 x + a1 y) + a2 z) + t0 per row, every operation rounded (no fused
+ * multiply-add), so the same evaluation on the host gives the same bits.
+ * slot < 0: the dataset owns its arrays.  slot in [0, PZ_INSTANCE_SLOTS): its
+ * arrays are the context's storage for that slot, kept and grown across
+ * calls (no allocation once large enough), and the next instantiation into
+ * the same slot takes them over: joins then reject the older dataset. */
+#define PZ_INSTANCE_SLOTS 8
+pz_status pz_library_instantiate(pz_ctx* ctx, const pz_library* lib, uint32_t nparts, const uint64_t* counts,
+                                 const pz_instance* instances, int32_t slot, pz_dataset** out);
 
 /* --------------------------------------------------- learned output sizing */
 /* A 32-128-128-2 MLP predicts, per join (or per query of a batch),

@@ -119,7 +119,8 @@ class JoinResult(C.Structure):
                 ("graph_launches", C.c_int32), ("hash_accesses", C.c_uint64), ("hash_contentions", C.c_uint64),
                 ("narrow_fallback_pairs", C.c_uint64), ("walks_started", C.c_uint64), ("pairs_touching", C.c_uint64),
                 ("parity_rays", C.c_uint64), ("parity_crossings", C.c_uint64), ("parity_rays_crossing", C.c_uint64),
-                ("parity_inside", C.c_uint64), ("parity_tests", C.c_uint64), ("ms_total", C.c_double),
+                ("parity_inside", C.c_uint64), ("parity_tests", C.c_uint64), ("index_height", C.c_uint32),
+                ("reserved_u32", C.c_uint32), ("ms_total", C.c_double),
                 ("phases", Phase * MAX_PHASES)]
 
 
@@ -181,6 +182,12 @@ _SIGS = [
     ("pz_track_containment_rays", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
                                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("pz_dataset_upload_batch", C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(MeshView), C.POINTER(C.c_void_p)]),
+    ("pz_dataset_download", C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("pz_library_create", C.c_int32, [C.c_void_p, C.POINTER(MeshView), C.POINTER(C.c_void_p)]),
+    ("pz_library_release", None, [C.c_void_p]),
+    ("pz_library_info", C.c_int32, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    ("pz_library_instantiate", C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                           C.c_int32, C.POINTER(C.c_void_p)]),
     ("pz_join_batch", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(JoinDesc),
                                   C.POINTER(JoinResult), C.POINTER(JoinResult)]),
     ("pz_profile_enable", None, [C.c_int32]),
@@ -300,16 +307,33 @@ class Mesh:
         return v
 
 
+class _HostMeshOwner:
+    """Keeps a library-owned host mesh alive while numpy arrays view it."""
+
+    def __init__(self, h):
+        self.h = h
+
+    def __del__(self):
+        try:
+            lib().pz_host_mesh_free(self.h)
+        except Exception:
+            pass
+
+
 def _mesh_from_handle(h: C.c_void_p) -> Mesh:
+    """Arrays view the library's host mesh without a copy (large sets would
+    otherwise be held twice); the mesh is freed when the last view goes."""
     L = lib()
     v = MeshView()
     _check(L.pz_host_mesh_view(h, C.byref(v)))
+    owner = _HostMeshOwner(h)
 
     def arr(p, n, dt, shape):
         if n == 0 or not p:
             return np.zeros(tuple(0 if x == -1 else x for x in shape), dt)
         buf = (C.c_char * (n * np.dtype(dt).itemsize)).from_address(p)
-        return np.frombuffer(buf, dt).reshape(shape).copy()
+        buf.owner = owner
+        return np.frombuffer(buf, dt).reshape(shape)
 
     m = Mesh(arr(v.vertices, 3 * v.num_vertices, np.float32, (-1, 3)),
              arr(v.indices, 3 * v.num_triangles, np.uint32, (-1, 3)),
@@ -319,7 +343,6 @@ def _mesh_from_handle(h: C.c_void_p) -> Mesh:
              arr(v.edge_objects, v.num_edges, np.int32, (-1,)),
              arr(v.grid_cells, v.num_grid_cells, GRID_DTYPE, (-1,)),
              float(v.grid_cell_size))
-    L.pz_host_mesh_free(h)
     return m
 
 
@@ -470,6 +493,12 @@ class Context:
     def set_graphs(self, on: bool) -> None:
         lib().pz_ctx_set_graphs(self._h, 1 if on else 0)
 
+    def mem_info(self) -> tuple:
+        """(free, total) bytes of the context's device."""
+        f, t = C.c_uint64(), C.c_uint64()
+        _check(lib().pz_ctx_mem_info(self._h, C.byref(f), C.byref(t)))
+        return int(f.value), int(t.value)
+
     def trim(self) -> None:
         """Free the cached device work buffers (pz_ctx_trim)."""
         _check(lib().pz_ctx_trim(self._h))
@@ -483,6 +512,10 @@ class Context:
     def upload_batch(self, meshes) -> "Dataset":
         """One dataset whose part p is meshes[p] (for join_batch)."""
         return Dataset(self, None, None, parts=list(meshes))
+
+    def library(self, mesh: Mesh) -> "Library":
+        """A mesh catalogue resident on the device (pz_library_create)."""
+        return Library(self, mesh)
 
 
 class Dataset:
@@ -511,9 +544,66 @@ class Dataset:
                                               C.byref(out)))
         return out.as_dict()
 
+    def download(self) -> Mesh:
+        """The device-resident arrays read back (pz_dataset_download)."""
+        h = C.c_void_p()
+        _check(lib().pz_dataset_download(self._h, C.byref(h)))
+        return _mesh_from_handle(h)
+
+    @classmethod
+    def _adopt(cls, ctx: "Context", h: C.c_void_p) -> "Dataset":
+        d = cls.__new__(cls)
+        d.ctx = ctx
+        d._h = h
+        return d
+
     def close(self):
         if self._h:
             lib().pz_dataset_release(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# One instance of a library mesh: mesh id, pad, x' = A x + t (A row-major, then t).
+INSTANCE_DTYPE = np.dtype([("mesh", "<u4"), ("pad", "<u4"), ("xform", "<f4", (12,))])
+
+
+class Library:
+    """A mesh catalogue in HBM; sets are built from it by instancing."""
+
+    def __init__(self, ctx: Context, mesh: Mesh):
+        self.ctx = ctx
+        self._h = C.c_void_p()
+        v = mesh.view()
+        _check(lib().pz_library_create(ctx._h, C.byref(v), C.byref(self._h)))
+
+    def info(self) -> dict:
+        out = (C.c_uint64 * 4)()
+        _check(lib().pz_library_info(self._h, out))
+        return dict(vertices=out[0], triangles=out[1], edges=out[2], objects=out[3])
+
+    def instantiate(self, counts, instances: np.ndarray, slot: int = -1) -> Dataset:
+        """Batched dataset: part p holds the next counts[p] instances (structured
+        INSTANCE_DTYPE array) as its objects (pz_library_instantiate).  slot >= 0
+        builds it in the context's reusable storage for that slot (valid until
+        the next instantiation into the same slot)."""
+        cnt = np.ascontiguousarray(counts, np.uint64).reshape(-1)
+        inst = np.ascontiguousarray(instances, INSTANCE_DTYPE).reshape(-1)
+        if int(cnt.sum()) != inst.shape[0]:
+            raise ValueError("counts must add up to the number of instances")
+        h = C.c_void_p()
+        _check(lib().pz_library_instantiate(self.ctx._h, self._h, cnt.shape[0], cnt.ctypes.data,
+                                            inst.ctypes.data if inst.size else None, int(slot), C.byref(h)))
+        return Dataset._adopt(self.ctx, h)
+
+    def close(self):
+        if self._h:
+            lib().pz_library_release(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):
@@ -555,6 +645,7 @@ class JoinOutput:
     parity_inside: int = 0
     stack_overflows: int = 0
     parity_tests: int = 0
+    index_height: int = 0
 
 
 def join_desc(**kw) -> JoinDesc:
@@ -589,7 +680,7 @@ def _output(r: JoinResult) -> JoinOutput:
                       int(r.hash_accesses), int(r.hash_contentions), int(r.narrow_fallback_pairs),
                       int(r.walks_started), int(r.pairs_touching), int(r.parity_rays), int(r.parity_crossings),
                       int(r.parity_rays_crossing), int(r.parity_inside), int(r.stack_overflows),
-                      int(r.parity_tests))
+                      int(r.parity_tests), int(r.index_height))
 
 
 def join_batch(ctx: Context, a: Dataset, b: Dataset, queries, **common):

@@ -43,6 +43,11 @@ uint8_t pair_meta_of(const pz_join_desc& d) {
     return (uint8_t)(d.predicate | (dir << 2));
 }
 
+uint64_t next_dataset_uid() {
+    static std::atomic<uint64_t> next{1};
+    return next++;
+}
+
 SetPrep prep_set(pz_ctx* c, const pz_dataset* d, const std::string& tag) {
     SetPrep p;
     size_t n = std::max<int32_t>(d->nobj, 1);
@@ -537,10 +542,11 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         pzk::fill_u32(acc.height, 0u, 1, s);
         uint32_t* cursor = buf<uint32_t>(c, tag + ".cursor", (size_t)std::max(D->nobj, 1));
         if (nb_tri > 0) {  // global pipeline for the big objects (segment prefix)
-            uint64_t* keys = buf<uint64_t>(c, tag + ".tkeys", nb_tri + 1);
-            uint32_t* vals = buf<uint32_t>(c, tag + ".tvals", nb_tri + 1);
-            uint64_t* keys2 = buf<uint64_t>(c, tag + ".tkeys2", nb_tri + 1);
-            uint32_t* vals2 = buf<uint32_t>(c, tag + ".tvals2", nb_tri + 1);
+            // sort buffers are shared by both sets' builds (one after the other on the stream)
+            uint64_t* keys = buf<uint64_t>(c, "big.tkeys", nb_tri + 1);
+            uint32_t* vals = buf<uint32_t>(c, "big.tvals", nb_tri + 1);
+            uint64_t* keys2 = buf<uint64_t>(c, "big.tkeys2", nb_tri + 1);
+            uint32_t* vals2 = buf<uint32_t>(c, "big.tvals2", nb_tri + 1);
             pzk::fill_u32(cursor, 0u, (size_t)std::max(D->nobj, 1), s);
             // Morton bits of the keys: of 30, 27, 24 and 21, the most that need the
             // fewest 8-bit radix passes beside the segment bits (chunks only need
@@ -670,6 +676,7 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         auto hh = readback<uint32_t>(c, {xa.height, xb.height});
         height = std::max(hh[0], hh[1]);
     }
+    r->index_height = height;
     if (pzk::stack_for_height(height) == 0)
         throw PzError(PZ_E_LIMIT, "per-object BVH height " + std::to_string(height) + " exceeds the traversal stack (63)");
     r->narrow_fallback_pairs = d.narrow_mode == 0 ? nfb : 0;  // pairs re-run by the per-edge kernel
@@ -757,22 +764,23 @@ GlobalBvh build_global(pz_ctx* c, const pz_dataset* D, const SetPrep& P, const s
     GlobalBvh g;
     uint32_t* scene = buf<uint32_t>(c, tag + ".gscene", 8);
     pzk::scene_box(P.obox, D->nobj, scene, s);
-    uint64_t* keys = buf<uint64_t>(c, tag + ".gkeys", n + 1);
-    uint32_t* vals = buf<uint32_t>(c, tag + ".gvals", n + 1);
-    uint64_t* keys2 = buf<uint64_t>(c, tag + ".gkeys2", n + 1);
-    uint32_t* vals2 = buf<uint32_t>(c, tag + ".gvals2", n + 1);
+    // build temporaries are shared by both sets' builds; only nodes and leaves stay per set
+    uint64_t* keys = buf<uint64_t>(c, "gbuild.keys", n + 1);
+    uint32_t* vals = buf<uint32_t>(c, "gbuild.vals", n + 1);
+    uint64_t* keys2 = buf<uint64_t>(c, "gbuild.keys2", n + 1);
+    uint32_t* vals2 = buf<uint32_t>(c, "gbuild.vals2", n + 1);
     pzk::tri_keys_global(D->v, D->idx, n, scene, keys, vals, s);
     void* tmp = buf<char>(c, "sort.tmp." + tag, pzk::radix_sort_temp_bytes(n));
     bool alt = pzk::radix_sort_pairs(keys, vals, keys2, vals2, n, 0, 30, tmp, s);
     uint64_t* sk = alt ? keys2 : keys;
     uint32_t* order = alt ? vals2 : vals;
     g.nodes = buf<pzd::Node>(c, tag + ".gnodes", n + 1);
-    uint32_t* pint = buf<uint32_t>(c, tag + ".gpint", n + 1);
-    uint32_t* pleaf = buf<uint32_t>(c, tag + ".gpleaf", n + 1);
-    uint32_t* cnt = buf<uint32_t>(c, tag + ".gcnt", n + 1);
+    uint32_t* pint = buf<uint32_t>(c, "gbuild.pint", n + 1);
+    uint32_t* pleaf = buf<uint32_t>(c, "gbuild.pleaf", n + 1);
+    uint32_t* cnt = buf<uint32_t>(c, "gbuild.cnt", n + 1);
     uint32_t* h = buf<uint32_t>(c, tag + ".gheight", 1);
     pzk::fill_u32(h, 0u, 1, s);
-    uint32_t* rng = buf<uint32_t>(c, tag + ".grange", n + 1);
+    uint32_t* rng = buf<uint32_t>(c, "gbuild.range", n + 1);
     pzk::lbvh_karras(sk, n, nullptr, 0, g.nodes, pint, pleaf, cnt, rng, s);
     // leaf records first: the refit then reads its boxes from them in sorted
     // order instead of gathering every triangle a second time
@@ -803,6 +811,7 @@ void join_global(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const pz_j
     ph.begin("Build Mesh2 Index");
     if (do12) gb = build_global(c, B, pb, "B");
     r->index_triangles = (do21 ? A->nt : 0) + (do12 ? B->nt : 0);
+    r->index_height = std::max(ga.height, gb.height);
     const bool f32 = d.precision == PZ_PREC_FP32_FILTERED;
     const int allow_f32 = (std::max(A->max_abs, B->max_abs) <= 0x1p39f) ? 1 : 0;
     uint32_t* overflow = buf<uint32_t>(c, "g.overflow", 1);
@@ -971,6 +980,7 @@ pz_status pz_ctx_trim(pz_ctx* c) {
             if (kv.second.p) HIP_OK(hipFree(kv.second.p));
         c->bufs.clear();
         ++c->pool_epoch;
+        for (uint64_t& g : c->slot_gen) ++g;  // datasets instanced into slots lost their arrays
         return PZ_OK;
     });
 }
@@ -1016,23 +1026,43 @@ pz_status pz_dataset_upload_parts(pz_ctx* c, const pz_mesh_view* v, const std::v
         return fail(PZ_E_INVALID, "null geometry array");
     return guarded([&]() -> pz_status {
         HIP_OK(hipSetDevice(c->device));
-        static std::atomic<uint64_t> next_uid{1};
         std::unique_ptr<pz_dataset> d(new pz_dataset());
         d->ctx = c;
-        d->uid = next_uid++;
+        d->uid = pz::next_dataset_uid();
         d->nv = v->num_vertices;
         d->nt = v->num_triangles;
         d->ne = v->num_edges;
+        // largest object id, index bounds and largest |coordinate|, over 256 slices in parallel
+        constexpr uint64_t kSlices = 256;
+        std::vector<int32_t> smx(kSlices, -1);
+        std::vector<float> sma(kSlices, 0.f);
+        std::vector<uint8_t> sbad(kSlices, 0);
+        pz::parallel_for(kSlices, [&](uint64_t k) {
+            auto range = [&](uint64_t n) { return std::make_pair(n * k / kSlices, n * (k + 1) / kSlices); };
+            int32_t mx = -1;
+            float ma = 0.f;
+            bool bad = false;
+            auto [t0, t1] = range(d->nt);
+            for (uint64_t t = t0; t < t1; ++t) mx = std::max(mx, v->triangle_to_object[t]);
+            for (uint64_t i = 3 * t0; i < 3 * t1; ++i) bad |= v->indices[i] >= d->nv;
+            auto [e0, e1] = range(d->ne);
+            for (uint64_t e = e0; e < e1; ++e) mx = std::max(mx, v->edge_objects[e]);
+            for (uint64_t i = 3 * e0; i < 3 * e1; ++i)
+                ma = std::max(ma, std::max(std::fabs(v->edge_starts[i]), std::fabs(v->edge_ends[i])));
+            auto [v0, v1] = range(d->nv);
+            for (uint64_t i = 3 * v0; i < 3 * v1; ++i) ma = std::max(ma, std::fabs(v->vertices[i]));
+            smx[k] = mx;
+            sma[k] = ma;
+            sbad[k] = bad;
+        });
         int32_t mx = -1;
-        for (uint64_t t = 0; t < d->nt; ++t) mx = std::max(mx, v->triangle_to_object[t]);
-        for (uint64_t i = 0; i < 3 * d->nt; ++i)
-            if (v->indices[i] >= d->nv) throw PzError(PZ_E_INVALID, "triangle index beyond the vertex array");
-        for (uint64_t e = 0; e < d->ne; ++e) mx = std::max(mx, v->edge_objects[e]);
-        d->nobj = mx + 1;
         float ma = 0.f;
-        for (uint64_t i = 0; i < 3 * d->nv; ++i) ma = std::max(ma, std::fabs(v->vertices[i]));
-        for (uint64_t i = 0; i < 3 * d->ne; ++i)
-            ma = std::max(ma, std::max(std::fabs(v->edge_starts[i]), std::fabs(v->edge_ends[i])));
+        for (uint64_t k = 0; k < kSlices; ++k) {
+            if (sbad[k]) throw PzError(PZ_E_INVALID, "triangle index beyond the vertex array");
+            mx = std::max(mx, smx[k]);
+            ma = std::max(ma, sma[k]);
+        }
+        d->nobj = mx + 1;
         d->max_abs = std::isfinite(ma) ? ma : INFINITY;
         auto up = [&](void** dst, const void* src, size_t bytes) {
             HIP_OK(hipMalloc(dst, std::max<size_t>(bytes, 16)));
@@ -1086,6 +1116,10 @@ pz_status pz_dataset_load_pre(pz_ctx* c, const char* path, pz_dataset** out) {
 
 void pz_dataset_release(pz_dataset* d) {
     if (!d) return;
+    if (d->slot >= 0) {  // the arrays are the context's (instancing slot)
+        delete d;
+        return;
+    }
     for (void* p : {(void*)d->v, (void*)d->idx, (void*)d->t2o, (void*)d->es, (void*)d->ee, (void*)d->eo,
                     (void*)d->part_of, (void*)d->part_off})
         if (p) (void)hipFree(p);
@@ -1119,6 +1153,8 @@ void pz_join_desc_defaults(pz_join_desc* d) {
 pz_status pz_join(pz_ctx* c, const pz_dataset* a, const pz_dataset* b, const pz_join_desc* desc, pz_join_result* r) {
     if (!c || !a || !b || !desc || !r) return fail(PZ_E_INVALID, "pz_join: null argument");
     if (a->ctx != c || b->ctx != c) return fail(PZ_E_INVALID, "datasets belong to another context");
+    if (!dataset_live(a) || !dataset_live(b))
+        return fail(PZ_E_INVALID, "a dataset's instancing slot has been reused by a later instantiation");
     pz_join_desc d = *desc;
     if (d.predicate < 0 || d.predicate > 2) return fail(PZ_E_INVALID, "unknown predicate");
     if (d.direction < 0 || d.direction > 2) return fail(PZ_E_INVALID, "unknown query direction");
@@ -1297,6 +1333,9 @@ pz_status pz_join_batch(pz_ctx* c, const pz_dataset* a, const pz_dataset* b, uin
     if (!c || !a || !b || !descs || !results || nq == 0) return fail(PZ_E_INVALID, "pz_join_batch: bad argument");
     if (a->part_obj_off.size() != (size_t)nq + 1 || b->part_obj_off.size() != (size_t)nq + 1)
         return fail(PZ_E_INVALID, "pz_join_batch: both datasets must come from pz_dataset_upload_batch with nq parts");
+    if (a->ctx != c || b->ctx != c) return fail(PZ_E_INVALID, "datasets belong to another context");
+    if (!dataset_live(a) || !dataset_live(b))
+        return fail(PZ_E_INVALID, "a dataset's instancing slot has been reused by a later instantiation");
     for (uint32_t q = 0; q < nq; ++q) {
         if (descs[q].predicate < 0 || descs[q].predicate > 2) return fail(PZ_E_INVALID, "unknown predicate");
         init_result(&results[q]);

@@ -65,7 +65,40 @@ V3 random_dir(Rng& r) { return unit({r.normal(), r.normal(), r.normal()}); }
 struct MeshOut {
     std::vector<float> v;
     std::vector<uint32_t> f;
+    size_t nv = 0;  // vertex count (kept after v is released)
 };
+
+// Unique undirected edges of a face list over nv vertices, ordered by
+// (smaller, larger) vertex index, each with the orientation of the first face
+// that has it: the order of a stable sort of every face edge by that key,
+// obtained here by a counting sort on the smaller index (stable in face order)
+// and an insertion sort of each small bucket on the larger one.
+void unique_edges(const std::vector<uint32_t>& F, uint32_t nv, std::vector<std::pair<uint32_t, uint32_t>>& out) {
+    const size_t ne = F.size();  // 3 per face
+    std::vector<uint32_t> start((size_t)nv + 1, 0);
+    for (size_t k = 0; k < ne; ++k) {
+        const uint32_t a = F[k], b = F[k - k % 3 + (k % 3 + 1) % 3];
+        ++start[std::min(a, b) + 1];
+    }
+    for (uint32_t v = 0; v < nv; ++v) start[v + 1] += start[v];
+    std::vector<std::pair<uint32_t, uint32_t>> bucket(ne);  // (a, b) as in the face
+    std::vector<uint32_t> fill(start.begin(), start.end() - 1);
+    for (size_t k = 0; k < ne; ++k) {
+        const uint32_t a = F[k], b = F[k - k % 3 + (k % 3 + 1) % 3];
+        bucket[fill[std::min(a, b)]++] = {a, b};
+    }
+    out.clear();
+    out.reserve(ne / 2 + 1);
+    for (uint32_t v = 0; v < nv; ++v) {
+        auto* lo = bucket.data() + start[v];
+        auto* hi = bucket.data() + start[v + 1];
+        auto larger = [](const std::pair<uint32_t, uint32_t>& e) { return std::max(e.first, e.second); };
+        for (auto* p = lo + 1; p < hi; ++p)  // stable insertion sort on the larger index
+            for (auto* q = p; q > lo && larger(*(q - 1)) > larger(*q); --q) std::swap(*(q - 1), *q);
+        for (auto* p = lo; p < hi; ++p)
+            if (p == lo || larger(*p) != larger(*(p - 1))) out.push_back(*p);
+    }
+}
 
 // Emit one capped tube through skeleton points P (>= 2 points).
 void emit_tube(const std::vector<V3>& P, double radius, int seg, MeshOut& out) {
@@ -224,6 +257,7 @@ pz_status pz_gen_tubes_impl(const pz_gen_params& p, pz_host_mesh& m) {
     m.indices.resize(3 * nt);
     m.tri_to_object.resize(nt);
     pz::parallel_for(n, [&](uint64_t i) {
+        meshes[i].nv = meshes[i].v.size() / 3;
         std::copy(meshes[i].v.begin(), meshes[i].v.end(), m.vertices.begin() + 3 * voff[i]);
         for (size_t k = 0; k < meshes[i].f.size(); ++k)
             m.indices[3 * toff[i] + k] = static_cast<uint32_t>(voff[i] + meshes[i].f[k]);
@@ -238,21 +272,7 @@ pz_status pz_gen_tubes_impl(const pz_gen_params& p, pz_host_mesh& m) {
         std::vector<uint64_t> eoff(n + 1, 0);
         std::vector<std::vector<std::pair<uint32_t, uint32_t>>> per(n);
         pz::parallel_for(n, [&](uint64_t i) {
-            auto& E = per[i];
-            const auto& F = meshes[i].f;
-            E.reserve(F.size() / 2 + 1);
-            std::vector<std::pair<uint64_t, std::pair<uint32_t, uint32_t>>> keyed;
-            keyed.reserve(F.size());
-            for (size_t k = 0; k < F.size(); k += 3)
-                for (int j = 0; j < 3; ++j) {
-                    uint32_t a = F[k + j], b = F[k + (j + 1) % 3];
-                    uint64_t key = (static_cast<uint64_t>(std::min(a, b)) << 32) | std::max(a, b);
-                    keyed.push_back({key, {a, b}});
-                }
-            std::stable_sort(keyed.begin(), keyed.end(),
-                             [](const auto& x, const auto& y) { return x.first < y.first; });
-            for (size_t k = 0; k < keyed.size(); ++k)
-                if (k == 0 || keyed[k].first != keyed[k - 1].first) E.push_back(keyed[k].second);
+            unique_edges(meshes[i].f, (uint32_t)(meshes[i].nv), per[i]);
             std::vector<uint32_t>().swap(meshes[i].f);
         });
         for (uint64_t i = 0; i < n; ++i) eoff[i + 1] = eoff[i] + per[i].size();

@@ -48,6 +48,10 @@ void compute_grid(const pz_mesh_view& v, float cell_size, std::vector<pz_grid_ce
 
 #include <thread>
 
+#include <atomic>
+
+// Threads take runs of indices from a shared counter (about 16 runs per
+// thread), so items of very different cost still balance.
 template <class F>
 void pz::parallel_for(uint64_t n, F fn) {
     unsigned hw = std::thread::hardware_concurrency();
@@ -58,12 +62,16 @@ void pz::parallel_for(uint64_t n, F fn) {
         for (uint64_t i = 0; i < n; ++i) fn(i);
         return;
     }
+    const uint64_t run = n / (nt * 16) > 0 ? n / (nt * 16) : 1;
+    std::atomic<uint64_t> next{0};
     std::vector<std::thread> ts;
     ts.reserve(nt);
     for (uint64_t t = 0; t < nt; ++t) {
-        uint64_t lo = n * t / nt, hi = n * (t + 1) / nt;
-        ts.emplace_back([lo, hi, &fn]() {
-            for (uint64_t i = lo; i < hi; ++i) fn(i);
+        ts.emplace_back([n, run, &next, &fn]() {
+            for (uint64_t lo = next.fetch_add(run); lo < n; lo = next.fetch_add(run)) {
+                const uint64_t hi = lo + run < n ? lo + run : n;
+                for (uint64_t i = lo; i < hi; ++i) fn(i);
+            }
         });
     }
     for (auto& t : ts) t.join();

@@ -74,4 +74,18 @@ struct KernelTimer {
 };
 void fill_u64(uint64_t* p, uint64_t v, uint64_t n, hipStream_t s);
 
+// ---------------------------------------------------------- instancing
+// One instance of a library mesh in a set (instance.hip): the library's runs
+// of vertices, triangles and edges, where they go in the set, the object id
+// they get, and the map x' = A x + t (A row-major in m[0..8], t in m[9..11]).
+struct InstanceRec {
+    uint32_t sv, nv, st, nt, se, ne;  // library: first vertex / count, first triangle / count, first edge / count
+    uint32_t dv, dt, de;              // set: first vertex, first triangle, first edge
+    uint32_t obj;                     // object id in the set
+    float m[12];
+};
+void instantiate(const InstanceRec* inst, uint32_t n, const float* lv, const uint32_t* lidx, const float* les,
+                 const float* lee, float* v, uint32_t* idx, int32_t* t2o, float* es, float* ee, int32_t* eo,
+                 hipStream_t s);
+
 }  // namespace pzk

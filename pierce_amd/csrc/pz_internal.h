@@ -50,6 +50,9 @@ struct pz_ctx {
     void* pinned = nullptr;  // small readbacks
     size_t pinned_n = 0;
     hipEvent_t ev[PZ_MAX_PHASES + 1];
+    // generation of each instancing slot (a dataset built into slot k is valid
+    // while slot_gen[k] still equals its own generation)
+    uint64_t slot_gen[PZ_INSTANCE_SLOTS] = {};
 };
 
 struct pz_dataset {
@@ -70,7 +73,15 @@ struct pz_dataset {
     std::vector<uint64_t> part_obj_off;
     uint32_t* part_of = nullptr;   // device: part id per object
     uint32_t* part_off = nullptr;  // device: part_obj_off as uint32
+    // instanced into a context slot: the arrays belong to the context's pool
+    int32_t slot = -1;
+    uint64_t slot_gen = 0;
 };
+
+// A dataset whose slot has been reused no longer holds its own geometry.
+inline bool dataset_live(const pz_dataset* d) {
+    return d->slot < 0 || d->ctx->slot_gen[d->slot] == d->slot_gen;
+}
 
 namespace pz {
 
@@ -163,6 +174,8 @@ struct SetPrep {
     uint32_t* edge_cnt;
 };
 SetPrep prep_set(pz_ctx* c, const pz_dataset* d, const std::string& tag);
+// Unique id of a new dataset (keys the captured HIP graphs).
+uint64_t next_dataset_uid();
 
 // LBVH over the object boxes of one set (broad-phase target).
 struct ObjBvh {

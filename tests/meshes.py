@@ -41,3 +41,4 @@ def containment_sets(n=300, box=60.0):
                     box=box, copy_fraction=0.2, copy_scale_min=0.3, copy_scale_max=0.8, copy_source_count=n,
                     copy_source_seed=7)
     return a, b
+
