@@ -793,10 +793,46 @@ GlobalBvh build_global(pz_ctx* c, const pz_dataset* D, const SetPrep& P, const s
     return g;
 }
 
+// Points in objects the reference's way (mesh_intersection_anyhit.cu:46-240,
+// mesh_containment.cu:224-293): one +Z ray per source object from the first
+// vertex of its first triangle through the set-wide BVH of the other set,
+// counting crossed triangles per target object.  Returns the device keys of
+// the (source, target) list entries, ~0 where the count is even, and their
+// number.  A ray that meets more than 256 objects is re-cast with room for
+// 4,096; beyond that the join fails (PZ_E_LIMIT) instead of dropping targets.
+std::pair<uint64_t*, uint64_t> containment_rays(pz_ctx* c, const pz_dataset* S, const SetPrep& PS, const GlobalBvh& gT,
+                                                uint64_t ntT, bool src_is_mesh2, const std::string& tag,
+                                                unsigned long long* stats) {
+    hipStream_t s = c->s;
+    const int32_t ns = std::max<int32_t>(S->nobj, 0);
+    if (ns == 0 || ntT == 0) return {nullptr, 0};
+    uint32_t* cnt = buf<uint32_t>(c, tag + ".ray_cnt", 3 * (size_t)ns);
+    uint64_t* off = buf<uint64_t>(c, tag + ".ray_off", (size_t)ns + 1);
+    uint64_t* ovo = buf<uint64_t>(c, tag + ".ray_ovf_off", (size_t)ns + 1);
+    void* tmp = buf<char>(c, "scan.tmp.ray", pzk::scan_temp_bytes((uint64_t)ns));
+    for (int cap : {256, 4096}) {
+        int32_t* ids = buf<int32_t>(c, "ray.ids", (size_t)ns * cap);
+        uint32_t* hits = buf<uint32_t>(c, "ray.hits", (size_t)ns * cap);
+        if (!pzk::track_rays(S->v, S->idx, PS.first_tri, PS.tri_cnt, ns, gT.nodes, gT.leaves, ntT, gT.height, cap, ids,
+                             hits, cnt, cnt + ns, cnt + 2 * ns, stats, s))
+            throw PzError(PZ_E_LIMIT, "set-wide BVH too tall for the containment ray's stack");
+        pzk::exclusive_scan_u32_u64(cnt, off, (uint64_t)ns, tmp, s);
+        pzk::exclusive_scan_u32_u64(cnt + ns, ovo, (uint64_t)ns, tmp, s);
+        const auto tot = readback<uint64_t>(c, {off + ns, ovo + ns});
+        if (tot[1] != 0) {
+            if (cap == 4096)
+                throw PzError(PZ_E_LIMIT, "a containment ray met more than 4096 objects");
+            continue;
+        }
+        uint64_t* keys = buf<uint64_t>(c, tag + ".ray_keys", tot[0] + 1);
+        pzk::parity_keys(ids, hits, cnt, off, ns, cap, src_is_mesh2, keys, s);
+        return {keys, tot[0]};
+    }
+    return {nullptr, 0};
+}
+
 void join_global(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const pz_join_desc& d, pz_join_result* r,
                  Phases& ph) {
-    if (d.predicate != PZ_PRED_OVERLAP)
-        throw PzError(PZ_E_INVALID, "the global strategy evaluates the overlap predicate only");
     hipStream_t s = c->s;
     unsigned long long* stats = buf<unsigned long long>(c, "stats", pzk::kStatSlots);
     pzk::fill_ull(stats, 0ull, pzk::kStatSlots, s);
@@ -805,7 +841,9 @@ void join_global(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const pz_j
     // LEARNED: features and inference overlap the set-wide index builds on the side stream
     uint32_t npred = 0;
     if (d.sizing == PZ_SIZING_LEARNED) npred = mlp_predict_async(c, d.model, A, B, pa, pb, nullptr, pair_meta_of(d));
-    const bool do12 = d.direction != PZ_DIR_2TO1, do21 = d.direction != PZ_DIR_1TO2;
+    // direction bits (containment always runs both edge directions)
+    const int dir_mask = pair_meta_of(d) >> 2;
+    const bool do12 = (dir_mask & 1) != 0, do21 = (dir_mask & 2) != 0;
     GlobalBvh ga{}, gb{};
     if (do21) ga = build_global(c, A, pa, "A");
     ph.begin("Build Mesh2 Index");
@@ -914,13 +952,63 @@ void join_global(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const pz_j
         nuniq = readback<unsigned long long>(c, {ucnt})[0];
         if (d.sizing == PZ_SIZING_TWO_PASS) r->capacity = raw;
     }
+    r->pairs_touching = nuniq;
+    uint32_t* types = nullptr;
+    bool want_types = false;
+    if (d.predicate != PZ_PRED_OVERLAP) {
+        // the overlap keys sorted (the hash set's compaction leaves them in slot order)
+        if (done) {
+            uint64_t* u2 = buf<uint64_t>(c, "g.uniq2", nuniq + 1);
+            void* tmp = buf<char>(c, "sort.tmp.g", pzk::radix_sort_temp_bytes(nuniq));
+            if (pzk::radix_sort_keys(uniq, u2, nuniq, 0, 32 + bits_for((uint64_t)std::max(A->nobj, 1)), tmp, s))
+                uniq = u2;
+        }
+        ph.begin("Containment Parity Rays");
+        std::pair<uint64_t*, uint64_t> p12{nullptr, 0}, p21{nullptr, 0};
+        if (d.predicate == PZ_PRED_CONTAINMENT) {
+            p21 = containment_rays(c, B, pb, ga, A->nt, true, "B", stats);  // points of B in objects of A
+        } else {
+            if (do12) p12 = containment_rays(c, A, pa, gb, B->nt, false, "A", stats);
+            if (do21) p21 = containment_rays(c, B, pb, ga, A->nt, true, "B", stats);
+        }
+        const uint64_t np = nuniq + p12.second + p21.second;
+        uint64_t* all = buf<uint64_t>(c, "g.pred_keys", np + 1);
+        uint32_t* ty = buf<uint32_t>(c, "g.pred_types", np + 1);
+        if (d.predicate == PZ_PRED_INTERSECTION) {  // overlap pairs and every odd ray entry, sorted unique
+            if (nuniq) HIP_OK(hipMemcpyAsync(all, uniq, 8 * nuniq, hipMemcpyDeviceToDevice, s));
+            if (p12.second) HIP_OK(hipMemcpyAsync(all + nuniq, p12.first, 8 * p12.second, hipMemcpyDeviceToDevice, s));
+            if (p21.second)
+                HIP_OK(hipMemcpyAsync(all + nuniq + p12.second, p21.first, 8 * p21.second, hipMemcpyDeviceToDevice, s));
+            uint64_t* all2 = buf<uint64_t>(c, "g.pred_keys2", np + 1);
+            void* tmp = buf<char>(c, "sort.tmp.g", pzk::radix_sort_temp_bytes(np));
+            uint64_t* sk = pzk::radix_sort_keys(all, all2, np, 0, 64, tmp, s) ? all2 : all;
+            uint64_t* u = buf<uint64_t>(c, "g.pred_uniq", np + 1);
+            unsigned long long* ucnt = buf<unsigned long long>(c, "g.ucnt", 1);
+            void* utmp = buf<char>(c, "uniq.tmp", pzk::unique_temp_bytes(np));
+            pzk::unique_u64(sk, np, u, reinterpret_cast<uint64_t*>(ucnt), utmp, s);
+            uint64_t nu = readback<unsigned long long>(c, {ucnt})[0];
+            if (nu && readback<uint64_t>(c, {u + nu - 1})[0] == ~0ull) --nu;  // the even entries' key
+            uniq = u;
+            nuniq = nu;
+        } else {  // containment: odd entries without surface contact; overlap pairs typed 1 on request
+            unsigned long long* oc = buf<unsigned long long>(c, "g.pred_cnt", 1);
+            pzk::fill_ull(oc, 0ull, 1, s);
+            pzk::key_difference(p21.first, p21.second, uniq, nuniq, all, ty, 0u, oc, s);
+            if (d.include_overlap_pairs) pzk::key_difference(uniq, nuniq, nullptr, 0, all, ty, 1u, oc, s);
+            nuniq = readback<unsigned long long>(c, {oc})[0];
+            uniq = all;
+            types = ty;
+            want_types = d.include_overlap_pairs != 0;
+        }
+    }
     ph.begin("Download Results");
-    finish_pairs(c, uniq, nullptr, nuniq, A->nobj, false, r);
+    finish_pairs(c, uniq, want_types ? types : nullptr, nuniq, A->nobj, want_types, r);
     std::vector<unsigned long long> st(pzk::kStatSlots);
     HIP_OK(hipMemcpyAsync(st.data(), stats, 8 * pzk::kStatSlots, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
+    const uint64_t touching = r->pairs_touching;
     set_counters(r, st);
-    r->pairs_touching = nuniq;
+    r->pairs_touching = touching;
 }
 
 }  // namespace

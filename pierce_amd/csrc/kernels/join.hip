@@ -1212,6 +1212,72 @@ __global__ __launch_bounds__(kBlock, 4) void k_track_rays(const float* __restric
         iters[src] = act ? 1u : 0u;
     }
     flush_stats(st, stats);
+    uint32_t crossings = 0, inside = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        crossings += my_hits[i];
+        inside += my_hits[i] & 1u;
+    }
+    // rays cast, triangles crossed, rays crossing something, targets containing the point, tests
+    const uint32_t pv[5] = {act ? 1u : 0u, crossings + ovf, (crossings + ovf) ? 1u : 0u, inside, st.tests};
+    for (int k = 0; k < 5; ++k) {
+        const uint32_t sum = wave_sum(pv[k]);
+        if (lane == 0 && sum) atomicAdd(&stats[pzk::kStatParityRays + k], (unsigned long long)sum);
+    }
+}
+
+// Pair keys of a set-wide containment-ray pass: source s's list entries
+// [0, count[s]) go to keys[off[s] + i] as (mesh1 << 32 | mesh2), with the
+// source on the mesh2 side when src_is_mesh2; entries with an even hit count
+// (the point is outside that target) write ~0, which sorts last.
+__global__ __launch_bounds__(kBlock) void k_parity_keys(const int32_t* __restrict__ ids,
+                                                        const uint32_t* __restrict__ hits,
+                                                        const uint32_t* __restrict__ count,
+                                                        const uint64_t* __restrict__ off, int32_t nsrc,
+                                                        int max_targets, int src_is_mesh2,
+                                                        uint64_t* __restrict__ keys) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= nsrc) return;
+    const uint32_t n = count[s];
+    for (uint32_t i = 0; i < n; ++i) {
+        const int64_t e = s * (int64_t)max_targets + i;
+        const int32_t t = ids[e];
+        keys[off[s] + i] = (hits[e] & 1u) ? (src_is_mesh2 ? pack_pair(t, (int32_t)s) : pack_pair((int32_t)s, t))
+                                          : ~0ull;
+    }
+}
+
+// Keys of `in` (not ~0) absent from the sorted unique array `ref`, appended
+// to out with type `type`; one wave reservation per wavefront.
+__global__ __launch_bounds__(kBlock) void k_key_difference(const uint64_t* __restrict__ in, uint64_t n,
+                                                           const uint64_t* __restrict__ ref, uint64_t nref,
+                                                           uint64_t* __restrict__ out, uint32_t* __restrict__ out_type,
+                                                           uint32_t type, unsigned long long* __restrict__ count) {
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    bool keep = false;
+    uint64_t k = ~0ull;
+    if (i < n) {
+        k = in[i];
+        if (k != ~0ull) {
+            uint64_t lo = 0, hi = nref;  // first ref >= k
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi) >> 1;
+                if (ref[mid] < k) lo = mid + 1;
+                else hi = mid;
+            }
+            keep = !(lo < nref && ref[lo] == k);
+        }
+    }
+    const uint64_t m = __ballot(keep);
+    if (m == 0) return;
+    unsigned long long base = 0;
+    const int leader = __ffsll((long long)m) - 1;
+    if (__lane_id() == leader) base = atomicAdd(count, (unsigned long long)__popcll(m));
+    base = __shfl((long long)base, leader, 64);
+    if (keep) {
+        const uint64_t pos = base + lanes_below(m);
+        out[pos] = k;
+        out_type[pos] = type;
+    }
 }
 
 // Ballot + prefix-sum compaction of selected candidate pairs.
@@ -1719,6 +1785,20 @@ bool track_rays(const float* vS, const uint32_t* iS, const uint32_t* ftS, const 
         hipLaunchKernelGGL((k_track_rays<64>), dim3(grid1((uint64_t)nsrc)), dim3(kBlock), 0, s, vS, iS, ftS, tcS, nsrc,
                            nodes, leaves, ntri, max_targets, ids, hits, count, overflow, iters, stats);
     return true;
+}
+
+void parity_keys(const int32_t* ids, const uint32_t* hits, const uint32_t* count, const uint64_t* off, int32_t nsrc,
+                 int max_targets, bool src_is_mesh2, uint64_t* keys, hipStream_t s) {
+    if (nsrc > 0)
+        hipLaunchKernelGGL(k_parity_keys, dim3(grid1((uint64_t)nsrc)), dim3(kBlock), 0, s, ids, hits, count, off, nsrc,
+                           max_targets, src_is_mesh2 ? 1 : 0, keys);
+}
+
+void key_difference(const uint64_t* in, uint64_t n, const uint64_t* ref, uint64_t nref, uint64_t* out,
+                    uint32_t* out_type, uint32_t type, unsigned long long* count, hipStream_t s) {
+    if (n)
+        hipLaunchKernelGGL(k_key_difference, dim3(grid1(n)), dim3(kBlock), 0, s, in, n, ref, nref, out, out_type, type,
+                           count);
 }
 
 void select_pairs(const uint64_t* pairs, uint64_t np, const uint32_t* found, const uint32_t* inside,

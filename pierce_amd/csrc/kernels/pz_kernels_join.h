@@ -178,6 +178,14 @@ bool track_rays(const float* vS, const uint32_t* iS, const uint32_t* ftS, const 
                 const Node* nodes, const LeafTri* leaves, uint64_t ntri, uint32_t height, int max_targets,
                 int32_t* ids, uint32_t* hits, uint32_t* count, uint32_t* overflow, uint32_t* iters,
                 unsigned long long* stats, hipStream_t s);
+// Set-wide containment rays -> pair keys (see k_parity_keys): keys[off[s] + i]
+// for each entry of each source's list, ~0 for an even hit count.
+void parity_keys(const int32_t* ids, const uint32_t* hits, const uint32_t* count, const uint64_t* off, int32_t nsrc,
+                 int max_targets, bool src_is_mesh2, uint64_t* keys, hipStream_t s);
+// Appends the keys of `in` (other than ~0) that the sorted unique `ref` lacks,
+// with type `type` (count: device cursor).
+void key_difference(const uint64_t* in, uint64_t n, const uint64_t* ref, uint64_t nref, uint64_t* out,
+                    uint32_t* out_type, uint32_t type, unsigned long long* count, hipStream_t s);
 void select_pairs(const uint64_t* pairs, uint64_t np, const uint32_t* found, const uint32_t* inside,
                   const uint8_t* meta, int include_overlap, uint64_t* out_keys, uint32_t* out_type,
                   unsigned long long* count, unsigned long long* touching, hipStream_t s);

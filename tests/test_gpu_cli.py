@@ -231,8 +231,6 @@ def larger_pre(tmp_path_factory):
 def test_adapter_query_time_covers_the_join(larger_pre, tmp_path, tool, strategy):
     """Every device phase of the join sits under a key the tool's adapter sums,
     so the adapter's query time is at least 95% of the join's own time."""
-    if strategy == "global" and tool not in ("pierce_overlap", "pierce_overlap_two_pass"):
-        pytest.skip("the global strategy evaluates overlap only")
     d = larger_pre
     out = run(tool, ["--mesh1", str(d / "a.pre"), "--mesh2", str(d / "b.pre"), "--output", "t.json",
                      "--warmup-runs", "2", "--no-export", "--strategy", strategy], tmp_path)

@@ -30,12 +30,16 @@ def test_two_level_matches_golden(ctx, uploaded, case, pred, precision, narrow_m
 
 
 @pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("pred", list(PREDICATES))
 @pytest.mark.parametrize("dedup,sizing", [(pz.DEDUP_SORT, pz.SIZING_TWO_PASS), (pz.DEDUP_HASH, pz.SIZING_FIXED)])
-def test_global_matches_golden(ctx, uploaded, case, dedup, sizing):
+def test_global_matches_golden(ctx, uploaded, case, pred, dedup, sizing):
+    """The set-wide (reference-shaped) path: edges traced through a BVH over all
+    triangles, and for intersection / containment one +Z ray per object through
+    the other set's BVH."""
     _, _, da, db = uploaded[case]
-    out = pz.join(ctx, da, db, strategy=pz.STRATEGY_GLOBAL, precision=pz.PREC_FP64, dedup=dedup, sizing=sizing,
-                  capacity=1 << 10)
-    np.testing.assert_array_equal(out.pairs, expected(case, "overlap"))
+    out = pz.join(ctx, da, db, predicate=PREDICATES[pred], strategy=pz.STRATEGY_GLOBAL, precision=pz.PREC_FP64,
+                  dedup=dedup, sizing=sizing, capacity=1 << 10)
+    np.testing.assert_array_equal(out.pairs, expected(case, pred))
 
 
 def test_batch_of_all_golden_queries(ctx):

@@ -117,13 +117,32 @@ def test_predicates_vs_oracle(ctx, predicate, narrow_mode):
         assert len(exp) > 0
 
 
-def test_containment_with_overlap_pairs(ctx):
+@pytest.mark.parametrize("strategy", [pz.STRATEGY_TWO_LEVEL, pz.STRATEGY_GLOBAL])
+def test_containment_with_overlap_pairs(ctx, strategy):
     a, b = containment_sets()
     da, db = ctx.upload(a), ctx.upload(b)
     exp, types = orc.predicate_join(a, b, pz.PRED_CONTAINMENT, include_overlap=True)
-    out = pz.join(ctx, da, db, predicate=pz.PRED_CONTAINMENT, include_overlap_pairs=1)
+    out = pz.join(ctx, da, db, predicate=pz.PRED_CONTAINMENT, include_overlap_pairs=1, strategy=strategy)
     np.testing.assert_array_equal(out.pairs, exp)
     np.testing.assert_array_equal(out.types, types)
+
+
+@pytest.mark.parametrize("dedup,sizing", [(pz.DEDUP_SORT, pz.SIZING_TWO_PASS), (pz.DEDUP_HASH, pz.SIZING_FIXED)])
+@pytest.mark.parametrize("direction", [pz.DIR_BOTH, pz.DIR_1TO2, pz.DIR_2TO1])
+@pytest.mark.parametrize("predicate", [pz.PRED_INTERSECTION, pz.PRED_CONTAINMENT])
+def test_global_predicates_vs_oracle(ctx, predicate, direction, dedup, sizing):
+    """Set-wide intersection and containment: edges against a BVH over all
+    triangles, plus one +Z ray per source object through the other set's BVH
+    (pierce_intersection_main.cpp:103-155, pierce_containment_main.cpp:385-550)."""
+    for name, (a, b) in (("ellipsoids", containment_sets()), ("nested cubes", nested_cubes())):
+        exp, _ = orc.predicate_join(a, b, predicate, direction=direction)
+        da, db = ctx.upload(a), ctx.upload(b)
+        print(f"[global predicate {predicate} dir {direction}] {name}", flush=True)
+        out = pz.join(ctx, da, db, predicate=predicate, direction=direction, strategy=pz.STRATEGY_GLOBAL,
+                      precision=pz.PREC_FP64, dedup=dedup, sizing=sizing, capacity=64)
+        np.testing.assert_array_equal(out.pairs, exp)
+        if predicate == pz.PRED_CONTAINMENT:
+            assert out.parity_rays == b.num_objects
 
 
 @pytest.mark.parametrize("precision", [pz.PREC_FP32_FILTERED, pz.PREC_FP64])
