@@ -142,7 +142,7 @@ Needed needed_layout(pz_ctx* c, const pz_dataset* d, const SetPrep& p, const std
 }
 
 struct SetAccel {
-    pzd::Node* nodes;
+    pzd::BNode* nodes;  // per-object BVHs, own-box nodes
     pzd::LeafTri* leaves;
     pzd::EdgeRec* edges;
     uint32_t* height;
@@ -536,7 +536,7 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
                      const std::string& tag) -> SetAccel {
         SetAccel acc;
         const uint64_t n = T.ntri, nb_tri = T.nbig_tri;
-        acc.nodes = buf<pzd::Node>(c, tag + ".nodes", n + 1);
+        acc.nodes = buf<pzd::BNode>(c, tag + ".nodes", n + 1);
         acc.leaves = buf<pzd::LeafTri>(c, tag + ".leaves", n + 1);
         acc.height = buf<uint32_t>(c, tag + ".height", 1);
         pzk::fill_u32(acc.height, 0u, 1, s);

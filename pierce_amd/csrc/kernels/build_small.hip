@@ -27,8 +27,9 @@
 //      and climbs on.  Boxes live in one LDS array: leaf p's box at p until
 //      node p completes, node i's box at i afterwards (every node containing
 //      leaf p completes after leaf p's parent, which consumed it);
-//   5. node records written in index order from LDS (coalesced); a leaf
-//      child's box is re-read from its just-written leaf record.  The root
+//   5. node records (own box + child links, 32 B) written in index order
+//      from LDS (coalesced), two lanes per record; every internal node's box
+//      is in LDS after the climb, so no leaf record is read back.  The root
 //      must sit at the segment's first node (the traversals start there): it
 //      is written at index 0 and internal node 0's record at the root's
 //      index, with the one link into node 0 redirected.
@@ -153,7 +154,7 @@ template <int T, bool CHUNK>
 __global__ __launch_bounds__(T) void k_build_small(
     const float* __restrict__ verts, const uint32_t* __restrict__ idx, const int32_t* __restrict__ obj_of_loc,
     const uint32_t* __restrict__ first_tri, const uint32_t* __restrict__ tri_cnt, const uint32_t* __restrict__ obox, const uint32_t* __restrict__ seg_off,
-    uint32_t l_begin, int cap, Node* __restrict__ nodes, LeafTri* __restrict__ leaves, uint32_t* __restrict__ height_out,
+    uint32_t l_begin, int cap, BNode* __restrict__ nodes, LeafTri* __restrict__ leaves, uint32_t* __restrict__ height_out,
     ChunkSrc ch) {
     constexpr int PR = T == 1024 ? 2 : (T == 512 ? 3 : 4);  // local triangles per thread kept in registers
     constexpr int NP = (kMax / T + 1) / 2;                 // sorted positions per thread, two per register
@@ -484,46 +485,28 @@ __global__ __launch_bounds__(T) void k_build_small(
         }
     }
     __syncthreads();
-    // 5. node records in index order (root at 0, node 0 at the root's index)
+    // 5. node records in index order (root at 0, node 0 at the root's index):
+    // node i's own box is box[6 i ..] after the climb
     if (n >= 2) {
         const int rt = __float_as_int(box[6 * (n - 1)]);
-        auto child_box = [&](int c, float* bx) {
-            if (c >= 0) {
-                for (int k = 0; k < 6; ++k) bx[k] = box[6 * c + k];
-            } else {
-                const LeafTri& L = leaves[lbase + (uint32_t)(~c)];
-                const float4 v0 = L.v0, v1 = L.v1, v2 = L.v2;
-                bx[0] = fminf(v0.x, fminf(v1.x, v2.x));
-                bx[1] = fminf(v0.y, fminf(v1.y, v2.y));
-                bx[2] = fminf(v0.z, fminf(v1.z, v2.z));
-                bx[3] = fmaxf(v0.x, fmaxf(v1.x, v2.x));
-                bx[4] = fmaxf(v0.y, fmaxf(v1.y, v2.y));
-                bx[5] = fmaxf(v0.z, fmaxf(v1.z, v2.z));
-            }
-        };
         auto link = [&](int c) {
             if (c < 0) return ~(int)(lbase + (uint32_t)(~c));
             return (int)(nbase + (uint32_t)(c == 0 ? rt : c));
         };
-        // four lanes per record, lane c writing its 16-byte quarter, so that
-        // each store instruction covers 16 consecutive records
-        const int c = tid & 3;
-        for (int i = tid >> 2; i < n - 1; i += T / 4) {
-            const uint32_t v = (uint32_t)bound[i];
-            const int c0 = (int16_t)(v & 0xFFFFu), c1 = (int16_t)(v >> 16);
+        // two lanes per record, lane h writing its 16-byte half, so that each
+        // store instruction covers 32 consecutive records
+        const int h = tid & 1;
+        for (int i = tid >> 1; i < n - 1; i += T / 2) {
+            const float* bx = box + 6 * i;
             float4 q;
-            if (c == 3) {
-                q = make_float4(__int_as_float(link(c0)), __int_as_float(link(c1)), 0.0f, 0.0f);
+            if (h == 0) {
+                q = make_float4(bx[0], bx[1], bx[2], bx[3]);
             } else {
-                // quarter 0: child 0's lo, hi.x; 1: child 0's hi.yz, child 1's lo.xy;
-                // 2: child 1's lo.z, hi
-                float bl[6], br[6];
-                if (c < 2) child_box(c0, bl);
-                if (c > 0) child_box(c1, br);
-                q = c == 0 ? make_float4(bl[0], bl[1], bl[2], bl[3])
-                           : (c == 1 ? make_float4(bl[4], bl[5], br[0], br[1]) : make_float4(br[2], br[3], br[4], br[5]));
+                const uint32_t v = (uint32_t)bound[i];
+                const int c0 = (int16_t)(v & 0xFFFFu), c1 = (int16_t)(v >> 16);
+                q = make_float4(bx[4], bx[5], __int_as_float(link(c0)), __int_as_float(link(c1)));
             }
-            reinterpret_cast<float4*>(nodes + nbase + (i == rt ? 0 : (i == 0 ? rt : i)))[c] = q;
+            reinterpret_cast<float4*>(nodes + nbase + (i == rt ? 0 : (i == 0 ? rt : i)))[h] = q;
         }
     }
     if constexpr (CHUNK) {  // the chunk's root for the top tree
@@ -565,7 +548,7 @@ __global__ void k_chunk_count(const uint32_t* __restrict__ seg_off, uint32_t nbi
 // last level's single node) is top node 0, which is slot seg_off[l].
 __global__ void k_top_level(const uint32_t* __restrict__ seg_off, const uint32_t* __restrict__ chunk_off,
                             uint32_t nbig, uint32_t level, const pzk::ChunkRoot* __restrict__ in,
-                            pzk::ChunkRoot* __restrict__ out, Node* __restrict__ nodes) {
+                            pzk::ChunkRoot* __restrict__ out, BNode* __restrict__ nodes) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= chunk_off[nbig]) return;
     const uint32_t l = seg_of_chunk(chunk_off, nbig, g);
@@ -578,17 +561,15 @@ __global__ void k_top_level(const uint32_t* __restrict__ seg_off, const uint32_t
         uint32_t slot = i;
         for (uint32_t nl = (np + 1) / 2; nl >= 2; nl = (nl + 1) / 2) slot += nl / 2;
         const uint32_t node = seg_off[l] + slot * kCh;
-        Node nd;
-        nd.a = make_float4(a.lo[0], a.lo[1], a.lo[2], a.hi[0]);
-        nd.b = make_float4(a.hi[1], a.hi[2], b.lo[0], b.lo[1]);
-        nd.c = make_float4(b.lo[2], b.hi[0], b.hi[1], b.hi[2]);
-        nd.d = make_int4(a.link, b.link, 0, 0);
-        nodes[node] = nd;
         pzk::ChunkRoot e;
         for (int k = 0; k < 3; ++k) {
             e.lo[k] = fminf(a.lo[k], b.lo[k]);
             e.hi[k] = fmaxf(a.hi[k], b.hi[k]);
         }
+        BNode nd;
+        nd.a = make_float4(e.lo[0], e.lo[1], e.lo[2], e.hi[0]);
+        nd.b = make_float4(e.hi[1], e.hi[2], __int_as_float(a.link), __int_as_float(b.link));
+        nodes[node] = nd;
         e.link = (int32_t)node;
         e.pad = 0;
         out[cb + i] = e;
@@ -629,7 +610,7 @@ size_t build_small_lds(uint32_t cap) { return 32ull * std::max<uint32_t>(cap, 2)
 
 bool build_small(const float* verts, const uint32_t* idx, const int32_t* obj_of_loc,
                  const uint32_t* first_tri, const uint32_t* tri_cnt, const uint32_t* obox, const uint32_t* seg_off,
-                 uint32_t l_begin, uint32_t count, uint32_t cap, int threads, Node* nodes, LeafTri* leaves,
+                 uint32_t l_begin, uint32_t count, uint32_t cap, int threads, BNode* nodes, LeafTri* leaves,
                  uint32_t* height, const char* timer, hipStream_t s) {
     if (count == 0) return true;
     if (cap == 0 || cap > pzk::kSmallMax) return false;
@@ -655,7 +636,7 @@ uint64_t big_chunk_bound(uint64_t nbig, uint64_t nbig_tri) { return nbig_tri / k
 void build_big_chunked(const float* verts, const uint32_t* idx, const int32_t* obj_of_loc, const uint32_t* seg_off,
                        uint32_t nbig, uint64_t nbig_tri, const uint64_t* sorted_keys, int mbits, const uint32_t* order,
                        uint32_t* chunk_cnt, uint32_t* chunk_off, void* scan_tmp, ChunkRoot* ent0, ChunkRoot* ent1,
-                       Node* nodes, LeafTri* leaves, uint32_t* height, hipStream_t s) {
+                       BNode* nodes, LeafTri* leaves, uint32_t* height, hipStream_t s) {
     if (nbig == 0 || nbig_tri == 0) return;
     const uint64_t bound = big_chunk_bound(nbig, nbig_tri);
     hipLaunchKernelGGL(k_chunk_count, dim3(grid1(nbig)), dim3(256), 0, s, seg_off, nbig, chunk_cnt);

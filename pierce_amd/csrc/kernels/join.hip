@@ -659,18 +659,12 @@ __global__ __launch_bounds__(kBlock, 4) void k_narrow(const uint64_t* __restrict
         bool hit = false;
         while (__ballot(act) != 0) {
             if (act) {
-                if (cur >= 0) {
-                    const Node nd = T.nodes[cur];
-                    bool h0 = boxes_meet(elo, ehi, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
-                    bool h1 = boxes_meet(elo, ehi, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w);
-                    if (h0 && h1) {
-                        my[sp * PZ_WAVE] = nd.d.y;
+                if (cur >= 0) {  // own-box node: descend if the edge's box meets it
+                    const BNode nd = T.nodes[cur];
+                    if (boxes_meet(elo, ehi, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y)) {
+                        my[sp * PZ_WAVE] = bnode_c1(nd);
                         ++sp;
-                        cur = nd.d.x;
-                    } else if (h0) {
-                        cur = nd.d.x;
-                    } else if (h1) {
-                        cur = nd.d.y;
+                        cur = bnode_c0(nd);
                     } else if (sp > 0) {
                         --sp;
                         cur = my[sp * PZ_WAVE];
@@ -821,6 +815,55 @@ __device__ __forceinline__ bool found_now(const uint32_t* found, uint32_t p) {
 // goes to esc_list (pair, mesh1 leaf, mesh2 leaf) for k_escalate, so the
 // fp64 and exact stages (and their registers) stay out of this kernel; if
 // esc_list is full the pair is sent to the per-edge kernel instead.
+// Queue entries are (children of an x node, children of a y node): four
+// links, kNone marking the second of a side that is a single leaf (a leaf's
+// "children" are itself), or a leaf pair to test, marked by kTest in the
+// second slot.  A step loads the up-to-four records of a node entry in one
+// batch (own-box nodes, or leaf triangles), tests the up-to-four box pairs
+// and pushes one entry per meeting pair: the pair's children links, taken
+// from the records just loaded, or the leaf pair.  A lane thus runs at most
+// one triangle-pair test per step.
+constexpr int kNone = 0x7FFFFFFF;
+constexpr int kTest = 0x7FFFFFFE;
+
+// Box and child links of one record of a side (a leaf offers itself).
+struct Elem {
+    float lo[3], hi[3];
+    int c0, c1;
+    bool leaf;
+};
+
+template <class SetV>
+__device__ __forceinline__ void load_elem(const SetV& S, int link, Elem& e) {
+    if (link >= 0) {
+        const BNode nd = S.nodes[link];
+        e.lo[0] = nd.a.x; e.lo[1] = nd.a.y; e.lo[2] = nd.a.z;
+        e.hi[0] = nd.a.w; e.hi[1] = nd.b.x; e.hi[2] = nd.b.y;
+        e.c0 = bnode_c0(nd);
+        e.c1 = bnode_c1(nd);
+        e.leaf = false;
+    } else {
+        leaf_box(S.leaves[~link], e.lo, e.hi);
+        e.c0 = link;
+        e.c1 = kNone;
+        e.leaf = true;
+    }
+}
+
+// The entry a walk starts from: the children of both roots (the roots' own
+// boxes are the objects' boxes, which the broad phase found to meet).
+template <class SetV>
+__device__ __forceinline__ void root_children(const SetV& S, uint32_t s0, uint32_t s1, int& c0, int& c1) {
+    if (s1 - s0 >= 2) {
+        const BNode nd = S.nodes[s0];
+        c0 = bnode_c0(nd);
+        c1 = bnode_c1(nd);
+    } else {
+        c0 = ~(int)s0;
+        c1 = kNone;
+    }
+}
+
 template <bool F32, int QLEN>
 __global__ __launch_bounds__(kDualWaves * PZ_WAVE, 2) void k_narrow_dual(
     const uint64_t* __restrict__ pairs, SetView A, SetView B, uint32_t* __restrict__ found,
@@ -829,9 +872,9 @@ __global__ __launch_bounds__(kDualWaves * PZ_WAVE, 2) void k_narrow_dual(
     DualTask* __restrict__ tasks_out, unsigned long long* __restrict__ nout, uint64_t out_cap,
     uint4* __restrict__ esc_list, unsigned long long* __restrict__ esc_count, uint64_t esc_cap,
     unsigned long long* __restrict__ stats) {
-    __shared__ int2 q[kDualWaves][QLEN];
+    __shared__ int4 q[kDualWaves][QLEN];
     const int lane = __lane_id(), w = threadIdx.x / PZ_WAVE;
-    int2* Q = q[w];  // ring buffer: entries [bot, top) modulo QLEN, LIFO at the top
+    int4* Q = q[w];  // ring buffer: entries [bot, top) modulo QLEN, LIFO at the top
     LaneStats st{0, 0, 0, 0, 0};
     uint32_t walks = 0;  // tasks this wave walked
     while (true) {
@@ -842,18 +885,19 @@ __global__ __launch_bounds__(kDualWaves * PZ_WAVE, 2) void k_narrow_dual(
         const uint64_t t1 = t0 + (uint64_t)chunk < ntasks ? t0 + (uint64_t)chunk : ntasks;
         for (uint64_t t = t0; t < t1; ++t) {
             uint32_t p;
-            int2 root;
+            int4 root;
             if (tasks_in) {
                 const DualTask tk = tasks_in[t];
                 p = tk.pair;
-                root = make_int2(tk.x, tk.y);
+                root = make_int4(tk.x0, tk.x1, tk.y0, tk.y1);
             } else {
                 p = (uint32_t)t;
                 const uint64_t key = pairs[p];
                 const int32_t oa = (int32_t)(key >> 32), ob = (int32_t)(uint32_t)key;
                 const uint32_t la = A.loc[oa], lb = B.loc[ob];
                 const uint32_t a0 = A.seg_off[la], a1 = A.seg_off[la + 1], b0 = B.seg_off[lb], b1 = B.seg_off[lb + 1];
-                root = make_int2((a1 - a0 >= 2) ? (int)a0 : ~(int)a0, (b1 - b0 >= 2) ? (int)b0 : ~(int)b0);
+                root_children(A, a0, a1, root.x, root.y);
+                root_children(B, b0, b1, root.z, root.w);
             }
             if (found_now(found, p)) continue;  // another task of this pair already proved a hit
             ++walks;
@@ -868,76 +912,50 @@ __global__ __launch_bounds__(kDualWaves * PZ_WAVE, 2) void k_narrow_dual(
                 const int qn = top - bot;
                 const int take = qn < PZ_WAVE ? qn : PZ_WAVE;
                 const bool act = lane < take;
-                int2 e = act ? Q[(top - 1 - lane) % QLEN] : make_int2(0, 0);
+                const int4 e = act ? Q[(top - 1 - lane) % QLEN] : make_int4(kNone, kNone, kNone, kNone);
                 top -= take;
-                int2 o0 = make_int2(0, 0), o1 = o0, o2 = o0, o3 = o0;
+                int4 o0 = make_int4(0, 0, 0, 0), o1 = o0, o2 = o0, o3 = o0;
                 int nout_l = 0;
-                if (act) {
-                    const int x = e.x, y = e.y;
-                    // Both records in one batch of loads, whatever their kind, so
-                    // a step that mixes node and leaf pairs waits for memory once:
-                    // a node's boxes are its first 48 bytes, a leaf's corners its
-                    // 48 bytes, and a node's child links the next 16.
-                    const float4* ra = x >= 0 ? reinterpret_cast<const float4*>(A.nodes + x)
-                                              : reinterpret_cast<const float4*>(A.leaves + ~x);
-                    const float4* rb = y >= 0 ? reinterpret_cast<const float4*>(B.nodes + y)
-                                              : reinterpret_cast<const float4*>(B.leaves + ~y);
-                    const float4 a0 = ra[0], a1 = ra[1], a2 = ra[2], b0 = rb[0], b1 = rb[1], b2 = rb[2];
-                    int4 ac = make_int4(0, 0, 0, 0), bc = ac;
-                    if (x >= 0) ac = reinterpret_cast<const int4*>(ra)[3];
-                    if (y >= 0) bc = reinterpret_cast<const int4*>(rb)[3];
-                    if (x < 0 && y < 0) {
-                        const LeafTri LA{a0, a1, a2}, LB{b0, b1, b2};
-                        if (F32) {
-                            const int r = tri_pair_f32(LA, LB, dir_mask, st);
-                            hit = r > 0;
-                            if (r < 0) {
-                                const unsigned long long slot = atomicAdd(esc_count, 1ull);
-                                if (slot < esc_cap) esc_list[slot] = make_uint4(p, (uint32_t)~x, (uint32_t)~y, 0u);
-                                else esc_full = true;
-                            }
-                        } else {
-                            hit = tri_pair_test<false>(LA, LB, dir_mask, false, st);
+                if (act && e.y == kTest) {
+                    const LeafTri LA = A.leaves[~e.x], LB = B.leaves[~e.z];
+                    if (F32) {
+                        const int r = tri_pair_f32(LA, LB, dir_mask, st);
+                        hit = r > 0;
+                        if (r < 0) {
+                            const unsigned long long slot = atomicAdd(esc_count, 1ull);
+                            if (slot < esc_cap)
+                                esc_list[slot] = make_uint4(p, (uint32_t)~e.x, (uint32_t)~e.z, 0u);
+                            else
+                                esc_full = true;
                         }
                     } else {
-                        // each side offers its two children (a node) or itself (a leaf)
-                        float xl[2][3], xh[2][3], yl[2][3], yh[2][3];
-                        int xc[2], yc[2];
-                        const int nx = x >= 0 ? 2 : 1, ny = y >= 0 ? 2 : 1;
-                        if (x >= 0) {
-                            const Node X{a0, a1, a2, ac};
-                            child_box(X, 0, xl[0], xh[0]);
-                            child_box(X, 1, xl[1], xh[1]);
-                            xc[0] = ac.x;
-                            xc[1] = ac.y;
-                        } else {
-                            leaf_box(LeafTri{a0, a1, a2}, xl[0], xh[0]);
-                            xc[0] = x;
-                        }
-                        if (y >= 0) {
-                            const Node Y{b0, b1, b2, bc};
-                            child_box(Y, 0, yl[0], yh[0]);
-                            child_box(Y, 1, yl[1], yh[1]);
-                            yc[0] = bc.x;
-                            yc[1] = bc.y;
-                        } else {
-                            leaf_box(LeafTri{b0, b1, b2}, yl[0], yh[0]);
-                            yc[0] = y;
-                        }
-                        for (int i = 0; i < 2; ++i)
-                            for (int j = 0; j < 2; ++j)
-                                if (i < nx && j < ny && box_box(xl[i], xh[i], yl[j], yh[j])) {
-                                    const int2 v = make_int2(xc[i], yc[j]);
-                                    if (nout_l == 0) o0 = v;
-                                    else if (nout_l == 1) o1 = v;
-                                    else if (nout_l == 2) o2 = v;
-                                    else o3 = v;
-                                    ++nout_l;
-                                }
+                        hit = tri_pair_test<false>(LA, LB, dir_mask, false, st);
                     }
+                } else if (act) {
+                    // all records of the entry in one batch of loads
+                    Elem x[2], y[2];
+                    const int nx = e.y == kNone ? 1 : 2, ny = e.w == kNone ? 1 : 2;
+                    load_elem(A, e.x, x[0]);
+                    if (nx > 1) load_elem(A, e.y, x[1]);
+                    load_elem(B, e.z, y[0]);
+                    if (ny > 1) load_elem(B, e.w, y[1]);
+#pragma unroll
+                    for (int i = 0; i < 2; ++i)
+#pragma unroll
+                        for (int j = 0; j < 2; ++j) {
+                            if (i >= nx || j >= ny || !box_box(x[i].lo, x[i].hi, y[j].lo, y[j].hi)) continue;
+                            const bool lp = x[i].leaf && y[j].leaf;
+                            const int4 v = lp ? make_int4(x[i].c0, kTest, y[j].c0, 0)
+                                              : make_int4(x[i].c0, x[i].c1, y[j].c0, y[j].c1);
+                            if (nout_l == 0) o0 = v;
+                            else if (nout_l == 1) o1 = v;
+                            else if (nout_l == 2) o2 = v;
+                            else o3 = v;
+                            ++nout_l;
+                        }
                 }
                 if (__ballot(hit) != 0) break;
-                // append children pairs: nout_l is 0..4, so its wave-wide
+                // append the children entries: nout_l is 0..4, so its wave-wide
                 // prefix is the bit-weighted count of lower lanes in three
                 // ballots (no cross-lane data movement)
                 const uint64_t b0 = __ballot(nout_l & 1), b1 = __ballot(nout_l & 2), b2 = __ballot(nout_l & 4);
@@ -953,12 +971,14 @@ __global__ __launch_bounds__(kDualWaves * PZ_WAVE, 2) void k_narrow_dual(
                     base = __shfl(base, 0, PZ_WAVE);
                     const int kk = base < out_cap ? (int)((out_cap - base) < (uint64_t)k ? out_cap - base : k) : 0;
                     for (int i = lane; i < kk; i += PZ_WAVE) {
-                        const int2 v = Q[(bot + i) % QLEN];
+                        const int4 v = Q[(bot + i) % QLEN];
                         DualTask tk;
                         tk.pair = p;
-                        tk.x = v.x;
-                        tk.y = v.y;
-                        tk.pad = 0;
+                        tk.x0 = v.x;
+                        tk.x1 = v.y;
+                        tk.y0 = v.z;
+                        tk.y1 = v.w;
+                        tk.pad[0] = tk.pad[1] = tk.pad[2] = 0;
                         tasks_out[base + i] = tk;
                     }
                     bot += kk;
@@ -1089,20 +1109,12 @@ __global__ __launch_bounds__(kBlock, 4) void k_parity(const uint64_t* __restrict
     }
     const float rlo[3] = {pt.x, pt.y, pt.z}, rhi[3] = {pt.x, pt.y, INFINITY};
     while (act) {
-        if (cur >= 0) {
-            const Node nd = T->nodes[cur];
-            bool h0 = boxes_meet(rlo, rhi, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
-            bool h1 = boxes_meet(rlo, rhi, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w);
-            if (h0 && h1) {
-                my[sp * PZ_WAVE] = nd.d.y;
+        if (cur >= 0) {  // own-box node: descend if the ray's box meets it
+            const BNode nd = T->nodes[cur];
+            if (boxes_meet(rlo, rhi, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y)) {
+                my[sp * PZ_WAVE] = bnode_c1(nd);
                 ++sp;
-                cur = nd.d.x;
-                continue;
-            } else if (h0) {
-                cur = nd.d.x;
-                continue;
-            } else if (h1) {
-                cur = nd.d.y;
+                cur = bnode_c0(nd);
                 continue;
             }
         } else {

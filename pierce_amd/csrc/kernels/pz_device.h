@@ -20,6 +20,17 @@ struct __align__(16) Node {
     int4 d;    // child0 child1 - -
 };
 
+// Node of a per-object BVH (the two-level join's index): its own box and its
+// two child links (as in Node), 32 bytes.  A walk tests a node's box when it
+// reaches the node, so a builder writes each record from the node's own box
+// alone, with no leaf record read back.
+struct __align__(16) BNode {
+    float4 a;  // lo.x lo.y lo.z hi.x
+    float4 b;  // hi.y hi.z child0 (bits) child1 (bits)
+};
+__device__ __forceinline__ int bnode_c0(const BNode& n) { return __float_as_int(n.b.z); }
+__device__ __forceinline__ int bnode_c1(const BNode& n) { return __float_as_int(n.b.w); }
+
 // Leaf triangle in sorted order: three vertices, w of v0 = original triangle
 // index (bit pattern), w of v1 = object id (bit pattern).
 struct __align__(16) LeafTri {

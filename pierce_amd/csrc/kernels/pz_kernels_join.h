@@ -6,7 +6,7 @@
 namespace pzd {
 // Device view of one set's per-object acceleration data for a two-level join.
 struct SetView {
-    const Node* nodes;
+    const BNode* nodes;        // per-object BVHs (own-box nodes)
     const LeafTri* leaves;
     const uint32_t* seg_off;   // per needed object: first sorted triangle (n + 1 entries)
     const EdgeRec* edges;      // edges grouped per needed object
@@ -14,12 +14,14 @@ struct SetView {
     const uint32_t* loc;       // object id -> needed-object index
     const uint32_t* obox;      // ordered-uint boxes by object id (6 per object)
 };
-// One task of the narrow phase's work queue: a (mesh1 node, mesh2 node)
-// subtree pair of candidate pair `pair` still to be walked (links as in Node).
+// One task of the narrow phase's work queue: candidate pair `pair` still to
+// be walked below a (mesh1 node, mesh2 node) pair, given by both nodes' child
+// links (links as in BNode; a side that is a single leaf has x1 / y1 =
+// 0x7FFFFFFF).
 struct DualTask {
     uint32_t pair;
-    int32_t x, y;
-    uint32_t pad;
+    int32_t x0, x1, y0, y1;
+    uint32_t pad[3];
 };
 }  // namespace pzd
 
@@ -36,6 +38,7 @@ constexpr int kStatParityRays = 7, kStatWalks = 12, kStatTouching = 13;
 using pzd::DualTask;
 using pzd::EdgeRec;
 using pzd::LeafTri;
+using pzd::BNode;
 using pzd::Node;
 using pzd::SetView;
 
@@ -70,7 +73,7 @@ size_t build_small_lds(uint32_t cap);
 // [l_begin, l_begin + count), all of at most cap triangles.  false: bad cap or threads.
 bool build_small(const float* verts, const uint32_t* idx, const int32_t* obj_of_loc,
                  const uint32_t* first_tri, const uint32_t* tri_cnt, const uint32_t* obox, const uint32_t* seg_off,
-                 uint32_t l_begin, uint32_t count, uint32_t cap, int threads, pzd::Node* nodes, pzd::LeafTri* leaves,
+                 uint32_t l_begin, uint32_t count, uint32_t cap, int threads, pzd::BNode* nodes, pzd::LeafTri* leaves,
                  uint32_t* height, const char* timer, hipStream_t s);
 // Big objects (segments [0, nbig), triangles Morton-sorted per segment into
 // keys / order by the global pipeline's sort) are built in chunks of
@@ -93,7 +96,7 @@ uint64_t big_chunk_bound(uint64_t nbig, uint64_t nbig_tri);
 void build_big_chunked(const float* verts, const uint32_t* idx, const int32_t* obj_of_loc, const uint32_t* seg_off,
                        uint32_t nbig, uint64_t nbig_tri, const uint64_t* sorted_keys, int mbits, const uint32_t* order,
                        uint32_t* chunk_cnt, uint32_t* chunk_off, void* scan_tmp, ChunkRoot* ent0, ChunkRoot* ent1,
-                       pzd::Node* nodes, pzd::LeafTri* leaves, uint32_t* height, hipStream_t s);
+                       pzd::BNode* nodes, pzd::LeafTri* leaves, uint32_t* height, hipStream_t s);
 void edge_count(const int32_t* eo, uint64_t ne, int32_t nobj, uint32_t* edge_cnt, hipStream_t s);
 void scene_box(const uint32_t* obox, int32_t nobj, uint32_t* scene, hipStream_t s);
 // keys = (part_of[o] << 30) | Morton of the box centre (part 0 when part_of is null)
