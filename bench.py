@@ -50,6 +50,8 @@ def alg_bytes(kernel, k):
     (design_bytes).  None for kernels without a payload unit."""
     if kernel == "object_stats":    # both sets' triangles and vertices, once
         return 16 * k["T"] + 12 * k["V"]
+    if kernel == "object_bounds_runs":  # both sets' vertices, once (objects own vertex runs)
+        return 12 * k["V"]
     if kernel in ("build_small", "build_chunks", "tri_keys", "narrow_dual", "gather_leaves", "refit"):
         return 16 * k["IT"] + 12 * k["IV"]
     if kernel == "narrow":          # the indexed edges
@@ -61,12 +63,14 @@ def alg_bytes(kernel, k):
 
 def design_bytes(kernel, k):
     """What the kernel reads and writes by its own design, per join."""
-    if kernel == "build_small":     # indices + 3 corners read, one 64-B node and one 48-B leaf written
-        return (12 + 36 + 64 + 48) * k["IT"]
-    if kernel == "narrow_dual":     # each indexed leaf (48 B) and node (64 B) at most once per side
-        return (48 + 64) * k["IT"]
+    if kernel == "build_small":     # indices + 3 corners read, one 32-B node and one 48-B leaf written
+        return (12 + 36 + 32 + 48) * k["IT"]
+    if kernel == "narrow_dual":     # each indexed leaf (48 B) and node (32 B) at most once per side
+        return (48 + 32) * k["IT"]
     if kernel == "object_stats":
         return 16 * k["T"] + 12 * k["V"]
+    if kernel == "object_bounds_runs":
+        return 12 * k["V"]
     return None
 
 
@@ -522,7 +526,7 @@ def main():
         time_join(pz, ctx, da, db, args.steps, **chosen)
         pz.profile_enable(False)
         kernels = {}
-        for k in ("object_stats", "edge_count", "tri_keys", "edge_scatter", "karras", "refit", "gather_leaves",
+        for k in ("object_stats", "object_bounds_runs", "edge_count", "tri_keys", "edge_scatter", "karras", "refit", "gather_leaves",
                   "radix_scatter", "build_small", "build_chunks", "narrow_dual", "narrow", "parity", "features",
                   "mlp_infer"):
             tot, n = pz.profile_query(k)

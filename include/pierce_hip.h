@@ -195,6 +195,13 @@ pz_status pz_dataset_load_pre(pz_ctx* ctx, const char* path, pz_dataset** out);
 void      pz_dataset_release(pz_dataset* ds);
 /* out[0..4] = vertices, triangles, edges, objects (max id + 1), grid cells */
 pz_status pz_dataset_info(const pz_dataset* ds, uint64_t out[5]);
+/* Layout facts found at upload.  PZ_DATASET_VERTEX_RUNS: every object owns a
+ * disjoint, gap-free run of vertices (ordered by object id) referenced by its
+ * triangles alone, so a join computes the object boxes from those runs (12 B
+ * per vertex) rather than from every triangle's corners.  Diagnostics; no
+ * reference counterpart (the reference's GAS build reads every triangle). */
+#define PZ_DATASET_VERTEX_RUNS 1u
+pz_status pz_dataset_flags(const pz_dataset* ds, uint32_t* flags);
 
 /* ------------------------------------------------------------------- join */
 #define PZ_PRED_OVERLAP       0  /* surfaces intersect (pierce_overlap*) */

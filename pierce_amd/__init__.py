@@ -175,6 +175,7 @@ _SIGS = [
     ("pz_dataset_load_pre", C.c_int32, [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p)]),
     ("pz_dataset_release", None, [C.c_void_p]),
     ("pz_dataset_info", C.c_int32, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    ("pz_dataset_flags", C.c_int32, [C.c_void_p, C.POINTER(C.c_uint32)]),
     ("pz_join_desc_defaults", None, [C.POINTER(JoinDesc)]),
     ("pz_join", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(JoinDesc), C.POINTER(JoinResult)]),
     ("pz_join_result_release", None, [C.POINTER(JoinResult)]),
@@ -535,6 +536,14 @@ class Dataset:
         out = (C.c_uint64 * 5)()
         _check(lib().pz_dataset_info(self._h, out))
         return dict(vertices=out[0], triangles=out[1], edges=out[2], objects=out[3], grid_cells=out[4])
+
+    @property
+    def vertex_runs(self) -> bool:
+        """True when every object owns a disjoint, gap-free run of vertices
+        (found at upload): joins then bound objects from those runs."""
+        f = C.c_uint32(0)
+        _check(lib().pz_dataset_flags(self._h, C.byref(f)))
+        return bool(f.value & 1)
 
     def estimate_grid(self, other: "Dataset", epsilon: float = 0.001, gamma: float = 0.8,
                       alpha_correction: bool = True) -> dict:

@@ -43,6 +43,72 @@ uint8_t pair_meta_of(const pz_join_desc& d) {
     return (uint8_t)(d.predicate | (dir << 2));
 }
 
+// Finds whether the objects of a just-uploaded dataset own disjoint, gap-free
+// vertex runs and, if so, keeps the per-object triangle ranges and the
+// chunks of the runs.  The runs are ordered by object id (every non-empty
+// object's least vertex beyond the previous one's greatest), no vertex is
+// referenced by two objects, and the runs' lengths add up to the number of
+// referenced vertices, so every vertex of a run is referenced by its object.
+void analyze_topology(pz_ctx* c, pz_dataset* d) {
+    const uint64_t n = (uint64_t)std::max(d->nobj, 1);
+    if (d->nobj <= 0 || d->nt == 0 || d->nv >= (1ull << 32) - pzk::run_chunk_vertices()) return;
+    uint32_t* topo = nullptr;
+    uint32_t* vmin = nullptr;
+    int32_t* owner = nullptr;
+    uint32_t* flags = nullptr;  // clash flag, then the referenced-vertex count (8 B)
+    auto cleanup = [&]() {
+        for (void* p : {(void*)vmin, (void*)owner, (void*)flags})
+            if (p) (void)hipFree(p);
+    };
+    try {
+        HIP_OK(hipMalloc((void**)&topo, 16 * n));
+        HIP_OK(hipMalloc((void**)&vmin, 4 * n));
+        HIP_OK(hipMalloc((void**)&owner, 4 * std::max<uint64_t>(d->nv, 1)));
+        HIP_OK(hipMalloc((void**)&flags, 16));
+        pzk::topology_scan(d->idx, d->t2o, d->nt, d->nv, d->nobj, topo, topo + n, topo + 2 * n, vmin, topo + 3 * n,
+                           owner, flags, reinterpret_cast<unsigned long long*>(flags + 2), c->s);
+        HIP_OK(hipGetLastError());
+        std::vector<uint32_t> cnt(n), lo(n), hi(n), fl(4);
+        HIP_OK(hipMemcpyAsync(cnt.data(), topo + 2 * n, 4 * n, hipMemcpyDeviceToHost, c->s));
+        HIP_OK(hipMemcpyAsync(lo.data(), vmin, 4 * n, hipMemcpyDeviceToHost, c->s));
+        HIP_OK(hipMemcpyAsync(hi.data(), topo + 3 * n, 4 * n, hipMemcpyDeviceToHost, c->s));
+        HIP_OK(hipMemcpyAsync(fl.data(), flags, 16, hipMemcpyDeviceToHost, c->s));
+        HIP_OK(hipStreamSynchronize(c->s));
+        uint64_t referenced = (uint64_t)fl[2] | ((uint64_t)fl[3] << 32), run_total = 0;
+        bool ok = fl[0] == 0;
+        int64_t prev = -1;
+        std::vector<uint2> chunks;
+        const uint32_t K = pzk::run_chunk_vertices();
+        for (uint64_t o = 0; o < n && ok; ++o) {
+            if (cnt[o] == 0) continue;
+            ok = (int64_t)lo[o] > prev && hi[o] >= lo[o];
+            prev = hi[o];
+            run_total += (uint64_t)hi[o] - lo[o] + 1;
+            for (uint64_t v = lo[o]; v <= hi[o]; v += K) chunks.push_back(make_uint2((uint32_t)o, (uint32_t)v));
+        }
+        ok = ok && run_total == referenced;
+        cleanup();
+        vmin = nullptr;
+        owner = nullptr;
+        flags = nullptr;
+        if (!ok) {
+            (void)hipFree(topo);
+            return;
+        }
+        d->topo = topo;
+        topo = nullptr;
+        HIP_OK(hipMalloc((void**)&d->topo_chunks, 8 * std::max<size_t>(chunks.size(), 1)));
+        if (!chunks.empty())
+            HIP_OK(hipMemcpy(d->topo_chunks, chunks.data(), 8 * chunks.size(), hipMemcpyHostToDevice));
+        d->topo_nchunks = chunks.size();
+        d->vruns = true;
+    } catch (...) {
+        cleanup();
+        if (topo) (void)hipFree(topo);
+        throw;
+    }
+}
+
 uint64_t next_dataset_uid() {
     static std::atomic<uint64_t> next{1};
     return next++;
@@ -52,10 +118,17 @@ SetPrep prep_set(pz_ctx* c, const pz_dataset* d, const std::string& tag) {
     SetPrep p;
     size_t n = std::max<int32_t>(d->nobj, 1);
     p.obox = buf<uint32_t>(c, tag + ".obox", 6 * n);
+    p.edge_cnt = buf<uint32_t>(c, tag + ".edge_cnt", n);
+    if (d->vruns) {  // triangle ranges from the upload's topology, boxes from the vertex runs
+        p.first_tri = d->topo;
+        p.last_tri = d->topo + n;
+        p.tri_cnt = d->topo + 2 * n;
+        pzk::object_bounds_runs(d->v, d->topo_chunks, d->topo_nchunks, d->topo + 3 * n, d->nobj, p.obox, c->s);
+        return p;
+    }
     p.tri_cnt = buf<uint32_t>(c, tag + ".tri_cnt", n);
     p.first_tri = buf<uint32_t>(c, tag + ".first_tri", n);
     p.last_tri = buf<uint32_t>(c, tag + ".last_tri", n);
-    p.edge_cnt = buf<uint32_t>(c, tag + ".edge_cnt", n);
     pzk::object_stats(d->v, d->nv, d->idx, d->t2o, d->nt, d->nobj, p.obox, p.tri_cnt, p.first_tri, p.last_tri,
                       c->s);
     return p;
@@ -1187,6 +1260,12 @@ pz_status pz_dataset_upload_parts(pz_ctx* c, const pz_mesh_view* v, const std::v
                 throw;
             }
         }
+        try {
+            pz::analyze_topology(c, d.get());
+        } catch (...) {
+            pz_dataset_release(d.release());
+            throw;
+        }
         *out = d.release();
         return PZ_OK;
     });
@@ -1209,7 +1288,7 @@ void pz_dataset_release(pz_dataset* d) {
         return;
     }
     for (void* p : {(void*)d->v, (void*)d->idx, (void*)d->t2o, (void*)d->es, (void*)d->ee, (void*)d->eo,
-                    (void*)d->part_of, (void*)d->part_off})
+                    (void*)d->part_of, (void*)d->part_off, (void*)d->topo, (void*)d->topo_chunks})
         if (p) (void)hipFree(p);
     delete d;
 }
@@ -1221,6 +1300,12 @@ pz_status pz_dataset_info(const pz_dataset* d, uint64_t out[5]) {
     out[2] = d->ne;
     out[3] = (uint64_t)std::max(d->nobj, 0);
     out[4] = d->grid.size();
+    return PZ_OK;
+}
+
+pz_status pz_dataset_flags(const pz_dataset* d, uint32_t* flags) {
+    if (!d || !flags) return fail(PZ_E_INVALID, "pz_dataset_flags: null argument");
+    *flags = d->vruns ? PZ_DATASET_VERTEX_RUNS : 0u;
     return PZ_OK;
 }
 

@@ -147,6 +147,113 @@ __global__ __launch_bounds__(kBlock) void k_object_stats(const float* __restrict
     flush();
 }
 
+// ------------------------------------------- object bounds over vertex runs
+// A dataset whose objects own disjoint, gap-free vertex runs (every vertex of
+// object o's run [vbeg, vend] is referenced by o's triangles and by no other
+// object's) has per-object boxes that are the boxes of those runs, so the
+// per-join bounds read each vertex once (12 B) instead of every triangle's
+// index triple, object id and three gathered corners.  The runs, and the
+// per-object triangle ranges, are topology: they are found once, at upload,
+// by the two kernels below and checked on the host.
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+    for (int sft = 32; sft > 0; sft >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, sft, 64));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+    for (int sft = 32; sft > 0; sft >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, sft, 64));
+    return v;
+}
+
+// Per object: first and last triangle, triangle count, least and greatest
+// vertex index; per vertex: the object referencing it (a second object sets
+// the clash flag).  Thread per triangle; a wave whose valid triangles share
+// one object reduces before its atomics.
+__global__ __launch_bounds__(kBlock) void k_topo_tris(const uint32_t* __restrict__ idx, const int32_t* __restrict__ t2o,
+                                                     uint64_t nt, int32_t nobj, uint32_t* __restrict__ first,
+                                                     uint32_t* __restrict__ last, uint32_t* __restrict__ cnt,
+                                                     uint32_t* __restrict__ vmin, uint32_t* __restrict__ vmax,
+                                                     int32_t* __restrict__ owner, uint32_t* __restrict__ clash) {
+    const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int32_t o = t < nt ? t2o[t] : -1;
+    const bool valid = o >= 0 && o < nobj;
+    uint32_t mn = 0xFFFFFFFFu, mx = 0u;
+    bool cl = false;
+    if (valid) {
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t i = idx[3 * t + k];
+            mn = min(mn, i);
+            mx = max(mx, i);
+            const int32_t old = atomicCAS(&owner[i], -1, o);
+            cl |= old != -1 && old != o;
+        }
+    }
+    if (cl) atomicOr(clash, 1u);
+    const uint64_t vm = __ballot(valid);
+    if (vm == 0) return;
+    const int32_t o0 = __shfl(o, __ffsll((long long)vm) - 1, 64);
+    if (__ballot(valid && o != o0) == 0) {
+        const uint32_t tmn = wave_min_u32(valid ? (uint32_t)t : 0xFFFFFFFFu);
+        const uint32_t tmx = wave_max_u32(valid ? (uint32_t)t : 0u);
+        const uint32_t vmn = wave_min_u32(mn), vmx = wave_max_u32(mx);
+        if (__lane_id() == 0) {
+            atomicMin(&first[o0], tmn);
+            atomicMax(&last[o0], tmx);
+            atomicAdd(&cnt[o0], (uint32_t)__popcll(vm));
+            atomicMin(&vmin[o0], vmn);
+            atomicMax(&vmax[o0], vmx);
+        }
+    } else if (valid) {
+        atomicMin(&first[o], (uint32_t)t);
+        atomicMax(&last[o], (uint32_t)t);
+        atomicAdd(&cnt[o], 1u);
+        atomicMin(&vmin[o], mn);
+        atomicMax(&vmax[o], mx);
+    }
+}
+
+// Vertices some object references.
+__global__ __launch_bounds__(kBlock) void k_topo_verts(const int32_t* __restrict__ owner, uint64_t nv,
+                                                      unsigned long long* __restrict__ referenced) {
+    const uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t n = wave_sum((uint32_t)(v < nv && owner[v] >= 0));
+    if (__lane_id() == 0 && n) atomicAdd(referenced, (unsigned long long)n);
+}
+
+// Wave per chunk (object, first vertex) of at most kRunChunk vertices of one
+// run: ordered-uint minima and maxima (a total order, so the result does not
+// depend on the order of the reduction), one set of atomics per chunk.
+constexpr uint32_t kRunChunk = 4096;
+
+__global__ __launch_bounds__(kBlock) void k_object_bounds_runs(const float* __restrict__ verts,
+                                                              const uint2* __restrict__ chunks, uint64_t nch,
+                                                              const uint32_t* __restrict__ vmax,
+                                                              uint32_t* __restrict__ obox) {
+    const uint64_t c = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) / PZ_WAVE;
+    if (c >= nch) return;  // wave-uniform
+    const uint2 ch = chunks[c];
+    const uint32_t o = ch.x, v0 = ch.y;
+    const uint32_t v1 = min(v0 + kRunChunk - 1u, vmax[o]);  // last vertex of the chunk
+    uint32_t lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u};
+#pragma unroll 4
+    for (uint32_t v = v0 + __lane_id(); v <= v1; v += PZ_WAVE) {
+        const float3 p = vtx(verts, v);
+        const uint32_t q[3] = {f2ord(p.x), f2ord(p.y), f2ord(p.z)};
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = min(lo[a], q[a]);
+            hi[a] = max(hi[a], q[a]);
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = wave_min_u32(lo[a]);
+        hi[a] = wave_max_u32(hi[a]);
+    }
+    if (__lane_id() == 0)
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(&obox[6ull * o + a], lo[a]);
+            atomicMax(&obox[6ull * o + 3 + a], hi[a]);
+        }
+}
+
 __global__ __launch_bounds__(kBlock) void k_edge_count(const int32_t* __restrict__ eo, uint64_t ne, int32_t nobj,
                                                        uint32_t* __restrict__ edge_cnt) {
     uint64_t e = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -712,16 +819,6 @@ __device__ __forceinline__ void leaf_box(const LeafTri& L, float* lo, float* hi)
     hi[0] = fmaxf(L.v0.x, fmaxf(L.v1.x, L.v2.x));
     hi[1] = fmaxf(L.v0.y, fmaxf(L.v1.y, L.v2.y));
     hi[2] = fmaxf(L.v0.z, fmaxf(L.v1.z, L.v2.z));
-}
-
-__device__ __forceinline__ void child_box(const Node& nd, int slot, float* lo, float* hi) {
-    if (slot == 0) {
-        lo[0] = nd.a.x; lo[1] = nd.a.y; lo[2] = nd.a.z;
-        hi[0] = nd.a.w; hi[1] = nd.b.x; hi[2] = nd.b.y;
-    } else {
-        lo[0] = nd.b.z; lo[1] = nd.b.w; lo[2] = nd.c.x;
-        hi[0] = nd.c.y; hi[1] = nd.c.z; hi[2] = nd.c.w;
-    }
 }
 
 __device__ __forceinline__ bool box_box(const float* alo, const float* ahi, const float* blo, const float* bhi) {
@@ -1574,6 +1671,37 @@ void object_stats(const float* verts, uint64_t nv, const uint32_t* idx, const in
         hipLaunchKernelGGL(k_object_stats, dim3(grid1(waves * PZ_WAVE)), dim3(kBlock), 0, s, verts, nv, idx, t2o, nt,
                            nobj, obox,
                            tri_cnt, first_tri, last_tri);
+}
+
+void topology_scan(const uint32_t* idx, const int32_t* t2o, uint64_t nt, uint64_t nv, int32_t nobj, uint32_t* first,
+                   uint32_t* last, uint32_t* cnt, uint32_t* vmin, uint32_t* vmax, int32_t* owner, uint32_t* clash,
+                   unsigned long long* referenced, hipStream_t s) {
+    const uint64_t n = (uint64_t)std::max(nobj, 1);
+    hipLaunchKernelGGL((k_fill_any<uint32_t>), dim3(grid1(n)), dim3(kBlock), 0, s, first, 0xFFFFFFFFu, n);
+    hipLaunchKernelGGL((k_fill_any<uint32_t>), dim3(grid1(n)), dim3(kBlock), 0, s, last, 0u, n);
+    hipLaunchKernelGGL((k_fill_any<uint32_t>), dim3(grid1(n)), dim3(kBlock), 0, s, cnt, 0u, n);
+    hipLaunchKernelGGL((k_fill_any<uint32_t>), dim3(grid1(n)), dim3(kBlock), 0, s, vmin, 0xFFFFFFFFu, n);
+    hipLaunchKernelGGL((k_fill_any<uint32_t>), dim3(grid1(n)), dim3(kBlock), 0, s, vmax, 0u, n);
+    if (nv)
+        hipLaunchKernelGGL((k_fill_any<int32_t>), dim3(grid1(nv)), dim3(kBlock), 0, s, owner, (int32_t)-1, nv);
+    hipLaunchKernelGGL((k_fill_any<uint32_t>), dim3(1), dim3(kBlock), 0, s, clash, 0u, (uint64_t)1);
+    hipLaunchKernelGGL((k_fill_any<unsigned long long>), dim3(1), dim3(kBlock), 0, s, referenced, 0ull, (uint64_t)1);
+    if (nt)
+        hipLaunchKernelGGL(k_topo_tris, dim3(grid1(nt)), dim3(kBlock), 0, s, idx, t2o, nt, nobj, first, last, cnt,
+                           vmin, vmax, owner, clash);
+    if (nv) hipLaunchKernelGGL(k_topo_verts, dim3(grid1(nv)), dim3(kBlock), 0, s, owner, nv, referenced);
+}
+
+uint32_t run_chunk_vertices() { return kRunChunk; }
+
+void object_bounds_runs(const float* verts, const uint2* chunks, uint64_t nch, const uint32_t* vmax, int32_t nobj,
+                        uint32_t* obox, hipStream_t s) {
+    if (nobj <= 0) return;
+    hipLaunchKernelGGL(k_init_boxes, dim3(grid1((uint64_t)nobj)), dim3(kBlock), 0, s, obox, (int64_t)nobj);
+    KernelTimer kt_("object_bounds_runs", s);
+    if (nch)
+        hipLaunchKernelGGL(k_object_bounds_runs, dim3(grid1(nch * PZ_WAVE)), dim3(kBlock), 0, s, verts, chunks, nch,
+                           vmax, obox);
 }
 
 void edge_count(const int32_t* eo, uint64_t ne, int32_t nobj, uint32_t* edge_cnt, hipStream_t s) {

@@ -48,6 +48,21 @@ int stack_for_height(uint32_t h);
 void object_stats(const float* verts, uint64_t nv, const uint32_t* idx, const int32_t* t2o, uint64_t nt, int32_t nobj,
                   uint32_t* obox, uint32_t* tri_cnt, uint32_t* first_tri, uint32_t* last_tri, hipStream_t s);
 
+// Topology of a dataset, once at upload: per object the first and last
+// triangle, the triangle count and the least and greatest vertex index; per
+// vertex the referencing object (owner, nv entries; a vertex referenced by
+// two objects sets *clash); *referenced = vertices some object references.
+void topology_scan(const uint32_t* idx, const int32_t* t2o, uint64_t nt, uint64_t nv, int32_t nobj, uint32_t* first,
+                   uint32_t* last, uint32_t* cnt, uint32_t* vmin, uint32_t* vmax, int32_t* owner, uint32_t* clash,
+                   unsigned long long* referenced, hipStream_t s);
+// Vertices per chunk of object_bounds_runs.
+uint32_t run_chunk_vertices();
+// Per-object boxes of a dataset whose objects own disjoint, gap-free vertex
+// runs: chunks[c] = (object, first vertex) of at most run_chunk_vertices()
+// vertices of one run, which ends at vmax[object].
+void object_bounds_runs(const float* verts, const uint2* chunks, uint64_t nch, const uint32_t* vmax, int32_t nobj,
+                        uint32_t* obox, hipStream_t s);
+
 // --- small-object LDS builder (build_small.hip)
 // Objects with at most kSmallMax triangles stored contiguously are built
 // entirely in LDS by one workgroup each; the rest use the global pipeline.

@@ -76,6 +76,13 @@ struct pz_dataset {
     // instanced into a context slot: the arrays belong to the context's pool
     int32_t slot = -1;
     uint64_t slot_gen = 0;
+    // Objects own disjoint, gap-free vertex runs (found at upload): device
+    // per-object first and last triangle, triangle count and last vertex, and
+    // the (object, first vertex) chunks the per-join bounds kernel reads.
+    bool vruns = false;
+    uint32_t* topo = nullptr;  // first[n], last[n], cnt[n], vmax[n] (n = max(nobj, 1))
+    uint2* topo_chunks = nullptr;
+    uint64_t topo_nchunks = 0;
 };
 
 // A dataset whose slot has been reused no longer holds its own geometry.
@@ -176,6 +183,7 @@ struct SetPrep {
 SetPrep prep_set(pz_ctx* c, const pz_dataset* d, const std::string& tag);
 // Unique id of a new dataset (keys the captured HIP graphs).
 uint64_t next_dataset_uid();
+void analyze_topology(pz_ctx* c, pz_dataset* d);
 
 // LBVH over the object boxes of one set (broad-phase target).
 struct ObjBvh {
