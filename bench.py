@@ -527,8 +527,8 @@ def main():
         pz.profile_enable(False)
         kernels = {}
         for k in ("object_stats", "object_bounds_runs", "edge_count", "tri_keys", "edge_scatter", "karras", "refit", "gather_leaves",
-                  "radix_scatter", "build_small", "build_chunks", "narrow_dual", "narrow", "parity", "features",
-                  "mlp_infer"):
+                  "radix_scatter", "build_small", "build_chunks", "narrow_dual", "narrow", "parity", "pair_sieve",
+                  "escalate_tris", "sieve_parity", "features", "mlp_infer"):
             tot, n = pz.profile_query(k)
             if n:
                 kernels[k] = (tot, n)

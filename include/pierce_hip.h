@@ -246,7 +246,11 @@ typedef struct {
     int32_t  no_alpha_correction;
     int32_t  keep_device_result; /* reserved, must be 0 */
     const pz_mlp* model;        /* PZ_SIZING_LEARNED */
-    int32_t  narrow_mode;       /* two-level: 0 = dual-tree with per-edge fallback, 1 = per-edge only */
+    int32_t  narrow_mode;       /* two-level: 0 = pair sieve (candidate pairs of objects with at most
+                                   8,192 contiguous triangles, decided in LDS without trees), the
+                                   other pairs through per-object trees as in 2; 1 = per-object
+                                   trees, per-edge kernel only; 2 = per-object trees, dual-tree walk
+                                   with per-edge fallback */
     int32_t  broad_mode;        /* two-level: 0 = auto (box scan for <= 4096 mesh2 objects), 1 = object
                                    LBVH walk, 2 = box scan (every mesh1 box against every mesh2 box) */
     int32_t  reserved[6];
@@ -299,6 +303,8 @@ typedef struct {
     uint64_t parity_rays_crossing; /* rays that crossed at least one triangle */
     uint64_t parity_inside;     /* rays that crossed an odd number of triangles (point inside) */
     uint64_t parity_tests;      /* ray-triangle tests of those rays (part of tests_total) */
+    uint64_t sieve_pairs;       /* two-level, narrow_mode 0: candidate pairs the pair sieve decided
+                                   (the others went to the per-object trees) */
     uint32_t index_height;      /* tallest triangle BVH the join built (at most 63, else PZ_E_LIMIT) */
     uint32_t reserved_u32;
     double   ms_total;          /* device-resident geometry -> sorted host pair set */
@@ -454,7 +460,9 @@ pz_status pz_tune_set(pz_ctx* ctx, const pz_tune* in);
  * workgroup size and times every size class's builder launches with HIP
  * events, keeping the fastest per class; then once per queue length and once
  * per work-queue chunk, keeping the fastest narrow phase for the workload's
- * class.  The choices stay in the context (13 joins in all). */
+ * class.  The choices stay in the context (13 joins in all).  The tuned
+ * parameters belong to the per-object trees, so a desc with narrow_mode 0
+ * (pair sieve) is tuned with narrow_mode 2 (every pair through the trees). */
 pz_status pz_autotune(pz_ctx* ctx, const pz_dataset* mesh1, const pz_dataset* mesh2, const pz_join_desc* desc,
                       pz_tune_report* report);
 

@@ -52,17 +52,16 @@ uint8_t pair_meta_of(const pz_join_desc& d) {
 void analyze_topology(pz_ctx* c, pz_dataset* d) {
     const uint64_t n = (uint64_t)std::max(d->nobj, 1);
     if (d->nobj <= 0 || d->nt == 0 || d->nv >= (1ull << 32) - pzk::run_chunk_vertices()) return;
-    uint32_t* topo = nullptr;
-    uint32_t* vmin = nullptr;
+    uint32_t* topo = nullptr;  // first, last, cnt, vmax, vmin (n each)
     int32_t* owner = nullptr;
     uint32_t* flags = nullptr;  // clash flag, then the referenced-vertex count (8 B)
     auto cleanup = [&]() {
-        for (void* p : {(void*)vmin, (void*)owner, (void*)flags})
+        for (void* p : {(void*)owner, (void*)flags})
             if (p) (void)hipFree(p);
     };
     try {
-        HIP_OK(hipMalloc((void**)&topo, 16 * n));
-        HIP_OK(hipMalloc((void**)&vmin, 4 * n));
+        HIP_OK(hipMalloc((void**)&topo, 20 * n));
+        uint32_t* vmin = topo + 4 * n;
         HIP_OK(hipMalloc((void**)&owner, 4 * std::max<uint64_t>(d->nv, 1)));
         HIP_OK(hipMalloc((void**)&flags, 16));
         pzk::topology_scan(d->idx, d->t2o, d->nt, d->nv, d->nobj, topo, topo + n, topo + 2 * n, vmin, topo + 3 * n,
@@ -88,7 +87,6 @@ void analyze_topology(pz_ctx* c, pz_dataset* d) {
         }
         ok = ok && run_total == referenced;
         cleanup();
-        vmin = nullptr;
         owner = nullptr;
         flags = nullptr;
         if (!ok) {
@@ -534,272 +532,334 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         return;
     }
 
-    ph.begin("Build Mesh Index");
     // Per-pair meta = predicate | direction bits << 2 (bit 0: mesh1 edges vs mesh2
     // triangles and mesh1 points in mesh2 objects; bit 1: the reverse).
     uint8_t* meta = buf<uint8_t>(c, "np.meta", np + 1);
     pzk::pair_meta(pairs, np, batched ? A->part_of : nullptr, meta_of_part, pz::pair_meta_of(d), meta, s);
-    Needed na = needed_layout(c, A, pa, "A"), nb = needed_layout(c, B, pb, "B");
-    const size_t sA = std::max(nA, 1), sB = std::max(nB, 1);
-    pzk::fill_u32(na.need, 0u, sA, s);
-    pzk::fill_u32(nb.need, 0u, sB, s);
-    pzk::mark_needed(pairs, np, na.need, nb.need, s);
-    struct Tot {
-        uint64_t nneed, ntri, nedge, nbig, nbig_tri;
-        uint32_t ncls[pzk::kSmallClasses], cls_max[pzk::kSmallClasses];  // LDS-built objects per size class
-    };
-    // Needed objects of one set: objects built by the global pipeline ("big":
-    // more than kSmallMax triangles or not stored contiguously) take the first
-    // segment indices, LDS-built ones follow.
-    auto layout = [&](const pz_dataset* D, const SetPrep& P, Needed& N, const std::string& tag) -> uint32_t* {
-        size_t n = std::max(D->nobj, 1);
-        const uint64_t no = (uint64_t)std::max(D->nobj, 0);
-        uint32_t* stc = buf<uint32_t>(c, tag + ".seg_tri_cnt", n);
-        uint32_t* sec = buf<uint32_t>(c, tag + ".seg_edge_cnt", n);
-        uint32_t* big = buf<uint32_t>(c, tag + ".big", n);
-        uint32_t* cls = buf<uint32_t>(c, tag + ".cls", pzk::kSmallClasses * (n + 1));
-        uint32_t* loc_big = buf<uint32_t>(c, tag + ".loc_big", n + 1);
-        uint32_t* loc_cls = buf<uint32_t>(c, tag + ".loc_cls", pzk::kSmallClasses * (n + 1));
-        uint32_t* totals = buf<uint32_t>(c, tag + ".totals", 8);
-        pzk::fill_u32(stc, 0u, n, s);
-        pzk::fill_u32(sec, 0u, n, s);
-        pzk::fill_u32(totals, 0u, 8, s);
-        void* tmp = buf<char>(c, "scan.tmp", pzk::scan_temp_bytes(n + 1));
-        pzk::need_split(N.need, P.tri_cnt, P.first_tri, P.last_tri, D->nobj, big, cls, totals, s);
-        pzk::exclusive_scan_u32_u32(big, loc_big, no, tmp, s);
-        for (int k = 0; k < pzk::kSmallClasses; ++k)
-            pzk::exclusive_scan_u32_u32(cls + (size_t)k * (no + 1), loc_cls + (size_t)k * (no + 1), no, tmp, s);
-        pzk::merge_locs(big, loc_big, loc_cls, D->nobj, N.loc, totals, s);
-        pzk::needed_layout(N.need, N.loc, D->nobj, P.tri_cnt, P.edge_cnt, N.obj_of_loc, stc, sec, s);
-        pzk::exclusive_scan_u32_u32(stc, N.seg_off, no, tmp, s);
-        pzk::big_prefix(N.seg_off, totals, s);
-        return big;
-    };
-    // about 40 small launches whose shapes depend only on both sets' object
-    // counts: one graph from the second join of the same datasets on
-    const SegOut lay = run_segment(c, A->uid, B->uid, kSegLayout, r, [&]() {
-        SegOut o;
-        o.big[0] = layout(A, pa, na, "A");
-        o.big[1] = layout(B, pb, nb, "B");
-        return o;
-    });
-    uint32_t* bigA = lay.big[0];
-    uint32_t* bigB = lay.big[1];
-    uint32_t* totA = buf<uint32_t>(c, "A.totals", 8);
-    uint32_t* totB = buf<uint32_t>(c, "B.totals", 8);
-    // needed objects and their triangles, then totals[0..7] of each set, in one synchronisation
-    uint32_t* hb = static_cast<uint32_t*>(pinned(c, 20 * sizeof(uint32_t)));
-    pzk::HostGather g{{na.loc + std::max(nA, 0), na.seg_off + std::max(nA, 0), totA, nb.loc + std::max(nB, 0),
-                       nb.seg_off + std::max(nB, 0), totB},
-                      {1, 1, 8, 1, 1, 8},
-                      6};
-    pzk::gather_to_host(g, hb, s);
-    HIP_OK(hipStreamSynchronize(s));
-    auto tot_of = [&](const uint32_t* h) {
-        Tot t{h[0], h[1], 0, h[2], h[3], {}, {}};
-        for (int k = 0; k < pzk::kSmallClasses; ++k) {
-            t.ncls[k] = h[4 + k];
-            t.cls_max[k] = h[7 + k];
-        }
-        return t;
-    };
-    const Tot ta = tot_of(hb), tb = tot_of(hb + 10);
-
-    auto build = [&](const pz_dataset* D, const SetPrep& P, const Needed& N, const Tot& T, uint32_t* big,
-                     const std::string& tag) -> SetAccel {
-        SetAccel acc;
-        const uint64_t n = T.ntri, nb_tri = T.nbig_tri;
-        acc.nodes = buf<pzd::BNode>(c, tag + ".nodes", n + 1);
-        acc.leaves = buf<pzd::LeafTri>(c, tag + ".leaves", n + 1);
-        acc.height = buf<uint32_t>(c, tag + ".height", 1);
-        pzk::fill_u32(acc.height, 0u, 1, s);
-        uint32_t* cursor = buf<uint32_t>(c, tag + ".cursor", (size_t)std::max(D->nobj, 1));
-        if (nb_tri > 0) {  // global pipeline for the big objects (segment prefix)
-            // sort buffers are shared by both sets' builds (one after the other on the stream)
-            uint64_t* keys = buf<uint64_t>(c, "big.tkeys", nb_tri + 1);
-            uint32_t* vals = buf<uint32_t>(c, "big.tvals", nb_tri + 1);
-            uint64_t* keys2 = buf<uint64_t>(c, "big.tkeys2", nb_tri + 1);
-            uint32_t* vals2 = buf<uint32_t>(c, "big.tvals2", nb_tri + 1);
-            pzk::fill_u32(cursor, 0u, (size_t)std::max(D->nobj, 1), s);
-            // Morton bits of the keys: of 30, 27, 24 and 21, the most that need the
-            // fewest 8-bit radix passes beside the segment bits (chunks only need
-            // a coherent order; each chunk's tree uses the bits below its common prefix)
-            const int sb = bits_for(std::max<uint64_t>(T.nbig, 1));
-            int mbits = 30;
-            for (int m : {27, 24, 21})
-                if ((m + sb + 7) / 8 < (mbits + sb + 7) / 8) mbits = m;
-            pzk::tri_keys_two_level(D->v, D->nv, D->idx, D->t2o, D->nt, D->nobj, big, N.loc, P.obox, N.seg_off,
-                                    P.first_tri, P.last_tri, P.tri_cnt, cursor, keys, vals, mbits, s);
-            void* tmp = buf<char>(c, "sort.tmp." + tag, pzk::radix_sort_temp_bytes(nb_tri));
-            bool alt = pzk::radix_sort_pairs(keys, vals, keys2, vals2, nb_tri, 0, mbits + sb, tmp, s);
-            uint64_t* sk = alt ? keys2 : keys;
-            uint32_t* order = alt ? vals2 : vals;
-            // runs of the sorted order through the LDS builder, then a top tree per object
-            const uint64_t nch = pzk::big_chunk_bound(T.nbig, nb_tri);
-            uint32_t* ccnt = buf<uint32_t>(c, tag + ".chunk_cnt", T.nbig + 1);
-            uint32_t* coff = buf<uint32_t>(c, tag + ".chunk_off", T.nbig + 1);
-            auto* e0 = buf<pzk::ChunkRoot>(c, tag + ".chunk_ent0", nch);
-            auto* e1 = buf<pzk::ChunkRoot>(c, tag + ".chunk_ent1", nch);
-            void* stmp = buf<char>(c, "scan.tmp", pzk::scan_temp_bytes(T.nbig + 1));
-            pzk::build_big_chunked(D->v, D->idx, N.obj_of_loc, N.seg_off, (uint32_t)T.nbig, nb_tri, sk, mbits, order, ccnt,
-                                   coff, stmp, e0, e1, acc.nodes, acc.leaves, acc.height, s);
-        }
-        static const char* const kTimer[pzk::kSmallClasses] = {"build_small.c0", "build_small.c1",
-                                                                "build_small.c2"};
-        uint32_t l0 = (uint32_t)T.nbig;
-        for (int k = 0; k < pzk::kSmallClasses; ++k) {  // one launch per size class
-            if (!pzk::build_small(D->v, D->idx, N.obj_of_loc, P.first_tri, P.tri_cnt, P.obox, N.seg_off, l0,
-                                  T.ncls[k], T.cls_max[k], c->tune.build_threads[k], acc.nodes, acc.leaves,
-                                  acc.height, kTimer[k], s))
-                throw PzError(PZ_E_INVALID, "invalid LDS-builder configuration");
-            l0 += T.ncls[k];
-        }
-        acc.edges = nullptr;
-        return acc;
-    };
-    SetAccel xa = build(A, pa, na, ta, bigA, "A");
-    SetAccel xb = build(B, pb, nb, tb, bigB, "B");
-    r->index_triangles = ta.ntri + tb.ntri;
-
-    ph.begin("Narrow Phase");
-    uint32_t* found = buf<uint32_t>(c, "np.found", np + 1);
-    uint32_t* fallback = buf<uint32_t>(c, "np.fallback", np + 1);
-    pzk::fill_u32(found, 0u, np + 1, s);
-    pzk::fill_u32(fallback, 0u, np + 1, s);
-    pzd::SetView va{xa.nodes, xa.leaves, na.seg_off, nullptr, na.eseg_off, na.loc, pa.obox};
-    pzd::SetView vb{xb.nodes, xb.leaves, nb.seg_off, nullptr, nb.eseg_off, nb.loc, pb.obox};
     const bool f32 = d.precision == PZ_PREC_FP32_FILTERED;
     const int allow_f32 = (std::max(A->max_abs, B->max_abs) <= 0x1p39f) ? 1 : 0;
-    uint64_t nfb = np;
-    const uint64_t* fb_pairs = pairs;
-    uint32_t* fb_index = nullptr;
-    uint32_t height = 0;  // of the per-object BVHs of both sets
-    if (d.narrow_mode == 0) {
-        const int qclass = (ta.nbig + tb.nbig > 0) ? 1 : 0;  // pairs with objects beyond the LDS builder
-        c->last_queue_class = qclass;
-        // Work queue of (pair, subtree pair) tasks, run in rounds: round 0 takes
-        // every candidate pair from its roots; a wave whose LDS queue fills up
-        // hands its shallowest subtree pairs to the next round.
-        // spill capacity per round (16 B a task): tasks beyond it send their pair to
-        // the per-edge kernel, which is slow on big objects, so keep it generous
-        const uint64_t cap_tasks = std::min<uint64_t>(std::max<uint64_t>(64 * np, 1ull << 20), 1ull << 24);
-        pzd::DualTask* tin = buf<pzd::DualTask>(c, "np.tasks0", cap_tasks);
-        pzd::DualTask* tout = buf<pzd::DualTask>(c, "np.tasks1", cap_tasks);
-        unsigned long long* ctr = buf<unsigned long long>(c, "np.task_ctr", 3);  // next task, spilled, open leaf pairs
-        // leaf pairs the fp32 walk leaves open, decided after the rounds
-        const uint64_t cap_esc = std::min<uint64_t>(std::max<uint64_t>(np, 1 << 16), 1ull << 22);
-        uint4* esc = buf<uint4>(c, "np.esc", cap_esc);
-        pzk::fill_ull(ctr + 2, 0ull, 1, s);
-        if (c->persistent_blocks == 0) {
-            int cus = 0;
-            HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-            c->persistent_blocks = (unsigned)std::max(cus, 1) * 8u;
-        }
-        const pzd::DualTask* cur = nullptr;
-        uint64_t ntasks = np, spilled_total = 0;
-        int rounds = 0;
-        uint64_t* fbp = buf<uint64_t>(c, "np.fb_pairs", np + 1);
-        fb_index = buf<uint32_t>(c, "np.fb_index", np + 1);
-        unsigned long long* fbc = buf<unsigned long long>(c, "np.fb_count", 1);
-        // after the last round: decide the open leaf pairs, list the fallback pairs
-        // The escalation list then starts empty again, so each open leaf pair is
-        // decided (and counted) once, whatever the number of rounds.
-        auto tail = [&]() {
-            pzk::escalate(esc, ctr + 2, cap_esc, va, vb, meta, found, stats, s);
-            pzk::fill_ull(ctr + 2, 0ull, 1, s);
-            pzk::fill_ull(fbc, 0ull, 1, s);
-            pzk::compact_flagged(pairs, np, fallback, fbp, fb_index, fbc, s);
-        };
-        while (ntasks > 0) {
-            pzk::fill_ull(ctr, 0ull, 2, s);
-            if (!pzk::narrow_dual(pairs, va, vb, found, fallback, meta, f32, allow_f32, c->tune.narrow_queue[qclass],
-                                  cur, ntasks, ctr, c->tune.narrow_chunk[qclass], tout, ctr + 1, cap_tasks, esc,
-                                  ctr + 2, cap_esc, c->persistent_blocks, stats, s))
-                throw PzError(PZ_E_INVALID, "invalid narrow-phase queue length");
-            ++rounds;
-            // The tail runs as if this were the last round (it usually is), so one
-            // read-back brings the spill count, the fallback count and both heights.
-            tail();
-            uint32_t* hb = static_cast<uint32_t*>(pinned(c, 6 * sizeof(uint32_t)));
-            pzk::HostGather g{{reinterpret_cast<const uint32_t*>(ctr + 1), reinterpret_cast<const uint32_t*>(fbc),
-                               xa.height, xb.height},
-                              {2, 2, 1, 1},
-                              4};
-            pzk::gather_to_host(g, hb, s);
-            HIP_OK(hipStreamSynchronize(s));
-            const uint64_t spilled = std::min<uint64_t>(hb[0] | ((uint64_t)hb[1] << 32), cap_tasks);
-            nfb = hb[2] | ((uint64_t)hb[3] << 32);
-            height = std::max(hb[4], hb[5]);
-            spilled_total += spilled;
-            if (spilled == 0) break;
-            if (rounds >= 32) {  // leave what is left to the per-edge kernel
-                pzk::mark_task_pairs(tout, spilled, fallback, s);
-                tail();
-                nfb = readback<unsigned long long>(c, {fbc})[0];
-                break;
-            }
-            std::swap(tin, tout);
-            cur = tin;
-            ntasks = spilled;
-        }
-        r->narrow_rounds = rounds;
-        r->tasks_spilled = spilled_total;
-        fb_pairs = fbp;
-    } else {
-        auto hh = readback<uint32_t>(c, {xa.height, xb.height});
-        height = std::max(hh[0], hh[1]);
+    uint32_t* found = buf<uint32_t>(c, "np.found", np + 1);
+    pzk::fill_u32(found, 0u, np + 1, s);
+    uint32_t* inside = nullptr;  // parity bits, for predicates and batches that need them
+    if (d.predicate != PZ_PRED_OVERLAP || batched) {
+        inside = buf<uint32_t>(c, "np.inside", np + 1);
+        pzk::fill_u32(inside, 0u, np + 1, s);
     }
-    r->index_height = height;
-    if (pzk::stack_for_height(height) == 0)
-        throw PzError(PZ_E_LIMIT, "per-object BVH height " + std::to_string(height) + " exceeds the traversal stack (63)");
-    r->narrow_fallback_pairs = d.narrow_mode == 0 ? nfb : 0;  // pairs re-run by the per-edge kernel
-    if (nfb > 0) {
-        // Per-edge narrow phase: every edge of the .pre payload regrouped by object.
-        auto edges_of = [&](const pz_dataset* D, const SetPrep& P, Needed& N, const std::string& tag) {
-            const size_t n = std::max(D->nobj, 1);
+
+    // The tree path: per-object BVHs of the objects of the given pairs, the
+    // dual-tree walk (or the per-edge kernel) and the parity rays through the
+    // trees; writes found[] and inside[] of those pairs.
+    auto tree_path = [&](const uint64_t* pairs, uint64_t np, uint8_t* meta, uint32_t* found, uint32_t* inside) {
+    ph.begin("Build Mesh Index");
+        Needed na = needed_layout(c, A, pa, "A"), nb = needed_layout(c, B, pb, "B");
+        const size_t sA = std::max(nA, 1), sB = std::max(nB, 1);
+        pzk::fill_u32(na.need, 0u, sA, s);
+        pzk::fill_u32(nb.need, 0u, sB, s);
+        pzk::mark_needed(pairs, np, na.need, nb.need, s);
+        struct Tot {
+            uint64_t nneed, ntri, nedge, nbig, nbig_tri;
+            uint32_t ncls[pzk::kSmallClasses], cls_max[pzk::kSmallClasses];  // LDS-built objects per size class
+        };
+        // Needed objects of one set: objects built by the global pipeline ("big":
+        // more than kSmallMax triangles or not stored contiguously) take the first
+        // segment indices, LDS-built ones follow.
+        auto layout = [&](const pz_dataset* D, const SetPrep& P, Needed& N, const std::string& tag) -> uint32_t* {
+            size_t n = std::max(D->nobj, 1);
             const uint64_t no = (uint64_t)std::max(D->nobj, 0);
             uint32_t* stc = buf<uint32_t>(c, tag + ".seg_tri_cnt", n);
             uint32_t* sec = buf<uint32_t>(c, tag + ".seg_edge_cnt", n);
-            pzk::edge_count(D->eo, D->ne, D->nobj, P.edge_cnt, s);
+            uint32_t* big = buf<uint32_t>(c, tag + ".big", n);
+            uint32_t* cls = buf<uint32_t>(c, tag + ".cls", pzk::kSmallClasses * (n + 1));
+            uint32_t* loc_big = buf<uint32_t>(c, tag + ".loc_big", n + 1);
+            uint32_t* loc_cls = buf<uint32_t>(c, tag + ".loc_cls", pzk::kSmallClasses * (n + 1));
+            uint32_t* totals = buf<uint32_t>(c, tag + ".totals", 8);
+            pzk::fill_u32(stc, 0u, n, s);
             pzk::fill_u32(sec, 0u, n, s);
-            pzk::needed_layout(N.need, N.loc, D->nobj, P.tri_cnt, P.edge_cnt, N.obj_of_loc, stc, sec, s);
+            pzk::fill_u32(totals, 0u, 8, s);
             void* tmp = buf<char>(c, "scan.tmp", pzk::scan_temp_bytes(n + 1));
-            pzk::exclusive_scan_u32_u32(sec, N.eseg_off, no, tmp, s);
-            uint64_t ne = readback<uint32_t>(c, {N.eseg_off + no})[0];
-            pzd::EdgeRec* edges = buf<pzd::EdgeRec>(c, tag + ".edges", ne + 1);
-            uint32_t* cursor = buf<uint32_t>(c, tag + ".cursor", n);
-            pzk::fill_u32(cursor, 0u, n, s);
-            pzk::edge_scatter(D->es, D->ee, D->eo, D->ne, D->nobj, N.need, N.loc, N.eseg_off, cursor, edges, s);
-            r->index_edges += ne;
-            return edges;
+            pzk::need_split(N.need, P.tri_cnt, P.first_tri, P.last_tri, D->nobj, big, cls, totals, s);
+            pzk::exclusive_scan_u32_u32(big, loc_big, no, tmp, s);
+            for (int k = 0; k < pzk::kSmallClasses; ++k)
+                pzk::exclusive_scan_u32_u32(cls + (size_t)k * (no + 1), loc_cls + (size_t)k * (no + 1), no, tmp, s);
+            pzk::merge_locs(big, loc_big, loc_cls, D->nobj, N.loc, totals, s);
+            pzk::needed_layout(N.need, N.loc, D->nobj, P.tri_cnt, P.edge_cnt, N.obj_of_loc, stc, sec, s);
+            pzk::exclusive_scan_u32_u32(stc, N.seg_off, no, tmp, s);
+            pzk::big_prefix(N.seg_off, totals, s);
+            return big;
         };
-        va.edges = edges_of(A, pa, na, "A");
-        vb.edges = edges_of(B, pb, nb, "B");
-        uint32_t* tcnt = buf<uint32_t>(c, "np.tcnt", nfb + 1);
-        uint64_t* toff = buf<uint64_t>(c, "np.toff", nfb + 2);
-        void* tmp = buf<char>(c, "scan.tmp.np", pzk::scan_temp_bytes(nfb + 1));
-        uint8_t* fb_meta = meta;
-        if (fb_index) {  // meta of the compacted fallback pairs
-            fb_meta = buf<uint8_t>(c, "np.fb_meta", nfb + 1);
-            pzk::pair_meta(fb_pairs, nfb, batched ? A->part_of : nullptr, meta_of_part, pz::pair_meta_of(d), fb_meta,
-                           s);
-        }
-        pzk::task_counts(fb_pairs, nfb, na.loc, na.eseg_off, nb.loc, nb.eseg_off, fb_meta, tcnt, s);
-        pzk::exclusive_scan_u32_u64(tcnt, toff, nfb, tmp, s);
-        const uint64_t ntasks = readback<uint64_t>(c, {toff + nfb})[0];
-        uint32_t* sub_found = fb_index ? buf<uint32_t>(c, "np.sub_found", nfb + 1) : found;
-        if (fb_index) pzk::fill_u32(sub_found, 0u, nfb + 1, s);
-        pzk::narrow(fb_pairs, nfb, toff, ntasks, va, vb, sub_found, fb_meta, f32, allow_f32, height, stats, s);
-        if (fb_index) pzk::scatter_found(sub_found, fb_index, nfb, found, s);
-    }
+        // about 40 small launches whose shapes depend only on both sets' object
+        // counts: one graph from the second join of the same datasets on
+        const SegOut lay = run_segment(c, A->uid, B->uid, kSegLayout, r, [&]() {
+            SegOut o;
+            o.big[0] = layout(A, pa, na, "A");
+            o.big[1] = layout(B, pb, nb, "B");
+            return o;
+        });
+        uint32_t* bigA = lay.big[0];
+        uint32_t* bigB = lay.big[1];
+        uint32_t* totA = buf<uint32_t>(c, "A.totals", 8);
+        uint32_t* totB = buf<uint32_t>(c, "B.totals", 8);
+        // needed objects and their triangles, then totals[0..7] of each set, in one synchronisation
+        uint32_t* hb = static_cast<uint32_t*>(pinned(c, 20 * sizeof(uint32_t)));
+        pzk::HostGather g{{na.loc + std::max(nA, 0), na.seg_off + std::max(nA, 0), totA, nb.loc + std::max(nB, 0),
+                           nb.seg_off + std::max(nB, 0), totB},
+                          {1, 1, 8, 1, 1, 8},
+                          6};
+        pzk::gather_to_host(g, hb, s);
+        HIP_OK(hipStreamSynchronize(s));
+        auto tot_of = [&](const uint32_t* h) {
+            Tot t{h[0], h[1], 0, h[2], h[3], {}, {}};
+            for (int k = 0; k < pzk::kSmallClasses; ++k) {
+                t.ncls[k] = h[4 + k];
+                t.cls_max[k] = h[7 + k];
+            }
+            return t;
+        };
+        const Tot ta = tot_of(hb), tb = tot_of(hb + 10);
 
-    uint32_t* inside = nullptr;
-    if (d.predicate != PZ_PRED_OVERLAP || batched) {
-        ph.begin("Containment Parity");
-        inside = buf<uint32_t>(c, "np.inside", np + 1);
-        pzk::fill_u32(inside, 0u, np + 1, s);
-        pzk::parity(pairs, np, found, va, vb, A->v, A->idx, pa.first_tri, B->v, B->idx, pb.first_tri, meta, inside,
-                    height, stats, s);
+        auto build = [&](const pz_dataset* D, const SetPrep& P, const Needed& N, const Tot& T, uint32_t* big,
+                         const std::string& tag) -> SetAccel {
+            SetAccel acc;
+            const uint64_t n = T.ntri, nb_tri = T.nbig_tri;
+            acc.nodes = buf<pzd::BNode>(c, tag + ".nodes", n + 1);
+            acc.leaves = buf<pzd::LeafTri>(c, tag + ".leaves", n + 1);
+            acc.height = buf<uint32_t>(c, tag + ".height", 1);
+            pzk::fill_u32(acc.height, 0u, 1, s);
+            uint32_t* cursor = buf<uint32_t>(c, tag + ".cursor", (size_t)std::max(D->nobj, 1));
+            if (nb_tri > 0) {  // global pipeline for the big objects (segment prefix)
+                // sort buffers are shared by both sets' builds (one after the other on the stream)
+                uint64_t* keys = buf<uint64_t>(c, "big.tkeys", nb_tri + 1);
+                uint32_t* vals = buf<uint32_t>(c, "big.tvals", nb_tri + 1);
+                uint64_t* keys2 = buf<uint64_t>(c, "big.tkeys2", nb_tri + 1);
+                uint32_t* vals2 = buf<uint32_t>(c, "big.tvals2", nb_tri + 1);
+                pzk::fill_u32(cursor, 0u, (size_t)std::max(D->nobj, 1), s);
+                // Morton bits of the keys: of 30, 27, 24 and 21, the most that need the
+                // fewest 8-bit radix passes beside the segment bits (chunks only need
+                // a coherent order; each chunk's tree uses the bits below its common prefix)
+                const int sb = bits_for(std::max<uint64_t>(T.nbig, 1));
+                int mbits = 30;
+                for (int m : {27, 24, 21})
+                    if ((m + sb + 7) / 8 < (mbits + sb + 7) / 8) mbits = m;
+                pzk::tri_keys_two_level(D->v, D->nv, D->idx, D->t2o, D->nt, D->nobj, big, N.loc, P.obox, N.seg_off,
+                                        P.first_tri, P.last_tri, P.tri_cnt, cursor, keys, vals, mbits, s);
+                void* tmp = buf<char>(c, "sort.tmp." + tag, pzk::radix_sort_temp_bytes(nb_tri));
+                bool alt = pzk::radix_sort_pairs(keys, vals, keys2, vals2, nb_tri, 0, mbits + sb, tmp, s);
+                uint64_t* sk = alt ? keys2 : keys;
+                uint32_t* order = alt ? vals2 : vals;
+                // runs of the sorted order through the LDS builder, then a top tree per object
+                const uint64_t nch = pzk::big_chunk_bound(T.nbig, nb_tri);
+                uint32_t* ccnt = buf<uint32_t>(c, tag + ".chunk_cnt", T.nbig + 1);
+                uint32_t* coff = buf<uint32_t>(c, tag + ".chunk_off", T.nbig + 1);
+                auto* e0 = buf<pzk::ChunkRoot>(c, tag + ".chunk_ent0", nch);
+                auto* e1 = buf<pzk::ChunkRoot>(c, tag + ".chunk_ent1", nch);
+                void* stmp = buf<char>(c, "scan.tmp", pzk::scan_temp_bytes(T.nbig + 1));
+                pzk::build_big_chunked(D->v, D->idx, N.obj_of_loc, N.seg_off, (uint32_t)T.nbig, nb_tri, sk, mbits, order, ccnt,
+                                       coff, stmp, e0, e1, acc.nodes, acc.leaves, acc.height, s);
+            }
+            static const char* const kTimer[pzk::kSmallClasses] = {"build_small.c0", "build_small.c1",
+                                                                    "build_small.c2"};
+            uint32_t l0 = (uint32_t)T.nbig;
+            for (int k = 0; k < pzk::kSmallClasses; ++k) {  // one launch per size class
+                if (!pzk::build_small(D->v, D->idx, N.obj_of_loc, P.first_tri, P.tri_cnt, P.obox, N.seg_off, l0,
+                                      T.ncls[k], T.cls_max[k], c->tune.build_threads[k], acc.nodes, acc.leaves,
+                                      acc.height, kTimer[k], s))
+                    throw PzError(PZ_E_INVALID, "invalid LDS-builder configuration");
+                l0 += T.ncls[k];
+            }
+            acc.edges = nullptr;
+            return acc;
+        };
+        SetAccel xa = build(A, pa, na, ta, bigA, "A");
+        SetAccel xb = build(B, pb, nb, tb, bigB, "B");
+        r->index_triangles = ta.ntri + tb.ntri;
+
+        ph.begin("Narrow Phase");
+        uint32_t* fallback = buf<uint32_t>(c, "np.fallback", np + 1);
+        pzk::fill_u32(fallback, 0u, np + 1, s);
+        pzd::SetView va{xa.nodes, xa.leaves, na.seg_off, nullptr, na.eseg_off, na.loc, pa.obox};
+        pzd::SetView vb{xb.nodes, xb.leaves, nb.seg_off, nullptr, nb.eseg_off, nb.loc, pb.obox};
+        uint64_t nfb = np;
+        const uint64_t* fb_pairs = pairs;
+        uint32_t* fb_index = nullptr;
+        uint32_t height = 0;  // of the per-object BVHs of both sets
+        if (d.narrow_mode != 1) {  // the dual-tree walk
+            const int qclass = (ta.nbig + tb.nbig > 0) ? 1 : 0;  // pairs with objects beyond the LDS builder
+            c->last_queue_class = qclass;
+            // Work queue of (pair, subtree pair) tasks, run in rounds: round 0 takes
+            // every candidate pair from its roots; a wave whose LDS queue fills up
+            // hands its shallowest subtree pairs to the next round.
+            // spill capacity per round (16 B a task): tasks beyond it send their pair to
+            // the per-edge kernel, which is slow on big objects, so keep it generous
+            const uint64_t cap_tasks = std::min<uint64_t>(std::max<uint64_t>(64 * np, 1ull << 20), 1ull << 24);
+            pzd::DualTask* tin = buf<pzd::DualTask>(c, "np.tasks0", cap_tasks);
+            pzd::DualTask* tout = buf<pzd::DualTask>(c, "np.tasks1", cap_tasks);
+            unsigned long long* ctr = buf<unsigned long long>(c, "np.task_ctr", 3);  // next task, spilled, open leaf pairs
+            // leaf pairs the fp32 walk leaves open, decided after the rounds
+            const uint64_t cap_esc = std::min<uint64_t>(std::max<uint64_t>(np, 1 << 16), 1ull << 22);
+            uint4* esc = buf<uint4>(c, "np.esc", cap_esc);
+            pzk::fill_ull(ctr + 2, 0ull, 1, s);
+            if (c->persistent_blocks == 0) {
+                int cus = 0;
+                HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+                c->persistent_blocks = (unsigned)std::max(cus, 1) * 8u;
+            }
+            const pzd::DualTask* cur = nullptr;
+            uint64_t ntasks = np, spilled_total = 0;
+            int rounds = 0;
+            uint64_t* fbp = buf<uint64_t>(c, "np.fb_pairs", np + 1);
+            fb_index = buf<uint32_t>(c, "np.fb_index", np + 1);
+            unsigned long long* fbc = buf<unsigned long long>(c, "np.fb_count", 1);
+            // after the last round: decide the open leaf pairs, list the fallback pairs
+            // The escalation list then starts empty again, so each open leaf pair is
+            // decided (and counted) once, whatever the number of rounds.
+            auto tail = [&]() {
+                pzk::escalate(esc, ctr + 2, cap_esc, va, vb, meta, found, stats, s);
+                pzk::fill_ull(ctr + 2, 0ull, 1, s);
+                pzk::fill_ull(fbc, 0ull, 1, s);
+                pzk::compact_flagged(pairs, np, fallback, fbp, fb_index, fbc, s);
+            };
+            while (ntasks > 0) {
+                pzk::fill_ull(ctr, 0ull, 2, s);
+                if (!pzk::narrow_dual(pairs, va, vb, found, fallback, meta, f32, allow_f32, c->tune.narrow_queue[qclass],
+                                      cur, ntasks, ctr, c->tune.narrow_chunk[qclass], tout, ctr + 1, cap_tasks, esc,
+                                      ctr + 2, cap_esc, c->persistent_blocks, stats, s))
+                    throw PzError(PZ_E_INVALID, "invalid narrow-phase queue length");
+                ++rounds;
+                // The tail runs as if this were the last round (it usually is), so one
+                // read-back brings the spill count, the fallback count and both heights.
+                tail();
+                uint32_t* hb = static_cast<uint32_t*>(pinned(c, 6 * sizeof(uint32_t)));
+                pzk::HostGather g{{reinterpret_cast<const uint32_t*>(ctr + 1), reinterpret_cast<const uint32_t*>(fbc),
+                                   xa.height, xb.height},
+                                  {2, 2, 1, 1},
+                                  4};
+                pzk::gather_to_host(g, hb, s);
+                HIP_OK(hipStreamSynchronize(s));
+                const uint64_t spilled = std::min<uint64_t>(hb[0] | ((uint64_t)hb[1] << 32), cap_tasks);
+                nfb = hb[2] | ((uint64_t)hb[3] << 32);
+                height = std::max(hb[4], hb[5]);
+                spilled_total += spilled;
+                if (spilled == 0) break;
+                if (rounds >= 32) {  // leave what is left to the per-edge kernel
+                    pzk::mark_task_pairs(tout, spilled, fallback, s);
+                    tail();
+                    nfb = readback<unsigned long long>(c, {fbc})[0];
+                    break;
+                }
+                std::swap(tin, tout);
+                cur = tin;
+                ntasks = spilled;
+            }
+            r->narrow_rounds = rounds;
+            r->tasks_spilled = spilled_total;
+            fb_pairs = fbp;
+        } else {
+            auto hh = readback<uint32_t>(c, {xa.height, xb.height});
+            height = std::max(hh[0], hh[1]);
+        }
+        r->index_height = height;
+        if (pzk::stack_for_height(height) == 0)
+            throw PzError(PZ_E_LIMIT, "per-object BVH height " + std::to_string(height) + " exceeds the traversal stack (63)");
+        r->narrow_fallback_pairs = d.narrow_mode != 1 ? nfb : 0;  // pairs re-run by the per-edge kernel
+        if (nfb > 0) {
+            // Per-edge narrow phase: every edge of the .pre payload regrouped by object.
+            auto edges_of = [&](const pz_dataset* D, const SetPrep& P, Needed& N, const std::string& tag) {
+                const size_t n = std::max(D->nobj, 1);
+                const uint64_t no = (uint64_t)std::max(D->nobj, 0);
+                uint32_t* stc = buf<uint32_t>(c, tag + ".seg_tri_cnt", n);
+                uint32_t* sec = buf<uint32_t>(c, tag + ".seg_edge_cnt", n);
+                pzk::edge_count(D->eo, D->ne, D->nobj, P.edge_cnt, s);
+                pzk::fill_u32(sec, 0u, n, s);
+                pzk::needed_layout(N.need, N.loc, D->nobj, P.tri_cnt, P.edge_cnt, N.obj_of_loc, stc, sec, s);
+                void* tmp = buf<char>(c, "scan.tmp", pzk::scan_temp_bytes(n + 1));
+                pzk::exclusive_scan_u32_u32(sec, N.eseg_off, no, tmp, s);
+                uint64_t ne = readback<uint32_t>(c, {N.eseg_off + no})[0];
+                pzd::EdgeRec* edges = buf<pzd::EdgeRec>(c, tag + ".edges", ne + 1);
+                uint32_t* cursor = buf<uint32_t>(c, tag + ".cursor", n);
+                pzk::fill_u32(cursor, 0u, n, s);
+                pzk::edge_scatter(D->es, D->ee, D->eo, D->ne, D->nobj, N.need, N.loc, N.eseg_off, cursor, edges, s);
+                r->index_edges += ne;
+                return edges;
+            };
+            va.edges = edges_of(A, pa, na, "A");
+            vb.edges = edges_of(B, pb, nb, "B");
+            uint32_t* tcnt = buf<uint32_t>(c, "np.tcnt", nfb + 1);
+            uint64_t* toff = buf<uint64_t>(c, "np.toff", nfb + 2);
+            void* tmp = buf<char>(c, "scan.tmp.np", pzk::scan_temp_bytes(nfb + 1));
+            uint8_t* fb_meta = meta;
+            if (fb_index) {  // meta of the compacted fallback pairs
+                fb_meta = buf<uint8_t>(c, "np.fb_meta", nfb + 1);
+                pzk::pair_meta(fb_pairs, nfb, batched ? A->part_of : nullptr, meta_of_part, pz::pair_meta_of(d), fb_meta,
+                               s);
+            }
+            pzk::task_counts(fb_pairs, nfb, na.loc, na.eseg_off, nb.loc, nb.eseg_off, fb_meta, tcnt, s);
+            pzk::exclusive_scan_u32_u64(tcnt, toff, nfb, tmp, s);
+            const uint64_t ntasks = readback<uint64_t>(c, {toff + nfb})[0];
+            uint32_t* sub_found = fb_index ? buf<uint32_t>(c, "np.sub_found", nfb + 1) : found;
+            if (fb_index) pzk::fill_u32(sub_found, 0u, nfb + 1, s);
+            pzk::narrow(fb_pairs, nfb, toff, ntasks, va, vb, sub_found, fb_meta, f32, allow_f32, height, stats, s);
+            if (fb_index) pzk::scatter_found(sub_found, fb_index, nfb, found, s);
+        }
+
+        if (inside) {
+            ph.begin("Containment Parity");
+            pzk::parity(pairs, np, found, va, vb, A->v, A->idx, pa.first_tri, B->v, B->idx, pb.first_tri, meta, inside,
+                        height, stats, s);
+        }
+    };
+
+    // Pairs the tree path decides: all of them, or the ones the pair sieve defers.
+    const uint64_t* tpairs = pairs;
+    uint64_t ntp = np;
+    uint8_t* tmeta = meta;
+    uint32_t* tindex = nullptr;  // sieve: position of each deferred pair in pairs[]
+    if (d.narrow_mode == 0) {
+        ph.begin("Pair Sieve");
+        uint32_t* defer = buf<uint32_t>(c, "sv.defer", np + 1);
+        pzk::fill_u32(defer, 0u, np + 1, s);
+        const uint64_t cap_esc = std::min<uint64_t>(std::max<uint64_t>(4 * np, 1 << 16), 1ull << 22);
+        uint4* esc = buf<uint4>(c, "sv.esc", cap_esc);
+        unsigned long long* sctr = buf<unsigned long long>(c, "sv.ctr", 2);  // open triangle pairs, deferred pairs
+        pzk::fill_ull(sctr, 0ull, 2, s);
+        auto sieve_set = [](const pz_dataset* D, const SetPrep& P) {
+            const uint64_t n = (uint64_t)std::max(D->nobj, 1);
+            return pzd::SieveSet{D->v, D->idx, P.first_tri, P.tri_cnt, P.last_tri, P.obox,
+                                 D->vruns ? D->topo + 4 * n : nullptr, D->vruns ? D->topo + 3 * n : nullptr};
+        };
+        const pzd::SieveSet sa = sieve_set(A, pa), sb = sieve_set(B, pb);
+        pzk::pair_sieve(pairs, np, sa, sb, meta, f32, allow_f32, found, defer, esc, sctr, cap_esc, stats, s);
+        if (f32 && allow_f32) pzk::escalate_tris(esc, sctr, cap_esc, sa, sb, meta, found, stats, s);
+        if (inside) {
+            ph.begin("Containment Parity");
+            pzk::sieve_parity(pairs, np, sa, sb, meta, found, defer, inside, stats, s);
+        }
+        uint64_t* dp = buf<uint64_t>(c, "sv.pairs", np + 1);
+        tindex = buf<uint32_t>(c, "sv.index", np + 1);
+        pzk::compact_flagged(pairs, np, defer, dp, tindex, sctr + 1, s);
+        ntp = readback<unsigned long long>(c, {sctr + 1})[0];
+        r->sieve_pairs = np - ntp;
+        tpairs = dp;
+        if (ntp > 0) {
+            tmeta = buf<uint8_t>(c, "sv.meta", ntp + 1);
+            pzk::pair_meta(dp, ntp, batched ? A->part_of : nullptr, meta_of_part, pz::pair_meta_of(d), tmeta, s);
+        }
+    }
+    if (ntp > 0) {
+        uint32_t* tfound = found;
+        uint32_t* tinside = inside;
+        if (tindex) {
+            tfound = buf<uint32_t>(c, "sv.tfound", ntp + 1);
+            pzk::fill_u32(tfound, 0u, ntp + 1, s);
+            if (inside) {
+                tinside = buf<uint32_t>(c, "sv.tinside", ntp + 1);
+                pzk::fill_u32(tinside, 0u, ntp + 1, s);
+            }
+        }
+        tree_path(tpairs, ntp, tmeta, tfound, tinside);
+        if (tindex) {
+            pzk::scatter_found(tfound, tindex, ntp, found, s);
+            if (inside) pzk::scatter_found(tinside, tindex, ntp, inside, s);
+        }
     }
 
     ph.begin("Download Results");
@@ -1621,8 +1681,13 @@ pz_status pz_autotune(pz_ctx* c, const pz_dataset* a, const pz_dataset* b, const
     const TuneTable saved = c->tune;
     const bool was_on = pzk::prof_enabled();
     pz_join_result r;
+    // The tuned parameters are the per-object trees' (builder workgroups, the
+    // dual-tree walk's queue and chunk): the pair sieve would leave most
+    // pairs out of them, so the tuning joins send every pair to the trees.
+    pz_join_desc td = *desc;
+    if (td.narrow_mode == 0) td.narrow_mode = 2;
     auto run = [&]() -> pz_status {
-        pz_status st = pz_join(c, a, b, desc, &r);
+        pz_status st = pz_join(c, a, b, &td, &r);
         if (st == PZ_OK) pz_join_result_release(&r);
         ++report->joins_run;
         return st;

@@ -23,6 +23,18 @@ struct DualTask {
     int32_t x0, x1, y0, y1;
     uint32_t pad[3];
 };
+// One set's geometry as the pair sieve reads it: the dataset's arrays and the
+// per-object triangle ranges and boxes of the join.
+struct SieveSet {
+    const float* v;
+    const uint32_t* idx;
+    const uint32_t* first_tri;
+    const uint32_t* tri_cnt;
+    const uint32_t* last_tri;
+    const uint32_t* obox;
+    const uint32_t* vmin;  // objects' vertex runs [vmin, vmax] (null: no runs)
+    const uint32_t* vmax;
+};
 }  // namespace pzd
 
 namespace pzk {
@@ -31,9 +43,10 @@ namespace pzk {
 // 2 decided exactly, 3 fp32 misclassified, 4 edges traced, 5 hash-set slots
 // probed, 6 probed slots holding another pair, 7 parity rays cast, 8 triangles
 // they crossed, 9 rays crossing at least one, 10 rays ending inside, 11 their
-// ray-triangle tests, 12 dual-tree walks, 13 candidate pairs proven touching.
+// ray-triangle tests, 12 dual-tree walks, 13 candidate pairs proven touching,
+// 14 candidate pairs the pair sieve decided.
 constexpr int kStatSlots = 16;
-constexpr int kStatParityRays = 7, kStatWalks = 12, kStatTouching = 13;
+constexpr int kStatParityRays = 7, kStatWalks = 12, kStatTouching = 13, kStatSieved = 14;
 
 using pzd::DualTask;
 using pzd::EdgeRec;
@@ -41,6 +54,7 @@ using pzd::LeafTri;
 using pzd::BNode;
 using pzd::Node;
 using pzd::SetView;
+using pzd::SieveSet;
 
 // Stack depth for a BVH of the given height (0: unsupported).
 int stack_for_height(uint32_t h);
@@ -214,5 +228,22 @@ bool trace_global(int mode, const float* es, const float* ee, const int32_t* eo,
 void compact_table(const unsigned long long* table, uint64_t size, uint64_t* out, uint64_t cap,
                    unsigned long long* count, hipStream_t s);
 void fill_ull(unsigned long long* p, unsigned long long v, uint64_t n, hipStream_t s);
+
+// --- pair sieve (sieve.hip)
+// Narrow phase (and parity rays) of candidate pairs whose objects hold at most
+// 8,192 contiguous triangles, one workgroup per pair, without per-object
+// trees: found[p] = 1 for a proven hit, defer[p] = 1 for a pair left to the
+// tree path (nothing else written for it).  With f32 (and allow_f32) open triangle pairs go to esc as (pair,
+// mesh1 triangle, mesh2 triangle) for escalate_tris.
+void pair_sieve(const uint64_t* pairs, uint64_t np, const SieveSet& A, const SieveSet& B, const uint8_t* meta,
+                bool f32, int allow_f32, uint32_t* found, uint32_t* defer, uint4* esc,
+                unsigned long long* esc_count, uint64_t esc_cap, unsigned long long* stats, hipStream_t s);
+// Parity rays (parity()'s semantics) of the pairs the sieve decided that do
+// not touch; run after escalate_tris.
+void sieve_parity(const uint64_t* pairs, uint64_t np, const SieveSet& A, const SieveSet& B, const uint8_t* meta,
+                  const uint32_t* found, const uint32_t* defer, uint32_t* inside, unsigned long long* stats,
+                  hipStream_t s);
+void escalate_tris(const uint4* list, const unsigned long long* count, uint64_t cap, const SieveSet& A,
+                   const SieveSet& B, const uint8_t* meta, uint32_t* found, unsigned long long* stats, hipStream_t s);
 
 }  // namespace pzk

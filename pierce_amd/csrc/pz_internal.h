@@ -80,7 +80,7 @@ struct pz_dataset {
     // per-object first and last triangle, triangle count and last vertex, and
     // the (object, first vertex) chunks the per-join bounds kernel reads.
     bool vruns = false;
-    uint32_t* topo = nullptr;  // first[n], last[n], cnt[n], vmax[n] (n = max(nobj, 1))
+    uint32_t* topo = nullptr;  // first[n], last[n], cnt[n], vmax[n], vmin[n] (n = max(nobj, 1))
     uint2* topo_chunks = nullptr;
     uint64_t topo_nchunks = 0;
 };

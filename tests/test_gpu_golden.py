@@ -21,7 +21,7 @@ def uploaded(ctx):
 @pytest.mark.parametrize("case", CASES)
 @pytest.mark.parametrize("pred", list(PREDICATES))
 @pytest.mark.parametrize("precision", [pz.PREC_FP32_FILTERED, pz.PREC_FP64])
-@pytest.mark.parametrize("narrow_mode", [0, 1])
+@pytest.mark.parametrize("narrow_mode", [0, 1, 2])
 def test_two_level_matches_golden(ctx, uploaded, case, pred, precision, narrow_mode):
     print(f"[golden] {case} {pred} precision={precision} narrow={narrow_mode}", flush=True)
     _, _, da, db = uploaded[case]
@@ -70,7 +70,7 @@ def test_golden_scaled_beyond_fp32_filter_range(ctx, case):
     assert max(np.abs(A.vertices).max(), np.abs(B.vertices).max()) > 2.0 ** 39
     da, db = ctx.upload(A), ctx.upload(B)
     for pred in PREDICATES:
-        for kw in (dict(), dict(narrow_mode=1), dict(precision=pz.PREC_FP64), dict(broad_mode=1)):
+        for kw in (dict(), dict(narrow_mode=1), dict(narrow_mode=2), dict(precision=pz.PREC_FP64), dict(broad_mode=1)):
             print(f"[golden x2^40] {case} {pred} {kw}", flush=True)
             out = pz.join(ctx, da, db, predicate=PREDICATES[pred], **kw)
             np.testing.assert_array_equal(out.pairs, expected(case, pred), err_msg=f"{case} {pred} {kw}")

@@ -6,8 +6,9 @@ has moved the context's buffers (the graphs must be dropped and re-captured).
 The segments are the object statistics of both sets, the layout of the
 objects that occur in candidate pairs (about 40 small launches whose shapes
 depend only on the object counts) and, when the broad phase walks one (WALK),
-the mesh2 object index: three graph launches per join.  With the box scan
-there is no object index and two.
+the mesh2 object index: three graph launches per join through the
+per-object trees (narrow_mode 2).  With the box scan there is no object index
+and two; the pair sieve (narrow_mode 0) needs no layout, so one segment fewer.
 """
 import numpy as np
 import pytest
@@ -18,7 +19,7 @@ from oracle import oracle as orc
 pytestmark = pytest.mark.gpu
 
 
-WALK = dict(broad_mode=1)
+WALK = dict(broad_mode=1, narrow_mode=2)
 
 
 def _sets(count, seed, box):
@@ -65,8 +66,19 @@ def test_buffer_growth_drops_graphs(ctx):
         np.testing.assert_array_equal(out.pairs, want)
     assert out.graph_launches == 3
     for _ in range(3):  # box scan: statistics and layout segments only
-        out = pz.join(ctx, da, db, broad_mode=2)
+        out = pz.join(ctx, da, db, broad_mode=2, narrow_mode=2)
         np.testing.assert_array_equal(out.pairs, want)
+    assert out.graph_launches == 2
+
+
+def test_sieve_replays_statistics_and_object_index(ctx):
+    a, b = _sets(600, 31, 80.0)
+    da, db = ctx.upload(a), ctx.upload(b)
+    want = orc.face_overlap(a, b)
+    for _ in range(4):
+        out = pz.join(ctx, da, db, broad_mode=1)
+        np.testing.assert_array_equal(out.pairs, want)
+        assert out.sieve_pairs == out.num_candidates
     assert out.graph_launches == 2
 
 

@@ -29,7 +29,7 @@ def c1(ctx):
     return a, b, ctx.upload(a), ctx.upload(b), orc.face_overlap(a, b)
 
 
-@pytest.mark.parametrize("narrow_mode", [0, 1])
+@pytest.mark.parametrize("narrow_mode", [0, 1, 2])
 @pytest.mark.parametrize("precision", [pz.PREC_FP32_FILTERED, pz.PREC_FP64])
 @pytest.mark.parametrize("sizing", [pz.SIZING_TWO_PASS, pz.SIZING_FIXED])
 def test_two_level_overlap_c1(ctx, c1, precision, sizing, narrow_mode):
@@ -56,7 +56,7 @@ def test_hash_overflow_falls_back(ctx, c1):
     np.testing.assert_array_equal(out.pairs, expected)
 
 
-@pytest.mark.parametrize("narrow_mode", [0, 1])
+@pytest.mark.parametrize("narrow_mode", [0, 1, 2])
 @pytest.mark.parametrize("direction", [pz.DIR_1TO2, pz.DIR_2TO1])
 def test_direction_split_c1(ctx, c1, direction, narrow_mode):
     a, b, da, db, expected = c1
@@ -64,7 +64,7 @@ def test_direction_split_c1(ctx, c1, direction, narrow_mode):
     np.testing.assert_array_equal(out.pairs, orc.edge_overlap(a, b, direction))
 
 
-@pytest.mark.parametrize("narrow_mode", [0, 1])
+@pytest.mark.parametrize("narrow_mode", [0, 1, 2])
 @pytest.mark.parametrize("direction", [pz.DIR_1TO2, pz.DIR_2TO1])
 def test_intersection_direction(ctx, direction, narrow_mode):
     """--query-direction of pierce_intersection (pierce_intersection_main.cpp:103-155):
@@ -104,7 +104,7 @@ def test_candidate_count_c1(ctx, c1):
     assert pz.count_candidate_tests(ctx, da, db) == orc.count_candidate_tests(a, b)
 
 
-@pytest.mark.parametrize("narrow_mode", [0, 1])
+@pytest.mark.parametrize("narrow_mode", [0, 1, 2])
 @pytest.mark.parametrize("predicate", [pz.PRED_INTERSECTION, pz.PRED_CONTAINMENT])
 def test_predicates_vs_oracle(ctx, predicate, narrow_mode):
     a, b = containment_sets()
@@ -147,6 +147,7 @@ def test_global_predicates_vs_oracle(ctx, predicate, direction, dedup, sizing):
 
 @pytest.mark.parametrize("precision", [pz.PREC_FP32_FILTERED, pz.PREC_FP64])
 @pytest.mark.parametrize("strategy,narrow_mode", [(pz.STRATEGY_TWO_LEVEL, 0), (pz.STRATEGY_TWO_LEVEL, 1),
+                                                  (pz.STRATEGY_TWO_LEVEL, 2),
                                                   (pz.STRATEGY_GLOBAL, 0)])
 def test_touching_cubes_exact(ctx, precision, strategy, narrow_mode):
     """Exact face/edge/vertex contact and coplanar faces: the escalation paths decide these."""
@@ -235,7 +236,7 @@ def test_ragged_object_ids_and_single_triangles(ctx, predicate):
     exp = orc.face_overlap(a, b) if predicate == pz.PRED_OVERLAP else orc.predicate_join(a, b, predicate)[0]
     assert len(exp) > 0
     da, db = ctx.upload(a), ctx.upload(b)
-    for kw in (dict(), dict(broad_mode=1), dict(narrow_mode=1), dict(precision=pz.PREC_FP64)):
+    for kw in (dict(), dict(broad_mode=1), dict(narrow_mode=1), dict(narrow_mode=2), dict(precision=pz.PREC_FP64)):
         print(f"[ragged] predicate={predicate} {kw}", flush=True)
         np.testing.assert_array_equal(pz.join(ctx, da, db, predicate=predicate, **kw).pairs, exp)
     if predicate == pz.PRED_OVERLAP:

@@ -100,7 +100,7 @@ def test_tubes_work_queue_rounds(ctx, tubes, queue, chunk):
     try:
         pz.tune_set(ctx, saved["build_threads"], narrow_queue=(queue, queue), narrow_chunk=(chunk, chunk))
         for pred in PREDS:
-            out = pz.join(ctx, da, db, predicate=pred)
+            out = pz.join(ctx, da, db, predicate=pred, narrow_mode=2)
             np.testing.assert_array_equal(out.pairs, exp[pred], err_msg=f"predicate {pred}")
             print(f"[tubes] queue={queue} chunk={chunk} predicate={pred}: {out.narrow_rounds} rounds, "
                   f"{out.tasks_spilled} tasks spilled", flush=True)
@@ -160,7 +160,7 @@ def test_escalations_counted_once_over_rounds(ctx):
     try:
         for queue in (1536, 256):
             pz.tune_set(ctx, saved["build_threads"], narrow_queue=(queue, queue), narrow_chunk=(1, 1))
-            out = pz.join(ctx, da, db)
+            out = pz.join(ctx, da, db, narrow_mode=2)
             assert len(out.pairs) == 0
             got[queue] = (out.narrow_rounds, out.tests_fp64 + out.tests_exact, out.narrow_fallback_pairs)
             print(f"[escalation] queue {queue}: rounds {out.narrow_rounds}, escalated {got[queue][1]}, "
