@@ -54,6 +54,8 @@ def alg_bytes(kernel, k):
         return 12 * k["V"]
     if kernel in ("build_small", "build_chunks", "tri_keys", "narrow_dual", "gather_leaves", "refit"):
         return 16 * k["IT"] + 12 * k["IV"]
+    if kernel == "pair_sieve":      # the triangles and vertices of each sieved pair's two objects
+        return 16 * k["ST"] + 12 * k["SV"]
     if kernel == "narrow":          # the indexed edges
         return 28 * k["IE"]
     if kernel == "edge_count":
@@ -71,6 +73,8 @@ def design_bytes(kernel, k):
         return 16 * k["T"] + 12 * k["V"]
     if kernel == "object_bounds_runs":
         return 12 * k["V"]
+    if kernel == "pair_sieve":      # index triples and vertex runs read once per pair, nothing written per triangle
+        return 12 * k["ST"] + 12 * k["SV"]
     return None
 
 
@@ -599,7 +603,8 @@ def main():
 
         # Roofline of the dominant profiled kernel (largest device time in the timed steps).
         iv = int(round(out.index_triangles * counts["V"] / counts["T"]))  # indexed objects' vertices
-        counts.update(IT=out.index_triangles, IV=iv, IE=out.index_edges)
+        sv = int(round(out.sieve_triangles * counts["V"] / counts["T"]))  # sieved pairs' vertices
+        counts.update(IT=out.index_triangles, IV=iv, IE=out.index_edges, ST=out.sieve_triangles, SV=sv)
         dom = max(kernels, key=lambda k: kernels[k][0]) if kernels else None
         roofline = None
         if dom:

@@ -305,6 +305,7 @@ typedef struct {
     uint64_t parity_tests;      /* ray-triangle tests of those rays (part of tests_total) */
     uint64_t sieve_pairs;       /* two-level, narrow_mode 0: candidate pairs the pair sieve decided
                                    (the others went to the per-object trees) */
+    uint64_t sieve_triangles;   /* triangles of those pairs' objects, per pair (what the sieve read) */
     uint32_t index_height;      /* tallest triangle BVH the join built (at most 63, else PZ_E_LIMIT) */
     uint32_t reserved_u32;
     double   ms_total;          /* device-resident geometry -> sorted host pair set */

@@ -120,6 +120,7 @@ class JoinResult(C.Structure):
                 ("narrow_fallback_pairs", C.c_uint64), ("walks_started", C.c_uint64), ("pairs_touching", C.c_uint64),
                 ("parity_rays", C.c_uint64), ("parity_crossings", C.c_uint64), ("parity_rays_crossing", C.c_uint64),
                 ("parity_inside", C.c_uint64), ("parity_tests", C.c_uint64), ("sieve_pairs", C.c_uint64),
+                ("sieve_triangles", C.c_uint64),
                 ("index_height", C.c_uint32),
                 ("reserved_u32", C.c_uint32), ("ms_total", C.c_double),
                 ("phases", Phase * MAX_PHASES)]
@@ -657,6 +658,7 @@ class JoinOutput:
     parity_tests: int = 0
     index_height: int = 0
     sieve_pairs: int = 0
+    sieve_triangles: int = 0
 
 
 def join_desc(**kw) -> JoinDesc:
@@ -691,7 +693,8 @@ def _output(r: JoinResult) -> JoinOutput:
                       int(r.hash_accesses), int(r.hash_contentions), int(r.narrow_fallback_pairs),
                       int(r.walks_started), int(r.pairs_touching), int(r.parity_rays), int(r.parity_crossings),
                       int(r.parity_rays_crossing), int(r.parity_inside), int(r.stack_overflows),
-                      int(r.parity_tests), int(r.index_height), int(r.sieve_pairs))
+                      int(r.parity_tests), int(r.index_height), int(r.sieve_pairs),
+                      int(r.sieve_triangles))
 
 
 def join_batch(ctx: Context, a: Dataset, b: Dataset, queries, **common):

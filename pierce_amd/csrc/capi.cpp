@@ -239,6 +239,7 @@ void set_counters(pz_join_result* r, const std::vector<unsigned long long>& st) 
     r->parity_tests = st[pzk::kStatParityRays + 4];
     r->walks_started = st[pzk::kStatWalks] + st[4];
     r->pairs_touching = st[pzk::kStatTouching];
+    r->sieve_triangles = st[pzk::kStatSieveTris];
 }
 
 // Final stage shared by both strategies: sort unique keys (+types), copy to host.

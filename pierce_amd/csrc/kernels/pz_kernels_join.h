@@ -44,9 +44,10 @@ namespace pzk {
 // probed, 6 probed slots holding another pair, 7 parity rays cast, 8 triangles
 // they crossed, 9 rays crossing at least one, 10 rays ending inside, 11 their
 // ray-triangle tests, 12 dual-tree walks, 13 candidate pairs proven touching,
-// 14 candidate pairs the pair sieve decided.
+// 14 candidate pairs the pair sieve decided, 15 triangles those pairs' objects
+// hold (what the sieve read for them).
 constexpr int kStatSlots = 16;
-constexpr int kStatParityRays = 7, kStatWalks = 12, kStatTouching = 13, kStatSieved = 14;
+constexpr int kStatParityRays = 7, kStatWalks = 12, kStatTouching = 13, kStatSieved = 14, kStatSieveTris = 15;
 
 using pzd::DualTask;
 using pzd::EdgeRec;

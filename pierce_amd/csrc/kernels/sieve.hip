@@ -177,6 +177,7 @@ __global__ __launch_bounds__(kSieveThreads) void k_pair_sieve(
     const int tid = threadIdx.x;
     LaneStats st{0, 0, 0, 0, 0};
     uint32_t n_sieved = 0;  // thread 0: pairs decided here
+    unsigned long long n_tris = 0;  // and their objects' triangles
     for (uint64_t p = blockIdx.x; p < np; p += gridDim.x) {
         if (second && defer[p] != kDeferGather) continue;  // workgroup-uniform
         const uint64_t key = pairs[p];
@@ -386,6 +387,7 @@ __global__ __launch_bounds__(kSieveThreads) void k_pair_sieve(
                 defer[p] = kDeferTrees;
             } else {
                 ++n_sieved;
+                n_tris += ca + cb;
                 if (s_hit) found[p] = 1u;
                 if (second) defer[p] = 0u;
             }
@@ -393,7 +395,10 @@ __global__ __launch_bounds__(kSieveThreads) void k_pair_sieve(
         __syncthreads();
     }
     flush_stats(st, stats);
-    if (tid == 0 && n_sieved) atomicAdd(&stats[pzk::kStatSieved], (unsigned long long)n_sieved);
+    if (tid == 0 && n_sieved) {
+        atomicAdd(&stats[pzk::kStatSieved], (unsigned long long)n_sieved);
+        atomicAdd(&stats[pzk::kStatSieveTris], n_tris);
+    }
 }
 
 // ------------------------------------------------- sieve over vertex runs
@@ -419,7 +424,8 @@ __global__ __launch_bounds__(kRunThreads, 4) void k_pair_sieve_runs(
     constexpr int T = kRunThreads;
     __shared__ float vx[2][kRunV], vy[2][kRunV], vz[2][kRunV];
     __shared__ uint2 keep[2][kRunKeep];  // (v0 | v1 << 16, v2 | k << 16)
-    __shared__ float gbox[6][kRunG];
+    __shared__ float4 gb0[kRunG];  // grid-side boxes: lo.xyz, hi.x
+    __shared__ float2 gb1[kRunG];  // hi.yz
     __shared__ uint16_t refs[kRunRefs];
     __shared__ uint16_t cand[kRunKeep];  // other-side triangles with references in their cells
     __shared__ uint32_t wpairs[T / PZ_WAVE][2 * PZ_WAVE];  // per wave: box-meeting pairs to test
@@ -432,6 +438,7 @@ __global__ __launch_bounds__(kRunThreads, 4) void k_pair_sieve_runs(
     const int tid = threadIdx.x, lane = __lane_id();
     LaneStats st{0, 0, 0, 0, 0};
     uint32_t n_sieved = 0;
+    unsigned long long n_tris = 0;
     auto corner = [&](int side, uint32_t v) { return make_float3(vx[side][v], vy[side][v], vz[side][v]); };
     auto kept_tri = [&](int side, uint2 r) {
         return Tri{corner(side, r.x & 0xFFFFu), corner(side, r.x >> 16), corner(side, r.y & 0xFFFFu)};
@@ -531,11 +538,9 @@ __global__ __launch_bounds__(kRunThreads, 4) void k_pair_sieve_runs(
                 float lo[3], hi[3];
                 tri_box(kept_tri(gs, keep[gs][i]), lo, hi);
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    gbox[k][i] = lo[k];
-                    gbox[3 + k][i] = hi[k];
-                    ex[k] += hi[k] - lo[k];
-                }
+                for (int k = 0; k < 3; ++k) ex[k] += hi[k] - lo[k];
+                gb0[i] = make_float4(lo[0], lo[1], lo[2], hi[0]);
+                gb1[i] = make_float2(hi[1], hi[2]);
             }
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
@@ -551,11 +556,14 @@ __global__ __launch_bounds__(kRunThreads, 4) void k_pair_sieve_runs(
             for (int c = tid; c < ncell; c += T) ccnt[c] = 0u;
             __syncthreads();
             auto cells_of_g = [&](int i, int* cl, int* ch) {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    cl[k] = g.cell(gbox[k][i], k);
-                    ch[k] = g.cell(gbox[3 + k][i], k);
-                }
+                const float4 b0 = gb0[i];
+                const float2 b1 = gb1[i];
+                cl[0] = g.cell(b0.x, 0);
+                cl[1] = g.cell(b0.y, 1);
+                cl[2] = g.cell(b0.z, 2);
+                ch[0] = g.cell(b0.w, 0);
+                ch[1] = g.cell(b1.x, 1);
+                ch[2] = g.cell(b1.y, 2);
             };
             int nref = 0;
             for (int i = tid; i < nG; i += T) {
@@ -707,8 +715,9 @@ __global__ __launch_bounds__(kRunThreads, 4) void k_pair_sieve_runs(
                 r += nr;
 #pragma unroll
                 for (int u = 0; u < kWalkILP; ++u) {
-                    const float glo[3] = {gbox[0][gi[u]], gbox[1][gi[u]], gbox[2][gi[u]]};
-                    const float ghi[3] = {gbox[3][gi[u]], gbox[4][gi[u]], gbox[5][gi[u]]};
+                    const float4 b0 = gb0[gi[u]];
+                    const float2 b1 = gb1[gi[u]];
+                    const float glo[3] = {b0.x, b0.y, b0.z}, ghi[3] = {b0.w, b1.x, b1.y};
                     // once per pair: in the cell of the larger lower corner
                     match[u] = u < nr && box_box(lo, hi, glo, ghi) && max(g.cell(glo[0], 0), cl[0]) == cx &&
                                max(g.cell(glo[1], 1), cl[1]) == cy && max(g.cell(glo[2], 2), cl[2]) == cz;
@@ -740,13 +749,17 @@ __global__ __launch_bounds__(kRunThreads, 4) void k_pair_sieve_runs(
                 defer[p] = kDeferTrees;
             } else {
                 ++n_sieved;
+                n_tris += ca + cb;
                 if (s_hit) found[p] = 1u;
             }
         }
         __syncthreads();
     }
     flush_stats(st, stats);
-    if (tid == 0 && n_sieved) atomicAdd(&stats[pzk::kStatSieved], (unsigned long long)n_sieved);
+    if (tid == 0 && n_sieved) {
+        atomicAdd(&stats[pzk::kStatSieved], (unsigned long long)n_sieved);
+        atomicAdd(&stats[pzk::kStatSieveTris], n_tris);
+    }
 }
 
 // Parity rays of the sieved pairs, wave per (pair, direction), as the tree
