@@ -828,7 +828,13 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
                                  D->vruns ? D->topo + 4 * n : nullptr, D->vruns ? D->topo + 3 * n : nullptr};
         };
         const pzd::SieveSet sa = sieve_set(A, pa), sb = sieve_set(B, pb);
-        pzk::pair_sieve(pairs, np, sa, sb, meta, f32, allow_f32, found, defer, esc, sctr, cap_esc, stats, s);
+        uint32_t* work = buf<uint32_t>(c, "sv.work", np + 1);
+        uint32_t* gwork = buf<uint32_t>(c, "sv.gwork", np + 1);
+        uint32_t* wctr = buf<uint32_t>(c, "sv.wctr", 4);
+        int cus = 0;
+        HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+        pzk::pair_sieve(pairs, np, sa, sb, meta, f32, allow_f32, found, defer, work, gwork, wctr, esc, sctr, cap_esc,
+                        (unsigned)std::max(cus, 1), stats, s);
         if (f32 && allow_f32) pzk::escalate_tris(esc, sctr, cap_esc, sa, sb, meta, found, stats, s);
         if (inside) {
             ph.begin("Containment Parity");

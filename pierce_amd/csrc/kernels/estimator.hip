@@ -92,19 +92,6 @@ __global__ void k_feat_init(PartAcc* acc, uint32_t n) {
     acc[i] = a;
 }
 
-__device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
-    for (int s = 32; s > 0; s >>= 1) v += __shfl_xor(v, s, 64);
-    return v;
-}
-__device__ inline uint32_t wave_min_u32(uint32_t v) {
-    for (int s = 32; s > 0; s >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, s, 64));
-    return v;
-}
-__device__ inline uint32_t wave_max_u32(uint32_t v) {
-    for (int s = 32; s > 0; s >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, s, 64));
-    return v;
-}
-
 // One object per lane.  The objects of one part are contiguous ids, so the
 // lanes of a wave holding one part form one run: a segmented wave scan
 // (combining only lanes of the same part) leaves each run's totals in its

@@ -236,9 +236,11 @@ void fill_ull(unsigned long long* p, unsigned long long v, uint64_t n, hipStream
 // trees: found[p] = 1 for a proven hit, defer[p] = 1 for a pair left to the
 // tree path (nothing else written for it).  With f32 (and allow_f32) open triangle pairs go to esc as (pair,
 // mesh1 triangle, mesh2 triangle) for escalate_tris.
+// work and gwork hold np pair indices each, ctr 4 words; blocks = compute units.
 void pair_sieve(const uint64_t* pairs, uint64_t np, const SieveSet& A, const SieveSet& B, const uint8_t* meta,
-                bool f32, int allow_f32, uint32_t* found, uint32_t* defer, uint4* esc,
-                unsigned long long* esc_count, uint64_t esc_cap, unsigned long long* stats, hipStream_t s);
+                bool f32, int allow_f32, uint32_t* found, uint32_t* defer, uint32_t* work, uint32_t* gwork,
+                uint32_t* ctr, uint4* esc, unsigned long long* esc_count, uint64_t esc_cap, unsigned blocks,
+                unsigned long long* stats, hipStream_t s);
 // Parity rays (parity()'s semantics) of the pairs the sieve decided that do
 // not touch; run after escalate_tris.
 void sieve_parity(const uint64_t* pairs, uint64_t np, const SieveSet& A, const SieveSet& B, const uint8_t* meta,

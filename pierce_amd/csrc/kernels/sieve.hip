@@ -45,8 +45,8 @@ constexpr int kSieveCells = 512;
 constexpr float kCellScale = 1.0f;
 constexpr int kU = 2;                     // triangles per object a thread loads at a time
 constexpr int kUS = 1;                    // the same in the grid lookups
-// defer[] codes: the gather sieve can take the pair, or only the trees can
-constexpr uint32_t kDeferGather = 1, kDeferTrees = 2;
+// defer[] value of a pair left to the trees
+constexpr uint32_t kDeferTrees = 1;
 
 struct Tri {
     float3 a, b, c;
@@ -160,8 +160,9 @@ __device__ void block_scan(const uint32_t* cnt, uint16_t* start, int n, uint32_t
 
 template <bool F32>
 __global__ __launch_bounds__(kSieveThreads) void k_pair_sieve(
-    const uint64_t* __restrict__ pairs, uint64_t np, SieveSet A, SieveSet B, const uint8_t* __restrict__ meta,
-    int allow_f32, int second, uint32_t* __restrict__ found, uint32_t* __restrict__ defer,
+    const uint64_t* __restrict__ pairs, const uint32_t* __restrict__ work, uint32_t* __restrict__ ctr,
+    SieveSet A, SieveSet B, const uint8_t* __restrict__ meta, int allow_f32, uint32_t* __restrict__ found,
+    uint32_t* __restrict__ defer,
     uint4* __restrict__ esc, unsigned long long* __restrict__ esc_count, uint64_t esc_cap,
     unsigned long long* __restrict__ stats) {
     constexpr int T = kSieveThreads;
@@ -178,8 +179,13 @@ __global__ __launch_bounds__(kSieveThreads) void k_pair_sieve(
     LaneStats st{0, 0, 0, 0, 0};
     uint32_t n_sieved = 0;  // thread 0: pairs decided here
     unsigned long long n_tris = 0;  // and their objects' triangles
-    for (uint64_t p = blockIdx.x; p < np; p += gridDim.x) {
-        if (second && defer[p] != kDeferGather) continue;  // workgroup-uniform
+    __shared__ uint32_t s_item;
+    while (true) {  // pairs of the work list, one at a time
+        __syncthreads();
+        if (tid == 0) s_item = atomicAdd(&ctr[3], 1u);
+        __syncthreads();
+        if (s_item >= ctr[1]) break;  // workgroup-uniform
+        const uint32_t p = work[s_item];
         const uint64_t key = pairs[p];
         const int32_t oa = (int32_t)(key >> 32), ob = (int32_t)(uint32_t)key;
         const uint32_t fa = A.first_tri[oa], ca = A.tri_cnt[oa], fb = B.first_tri[ob], cb = B.tri_cnt[ob];
@@ -389,7 +395,6 @@ __global__ __launch_bounds__(kSieveThreads) void k_pair_sieve(
                 ++n_sieved;
                 n_tris += ca + cb;
                 if (s_hit) found[p] = 1u;
-                if (second) defer[p] = 0u;
             }
         }
         __syncthreads();
@@ -418,8 +423,9 @@ constexpr int kRunThreads = 512;
 
 template <bool F32>
 __global__ __launch_bounds__(kRunThreads, 4) void k_pair_sieve_runs(
-    const uint64_t* __restrict__ pairs, uint64_t np, SieveSet A, SieveSet B, const uint8_t* __restrict__ meta,
-    uint32_t* __restrict__ found, uint32_t* __restrict__ defer, uint4* __restrict__ esc,
+    const uint64_t* __restrict__ pairs, const uint32_t* __restrict__ work, uint32_t* __restrict__ ctr,
+    SieveSet A, SieveSet B, const uint8_t* __restrict__ meta, uint32_t* __restrict__ found,
+    uint32_t* __restrict__ defer, uint32_t* __restrict__ gwork, uint4* __restrict__ esc,
     unsigned long long* __restrict__ esc_count, uint64_t esc_cap, unsigned long long* __restrict__ stats) {
     constexpr int T = kRunThreads;
     __shared__ float vx[2][kRunV], vy[2][kRunV], vz[2][kRunV];
@@ -443,18 +449,20 @@ __global__ __launch_bounds__(kRunThreads, 4) void k_pair_sieve_runs(
     auto kept_tri = [&](int side, uint2 r) {
         return Tri{corner(side, r.x & 0xFFFFu), corner(side, r.x >> 16), corner(side, r.y & 0xFFFFu)};
     };
-    for (uint64_t p = blockIdx.x; p < np; p += gridDim.x) {
+    __shared__ uint32_t s_item;
+    // a pair handed on to the gather sieve
+    auto to_gather = [&](uint32_t pp) { gwork[atomicAdd(&ctr[1], 1u)] = pp; };
+    while (true) {  // pairs of the work list (classified by k_sieve_classify), one at a time
+        __syncthreads();
+        if (tid == 0) s_item = atomicAdd(&ctr[2], 1u);
+        __syncthreads();
+        if (s_item >= ctr[0]) break;  // workgroup-uniform
+        const uint32_t p = work[s_item];
         const uint64_t key = pairs[p];
         const int32_t oa = (int32_t)(key >> 32), ob = (int32_t)(uint32_t)key;
         const uint32_t fa = A.first_tri[oa], ca = A.tri_cnt[oa], fb = B.first_tri[ob], cb = B.tri_cnt[ob];
-        const bool tris_fit = ca > 0 && cb > 0 && ca <= kSieveMaxTris && cb <= kSieveMaxTris &&
-                              A.last_tri[oa] - fa + 1 == ca && B.last_tri[ob] - fb + 1 == cb;
         const uint32_t va = A.vmin[oa], vb = B.vmin[ob];
         const uint32_t na = A.vmax[oa] - va + 1, nb = B.vmax[ob] - vb + 1;
-        if (!tris_fit || na > (uint32_t)kRunV || nb > (uint32_t)kRunV) {  // workgroup-uniform
-            if (tid == 0) defer[p] = tris_fit ? kDeferGather : kDeferTrees;
-            continue;
-        }
         const int dm = meta[p] >> 2;
         float olo[3], ohi[3];
 #pragma unroll
@@ -519,13 +527,11 @@ __global__ __launch_bounds__(kRunThreads, 4) void k_pair_sieve_runs(
         __syncthreads();
         const int nA = s_n[0], nB = s_n[1];
         if (nA > kRunKeep || nB > kRunKeep) {  // workgroup-uniform
-            if (tid == 0) defer[p] = kDeferGather;
-            __syncthreads();
+            if (tid == 0) to_gather(p);
             continue;
         }
         if (nA > 0 && nB > 0 && min(nA, nB) > kRunG) {  // workgroup-uniform
-            if (tid == 0) defer[p] = kDeferGather;
-            __syncthreads();
+            if (tid == 0) to_gather(p);
             continue;
         }
         if (nA > 0 && nB > 0) {
@@ -857,35 +863,76 @@ __global__ __launch_bounds__(256) void k_escalate_tris(const uint4* __restrict__
     flush_stats(st, stats);
 }
 
+// Work lists of the sieve, thread per candidate pair: pairs of objects with
+// contiguous triangles (at most kSieveMaxTris each) go to the vertex-run
+// sieve when both objects own runs of at most kRunV vertices, else to the
+// gather sieve; the others are deferred to the trees.  ctr = (vertex-run
+// pairs, gather pairs, and each list's next-item counter).
+__global__ __launch_bounds__(256) void k_sieve_classify(const uint64_t* __restrict__ pairs, uint64_t np, SieveSet A,
+                                                       SieveSet B, int runs, uint32_t* __restrict__ work,
+                                                       uint32_t* __restrict__ gwork, uint32_t* __restrict__ ctr,
+                                                       uint32_t* __restrict__ defer) {
+    const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    int cls = -1;  // 0 vertex-run sieve, 1 gather sieve, 2 trees
+    if (p < np) {
+        const uint64_t key = pairs[p];
+        const int32_t oa = (int32_t)(key >> 32), ob = (int32_t)(uint32_t)key;
+        const uint32_t fa = A.first_tri[oa], ca = A.tri_cnt[oa], fb = B.first_tri[ob], cb = B.tri_cnt[ob];
+        const bool tris_fit = ca > 0 && cb > 0 && ca <= kSieveMaxTris && cb <= kSieveMaxTris &&
+                              A.last_tri[oa] - fa + 1 == ca && B.last_tri[ob] - fb + 1 == cb;
+        cls = tris_fit ? 1 : 2;
+        if (tris_fit && runs && A.vmax[oa] - A.vmin[oa] < (uint32_t)kRunV && B.vmax[ob] - B.vmin[ob] < (uint32_t)kRunV)
+            cls = 0;
+        if (cls == 2) defer[p] = kDeferTrees;
+    }
+    for (int c = 0; c < 2; ++c) {
+        const uint64_t bal = __ballot(cls == c);
+        if (!bal) continue;
+        const int leader = __ffsll((long long)bal) - 1;
+        uint32_t base = 0;
+        if (__lane_id() == leader) base = atomicAdd(&ctr[c], (uint32_t)__popcll(bal));
+        base = (uint32_t)__shfl((int)base, leader, PZ_WAVE);
+        if (cls == c) (c == 0 ? work : gwork)[base + lanes_below(bal)] = (uint32_t)p;
+    }
+}
+
+__global__ void k_fill_ctr(uint32_t* ctr) {
+    if (threadIdx.x < 4) ctr[threadIdx.x] = 0u;
+}
+
 }  // namespace
 
 namespace pzk {
 
 void pair_sieve(const uint64_t* pairs, uint64_t np, const SieveSet& A, const SieveSet& B, const uint8_t* meta,
-                bool f32, int allow_f32, uint32_t* found, uint32_t* defer, uint4* esc,
-                unsigned long long* esc_count, uint64_t esc_cap, unsigned long long* stats, hipStream_t s) {
+                bool f32, int allow_f32, uint32_t* found, uint32_t* defer, uint32_t* work, uint32_t* gwork,
+                uint32_t* ctr, uint4* esc, unsigned long long* esc_count, uint64_t esc_cap, unsigned blocks,
+                unsigned long long* stats, hipStream_t s) {
     if (np == 0) return;
-    const unsigned g = (unsigned)std::min<uint64_t>(np, 1u << 20);
     const bool fp32 = f32 && allow_f32;
-    KernelTimer kt_("pair_sieve", s);
-    // pairs of objects with vertex runs first; what they defer to the gather
-    // sieve runs next (every pair, when a set has no runs)
     const bool runs = A.vmin && B.vmin;
+    KernelTimer kt_("pair_sieve", s);
+    hipLaunchKernelGGL(k_fill_ctr, dim3(1), dim3(64), 0, s, ctr);
+    hipLaunchKernelGGL(k_sieve_classify, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, pairs, np, A, B,
+                       runs ? 1 : 0, work, gwork, ctr, defer);
+    // persistent grids over the two work lists: pairs of objects with vertex
+    // runs first; what they hand on, and pairs without runs, next
+    const unsigned gr = (unsigned)std::min<uint64_t>(np, 2ull * blocks);
     if (runs) {
         if (fp32)
-            hipLaunchKernelGGL(k_pair_sieve_runs<true>, dim3(g), dim3(kRunThreads), 0, s, pairs, np, A, B, meta,
-                               found, defer, esc, esc_count, esc_cap, stats);
+            hipLaunchKernelGGL(k_pair_sieve_runs<true>, dim3(gr), dim3(kRunThreads), 0, s, pairs, work, ctr, A, B,
+                               meta, found, defer, gwork, esc, esc_count, esc_cap, stats);
         else
-            hipLaunchKernelGGL(k_pair_sieve_runs<false>, dim3(g), dim3(kRunThreads), 0, s, pairs, np, A, B, meta,
-                               found, defer, esc, esc_count, esc_cap, stats);
+            hipLaunchKernelGGL(k_pair_sieve_runs<false>, dim3(gr), dim3(kRunThreads), 0, s, pairs, work, ctr, A, B,
+                               meta, found, defer, gwork, esc, esc_count, esc_cap, stats);
     }
-    const int second = runs ? 1 : 0;
+    const unsigned gg = (unsigned)std::min<uint64_t>(np, 4ull * blocks);
     if (fp32)
-        hipLaunchKernelGGL(k_pair_sieve<true>, dim3(g), dim3(kSieveThreads), 0, s, pairs, np, A, B, meta, allow_f32,
-                           second, found, defer, esc, esc_count, esc_cap, stats);
+        hipLaunchKernelGGL(k_pair_sieve<true>, dim3(gg), dim3(kSieveThreads), 0, s, pairs, gwork, ctr, A, B, meta,
+                           allow_f32, found, defer, esc, esc_count, esc_cap, stats);
     else
-        hipLaunchKernelGGL(k_pair_sieve<false>, dim3(g), dim3(kSieveThreads), 0, s, pairs, np, A, B, meta, allow_f32,
-                           second, found, defer, esc, esc_count, esc_cap, stats);
+        hipLaunchKernelGGL(k_pair_sieve<false>, dim3(gg), dim3(kSieveThreads), 0, s, pairs, gwork, ctr, A, B, meta,
+                           allow_f32, found, defer, esc, esc_count, esc_cap, stats);
 }
 
 void sieve_parity(const uint64_t* pairs, uint64_t np, const SieveSet& A, const SieveSet& B, const uint8_t* meta,
