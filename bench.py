@@ -459,8 +459,8 @@ def main():
                     help="only copy 1 GiB device to device three times (scale of the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: the join runs on a single device (multi-GPU sharding is described "
-              "in DESIGN.md, not run); use --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the join runs on a single device (sharding over several GPUs is "
+              "designed, not run); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_calibration:

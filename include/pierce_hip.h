@@ -558,7 +558,8 @@ typedef struct {
  * terms and their sum are evaluated on the current HIP device (the sum in
  * double, in a fixed order), and the total is returned as float; -1.0f on
  * failure, as the reference returns.  A C++ definition of the reference's
- * own declaration that forwards here is shown in INTEGRATION.md. */
+ * own declaration can forward here with the arrays reinterpreted (the
+ * layouts are the same). */
 float pz_estimate_overlap_selectivity_cells(const pz_cell_stats* h_grid_a, const pz_cell_stats* h_grid_b,
                                             int32_t num_cells, float cell_volume, float epsilon, float gamma);
 
