@@ -833,12 +833,23 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         uint32_t* wctr = buf<uint32_t>(c, "sv.wctr", 4);
         int cus = 0;
         HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-        pzk::pair_sieve(pairs, np, sa, sb, meta, f32, allow_f32, found, defer, work, gwork, wctr, esc, sctr, cap_esc,
+        // parity-ray candidates: about a dozen target triangles per ray
+        const uint64_t cap_rays = inside ? std::min<uint64_t>(std::max<uint64_t>(64 * np, 1 << 16), 1ull << 26) : 1;
+        uint2* rays = buf<uint2>(c, "sv.rays", cap_rays);
+        unsigned long long* nrays = buf<unsigned long long>(c, "sv.nrays", 1);
+        uint32_t* rovf = buf<uint32_t>(c, "sv.ray_overflow", np + 1);
+        pzk::fill_ull(nrays, 0ull, 1, s);
+        if (inside) pzk::fill_u32(rovf, 0u, np + 1, s);
+        pzk::pair_sieve(pairs, np, sa, sb, meta, f32, allow_f32, found, defer, work, gwork, wctr,
+                        inside ? rays : nullptr, nrays, inside ? cap_rays : 0, rovf, esc, sctr, cap_esc,
                         (unsigned)std::max(cus, 1), stats, s);
         if (f32 && allow_f32) pzk::escalate_tris(esc, sctr, cap_esc, sa, sb, meta, found, stats, s);
         if (inside) {
             ph.begin("Containment Parity");
-            pzk::sieve_parity(pairs, np, sa, sb, meta, found, defer, inside, stats, s);
+            uint32_t* cross = buf<uint32_t>(c, "sv.cross", 2 * np + 2);
+            pzk::fill_u32(cross, 0u, 2 * np + 2, s);
+            pzk::sieve_parity(pairs, np, sa, sb, meta, found, defer, rays, nrays, cap_rays, rovf, cross, inside,
+                              stats, s);
         }
         uint64_t* dp = buf<uint64_t>(c, "sv.pairs", np + 1);
         tindex = buf<uint32_t>(c, "sv.index", np + 1);

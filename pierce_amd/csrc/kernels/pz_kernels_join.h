@@ -237,15 +237,22 @@ void fill_ull(unsigned long long* p, unsigned long long v, uint64_t n, hipStream
 // tree path (nothing else written for it).  With f32 (and allow_f32) open triangle pairs go to esc as (pair,
 // mesh1 triangle, mesh2 triangle) for escalate_tris.
 // work and gwork hold np pair indices each, ctr 4 words; blocks = compute units.
+// rays (rays_cap entries) and nrays: the parity-ray candidates the first pass
+// records for sieve_parity; ray_overflow (np flags) marks pairs some of whose
+// candidates did not fit.
 void pair_sieve(const uint64_t* pairs, uint64_t np, const SieveSet& A, const SieveSet& B, const uint8_t* meta,
                 bool f32, int allow_f32, uint32_t* found, uint32_t* defer, uint32_t* work, uint32_t* gwork,
-                uint32_t* ctr, uint4* esc, unsigned long long* esc_count, uint64_t esc_cap, unsigned blocks,
+                uint32_t* ctr, uint2* rays, unsigned long long* nrays, uint64_t rays_cap, uint32_t* ray_overflow,
+                uint4* esc, unsigned long long* esc_count, uint64_t esc_cap, unsigned blocks,
                 unsigned long long* stats, hipStream_t s);
 // Parity rays (parity()'s semantics) of the pairs the sieve decided that do
-// not touch; run after escalate_tris.
+// not touch, from the candidates its first pass recorded; run after
+// escalate_tris.
+// cross: 2 np counters, zeroed.
 void sieve_parity(const uint64_t* pairs, uint64_t np, const SieveSet& A, const SieveSet& B, const uint8_t* meta,
-                  const uint32_t* found, const uint32_t* defer, uint32_t* inside, unsigned long long* stats,
-                  hipStream_t s);
+                  const uint32_t* found, const uint32_t* defer, const uint2* rays, const unsigned long long* nrays,
+                  uint64_t rays_cap, const uint32_t* overflow, uint32_t* cross, uint32_t* inside,
+                  unsigned long long* stats, hipStream_t s);
 void escalate_tris(const uint4* list, const unsigned long long* count, uint64_t cap, const SieveSet& A,
                    const SieveSet& B, const uint8_t* meta, uint32_t* found, unsigned long long* stats, hipStream_t s);
 

@@ -91,6 +91,60 @@ __device__ __forceinline__ bool meets_ray(const float* lo, const float* hi, cons
     return lo[0] <= p.x && hi[0] >= p.x && lo[1] <= p.y && hi[1] >= p.y && hi[2] >= p.z;
 }
 
+// The +Z parity rays a pair asks for, as the tree path's parity kernel casts
+// them: dir 0 from the first vertex of the mesh2 object into the mesh1 object
+// (containment; intersection's mesh2->mesh1 direction), dir 1 from the first
+// vertex of the mesh1 object into the mesh2 object (intersection's
+// mesh1->mesh2 direction).  cast[d]: the ray is wanted and starts inside the
+// target's box footprint, below its top (a ray that does not crosses
+// nothing).
+struct PairRays {
+    float3 o[2];
+    bool want[2], cast[2];
+    __device__ void init(const SieveSet& A, const SieveSet& B, int32_t oa, int32_t ob, uint32_t fa, uint32_t fb,
+                         int meta) {
+        const int pred = meta & 3, dm = meta >> 2;
+        want[0] = pred == 2 || (pred == 1 && (dm & 2));
+        want[1] = pred == 1 && (dm & 1);
+        o[0] = want[0] ? vtx(B.v, B.idx[3ull * fb]) : make_float3(0.f, 0.f, 0.f);
+        o[1] = want[1] ? vtx(A.v, A.idx[3ull * fa]) : make_float3(0.f, 0.f, 0.f);
+        for (int d = 0; d < 2; ++d) {
+            const SieveSet& T = d == 0 ? A : B;
+            const int32_t ot = d == 0 ? oa : ob;
+            const float x0 = ord2f(T.obox[6ll * ot]), y0 = ord2f(T.obox[6ll * ot + 1]);
+            const float x1 = ord2f(T.obox[6ll * ot + 3]), y1 = ord2f(T.obox[6ll * ot + 4]);
+            const float z1 = ord2f(T.obox[6ll * ot + 5]);
+            cast[d] = want[d] && o[d].x >= x0 && o[d].x <= x1 && o[d].y >= y0 && o[d].y <= y1 && o[d].z <= z1;
+        }
+    }
+};
+
+// Ray candidates the sieve's first pass records: (pair, triangle | dir << 31)
+// for every triangle of a ray's target whose box meets the ray's box.
+struct RayList {
+    uint2* list;
+    unsigned long long* count;
+    uint64_t cap;
+    uint32_t* overflow;  // per pair: a candidate did not fit
+};
+
+// Appends lane's candidate (if any) with one atomic per wave.
+__device__ __forceinline__ void append_ray(const RayList& R, bool m, uint32_t p, uint32_t t, int dir) {
+    const uint64_t bal = __ballot(m);
+    if (!bal) return;
+    const int leader = __ffsll((long long)bal) - 1;
+    unsigned long long base = 0;
+    if (__lane_id() == leader) base = atomicAdd(R.count, (unsigned long long)__popcll(bal));
+    base = __shfl((long long)base, leader, PZ_WAVE);
+    if (m) {
+        const unsigned long long slot = base + lanes_below(bal);
+        if (slot < R.cap)
+            R.list[slot] = make_uint2(p, t | ((uint32_t)dir << 31));
+        else
+            R.overflow[p] = 1u;
+    }
+}
+
 struct Grid {
     float lo[3], inv[3];
     int R[3];
@@ -162,7 +216,7 @@ template <bool F32>
 __global__ __launch_bounds__(kSieveThreads) void k_pair_sieve(
     const uint64_t* __restrict__ pairs, const uint32_t* __restrict__ work, uint32_t* __restrict__ ctr,
     SieveSet A, SieveSet B, const uint8_t* __restrict__ meta, int allow_f32, uint32_t* __restrict__ found,
-    uint32_t* __restrict__ defer,
+    uint32_t* __restrict__ defer, RayList rays,
     uint4* __restrict__ esc, unsigned long long* __restrict__ esc_count, uint64_t esc_cap,
     unsigned long long* __restrict__ stats) {
     constexpr int T = kSieveThreads;
@@ -185,7 +239,9 @@ __global__ __launch_bounds__(kSieveThreads) void k_pair_sieve(
         if (tid == 0) s_item = atomicAdd(&ctr[3], 1u);
         __syncthreads();
         if (s_item >= ctr[1]) break;  // workgroup-uniform
-        const uint32_t p = work[s_item];
+        // a pair handed on by the vertex-run sieve has its ray candidates already
+        const uint32_t w = work[s_item], p = w & 0x7FFFFFFFu;
+        const bool record_rays = (w >> 31) == 0;
         const uint64_t key = pairs[p];
         const int32_t oa = (int32_t)(key >> 32), ob = (int32_t)(uint32_t)key;
         const uint32_t fa = A.first_tri[oa], ca = A.tri_cnt[oa], fb = B.first_tri[ob], cb = B.tri_cnt[ob];
@@ -202,6 +258,9 @@ __global__ __launch_bounds__(kSieveThreads) void k_pair_sieve(
             olo[k] = fmaxf(ord2f(A.obox[6ll * oa + k]), ord2f(B.obox[6ll * ob + k]));
             ohi[k] = fminf(ord2f(A.obox[6ll * oa + 3 + k]), ord2f(B.obox[6ll * ob + 3 + k]));
         }
+        PairRays pr;
+        pr.init(A, B, oa, ob, fa, fb, meta[p]);
+        const bool rays0 = record_rays && pr.cast[0], rays1 = record_rays && pr.cast[1];
         if (tid == 0) {
             s_nA = s_nB = s_nref = s_hit = s_defer = 0;
             s_ext[0] = s_ext[1] = s_ext[2] = 0.f;
@@ -224,12 +283,14 @@ __global__ __launch_bounds__(kSieveThreads) void k_pair_sieve(
             for (int u = 0; u < kU; ++u) {
                 const uint32_t k = k0 + u * T;
                 float lo[3], hi[3];
+                bool ra = false, rb = false;
                 if (k < ca) {
                     tri_box(ta[u], lo, hi);
                     if (box_box(lo, hi, olo, ohi)) {
                         const int s = atomicAdd(&s_nA, 1);
                         if (s < kSieveKeep) keepA[s] = (uint16_t)k;
                     }
+                    ra = rays0 && meets_ray(lo, hi, pr.o[0]);
                 }
                 if (k < cb) {
                     tri_box(tb[u], lo, hi);
@@ -237,7 +298,10 @@ __global__ __launch_bounds__(kSieveThreads) void k_pair_sieve(
                         const int s = atomicAdd(&s_nB, 1);
                         if (s < kSieveKeep) keepB[s] = (uint16_t)k;
                     }
+                    rb = rays1 && meets_ray(lo, hi, pr.o[1]);
                 }
+                if (rays0) append_ray(rays, ra, p, fa + k, 0);
+                if (rays1) append_ray(rays, rb, p, fb + k, 1);
             }
         }
         __syncthreads();
@@ -425,7 +489,7 @@ template <bool F32>
 __global__ __launch_bounds__(kRunThreads, 4) void k_pair_sieve_runs(
     const uint64_t* __restrict__ pairs, const uint32_t* __restrict__ work, uint32_t* __restrict__ ctr,
     SieveSet A, SieveSet B, const uint8_t* __restrict__ meta, uint32_t* __restrict__ found,
-    uint32_t* __restrict__ defer, uint32_t* __restrict__ gwork, uint4* __restrict__ esc,
+    uint32_t* __restrict__ defer, uint32_t* __restrict__ gwork, RayList rays, uint4* __restrict__ esc,
     unsigned long long* __restrict__ esc_count, uint64_t esc_cap, unsigned long long* __restrict__ stats) {
     constexpr int T = kRunThreads;
     __shared__ float vx[2][kRunV], vy[2][kRunV], vz[2][kRunV];
@@ -451,7 +515,7 @@ __global__ __launch_bounds__(kRunThreads, 4) void k_pair_sieve_runs(
     };
     __shared__ uint32_t s_item;
     // a pair handed on to the gather sieve
-    auto to_gather = [&](uint32_t pp) { gwork[atomicAdd(&ctr[1], 1u)] = pp; };
+    auto to_gather = [&](uint32_t pp) { gwork[atomicAdd(&ctr[1], 1u)] = pp | 0x80000000u; };
     while (true) {  // pairs of the work list (classified by k_sieve_classify), one at a time
         __syncthreads();
         if (tid == 0) s_item = atomicAdd(&ctr[2], 1u);
@@ -470,6 +534,8 @@ __global__ __launch_bounds__(kRunThreads, 4) void k_pair_sieve_runs(
             olo[k] = fmaxf(ord2f(A.obox[6ll * oa + k]), ord2f(B.obox[6ll * ob + k]));
             ohi[k] = fminf(ord2f(A.obox[6ll * oa + 3 + k]), ord2f(B.obox[6ll * ob + 3 + k]));
         }
+        PairRays pr;
+        pr.init(A, B, oa, ob, fa, fb, meta[p]);
         if (tid == 0) {
             s_n[0] = s_n[1] = s_nref = s_ncand = s_next = s_hit = s_defer = 0;
             s_ext[0] = s_ext[1] = s_ext[2] = 0.f;
@@ -489,6 +555,7 @@ __global__ __launch_bounds__(kRunThreads, 4) void k_pair_sieve_runs(
         }
         __syncthreads();
         // 1. triangles whose boxes meet O, appended a wave at a time
+#pragma unroll
         for (int side = 0; side < 2; ++side) {
             const SieveSet S = side == 0 ? A : B;
             const uint32_t f = side == 0 ? fa : fb, cnt = side == 0 ? ca : cb, v0 = side == 0 ? va : vb;
@@ -504,13 +571,15 @@ __global__ __launch_bounds__(kRunThreads, 4) void k_pair_sieve_runs(
 #pragma unroll
                 for (int u = 0; u < kUR; ++u) {
                     const uint32_t k = k0 + u * T;
-                    bool m = false;
+                    bool m = false, rm = false;
                     if (k < cnt) {
                         const Tri t{corner(side, ix[u][0]), corner(side, ix[u][1]), corner(side, ix[u][2])};
                         float lo[3], hi[3];
                         tri_box(t, lo, hi);
                         m = box_box(lo, hi, olo, ohi);
+                        rm = pr.cast[side] && meets_ray(lo, hi, pr.o[side]);
                     }
+                    if (pr.cast[side]) append_ray(rays, rm, p, f + k, side);
                     const uint64_t bal = __ballot(m);
                     if (bal) {
                         const int leader = __ffsll((long long)bal) - 1;
@@ -768,6 +837,71 @@ __global__ __launch_bounds__(kRunThreads, 4) void k_pair_sieve_runs(
     }
 }
 
+// The ray candidates the sieve recorded, thread per candidate: the perturbed
+// crossing test of the tree path's parity kernel, crossings counted per
+// (pair, direction) for pairs neither touching nor deferred.
+__global__ __launch_bounds__(256) void k_sieve_rays(const uint2* __restrict__ list,
+                                                   const unsigned long long* __restrict__ count, uint64_t cap,
+                                                   const uint64_t* __restrict__ pairs, SieveSet A, SieveSet B,
+                                                   const uint32_t* __restrict__ found,
+                                                   const uint32_t* __restrict__ defer,
+                                                   const uint32_t* __restrict__ overflow,
+                                                   uint32_t* __restrict__ cross,
+                                                   unsigned long long* __restrict__ stats) {
+    const uint64_t n = *count < cap ? *count : cap;
+    LaneStats st{0, 0, 0, 0, 0};
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+        const uint2 e = list[i];
+        const uint32_t p = e.x, t = e.y & 0x7FFFFFFFu;
+        const int dir = (int)(e.y >> 31);
+        if (found[p] || defer[p] || overflow[p]) continue;
+        const uint64_t key = pairs[p];
+        const int32_t oa = (int32_t)(key >> 32), ob = (int32_t)(uint32_t)key;
+        const SieveSet Tg = dir == 0 ? A : B, Src = dir == 0 ? B : A;
+        const float3 pt = vtx(Src.v, Src.idx[3ull * Src.first_tri[dir == 0 ? ob : oa]]);
+        const Tri tr = load_tri(Tg, t);
+        Esc esc{0};
+        if (ray_cross(pt, tr.a, tr.b, tr.c, esc)) atomicAdd(&cross[2ull * p + dir], 1u);
+        ++st.tests;
+        if (esc.level == 2) ++st.exact;
+    }
+    flush_stats(st, stats);
+    const uint32_t tests = wave_sum(st.tests);
+    if (__lane_id() == 0 && tests) atomicAdd(&stats[pzk::kStatParityRays + 4], (unsigned long long)tests);
+}
+
+// Parity results of the sieved pairs, thread per (pair, direction): the rays
+// the tree path would cast (wanted, pair not touching), their crossings, and
+// inside[] bit dir for an odd count.  Pairs whose candidates overflowed are
+// left to k_sieve_parity.
+__global__ __launch_bounds__(256) void k_sieve_parity_finish(const uint64_t* __restrict__ pairs, uint64_t np,
+                                                            const uint8_t* __restrict__ meta,
+                                                            const uint32_t* __restrict__ found,
+                                                            const uint32_t* __restrict__ defer,
+                                                            const uint32_t* __restrict__ overflow,
+                                                            const uint32_t* __restrict__ cross,
+                                                            uint32_t* __restrict__ inside,
+                                                            unsigned long long* __restrict__ stats) {
+    const uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint64_t p = g >> 1;
+    const int dir = (int)(g & 1);
+    bool cast = false;
+    uint32_t c = 0;
+    if (p < np && !found[p] && !defer[p] && !overflow[p]) {
+        const int pred = meta[p] & 3, dm = meta[p] >> 2;
+        cast = (pred == 2 && dir == 0) || (pred == 1 && (dm & (dir == 0 ? 2 : 1)));
+        if (cast) c = cross[2 * p + dir];
+    }
+    const bool odd = (c & 1u) != 0;
+    if (odd) atomicOr(&inside[p], 1u << dir);
+    // rays cast, triangles crossed, rays crossing something, points inside
+    const uint32_t pv[4] = {cast ? 1u : 0u, c, c ? 1u : 0u, odd ? 1u : 0u};
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t sum = wave_sum(pv[k]);
+        if (__lane_id() == 0 && sum) atomicAdd(&stats[pzk::kStatParityRays + k], (unsigned long long)sum);
+    }
+}
+
 // Parity rays of the sieved pairs, wave per (pair, direction), as the tree
 // path's parity kernel casts them: direction 0 = first vertex of the mesh2
 // object in the mesh1 object (containment; intersection's mesh2->mesh1
@@ -779,13 +913,14 @@ __global__ __launch_bounds__(256) void k_sieve_parity(const uint64_t* __restrict
                                                      SieveSet A, SieveSet B, const uint8_t* __restrict__ meta,
                                                      const uint32_t* __restrict__ found,
                                                      const uint32_t* __restrict__ defer,
+                                                     const uint32_t* __restrict__ only,
                                                      uint32_t* __restrict__ inside,
                                                      unsigned long long* __restrict__ stats) {
     const uint64_t wid = ((uint64_t)blockIdx.x * 256 + threadIdx.x) / PZ_WAVE;
     const uint64_t p = wid >> 1;
     const int dir = (int)(wid & 1);
     const int lane = __lane_id();
-    if (p >= np || found[p] || defer[p]) return;  // wave-uniform
+    if (p >= np || found[p] || defer[p] || !only[p]) return;  // wave-uniform
     const int pred = meta[p] & 3, dm = meta[p] >> 2;
     if (!((pred == 2 && dir == 0) || (pred == 1 && (dm & (dir == 0 ? 2 : 1))))) return;
     const uint64_t key = pairs[p];
@@ -906,9 +1041,11 @@ namespace pzk {
 
 void pair_sieve(const uint64_t* pairs, uint64_t np, const SieveSet& A, const SieveSet& B, const uint8_t* meta,
                 bool f32, int allow_f32, uint32_t* found, uint32_t* defer, uint32_t* work, uint32_t* gwork,
-                uint32_t* ctr, uint4* esc, unsigned long long* esc_count, uint64_t esc_cap, unsigned blocks,
+                uint32_t* ctr, uint2* rays, unsigned long long* nrays, uint64_t rays_cap, uint32_t* ray_overflow,
+                uint4* esc, unsigned long long* esc_count, uint64_t esc_cap, unsigned blocks,
                 unsigned long long* stats, hipStream_t s) {
     if (np == 0) return;
+    const RayList rl{rays, nrays, rays_cap, ray_overflow};
     const bool fp32 = f32 && allow_f32;
     const bool runs = A.vmin && B.vmin;
     KernelTimer kt_("pair_sieve", s);
@@ -921,28 +1058,33 @@ void pair_sieve(const uint64_t* pairs, uint64_t np, const SieveSet& A, const Sie
     if (runs) {
         if (fp32)
             hipLaunchKernelGGL(k_pair_sieve_runs<true>, dim3(gr), dim3(kRunThreads), 0, s, pairs, work, ctr, A, B,
-                               meta, found, defer, gwork, esc, esc_count, esc_cap, stats);
+                               meta, found, defer, gwork, rl, esc, esc_count, esc_cap, stats);
         else
             hipLaunchKernelGGL(k_pair_sieve_runs<false>, dim3(gr), dim3(kRunThreads), 0, s, pairs, work, ctr, A, B,
-                               meta, found, defer, gwork, esc, esc_count, esc_cap, stats);
+                               meta, found, defer, gwork, rl, esc, esc_count, esc_cap, stats);
     }
     const unsigned gg = (unsigned)std::min<uint64_t>(np, 4ull * blocks);
     if (fp32)
         hipLaunchKernelGGL(k_pair_sieve<true>, dim3(gg), dim3(kSieveThreads), 0, s, pairs, gwork, ctr, A, B, meta,
-                           allow_f32, found, defer, esc, esc_count, esc_cap, stats);
+                           allow_f32, found, defer, rl, esc, esc_count, esc_cap, stats);
     else
         hipLaunchKernelGGL(k_pair_sieve<false>, dim3(gg), dim3(kSieveThreads), 0, s, pairs, gwork, ctr, A, B, meta,
-                           allow_f32, found, defer, esc, esc_count, esc_cap, stats);
+                           allow_f32, found, defer, rl, esc, esc_count, esc_cap, stats);
 }
 
 void sieve_parity(const uint64_t* pairs, uint64_t np, const SieveSet& A, const SieveSet& B, const uint8_t* meta,
-                  const uint32_t* found, const uint32_t* defer, uint32_t* inside, unsigned long long* stats,
-                  hipStream_t s) {
+                  const uint32_t* found, const uint32_t* defer, const uint2* rays, const unsigned long long* nrays,
+                  uint64_t rays_cap, const uint32_t* overflow, uint32_t* cross, uint32_t* inside,
+                  unsigned long long* stats, hipStream_t s) {
     if (np == 0) return;
     KernelTimer kt_("sieve_parity", s);
-    const uint64_t waves = 2 * np;
+    hipLaunchKernelGGL(k_sieve_rays, dim3(1024), dim3(256), 0, s, rays, nrays, rays_cap, pairs, A, B, found, defer,
+                       overflow, cross, stats);
+    hipLaunchKernelGGL(k_sieve_parity_finish, dim3((unsigned)((2 * np + 255) / 256)), dim3(256), 0, s, pairs, np,
+                       meta, found, defer, overflow, cross, inside, stats);
+    const uint64_t waves = 2 * np;  // pairs whose candidates did not fit: every target triangle
     hipLaunchKernelGGL(k_sieve_parity, dim3((unsigned)((waves * PZ_WAVE + 255) / 256)), dim3(256), 0, s, pairs, np, A,
-                       B, meta, found, defer, inside, stats);
+                       B, meta, found, defer, overflow, inside, stats);
 }
 
 void escalate_tris(const uint4* list, const unsigned long long* count, uint64_t cap, const SieveSet& A,
