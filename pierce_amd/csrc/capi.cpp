@@ -828,7 +828,7 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
                                  D->vruns ? D->topo + 4 * n : nullptr, D->vruns ? D->topo + 3 * n : nullptr};
         };
         const pzd::SieveSet sa = sieve_set(A, pa), sb = sieve_set(B, pb);
-        uint32_t* work = buf<uint32_t>(c, "sv.work", np + 1);
+        uint4* rdesc = buf<uint4>(c, "sv.rdesc", 5 * (np + 1));
         uint32_t* gwork = buf<uint32_t>(c, "sv.gwork", np + 1);
         uint32_t* wctr = buf<uint32_t>(c, "sv.wctr", 4);
         int cus = 0;
@@ -840,7 +840,7 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         uint32_t* rovf = buf<uint32_t>(c, "sv.ray_overflow", np + 1);
         pzk::fill_ull(nrays, 0ull, 1, s);
         if (inside) pzk::fill_u32(rovf, 0u, np + 1, s);
-        pzk::pair_sieve(pairs, np, sa, sb, meta, f32, allow_f32, found, defer, work, gwork, wctr,
+        pzk::pair_sieve(pairs, np, sa, sb, meta, f32, allow_f32, found, defer, rdesc, gwork, wctr,
                         inside ? rays : nullptr, nrays, inside ? cap_rays : 0, rovf, esc, sctr, cap_esc,
                         (unsigned)std::max(cus, 1), stats, s);
         if (f32 && allow_f32) pzk::escalate_tris(esc, sctr, cap_esc, sa, sb, meta, found, stats, s);

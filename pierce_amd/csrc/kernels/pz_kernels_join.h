@@ -236,12 +236,13 @@ void fill_ull(unsigned long long* p, unsigned long long v, uint64_t n, hipStream
 // trees: found[p] = 1 for a proven hit, defer[p] = 1 for a pair left to the
 // tree path (nothing else written for it).  With f32 (and allow_f32) open triangle pairs go to esc as (pair,
 // mesh1 triangle, mesh2 triangle) for escalate_tris.
-// work and gwork hold np pair indices each, ctr 4 words; blocks = compute units.
+// run_desc holds 5 np uint4 (the vertex-run sieve's per-pair descriptors), gwork
+// np pair indices, ctr 4 words; blocks = compute units.
 // rays (rays_cap entries) and nrays: the parity-ray candidates the first pass
 // records for sieve_parity; ray_overflow (np flags) marks pairs some of whose
 // candidates did not fit.
 void pair_sieve(const uint64_t* pairs, uint64_t np, const SieveSet& A, const SieveSet& B, const uint8_t* meta,
-                bool f32, int allow_f32, uint32_t* found, uint32_t* defer, uint32_t* work, uint32_t* gwork,
+                bool f32, int allow_f32, uint32_t* found, uint32_t* defer, uint4* run_desc, uint32_t* gwork,
                 uint32_t* ctr, uint2* rays, unsigned long long* nrays, uint64_t rays_cap, uint32_t* ray_overflow,
                 uint4* esc, unsigned long long* esc_count, uint64_t esc_cap, unsigned blocks,
                 unsigned long long* stats, hipStream_t s);

@@ -472,253 +472,450 @@ __global__ __launch_bounds__(kSieveThreads) void k_pair_sieve(
 
 // ------------------------------------------------- sieve over vertex runs
 // The same three steps for pairs of objects that own vertex runs of at most
-// kRunV vertices: both runs are first copied into LDS (coalesced), so the
-// triangles' corners are LDS reads and the only other global reads are the
-// objects' index triples, kUR per object per thread in flight at once.  A
-// kept triangle is its three run-local vertex numbers and its number in the
-// object (8 B); grid-side boxes are recomputed from LDS where needed.
+// kRunV vertices.  k_sieve_classify writes each such pair a descriptor (its
+// objects' triangle and vertex ranges, the box O, the parity rays it casts,
+// and the grid side: the object expected to keep fewer triangles), so a
+// workgroup reads one descriptor and then issues every global load of the
+// pair at once: both vertex runs (copied to LDS, coalesced) and the first kUR
+// triangles per thread of each object's index triples (kept in registers).
+// The next work item is fetched while the current pair runs.
+//  1. the grid side's triangles whose boxes meet O are kept as 16-bit
+//     quantized boxes over O (GridQ) with their numbers in the object;
+//  2. they are binned into up to kRunCells cells over O;
+//  3. one pass over the other side keeps its triangles whose boxes meet O and
+//     whose rows of cells hold references, again as quantized boxes, and the
+//     walk tests those against the references with integer box tests.
+// Triangle corners for the exact tests are read back through the index
+// triples (a few box-meeting pairs per object pair).
 constexpr int kRunV = 672;      // vertices per object
 constexpr int kRunKeep = 1280;  // kept triangles per side
 constexpr int kRunG = 1088;     // grid-side triangles
-constexpr int kRunRefs = 3584;
+constexpr int kRunRefs = 6144;
 constexpr int kUR = 4;
 constexpr int kWalkILP = 4;  // grid references a lane tests per step
 constexpr int kRunThreads = 512;
 
+// One vertex-run work item (5 x 16 B): x = pair, first triangles (A, B),
+// meta byte | cast flags << 8 | grid side (1 = mesh2) << 10; y = first
+// vertices (A, B), triangle count | vertex count << 16 (A, B); z, w, v = O's
+// corners and the two ray origins.
+struct RunDesc {
+    uint4 x, y;
+    float4 z, w, v;
+};
+constexpr int kRunDescWords = 5;
+
+__device__ __forceinline__ RunDesc make_run_desc(const SieveSet& A, const SieveSet& B, uint32_t p, int32_t oa,
+                                                 int32_t ob, uint8_t meta) {
+    RunDesc d;
+    const uint32_t fa = A.first_tri[oa], fb = B.first_tri[ob];
+    PairRays pr;
+    pr.init(A, B, oa, ob, fa, fb, meta);
+    float olo[3], ohi[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        olo[k] = fmaxf(ord2f(A.obox[6ll * oa + k]), ord2f(B.obox[6ll * ob + k]));
+        ohi[k] = fminf(ord2f(A.obox[6ll * oa + 3 + k]), ord2f(B.obox[6ll * ob + 3 + k]));
+    }
+    const uint32_t va = A.vmin[oa], vb = B.vmin[ob];
+    const uint32_t na = A.vmax[oa] - va + 1, nb = B.vmax[ob] - vb + 1;
+    // the grid side: the object expected to keep fewer triangles (its triangle
+    // count times the share of its box that O covers)
+    float va_ = 1.f, vb_ = 1.f, vo = 1.f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        va_ *= ord2f(A.obox[6ll * oa + 3 + k]) - ord2f(A.obox[6ll * oa + k]);
+        vb_ *= ord2f(B.obox[6ll * ob + 3 + k]) - ord2f(B.obox[6ll * ob + k]);
+        vo *= ohi[k] - olo[k];
+    }
+    const float ka = (float)A.tri_cnt[oa] * (va_ > 0.f ? fminf(1.f, vo / va_) : 1.f);
+    const float kb = (float)B.tri_cnt[ob] * (vb_ > 0.f ? fminf(1.f, vo / vb_) : 1.f);
+    d.x = make_uint4(p, fa, fb, (uint32_t)meta | (pr.cast[0] ? 0x100u : 0u) | (pr.cast[1] ? 0x200u : 0u) |
+                                    (kb < ka ? 0x400u : 0u));
+    d.y = make_uint4(va, vb, A.tri_cnt[oa] | (na << 16), B.tri_cnt[ob] | (nb << 16));
+    d.z = make_float4(olo[0], olo[1], olo[2], ohi[0]);
+    d.w = make_float4(ohi[1], ohi[2], pr.o[0].x, pr.o[0].y);
+    d.v = make_float4(pr.o[0].z, pr.o[1].x, pr.o[1].y, pr.o[1].z);
+    return d;
+}
+
+// Grid of the vertex-run sieve over O in 16-bit fixed point: q = (x - lo) *
+// scale per axis, clamped to [0, 65535], lower corners rounded down and upper
+// corners up (one step of margin each way), so a quantized box contains the
+// part of the triangle's box inside O; cells are (q * R) >> 16.  Boxes and
+// cells of both sides come from the same quantization, so two triangles whose
+// quantized boxes meet share the cell of the larger lower corner.
+constexpr int kRunCells = 2048;
+struct GridQ {
+    float lo[3], scale[3];
+    uint32_t R[3];
+    __device__ void quant(const float* blo, const float* bhi, uint32_t* qlo, uint32_t* qhi) const {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float a = (blo[k] - lo[k]) * scale[k], b = (bhi[k] - lo[k]) * scale[k];
+            qlo[k] = (uint32_t)fminf(fmaxf(floorf(a) - 1.f, 0.f), 65535.f);
+            qhi[k] = (uint32_t)fminf(fmaxf(ceilf(b) + 1.f, 0.f), 65535.f);
+        }
+    }
+    __device__ uint32_t cell(uint32_t q, int k) const { return (q * R[k]) >> 16; }
+};
+
+__device__ GridQ make_grid_q(const float* olo, const float* ohi, const float* ext_sum, int n) {
+    GridQ g;
+    uint32_t r[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float e = ohi[k] - olo[k], m = ext_sum[k] / (float)n;
+        const float q = (e > 0.f && m > 0.f) ? e / (kCellScale * m) : 1.f;
+        r[k] = q >= 16.f ? 16u : (q >= 1.f ? (uint32_t)q : 1u);
+    }
+    while (r[0] * r[1] * r[2] > (uint32_t)kRunCells) {  // halve the finest axis
+        if (r[0] >= r[1] && r[0] >= r[2])
+            r[0] = (r[0] + 1) / 2;
+        else if (r[1] >= r[2])
+            r[1] = (r[1] + 1) / 2;
+        else
+            r[2] = (r[2] + 1) / 2;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float e = ohi[k] - olo[k];
+        g.R[k] = r[k];
+        g.lo[k] = olo[k];
+        g.scale[k] = e > 0.f ? 65535.f / e : 0.f;
+    }
+    return g;
+}
+
 template <bool F32>
 __global__ __launch_bounds__(kRunThreads, 4) void k_pair_sieve_runs(
-    const uint64_t* __restrict__ pairs, const uint32_t* __restrict__ work, uint32_t* __restrict__ ctr,
-    SieveSet A, SieveSet B, const uint8_t* __restrict__ meta, uint32_t* __restrict__ found,
-    uint32_t* __restrict__ defer, uint32_t* __restrict__ gwork, RayList rays, uint4* __restrict__ esc,
-    unsigned long long* __restrict__ esc_count, uint64_t esc_cap, unsigned long long* __restrict__ stats) {
+    const RunDesc* __restrict__ rdesc, uint32_t* __restrict__ ctr, SieveSet A, SieveSet B,
+    uint32_t* __restrict__ found, uint32_t* __restrict__ defer, uint32_t* __restrict__ gwork, RayList rays,
+    uint4* __restrict__ esc, unsigned long long* __restrict__ esc_count, uint64_t esc_cap,
+    unsigned long long* __restrict__ stats) {
     constexpr int T = kRunThreads;
     __shared__ float vx[2][kRunV], vy[2][kRunV], vz[2][kRunV];
-    __shared__ uint2 keep[2][kRunKeep];  // (v0 | v1 << 16, v2 | k << 16)
-    __shared__ float4 gb0[kRunG];  // grid-side boxes: lo.xyz, hi.x
-    __shared__ float2 gb1[kRunG];  // hi.yz
-    __shared__ uint16_t refs[kRunRefs];
-    __shared__ uint16_t cand[kRunKeep];  // other-side triangles with references in their cells
+    // grid-side kept triangles: quantized box (lo | hi << 16 per axis), then
+    // lower cells (x | y << 5 | z << 10) | number in the object << 16
+    __shared__ uint4 gq[kRunG];
+    __shared__ uint16_t refs[kRunRefs];  // grid-side triangle | cell x << 11
+    // other-side triangles with references in their cells: quantized box, number
+    __shared__ uint4 cq[kRunKeep];
     __shared__ uint32_t wpairs[T / PZ_WAVE][2 * PZ_WAVE];  // per wave: box-meeting pairs to test
-    __shared__ uint16_t cstart[kSieveCells + 1];
-    __shared__ uint32_t ccnt[kSieveCells];
-    __shared__ uint32_t wsum[T / PZ_WAVE];
-    __shared__ int s_n[2], s_nref, s_ncand, s_next, s_hit, s_defer;
+    __shared__ uint32_t ccnt[kRunCells + 1];  // counts, then inclusive ends, then starts (ccnt[n] = total)
+    __shared__ int s_nG, s_nref, s_ncand, s_next, s_hit, s_defer;
     __shared__ float s_ext[3];
-    __shared__ Grid s_grid;
+    __shared__ uint32_t s_item;
     const int tid = threadIdx.x, lane = __lane_id();
     LaneStats st{0, 0, 0, 0, 0};
     uint32_t n_sieved = 0;
     unsigned long long n_tris = 0;
     auto corner = [&](int side, uint32_t v) { return make_float3(vx[side][v], vy[side][v], vz[side][v]); };
-    auto kept_tri = [&](int side, uint2 r) {
-        return Tri{corner(side, r.x & 0xFFFFu), corner(side, r.x >> 16), corner(side, r.y & 0xFFFFu)};
-    };
-    __shared__ uint32_t s_item;
-    // a pair handed on to the gather sieve
-    auto to_gather = [&](uint32_t pp) { gwork[atomicAdd(&ctr[1], 1u)] = pp | 0x80000000u; };
-    while (true) {  // pairs of the work list (classified by k_sieve_classify), one at a time
-        __syncthreads();
-        if (tid == 0) s_item = atomicAdd(&ctr[2], 1u);
-        __syncthreads();
-        if (s_item >= ctr[0]) break;  // workgroup-uniform
-        const uint32_t p = work[s_item];
-        const uint64_t key = pairs[p];
-        const int32_t oa = (int32_t)(key >> 32), ob = (int32_t)(uint32_t)key;
-        const uint32_t fa = A.first_tri[oa], ca = A.tri_cnt[oa], fb = B.first_tri[ob], cb = B.tri_cnt[ob];
-        const uint32_t va = A.vmin[oa], vb = B.vmin[ob];
-        const uint32_t na = A.vmax[oa] - va + 1, nb = B.vmax[ob] - vb + 1;
-        const int dm = meta[p] >> 2;
-        float olo[3], ohi[3];
+    const uint32_t nrun = ctr[0];
+    if (tid == 0) s_item = atomicAdd(&ctr[2], 1u);
+    __syncthreads();
+    uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
+    while (true) {  // work items (classified by k_sieve_classify), one at a time
+        if (item >= nrun) break;  // workgroup-uniform
+        uint32_t nxt = 0;  // thread 0: the next item, fetched while this one runs
+        if (tid == 0) nxt = atomicAdd(&ctr[2], 1u);
+        const RunDesc D = rdesc[item];
+        const uint32_t p = D.x.x, fa = D.x.y, fb = D.x.z;
+        const int dm = (int)(D.x.w & 0xFFu) >> 2;
+        const bool cast[2] = {(D.x.w & 0x100u) != 0, (D.x.w & 0x200u) != 0};
+        const int gs = (D.x.w & 0x400u) ? 1 : 0, ss = 1 - gs;  // the grid side, the other side
+        const uint32_t va = D.y.x, vb = D.y.y, ca = D.y.z & 0xFFFFu, cb = D.y.w & 0xFFFFu;
+        const uint32_t na = D.y.z >> 16, nb = D.y.w >> 16;
+        const float olo[3] = {D.z.x, D.z.y, D.z.z}, ohi[3] = {D.z.w, D.w.x, D.w.y};
+        const float3 ro[2] = {make_float3(D.w.z, D.w.w, D.v.x), make_float3(D.v.y, D.v.z, D.v.w)};
+        // O in 16-bit fixed point (the grid's resolution comes later)
+        GridQ g;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            olo[k] = fmaxf(ord2f(A.obox[6ll * oa + k]), ord2f(B.obox[6ll * ob + k]));
-            ohi[k] = fminf(ord2f(A.obox[6ll * oa + 3 + k]), ord2f(B.obox[6ll * ob + 3 + k]));
+            const float e = ohi[k] - olo[k];
+            g.lo[k] = olo[k];
+            g.scale[k] = e > 0.f ? 65535.f / e : 0.f;
+            g.R[k] = 1;
         }
-        PairRays pr;
-        pr.init(A, B, oa, ob, fa, fb, meta[p]);
-        if (tid == 0) {
-            s_n[0] = s_n[1] = s_nref = s_ncand = s_next = s_hit = s_defer = 0;
-            s_ext[0] = s_ext[1] = s_ext[2] = 0.f;
-        }
-        // both vertex runs into LDS
-        for (uint32_t v = tid; v < na; v += T) {
-            const float3 q = vtx(A.v, va + v);
-            vx[0][v] = q.x;
-            vy[0][v] = q.y;
-            vz[0][v] = q.z;
-        }
-        for (uint32_t v = tid; v < nb; v += T) {
-            const float3 q = vtx(B.v, vb + v);
-            vx[1][v] = q.x;
-            vy[1][v] = q.y;
-            vz[1][v] = q.z;
-        }
-        __syncthreads();
-        // 1. triangles whose boxes meet O, appended a wave at a time
+        // every global load of the pair in flight together: both vertex runs
+        // and the first kUR triangles per thread of both objects' index triples
+        float3 q[2][2];
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
-            const SieveSet S = side == 0 ? A : B;
+            const SieveSet& S = side == 0 ? A : B;
+            const uint32_t v0 = side == 0 ? va : vb, nv = side == 0 ? na : nb;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const uint32_t v = tid + h * T;
+                q[side][h] = vtx(S.v, v0 + (v < nv ? v : 0u));
+            }
+        }
+        uint32_t ix0[2][kUR][3];  // [0] grid side, [1] the other side
+#pragma unroll
+        for (int role = 0; role < 2; ++role) {
+            const int side = role == 0 ? gs : ss;
+            const uint32_t* idx = side == 0 ? A.idx : B.idx;
+            const uint32_t f = side == 0 ? fa : fb, cnt = side == 0 ? ca : cb, v0 = side == 0 ? va : vb;
+#pragma unroll
+            for (int u = 0; u < kUR; ++u) {
+                const uint64_t t = (uint64_t)f + (tid + u * T < cnt ? tid + u * T : 0u);
+                ix0[role][u][0] = idx[3 * t] - v0;
+                ix0[role][u][1] = idx[3 * t + 1] - v0;
+                ix0[role][u][2] = idx[3 * t + 2] - v0;
+            }
+        }
+        if (tid == 0) {
+            s_nG = s_nref = s_ncand = s_next = s_hit = s_defer = 0;
+            s_ext[0] = s_ext[1] = s_ext[2] = 0.f;
+        }
+        for (int c = tid; c <= kRunCells; c += T) ccnt[c] = 0u;
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const uint32_t nv = side == 0 ? na : nb;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const uint32_t v = tid + h * T;
+                if (v < nv) {
+                    vx[side][v] = q[side][h].x;
+                    vy[side][v] = q[side][h].y;
+                    vz[side][v] = q[side][h].z;
+                }
+            }
+        }
+        if (tid == 0) s_item = nxt;  // the vertex stores above waited for this atomic already
+        __syncthreads();
+        const uint32_t item_next = __builtin_amdgcn_readfirstlane(s_item);
+        // ends this item on a path without a closing barrier of its own
+        auto next_item = [&]() {
+            item = item_next;
+            __syncthreads();
+        };
+        // Visits the triangles of one side (index triples of the first chunk
+        // from registers), casting the side's parity-ray candidates; fn(k, t,
+        // lo, hi, m) gets each triangle with its box and whether it meets O.
+        auto each_tri = [&](auto role_c, auto&& fn) {
+            constexpr int role = decltype(role_c)::value;
+            const int side = role == 0 ? gs : ss;
+            const SieveSet& S = side == 0 ? A : B;
             const uint32_t f = side == 0 ? fa : fb, cnt = side == 0 ? ca : cb, v0 = side == 0 ? va : vb;
             for (uint32_t k0 = tid; k0 < cnt; k0 += kUR * T) {
                 uint32_t ix[kUR][3];
+                if (k0 == (uint32_t)tid) {
 #pragma unroll
-                for (int u = 0; u < kUR; ++u) {
-                    const uint64_t t = (uint64_t)f + (k0 + u * T < cnt ? k0 + u * T : 0u);
-                    ix[u][0] = S.idx[3 * t] - v0;
-                    ix[u][1] = S.idx[3 * t + 1] - v0;
-                    ix[u][2] = S.idx[3 * t + 2] - v0;
+                    for (int u = 0; u < kUR; ++u) {
+                        ix[u][0] = ix0[role][u][0];
+                        ix[u][1] = ix0[role][u][1];
+                        ix[u][2] = ix0[role][u][2];
+                    }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < kUR; ++u) {
+                        const uint64_t t = (uint64_t)f + (k0 + u * T < cnt ? k0 + u * T : 0u);
+                        ix[u][0] = S.idx[3 * t] - v0;
+                        ix[u][1] = S.idx[3 * t + 1] - v0;
+                        ix[u][2] = S.idx[3 * t + 2] - v0;
+                    }
                 }
 #pragma unroll
                 for (int u = 0; u < kUR; ++u) {
                     const uint32_t k = k0 + u * T;
                     bool m = false, rm = false;
+                    float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {0.f, 0.f, 0.f};
                     if (k < cnt) {
                         const Tri t{corner(side, ix[u][0]), corner(side, ix[u][1]), corner(side, ix[u][2])};
-                        float lo[3], hi[3];
                         tri_box(t, lo, hi);
                         m = box_box(lo, hi, olo, ohi);
-                        rm = pr.cast[side] && meets_ray(lo, hi, pr.o[side]);
+                        rm = cast[side] && meets_ray(lo, hi, ro[side]);
                     }
-                    if (pr.cast[side]) append_ray(rays, rm, p, f + k, side);
-                    const uint64_t bal = __ballot(m);
-                    if (bal) {
-                        const int leader = __ffsll((long long)bal) - 1;
-                        int base = 0;
-                        if (lane == leader) base = atomicAdd(&s_n[side], __popcll(bal));
-                        base = __shfl(base, leader, PZ_WAVE);
-                        const int pos = base + (int)lanes_below(bal);
-                        if (m && pos < kRunKeep)
-                            keep[side][pos] = make_uint2(ix[u][0] | (ix[u][1] << 16), ix[u][2] | (k << 16));
+                    if (cast[side]) append_ray(rays, rm, p, f + k, side);
+                    fn(k, lo, hi, m);
+                }
+            }
+        };
+        // wave-aggregated append of lanes with m to an LDS list (count in *n)
+        auto append = [&](bool m, int* n) {
+            const uint64_t bal = __ballot(m);
+            int pos = -1;
+            if (bal) {
+                const int leader = __ffsll((long long)bal) - 1;
+                int base = 0;
+                if (lane == leader) base = atomicAdd(n, __popcll(bal));
+                base = __shfl(base, leader, PZ_WAVE);
+                if (m) pos = base + (int)lanes_below(bal);
+            }
+            return pos;
+        };
+        // 1. grid-side triangles whose boxes meet O, kept as quantized boxes;
+        // their box extents summed (for the grid's resolution)
+        {
+            float ex[3] = {0.f, 0.f, 0.f};
+            each_tri(std::integral_constant<int, 0>{}, [&](uint32_t k, const float* lo, const float* hi, bool m) {
+                const int pos = append(m, &s_nG);
+                if (m) {
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) ex[a] += hi[a] - lo[a];
+                    if (pos < kRunG) {
+                        uint32_t qlo[3], qhi[3];
+                        g.quant(lo, hi, qlo, qhi);
+                        gq[pos] = make_uint4(qlo[0] | (qhi[0] << 16), qlo[1] | (qhi[1] << 16), qlo[2] | (qhi[2] << 16),
+                                             k << 16);
                     }
                 }
+            });
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                float v = ex[a];
+                for (int sft = 32; sft > 0; sft >>= 1) v += __shfl_xor(v, sft, PZ_WAVE);
+                if (lane == 0 && v != 0.f) atomicAdd(&s_ext[a], v);
             }
         }
         __syncthreads();
-        const int nA = s_n[0], nB = s_n[1];
-        if (nA > kRunKeep || nB > kRunKeep) {  // workgroup-uniform
-            if (tid == 0) to_gather(p);
+        const int nG = s_nG;
+        if (nG > kRunG) {  // workgroup-uniform: to the gather sieve, with every ray candidate recorded
+            if (cast[ss]) each_tri(std::integral_constant<int, 1>{}, [&](uint32_t, const float*, const float*, bool) {});
+            if (tid == 0) gwork[atomicAdd(&ctr[1], 1u)] = p | 0x80000000u;
+            next_item();
             continue;
         }
-        if (nA > 0 && nB > 0 && min(nA, nB) > kRunG) {  // workgroup-uniform
-            if (tid == 0) to_gather(p);
-            continue;
-        }
-        if (nA > 0 && nB > 0) {
-            const int gs = nA <= nB ? 0 : 1, ss = 1 - gs;  // the grid side, the other side
-            const int nG = gs == 0 ? nA : nB, nS = gs == 0 ? nB : nA;
-            const uint32_t fG = gs == 0 ? fa : fb, fS = gs == 0 ? fb : fa;
-            // 2. grid-side boxes and mean extent, grid
-            float ex[3] = {0.f, 0.f, 0.f};
-            for (int i = tid; i < nG; i += T) {
-                float lo[3], hi[3];
-                tri_box(kept_tri(gs, keep[gs][i]), lo, hi);
-#pragma unroll
-                for (int k = 0; k < 3; ++k) ex[k] += hi[k] - lo[k];
-                gb0[i] = make_float4(lo[0], lo[1], lo[2], hi[0]);
-                gb1[i] = make_float2(hi[1], hi[2]);
+        if (nG == 0) {  // nothing of the grid side meets O: no triangle pair can meet
+            if (cast[ss]) each_tri(std::integral_constant<int, 1>{}, [&](uint32_t, const float*, const float*, bool) {});  // its ray candidates
+            if (tid == 0) {
+                ++n_sieved;
+                n_tris += ca + cb;
             }
+            next_item();
+            continue;
+        }
+        // 2. grid over O from the grid side's mean kept extent (every thread)
+        {
+            const GridQ gg = make_grid_q(olo, ohi, s_ext, nG);
+            g.R[0] = gg.R[0];
+            g.R[1] = gg.R[1];
+            g.R[2] = gg.R[2];
+        }
+        const uint32_t ncell = g.R[0] * g.R[1] * g.R[2];
+        auto box_cells = [&](const uint4& b, uint32_t* cl, uint32_t* ch) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                float v = ex[k];
-                for (int sft = 32; sft > 0; sft >>= 1) v += __shfl_xor(v, sft, PZ_WAVE);
-                if (lane == 0) atomicAdd(&s_ext[k], v);
+                const uint32_t w = k == 0 ? b.x : (k == 1 ? b.y : b.z);
+                cl[k] = g.cell(w & 0xFFFFu, k);
+                ch[k] = g.cell(w >> 16, k);
             }
-            __syncthreads();
-            if (tid == 0) s_grid = make_grid(olo, ohi, s_ext, nG);
-            __syncthreads();
-            const Grid g = s_grid;
-            const int ncell = g.R[0] * g.R[1] * g.R[2];
-            for (int c = tid; c < ncell; c += T) ccnt[c] = 0u;
-            __syncthreads();
-            auto cells_of_g = [&](int i, int* cl, int* ch) {
-                const float4 b0 = gb0[i];
-                const float2 b1 = gb1[i];
-                cl[0] = g.cell(b0.x, 0);
-                cl[1] = g.cell(b0.y, 1);
-                cl[2] = g.cell(b0.z, 2);
-                ch[0] = g.cell(b0.w, 0);
-                ch[1] = g.cell(b1.x, 1);
-                ch[2] = g.cell(b1.y, 2);
-            };
+        };
+        // references per cell; lower cells into the entry
+        {
             int nref = 0;
             for (int i = tid; i < nG; i += T) {
-                int cl[3], ch[3];
-                cells_of_g(i, cl, ch);
+                const uint4 b = gq[i];
+                uint32_t cl[3], ch[3];
+                box_cells(b, cl, ch);
+                gq[i].w = (b.w & 0xFFFF0000u) | cl[0] | (cl[1] << 5) | (cl[2] << 10);
                 nref += (ch[0] - cl[0] + 1) * (ch[1] - cl[1] + 1) * (ch[2] - cl[2] + 1);
                 if (nref > kRunRefs) break;
-                for (int z = cl[2]; z <= ch[2]; ++z)
-                    for (int y = cl[1]; y <= ch[1]; ++y)
-                        for (int x = cl[0]; x <= ch[0]; ++x) atomicAdd(&ccnt[(z * g.R[1] + y) * g.R[0] + x], 1u);
+                for (uint32_t z = cl[2]; z <= ch[2]; ++z)
+                    for (uint32_t y = cl[1]; y <= ch[1]; ++y)
+                        for (uint32_t x = cl[0]; x <= ch[0]; ++x) atomicAdd(&ccnt[(z * g.R[1] + y) * g.R[0] + x], 1u);
             }
             if (nref) atomicAdd(&s_nref, nref);
-            __syncthreads();
-            if (s_nref > kRunRefs) {  // workgroup-uniform
-                if (tid == 0) defer[p] = kDeferTrees;
-                __syncthreads();
-                continue;
+        }
+        __syncthreads();
+        if (s_nref > kRunRefs) {  // workgroup-uniform
+            if (tid == 0) defer[p] = kDeferTrees;
+            next_item();
+            continue;
+        }
+        // inclusive scan of the counts by the first wave
+        if (tid < PZ_WAVE) {
+            const uint32_t per = (ncell + PZ_WAVE - 1) / PZ_WAVE;
+            uint32_t run = 0;
+            for (uint32_t j = 0; j < per; ++j) {
+                const uint32_t c = tid * per + j;
+                run += c < ncell ? ccnt[c] : 0u;
             }
-            block_scan<T>(ccnt, cstart, ncell, wsum);
-            for (int c = tid; c < ncell; c += T) ccnt[c] = cstart[c];  // cursors
-            __syncthreads();
-            for (int i = tid; i < nG; i += T) {
-                int cl[3], ch[3];
-                cells_of_g(i, cl, ch);
-                for (int z = cl[2]; z <= ch[2]; ++z)
-                    for (int y = cl[1]; y <= ch[1]; ++y)
-                        for (int x = cl[0]; x <= ch[0]; ++x)
-                            refs[atomicAdd(&ccnt[(z * g.R[1] + y) * g.R[0] + x], 1u)] = (uint16_t)i;
+            uint32_t incl = run;
+            for (int d = 1; d < PZ_WAVE; d <<= 1) {
+                const uint32_t y = (uint32_t)__shfl_up((int)incl, d, PZ_WAVE);
+                if (lane >= d) incl += y;
             }
-            __syncthreads();
-            auto cells_of = [&](const float* lo, const float* hi, int* cl, int* ch) {
+            uint32_t acc = incl - run;
+            for (uint32_t j = 0; j < per; ++j) {
+                const uint32_t c = tid * per + j;
+                if (c < ncell) {
+                    acc += ccnt[c];
+                    ccnt[c] = acc;
+                }
+            }
+            if (tid == PZ_WAVE - 1) ccnt[ncell] = incl;
+        }
+        __syncthreads();
+        // scatter: each reference takes the slot below its cell's end, which
+        // leaves ccnt[c] = start of cell c
+        for (int i = tid; i < nG; i += T) {
+            uint32_t cl[3], ch[3];
+            box_cells(gq[i], cl, ch);
+            for (uint32_t z = cl[2]; z <= ch[2]; ++z)
+                for (uint32_t y = cl[1]; y <= ch[1]; ++y)
+                    for (uint32_t x = cl[0]; x <= ch[0]; ++x)
+                        refs[atomicSub(&ccnt[(z * g.R[1] + y) * g.R[0] + x], 1u) - 1u] = (uint16_t)(i | (x << 11));
+        }
+        __syncthreads();
+        // 3a. the other side's triangles whose boxes meet O and whose rows of
+        // cells hold references (most hold none), with their quantized boxes
+        each_tri(std::integral_constant<int, 1>{}, [&](uint32_t k, const float* lo, const float* hi, bool m) {
+            uint32_t qlo[3], qhi[3];
+            bool any = false;
+            if (m) {
+                g.quant(lo, hi, qlo, qhi);
+                uint32_t cl[3], ch[3];
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    cl[k] = g.cell(lo[k], k);
-                    ch[k] = g.cell(hi[k], k);
+                for (int a = 0; a < 3; ++a) {
+                    cl[a] = g.cell(qlo[a], a);
+                    ch[a] = g.cell(qhi[a], a);
                 }
-            };
-            // 3a. the other side's kept triangles whose cells hold references
-            // (most hold none), appended a wave at a time
-            for (int j0 = tid; j0 < nS; j0 += T) {
-                float lo[3], hi[3];
-                tri_box(kept_tri(ss, keep[ss][j0]), lo, hi);
-                int cl[3], ch[3];
-                cells_of(lo, hi, cl, ch);
-                bool any = false;
-                for (int z = cl[2]; z <= ch[2] && !any; ++z)
-                    for (int y = cl[1]; y <= ch[1] && !any; ++y)
-                        for (int x = cl[0]; x <= ch[0] && !any; ++x) {
-                            const int c = (z * g.R[1] + y) * g.R[0] + x;
-                            any = cstart[c + 1] > cstart[c];
-                        }
-                const uint64_t bal = __ballot(any);
-                if (bal) {
-                    const int leader = __ffsll((long long)bal) - 1;
-                    int base = 0;
-                    if (lane == leader) base = atomicAdd(&s_ncand, __popcll(bal));
-                    base = __shfl(base, leader, PZ_WAVE);
-                    if (any) cand[base + (int)lanes_below(bal)] = (uint16_t)j0;
-                }
+                for (uint32_t z = cl[2]; z <= ch[2] && !any; ++z)
+                    for (uint32_t y = cl[1]; y <= ch[1] && !any; ++y) {
+                        const uint32_t c0 = (z * g.R[1] + y) * g.R[0];
+                        any = ccnt[c0 + ch[0] + 1] > ccnt[c0 + cl[0]];  // the row's cells [cl0, ch0]
+                    }
             }
-            __syncthreads();
-            // 3b. lanes walk the references in those triangles' cells one at a
-            // time, a lane taking the next triangle of the list (block-wide
-            // counter) when its own is done; the box-meeting pairs, each once,
-            // gather in a per-wave list that the wave tests, a pair per lane,
-            // whenever 64 are waiting
-            const int ncand = s_ncand;
+            const int pos = append(any, &s_ncand);
+            if (any && pos < kRunKeep)
+                cq[pos] = make_uint4(qlo[0] | (qhi[0] << 16), qlo[1] | (qhi[1] << 16), qlo[2] | (qhi[2] << 16), k);
+        });
+        __syncthreads();
+        const int ncand = s_ncand;
+        if (ncand > kRunKeep) {  // workgroup-uniform
+            if (tid == 0) gwork[atomicAdd(&ctr[1], 1u)] = p | 0x80000000u;  // to the gather sieve
+            next_item();
+            continue;
+        }
+        if (ncand > 0) {
+            const uint32_t fG = gs == 0 ? fa : fb, fS = gs == 0 ? fb : fa;
+            const uint32_t vG = gs == 0 ? va : vb, vS = gs == 0 ? vb : va;
+            const SieveSet& SG = gs == 0 ? A : B;
+            const SieveSet& SS = gs == 0 ? B : A;
+            // 3b. lanes walk the rows of cells of those triangles, up to
+            // kWalkILP references at a time, a lane taking the next triangle of
+            // the list (block-wide counter) when its own is done; the
+            // box-meeting pairs, each once, gather in a per-wave list that the
+            // wave tests, a pair per lane, whenever 64 are waiting
             uint32_t* wq = wpairs[tid / PZ_WAVE];
             int wn = 0;  // pairs waiting (wave-uniform)
             bool live = false, exhausted = false;
-            int jc = 0, r = 0, rend = 0, cx = 0, cy = 0, cz = 0;
-            float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {0.f, 0.f, 0.f};
-            int cl[3] = {0, 0, 0}, ch[3] = {0, 0, 0};
+            int jc = 0;
+            uint32_t r = 0, rend = 0, cy = 0, cz = 0;
+            uint32_t qlo[3] = {0, 0, 0}, qhi[3] = {0, 0, 0}, cl[3] = {0, 0, 0}, ch[3] = {0, 0, 0};
             auto test_pairs = [&](int n) {  // wave-uniform n <= 64
                 if (lane < n) {
                     const uint32_t e = wq[lane];
-                    const uint2 rs = keep[ss][e >> 16], rg = keep[gs][e & 0xFFFFu];
-                    const Tri t = kept_tri(ss, rs), gt = kept_tri(gs, rg);
+                    const uint32_t ts = fS + cq[e >> 16].w, tg = fG + (gq[e & 0xFFFFu].w >> 16);
+                    const Tri t{vtx(SS.v, SS.idx[3ull * ts]), vtx(SS.v, SS.idx[3ull * ts + 1]),
+                                vtx(SS.v, SS.idx[3ull * ts + 2])};
+                    const Tri gt{vtx(SG.v, SG.idx[3ull * tg]), vtx(SG.v, SG.idx[3ull * tg + 1]),
+                                 vtx(SG.v, SG.idx[3ull * tg + 2])};
+                    (void)vG;
+                    (void)vS;
                     const LeafTri X = as_leaf(gs == 0 ? gt : t), Y = as_leaf(gs == 0 ? t : gt);
-                    const uint32_t tg = fG + (rg.y >> 16), ts = fS + (rs.y >> 16);
                     const uint32_t xa = gs == 0 ? tg : ts, yb = gs == 0 ? ts : tg;
                     bool hit;
                     if constexpr (F32) {
@@ -738,69 +935,73 @@ __global__ __launch_bounds__(kRunThreads, 4) void k_pair_sieve_runs(
                 }
             };
             while (true) {
-                if (*(volatile int*)&s_hit) break;  // one LDS read for the wave: uniform
+                if (__hip_atomic_load(&s_hit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;  // uniform
                 const bool need = !live && !exhausted;
-                const uint64_t nb = __ballot(need);
-                if (nb) {
-                    const int leader = __ffsll((long long)nb) - 1;
+                const uint64_t nbal = __ballot(need);
+                if (nbal) {
+                    const int leader = __ffsll((long long)nbal) - 1;
                     int base = 0;
-                    if (lane == leader) base = atomicAdd(&s_next, __popcll(nb));
+                    if (lane == leader) base = atomicAdd(&s_next, __popcll(nbal));
                     base = __shfl(base, leader, PZ_WAVE);
                     if (need) {
-                        const int q = base + (int)lanes_below(nb);
-                        if (q < ncand) {
-                            jc = cand[q];
-                            tri_box(kept_tri(ss, keep[ss][jc]), lo, hi);
-                            cells_of(lo, hi, cl, ch);
-                            cx = cl[0];
+                        const int qq = base + (int)lanes_below(nbal);
+                        if (qq < ncand) {
+                            jc = qq;
+                            const uint4 b = cq[qq];
+                            qlo[0] = b.x & 0xFFFFu;
+                            qhi[0] = b.x >> 16;
+                            qlo[1] = b.y & 0xFFFFu;
+                            qhi[1] = b.y >> 16;
+                            qlo[2] = b.z & 0xFFFFu;
+                            qhi[2] = b.z >> 16;
+                            box_cells(b, cl, ch);
                             cy = cl[1];
                             cz = cl[2];
-                            const int c = (cz * g.R[1] + cy) * g.R[0] + cx;
-                            r = cstart[c];
-                            rend = cstart[c + 1];
+                            const uint32_t c0 = (cz * g.R[1] + cy) * g.R[0];
+                            r = ccnt[c0 + cl[0]];
+                            rend = ccnt[c0 + ch[0] + 1];
                             live = true;
                         } else {
                             exhausted = true;
                         }
                     }
                 }
-                // skip to the next reference: the next non-empty cell of the range
+                // skip to the next reference: the next non-empty row of the range
                 while (live && r >= rend) {
-                    if (++cx > ch[0]) {
-                        cx = cl[0];
-                        if (++cy > ch[1]) {
-                            cy = cl[1];
-                            if (++cz > ch[2]) {
-                                live = false;
-                                break;
-                            }
+                    if (++cy > ch[1]) {
+                        cy = cl[1];
+                        if (++cz > ch[2]) {
+                            live = false;
+                            break;
                         }
                     }
-                    const int c = (cz * g.R[1] + cy) * g.R[0] + cx;
-                    r = cstart[c];
-                    rend = cstart[c + 1];
+                    const uint32_t c0 = (cz * g.R[1] + cy) * g.R[0];
+                    r = ccnt[c0 + cl[0]];
+                    rend = ccnt[c0 + ch[0] + 1];
                 }
-                // up to kWalkILP references of the current cell per step, their
+                // up to kWalkILP references of the current row per step, their
                 // loads in flight together
-                int gi[kWalkILP];
+                uint32_t gr[kWalkILP];
                 bool match[kWalkILP];
-                const int nr = live ? min(kWalkILP, rend - r) : 0;
+                const uint32_t nr = live ? min((uint32_t)kWalkILP, rend - r) : 0u;
 #pragma unroll
-                for (int u = 0; u < kWalkILP; ++u) gi[u] = u < nr ? refs[r + u] : 0;
+                for (int u = 0; u < kWalkILP; ++u) gr[u] = (uint32_t)u < nr ? refs[r + u] : 0u;
                 r += nr;
 #pragma unroll
                 for (int u = 0; u < kWalkILP; ++u) {
-                    const float4 b0 = gb0[gi[u]];
-                    const float2 b1 = gb1[gi[u]];
-                    const float glo[3] = {b0.x, b0.y, b0.z}, ghi[3] = {b0.w, b1.x, b1.y};
-                    // once per pair: in the cell of the larger lower corner
-                    match[u] = u < nr && box_box(lo, hi, glo, ghi) && max(g.cell(glo[0], 0), cl[0]) == cx &&
-                               max(g.cell(glo[1], 1), cl[1]) == cy && max(g.cell(glo[2], 2), cl[2]) == cz;
+                    const uint4 b = gq[gr[u] & 0x7FFu];
+                    // boxes meet, and once per pair: in the cell of the larger lower corner
+                    const bool meet = (qlo[0] <= (b.x >> 16)) & ((b.x & 0xFFFFu) <= qhi[0]) &
+                                      (qlo[1] <= (b.y >> 16)) & ((b.y & 0xFFFFu) <= qhi[1]) &
+                                      (qlo[2] <= (b.z >> 16)) & ((b.z & 0xFFFFu) <= qhi[2]);
+                    const uint32_t gx = b.w & 31u, gy = (b.w >> 5) & 31u, gz = (b.w >> 10) & 31u;
+                    match[u] = ((uint32_t)u < nr) & meet & (max(gx, cl[0]) == (gr[u] >> 11)) &
+                               (max(gy, cl[1]) == cy) & (max(gz, cl[2]) == cz);
                 }
 #pragma unroll
                 for (int u = 0; u < kWalkILP; ++u) {
                     const uint64_t mb = __ballot(match[u]);
-                    if (match[u]) wq[wn + (int)lanes_below(mb)] = ((uint32_t)jc << 16) | (uint32_t)gi[u];
+                    if (match[u]) wq[wn + (int)lanes_below(mb)] = ((uint32_t)jc << 16) | (gr[u] & 0x7FFu);
                     wn += __popcll(mb);
                     if (wn >= PZ_WAVE) {
                         __builtin_amdgcn_wave_barrier();
@@ -813,7 +1014,7 @@ __global__ __launch_bounds__(kRunThreads, 4) void k_pair_sieve_runs(
                 }
                 if (__ballot(live || !exhausted) == 0) break;
             }
-            if (wn > 0 && !*(volatile int*)&s_hit) {
+            if (wn > 0 && !__hip_atomic_load(&s_hit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
                 __builtin_amdgcn_wave_barrier();
                 test_pairs(wn);
             }
@@ -828,7 +1029,10 @@ __global__ __launch_bounds__(kRunThreads, 4) void k_pair_sieve_runs(
                 if (s_hit) found[p] = 1u;
             }
         }
-        __syncthreads();
+        if (ncand > 0)
+            item = item_next;  // the walk closed with a barrier
+        else
+            next_item();
     }
     flush_stats(st, stats);
     if (tid == 0 && n_sieved) {
@@ -1004,9 +1208,9 @@ __global__ __launch_bounds__(256) void k_escalate_tris(const uint4* __restrict__
 // gather sieve; the others are deferred to the trees.  ctr = (vertex-run
 // pairs, gather pairs, and each list's next-item counter).
 __global__ __launch_bounds__(256) void k_sieve_classify(const uint64_t* __restrict__ pairs, uint64_t np, SieveSet A,
-                                                       SieveSet B, int runs, uint32_t* __restrict__ work,
-                                                       uint32_t* __restrict__ gwork, uint32_t* __restrict__ ctr,
-                                                       uint32_t* __restrict__ defer) {
+                                                       SieveSet B, const uint8_t* __restrict__ meta, int runs,
+                                                       RunDesc* __restrict__ rdesc, uint32_t* __restrict__ gwork,
+                                                       uint32_t* __restrict__ ctr, uint32_t* __restrict__ defer) {
     const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     int cls = -1;  // 0 vertex-run sieve, 1 gather sieve, 2 trees
     if (p < np) {
@@ -1027,7 +1231,15 @@ __global__ __launch_bounds__(256) void k_sieve_classify(const uint64_t* __restri
         uint32_t base = 0;
         if (__lane_id() == leader) base = atomicAdd(&ctr[c], (uint32_t)__popcll(bal));
         base = (uint32_t)__shfl((int)base, leader, PZ_WAVE);
-        if (cls == c) (c == 0 ? work : gwork)[base + lanes_below(bal)] = (uint32_t)p;
+        if (cls == c) {
+            const uint32_t slot = base + lanes_below(bal);
+            if (c == 0) {
+                const uint64_t key = pairs[p];
+                rdesc[slot] = make_run_desc(A, B, (uint32_t)p, (int32_t)(key >> 32), (int32_t)(uint32_t)key, meta[p]);
+            } else {
+                gwork[slot] = (uint32_t)p;
+            }
+        }
     }
 }
 
@@ -1040,7 +1252,7 @@ __global__ void k_fill_ctr(uint32_t* ctr) {
 namespace pzk {
 
 void pair_sieve(const uint64_t* pairs, uint64_t np, const SieveSet& A, const SieveSet& B, const uint8_t* meta,
-                bool f32, int allow_f32, uint32_t* found, uint32_t* defer, uint32_t* work, uint32_t* gwork,
+                bool f32, int allow_f32, uint32_t* found, uint32_t* defer, uint4* run_desc, uint32_t* gwork,
                 uint32_t* ctr, uint2* rays, unsigned long long* nrays, uint64_t rays_cap, uint32_t* ray_overflow,
                 uint4* esc, unsigned long long* esc_count, uint64_t esc_cap, unsigned blocks,
                 unsigned long long* stats, hipStream_t s) {
@@ -1050,18 +1262,20 @@ void pair_sieve(const uint64_t* pairs, uint64_t np, const SieveSet& A, const Sie
     const bool runs = A.vmin && B.vmin;
     KernelTimer kt_("pair_sieve", s);
     hipLaunchKernelGGL(k_fill_ctr, dim3(1), dim3(64), 0, s, ctr);
+    static_assert(sizeof(RunDesc) == kRunDescWords * sizeof(uint4), "run descriptor layout");
+    RunDesc* rdesc = reinterpret_cast<RunDesc*>(run_desc);
     hipLaunchKernelGGL(k_sieve_classify, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, pairs, np, A, B,
-                       runs ? 1 : 0, work, gwork, ctr, defer);
+                       meta, runs ? 1 : 0, rdesc, gwork, ctr, defer);
     // persistent grids over the two work lists: pairs of objects with vertex
     // runs first; what they hand on, and pairs without runs, next
     const unsigned gr = (unsigned)std::min<uint64_t>(np, 2ull * blocks);
     if (runs) {
         if (fp32)
-            hipLaunchKernelGGL(k_pair_sieve_runs<true>, dim3(gr), dim3(kRunThreads), 0, s, pairs, work, ctr, A, B,
-                               meta, found, defer, gwork, rl, esc, esc_count, esc_cap, stats);
+            hipLaunchKernelGGL(k_pair_sieve_runs<true>, dim3(gr), dim3(kRunThreads), 0, s, rdesc, ctr, A, B, found,
+                               defer, gwork, rl, esc, esc_count, esc_cap, stats);
         else
-            hipLaunchKernelGGL(k_pair_sieve_runs<false>, dim3(gr), dim3(kRunThreads), 0, s, pairs, work, ctr, A, B,
-                               meta, found, defer, gwork, rl, esc, esc_count, esc_cap, stats);
+            hipLaunchKernelGGL(k_pair_sieve_runs<false>, dim3(gr), dim3(kRunThreads), 0, s, rdesc, ctr, A, B, found,
+                               defer, gwork, rl, esc, esc_count, esc_cap, stats);
     }
     const unsigned gg = (unsigned)std::min<uint64_t>(np, 4ull * blocks);
     if (fp32)
