@@ -251,8 +251,11 @@ typedef struct {
                                    other pairs through per-object trees as in 2; 1 = per-object
                                    trees, per-edge kernel only; 2 = per-object trees, dual-tree walk
                                    with per-edge fallback */
-    int32_t  broad_mode;        /* two-level: 0 = auto (box scan for <= 4096 mesh2 objects), 1 = object
-                                   LBVH walk, 2 = box scan (every mesh1 box against every mesh2 box) */
+    int32_t  broad_mode;        /* two-level: 0 = auto (box scan for <= 4096 mesh2 objects, else the
+                                   hashed grid for a single join and the object LBVH walk for a
+                                   batch), 1 = object LBVH walk, 2 = box scan (every mesh1 box
+                                   against every mesh2 box), 3 = hashed grid of the mesh2 boxes
+                                   (single joins only; pz_join_batch rejects it) */
     int32_t  reserved[6];
 } pz_join_desc;
 

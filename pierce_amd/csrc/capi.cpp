@@ -192,6 +192,26 @@ ObjBvh build_object_bvh(pz_ctx* c, const pz_dataset* d, const SetPrep& p, const 
     return b;
 }
 
+// Hashed grid of one set's object boxes for the broad phase of a single join:
+// twice as many buckets as objects (a power of two).
+pzk::ObjGrid build_object_grid(pz_ctx* c, const pz_dataset* d, const SetPrep& p) {
+    const int32_t n = std::max<int32_t>(d->nobj, 0);
+    pzk::ObjGrid g;
+    uint32_t nb = 1024;
+    while (nb < 2ull * (uint64_t)n && nb < (1u << 30)) nb <<= 1;
+    g.nb = nb;
+    g.counts = buf<uint32_t>(c, "grid.counts", (size_t)nb + 1);
+    g.start = buf<uint32_t>(c, "grid.start", (size_t)nb + 1);
+    g.entries = buf<uint4>(c, "grid.entries", pzk::grid_entry_bound(n));
+    g.ebox = buf<float4>(c, "grid.ebox", 2 * pzk::grid_entry_bound(n));
+    g.big = buf<uint32_t>(c, "grid.big", (size_t)std::max(n, 1));
+    g.nbig = buf<uint32_t>(c, "grid.nbig", 1);
+    g.stats = buf<float>(c, "grid.stats", 4);
+    void* tmp = buf<char>(c, "scan.tmp.grid", pzk::scan_temp_bytes((uint64_t)nb + 1));
+    pzk::grid_build(p.obox, n, g, tmp, c->s);
+    return g;
+}
+
 // Per-object LBVHs and grouped edges of the needed objects of one set.
 struct Needed {
     uint32_t* need;
@@ -243,34 +263,44 @@ void set_counters(pz_join_result* r, const std::vector<unsigned long long>& st) 
 }
 
 // Final stage shared by both strategies: sort unique keys (+types), copy to host.
+// The result's host arrays from n pairs (and, with types, n u32 pair types)
+// that have landed in pinned memory.
+void land_pairs(pz_join_result* r, const char* h_pairs, const uint32_t* h_types, uint64_t n, bool want_types) {
+    r->num_pairs = n;
+    r->pairs = static_cast<pz_pair*>(std::malloc(std::max<uint64_t>(n, 1) * sizeof(pz_pair)));
+    if (!r->pairs) throw PzError(PZ_E_NOMEM, "host result allocation failed");
+    if (n) std::memcpy(r->pairs, h_pairs, n * sizeof(pz_pair));
+    if (want_types) {
+        r->pair_types = static_cast<uint8_t*>(std::malloc(std::max<uint64_t>(n, 1)));
+        if (!r->pair_types) throw PzError(PZ_E_NOMEM, "host result allocation failed");
+        for (uint64_t i = 0; i < n; ++i) r->pair_types[i] = static_cast<uint8_t>(h_types[i]);
+    }
+}
+
 void finish_pairs(pz_ctx* c, uint64_t* keys, uint32_t* types, uint64_t n, int32_t nA, bool want_types,
                   pz_join_result* r) {
     uint64_t* keys2 = buf<uint64_t>(c, "fin.keys2", n + 1);
     uint32_t* types2 = buf<uint32_t>(c, "fin.types2", n + 1);
-    void* tmp = buf<char>(c, "sort.tmp.fin", pzk::radix_sort_temp_bytes(n));
-    int bits = 32 + bits_for((uint64_t)std::max<int32_t>(nA, 1));
-    bool alt = types ? pzk::radix_sort_pairs(keys, types, keys2, types2, n, 0, bits, tmp, c->s)
-                     : pzk::radix_sort_keys(keys, keys2, n, 0, bits, tmp, c->s);
+    bool alt;
+    if (n > pzk::kSortTileKeys && n <= pzk::kSortTilesMax) {  // sorted tiles merged by rank: two launches
+        alt = pzk::sort_tiles(keys, types, keys2, types2, n, c->s);
+    } else {
+        void* tmp = buf<char>(c, "sort.tmp.fin", pzk::radix_sort_temp_bytes(n));
+        const int bits = 32 + bits_for((uint64_t)std::max<int32_t>(nA, 1));
+        alt = types ? pzk::radix_sort_pairs(keys, types, keys2, types2, n, 0, bits, tmp, c->s)
+                    : pzk::radix_sort_keys(keys, keys2, n, 0, bits, tmp, c->s);
+    }
     uint64_t* sk = alt ? keys2 : keys;
     uint32_t* st = alt ? types2 : types;
     int32_t* dpairs = buf<int32_t>(c, "fin.pairs", 2 * n + 2);
     pzk::keys_to_pairs(sk, n, dpairs, c->s);
-    r->num_pairs = n;
-    r->pairs = static_cast<pz_pair*>(std::malloc(std::max<uint64_t>(n, 1) * sizeof(pz_pair)));
-    if (!r->pairs) throw PzError(PZ_E_NOMEM, "host result allocation failed");
     // through the pinned landing buffer: device-to-pinned copies, then host copies
     const size_t pb = n * sizeof(pz_pair), tb = want_types ? n * 4 : 0;
     char* h = static_cast<char*>(pinned(c, std::max<size_t>(pb + tb, 8)));
     if (n) HIP_OK(hipMemcpyAsync(h, dpairs, pb, hipMemcpyDeviceToHost, c->s));
     if (want_types && n) HIP_OK(hipMemcpyAsync(h + pb, st, tb, hipMemcpyDeviceToHost, c->s));
     HIP_OK(hipStreamSynchronize(c->s));
-    if (n) std::memcpy(r->pairs, h, pb);
-    if (want_types) {
-        r->pair_types = static_cast<uint8_t*>(std::malloc(std::max<uint64_t>(n, 1)));
-        if (!r->pair_types) throw PzError(PZ_E_NOMEM, "host result allocation failed");
-        const uint32_t* ht = reinterpret_cast<const uint32_t*>(h + pb);
-        for (uint64_t i = 0; i < n; ++i) r->pair_types[i] = static_cast<uint8_t>(ht[i]);
-    }
+    land_pairs(r, h, reinterpret_cast<const uint32_t*>(h + pb), n, want_types);
 }
 
 // ------------------------------------------------------------- estimates
@@ -410,7 +440,7 @@ SegOut run_segment(pz_ctx* c, uint64_t k1, uint64_t k2, int tag, pz_join_result*
     return o;
 }
 
-enum { kSegPrep = 1, kSegObjBvh = 2, kSegLayout = 3 };
+enum { kSegPrep = 1, kSegObjBvh = 2, kSegLayout = 3, kSegObjGrid = 4 };
 // Largest mesh2 object count whose broad phase tests every box instead of
 // walking an object LBVH (a walk costs tens of microseconds of dependent
 // loads whatever the set size; a full scan of this many boxes costs less).
@@ -441,11 +471,21 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
     if (d.sizing == PZ_SIZING_LEARNED) npred = mlp_predict_async(c, d.model, A, B, pa, pb, meta_of_part, pair_meta_of(d));
 
     // Few mesh2 objects: the broad phase tests every box (k_broad_brute) and
-    // no object index is built; otherwise mesh1 boxes walk an object LBVH.
+    // no object index is built; otherwise a single join hashes the mesh2
+    // boxes into a grid of their mean extent, and a batched join (whose parts
+    // keep to themselves) walks an object LBVH with one subtree per part.
     const bool brute = d.broad_mode == 2 || (d.broad_mode == 0 && nB <= kBruteMaxObjects);
+    const bool grid = !brute && !batched && (d.broad_mode == 0 || d.broad_mode == 3);
     ph.begin("Build Object Index");
     ObjBvh ob;
-    if (!brute)
+    pzk::ObjGrid og{};
+    if (grid)
+        og = run_segment(c, B->uid, 0, kSegObjGrid, r, [&]() {
+                 SegOut o;
+                 o.og = build_object_grid(c, B, pb);
+                 return o;
+             }).og;
+    else if (!brute)
         ob = run_segment(c, B->uid, 0, kSegObjBvh, r, [&]() {
                  SegOut o;
                  o.ob = build_object_bvh(c, B, pb, "B");
@@ -454,7 +494,7 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
 
     ph.begin("Broad Phase");
     // The broad-phase walk keeps a 64-entry stack: the object LBVH must be shallower.
-    const uint32_t oheight = brute ? 0u : readback<uint32_t>(c, {ob.height})[0];
+    const uint32_t oheight = (brute || grid) ? 0u : readback<uint32_t>(c, {ob.height})[0];
     double learned_upper = 0;  // LEARNED: calibrated upper bound of the candidate-pair count
     if (d.sizing == PZ_SIZING_LEARNED) {
         std::vector<SizePrediction> pr = mlp_predict_wait(c, d.model, npred);
@@ -476,13 +516,19 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         uint32_t* cnt = buf<uint32_t>(c, "bp.cnt", (size_t)std::max(nA, 1));
         uint64_t* off = buf<uint64_t>(c, "bp.off", (size_t)std::max(nA, 1) + 1);
         void* tmp = buf<char>(c, "scan.tmp", pzk::scan_temp_bytes((uint64_t)std::max(nA, 1)));
-        pzk::broad_count(pa.obox, nA, pb.obox, ob.nodes, ob.order, nB, batched ? A->part_of : nullptr,
-                         batched ? B->part_off : nullptr, cnt, s);
+        if (grid)
+            pzk::grid_count(pa.obox, nA, pb.obox, nB, og, cnt, s);
+        else
+            pzk::broad_count(pa.obox, nA, pb.obox, ob.nodes, ob.order, nB, batched ? A->part_of : nullptr,
+                             batched ? B->part_off : nullptr, cnt, s);
         pzk::exclusive_scan_u32_u64(cnt, off, (uint64_t)std::max(nA, 0), tmp, s);
         np = readback<uint64_t>(c, {off + std::max(nA, 0)})[0];
         pairs = buf<uint64_t>(c, "bp.pairs", np + 1);
-        pzk::broad_write(pa.obox, nA, pb.obox, ob.nodes, ob.order, nB, batched ? A->part_of : nullptr,
-                         batched ? B->part_off : nullptr, off, pairs, s);
+        if (grid)
+            pzk::grid_write(pa.obox, nA, pb.obox, nB, og, off, pairs, s);
+        else
+            pzk::broad_write(pa.obox, nA, pb.obox, ob.nodes, ob.order, nB, batched ? A->part_of : nullptr,
+                             batched ? B->part_off : nullptr, off, pairs, s);
     };
     if (d.sizing == PZ_SIZING_TWO_PASS) {
         two_pass();
@@ -508,8 +554,11 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         pairs = buf<uint64_t>(c, "bp.pairs", cap + 1);
         unsigned long long* cursor = buf<unsigned long long>(c, "bp.cursor", 1);
         pzk::fill_ull(cursor, 0ull, 1, s);
-        pzk::broad_append(pa.obox, nA, pb.obox, ob.nodes, ob.order, nB, batched ? A->part_of : nullptr,
-                          batched ? B->part_off : nullptr, pairs, cursor, cap, s);
+        if (grid)
+            pzk::grid_append(pa.obox, nA, pb.obox, nB, og, pairs, cursor, cap, s);
+        else
+            pzk::broad_append(pa.obox, nA, pb.obox, ob.nodes, ob.order, nB, batched ? A->part_of : nullptr,
+                              batched ? B->part_off : nullptr, pairs, cursor, cap, s);
         np = readback<unsigned long long>(c, {cursor})[0];
         if (np > cap) {
             r->overflowed = 1;
@@ -809,6 +858,33 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         }
     };
 
+    // Result pairs: the candidates that pass their predicate, as keys (and pair
+    // types), then sorted and copied to the host.
+    uint64_t* okeys = buf<uint64_t>(c, "sel.keys", np + 1);
+    uint32_t* otype = buf<uint32_t>(c, "sel.type", np + 1);
+    unsigned long long* ocnt = buf<unsigned long long>(c, "sel.cnt", 1);
+    const bool want_types = batched || (d.predicate == PZ_PRED_CONTAINMENT && d.include_overlap_pairs);
+    auto select = [&]() {
+        pzk::fill_ull(ocnt, 0ull, 1, s);
+        // batches keep overlap pairs of containment queries (typed) and filter per query on the host
+        pzk::select_pairs(pairs, np, found, inside, meta, batched ? 1 : d.include_overlap_pairs, okeys, otype, ocnt,
+                          stats + pzk::kStatTouching, s);
+    };
+    constexpr uint64_t kSpec = pzk::kSortTileKeys;           // result pairs of the speculative download
+    constexpr size_t kHead = (2 + pzk::kStatSlots) * 8;      // counts and counters at the front of the landing area
+    // One read-back into h: [0] *deferred (0 without), [1] result pairs, [2..] the counters.
+    auto gather_counts = [&](const unsigned long long* deferred, void* h) {
+        pzk::HostGather g{{reinterpret_cast<const uint32_t*>(deferred ? deferred : ocnt),
+                           reinterpret_cast<const uint32_t*>(ocnt), reinterpret_cast<const uint32_t*>(stats)},
+                          {2, 2, 2 * pzk::kStatSlots},
+                          3};
+        pzk::gather_to_host(g, static_cast<uint32_t*>(h), s);
+        HIP_OK(hipStreamSynchronize(s));
+        unsigned long long* hs = static_cast<unsigned long long*>(h);
+        if (!deferred) hs[0] = 0;
+        return static_cast<const unsigned long long*>(hs);
+    };
+
     // Pairs the tree path decides: all of them, or the ones the pair sieve defers.
     const uint64_t* tpairs = pairs;
     uint64_t ntp = np;
@@ -854,13 +930,38 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         uint64_t* dp = buf<uint64_t>(c, "sv.pairs", np + 1);
         tindex = buf<uint32_t>(c, "sv.index", np + 1);
         pzk::compact_flagged(pairs, np, defer, dp, tindex, sctr + 1, s);
-        ntp = readback<unsigned long long>(c, {sctr + 1})[0];
-        r->sieve_pairs = np - ntp;
         tpairs = dp;
-        if (ntp > 0) {
-            tmeta = buf<uint8_t>(c, "sv.meta", ntp + 1);
-            pzk::pair_meta(dp, ntp, batched ? A->part_of : nullptr, meta_of_part, pz::pair_meta_of(d), tmeta, s);
+        // Download as if the sieve deferred nothing (the usual case), reading
+        // back the deferred count with the results: up to one sort tile of
+        // result pairs needs no further round trip.
+        ph.begin("Download Results");
+        select();
+        pzk::sort_tile_counted(okeys, want_types ? otype : nullptr, ocnt, s);
+        int32_t* dpairs = buf<int32_t>(c, "spec.pairs", 2 * kSpec);
+        pzk::keys_to_pairs_counted(okeys, ocnt, kSpec, dpairs, s);
+        char* hp = static_cast<char*>(pinned(c, kHead + kSpec * (sizeof(pz_pair) + 4)));
+        const uint64_t m = std::min<uint64_t>(kSpec, np);  // no more result pairs than candidates
+        HIP_OK(hipMemcpyAsync(hp + kHead, dpairs, m * sizeof(pz_pair), hipMemcpyDeviceToHost, s));
+        if (want_types)
+            HIP_OK(hipMemcpyAsync(hp + kHead + kSpec * sizeof(pz_pair), otype, m * 4, hipMemcpyDeviceToHost, s));
+        const unsigned long long* hs = gather_counts(sctr + 1, hp);
+        ntp = hs[0];
+        r->sieve_pairs = np - ntp;
+        if (ntp == 0) {
+            const uint64_t nout = hs[1];
+            const std::vector<unsigned long long> st(hs + 2, hs + 2 + pzk::kStatSlots);
+            if (nout <= kSpec)
+                land_pairs(r, hp + kHead, reinterpret_cast<const uint32_t*>(hp + kHead + kSpec * sizeof(pz_pair)),
+                           nout, want_types);
+            else  // more result pairs than one tile: sort them with the count now known
+                finish_pairs(c, okeys, want_types ? otype : nullptr, nout, nA, want_types, r);
+            set_counters(r, st);
+            return;
         }
+        // the sieve deferred pairs to the trees: they are decided, then the results selected again
+        pzk::fill_ull(stats + pzk::kStatTouching, 0ull, 1, s);
+        tmeta = buf<uint8_t>(c, "sv.meta", ntp + 1);
+        pzk::pair_meta(dp, ntp, batched ? A->part_of : nullptr, meta_of_part, pz::pair_meta_of(d), tmeta, s);
     }
     if (ntp > 0) {
         uint32_t* tfound = found;
@@ -881,24 +982,10 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
     }
 
     ph.begin("Download Results");
-    uint64_t* okeys = buf<uint64_t>(c, "sel.keys", np + 1);
-    uint32_t* otype = buf<uint32_t>(c, "sel.type", np + 1);
-    unsigned long long* ocnt = buf<unsigned long long>(c, "sel.cnt", 1);
-    pzk::fill_ull(ocnt, 0ull, 1, s);
-    // batches keep overlap pairs of containment queries (typed) and filter per query on the host
-    pzk::select_pairs(pairs, np, found, inside, meta, batched ? 1 : d.include_overlap_pairs, okeys, otype, ocnt,
-                      stats + pzk::kStatTouching, s);
-    // the result count and the counters in one read-back
-    unsigned long long* hs = static_cast<unsigned long long*>(pinned(c, (1 + pzk::kStatSlots) * 8));
-    pzk::HostGather gs{{reinterpret_cast<const uint32_t*>(ocnt), reinterpret_cast<const uint32_t*>(stats)},
-                       {2, 2 * pzk::kStatSlots},
-                       2};
-    pzk::gather_to_host(gs, reinterpret_cast<uint32_t*>(hs), s);
-    HIP_OK(hipStreamSynchronize(s));
-    const std::vector<unsigned long long> st(hs + 1, hs + 1 + pzk::kStatSlots);
-    const uint64_t nout = hs[0];
-    const bool want_types = batched || (d.predicate == PZ_PRED_CONTAINMENT && d.include_overlap_pairs);
-    finish_pairs(c, okeys, want_types ? otype : nullptr, nout, nA, want_types, r);
+    select();
+    const unsigned long long* hs = gather_counts(nullptr, pinned(c, kHead));
+    const std::vector<unsigned long long> st(hs + 2, hs + 2 + pzk::kStatSlots);
+    finish_pairs(c, okeys, want_types ? otype : nullptr, hs[1], nA, want_types, r);
     set_counters(r, st);
 }
 
@@ -1413,7 +1500,7 @@ pz_status pz_join(pz_ctx* c, const pz_dataset* a, const pz_dataset* b, const pz_
     if (d.load_factor < 0 || d.load_factor > 1) return fail(PZ_E_INVALID, "load factor must be in (0, 1]");
     if (d.sizing == PZ_SIZING_LEARNED && !d.model) return fail(PZ_E_INVALID, "learned sizing needs a model");
     if (d.keep_device_result) return fail(PZ_E_INVALID, "keep_device_result is reserved");
-    if (d.broad_mode < 0 || d.broad_mode > 2) return fail(PZ_E_INVALID, "unknown broad-phase mode");
+    if (d.broad_mode < 0 || d.broad_mode > 3) return fail(PZ_E_INVALID, "unknown broad-phase mode");
     init_result(r);
     return guarded([&]() -> pz_status {
         HIP_OK(hipSetDevice(c->device));

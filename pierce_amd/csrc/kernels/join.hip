@@ -437,6 +437,241 @@ __global__ __launch_bounds__(kBlock) void k_broad(const uint32_t* __restrict__ o
     }
 }
 
+// ------------------------------------------------ broad phase over a hashed grid
+// Cells are the mesh2 objects' mean box extent per axis, on an unbounded
+// integer lattice hashed into nb (a power of two) buckets, so the grid needs
+// no scene box and spreads dense clusters as well as sparse sets.  A mesh2
+// object is referenced in every cell its box overlaps (bucket of the cell,
+// entry = object and cell), or, when that is more than kGridSpan cells, kept
+// in a list every query tests.  A mesh1 object visits the cells of its box and
+// accepts a referenced object once: in the cell of the larger of the two
+// boxes' lower corners (cells are a monotone function of the coordinate, so
+// two meeting boxes share that cell); entries of other cells that share the
+// bucket are skipped by their cell.
+constexpr uint32_t kGridSpan = 27;    // cells a referenced mesh2 box may overlap
+constexpr uint64_t kQuerySpan = 4096; // cells beyond which a mesh1 box tests every mesh2 box
+constexpr int32_t kCellClamp = 1 << 20;
+
+__device__ __forceinline__ uint32_t cell_hash(int32_t x, int32_t y, int32_t z, uint32_t nb) {
+    return (((uint32_t)x * 73856093u) ^ ((uint32_t)y * 19349663u) ^ ((uint32_t)z * 83492791u)) & (nb - 1u);
+}
+
+struct GridCells {
+    float inv[3];
+    __device__ int32_t cell(float x, int k) const {
+        const float f = floorf(x * inv[k]);
+        return (int32_t)fminf(fmaxf(f, (float)-kCellClamp), (float)kCellClamp);
+    }
+};
+
+// stats[0..2]: per-axis sums of the extents of the non-empty mesh2 boxes, stats[3]: their number.
+__device__ __forceinline__ GridCells grid_cells(const float* __restrict__ stats) {
+    GridCells g;
+    const float n = fmaxf(stats[3], 1.f);
+    float m[3], mx = 0.f;
+    for (int k = 0; k < 3; ++k) {
+        m[k] = stats[k] / n;
+        mx = fmaxf(mx, m[k]);
+    }
+    for (int k = 0; k < 3; ++k) {
+        // a flat axis takes the largest mean extent (or 1 when every box is a point)
+        const float h = m[k] > 0.f ? m[k] : (mx > 0.f ? mx : 1.f);
+        g.inv[k] = 1.f / h;
+    }
+    return g;
+}
+
+__device__ __forceinline__ bool load_box(const uint32_t* __restrict__ obox, int32_t o, float* lo, float* hi) {
+    bool ok = true;
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = ord2f(obox[6ll * o + k]);
+        hi[k] = ord2f(obox[6ll * o + 3 + k]);
+        ok &= lo[k] <= hi[k];
+    }
+    return ok;
+}
+
+// Sums of the non-empty mesh2 boxes' extents (a grid-stride loop, wave and
+// block reductions, four atomics per block).
+__global__ __launch_bounds__(kBlock) void k_grid_stats(const uint32_t* __restrict__ obox, int32_t n,
+                                                       float* __restrict__ stats) {
+    __shared__ float part[kBlock / PZ_WAVE][4];
+    float v[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int32_t o = blockIdx.x * kBlock + threadIdx.x; o < n; o += gridDim.x * kBlock) {
+        float lo[3], hi[3];
+        if (load_box(obox, o, lo, hi)) {
+            for (int k = 0; k < 3; ++k) v[k] += hi[k] - lo[k];
+            v[3] += 1.f;
+        }
+    }
+    const int w = threadIdx.x / PZ_WAVE;
+    for (int k = 0; k < 4; ++k) {
+        const float t = wave_sum(v[k]);
+        if (__lane_id() == 0) part[w][k] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        float t = 0.f;
+        for (int i = 0; i < kBlock / PZ_WAVE; ++i) t += part[i][threadIdx.x];
+        atomicAdd(&stats[threadIdx.x], t);
+    }
+}
+
+// Mode 0: count references per bucket (and list the oversized boxes); mode 1:
+// place each reference in its bucket's range [start, start + count), counting
+// the bucket down (the counts end at zero).
+template <int Mode>
+__global__ __launch_bounds__(kBlock) void k_grid_refs(const uint32_t* __restrict__ obox, int32_t n,
+                                                      const float* __restrict__ stats, uint32_t nb,
+                                                      uint32_t* __restrict__ counts, const uint32_t* __restrict__ start,
+                                                      uint4* __restrict__ entries, float4* __restrict__ ebox,
+                                                      uint32_t* __restrict__ big, uint32_t* __restrict__ nbig) {
+    const int32_t o = blockIdx.x * kBlock + threadIdx.x;
+    if (o >= n) return;
+    float lo[3], hi[3];
+    if (!load_box(obox, o, lo, hi)) return;
+    const GridCells g = grid_cells(stats);
+    int32_t cl[3], ch[3];
+    uint64_t span = 1;
+    for (int k = 0; k < 3; ++k) {
+        cl[k] = g.cell(lo[k], k);
+        ch[k] = g.cell(hi[k], k);
+        span *= (uint64_t)(ch[k] - cl[k] + 1);
+    }
+    if (span > kGridSpan) {
+        if (Mode == 0) big[atomicAdd(nbig, 1u)] = (uint32_t)o;
+        return;
+    }
+    for (int32_t z = cl[2]; z <= ch[2]; ++z)
+        for (int32_t y = cl[1]; y <= ch[1]; ++y)
+            for (int32_t x = cl[0]; x <= ch[0]; ++x) {
+                const uint32_t b = cell_hash(x, y, z, nb);
+                if (Mode == 0) {
+                    atomicAdd(&counts[b], 1u);
+                } else {
+                    const uint32_t e = start[b] + atomicSub(&counts[b], 1u) - 1u;
+                    entries[e] = make_uint4((uint32_t)o, (uint32_t)x, (uint32_t)y, (uint32_t)z);
+                    ebox[2ull * e] = make_float4(lo[0], lo[1], lo[2], hi[0]);
+                    ebox[2ull * e + 1] = make_float4(hi[1], hi[2], 0.f, 0.f);
+                }
+            }
+}
+
+// kGridLanes lanes per mesh1 object: the mesh2 objects whose boxes meet its
+// box, each once (modes as k_broad: count, write at off[a], append with
+// capacity).  The lanes share the object's cells and the oversized list; each
+// tests the entries of its cells' buckets (their boxes stored beside them).
+constexpr int kGridLanes = 8;
+template <int Mode>
+__global__ __launch_bounds__(kBlock) void k_grid_broad(const uint32_t* __restrict__ oboxA, int32_t nA,
+                                                       const uint32_t* __restrict__ oboxB, int32_t nB,
+                                                       const float* __restrict__ stats, uint32_t nb,
+                                                       const uint32_t* __restrict__ start,
+                                                       const uint4* __restrict__ entries,
+                                                       const float4* __restrict__ ebox,
+                                                       const uint32_t* __restrict__ big,
+                                                       const uint32_t* __restrict__ nbig, uint32_t* __restrict__ cnt,
+                                                       const uint64_t* __restrict__ off, uint64_t* __restrict__ out,
+                                                       unsigned long long* __restrict__ cursor, uint64_t cap) {
+    const uint64_t gid = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int32_t a = (int32_t)(gid / kGridLanes);
+    const int sub = (int)(gid % kGridLanes), lane = __lane_id();
+    if (Mode != 2 && (int64_t)(gid - sub) >= (int64_t)nA * kGridLanes) return;  // whole groups only
+    float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {-1.f, -1.f, -1.f};
+    const bool active = a < nA && load_box(oboxA, a, lo, hi);
+    const GridCells g = grid_cells(stats);
+    int32_t cl[3] = {0, 0, 0}, ch[3] = {-1, -1, -1};
+    uint64_t span = 0;
+    if (active) {
+        span = 1;
+        for (int k = 0; k < 3; ++k) {
+            cl[k] = g.cell(lo[k], k);
+            ch[k] = g.cell(hi[k], k);
+            span *= (uint64_t)(ch[k] - cl[k] + 1);
+        }
+    }
+    const int32_t rx = ch[0] - cl[0] + 1, ry = ch[1] - cl[1] + 1;
+    auto box_meets = [&](const float* blo, const float* bhi) {
+        return blo[0] <= hi[0] && bhi[0] >= lo[0] && blo[1] <= hi[1] && bhi[1] >= lo[1] && blo[2] <= hi[2] &&
+               bhi[2] >= lo[2];
+    };
+    auto visit = [&](auto&& emit) {
+        if (!active) return;
+        float blo[3], bhi[3];
+        if (span > kQuerySpan) {  // a box over very many cells: test every referenced mesh2 box
+            for (int32_t b = sub; b < nB; b += kGridLanes) {
+                if (!load_box(oboxB, b, blo, bhi) || !box_meets(blo, bhi)) continue;
+                uint64_t sb = 1;
+                for (int k = 0; k < 3; ++k) sb *= (uint64_t)(g.cell(bhi[k], k) - g.cell(blo[k], k) + 1);
+                if (sb <= kGridSpan) emit(b);  // the oversized ones follow below
+            }
+        } else {
+            for (uint32_t i = (uint32_t)sub; i < (uint32_t)span; i += kGridLanes) {
+                const int32_t x = cl[0] + (int32_t)(i % (uint32_t)rx);
+                const int32_t y = cl[1] + (int32_t)((i / (uint32_t)rx) % (uint32_t)ry);
+                const int32_t z = cl[2] + (int32_t)(i / ((uint32_t)rx * (uint32_t)ry));
+                const uint32_t bk = cell_hash(x, y, z, nb);
+                for (uint32_t e = start[bk], e1 = start[bk + 1]; e < e1; ++e) {
+                    const uint4 en = entries[e];
+                    const float4 b0 = ebox[2ull * e], b1 = ebox[2ull * e + 1];
+                    if ((int32_t)en.y != x || (int32_t)en.z != y || (int32_t)en.w != z) continue;
+                    blo[0] = b0.x;
+                    blo[1] = b0.y;
+                    blo[2] = b0.z;
+                    bhi[0] = b0.w;
+                    bhi[1] = b1.x;
+                    bhi[2] = b1.y;
+                    if (!box_meets(blo, bhi)) continue;
+                    // once per pair: in the cell of the larger lower corner
+                    if (max(g.cell(blo[0], 0), cl[0]) != x || max(g.cell(blo[1], 1), cl[1]) != y ||
+                        max(g.cell(blo[2], 2), cl[2]) != z)
+                        continue;
+                    emit((int32_t)en.x);
+                }
+            }
+        }
+        const uint32_t nbg = *nbig;
+        for (uint32_t i = (uint32_t)sub; i < nbg; i += kGridLanes) {
+            const int32_t b = (int32_t)big[i];
+            if (load_box(oboxB, b, blo, bhi) && box_meets(blo, bhi)) emit(b);
+        }
+    };
+    // per lane count, then the group's (count) or the wave's (append) prefix
+    uint32_t n = 0;
+    visit([&](int32_t) { ++n; });
+    uint32_t incl = n;  // inclusive prefix over the wave
+    for (int sft = 1; sft < PZ_WAVE; sft <<= 1) {
+        const uint32_t u = (uint32_t)__shfl_up((int)incl, sft, PZ_WAVE);
+        if (lane >= sft) incl += u;
+    }
+    const int first = lane - sub;  // the group's first lane
+    const uint32_t prev = (uint32_t)__shfl((int)incl, first > 0 ? first - 1 : 0, PZ_WAVE);  // every lane takes part
+    const uint32_t before = first > 0 ? prev : 0u;
+    if (Mode == 0) {
+        const uint32_t group = (uint32_t)__shfl((int)incl, first + kGridLanes - 1, PZ_WAVE) - before;
+        if (sub == 0 && a < nA) cnt[a] = group;
+    } else if (Mode == 1) {
+        if (active && n > 0) {
+            const uint64_t base = off[a] + (incl - n - before);
+            uint32_t i = 0;
+            visit([&](int32_t b) { out[base + i++] = pack_pair(a, b); });
+        }
+    } else {
+        const uint32_t total = (uint32_t)__shfl((int)incl, PZ_WAVE - 1, PZ_WAVE);
+        unsigned long long wbase = 0ull;
+        if (lane == 0 && total > 0) wbase = atomicAdd(cursor, (unsigned long long)total);
+        wbase = __shfl(wbase, 0, PZ_WAVE);
+        const uint64_t base = wbase + incl - n;
+        if (active && n > 0 && base < cap) {
+            uint32_t i = 0;
+            visit([&](int32_t b) {
+                if (base + i < cap) out[base + i] = pack_pair(a, b);
+                ++i;
+            });
+        }
+    }
+}
+
 // Broad phase without an object index, for small mesh2 sets: each mesh1
 // object tests the box of every mesh2 object of its part (the part's objects
 // have ids [boff[part], boff[part + 1])).  Same modes and output as k_broad,
@@ -1676,6 +1911,53 @@ void broad_append(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, cons
 #undef PZ_BRUTE
 #undef PZ_BRUTE_P
 #undef PZ_BRUTE_S
+
+uint64_t grid_entry_bound(int32_t nB) { return (uint64_t)std::max(nB, 1) * kGridSpan; }
+
+void grid_build(const uint32_t* oboxB, int32_t nB, const ObjGrid& g, void* scan_tmp, hipStream_t s) {
+    KernelTimer kt_("grid_build", s);
+    hipLaunchKernelGGL((k_fill_any<uint32_t>), dim3(grid1((uint64_t)g.nb + 1)), dim3(kBlock), 0, s, g.counts, 0u,
+                       (uint64_t)g.nb + 1);
+    hipLaunchKernelGGL((k_fill_any<float>), dim3(1), dim3(64), 0, s, g.stats, 0.f, (uint64_t)4);
+    hipLaunchKernelGGL((k_fill_any<uint32_t>), dim3(1), dim3(64), 0, s, g.nbig, 0u, (uint64_t)1);
+    if (nB <= 0) return;
+    hipLaunchKernelGGL(k_grid_stats, dim3(std::min(grid1((uint64_t)nB), kSceneBlocks)), dim3(kBlock), 0, s, oboxB, nB,
+                       g.stats);
+    hipLaunchKernelGGL((k_grid_refs<0>), dim3(grid1((uint64_t)nB)), dim3(kBlock), 0, s, oboxB, nB,
+                       (const float*)g.stats, g.nb, g.counts, (const uint32_t*)nullptr, (uint4*)nullptr,
+                       (float4*)nullptr, g.big, g.nbig);
+    exclusive_scan_u32_u32(g.counts, g.start, (uint64_t)g.nb + 1, scan_tmp, s);  // start[nb] = every reference
+    hipLaunchKernelGGL((k_grid_refs<1>), dim3(grid1((uint64_t)nB)), dim3(kBlock), 0, s, oboxB, nB,
+                       (const float*)g.stats, g.nb, g.counts, (const uint32_t*)g.start, g.entries, g.ebox, g.big,
+                       g.nbig);
+}
+void grid_count(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, int32_t nB, const ObjGrid& g, uint32_t* cnt,
+                hipStream_t s) {
+    if (nA > 0)
+        hipLaunchKernelGGL((k_grid_broad<0>), dim3(grid1((uint64_t)nA * kGridLanes)), dim3(kBlock), 0, s, oboxA, nA,
+                           oboxB, nB, (const float*)g.stats, g.nb, (const uint32_t*)g.start, (const uint4*)g.entries,
+                           (const float4*)g.ebox,
+                           (const uint32_t*)g.big, (const uint32_t*)g.nbig, cnt, (const uint64_t*)nullptr,
+                           (uint64_t*)nullptr, (unsigned long long*)nullptr, (uint64_t)0);
+}
+void grid_write(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, int32_t nB, const ObjGrid& g,
+                const uint64_t* off, uint64_t* out, hipStream_t s) {
+    if (nA > 0)
+        hipLaunchKernelGGL((k_grid_broad<1>), dim3(grid1((uint64_t)nA * kGridLanes)), dim3(kBlock), 0, s, oboxA, nA,
+                           oboxB, nB, (const float*)g.stats, g.nb, (const uint32_t*)g.start, (const uint4*)g.entries,
+                           (const float4*)g.ebox,
+                           (const uint32_t*)g.big, (const uint32_t*)g.nbig, (uint32_t*)nullptr, off, out,
+                           (unsigned long long*)nullptr, (uint64_t)0);
+}
+void grid_append(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, int32_t nB, const ObjGrid& g, uint64_t* out,
+                 unsigned long long* cursor, uint64_t cap, hipStream_t s) {
+    if (nA > 0)
+        hipLaunchKernelGGL((k_grid_broad<2>), dim3(grid1((uint64_t)nA * kGridLanes)), dim3(kBlock), 0, s, oboxA, nA,
+                           oboxB, nB, (const float*)g.stats, g.nb, (const uint32_t*)g.start, (const uint4*)g.entries,
+                           (const float4*)g.ebox,
+                           (const uint32_t*)g.big, (const uint32_t*)g.nbig, (uint32_t*)nullptr,
+                           (const uint64_t*)nullptr, out, cursor, cap);
+}
 
 void mark_needed(const uint64_t* pairs, uint64_t np, uint32_t* needA, uint32_t* needB, hipStream_t s) {
     if (np) hipLaunchKernelGGL(k_mark_needed, dim3(grid1(np)), dim3(kBlock), 0, s, pairs, np, needA, needB);

@@ -30,6 +30,16 @@ size_t unique_temp_bytes(uint64_t n);
 
 // Unpack (a << 32 | b) keys into pairs {a, b}.
 void keys_to_pairs(const uint64_t* keys, uint64_t n, int32_t* pairs, hipStream_t s);
+// Keys below 2^63 sorted as tiles of 4,096 (one workgroup each) merged by
+// rank: two launches up to kSortTilesMax keys; true when the sorted keys are
+// in keys_alt / vals_alt (more than one tile).  vals may be null.
+constexpr uint64_t kSortTileKeys = 4096, kSortTilesMax = 16 * kSortTileKeys;
+bool sort_tiles(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, uint64_t n, hipStream_t s);
+// One tile sorted in place with the key count read from *dn (nothing done
+// when it exceeds 4,096), and those keys as pairs (nothing when above cap).
+void sort_tile_counted(uint64_t* keys, uint32_t* vals, const unsigned long long* dn, hipStream_t s);
+void keys_to_pairs_counted(const uint64_t* keys, const unsigned long long* dn, uint64_t cap, int32_t* pairs,
+                           hipStream_t s);
 void pairs_to_keys(const int32_t* pairs, uint64_t n, uint64_t* keys, hipStream_t s);
 
 void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);

@@ -145,6 +145,29 @@ void broad_write(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const
 void broad_append(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const Node* nodesB,
                   const uint32_t* orderB, int32_t nB, const uint32_t* part_of_a, const uint32_t* boff, uint64_t* out,
                   unsigned long long* cursor, uint64_t cap, hipStream_t s);
+// Broad phase over a hashed grid of the mesh2 object boxes (non-batched sets):
+// cells are the boxes' mean extent per axis, hashed into nb (a power of two)
+// buckets; boxes over more than 27 cells are kept in `big` and tested by every
+// query.  counts and start hold nb + 1 words, entries grid_entry_bound(nB) (ebox twice that),
+// big nB, nbig 1 word, stats 4 floats.  Same outputs as the broad_* calls.
+struct ObjGrid {
+    uint32_t* counts;
+    uint32_t* start;
+    uint4* entries;
+    float4* ebox;  // two per entry: the box of the entry's object
+    uint32_t* big;
+    uint32_t* nbig;
+    float* stats;
+    uint32_t nb;
+};
+uint64_t grid_entry_bound(int32_t nB);
+void grid_build(const uint32_t* oboxB, int32_t nB, const ObjGrid& g, void* scan_tmp, hipStream_t s);
+void grid_count(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, int32_t nB, const ObjGrid& g, uint32_t* cnt,
+                hipStream_t s);
+void grid_write(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, int32_t nB, const ObjGrid& g,
+                const uint64_t* off, uint64_t* out, hipStream_t s);
+void grid_append(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, int32_t nB, const ObjGrid& g, uint64_t* out,
+                 unsigned long long* cursor, uint64_t cap, hipStream_t s);
 // meta[p] = predicate | dir_mask << 2, from the pair's part or `uniform`.
 void pair_meta(const uint64_t* pairs, uint64_t np, const uint32_t* part_of_a, const uint8_t* meta_of_part,
                uint8_t uniform, uint8_t* meta, hipStream_t s);

@@ -232,6 +232,84 @@ __global__ __launch_bounds__(kSmallSortThreads) void k_sort_small(uint64_t* __re
     }
 }
 
+// Tiles of kSortTile keys, each sorted by one workgroup in LDS (the bitonic
+// network of k_sort_small).  With dn the key count is read from device
+// memory, and a count above cap leaves the keys as they are (the caller then
+// sorts with the count it has read back).
+template <bool HasVals>
+__global__ __launch_bounds__(kSmallSortThreads) void k_sort_tiles(uint64_t* __restrict__ keys,
+                                                                  uint32_t* __restrict__ vals, uint64_t n,
+                                                                  const unsigned long long* __restrict__ dn,
+                                                                  uint64_t cap) {
+    __shared__ uint64_t K[kSortTile];
+    __shared__ uint32_t V[HasVals ? kSortTile : 1];
+    if (dn) {
+        n = *dn;
+        if (n > cap) return;  // workgroup-uniform
+    }
+    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+    if (base >= n) return;  // workgroup-uniform
+    const uint32_t len = (uint32_t)min((uint64_t)kSortTile, n - base);
+    uint32_t npow = 2;
+    while (npow < len) npow <<= 1;
+    for (uint32_t i = threadIdx.x; i < npow; i += kSmallSortThreads) {
+        K[i] = i < len ? keys[base + i] : ~0ull;
+        if (HasVals) V[i] = i < len ? vals[base + i] : 0u;
+    }
+    __syncthreads();
+    for (uint32_t size = 2; size <= npow; size <<= 1)
+        for (uint32_t st = size >> 1; st > 0; st >>= 1) {
+            for (uint32_t i = threadIdx.x; i < npow / 2; i += kSmallSortThreads) {
+                const uint32_t a = 2 * i - (i & (st - 1)), b = a + st;
+                const bool up = (a & size) == 0;
+                const uint64_t x = K[a], y = K[b];
+                if ((x > y) == up) {
+                    K[a] = y;
+                    K[b] = x;
+                    if (HasVals) {
+                        const uint32_t t = V[a];
+                        V[a] = V[b];
+                        V[b] = t;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (uint32_t i = threadIdx.x; i < len; i += kSmallSortThreads) {
+        keys[base + i] = K[i];
+        if (HasVals) vals[base + i] = V[i];
+    }
+}
+
+// Thread per key of sorted tiles: its rank in the whole is its place in its
+// tile plus, in every other tile, the keys below it (tiles before its own also
+// count equal keys, so equal keys keep distinct places).
+template <bool HasVals>
+__global__ __launch_bounds__(256) void k_merge_tiles(const uint64_t* __restrict__ keys,
+                                                     const uint32_t* __restrict__ vals, uint64_t n,
+                                                     uint64_t* __restrict__ okeys, uint32_t* __restrict__ ovals) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t t = i / kSortTile, ntiles = (n + kSortTile - 1) / kSortTile;
+    const uint64_t x = keys[i];
+    uint64_t pos = i - t * kSortTile;
+    for (uint64_t j = 0; j < ntiles; ++j) {
+        if (j == t) continue;
+        uint64_t lo = j * kSortTile, hi = min(n, lo + kSortTile);  // first key of tile j not before x
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) >> 1;
+            const uint64_t y = keys[mid];
+            if (j < t ? y <= x : y < x)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        pos += lo - j * kSortTile;
+    }
+    okeys[pos] = x;
+    if (HasVals) ovals[pos] = vals[i];
+}
+
 template <bool HasVals>
 bool radix_impl(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, uint64_t n, int begin_bit,
                 int end_bit, void* temp, hipStream_t s) {
@@ -392,6 +470,48 @@ bool radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32
 bool radix_sort_keys(uint64_t* keys, uint64_t* keys_alt, uint64_t n, int begin_bit, int end_bit, void* temp,
                      hipStream_t s) {
     return radix_impl<false>(keys, nullptr, keys_alt, nullptr, n, begin_bit, end_bit, temp, s);
+}
+
+bool sort_tiles(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, uint64_t n, hipStream_t s) {
+    if (n <= 1) return false;
+    const uint64_t ntiles = (n + kSortTile - 1) / kSortTile;
+    if (vals)
+        hipLaunchKernelGGL((k_sort_tiles<true>), dim3((unsigned)ntiles), dim3(kSmallSortThreads), 0, s, keys, vals, n,
+                           (const unsigned long long*)nullptr, (uint64_t)0);
+    else
+        hipLaunchKernelGGL((k_sort_tiles<false>), dim3((unsigned)ntiles), dim3(kSmallSortThreads), 0, s, keys, vals,
+                           n, (const unsigned long long*)nullptr, (uint64_t)0);
+    if (ntiles == 1) return false;
+    if (vals)
+        hipLaunchKernelGGL((k_merge_tiles<true>), dim3(grid1(n)), dim3(256), 0, s, keys, vals, n, keys_alt, vals_alt);
+    else
+        hipLaunchKernelGGL((k_merge_tiles<false>), dim3(grid1(n)), dim3(256), 0, s, keys, vals, n, keys_alt,
+                           vals_alt);
+    return true;
+}
+void sort_tile_counted(uint64_t* keys, uint32_t* vals, const unsigned long long* dn, hipStream_t s) {
+    if (vals)
+        hipLaunchKernelGGL((k_sort_tiles<true>), dim3(1), dim3(kSmallSortThreads), 0, s, keys, vals, (uint64_t)0, dn,
+                           (uint64_t)kSortTile);
+    else
+        hipLaunchKernelGGL((k_sort_tiles<false>), dim3(1), dim3(kSmallSortThreads), 0, s, keys, vals, (uint64_t)0, dn,
+                           (uint64_t)kSortTile);
+}
+
+__global__ void k_keys_to_pairs_counted(const uint64_t* __restrict__ keys, const unsigned long long* __restrict__ dn,
+                                        uint64_t cap, int2* __restrict__ pairs) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t n = *dn;
+    if (n <= cap && i < n) {
+        const uint64_t k = keys[i];
+        pairs[i] = make_int2((int32_t)(uint32_t)(k >> 32), (int32_t)(uint32_t)k);
+    }
+}
+void keys_to_pairs_counted(const uint64_t* keys, const unsigned long long* dn, uint64_t cap, int32_t* pairs,
+                           hipStream_t s) {
+    if (cap)
+        hipLaunchKernelGGL(k_keys_to_pairs_counted, dim3(grid1(cap)), dim3(256), 0, s, keys, dn, cap,
+                           reinterpret_cast<int2*>(pairs));
 }
 
 size_t unique_temp_bytes(uint64_t n) { return (n * 4 + 255) / 256 * 256 + (n + 1) * 8 + 256 + scan_temp_bytes(n); }

@@ -17,6 +17,7 @@
 #include "host/host_mesh.hpp"
 #include "kernels/pz_device.h"
 #include "kernels/pz_kernels.h"
+#include "kernels/pz_kernels_join.h"
 #include "pierce_hip.h"
 
 struct DevBuf {
@@ -196,6 +197,7 @@ struct ObjBvh {
 struct SegOut {
     SetPrep pa{}, pb{};
     ObjBvh ob{};
+    pzk::ObjGrid og{};
     uint32_t* big[2] = {nullptr, nullptr};  // per-object "built by the big-object path" flags of both sets
 };
 

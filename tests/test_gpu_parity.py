@@ -180,7 +180,7 @@ def test_self_join_and_swap(ctx):
     assert set(map(tuple, ab.tolist())) == set(map(tuple, ba[:, ::-1].tolist()))
 
 
-@pytest.mark.parametrize("broad_mode", [1, 2])  # object LBVH walk, box scan
+@pytest.mark.parametrize("broad_mode", [1, 2, 3])  # object LBVH walk, box scan, hashed grid
 def test_sizing_modes_agree(ctx, c1, broad_mode):
     a, b, da, db, expected = c1
     for sizing, cap in ((pz.SIZING_TWO_PASS, 0), (pz.SIZING_FIXED, 1), (pz.SIZING_FIXED, 1 << 20),
@@ -193,10 +193,59 @@ def test_broad_modes_give_the_same_candidates(ctx, c1):
     a, b, da, db, expected = c1
     walk = pz.join(ctx, da, db, broad_mode=1)
     scan = pz.join(ctx, da, db, broad_mode=2)
-    assert walk.num_candidates == scan.num_candidates > 0
+    grid = pz.join(ctx, da, db, broad_mode=3)
+    assert walk.num_candidates == scan.num_candidates == grid.num_candidates > 0
     np.testing.assert_array_equal(walk.pairs, scan.pairs)
+    np.testing.assert_array_equal(grid.pairs, scan.pairs)
     with pytest.raises(pz.PierceError):
-        pz.join(ctx, da, db, broad_mode=3)
+        pz.join(ctx, da, db, broad_mode=4)
+
+
+def _scaled(m, factors):
+    """Copy of m with object o scaled by factors[o] about its vertex mean
+    (objects own contiguous vertex runs, as the generators write them)."""
+    v = m.vertices.copy()
+    t2o = m.triangle_to_object
+    vobj = np.full(v.shape[0], -1, np.int64)
+    vobj[m.indices.reshape(-1)] = np.repeat(t2o, 3)
+    for o in np.nonzero(factors != 1.0)[0]:
+        sel = vobj == o
+        c = v[sel].mean(0)
+        v[sel] = (c + (v[sel] - c) * np.float32(factors[o])).astype(np.float32)
+    return pz.Mesh(v, m.indices, t2o)
+
+
+def test_grid_broad_phase_mixed_sizes(ctx):
+    """Hashed-grid broad phase against the box scan: a few mesh2 boxes over
+    many cells (kept apart and tested by every query), mesh1 boxes over more
+    cells than a query visits, flat and empty objects, negative coordinates."""
+    rng = np.random.default_rng(7)
+    a = pz.generate(pz.GEN_ELLIPSOIDS, count=600, seed=11, box=60.0)
+    b = pz.generate(pz.GEN_ELLIPSOIDS, count=600, seed=12, box=60.0)
+    fa = np.ones(a.num_objects)
+    fb = np.ones(b.num_objects)
+    fb[rng.choice(b.num_objects, 6, replace=False)] = 12.0   # boxes over many cells
+    fa[rng.choice(a.num_objects, 3, replace=False)] = 40.0   # queries over very many cells
+    fb[rng.choice(b.num_objects, 20, replace=False)] = 0.05  # tiny boxes
+    a, b = _scaled(a, fa), _scaled(b, fb)
+    a.vertices -= np.float32(30.0)  # negative coordinates
+    b.vertices -= np.float32(30.0)
+    # flatten one mesh2 object onto a plane (zero extent on one axis)
+    vb = b.vertices
+    sel = np.unique(b.indices[b.triangle_to_object == 3])
+    vb[sel, 2] = vb[sel[0], 2]
+    # an object id without triangles in mesh2
+    keep = b.triangle_to_object != 5
+    a = pz.extract_edges(a)
+    b = pz.extract_edges(pz.Mesh(b.vertices, b.indices[keep], b.triangle_to_object[keep]))
+    da, db = ctx.upload(a), ctx.upload(b)
+    scan = pz.join(ctx, da, db, broad_mode=2)
+    grid = pz.join(ctx, da, db, broad_mode=3)
+    assert grid.num_candidates == scan.num_candidates > 0
+    np.testing.assert_array_equal(grid.pairs, scan.pairs)
+    for pred in (pz.PRED_INTERSECTION, pz.PRED_CONTAINMENT):
+        np.testing.assert_array_equal(pz.join(ctx, da, db, predicate=pred, broad_mode=3).pairs,
+                                      pz.join(ctx, da, db, predicate=pred, broad_mode=2).pairs)
 
 
 def test_empty_result(ctx):

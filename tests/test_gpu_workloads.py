@@ -61,6 +61,15 @@ def test_sweep_three_predicates_as_one_batch(ctx, sweep, broad_mode):
         np.testing.assert_array_equal(o.pairs, exp[p], err_msg=f"predicate {p}")
 
 
+def test_batch_rejects_grid_broad_phase(ctx, sweep):
+    """The hashed grid serves single joins; a batch keeps its parts apart in
+    the object LBVH, so broad_mode 3 is refused there."""
+    a, b, _, _, _ = sweep
+    da, db = ctx.upload_batch([a]), ctx.upload_batch([b])
+    with pytest.raises(pz.PierceError):
+        pz.join_batch(ctx, da, db, [dict(predicate=pz.PRED_OVERLAP, broad_mode=3)])
+
+
 @pytest.mark.parametrize("pred", PREDS)
 @pytest.mark.parametrize("precision", [pz.PREC_FP32_FILTERED, pz.PREC_FP64])
 def test_tubes_with_degenerate_copies(ctx, tubes, pred, precision):
