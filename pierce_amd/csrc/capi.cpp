@@ -585,15 +585,32 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
     // Per-pair meta = predicate | direction bits << 2 (bit 0: mesh1 edges vs mesh2
     // triangles and mesh1 points in mesh2 objects; bit 1: the reverse).
     uint8_t* meta = buf<uint8_t>(c, "np.meta", np + 1);
-    pzk::pair_meta(pairs, np, batched ? A->part_of : nullptr, meta_of_part, pz::pair_meta_of(d), meta, s);
     const bool f32 = d.precision == PZ_PREC_FP32_FILTERED;
     const int allow_f32 = (std::max(A->max_abs, B->max_abs) <= 0x1p39f) ? 1 : 0;
     uint32_t* found = buf<uint32_t>(c, "np.found", np + 1);
-    pzk::fill_u32(found, 0u, np + 1, s);
     uint32_t* inside = nullptr;  // parity bits, for predicates and batches that need them
-    if (d.predicate != PZ_PRED_OVERLAP || batched) {
-        inside = buf<uint32_t>(c, "np.inside", np + 1);
-        pzk::fill_u32(inside, 0u, np + 1, s);
+    if (d.predicate != PZ_PRED_OVERLAP || batched) inside = buf<uint32_t>(c, "np.inside", np + 1);
+    unsigned long long* ocnt = buf<unsigned long long>(c, "sel.cnt", 1);
+    // the pair sieve's flags and counters (narrow_mode 0)
+    const bool sieve = d.narrow_mode == 0;
+    uint32_t* defer = sieve ? buf<uint32_t>(c, "sv.defer", np + 1) : nullptr;
+    unsigned long long* sctr = sieve ? buf<unsigned long long>(c, "sv.ctr", 2) : nullptr;  // open triangle pairs, deferred pairs
+    unsigned long long* nrays = sieve ? buf<unsigned long long>(c, "sv.nrays", 1) : nullptr;
+    uint32_t* rovf = sieve ? buf<uint32_t>(c, "sv.ray_overflow", np + 1) : nullptr;
+    uint32_t* wctr = sieve ? buf<uint32_t>(c, "sv.wctr", 8) : nullptr;
+    {
+        pzk::PairPrepZero z{};
+        z.flags[0] = found;
+        z.flags[1] = inside;
+        z.flags[2] = defer;
+        z.flags[3] = (sieve && inside) ? rovf : nullptr;
+        z.counters[0] = ocnt;
+        z.counters[1] = sctr;
+        z.counters[2] = sieve ? sctr + 1 : nullptr;
+        z.counters[3] = nrays;
+        z.words = wctr;
+        z.nwords = sieve ? 8u : 0u;
+        pzk::pair_prep(pairs, np, batched ? A->part_of : nullptr, meta_of_part, pz::pair_meta_of(d), meta, z, s);
     }
 
     // The tree path: per-object BVHs of the objects of the given pairs, the
@@ -862,10 +879,11 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
     // types), then sorted and copied to the host.
     uint64_t* okeys = buf<uint64_t>(c, "sel.keys", np + 1);
     uint32_t* otype = buf<uint32_t>(c, "sel.type", np + 1);
-    unsigned long long* ocnt = buf<unsigned long long>(c, "sel.cnt", 1);
     const bool want_types = batched || (d.predicate == PZ_PRED_CONTAINMENT && d.include_overlap_pairs);
+    bool selected = false;  // the first selection finds the count zeroed by the pair prep
     auto select = [&]() {
-        pzk::fill_ull(ocnt, 0ull, 1, s);
+        if (selected) pzk::fill_ull(ocnt, 0ull, 1, s);
+        selected = true;
         // batches keep overlap pairs of containment queries (typed) and filter per query on the host
         pzk::select_pairs(pairs, np, found, inside, meta, batched ? 1 : d.include_overlap_pairs, okeys, otype, ocnt,
                           stats + pzk::kStatTouching, s);
@@ -890,14 +908,10 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
     uint64_t ntp = np;
     uint8_t* tmeta = meta;
     uint32_t* tindex = nullptr;  // sieve: position of each deferred pair in pairs[]
-    if (d.narrow_mode == 0) {
+    if (sieve) {
         ph.begin("Pair Sieve");
-        uint32_t* defer = buf<uint32_t>(c, "sv.defer", np + 1);
-        pzk::fill_u32(defer, 0u, np + 1, s);
         const uint64_t cap_esc = std::min<uint64_t>(std::max<uint64_t>(4 * np, 1 << 16), 1ull << 22);
         uint4* esc = buf<uint4>(c, "sv.esc", cap_esc);
-        unsigned long long* sctr = buf<unsigned long long>(c, "sv.ctr", 2);  // open triangle pairs, deferred pairs
-        pzk::fill_ull(sctr, 0ull, 2, s);
         auto sieve_set = [](const pz_dataset* D, const SetPrep& P) {
             const uint64_t n = (uint64_t)std::max(D->nobj, 1);
             return pzd::SieveSet{D->v, D->idx, P.first_tri, P.tri_cnt, P.last_tri, P.obox,
@@ -906,16 +920,11 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         const pzd::SieveSet sa = sieve_set(A, pa), sb = sieve_set(B, pb);
         uint4* rdesc = buf<uint4>(c, "sv.rdesc", 5 * (np + 1));
         uint32_t* gwork = buf<uint32_t>(c, "sv.gwork", np + 1);
-        uint32_t* wctr = buf<uint32_t>(c, "sv.wctr", 4);
         int cus = 0;
         HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
         // parity-ray candidates: about a dozen target triangles per ray
         const uint64_t cap_rays = inside ? std::min<uint64_t>(std::max<uint64_t>(64 * np, 1 << 16), 1ull << 26) : 1;
         uint2* rays = buf<uint2>(c, "sv.rays", cap_rays);
-        unsigned long long* nrays = buf<unsigned long long>(c, "sv.nrays", 1);
-        uint32_t* rovf = buf<uint32_t>(c, "sv.ray_overflow", np + 1);
-        pzk::fill_ull(nrays, 0ull, 1, s);
-        if (inside) pzk::fill_u32(rovf, 0u, np + 1, s);
         pzk::pair_sieve(pairs, np, sa, sb, meta, f32, allow_f32, found, defer, rdesc, gwork, wctr,
                         inside ? rays : nullptr, nrays, inside ? cap_rays : 0, rovf, esc, sctr, cap_esc,
                         (unsigned)std::max(cus, 1), stats, s);

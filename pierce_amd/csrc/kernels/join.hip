@@ -517,16 +517,21 @@ __global__ __launch_bounds__(kBlock) void k_grid_stats(const uint32_t* __restric
     }
 }
 
-// Mode 0: count references per bucket (and list the oversized boxes); mode 1:
-// place each reference in its bucket's range [start, start + count), counting
-// the bucket down (the counts end at zero).
+constexpr int kGridLanes = 8;  // lanes per object of the grid's reference and query kernels
+
+// kGridLanes lanes per mesh2 object.  Mode 0: count references per bucket
+// (and list the oversized boxes); mode 1: place each reference in its
+// bucket's range [start, start + count), counting the bucket down (the counts
+// end at zero).
 template <int Mode>
 __global__ __launch_bounds__(kBlock) void k_grid_refs(const uint32_t* __restrict__ obox, int32_t n,
                                                       const float* __restrict__ stats, uint32_t nb,
                                                       uint32_t* __restrict__ counts, const uint32_t* __restrict__ start,
                                                       uint4* __restrict__ entries, float4* __restrict__ ebox,
                                                       uint32_t* __restrict__ big, uint32_t* __restrict__ nbig) {
-    const int32_t o = blockIdx.x * kBlock + threadIdx.x;
+    const uint64_t gid = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int32_t o = (int32_t)(gid / kGridLanes);
+    const uint32_t sub = (uint32_t)(gid % kGridLanes);
     if (o >= n) return;
     float lo[3], hi[3];
     if (!load_box(obox, o, lo, hi)) return;
@@ -539,29 +544,37 @@ __global__ __launch_bounds__(kBlock) void k_grid_refs(const uint32_t* __restrict
         span *= (uint64_t)(ch[k] - cl[k] + 1);
     }
     if (span > kGridSpan) {
-        if (Mode == 0) big[atomicAdd(nbig, 1u)] = (uint32_t)o;
+        if (Mode == 0 && sub == 0) big[atomicAdd(nbig, 1u)] = (uint32_t)o;
         return;
     }
-    for (int32_t z = cl[2]; z <= ch[2]; ++z)
-        for (int32_t y = cl[1]; y <= ch[1]; ++y)
-            for (int32_t x = cl[0]; x <= ch[0]; ++x) {
-                const uint32_t b = cell_hash(x, y, z, nb);
-                if (Mode == 0) {
-                    atomicAdd(&counts[b], 1u);
-                } else {
-                    const uint32_t e = start[b] + atomicSub(&counts[b], 1u) - 1u;
-                    entries[e] = make_uint4((uint32_t)o, (uint32_t)x, (uint32_t)y, (uint32_t)z);
-                    ebox[2ull * e] = make_float4(lo[0], lo[1], lo[2], hi[0]);
-                    ebox[2ull * e + 1] = make_float4(hi[1], hi[2], 0.f, 0.f);
-                }
-            }
+    const uint32_t rx = (uint32_t)(ch[0] - cl[0] + 1), ry = (uint32_t)(ch[1] - cl[1] + 1);
+    for (uint32_t i = sub; i < (uint32_t)span; i += kGridLanes) {
+        const int32_t x = cl[0] + (int32_t)(i % rx), y = cl[1] + (int32_t)((i / rx) % ry), z = cl[2] + (int32_t)(i / (rx * ry));
+        const uint32_t b = cell_hash(x, y, z, nb);
+        if (Mode == 0) {
+            atomicAdd(&counts[b], 1u);
+        } else {
+            const uint32_t e = start[b] + atomicSub(&counts[b], 1u) - 1u;
+            entries[e] = make_uint4((uint32_t)o, (uint32_t)x, (uint32_t)y, (uint32_t)z);
+            ebox[2ull * e] = make_float4(lo[0], lo[1], lo[2], hi[0]);
+            ebox[2ull * e + 1] = make_float4(hi[1], hi[2], 0.f, 0.f);
+        }
+    }
+}
+
+// The grid's zeroes: bucket counts (nb + 1), extent sums, oversized count.
+__global__ void k_grid_clear(uint32_t* __restrict__ counts, uint64_t n, float* __restrict__ stats,
+                             uint32_t* __restrict__ nbig) {
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) counts[i] = 0u;
+    if (i < 4) stats[i] = 0.f;
+    if (i == 0) *nbig = 0u;
 }
 
 // kGridLanes lanes per mesh1 object: the mesh2 objects whose boxes meet its
 // box, each once (modes as k_broad: count, write at off[a], append with
 // capacity).  The lanes share the object's cells and the oversized list; each
 // tests the entries of its cells' buckets (their boxes stored beside them).
-constexpr int kGridLanes = 8;
 template <int Mode>
 __global__ __launch_bounds__(kBlock) void k_grid_broad(const uint32_t* __restrict__ oboxA, int32_t nA,
                                                        const uint32_t* __restrict__ oboxB, int32_t nB,
@@ -772,6 +785,23 @@ __global__ void k_pair_meta(const uint64_t* __restrict__ pairs, uint64_t np, con
     uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= np) return;
     meta[i] = part_of_a ? meta_of_part[part_of_a[(uint32_t)(pairs[i] >> 32)]] : uniform;
+}
+
+// Per candidate pair: meta (as k_pair_meta) and the zeroes of the per-pair
+// flag arrays; the first thread also zeroes the join's single counters.
+__global__ void k_pair_prep(const uint64_t* __restrict__ pairs, uint64_t np, const uint32_t* __restrict__ part_of_a,
+                            const uint8_t* __restrict__ meta_of_part, uint8_t uniform, uint8_t* __restrict__ meta,
+                            pzk::PairPrepZero z) {
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i <= np)
+        for (int k = 0; k < pzk::PairPrepZero::kFlags; ++k)
+            if (z.flags[k]) z.flags[k][i] = 0u;
+    if (i < np) meta[i] = part_of_a ? meta_of_part[part_of_a[(uint32_t)(pairs[i] >> 32)]] : uniform;
+    if (i == 0) {
+        for (int k = 0; k < pzk::PairPrepZero::kCounters; ++k)
+            if (z.counters[k]) *z.counters[k] = 0ull;
+        for (uint32_t w = 0; w < z.nwords; ++w) z.words[w] = 0u;
+    }
 }
 
 // ------------------------------------------------- needed objects / layout
@@ -1916,18 +1946,16 @@ uint64_t grid_entry_bound(int32_t nB) { return (uint64_t)std::max(nB, 1) * kGrid
 
 void grid_build(const uint32_t* oboxB, int32_t nB, const ObjGrid& g, void* scan_tmp, hipStream_t s) {
     KernelTimer kt_("grid_build", s);
-    hipLaunchKernelGGL((k_fill_any<uint32_t>), dim3(grid1((uint64_t)g.nb + 1)), dim3(kBlock), 0, s, g.counts, 0u,
-                       (uint64_t)g.nb + 1);
-    hipLaunchKernelGGL((k_fill_any<float>), dim3(1), dim3(64), 0, s, g.stats, 0.f, (uint64_t)4);
-    hipLaunchKernelGGL((k_fill_any<uint32_t>), dim3(1), dim3(64), 0, s, g.nbig, 0u, (uint64_t)1);
+    hipLaunchKernelGGL(k_grid_clear, dim3(grid1((uint64_t)g.nb + 1)), dim3(kBlock), 0, s, g.counts, (uint64_t)g.nb + 1,
+                       g.stats, g.nbig);
     if (nB <= 0) return;
     hipLaunchKernelGGL(k_grid_stats, dim3(std::min(grid1((uint64_t)nB), kSceneBlocks)), dim3(kBlock), 0, s, oboxB, nB,
                        g.stats);
-    hipLaunchKernelGGL((k_grid_refs<0>), dim3(grid1((uint64_t)nB)), dim3(kBlock), 0, s, oboxB, nB,
+    hipLaunchKernelGGL((k_grid_refs<0>), dim3(grid1((uint64_t)nB * kGridLanes)), dim3(kBlock), 0, s, oboxB, nB,
                        (const float*)g.stats, g.nb, g.counts, (const uint32_t*)nullptr, (uint4*)nullptr,
                        (float4*)nullptr, g.big, g.nbig);
     exclusive_scan_u32_u32(g.counts, g.start, (uint64_t)g.nb + 1, scan_tmp, s);  // start[nb] = every reference
-    hipLaunchKernelGGL((k_grid_refs<1>), dim3(grid1((uint64_t)nB)), dim3(kBlock), 0, s, oboxB, nB,
+    hipLaunchKernelGGL((k_grid_refs<1>), dim3(grid1((uint64_t)nB * kGridLanes)), dim3(kBlock), 0, s, oboxB, nB,
                        (const float*)g.stats, g.nb, g.counts, (const uint32_t*)g.start, g.entries, g.ebox, g.big,
                        g.nbig);
 }
@@ -2166,6 +2194,12 @@ void pair_meta(const uint64_t* pairs, uint64_t np, const uint32_t* part_of_a, co
     if (np)
         hipLaunchKernelGGL(k_pair_meta, dim3(grid1(np)), dim3(kBlock), 0, s, pairs, np, part_of_a, meta_of_part,
                            uniform, meta);
+}
+
+void pair_prep(const uint64_t* pairs, uint64_t np, const uint32_t* part_of_a, const uint8_t* meta_of_part,
+               uint8_t uniform, uint8_t* meta, const PairPrepZero& z, hipStream_t s) {
+    hipLaunchKernelGGL(k_pair_prep, dim3(grid1(np + 1)), dim3(kBlock), 0, s, pairs, np, part_of_a, meta_of_part, uniform,
+                       meta, z);
 }
 
 void compact_flagged(const uint64_t* pairs, uint64_t np, const uint32_t* flag, uint64_t* out, uint32_t* out_index,

@@ -168,6 +168,18 @@ void grid_write(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, int32_
                 const uint64_t* off, uint64_t* out, hipStream_t s);
 void grid_append(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, int32_t nB, const ObjGrid& g, uint64_t* out,
                  unsigned long long* cursor, uint64_t cap, hipStream_t s);
+// What pair_prep zeroes besides writing meta: per-pair u32 flag arrays of
+// np + 1 words, single u64 counters and a block of u32 words (null: none).
+struct PairPrepZero {
+    static constexpr int kFlags = 4, kCounters = 6;
+    uint32_t* flags[kFlags];
+    unsigned long long* counters[kCounters];
+    uint32_t* words;
+    uint32_t nwords;
+};
+// pair_meta's meta and the zeroes of z, in one launch.
+void pair_prep(const uint64_t* pairs, uint64_t np, const uint32_t* part_of_a, const uint8_t* meta_of_part,
+               uint8_t uniform, uint8_t* meta, const PairPrepZero& z, hipStream_t s);
 // meta[p] = predicate | dir_mask << 2, from the pair's part or `uniform`.
 void pair_meta(const uint64_t* pairs, uint64_t np, const uint32_t* part_of_a, const uint8_t* meta_of_part,
                uint8_t uniform, uint8_t* meta, hipStream_t s);
@@ -260,7 +272,7 @@ void fill_ull(unsigned long long* p, unsigned long long v, uint64_t n, hipStream
 // tree path (nothing else written for it).  With f32 (and allow_f32) open triangle pairs go to esc as (pair,
 // mesh1 triangle, mesh2 triangle) for escalate_tris.
 // run_desc holds 5 np uint4 (the vertex-run sieve's per-pair descriptors), gwork
-// np pair indices, ctr 4 words; blocks = compute units.
+// np pair indices, ctr 8 words, zeroed by the caller; blocks = compute units.
 // rays (rays_cap entries) and nrays: the parity-ray candidates the first pass
 // records for sieve_parity; ray_overflow (np flags) marks pairs some of whose
 // candidates did not fit.

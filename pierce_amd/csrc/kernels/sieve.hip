@@ -1243,9 +1243,6 @@ __global__ __launch_bounds__(256) void k_sieve_classify(const uint64_t* __restri
     }
 }
 
-__global__ void k_fill_ctr(uint32_t* ctr) {
-    if (threadIdx.x < 4) ctr[threadIdx.x] = 0u;
-}
 
 }  // namespace
 
@@ -1261,7 +1258,6 @@ void pair_sieve(const uint64_t* pairs, uint64_t np, const SieveSet& A, const Sie
     const bool fp32 = f32 && allow_f32;
     const bool runs = A.vmin && B.vmin;
     KernelTimer kt_("pair_sieve", s);
-    hipLaunchKernelGGL(k_fill_ctr, dim3(1), dim3(64), 0, s, ctr);
     static_assert(sizeof(RunDesc) == kRunDescWords * sizeof(uint4), "run descriptor layout");
     RunDesc* rdesc = reinterpret_cast<RunDesc*>(run_desc);
     hipLaunchKernelGGL(k_sieve_classify, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, pairs, np, A, B,
