@@ -220,6 +220,7 @@ __global__ __launch_bounds__(kBlock) void k_topo_verts(const int32_t* __restrict
 // run: ordered-uint minima and maxima (a total order, so the result does not
 // depend on the order of the reduction), one set of atomics per chunk.
 constexpr uint32_t kRunChunk = 4096;
+constexpr int kBoundsBatch = 12;  // measured against 8 and 16 on 642-vertex objects
 
 __global__ __launch_bounds__(kBlock) void k_object_bounds_runs(const float* __restrict__ verts,
                                                               const uint2* __restrict__ chunks, uint64_t nch,
@@ -231,13 +232,19 @@ __global__ __launch_bounds__(kBlock) void k_object_bounds_runs(const float* __re
     const uint32_t o = ch.x, v0 = ch.y;
     const uint32_t v1 = min(v0 + kRunChunk - 1u, vmax[o]);  // last vertex of the chunk
     uint32_t lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u};
-#pragma unroll 4
-    for (uint32_t v = v0 + __lane_id(); v <= v1; v += PZ_WAVE) {
-        const float3 p = vtx(verts, v);
-        const uint32_t q[3] = {f2ord(p.x), f2ord(p.y), f2ord(p.z)};
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = min(lo[a], q[a]);
-            hi[a] = max(hi[a], q[a]);
+    // kBoundsBatch vertex loads per lane in flight at once (a chunk of 768
+    // vertices in one round); past the chunk's end a lane repeats its last vertex
+    for (uint32_t vb = v0 + __lane_id(); vb <= v1; vb += kBoundsBatch * PZ_WAVE) {
+        float3 p[kBoundsBatch];
+#pragma unroll
+        for (int j = 0; j < kBoundsBatch; ++j) p[j] = vtx(verts, min(vb + j * PZ_WAVE, v1));
+#pragma unroll
+        for (int j = 0; j < kBoundsBatch; ++j) {
+            const uint32_t q[3] = {f2ord(p[j].x), f2ord(p[j].y), f2ord(p[j].z)};
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = min(lo[a], q[a]);
+                hi[a] = max(hi[a], q[a]);
+            }
         }
     }
     for (int a = 0; a < 3; ++a) {

@@ -491,7 +491,7 @@ constexpr int kRunV = 672;      // vertices per object
 constexpr int kRunKeep = 1280;  // kept triangles per side
 constexpr int kRunG = 1088;     // grid-side triangles
 constexpr int kRunRefs = 6144;
-constexpr int kUR = 4;
+constexpr int kUR = 3;  // measured against 2 and 4 on 1,280-triangle objects
 constexpr int kWalkILP = 4;  // grid references a lane tests per step
 constexpr int kRunThreads = 512;
 
