@@ -445,7 +445,7 @@ __global__ __launch_bounds__(kBlock) void k_broad(const uint32_t* __restrict__ o
 }
 
 // ------------------------------------------------ broad phase over a hashed grid
-// Cells are the mesh2 objects' mean box extent per axis, on an unbounded
+// Cells are kGridCellScale times the mesh2 objects' mean box extent per axis, on an unbounded
 // integer lattice hashed into nb (a power of two) buckets, so the grid needs
 // no scene box and spreads dense clusters as well as sparse sets.  A mesh2
 // object is referenced in every cell its box overlaps (bucket of the cell,
@@ -456,6 +456,10 @@ __global__ __launch_bounds__(kBlock) void k_broad(const uint32_t* __restrict__ o
 // two meeting boxes share that cell); entries of other cells that share the
 // bucket are skipped by their cell.
 constexpr uint32_t kGridSpan = 27;    // cells a referenced mesh2 box may overlap
+// Cell edge over the mean box extent: each reference costs an atomic and a
+// scattered write, while larger cells only add box tests where objects are
+// dense (measured 1, 1.5, 2, 3 and 4 on 100k vs 100k sparse ellipsoids).
+constexpr float kGridCellScale = 3.f;
 constexpr uint64_t kQuerySpan = 4096; // cells beyond which a mesh1 box tests every mesh2 box
 constexpr int32_t kCellClamp = 1 << 20;
 
@@ -483,7 +487,7 @@ __device__ __forceinline__ GridCells grid_cells(const float* __restrict__ stats)
     for (int k = 0; k < 3; ++k) {
         // a flat axis takes the largest mean extent (or 1 when every box is a point)
         const float h = m[k] > 0.f ? m[k] : (mx > 0.f ? mx : 1.f);
-        g.inv[k] = 1.f / h;
+        g.inv[k] = 1.f / (kGridCellScale * h);
     }
     return g;
 }

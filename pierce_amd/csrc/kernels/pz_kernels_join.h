@@ -146,7 +146,7 @@ void broad_append(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, cons
                   const uint32_t* orderB, int32_t nB, const uint32_t* part_of_a, const uint32_t* boff, uint64_t* out,
                   unsigned long long* cursor, uint64_t cap, hipStream_t s);
 // Broad phase over a hashed grid of the mesh2 object boxes (non-batched sets):
-// cells are the boxes' mean extent per axis, hashed into nb (a power of two)
+// cells are three times the boxes' mean extent per axis, hashed into nb (a power of two)
 // buckets; boxes over more than 27 cells are kept in `big` and tested by every
 // query.  counts and start hold nb + 1 words, entries grid_entry_bound(nB) (ebox twice that),
 // big nB, nbig 1 word, stats 4 floats.  Same outputs as the broad_* calls.
