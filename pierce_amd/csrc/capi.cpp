@@ -292,13 +292,14 @@ void finish_pairs(pz_ctx* c, uint64_t* keys, uint32_t* types, uint64_t n, int32_
     }
     uint64_t* sk = alt ? keys2 : keys;
     uint32_t* st = alt ? types2 : types;
-    int32_t* dpairs = buf<int32_t>(c, "fin.pairs", 2 * n + 2);
-    pzk::keys_to_pairs(sk, n, dpairs, c->s);
-    // through the pinned landing buffer: device-to-pinned copies, then host copies
+    // the pairs (and types) land in pinned memory straight from the kernel, then host copies
     const size_t pb = n * sizeof(pz_pair), tb = want_types ? n * 4 : 0;
-    char* h = static_cast<char*>(pinned(c, std::max<size_t>(pb + tb, 8)));
-    if (n) HIP_OK(hipMemcpyAsync(h, dpairs, pb, hipMemcpyDeviceToHost, c->s));
-    if (want_types && n) HIP_OK(hipMemcpyAsync(h + pb, st, tb, hipMemcpyDeviceToHost, c->s));
+    const size_t at = (pb + tb + 7) & ~size_t(7);
+    char* h = static_cast<char*>(pinned(c, at + 8));
+    unsigned long long* hn = reinterpret_cast<unsigned long long*>(h + at);  // the count, for the counted kernel
+    *hn = n;
+    pzk::keys_to_pairs_counted(sk, want_types ? st : nullptr, hn, n, reinterpret_cast<int32_t*>(h),
+                               reinterpret_cast<uint32_t*>(h + pb), c->s);
     HIP_OK(hipStreamSynchronize(c->s));
     land_pairs(r, h, reinterpret_cast<const uint32_t*>(h + pb), n, want_types);
 }
@@ -946,13 +947,11 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         ph.begin("Download Results");
         select();
         pzk::sort_tile_counted(okeys, want_types ? otype : nullptr, ocnt, s);
-        int32_t* dpairs = buf<int32_t>(c, "spec.pairs", 2 * kSpec);
-        pzk::keys_to_pairs_counted(okeys, ocnt, kSpec, dpairs, s);
+        // the pairs (and types) land in pinned memory straight from the kernel
         char* hp = static_cast<char*>(pinned(c, kHead + kSpec * (sizeof(pz_pair) + 4)));
-        const uint64_t m = std::min<uint64_t>(kSpec, np);  // no more result pairs than candidates
-        HIP_OK(hipMemcpyAsync(hp + kHead, dpairs, m * sizeof(pz_pair), hipMemcpyDeviceToHost, s));
-        if (want_types)
-            HIP_OK(hipMemcpyAsync(hp + kHead + kSpec * sizeof(pz_pair), otype, m * 4, hipMemcpyDeviceToHost, s));
+        pzk::keys_to_pairs_counted(okeys, want_types ? otype : nullptr, ocnt, std::min<uint64_t>(kSpec, np),
+                                   reinterpret_cast<int32_t*>(hp + kHead),
+                                   reinterpret_cast<uint32_t*>(hp + kHead + kSpec * sizeof(pz_pair)), s);
         const unsigned long long* hs = gather_counts(sctr + 1, hp);
         ntp = hs[0];
         r->sieve_pairs = np - ntp;

@@ -36,10 +36,12 @@ void keys_to_pairs(const uint64_t* keys, uint64_t n, int32_t* pairs, hipStream_t
 constexpr uint64_t kSortTileKeys = 4096, kSortTilesMax = 16 * kSortTileKeys;
 bool sort_tiles(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, uint64_t n, hipStream_t s);
 // One tile sorted in place with the key count read from *dn (nothing done
-// when it exceeds 4,096), and those keys as pairs (nothing when above cap).
+// when it exceeds 4,096), and those keys as pairs with their values copied
+// beside them (nothing when above cap; vals may be null).  pairs and ovals
+// may be pinned host memory: the kernel writes the result there directly.
 void sort_tile_counted(uint64_t* keys, uint32_t* vals, const unsigned long long* dn, hipStream_t s);
-void keys_to_pairs_counted(const uint64_t* keys, const unsigned long long* dn, uint64_t cap, int32_t* pairs,
-                           hipStream_t s);
+void keys_to_pairs_counted(const uint64_t* keys, const uint32_t* vals, const unsigned long long* dn, uint64_t cap,
+                           int32_t* pairs, uint32_t* ovals, hipStream_t s);
 void pairs_to_keys(const int32_t* pairs, uint64_t n, uint64_t* keys, hipStream_t s);
 
 void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);

@@ -498,20 +498,22 @@ void sort_tile_counted(uint64_t* keys, uint32_t* vals, const unsigned long long*
                            (uint64_t)kSortTile);
 }
 
-__global__ void k_keys_to_pairs_counted(const uint64_t* __restrict__ keys, const unsigned long long* __restrict__ dn,
-                                        uint64_t cap, int2* __restrict__ pairs) {
+__global__ void k_keys_to_pairs_counted(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                        const unsigned long long* __restrict__ dn, uint64_t cap,
+                                        int2* __restrict__ pairs, uint32_t* __restrict__ ovals) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t n = *dn;
     if (n <= cap && i < n) {
         const uint64_t k = keys[i];
         pairs[i] = make_int2((int32_t)(uint32_t)(k >> 32), (int32_t)(uint32_t)k);
+        if (vals) ovals[i] = vals[i];
     }
 }
-void keys_to_pairs_counted(const uint64_t* keys, const unsigned long long* dn, uint64_t cap, int32_t* pairs,
-                           hipStream_t s) {
+void keys_to_pairs_counted(const uint64_t* keys, const uint32_t* vals, const unsigned long long* dn, uint64_t cap,
+                           int32_t* pairs, uint32_t* ovals, hipStream_t s) {
     if (cap)
-        hipLaunchKernelGGL(k_keys_to_pairs_counted, dim3(grid1(cap)), dim3(256), 0, s, keys, dn, cap,
-                           reinterpret_cast<int2*>(pairs));
+        hipLaunchKernelGGL(k_keys_to_pairs_counted, dim3(grid1(cap)), dim3(256), 0, s, keys, vals, dn, cap,
+                           reinterpret_cast<int2*>(pairs), ovals);
 }
 
 size_t unique_temp_bytes(uint64_t n) { return (n * 4 + 255) / 256 * 256 + (n + 1) * 8 + 256 + scan_temp_bytes(n); }
