@@ -889,7 +889,7 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         pzk::select_pairs(pairs, np, found, inside, meta, batched ? 1 : d.include_overlap_pairs, okeys, otype, ocnt,
                           stats + pzk::kStatTouching, s);
     };
-    constexpr uint64_t kSpec = pzk::kSortTileKeys;           // result pairs of the speculative download
+    constexpr uint64_t kSpec = pzk::kSortTilesMax;           // result pairs of the speculative download
     constexpr size_t kHead = (2 + pzk::kStatSlots) * 8;      // counts and counters at the front of the landing area
     // One read-back into h: [0] *deferred (0 without), [1] result pairs, [2..] the counters.
     auto gather_counts = [&](const unsigned long long* deferred, void* h) {
@@ -946,22 +946,27 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         // result pairs needs no further round trip.
         ph.begin("Download Results");
         select();
-        pzk::sort_tile_counted(okeys, want_types ? otype : nullptr, ocnt, s);
-        // the pairs (and types) land in pinned memory straight from the kernel
-        char* hp = static_cast<char*>(pinned(c, kHead + kSpec * (sizeof(pz_pair) + 4)));
-        pzk::keys_to_pairs_counted(okeys, want_types ? otype : nullptr, ocnt, std::min<uint64_t>(kSpec, np),
+        // sorted tiles merged by rank, with the count read on the device; the
+        // pairs (and types) land in pinned memory straight from the kernel
+        const uint64_t spec = std::min<uint64_t>(kSpec, np);
+        uint64_t* okeys2 = buf<uint64_t>(c, "fin.keys2", spec + 1);
+        uint32_t* otype2 = buf<uint32_t>(c, "fin.types2", spec + 1);
+        pzk::sort_tiles_counted(okeys, want_types ? otype : nullptr, okeys2, want_types ? otype2 : nullptr, ocnt, spec,
+                                s);
+        char* hp = static_cast<char*>(pinned(c, kHead + spec * (sizeof(pz_pair) + 4)));
+        pzk::keys_to_pairs_counted(okeys2, want_types ? otype2 : nullptr, ocnt, spec,
                                    reinterpret_cast<int32_t*>(hp + kHead),
-                                   reinterpret_cast<uint32_t*>(hp + kHead + kSpec * sizeof(pz_pair)), s);
+                                   reinterpret_cast<uint32_t*>(hp + kHead + spec * sizeof(pz_pair)), s);
         const unsigned long long* hs = gather_counts(sctr + 1, hp);
         ntp = hs[0];
         r->sieve_pairs = np - ntp;
         if (ntp == 0) {
             const uint64_t nout = hs[1];
             const std::vector<unsigned long long> st(hs + 2, hs + 2 + pzk::kStatSlots);
-            if (nout <= kSpec)
-                land_pairs(r, hp + kHead, reinterpret_cast<const uint32_t*>(hp + kHead + kSpec * sizeof(pz_pair)),
+            if (nout <= spec)
+                land_pairs(r, hp + kHead, reinterpret_cast<const uint32_t*>(hp + kHead + spec * sizeof(pz_pair)),
                            nout, want_types);
-            else  // more result pairs than one tile: sort them with the count now known
+            else  // more result pairs than the tiles hold: sort them with the count now known
                 finish_pairs(c, okeys, want_types ? otype : nullptr, nout, nA, want_types, r);
             set_counters(r, st);
             return;

@@ -40,6 +40,10 @@ bool sort_tiles(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* va
 // beside them (nothing when above cap; vals may be null).  pairs and ovals
 // may be pinned host memory: the kernel writes the result there directly.
 void sort_tile_counted(uint64_t* keys, uint32_t* vals, const unsigned long long* dn, hipStream_t s);
+// sort_tiles with the key count read from *dn (at most cap, else nothing is
+// done); the sorted keys and values are always in keys_alt / vals_alt.
+void sort_tiles_counted(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
+                        const unsigned long long* dn, uint64_t cap, hipStream_t s);
 void keys_to_pairs_counted(const uint64_t* keys, const uint32_t* vals, const unsigned long long* dn, uint64_t cap,
                            int32_t* pairs, uint32_t* ovals, hipStream_t s);
 void pairs_to_keys(const int32_t* pairs, uint64_t n, uint64_t* keys, hipStream_t s);

@@ -287,7 +287,12 @@ __global__ __launch_bounds__(kSmallSortThreads) void k_sort_tiles(uint64_t* __re
 template <bool HasVals>
 __global__ __launch_bounds__(256) void k_merge_tiles(const uint64_t* __restrict__ keys,
                                                      const uint32_t* __restrict__ vals, uint64_t n,
-                                                     uint64_t* __restrict__ okeys, uint32_t* __restrict__ ovals) {
+                                                     uint64_t* __restrict__ okeys, uint32_t* __restrict__ ovals,
+                                                     const unsigned long long* __restrict__ dn, uint64_t cap) {
+    if (dn) {
+        n = *dn;
+        if (n > cap) return;
+    }
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const uint64_t t = i / kSortTile, ntiles = (n + kSortTile - 1) / kSortTile;
@@ -483,11 +488,28 @@ bool sort_tiles(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* va
                            n, (const unsigned long long*)nullptr, (uint64_t)0);
     if (ntiles == 1) return false;
     if (vals)
-        hipLaunchKernelGGL((k_merge_tiles<true>), dim3(grid1(n)), dim3(256), 0, s, keys, vals, n, keys_alt, vals_alt);
+        hipLaunchKernelGGL((k_merge_tiles<true>), dim3(grid1(n)), dim3(256), 0, s, keys, vals, n, keys_alt, vals_alt,
+                           (const unsigned long long*)nullptr, (uint64_t)0);
     else
         hipLaunchKernelGGL((k_merge_tiles<false>), dim3(grid1(n)), dim3(256), 0, s, keys, vals, n, keys_alt,
-                           vals_alt);
+                           vals_alt, (const unsigned long long*)nullptr, (uint64_t)0);
     return true;
+}
+void sort_tiles_counted(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
+                        const unsigned long long* dn, uint64_t cap, hipStream_t s) {
+    if (cap == 0) return;
+    const uint64_t ntiles = (cap + kSortTile - 1) / kSortTile;
+    if (vals) {
+        hipLaunchKernelGGL((k_sort_tiles<true>), dim3((unsigned)ntiles), dim3(kSmallSortThreads), 0, s, keys, vals,
+                           (uint64_t)0, dn, cap);
+        hipLaunchKernelGGL((k_merge_tiles<true>), dim3(grid1(cap)), dim3(256), 0, s, keys, vals, (uint64_t)0, keys_alt,
+                           vals_alt, dn, cap);
+    } else {
+        hipLaunchKernelGGL((k_sort_tiles<false>), dim3((unsigned)ntiles), dim3(kSmallSortThreads), 0, s, keys, vals,
+                           (uint64_t)0, dn, cap);
+        hipLaunchKernelGGL((k_merge_tiles<false>), dim3(grid1(cap)), dim3(256), 0, s, keys, vals, (uint64_t)0,
+                           keys_alt, vals_alt, dn, cap);
+    }
 }
 void sort_tile_counted(uint64_t* keys, uint32_t* vals, const unsigned long long* dn, hipStream_t s) {
     if (vals)
