@@ -312,3 +312,26 @@ def test_shuffled_triangle_order(ctx):
         print(f"[shuffled] predicate={predicate} expected={len(exp)}", flush=True)
         assert len(exp) > 0 or predicate == pz.PRED_CONTAINMENT
         np.testing.assert_array_equal(pz.join(ctx, da, db, predicate=predicate).pairs, exp)
+
+
+@pytest.mark.parametrize("count", [6000, 70000])  # two sort tiles; beyond the tiles (radix sort)
+def test_many_result_pairs_sorted_and_agreeing(ctx, count):
+    """A set joined with itself gives every (i, i) plus its neighbours: more
+    result pairs than one sort tile (6,000 cubes) and than the tile merge
+    (70,000 cubes).  The sieve's speculative download, the tree path's
+    download and the set-wide path agree and are sorted."""
+    m = pz.generate(pz.GEN_CUBES, count=count, seed=31, radius_min=1.0, radius_max=4.0,
+                    box=6.0 * count ** (1.0 / 3.0))
+    d = ctx.upload(m)
+    outs = [pz.join(ctx, d, d, **kw) for kw in (dict(), dict(narrow_mode=2),
+                                                dict(strategy=pz.STRATEGY_GLOBAL, precision=pz.PREC_FP64))]
+    p = outs[0].pairs
+    assert len(p) > count
+    keys = p[:, 0].astype(np.int64) << 32 | p[:, 1].astype(np.int64)
+    assert np.all(np.diff(keys) > 0)  # sorted, unique
+    assert set((i, i) for i in range(count)) <= set(map(tuple, p.tolist()))
+    for o in outs[1:]:
+        np.testing.assert_array_equal(o.pairs, p)
+    for pred in (pz.PRED_INTERSECTION, pz.PRED_CONTAINMENT):
+        np.testing.assert_array_equal(pz.join(ctx, d, d, predicate=pred).pairs,
+                                      pz.join(ctx, d, d, predicate=pred, narrow_mode=2).pairs)
