@@ -8,8 +8,8 @@
 //     can only meet inside O.
 //  2. the side with fewer kept triangles (G) is binned into a uniform grid
 //     over O in LDS, each triangle in every cell its box overlaps; cells are
-//     about twice the mean triangle extent, at most 16 per axis and
-//     kSieveCells in all.
+//     about kCellScale times the mean triangle extent, at most 16 per axis
+//     and kSieveCells in all.
 //  3. every kept triangle of the other side looks up the cells its box
 //     overlaps and tests the G triangles there whose boxes meet its own, each
 //     pair once: in the cell of the larger of the two boxes' lower corners
