@@ -310,7 +310,10 @@ typedef struct {
                                    (the others went to the per-object trees) */
     uint64_t sieve_triangles;   /* triangles of those pairs' objects, per pair (what the sieve read) */
     uint32_t index_height;      /* tallest triangle BVH the join built (at most 63, else PZ_E_LIMIT) */
-    uint32_t reserved_u32;
+    uint32_t chunk_sorted_sets; /* two-level: sets (0-2) whose objects over 2,048 triangles were cut into
+                                   chunks in a Morton-sorted order instead of their own triangle order:
+                                   some were not contiguous, or an object's own order was not spatially
+                                   coherent (its chunks' boxes summed to over 4 times its box's area) */
     double   ms_total;          /* device-resident geometry -> sorted host pair set */
     pz_phase phases[PZ_MAX_PHASES];
 } pz_join_result;

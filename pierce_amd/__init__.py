@@ -122,7 +122,7 @@ class JoinResult(C.Structure):
                 ("parity_inside", C.c_uint64), ("parity_tests", C.c_uint64), ("sieve_pairs", C.c_uint64),
                 ("sieve_triangles", C.c_uint64),
                 ("index_height", C.c_uint32),
-                ("reserved_u32", C.c_uint32), ("ms_total", C.c_double),
+                ("chunk_sorted_sets", C.c_uint32), ("ms_total", C.c_double),
                 ("phases", Phase * MAX_PHASES)]
 
 
@@ -659,6 +659,7 @@ class JoinOutput:
     index_height: int = 0
     sieve_pairs: int = 0
     sieve_triangles: int = 0
+    chunk_sorted_sets: int = 0
 
 
 def join_desc(**kw) -> JoinDesc:
@@ -694,7 +695,7 @@ def _output(r: JoinResult) -> JoinOutput:
                       int(r.walks_started), int(r.pairs_touching), int(r.parity_rays), int(r.parity_crossings),
                       int(r.parity_rays_crossing), int(r.parity_inside), int(r.stack_overflows),
                       int(r.parity_tests), int(r.index_height), int(r.sieve_pairs),
-                      int(r.sieve_triangles))
+                      int(r.sieve_triangles), int(r.chunk_sorted_sets))
 
 
 def join_batch(ctx: Context, a: Dataset, b: Dataset, queries, **common):

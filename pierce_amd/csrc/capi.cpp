@@ -627,6 +627,7 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         struct Tot {
             uint64_t nneed, ntri, nedge, nbig, nbig_tri;
             uint32_t ncls[pzk::kSmallClasses], cls_max[pzk::kSmallClasses];  // LDS-built objects per size class
+            uint32_t nscattered;  // needed objects whose triangles are not contiguous
         };
         // Needed objects of one set: objects built by the global pipeline ("big":
         // more than kSmallMax triangles or not stored contiguously) take the first
@@ -640,10 +641,10 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
             uint32_t* cls = buf<uint32_t>(c, tag + ".cls", pzk::kSmallClasses * (n + 1));
             uint32_t* loc_big = buf<uint32_t>(c, tag + ".loc_big", n + 1);
             uint32_t* loc_cls = buf<uint32_t>(c, tag + ".loc_cls", pzk::kSmallClasses * (n + 1));
-            uint32_t* totals = buf<uint32_t>(c, tag + ".totals", 8);
+            uint32_t* totals = buf<uint32_t>(c, tag + ".totals", 9);
             pzk::fill_u32(stc, 0u, n, s);
             pzk::fill_u32(sec, 0u, n, s);
-            pzk::fill_u32(totals, 0u, 8, s);
+            pzk::fill_u32(totals, 0u, 9, s);
             void* tmp = buf<char>(c, "scan.tmp", pzk::scan_temp_bytes(n + 1));
             pzk::need_split(N.need, P.tri_cnt, P.first_tri, P.last_tri, D->nobj, big, cls, totals, s);
             pzk::exclusive_scan_u32_u32(big, loc_big, no, tmp, s);
@@ -665,28 +666,31 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         });
         uint32_t* bigA = lay.big[0];
         uint32_t* bigB = lay.big[1];
-        uint32_t* totA = buf<uint32_t>(c, "A.totals", 8);
-        uint32_t* totB = buf<uint32_t>(c, "B.totals", 8);
-        // needed objects and their triangles, then totals[0..7] of each set, in one synchronisation
-        uint32_t* hb = static_cast<uint32_t*>(pinned(c, 20 * sizeof(uint32_t)));
+        uint32_t* totA = buf<uint32_t>(c, "A.totals", 9);
+        uint32_t* totB = buf<uint32_t>(c, "B.totals", 9);
+        // needed objects and their triangles, then totals[0..8] of each set, in one synchronisation
+        uint32_t* hb = static_cast<uint32_t*>(pinned(c, 22 * sizeof(uint32_t)));
         pzk::HostGather g{{na.loc + std::max(nA, 0), na.seg_off + std::max(nA, 0), totA, nb.loc + std::max(nB, 0),
                            nb.seg_off + std::max(nB, 0), totB},
-                          {1, 1, 8, 1, 1, 8},
+                          {1, 1, 9, 1, 1, 9},
                           6};
         pzk::gather_to_host(g, hb, s);
         HIP_OK(hipStreamSynchronize(s));
         auto tot_of = [&](const uint32_t* h) {
-            Tot t{h[0], h[1], 0, h[2], h[3], {}, {}};
+            Tot t{h[0], h[1], 0, h[2], h[3], {}, {}, h[10]};
             for (int k = 0; k < pzk::kSmallClasses; ++k) {
                 t.ncls[k] = h[4 + k];
                 t.cls_max[k] = h[7 + k];
             }
             return t;
         };
-        const Tot ta = tot_of(hb), tb = tot_of(hb + 10);
+        const Tot ta = tot_of(hb), tb = tot_of(hb + 11);
 
+        // Big objects are chunked in their own triangle order when every one is
+        // contiguous; `incoherent` then counts those whose order is not spatially
+        // coherent, and the set is built again with sorted = true.
         auto build = [&](const pz_dataset* D, const SetPrep& P, const Needed& N, const Tot& T, uint32_t* big,
-                         const std::string& tag) -> SetAccel {
+                         const std::string& tag, uint32_t* incoherent, bool sorted) -> SetAccel {
             SetAccel acc;
             const uint64_t n = T.ntri, nb_tri = T.nbig_tri;
             acc.nodes = buf<pzd::BNode>(c, tag + ".nodes", n + 1);
@@ -695,34 +699,42 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
             pzk::fill_u32(acc.height, 0u, 1, s);
             uint32_t* cursor = buf<uint32_t>(c, tag + ".cursor", (size_t)std::max(D->nobj, 1));
             if (nb_tri > 0) {  // global pipeline for the big objects (segment prefix)
-                // sort buffers are shared by both sets' builds (one after the other on the stream)
-                uint64_t* keys = buf<uint64_t>(c, "big.tkeys", nb_tri + 1);
-                uint32_t* vals = buf<uint32_t>(c, "big.tvals", nb_tri + 1);
-                uint64_t* keys2 = buf<uint64_t>(c, "big.tkeys2", nb_tri + 1);
-                uint32_t* vals2 = buf<uint32_t>(c, "big.tvals2", nb_tri + 1);
-                pzk::fill_u32(cursor, 0u, (size_t)std::max(D->nobj, 1), s);
-                // Morton bits of the keys: of 30, 27, 24 and 21, the most that need the
-                // fewest 8-bit radix passes beside the segment bits (chunks only need
-                // a coherent order; each chunk's tree uses the bits below its common prefix)
-                const int sb = bits_for(std::max<uint64_t>(T.nbig, 1));
-                int mbits = 30;
-                for (int m : {27, 24, 21})
-                    if ((m + sb + 7) / 8 < (mbits + sb + 7) / 8) mbits = m;
-                pzk::tri_keys_two_level(D->v, D->nv, D->idx, D->t2o, D->nt, D->nobj, big, N.loc, P.obox, N.seg_off,
-                                        P.first_tri, P.last_tri, P.tri_cnt, cursor, keys, vals, mbits, s);
-                void* tmp = buf<char>(c, "sort.tmp." + tag, pzk::radix_sort_temp_bytes(nb_tri));
-                bool alt = pzk::radix_sort_pairs(keys, vals, keys2, vals2, nb_tri, 0, mbits + sb, tmp, s);
-                uint64_t* sk = alt ? keys2 : keys;
-                uint32_t* order = alt ? vals2 : vals;
-                // runs of the sorted order through the LDS builder, then a top tree per object
+                const uint64_t* sk = nullptr;
+                const uint32_t* order = nullptr;
+                int mbits = 0;
+                const bool natural = !sorted && T.nscattered == 0;
+                if (!natural) {
+                    // sort buffers are shared by both sets' builds (one after the other on the stream)
+                    uint64_t* keys = buf<uint64_t>(c, "big.tkeys", nb_tri + 1);
+                    uint32_t* vals = buf<uint32_t>(c, "big.tvals", nb_tri + 1);
+                    uint64_t* keys2 = buf<uint64_t>(c, "big.tkeys2", nb_tri + 1);
+                    uint32_t* vals2 = buf<uint32_t>(c, "big.tvals2", nb_tri + 1);
+                    pzk::fill_u32(cursor, 0u, (size_t)std::max(D->nobj, 1), s);
+                    // Morton bits of the keys: of 30, 27, 24 and 21, the most that need the
+                    // fewest 8-bit radix passes beside the segment bits (chunks only need
+                    // a coherent order; each chunk's tree uses the bits below its common prefix)
+                    const int sb = bits_for(std::max<uint64_t>(T.nbig, 1));
+                    mbits = 30;
+                    for (int m : {27, 24, 21})
+                        if ((m + sb + 7) / 8 < (mbits + sb + 7) / 8) mbits = m;
+                    pzk::tri_keys_two_level(D->v, D->nv, D->idx, D->t2o, D->nt, D->nobj, big, N.loc, P.obox,
+                                            N.seg_off, P.first_tri, P.last_tri, P.tri_cnt, cursor, keys, vals, mbits, s);
+                    void* tmp = buf<char>(c, "sort.tmp." + tag, pzk::radix_sort_temp_bytes(nb_tri));
+                    bool alt = pzk::radix_sort_pairs(keys, vals, keys2, vals2, nb_tri, 0, mbits + sb, tmp, s);
+                    sk = alt ? keys2 : keys;
+                    order = alt ? vals2 : vals;
+                }
+                // runs of the sorted (or the objects' own) order through the LDS builder,
+                // then a top tree per object
                 const uint64_t nch = pzk::big_chunk_bound(T.nbig, nb_tri);
                 uint32_t* ccnt = buf<uint32_t>(c, tag + ".chunk_cnt", T.nbig + 1);
                 uint32_t* coff = buf<uint32_t>(c, tag + ".chunk_off", T.nbig + 1);
                 auto* e0 = buf<pzk::ChunkRoot>(c, tag + ".chunk_ent0", nch);
                 auto* e1 = buf<pzk::ChunkRoot>(c, tag + ".chunk_ent1", nch);
                 void* stmp = buf<char>(c, "scan.tmp", pzk::scan_temp_bytes(T.nbig + 1));
-                pzk::build_big_chunked(D->v, D->idx, N.obj_of_loc, N.seg_off, (uint32_t)T.nbig, nb_tri, sk, mbits, order, ccnt,
-                                       coff, stmp, e0, e1, acc.nodes, acc.leaves, acc.height, s);
+                pzk::build_big_chunked(D->v, D->idx, N.obj_of_loc, N.seg_off, P.first_tri, P.obox, (uint32_t)T.nbig,
+                                       nb_tri, sk, mbits, order, ccnt, coff, stmp, e0, e1, acc.nodes, acc.leaves,
+                                       acc.height, incoherent, s);
             }
             static const char* const kTimer[pzk::kSmallClasses] = {"build_small.c0", "build_small.c1",
                                                                     "build_small.c2"};
@@ -737,8 +749,18 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
             acc.edges = nullptr;
             return acc;
         };
-        SetAccel xa = build(A, pa, na, ta, bigA, "A");
-        SetAccel xb = build(B, pb, nb, tb, bigB, "B");
+        uint32_t* incoherent = buf<uint32_t>(c, "big.incoherent", 2);
+        const bool natA = ta.nbig_tri > 0 && ta.nscattered == 0, natB = tb.nbig_tri > 0 && tb.nscattered == 0;
+        if (natA || natB) pzk::fill_u32(incoherent, 0u, 2, s);
+        SetAccel xa = build(A, pa, na, ta, bigA, "A", incoherent, false);
+        SetAccel xb = build(B, pb, nb, tb, bigB, "B", incoherent + 1, false);
+        r->chunk_sorted_sets = (ta.nbig_tri > 0 && !natA) + (tb.nbig_tri > 0 && !natB);
+        if (natA || natB) {  // a set with an incoherent big object is built again from the sorted order
+            const std::vector<uint32_t> bad = readback<uint32_t>(c, {incoherent, incoherent + 1});
+            if (bad[0]) xa = build(A, pa, na, ta, bigA, "A", nullptr, true);
+            if (bad[1]) xb = build(B, pb, nb, tb, bigB, "B", nullptr, true);
+            r->chunk_sorted_sets += (bad[0] != 0) + (bad[1] != 0);
+        }
         r->index_triangles = ta.ntri + tb.ntri;
 
         ph.begin("Narrow Phase");

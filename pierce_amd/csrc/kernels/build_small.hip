@@ -79,6 +79,9 @@ __global__ void k_need_split(const uint32_t* __restrict__ need, const uint32_t* 
         big[o] = (need[o] && c < 0) ? 1u : 0u;
         for (int k = 0; k < pzk::kSmallClasses; ++k) cls[(uint64_t)k * (nobj + 1) + o] = (c == k) ? 1u : 0u;
     }
+    // needed objects whose triangles are not contiguous (they need the sorted big-object build)
+    const uint64_t nc = __ballot(o < nobj && need[o] && !(n > 0 && last_tri[o] - first_tri[o] + 1 == n));
+    if (__lane_id() == 0 && nc) atomicAdd(&totals[8], (uint32_t)__popcll(nc));
     // largest object per class: wave maximum first, one atomic per wave and class
     for (int k = 0; k < pzk::kSmallClasses; ++k) {
         uint32_t m = c == k ? n : 0u;
@@ -150,13 +153,17 @@ __device__ __forceinline__ uint32_t seg_of_chunk(const uint32_t* __restrict__ of
     return lo;
 }
 
-template <int T, bool CHUNK>
+// MODE 0: one object per workgroup; 1: a chunk of the sorted big objects;
+// 2: a chunk of a big object in its own triangle order.
+template <int T, int MODE>
 __global__ __launch_bounds__(T) void k_build_small(
     const float* __restrict__ verts, const uint32_t* __restrict__ idx, const int32_t* __restrict__ obj_of_loc,
     const uint32_t* __restrict__ first_tri, const uint32_t* __restrict__ tri_cnt, const uint32_t* __restrict__ obox, const uint32_t* __restrict__ seg_off,
     uint32_t l_begin, int cap, BNode* __restrict__ nodes, LeafTri* __restrict__ leaves, uint32_t* __restrict__ height_out,
     ChunkSrc ch) {
+    constexpr bool CHUNK = MODE != 0;
     constexpr int PR = T == 1024 ? 2 : (T == 512 ? 3 : 4);  // local triangles per thread kept in registers
+    static_assert(MODE != 2 || PR * T >= (int)kCh, "a natural-order chunk is held in registers whole");
     constexpr int NP = (kMax / T + 1) / 2;                 // sorted positions per thread, two per register
     extern __shared__ __align__(16) unsigned char lds[];
     const int capk = (cap + 3) & ~3;
@@ -168,7 +175,7 @@ __global__ __launch_bounds__(T) void k_build_small(
     // a sort key's Morton part, top-aligned in 30 bits
     auto code30 = [&](uint64_t k) { return ((uint32_t)k & ((1u << ch.mbits) - 1u)) << (30 - ch.mbits); };
     // leaves at lbase + sorted position, nodes at nbase + node index (root at nbase)
-    uint32_t l, t0, lbase, nbase, g = 0, cpl = 0;
+    uint32_t l, t0, lbase, nbase, g = 0, cpl = 0, tn0 = 0;
     int32_t o;
     int n;
     if constexpr (CHUNK) {
@@ -182,9 +189,13 @@ __global__ __launch_bounds__(T) void k_build_small(
         lbase = t0;
         nbase = nch == 1 ? t0 : t0 + 1;  // slot t0 belongs to the top tree
         o = obj_of_loc[l];
-        // codes of a sorted run share their leading bits: keep the 21 below them
-        const uint32_t x = code30(ch.keys[t0] ^ ch.keys[t0 + n - 1]);
-        cpl = x ? (uint32_t)__clz((int)x) - 2u : 30u;
+        if constexpr (MODE == 1) {
+            // codes of a sorted run share their leading bits: keep the 21 below them
+            const uint32_t x = code30(ch.keys[t0] ^ ch.keys[t0 + n - 1]);
+            cpl = x ? (uint32_t)__clz((int)x) - 2u : 30u;
+        } else {
+            tn0 = first_tri[o] + j * kCh;  // the chunk's first triangle
+        }
     } else {
         l = l_begin + blockIdx.x;
         o = obj_of_loc[l];
@@ -204,7 +215,7 @@ __global__ __launch_bounds__(T) void k_build_small(
     int npow = 1;
     while (npow < n) npow <<= 1;
     auto morton = [&](const float3& a, const float3& b, const float3& c, int q) {
-        if constexpr (CHUNK) {  // ascending in q: the run is sorted already
+        if constexpr (MODE == 1) {  // ascending in q: the run is sorted already
             const uint32_t m = ((code30(ch.keys[t0 + q]) << cpl) & kM30) >> 9;
             return (m << 11) | (uint32_t)q;
         }
@@ -217,7 +228,9 @@ __global__ __launch_bounds__(T) void k_build_small(
             m = (m << 3) | (((qq[0] >> b2) & 1u) << 2) | (((qq[1] >> b2) & 1u) << 1) | ((qq[2] >> b2) & 1u);
         return (m << 11) | (uint32_t)q;
     };
-    auto tri_id = [&](int q) { return CHUNK ? ch.order[t0 + (uint32_t)q] : t0 + (uint32_t)q; };
+    auto tri_id = [&](int q) {
+        return MODE == 1 ? ch.order[t0 + (uint32_t)q] : (MODE == 2 ? tn0 : t0) + (uint32_t)q;
+    };
     auto tri = [&](int q, float3& a, float3& b, float3& c) {
         const uint32_t t = tri_id(q);
         const uint32_t i0 = idx[3ull * t], i1 = idx[3ull * t + 1], i2 = idx[3ull * t + 2];
@@ -231,6 +244,39 @@ __global__ __launch_bounds__(T) void k_build_small(
     for (int k = 0; k < PR; ++k) {
         const int q = tid + k * T;
         if (q < n) tri(q, ra[k], rb[k], rc[k]);
+    }
+    if constexpr (MODE == 2) {  // the Morton grid over the chunk's centroid bounds
+        __shared__ uint32_t cbox[6];
+        if (tid < 6) cbox[tid] = tid < 3 ? 0xFFFFFFFFu : 0u;
+        float cl[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+#pragma unroll
+        for (int k = 0; k < PR; ++k)
+            if (tid + k * T < n) {
+                const float cx[3] = {(ra[k].x + rb[k].x + rc[k].x) * (1.0f / 3.0f),
+                                     (ra[k].y + rb[k].y + rc[k].y) * (1.0f / 3.0f),
+                                     (ra[k].z + rb[k].z + rc[k].z) * (1.0f / 3.0f)};
+                for (int a = 0; a < 3; ++a) {
+                    cl[a] = fminf(cl[a], cx[a]);
+                    chi[a] = fmaxf(chi[a], cx[a]);
+                }
+            }
+        for (int a = 0; a < 3; ++a)
+            for (int sft = 32; sft > 0; sft >>= 1) {
+                cl[a] = fminf(cl[a], __shfl_xor(cl[a], sft, 64));
+                chi[a] = fmaxf(chi[a], __shfl_xor(chi[a], sft, 64));
+            }
+        __syncthreads();
+        if (lane == 0 && tid < n)
+            for (int a = 0; a < 3; ++a) {
+                atomicMin(&cbox[a], f2ord(cl[a]));
+                atomicMax(&cbox[3 + a], f2ord(chi[a]));
+            }
+        __syncthreads();
+        for (int a = 0; a < 3; ++a) {
+            const float bl = ord2f(cbox[a]), bh = ord2f(cbox[3 + a]);
+            lo[a] = bl;
+            sc[a] = bh > bl ? 127.0f / (bh - bl) : 0.0f;
+        }
     }
 #pragma unroll
     for (int k = 0; k < PR; ++k) {
@@ -347,7 +393,7 @@ __global__ __launch_bounds__(T) void k_build_small(
             if (pb + 192 < n) key[pb + 192] = k3;
         }
     };
-    if (!CHUNK && n > 1) {
+    if (MODE != 1 && n > 1) {
         reg_pass(2, min(npow, 256));
         for (int s = 512; s <= npow; s <<= 1) {
             __syncthreads();
@@ -541,6 +587,37 @@ __global__ void k_chunk_count(const uint32_t* __restrict__ seg_off, uint32_t nbi
     if (l < nbig) cnt[l] = (seg_off[l + 1] - seg_off[l] + kCh - 1u) / kCh;
 }
 
+// Whether each big object's own triangle order is spatially coherent enough
+// for its chunks: thread per object, the chunk boxes' surface areas summed in
+// chunk order against kCoherentArea times its box's.  A coherent order (a
+// surface walked patch by patch, a tube ring by ring) sums to about one box
+// area; chunks drawn from all over the object each cover about the whole
+// box, K of them K areas.  Objects that fail are counted in *incoherent.
+constexpr float kCoherentArea = 4.0f;
+__device__ __forceinline__ float box_area(const float* lo, const float* hi) {
+    const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    return 2.0f * (dx * dy + dy * dz + dz * dx);
+}
+__global__ void k_chunk_check(const uint32_t* __restrict__ chunk_off, uint32_t nbig,
+                              const int32_t* __restrict__ obj_of_loc, const uint32_t* __restrict__ obox,
+                              const pzk::ChunkRoot* __restrict__ ent, uint32_t* __restrict__ incoherent) {
+    const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+    bool bad = false;
+    if (l < nbig && chunk_off[l + 1] - chunk_off[l] >= 2) {
+        float sum = 0.0f;
+        for (uint32_t g = chunk_off[l]; g < chunk_off[l + 1]; ++g) sum += box_area(ent[g].lo, ent[g].hi);
+        const int32_t o = obj_of_loc[l];
+        float lo[3], hi[3];
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = ord2f(obox[6ll * o + k]);
+            hi[k] = ord2f(obox[6ll * o + 3 + k]);
+        }
+        bad = !(sum <= kCoherentArea * box_area(lo, hi));
+    }
+    const uint64_t m = __ballot(bad);
+    if (__lane_id() == 0 && m) atomicAdd(incoherent, (uint32_t)__popcll(m));
+}
+
 // One level of the top trees: an object with np >= 2 entries left (np =
 // ceil(K / 2^(level-1))) pairs entries (2i, 2i+1) under a new node and moves
 // an odd last entry up.  Nodes are numbered from the root down: the nodes
@@ -618,7 +695,7 @@ bool build_small(const float* verts, const uint32_t* idx, const int32_t* obj_of_
     KernelTimer kt_(timer ? timer : "build_small", s);
     KernelTimer kt_all_("build_small", s);
 #define PZ_LAUNCH_BS(T)                                                                                         \
-    hipLaunchKernelGGL((k_build_small<T, false>), dim3(count), dim3(T), lds, s, verts, idx, obj_of_loc, first_tri, \
+    hipLaunchKernelGGL((k_build_small<T, 0>), dim3(count), dim3(T), lds, s, verts, idx, obj_of_loc, first_tri, \
                        tri_cnt, obox, seg_off, l_begin, (int)cap, nodes, leaves, height, ChunkSrc{})
     switch (threads) {
         case 128: PZ_LAUNCH_BS(128); break;
@@ -634,20 +711,30 @@ bool build_small(const float* verts, const uint32_t* idx, const int32_t* obj_of_
 uint64_t big_chunk_bound(uint64_t nbig, uint64_t nbig_tri) { return nbig_tri / kChunk + nbig; }
 
 void build_big_chunked(const float* verts, const uint32_t* idx, const int32_t* obj_of_loc, const uint32_t* seg_off,
-                       uint32_t nbig, uint64_t nbig_tri, const uint64_t* sorted_keys, int mbits, const uint32_t* order,
-                       uint32_t* chunk_cnt, uint32_t* chunk_off, void* scan_tmp, ChunkRoot* ent0, ChunkRoot* ent1,
-                       BNode* nodes, LeafTri* leaves, uint32_t* height, hipStream_t s) {
+                       const uint32_t* first_tri, const uint32_t* obox, uint32_t nbig, uint64_t nbig_tri,
+                       const uint64_t* sorted_keys, int mbits, const uint32_t* order, uint32_t* chunk_cnt,
+                       uint32_t* chunk_off, void* scan_tmp, ChunkRoot* ent0, ChunkRoot* ent1, BNode* nodes,
+                       LeafTri* leaves, uint32_t* height, uint32_t* incoherent, hipStream_t s) {
     if (nbig == 0 || nbig_tri == 0) return;
     const uint64_t bound = big_chunk_bound(nbig, nbig_tri);
     hipLaunchKernelGGL(k_chunk_count, dim3(grid1(nbig)), dim3(256), 0, s, seg_off, nbig, chunk_cnt);
     exclusive_scan_u32_u32(chunk_cnt, chunk_off, nbig, scan_tmp, s);
     {
         KernelTimer kt_("build_chunks", s);
-        hipLaunchKernelGGL((k_build_small<kChThreads, true>), dim3((unsigned)bound), dim3(kChThreads),
-                           build_small_lds(kChunk), s, verts, idx, obj_of_loc, nullptr, nullptr, nullptr, seg_off, 0u,
-                           (int)kChunk, nodes,
-                           leaves, height, ChunkSrc{sorted_keys, order, chunk_off, nbig, mbits, ent0});
+        if (sorted_keys)
+            hipLaunchKernelGGL((k_build_small<kChThreads, 1>), dim3((unsigned)bound), dim3(kChThreads),
+                               build_small_lds(kChunk), s, verts, idx, obj_of_loc, nullptr, nullptr, nullptr, seg_off,
+                               0u, (int)kChunk, nodes, leaves, height,
+                               ChunkSrc{sorted_keys, order, chunk_off, nbig, mbits, ent0});
+        else  // chunks in each object's own triangle order
+            hipLaunchKernelGGL((k_build_small<kChThreads, 2>), dim3((unsigned)bound), dim3(kChThreads),
+                               build_small_lds(kChunk), s, verts, idx, obj_of_loc, first_tri, nullptr, nullptr,
+                               seg_off, 0u, (int)kChunk, nodes, leaves, height,
+                               ChunkSrc{nullptr, nullptr, chunk_off, nbig, 0, ent0});
     }
+    if (!sorted_keys)
+        hipLaunchKernelGGL(k_chunk_check, dim3(grid1(nbig)), dim3(256), 0, s, chunk_off, nbig, obj_of_loc, obox, ent0,
+                           incoherent);
     // an object has at most ceil(nbig_tri / kChunk) chunks
     const uint64_t kmax = (nbig_tri + kChunk - 1) / kChunk;
     uint32_t levels = 0;

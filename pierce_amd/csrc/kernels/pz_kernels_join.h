@@ -122,11 +122,18 @@ struct ChunkRoot {
 // Scratch of build_big_chunked: chunk counts and offsets (nbig + 1 each) and
 // two ChunkRoot arrays of big_chunk_bound entries.
 uint64_t big_chunk_bound(uint64_t nbig, uint64_t nbig_tri);
-// sorted_keys = (segment << mbits) | the top mbits of the 30-bit Morton code.
+// sorted_keys = (segment << mbits) | the top mbits of the 30-bit Morton code,
+// order = the triangle at each sorted position.  Null sorted_keys: every big
+// object stores its triangles contiguously (from first_tri), its chunks are
+// runs of its own triangle order, each sorted in LDS, and the objects whose
+// order is not spatially coherent (chunk boxes against the object's box in
+// obox) are counted into *incoherent: their trees are valid but loose, and
+// the caller rebuilds such a set from the sorted order.
 void build_big_chunked(const float* verts, const uint32_t* idx, const int32_t* obj_of_loc, const uint32_t* seg_off,
-                       uint32_t nbig, uint64_t nbig_tri, const uint64_t* sorted_keys, int mbits, const uint32_t* order,
-                       uint32_t* chunk_cnt, uint32_t* chunk_off, void* scan_tmp, ChunkRoot* ent0, ChunkRoot* ent1,
-                       pzd::BNode* nodes, pzd::LeafTri* leaves, uint32_t* height, hipStream_t s);
+                       const uint32_t* first_tri, const uint32_t* obox, uint32_t nbig, uint64_t nbig_tri,
+                       const uint64_t* sorted_keys, int mbits, const uint32_t* order, uint32_t* chunk_cnt,
+                       uint32_t* chunk_off, void* scan_tmp, ChunkRoot* ent0, ChunkRoot* ent1, pzd::BNode* nodes,
+                       pzd::LeafTri* leaves, uint32_t* height, uint32_t* incoherent, hipStream_t s);
 void edge_count(const int32_t* eo, uint64_t ne, int32_t nobj, uint32_t* edge_cnt, hipStream_t s);
 void scene_box(const uint32_t* obox, int32_t nobj, uint32_t* scene, hipStream_t s);
 // keys = (part_of[o] << 30) | Morton of the box centre (part 0 when part_of is null)

@@ -311,7 +311,49 @@ def test_shuffled_triangle_order(ctx):
         exp = orc.face_overlap(a, b) if predicate == pz.PRED_OVERLAP else orc.predicate_join(a, b, predicate)[0]
         print(f"[shuffled] predicate={predicate} expected={len(exp)}", flush=True)
         assert len(exp) > 0 or predicate == pz.PRED_CONTAINMENT
-        np.testing.assert_array_equal(pz.join(ctx, da, db, predicate=predicate).pairs, exp)
+        out = pz.join(ctx, da, db, predicate=predicate)
+        np.testing.assert_array_equal(out.pairs, exp)
+        assert out.chunk_sorted_sets == 2  # objects not contiguous: chunks of the sorted order
+
+
+def _spheres_up_to_level5(seed):
+    return pz.generate(pz.GEN_SPHERES_MIXED, count=60, seed=seed, level_min=1, level_max=5, radius_min=1.0,
+                       radius_max=4.0, box=30.0)
+
+
+def test_object_chunks_in_own_order_and_incoherent_fallback(ctx):
+    """Objects over 2,048 triangles stored contiguously are cut into chunks
+    of their own triangle order (the spheres' subdivision order is coherent
+    enough).  With each object's triangles shuffled in place they stay
+    contiguous but incoherent: the 20,480-triangle spheres' ten chunks each
+    span the whole sphere, the check after the chunk builds counts them, and
+    the set is built again from the sorted order.  Both give the oracle's
+    pairs."""
+    rng = np.random.default_rng(7)
+    plain, local = [], []
+    for seed in (51, 52):
+        m = _spheres_up_to_level5(seed)
+        t2o = m.triangle_to_object
+        starts = np.flatnonzero(np.r_[True, t2o[1:] != t2o[:-1]])
+        ends = np.r_[starts[1:], len(t2o)]
+        assert len(np.unique(t2o)) == len(starts)  # contiguous objects
+        p = np.concatenate([s + rng.permutation(e - s) for s, e in zip(starts, ends)])
+        plain.append(m)
+        local.append(pz.extract_edges(pz.Mesh(m.vertices, m.indices[p], t2o[p])))
+    assert np.bincount(plain[0].triangle_to_object).max() == 20480
+    for (a, b), name in ((plain, "own order"), (local, "shuffled in place")):
+        da, db = ctx.upload(a), ctx.upload(b)
+        for predicate in (pz.PRED_OVERLAP, pz.PRED_CONTAINMENT):
+            exp = orc.face_overlap(a, b) if predicate == pz.PRED_OVERLAP else orc.predicate_join(a, b, predicate)[0]
+            out = pz.join(ctx, da, db, predicate=predicate)
+            print(f"[own-order chunks] {name} predicate={predicate} expected={len(exp)} "
+                  f"sorted sets={out.chunk_sorted_sets}", flush=True)
+            np.testing.assert_array_equal(out.pairs, exp)
+            if name == "own order":
+                assert out.chunk_sorted_sets == 0
+            else:
+                assert out.chunk_sorted_sets >= 1
+            np.testing.assert_array_equal(pz.join(ctx, da, db, predicate=predicate, narrow_mode=2).pairs, exp)
 
 
 @pytest.mark.parametrize("count", [6000, 70000])  # two sort tiles; beyond the tiles (radix sort)
