@@ -37,12 +37,15 @@
 // (4 B), boxes (24 B) and the exchange words (4 B), which hold the sorted
 // positions until the climb starts.
 //
-// Chunk mode (objects over 2,048 triangles or not contiguous): the big
-// objects' triangles are Morton-sorted once over the whole set; each
-// workgroup takes a run of up to 2,048 sorted positions of one object, whose
-// keys are already ascending (step 2 is skipped), writes its root for the
-// per-object top tree (k_top_level), and leaves the top tree one node slot
-// per chunk.
+// Chunk modes (objects over 2,048 triangles or not contiguous), one
+// workgroup per run of up to kChunk triangles of one object, which writes its
+// root for the per-object top tree (k_top_level) and leaves the top tree one
+// node slot per chunk:
+//  - own order (every big object contiguous): a run of the object's own
+//    triangles; the Morton grid is over the run's centroid bounds (a block
+//    reduction after the gather) and the run is sorted as in step 2;
+//  - sorted: the big objects' triangles are Morton-sorted once over the
+//    whole set, a run's keys are already ascending and step 2 is skipped.
 #include <type_traits>
 
 #include "pz_device.h"
@@ -589,11 +592,12 @@ __global__ void k_chunk_count(const uint32_t* __restrict__ seg_off, uint32_t nbi
 
 // Whether each big object's own triangle order is spatially coherent enough
 // for its chunks: thread per object, the chunk boxes' surface areas summed in
-// chunk order against kCoherentArea times its box's.  A coherent order (a
-// surface walked patch by patch, a tube ring by ring) sums to about one box
-// area; chunks drawn from all over the object each cover about the whole
-// box, K of them K areas.  Objects that fail are counted in *incoherent.
-constexpr float kCoherentArea = 4.0f;
+// chunk order against kCoherentArea times its box's.  A coherent order sums
+// to a few box areas (tubes generated ring by ring 0.5-1.2, subdivided
+// spheres 3.3-3.7 at 1,024-triangle chunks); chunks drawn from all over the
+// object each cover about the whole box, K of them K areas.  Objects that
+// fail are counted in *incoherent.
+constexpr float kCoherentArea = 6.0f;
 __device__ __forceinline__ float box_area(const float* lo, const float* hi) {
     const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
     return 2.0f * (dx * dy + dy * dz + dz * dx);

@@ -105,16 +105,19 @@ bool build_small(const float* verts, const uint32_t* idx, const int32_t* obj_of_
                  const uint32_t* first_tri, const uint32_t* tri_cnt, const uint32_t* obox, const uint32_t* seg_off,
                  uint32_t l_begin, uint32_t count, uint32_t cap, int threads, pzd::BNode* nodes, pzd::LeafTri* leaves,
                  uint32_t* height, const char* timer, hipStream_t s);
-// Big objects (segments [0, nbig), triangles Morton-sorted per segment into
-// keys / order by the global pipeline's sort) are built in chunks of
-// kChunk consecutive sorted triangles, each chunk by the LDS builder, and
-// joined by a balanced top tree per object whose K - 1 nodes take the node
-// slots seg_off[l] + j * kChunk (j < K - 1; the chunks' own nodes start one
-// slot later), so the object's root stays at its segment's first node.
-// 2,048 / 1,024 measured faster than 1,024 / 512 (reduced C4: chunk builds
-// 3.98 against 4.07 ms, narrow phase 0.72 against 0.77 ms).
-constexpr uint32_t kChunk = 2048;    // sorted triangles per chunk (<= kSmallMax)
-constexpr int kChunkThreads = 1024;  // one chunk build's workgroup
+// Big objects (segments [0, nbig)) are built in chunks of kChunk consecutive
+// triangles, each chunk by the LDS builder, joined by a balanced top tree per
+// object whose K - 1 nodes take the node slots seg_off[l] + j * kChunk
+// (j < K - 1; the chunks' own nodes start one slot later), so the object's
+// root stays at its segment's first node.  The chunks are runs of each
+// object's own triangle order (sorted in LDS), or of a Morton order of the
+// set's big objects (keys / order of the global pipeline's sort) when some
+// are not contiguous or an object's own order is not spatially coherent.
+// Chunk / workgroup sizes measured on 2,000 vs 2,000 tubes with own-order
+// chunks: 2,048 / 1,024 build 38.9 ms + walk 31.0 ms, 1,024 / 512 30.9 +
+// 32.0 ms, 512 / 256 28.5 + 34.5 ms, 1,024 / 256 34.2 + 31.9 ms.
+constexpr uint32_t kChunk = 1024;   // triangles per chunk (<= kSmallMax)
+constexpr int kChunkThreads = 512;  // one chunk build's workgroup
 struct ChunkRoot {
     float lo[3], hi[3];
     int32_t link, pad;  // as in Node: >= 0 node, < 0 leaf ~index
