@@ -344,12 +344,17 @@ __global__ __launch_bounds__(kBlock) void k_object_keys(const uint32_t* __restri
 }
 
 // --------------------------------------------------------------- broad phase
-// Thread per mesh1 object: walk the mesh2 object BVH with its box.
-// mode 0: count into cnt[a]; mode 1: write at off[a]; mode 2: append with
-// capacity (cursor counts every pair, writes only below cap).
-// Mode 0: count pairs per mesh1 object; 1: write them at off[a]; 2: append
-// into a capacity-sized buffer (one cursor reservation per wavefront: each
-// lane counts first, the wave reserves its total, then lanes write).
+// A wave per mesh1 object walks the mesh2 object BVH of its part with its box:
+// lane L takes the subtree reached by the bits of L from the part's root
+// (kSplitDepth levels; a leaf met on the way belongs to the lane whose
+// remaining bits are zero), so an object whose box meets thousands of boxes
+// (a long branching tube) is walked by 64 lanes, not one.  The lanes step in
+// lockstep, one node each, and place each step's hits by ballot prefix, so
+// the output order is fixed by the tree.
+// Mode 0: count pairs per mesh1 object into cnt[a]; 1: write them at off[a];
+// 2: append into a capacity-sized buffer (one cursor reservation per step
+// with hits; the cursor counts every pair, writes land only below cap).
+constexpr int kSplitDepth = 6;  // 2^6 = PZ_WAVE subtrees
 template <int Mode>
 __global__ __launch_bounds__(kBlock) void k_broad(const uint32_t* __restrict__ oboxA, int32_t nA,
                                                   const uint32_t* __restrict__ oboxB,
@@ -358,89 +363,123 @@ __global__ __launch_bounds__(kBlock) void k_broad(const uint32_t* __restrict__ o
                                                   const uint32_t* __restrict__ boff, uint32_t* __restrict__ cnt,
                                                   const uint64_t* __restrict__ off, uint64_t* __restrict__ out,
                                                   unsigned long long* __restrict__ cursor, uint64_t cap) {
-    const int32_t a = blockIdx.x * kBlock + threadIdx.x;
-    if (Mode != 2 && a >= nA) return;  // append keeps every lane for the wave-wide reservation
-    bool active = a < nA;
+    static_assert((1 << kSplitDepth) == PZ_WAVE, "one subtree per lane");
+    const int32_t a = (int32_t)(blockIdx.x * kWavesPerBlock + threadIdx.x / PZ_WAVE);
+    if (a >= nA) return;  // wave-uniform
+    const int lane = __lane_id();
     // mesh2 objects of this object's part occupy sorted positions [b0, b0 + nb)
-    const uint32_t part = (active && part_of_a) ? part_of_a[a] : 0u;
+    const uint32_t part = part_of_a ? part_of_a[a] : 0u;
     const int32_t b0 = boff ? (int32_t)boff[part] : 0;
     const int32_t nb = boff ? (int32_t)(boff[part + 1] - boff[part]) : nB;
-    float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {-1.f, -1.f, -1.f};
-    if (active) {
-        for (int k = 0; k < 3; ++k) {
-            lo[k] = ord2f(oboxA[6ll * a + k]);
-            hi[k] = ord2f(oboxA[6ll * a + 3 + k]);
-        }
+    float lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = ord2f(oboxA[6ll * a + k]);
+        hi[k] = ord2f(oboxA[6ll * a + 3 + k]);
     }
-    active = active && nb > 0 && lo[0] <= hi[0] && lo[1] <= hi[1] && lo[2] <= hi[2];
-    // Visit every mesh2 object whose box meets this object's box.
-    auto traverse = [&](auto&& emit) {
-        if (nb == 1) {  // single object: no internal node; test its box directly
-            const int32_t b = (int32_t)orderB[b0];
-            const uint32_t* bb = oboxB + 6ll * b;
-            if (boxes_meet(lo, hi, ord2f(bb[0]), ord2f(bb[1]), ord2f(bb[2]), ord2f(bb[3]), ord2f(bb[4]),
-                           ord2f(bb[5])))
-                emit(b);
-            return;
-        }
-        int stack[64];
-        int sp = 0;
-        int cur = b0;
-        while (true) {
-            const Node nd = nodesB[cur];
-            bool h0 = boxes_meet(lo, hi, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
-            bool h1 = boxes_meet(lo, hi, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w);
-            const int c0 = nd.d.x, c1 = nd.d.y;
-            if (h0 && c0 < 0) {
-                emit((int32_t)orderB[~c0]);
-                h0 = false;
-            }
-            if (h1 && c1 < 0) {
-                emit((int32_t)orderB[~c1]);
-                h1 = false;
-            }
-            if (h0 && h1) {
-                stack[sp++] = c1;
-                cur = c0;
-            } else if (h0) {
-                cur = c0;
-            } else if (h1) {
-                cur = c1;
+    if (!(nb > 0 && lo[0] <= hi[0] && lo[1] <= hi[1] && lo[2] <= hi[2])) {  // wave-uniform
+        if (Mode == 0 && lane == 0) cnt[a] = 0u;
+        return;
+    }
+    uint32_t n = 0;       // Mode 0: this lane's hits
+    uint64_t placed = 0;  // Mode 1: hits the wave has written
+    // one step's hits of every lane, placed in lane order (first hits, then second hits)
+    auto emit = [&](bool v0, int32_t e0, bool v1, int32_t e1) {
+        if constexpr (Mode == 0) {
+            n += (uint32_t)v0 + (uint32_t)v1;
+        } else {
+            const uint64_t m0 = __ballot(v0), m1 = __ballot(v1);
+            const uint32_t t0 = (uint32_t)__popcll(m0), tot = t0 + (uint32_t)__popcll(m1);
+            if (tot == 0) return;  // wave-uniform
+            uint64_t base;
+            if constexpr (Mode == 1) {
+                base = off[a] + placed;
+                placed += tot;
             } else {
-                if (sp == 0) break;
-                cur = stack[--sp];
+                unsigned long long w = 0ull;
+                if (lane == 0) w = atomicAdd(cursor, (unsigned long long)tot);
+                base = (uint64_t)__shfl((long long)w, 0, PZ_WAVE);
             }
+            const uint64_t p0 = base + lanes_below(m0), p1 = base + t0 + lanes_below(m1);
+            if (v0 && (Mode == 1 || p0 < cap)) out[p0] = pack_pair(a, e0);
+            if (v1 && (Mode == 1 || p1 < cap)) out[p1] = pack_pair(a, e1);
         }
     };
-    uint32_t n = 0;
-    if (Mode == 0) {
-        if (active) traverse([&](int32_t) { ++n; });
-        cnt[a] = n;
-    } else if (Mode == 1) {
-        if (active) {
-            const uint64_t base = off[a];
-            traverse([&](int32_t b) { out[base + n++] = pack_pair(a, b); });
+    if (nb == 1) {  // single object: no internal node; test its box directly
+        bool v = false;
+        int32_t b = 0;
+        if (lane == 0) {
+            b = (int32_t)orderB[b0];
+            const uint32_t* bb = oboxB + 6ll * b;
+            v = boxes_meet(lo, hi, ord2f(bb[0]), ord2f(bb[1]), ord2f(bb[2]), ord2f(bb[3]), ord2f(bb[4]), ord2f(bb[5]));
         }
+        emit(v, b, false, 0);
     } else {
-        if (active) traverse([&](int32_t) { ++n; });
-        const int lane = __lane_id();
-        uint32_t incl = n;  // inclusive prefix over the wave
-        for (int sft = 1; sft < PZ_WAVE; sft <<= 1) {
-            const uint32_t u = (uint32_t)__shfl_up((int)incl, sft, PZ_WAVE);
-            if (lane >= sft) incl += u;
+        // descent to this lane's subtree
+        bool alive = true;
+        int cur = b0;
+        for (int d = 0; d < kSplitDepth; ++d) {
+            bool v = false;
+            int32_t b = 0;
+            if (alive) {
+                const Node nd = nodesB[cur];
+                const int bit = (lane >> (kSplitDepth - 1 - d)) & 1;
+                const bool h = bit ? boxes_meet(lo, hi, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w)
+                                   : boxes_meet(lo, hi, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
+                const int ch = bit ? nd.d.y : nd.d.x;
+                if (!h) {
+                    alive = false;
+                } else if (ch < 0) {  // a leaf above the split depth: one lane reports it
+                    alive = false;
+                    if ((lane & ((1 << (kSplitDepth - 1 - d)) - 1)) == 0) {
+                        v = true;
+                        b = (int32_t)orderB[~ch];
+                    }
+                } else {
+                    cur = ch;
+                }
+            }
+            emit(v, b, false, 0);
         }
-        const uint32_t total = (uint32_t)__shfl((int)incl, PZ_WAVE - 1, PZ_WAVE);
-        unsigned long long wbase = 0ull;
-        if (lane == 0 && total > 0) wbase = atomicAdd(cursor, (unsigned long long)total);
-        wbase = __shfl(wbase, 0, PZ_WAVE);
-        const uint64_t base = wbase + incl - n;
-        if (active && n > 0 && base < cap) {
-            uint32_t i = 0;
-            traverse([&](int32_t b) {
-                if (base + i < cap) out[base + i] = pack_pair(a, b);
-                ++i;
-            });
+        // the subtree below cur (its own box meets), depth first
+        int stack[64];
+        int sp = 0;
+        while (__ballot(alive) != 0) {
+            bool v0 = false, v1 = false;
+            int32_t e0 = 0, e1 = 0;
+            if (alive) {
+                const Node nd = nodesB[cur];
+                bool h0 = boxes_meet(lo, hi, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
+                bool h1 = boxes_meet(lo, hi, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w);
+                const int c0 = nd.d.x, c1 = nd.d.y;
+                if (h0 && c0 < 0) {
+                    v0 = true;
+                    e0 = (int32_t)orderB[~c0];
+                    h0 = false;
+                }
+                if (h1 && c1 < 0) {
+                    v1 = true;
+                    e1 = (int32_t)orderB[~c1];
+                    h1 = false;
+                }
+                if (h0 && h1) {
+                    stack[sp++] = c1;
+                    cur = c0;
+                } else if (h0) {
+                    cur = c0;
+                } else if (h1) {
+                    cur = c1;
+                } else if (sp == 0) {
+                    alive = false;
+                } else {
+                    cur = stack[--sp];
+                }
+            }
+            emit(v0, e0, v1, e1);
         }
+    }
+    if constexpr (Mode == 0) {
+        for (int sft = 32; sft > 0; sft >>= 1) n += (uint32_t)__shfl_xor((int)n, sft, PZ_WAVE);
+        if (lane == 0) cnt[a] = n;
     }
 }
 
@@ -1926,7 +1965,7 @@ void broad_count(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const
     if (nA > 0 && !nodesB)
         PZ_BRUTE(0, cnt, (const uint64_t*)nullptr, (uint64_t*)nullptr, (unsigned long long*)nullptr, (uint64_t)0);
     else if (nA > 0)
-        hipLaunchKernelGGL((k_broad<0>), dim3(grid1((uint64_t)nA)), dim3(kBlock), 0, s, oboxA, nA, oboxB, nodesB, orderB, nB,
+        hipLaunchKernelGGL((k_broad<0>), dim3(grid1((uint64_t)nA * PZ_WAVE)), dim3(kBlock), 0, s, oboxA, nA, oboxB, nodesB, orderB, nB,
                            part_of_a, boff, cnt, (const uint64_t*)nullptr, (uint64_t*)nullptr, (unsigned long long*)nullptr,
                            (uint64_t)0);
 }
@@ -1936,7 +1975,7 @@ void broad_write(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const
     if (nA > 0 && !nodesB)
         PZ_BRUTE(1, (uint32_t*)nullptr, off, out, (unsigned long long*)nullptr, (uint64_t)0);
     else if (nA > 0)
-        hipLaunchKernelGGL((k_broad<1>), dim3(grid1((uint64_t)nA)), dim3(kBlock), 0, s, oboxA, nA, oboxB, nodesB, orderB, nB,
+        hipLaunchKernelGGL((k_broad<1>), dim3(grid1((uint64_t)nA * PZ_WAVE)), dim3(kBlock), 0, s, oboxA, nA, oboxB, nodesB, orderB, nB,
                            part_of_a, boff, (uint32_t*)nullptr, off, out, (unsigned long long*)nullptr, (uint64_t)0);
 }
 void broad_append(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const Node* nodesB,
@@ -1945,7 +1984,7 @@ void broad_append(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, cons
     if (nA > 0 && !nodesB)
         PZ_BRUTE(2, (uint32_t*)nullptr, (const uint64_t*)nullptr, out, cursor, cap);
     else if (nA > 0)
-        hipLaunchKernelGGL((k_broad<2>), dim3(grid1((uint64_t)nA)), dim3(kBlock), 0, s, oboxA, nA, oboxB, nodesB, orderB, nB,
+        hipLaunchKernelGGL((k_broad<2>), dim3(grid1((uint64_t)nA * PZ_WAVE)), dim3(kBlock), 0, s, oboxA, nA, oboxB, nodesB, orderB, nB,
                            part_of_a, boff, (uint32_t*)nullptr, (const uint64_t*)nullptr, out, cursor, cap);
 }
 
