@@ -559,7 +559,10 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
             pzk::grid_append(pa.obox, nA, pb.obox, nB, og, pairs, cursor, cap, s);
         else
             pzk::broad_append(pa.obox, nA, pb.obox, ob.nodes, ob.order, nB, batched ? A->part_of : nullptr,
-                              batched ? B->part_off : nullptr, pairs, cursor, cap, s);
+                              batched ? B->part_off : nullptr, pairs, cursor, cap,
+                              buf<uint32_t>(c, "bp.cnt", (size_t)std::max(nA, 1)),
+                              buf<uint64_t>(c, "bp.off", (size_t)std::max(nA, 1) + 1),
+                              buf<char>(c, "scan.tmp", pzk::scan_temp_bytes((uint64_t)std::max(nA, 1))), s);
         np = readback<unsigned long long>(c, {cursor})[0];
         if (np > cap) {
             r->overflowed = 1;

@@ -351,9 +351,8 @@ __global__ __launch_bounds__(kBlock) void k_object_keys(const uint32_t* __restri
 // (a long branching tube) is walked by 64 lanes, not one.  The lanes step in
 // lockstep, one node each, and place each step's hits by ballot prefix, so
 // the output order is fixed by the tree.
-// Mode 0: count pairs per mesh1 object into cnt[a]; 1: write them at off[a];
-// 2: append into a capacity-sized buffer (one cursor reservation per step
-// with hits; the cursor counts every pair, writes land only below cap).
+// Mode 0: count pairs per mesh1 object into cnt[a]; 1: write them at off[a],
+// those below cap (the append form is a count, a scan and this write).
 constexpr int kSplitDepth = 6;  // 2^6 = PZ_WAVE subtrees
 template <int Mode>
 __global__ __launch_bounds__(kBlock) void k_broad(const uint32_t* __restrict__ oboxA, int32_t nA,
@@ -362,7 +361,8 @@ __global__ __launch_bounds__(kBlock) void k_broad(const uint32_t* __restrict__ o
                                                   int32_t nB, const uint32_t* __restrict__ part_of_a,
                                                   const uint32_t* __restrict__ boff, uint32_t* __restrict__ cnt,
                                                   const uint64_t* __restrict__ off, uint64_t* __restrict__ out,
-                                                  unsigned long long* __restrict__ cursor, uint64_t cap) {
+                                                  uint64_t cap) {
+    static_assert(Mode == 0 || Mode == 1, "count or write");
     static_assert((1 << kSplitDepth) == PZ_WAVE, "one subtree per lane");
     const int32_t a = (int32_t)(blockIdx.x * kWavesPerBlock + threadIdx.x / PZ_WAVE);
     if (a >= nA) return;  // wave-uniform
@@ -380,40 +380,21 @@ __global__ __launch_bounds__(kBlock) void k_broad(const uint32_t* __restrict__ o
         if (Mode == 0 && lane == 0) cnt[a] = 0u;
         return;
     }
-    uint32_t n = 0;       // Mode 0: this lane's hits
-    uint64_t placed = 0;  // Mode 1: hits the wave has written
-    // one step's hits of every lane, placed in lane order (first hits, then second hits)
-    auto emit = [&](bool v0, int32_t e0, bool v1, int32_t e1) {
-        if constexpr (Mode == 0) {
-            n += (uint32_t)v0 + (uint32_t)v1;
-        } else {
-            const uint64_t m0 = __ballot(v0), m1 = __ballot(v1);
-            const uint32_t t0 = (uint32_t)__popcll(m0), tot = t0 + (uint32_t)__popcll(m1);
-            if (tot == 0) return;  // wave-uniform
-            uint64_t base;
-            if constexpr (Mode == 1) {
-                base = off[a] + placed;
-                placed += tot;
-            } else {
-                unsigned long long w = 0ull;
-                if (lane == 0) w = atomicAdd(cursor, (unsigned long long)tot);
-                base = (uint64_t)__shfl((long long)w, 0, PZ_WAVE);
+    // Visits the part's tree, handing each step's hits of every lane to step():
+    // (v0, e0) and (v1, e1) are a lane's first and second hit of the step.
+    auto walk = [&](auto&& step) {
+        if (nb == 1) {  // single object: no internal node; test its box directly
+            bool v = false;
+            int32_t b = 0;
+            if (lane == 0) {
+                b = (int32_t)orderB[b0];
+                const uint32_t* bb = oboxB + 6ll * b;
+                v = boxes_meet(lo, hi, ord2f(bb[0]), ord2f(bb[1]), ord2f(bb[2]), ord2f(bb[3]), ord2f(bb[4]),
+                               ord2f(bb[5]));
             }
-            const uint64_t p0 = base + lanes_below(m0), p1 = base + t0 + lanes_below(m1);
-            if (v0 && (Mode == 1 || p0 < cap)) out[p0] = pack_pair(a, e0);
-            if (v1 && (Mode == 1 || p1 < cap)) out[p1] = pack_pair(a, e1);
+            step(v, b, false, 0);
+            return;
         }
-    };
-    if (nb == 1) {  // single object: no internal node; test its box directly
-        bool v = false;
-        int32_t b = 0;
-        if (lane == 0) {
-            b = (int32_t)orderB[b0];
-            const uint32_t* bb = oboxB + 6ll * b;
-            v = boxes_meet(lo, hi, ord2f(bb[0]), ord2f(bb[1]), ord2f(bb[2]), ord2f(bb[3]), ord2f(bb[4]), ord2f(bb[5]));
-        }
-        emit(v, b, false, 0);
-    } else {
         // descent to this lane's subtree
         bool alive = true;
         int cur = b0;
@@ -438,7 +419,7 @@ __global__ __launch_bounds__(kBlock) void k_broad(const uint32_t* __restrict__ o
                     cur = ch;
                 }
             }
-            emit(v, b, false, 0);
+            step(v, b, false, 0);
         }
         // the subtree below cur (its own box meets), depth first
         int stack[64];
@@ -474,12 +455,32 @@ __global__ __launch_bounds__(kBlock) void k_broad(const uint32_t* __restrict__ o
                     cur = stack[--sp];
                 }
             }
-            emit(v0, e0, v1, e1);
+            step(v0, e0, v1, e1);
         }
-    }
-    if constexpr (Mode == 0) {
+    };
+    // the wave's hits: counted, or written from base in step order and lane order
+    auto count = [&]() {
+        uint32_t n = 0;
+        walk([&](bool v0, int32_t, bool v1, int32_t) { n += (uint32_t)v0 + (uint32_t)v1; });
         for (int sft = 32; sft > 0; sft >>= 1) n += (uint32_t)__shfl_xor((int)n, sft, PZ_WAVE);
+        return n;
+    };
+    auto write = [&](uint64_t base, uint64_t limit) {
+        walk([&](bool v0, int32_t e0, bool v1, int32_t e1) {
+            const uint64_t m0 = __ballot(v0), m1 = __ballot(v1);
+            const uint32_t t0 = (uint32_t)__popcll(m0);
+            const uint64_t p0 = base + lanes_below(m0), p1 = base + t0 + lanes_below(m1);
+            if (v0 && p0 < limit) out[p0] = pack_pair(a, e0);
+            if (v1 && p1 < limit) out[p1] = pack_pair(a, e1);
+            base += t0 + (uint32_t)__popcll(m1);
+        });
+    };
+    if constexpr (Mode == 0) {
+        const uint32_t n = count();
         if (lane == 0) cnt[a] = n;
+    } else {
+        const uint64_t base = off[a];
+        if (base < cap) write(base, cap);
     }
 }
 
@@ -1966,8 +1967,7 @@ void broad_count(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const
         PZ_BRUTE(0, cnt, (const uint64_t*)nullptr, (uint64_t*)nullptr, (unsigned long long*)nullptr, (uint64_t)0);
     else if (nA > 0)
         hipLaunchKernelGGL((k_broad<0>), dim3(grid1((uint64_t)nA * PZ_WAVE)), dim3(kBlock), 0, s, oboxA, nA, oboxB, nodesB, orderB, nB,
-                           part_of_a, boff, cnt, (const uint64_t*)nullptr, (uint64_t*)nullptr, (unsigned long long*)nullptr,
-                           (uint64_t)0);
+                           part_of_a, boff, cnt, (const uint64_t*)nullptr, (uint64_t*)nullptr, (uint64_t)0);
 }
 void broad_write(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const Node* nodesB, const uint32_t* orderB,
                  int32_t nB, const uint32_t* part_of_a, const uint32_t* boff, const uint64_t* off, uint64_t* out,
@@ -1976,16 +1976,21 @@ void broad_write(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const
         PZ_BRUTE(1, (uint32_t*)nullptr, off, out, (unsigned long long*)nullptr, (uint64_t)0);
     else if (nA > 0)
         hipLaunchKernelGGL((k_broad<1>), dim3(grid1((uint64_t)nA * PZ_WAVE)), dim3(kBlock), 0, s, oboxA, nA, oboxB, nodesB, orderB, nB,
-                           part_of_a, boff, (uint32_t*)nullptr, off, out, (unsigned long long*)nullptr, (uint64_t)0);
+                           part_of_a, boff, (uint32_t*)nullptr, off, out, ~0ull);
 }
 void broad_append(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const Node* nodesB,
                   const uint32_t* orderB, int32_t nB, const uint32_t* part_of_a, const uint32_t* boff, uint64_t* out,
-                  unsigned long long* cursor, uint64_t cap, hipStream_t s) {
-    if (nA > 0 && !nodesB)
+                  unsigned long long* cursor, uint64_t cap, uint32_t* cnt, uint64_t* off, void* scan_tmp,
+                  hipStream_t s) {
+    if (nA > 0 && !nodesB) {
         PZ_BRUTE(2, (uint32_t*)nullptr, (const uint64_t*)nullptr, out, cursor, cap);
-    else if (nA > 0)
-        hipLaunchKernelGGL((k_broad<2>), dim3(grid1((uint64_t)nA * PZ_WAVE)), dim3(kBlock), 0, s, oboxA, nA, oboxB, nodesB, orderB, nB,
-                           part_of_a, boff, (uint32_t*)nullptr, (const uint64_t*)nullptr, out, cursor, cap);
+    } else if (nA > 0) {  // count, scan, write below cap: no atomics; the total is the scan's last entry
+        broad_count(oboxA, nA, oboxB, nodesB, orderB, nB, part_of_a, boff, cnt, s);
+        exclusive_scan_u32_u64(cnt, off, (uint64_t)nA, scan_tmp, s);
+        hipLaunchKernelGGL((k_broad<1>), dim3(grid1((uint64_t)nA * PZ_WAVE)), dim3(kBlock), 0, s, oboxA, nA, oboxB,
+                           nodesB, orderB, nB, part_of_a, boff, (uint32_t*)nullptr, off, out, cap);
+        hipMemcpyAsync(cursor, off + nA, sizeof(unsigned long long), hipMemcpyDeviceToDevice, s);
+    }
 }
 
 #undef PZ_BRUTE

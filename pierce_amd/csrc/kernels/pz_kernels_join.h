@@ -152,9 +152,12 @@ void broad_count(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const
 void broad_write(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const Node* nodesB, const uint32_t* orderB,
                  int32_t nB, const uint32_t* part_of_a, const uint32_t* boff, const uint64_t* off, uint64_t* out,
                  hipStream_t s);
+// Pairs into out[0, cap) with *cursor = their number (which may exceed cap);
+// cnt (nA), off (nA + 1) and scan_tmp (scan_temp_bytes(nA)) are scratch.
 void broad_append(const uint32_t* oboxA, int32_t nA, const uint32_t* oboxB, const Node* nodesB,
                   const uint32_t* orderB, int32_t nB, const uint32_t* part_of_a, const uint32_t* boff, uint64_t* out,
-                  unsigned long long* cursor, uint64_t cap, hipStream_t s);
+                  unsigned long long* cursor, uint64_t cap, uint32_t* cnt, uint64_t* off, void* scan_tmp,
+                  hipStream_t s);
 // Broad phase over a hashed grid of the mesh2 object boxes (non-batched sets):
 // cells are three times the boxes' mean extent per axis, hashed into nb (a power of two)
 // buckets; boxes over more than 27 cells are kept in `big` and tested by every
