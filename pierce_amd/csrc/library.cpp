@@ -33,6 +33,9 @@ struct pz_library {
     // per library object: runs of vertices, triangles and edges, largest |coordinate|
     std::vector<uint32_t> v_off, v_cnt, t_off, t_cnt, e_off, e_cnt;
     std::vector<float> max_abs;
+    // every object's triangles reference every vertex of its run: instanced
+    // sets then own vertex runs (their topology is known without a scan)
+    bool runs_full = true;
 };
 
 extern "C" {
@@ -74,6 +77,7 @@ pz_status pz_library_create(pz_ctx* c, const pz_mesh_view* v, pz_library** out) 
         }
         // vertices: each object's triangles use one run of vertices no other object uses
         std::vector<uint32_t> owner(L->nv, UINT32_MAX);
+        std::vector<uint8_t> referenced(L->nv, 0);
         for (size_t o = 0; o < n; ++o) {
             if (!L->t_cnt[o]) continue;
             uint32_t lo = UINT32_MAX, hi = 0;
@@ -83,11 +87,13 @@ pz_status pz_library_create(pz_ctx* c, const pz_mesh_view* v, pz_library** out) 
                     if (i >= L->nv) throw PzError(PZ_E_INVALID, "triangle index beyond the vertex array");
                     lo = std::min(lo, i);
                     hi = std::max(hi, i);
+                    referenced[i] = 1;
                 }
             for (uint32_t i = lo; i <= hi; ++i) {
                 if (owner[i] != UINT32_MAX && owner[i] != (uint32_t)o)
                     throw PzError(PZ_E_INVALID, "library objects share vertices");
                 owner[i] = (uint32_t)o;
+                L->runs_full = L->runs_full && referenced[i];
             }
             L->v_off[o] = lo;
             L->v_cnt[o] = hi - lo + 1;
@@ -214,6 +220,29 @@ pz_status pz_library_instantiate(pz_ctx* c, const pz_library* L, uint32_t nparts
         for (uint32_t p = 0; p < nparts; ++p)
             for (uint64_t o = d->part_obj_off[p]; o < d->part_obj_off[p + 1]; ++o) part_of[o] = p;
         for (uint32_t p = 0; p <= nparts; ++p) off[p] = (uint32_t)d->part_obj_off[p];
+        // Each instance owns the vertex run and the triangle run it is written to,
+        // so the topology the upload's scan finds (first / last triangle, count,
+        // last / first vertex per object, and the vertex chunks of the bounds
+        // kernel) follows from the records when the library's runs are full.
+        const uint32_t K = pzk::run_chunk_vertices();
+        const bool runs = L->runs_full && ninst > 0 && nv < (1ull << 32) - K;
+        const size_t tn = std::max<uint64_t>(ninst, 1);
+        std::vector<uint32_t> topo;
+        std::vector<uint2> chunks;
+        if (runs) {
+            topo.assign(5 * tn, 0u);
+            for (uint64_t i = 0; i < ninst; ++i) {
+                const pzk::InstanceRec& r = rec[i];
+                topo[i] = r.nt ? r.dt : 0xFFFFFFFFu;  // as the scan leaves an object without triangles
+                topo[tn + i] = r.nt ? r.dt + r.nt - 1 : 0u;
+                topo[2 * tn + i] = r.nt;
+                topo[3 * tn + i] = r.nt ? r.dv + r.nv - 1 : 0u;
+                topo[4 * tn + i] = r.nt ? r.dv : 0xFFFFFFFFu;
+                if (r.nt)
+                    for (uint64_t v = r.dv; v < (uint64_t)r.dv + r.nv; v += K)
+                        chunks.push_back(make_uint2((uint32_t)i, (uint32_t)v));
+            }
+        }
         pzk::InstanceRec* drec = nullptr;
         const bool owned = slot < 0;
         try {
@@ -227,6 +256,10 @@ pz_status pz_library_instantiate(pz_ctx* c, const pz_library* L, uint32_t nparts
                 HIP_OK(hipMalloc((void**)&d->part_of, part_of.size() * 4));
                 HIP_OK(hipMalloc((void**)&d->part_off, off.size() * 4));
                 HIP_OK(hipMalloc((void**)&drec, rec.size() * sizeof(pzk::InstanceRec)));
+                if (runs) {
+                    HIP_OK(hipMalloc((void**)&d->topo, 4 * topo.size()));
+                    HIP_OK(hipMalloc((void**)&d->topo_chunks, 8 * std::max<size_t>(chunks.size(), 1)));
+                }
             } else {  // the context's storage for this slot, grown on demand
                 const std::string k = "inst" + std::to_string(slot) + ".";
                 d->v = buf<float>(c, k + "v", 3 * nv);
@@ -238,12 +271,23 @@ pz_status pz_library_instantiate(pz_ctx* c, const pz_library* L, uint32_t nparts
                 d->part_of = buf<uint32_t>(c, k + "part_of", part_of.size());
                 d->part_off = buf<uint32_t>(c, k + "part_off", off.size());
                 drec = buf<pzk::InstanceRec>(c, k + "rec", rec.size());
+                if (runs) {
+                    d->topo = buf<uint32_t>(c, k + "topo", topo.size());
+                    d->topo_chunks = buf<uint2>(c, k + "topo_chunks", std::max<size_t>(chunks.size(), 1));
+                }
                 d->slot = slot;
                 d->slot_gen = ++c->slot_gen[slot];
             }
             HIP_OK(hipMemcpy(d->part_of, part_of.data(), part_of.size() * 4, hipMemcpyHostToDevice));
             HIP_OK(hipMemcpy(d->part_off, off.data(), off.size() * 4, hipMemcpyHostToDevice));
             HIP_OK(hipMemcpy(drec, rec.data(), rec.size() * sizeof(pzk::InstanceRec), hipMemcpyHostToDevice));
+            if (runs) {
+                HIP_OK(hipMemcpy(d->topo, topo.data(), 4 * topo.size(), hipMemcpyHostToDevice));
+                if (!chunks.empty())
+                    HIP_OK(hipMemcpy(d->topo_chunks, chunks.data(), 8 * chunks.size(), hipMemcpyHostToDevice));
+                d->topo_nchunks = chunks.size();
+                d->vruns = true;
+            }
             pzk::instantiate(drec, (uint32_t)ninst, L->v, L->idx, L->es, L->ee, d->v, d->idx, d->t2o, d->es, d->ee,
                              d->eo, c->s);
             HIP_OK(hipStreamSynchronize(c->s));

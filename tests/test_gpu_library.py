@@ -59,6 +59,7 @@ def test_instanced_batch_joins_match_oracle(ctx, small_library, sizing):
     ia = random_instances(rng, sum(ca), m.num_objects, 25.0)
     ib = random_instances(rng, sum(cb), m.num_objects, 25.0)
     da, db = lib.instantiate(ca, ia), lib.instantiate(cb, ib)
+    assert da.vertex_runs and db.vertex_runs  # topology from the instance records
     pa, pb = instantiate_host(m, ca, ia), instantiate_host(m, cb, ib)
     preds = [PREDS[q % 3] for q in range(len(ca))]
     outs, _ = pz.join_batch(ctx, da, db, [dict(predicate=p) for p in preds], sizing=sizing, capacity=8)
@@ -73,6 +74,30 @@ def test_instanced_batch_joins_match_oracle(ctx, small_library, sizing):
     outs2, _ = pz.join_batch(ctx, ha, hb, [dict(predicate=p) for p in preds])
     for o, o2 in zip(outs, outs2):
         np.testing.assert_array_equal(o.pairs, o2.pairs)
+
+
+def test_library_with_unreferenced_vertex_in_a_run(ctx, small_library):
+    """A vertex inside an object's run that no triangle references: instanced
+    sets then carry no vertex runs (bounds and sieve from the triangles), and
+    joins still give the oracle's pairs."""
+    m, _ = small_library
+    k = int(m.indices[m.triangle_to_object == 0].min()) + 1
+    V = np.insert(m.vertices, k, m.vertices[k], axis=0)  # the new vertex k is unreferenced
+    I = m.indices.copy()
+    I[I >= k] += 1
+    mm = pz.extract_edges(pz.Mesh(V, I, m.triangle_to_object))
+    lib = ctx.library(mm)
+    rng = np.random.default_rng(8)
+    ca, cb = [10, 7], [9, 12]
+    ia = random_instances(rng, sum(ca), mm.num_objects, 25.0)
+    ia["mesh"][0] = 0  # the object with the unreferenced vertex is instanced
+    ib = random_instances(rng, sum(cb), mm.num_objects, 25.0)
+    da, db = lib.instantiate(ca, ia), lib.instantiate(cb, ib)
+    assert not da.vertex_runs and not db.vertex_runs
+    pa, pb = instantiate_host(mm, ca, ia), instantiate_host(mm, cb, ib)
+    outs, _ = pz.join_batch(ctx, da, db, [dict(predicate=PREDS[q]) for q in range(len(ca))])
+    for q, o in enumerate(outs):
+        np.testing.assert_array_equal(o.pairs, oracle_pairs(pa[q], pb[q], PREDS[q]), err_msg=f"query {q}")
 
 
 def test_stream_catalogue_queries(ctx):
