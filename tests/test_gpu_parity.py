@@ -243,6 +243,18 @@ def test_grid_broad_phase_mixed_sizes(ctx):
     grid = pz.join(ctx, da, db, broad_mode=3)
     assert grid.num_candidates == scan.num_candidates > 0
     np.testing.assert_array_equal(grid.pairs, scan.pairs)
+    # the object-LBVH walk (a wave per mesh1 object, the 40x boxes meet most of
+    # mesh2), exact and capacity-sized; capacity 64 overflows into the recount
+    lbvh = pz.join(ctx, da, db, broad_mode=1)
+    assert lbvh.num_candidates == scan.num_candidates
+    np.testing.assert_array_equal(lbvh.pairs, scan.pairs)
+    for kw in (dict(sizing=pz.SIZING_ESTIMATE), dict(sizing=pz.SIZING_FIXED, capacity=64)):
+        o = pz.join(ctx, da, db, broad_mode=1, **kw)
+        print(f"[mixed sizes] lbvh {kw}: candidates {o.num_candidates} overflowed {o.overflowed}", flush=True)
+        assert o.num_candidates == scan.num_candidates
+        np.testing.assert_array_equal(o.pairs, scan.pairs)
+        if kw.get("capacity") == 64:
+            assert o.overflowed
     for pred in (pz.PRED_INTERSECTION, pz.PRED_CONTAINMENT):
         np.testing.assert_array_equal(pz.join(ctx, da, db, predicate=pred, broad_mode=3).pairs,
                                       pz.join(ctx, da, db, predicate=pred, broad_mode=2).pairs)
