@@ -131,31 +131,6 @@ SetPrep prep_set(pz_ctx* c, const pz_dataset* d, const std::string& tag) {
                       c->s);
     return p;
 }
-
-// Both sets of a join: with vertex runs on both, their bounds share one
-// initialisation and one bounds launch.
-void prep_sets(pz_ctx* c, const pz_dataset* a, const pz_dataset* b, SetPrep& pa, SetPrep& pb) {
-    if (!(a->vruns && b->vruns)) {
-        pa = prep_set(c, a, "A");
-        pb = prep_set(c, b, "B");
-        return;
-    }
-    auto topo_of = [&](const pz_dataset* d, const std::string& tag) {
-        SetPrep p;
-        const size_t n = std::max<int32_t>(d->nobj, 1);
-        p.obox = buf<uint32_t>(c, tag + ".obox", 6 * n);
-        p.edge_cnt = buf<uint32_t>(c, tag + ".edge_cnt", n);
-        p.first_tri = d->topo;
-        p.last_tri = d->topo + n;
-        p.tri_cnt = d->topo + 2 * n;
-        return p;
-    };
-    pa = topo_of(a, "A");
-    pb = topo_of(b, "B");
-    const size_t na = std::max<int32_t>(a->nobj, 1), nb = std::max<int32_t>(b->nobj, 1);
-    pzk::object_bounds_runs2(a->v, a->topo_chunks, a->topo_nchunks, a->topo + 3 * na, a->nobj, pa.obox, b->v,
-                             b->topo_chunks, b->topo_nchunks, b->topo + 3 * nb, b->nobj, pb.obox, c->s);
-}
 }  // namespace pz
 
 namespace {
@@ -486,7 +461,8 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
     ph.begin("Object Bounds");
     const SegOut prep = run_segment(c, A->uid, B->uid, kSegPrep, r, [&]() {
         SegOut o;
-        prep_sets(c, A, B, o.pa, o.pb);
+        o.pa = prep_set(c, A, "A");
+        o.pb = prep_set(c, B, "B");
         return o;
     });
     const SetPrep pa = prep.pa, pb = prep.pb;
@@ -970,8 +946,8 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         const pzd::SieveSet sa = sieve_set(A, pa), sb = sieve_set(B, pb);
         uint4* rdesc = buf<uint4>(c, "sv.rdesc", 5 * (np + 1));
         uint32_t* gwork = buf<uint32_t>(c, "sv.gwork", np + 1);
-        if (c->cus == 0) HIP_OK(hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, c->device));
-        const int cus = c->cus;
+        int cus = 0;
+        HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
         // parity-ray candidates: about a dozen target triangles per ray
         const uint64_t cap_rays = inside ? std::min<uint64_t>(std::max<uint64_t>(64 * np, 1 << 16), 1ull << 26) : 1;
         uint2* rays = buf<uint2>(c, "sv.rays", cap_rays);
@@ -1137,8 +1113,7 @@ void join_global(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const pz_j
     unsigned long long* stats = buf<unsigned long long>(c, "stats", pzk::kStatSlots);
     pzk::fill_ull(stats, 0ull, pzk::kStatSlots, s);
     ph.begin("Build Mesh1 Index");
-    SetPrep pa, pb;
-    prep_sets(c, A, B, pa, pb);
+    SetPrep pa = prep_set(c, A, "A"), pb = prep_set(c, B, "B");
     // LEARNED: features and inference overlap the set-wide index builds on the side stream
     uint32_t npred = 0;
     if (d.sizing == PZ_SIZING_LEARNED) npred = mlp_predict_async(c, d.model, A, B, pa, pb, nullptr, pair_meta_of(d));

@@ -222,18 +222,12 @@ __global__ __launch_bounds__(kBlock) void k_topo_verts(const int32_t* __restrict
 constexpr uint32_t kRunChunk = 4096;
 constexpr int kBoundsBatch = 12;  // measured against 8 and 16 on 642-vertex objects
 
-// One set's vertex runs for the bounds kernel.
-struct RunBounds {
-    const float* verts;
-    const uint2* chunks;
-    uint64_t nch;
-    const uint32_t* vmax;
-    uint32_t* obox;
-};
-
-__device__ __forceinline__ void bounds_of_chunk(const float* __restrict__ verts, const uint2* __restrict__ chunks,
-                                                uint64_t c, const uint32_t* __restrict__ vmax,
-                                                uint32_t* __restrict__ obox) {
+__global__ __launch_bounds__(kBlock) void k_object_bounds_runs(const float* __restrict__ verts,
+                                                              const uint2* __restrict__ chunks, uint64_t nch,
+                                                              const uint32_t* __restrict__ vmax,
+                                                              uint32_t* __restrict__ obox) {
+    const uint64_t c = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) / PZ_WAVE;
+    if (c >= nch) return;  // wave-uniform
     const uint2 ch = chunks[c];
     const uint32_t o = ch.x, v0 = ch.y;
     const uint32_t v1 = min(v0 + kRunChunk - 1u, vmax[o]);  // last vertex of the chunk
@@ -262,15 +256,6 @@ __device__ __forceinline__ void bounds_of_chunk(const float* __restrict__ verts,
             atomicMin(&obox[6ull * o + a], lo[a]);
             atomicMax(&obox[6ull * o + 3 + a], hi[a]);
         }
-}
-
-// The chunks of set a, then those of set b (an empty b: one set), one launch.
-__global__ __launch_bounds__(kBlock) void k_object_bounds_runs(RunBounds a, RunBounds b) {
-    const uint64_t c = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) / PZ_WAVE;  // wave-uniform
-    if (c < a.nch)
-        bounds_of_chunk(a.verts, a.chunks, c, a.vmax, a.obox);
-    else if (c - a.nch < b.nch)
-        bounds_of_chunk(b.verts, b.chunks, c - a.nch, b.vmax, b.obox);
 }
 
 __global__ __launch_bounds__(kBlock) void k_edge_count(const int32_t* __restrict__ eo, uint64_t ne, int32_t nobj,
@@ -1840,20 +1825,6 @@ __global__ void k_fill_any(T* p, T v, uint64_t n) {
     if (i < n) p[i] = v;
 }
 
-// Empty boxes for objects [0, n) of obox, then [0, n2) of obox2.
-__global__ void k_init_boxes2(uint32_t* obox, int64_t n, uint32_t* obox2, int64_t n2) {
-    int64_t o = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (o >= n) {
-        o -= n;
-        obox = obox2;
-        if (o >= n2) return;
-    }
-    for (int a = 0; a < 3; ++a) {
-        obox[6 * o + a] = 0xFF800000u;      // ordered +inf
-        obox[6 * o + 3 + a] = 0x007FFFFFu;  // ordered -inf
-    }
-}
-
 __global__ void k_init_boxes(uint32_t* obox, int64_t n) {
     int64_t o = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (o >= n) return;
@@ -1939,23 +1910,12 @@ uint32_t run_chunk_vertices() { return kRunChunk; }
 
 void object_bounds_runs(const float* verts, const uint2* chunks, uint64_t nch, const uint32_t* vmax, int32_t nobj,
                         uint32_t* obox, hipStream_t s) {
-    object_bounds_runs2(verts, chunks, nch, vmax, nobj, obox, nullptr, nullptr, 0, nullptr, 0, nullptr, s);
-}
-
-void object_bounds_runs2(const float* va, const uint2* ca, uint64_t nca, const uint32_t* vmaxa, int32_t na,
-                         uint32_t* oboxa, const float* vb, const uint2* cb, uint64_t ncb, const uint32_t* vmaxb,
-                         int32_t nb, uint32_t* oboxb, hipStream_t s) {
-    na = std::max(na, 0);
-    nb = std::max(nb, 0);
-    if (na + nb == 0) return;
-    if (na == 0) nca = 0;
-    if (nb == 0) ncb = 0;
-    hipLaunchKernelGGL(k_init_boxes2, dim3(grid1((uint64_t)na + nb)), dim3(kBlock), 0, s, oboxa, (int64_t)na, oboxb,
-                       (int64_t)nb);
+    if (nobj <= 0) return;
+    hipLaunchKernelGGL(k_init_boxes, dim3(grid1((uint64_t)nobj)), dim3(kBlock), 0, s, obox, (int64_t)nobj);
     KernelTimer kt_("object_bounds_runs", s);
-    if (nca + ncb)
-        hipLaunchKernelGGL(k_object_bounds_runs, dim3(grid1((nca + ncb) * PZ_WAVE)), dim3(kBlock), 0, s,
-                           RunBounds{va, ca, nca, vmaxa, oboxa}, RunBounds{vb, cb, ncb, vmaxb, oboxb});
+    if (nch)
+        hipLaunchKernelGGL(k_object_bounds_runs, dim3(grid1(nch * PZ_WAVE)), dim3(kBlock), 0, s, verts, chunks, nch,
+                           vmax, obox);
 }
 
 void edge_count(const int32_t* eo, uint64_t ne, int32_t nobj, uint32_t* edge_cnt, hipStream_t s) {

@@ -77,10 +77,6 @@ uint32_t run_chunk_vertices();
 // vertices of one run, which ends at vmax[object].
 void object_bounds_runs(const float* verts, const uint2* chunks, uint64_t nch, const uint32_t* vmax, int32_t nobj,
                         uint32_t* obox, hipStream_t s);
-// Both sets' bounds from their runs in one initialisation and one bounds launch.
-void object_bounds_runs2(const float* va, const uint2* ca, uint64_t nca, const uint32_t* vmaxa, int32_t na,
-                         uint32_t* oboxa, const float* vb, const uint2* cb, uint64_t ncb, const uint32_t* vmaxb,
-                         int32_t nb, uint32_t* oboxb, hipStream_t s);
 
 // --- small-object LDS builder (build_small.hip)
 // Objects with at most kSmallMax triangles stored contiguously are built

@@ -234,9 +234,6 @@ __global__ __launch_bounds__(kSieveThreads) void k_pair_sieve(
     uint32_t n_sieved = 0;  // thread 0: pairs decided here
     unsigned long long n_tris = 0;  // and their objects' triangles
     __shared__ uint32_t s_item;
-    // an empty list (the usual case when every pair has vertex runs) ends the
-    // grid without a same-address atomic per workgroup
-    if (ctr[1] == 0) return;  // grid-uniform: the list is complete before this launch
     while (true) {  // pairs of the work list, one at a time
         __syncthreads();
         if (tid == 0) s_item = atomicAdd(&ctr[3], 1u);

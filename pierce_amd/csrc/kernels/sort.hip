@@ -90,36 +90,6 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const Tin* __restri
     if (write_total && blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanThreads - 1) out[n] = run;
 }
 
-// k_scan_apply with the block's offset summed from the tile sums of the
-// blocks before it (a few per thread while there are at most kScanDirect
-// tiles), which saves the launch that scans the tile sums.
-constexpr uint64_t kScanDirect = 512;
-template <class Tin, class Tout>
-__global__ __launch_bounds__(kScanThreads) void k_scan_apply_sums(const Tin* __restrict__ in, uint64_t n,
-                                                                  Tout* __restrict__ out,
-                                                                  const Tout* __restrict__ sums) {
-    Tout pre = 0;
-    for (uint32_t i = threadIdx.x; i < blockIdx.x; i += kScanThreads) pre += sums[i];
-    Tout off;
-    (void)block_excl_scan<Tout>(pre, &off);
-    uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
-    Tout v[kScanItems];
-    Tout t = 0;
-#pragma unroll
-    for (int i = 0; i < kScanItems; ++i) {
-        v[i] = base + i < n ? (Tout)in[base + i] : (Tout)0;
-        t += v[i];
-    }
-    Tout tot;
-    Tout run = block_excl_scan<Tout>(t, &tot) + off;
-#pragma unroll
-    for (int i = 0; i < kScanItems; ++i) {
-        if (base + i < n) out[base + i] = run;
-        run += v[i];
-    }
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanThreads - 1) out[n] = run;
-}
-
 // Exclusive scan of `in` (n items) into out[0..n]; recursion over tile sums.
 template <class Tin, class Tout>
 void scan_impl(const Tin* in, Tout* out, uint64_t n, char* temp, hipStream_t s) {
@@ -138,11 +108,6 @@ void scan_impl(const Tin* in, Tout* out, uint64_t n, char* temp, hipStream_t s) 
     Tout* sums_scan = sums + nb;
     char* rest = temp + ((2 * nb + 1) * sizeof(Tout) + 255) / 256 * 256;
     hipLaunchKernelGGL((k_scan_reduce<Tin, Tout>), dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, sums);
-    if (nb <= kScanDirect) {
-        hipLaunchKernelGGL((k_scan_apply_sums<Tin, Tout>), dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, out,
-                           (const Tout*)sums);
-        return;
-    }
     scan_impl<Tout, Tout>(sums, sums_scan, nb, rest, s);
     hipLaunchKernelGGL((k_scan_apply<Tin, Tout>), dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, out,
                        (const Tout*)sums_scan, 1);

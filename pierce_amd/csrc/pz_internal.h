@@ -43,7 +43,6 @@ struct pz_ctx {
     TuneTable tune;
     int last_queue_class = 0;  // narrow-queue class of the last two-level join (what pz_autotune tunes)
     unsigned persistent_blocks = 0;  // grid of the persistent narrow kernel: 8 blocks per CU
-    int cus = 0;  // compute units of the device (queried once)
     uint64_t pool_epoch = 0;         // bumped whenever a pool buffer moves (captured graphs go stale)
     bool graphs = true;              // capture repeated fixed-shape segments as HIP graphs
     std::vector<struct GraphSeg*> segs;
