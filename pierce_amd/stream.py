@@ -230,8 +230,10 @@ def instantiate_host(lib_mesh, counts, inst):
     t2o = lib_mesh.triangle_to_object
     f = np.float32
 
-    def runs(o):
+    def runs(o):  # an object id without triangles has an empty run (lo > hi)
         tri = lib_mesh.indices[t2o == o]
+        if tri.shape[0] == 0:
+            return tri, 0, -1
         lo, hi = int(tri.min()), int(tri.max())
         return tri, lo, hi
 

@@ -100,6 +100,29 @@ def test_library_with_unreferenced_vertex_in_a_run(ctx, small_library):
         np.testing.assert_array_equal(o.pairs, oracle_pairs(pa[q], pb[q], PREDS[q]), err_msg=f"query {q}")
 
 
+def test_library_with_empty_object_ids(ctx, small_library):
+    """Library object ids with no triangles (a gap in the ids) instanced
+    between real objects: the instanced topology gives them no triangles and
+    no vertex run, and joins still give the oracle's pairs."""
+    m, _ = small_library
+    t2o = m.triangle_to_object.copy()
+    t2o[t2o >= 3] += 2  # ids 3 and 4 hold no triangles
+    mm = pz.extract_edges(pz.Mesh(m.vertices, m.indices, t2o))
+    lib = ctx.library(mm)
+    rng = np.random.default_rng(9)
+    ca, cb = [8, 9], [7, 10]
+    ia = random_instances(rng, sum(ca), mm.num_objects, 25.0)
+    ib = random_instances(rng, sum(cb), mm.num_objects, 25.0)
+    ia["mesh"][[1, 4, 9]] = [3, 4, 3]  # empty objects among the instances of both parts
+    ib["mesh"][[0, 12]] = [4, 3]
+    da, db = lib.instantiate(ca, ia), lib.instantiate(cb, ib)
+    assert da.vertex_runs and db.vertex_runs
+    pa, pb = instantiate_host(mm, ca, ia), instantiate_host(mm, cb, ib)
+    outs, _ = pz.join_batch(ctx, da, db, [dict(predicate=PREDS[q]) for q in range(len(ca))])
+    for q, o in enumerate(outs):
+        np.testing.assert_array_equal(o.pairs, oracle_pairs(pa[q], pb[q], PREDS[q]), err_msg=f"query {q}")
+
+
 def test_stream_catalogue_queries(ctx):
     """A few queries of the library stream (small sets) against the oracle."""
     m, ids = stream_catalogue(ellipsoids_per_level=8, tubes=6, tube_tris=(5000, 8000, 12000))
