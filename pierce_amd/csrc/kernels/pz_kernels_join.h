@@ -115,7 +115,8 @@ bool build_small(const float* verts, const uint32_t* idx, const int32_t* obj_of_
 // are not contiguous or an object's own order is not spatially coherent.
 // Chunk / workgroup sizes measured on 2,000 vs 2,000 tubes with own-order
 // chunks: 2,048 / 1,024 build 38.9 ms + walk 31.0 ms, 1,024 / 512 30.9 +
-// 32.0 ms, 512 / 256 28.5 + 34.5 ms, 1,024 / 256 34.2 + 31.9 ms.
+// 32.0 ms, 512 / 256 28.5 + 34.5 ms, 1,024 / 256 34.2 + 31.9 ms, 1,024 /
+// 1,024 44.9 + 32.0 ms.
 constexpr uint32_t kChunk = 1024;   // triangles per chunk (<= kSmallMax)
 constexpr int kChunkThreads = 512;  // one chunk build's workgroup
 struct ChunkRoot {
