@@ -135,16 +135,21 @@ SetPrep prep_set(pz_ctx* c, const pz_dataset* d, const std::string& tag) {
 
 namespace {
 
-// Phase timer: events recorded on the stream at phase boundaries.
+// Phase timer: events recorded on the stream at phase boundaries.  The first
+// event is recorded when the timer is made, so the first phase also holds the
+// host's set-up before its first launch and the phases cover the whole call
+// up to the final synchronisation.
 struct Phases {
     pz_ctx* c;
     pz_join_result* r;
     int n = 0;
-    explicit Phases(pz_ctx* ctx, pz_join_result* res) : c(ctx), r(res) {}
+    explicit Phases(pz_ctx* ctx, pz_join_result* res) : c(ctx), r(res) {
+        HIP_OK(hipEventRecord(c->ev[0], c->s));
+    }
     void begin(const char* name) {
         if (n >= PZ_MAX_PHASES) return;
         std::strncpy(r->phases[n].name, name, sizeof(r->phases[n].name) - 1);
-        HIP_OK(hipEventRecord(c->ev[n], c->s));
+        if (n > 0) HIP_OK(hipEventRecord(c->ev[n], c->s));
         ++n;
     }
     void finish() {
@@ -791,11 +796,7 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
             const uint64_t cap_esc = std::min<uint64_t>(std::max<uint64_t>(np, 1 << 16), 1ull << 22);
             uint4* esc = buf<uint4>(c, "np.esc", cap_esc);
             pzk::fill_ull(ctr + 2, 0ull, 1, s);
-            if (c->persistent_blocks == 0) {
-                int cus = 0;
-                HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-                c->persistent_blocks = (unsigned)std::max(cus, 1) * 8u;
-            }
+            if (c->persistent_blocks == 0) c->persistent_blocks = (unsigned)c->cus * 8u;
             const pzd::DualTask* cur = nullptr;
             uint64_t ntasks = np, spilled_total = 0;
             int rounds = 0;
@@ -946,14 +947,12 @@ void join_two_level(pz_ctx* c, const pz_dataset* A, const pz_dataset* B, const p
         const pzd::SieveSet sa = sieve_set(A, pa), sb = sieve_set(B, pb);
         uint4* rdesc = buf<uint4>(c, "sv.rdesc", 5 * (np + 1));
         uint32_t* gwork = buf<uint32_t>(c, "sv.gwork", np + 1);
-        int cus = 0;
-        HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
         // parity-ray candidates: about a dozen target triangles per ray
         const uint64_t cap_rays = inside ? std::min<uint64_t>(std::max<uint64_t>(64 * np, 1 << 16), 1ull << 26) : 1;
         uint2* rays = buf<uint2>(c, "sv.rays", cap_rays);
         pzk::pair_sieve(pairs, np, sa, sb, meta, f32, allow_f32, found, defer, rdesc, gwork, wctr,
                         inside ? rays : nullptr, nrays, inside ? cap_rays : 0, rovf, esc, sctr, cap_esc,
-                        (unsigned)std::max(cus, 1), stats, s);
+                        (unsigned)c->cus, stats, s);
         if (f32 && allow_f32) pzk::escalate_tris(esc, sctr, cap_esc, sa, sb, meta, found, stats, s);
         if (inside) {
             ph.begin("Containment Parity");
@@ -1307,6 +1306,8 @@ pz_status pz_ctx_create(int32_t device, pz_ctx** out) {
         std::unique_ptr<pz_ctx> c(new pz_ctx());
         c->device = device;
         HIP_OK(hipSetDevice(device));
+        HIP_OK(hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device));
+        c->cus = std::max(c->cus, 1);
         HIP_OK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
         HIP_OK(hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking));
         HIP_OK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));

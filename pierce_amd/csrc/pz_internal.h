@@ -36,6 +36,7 @@ struct TuneTable {
 
 struct pz_ctx {
     int device = 0;
+    int cus = 1;                       // compute units of the device (read once at creation)
     hipStream_t s2 = nullptr;          // side stream (default priority) for work that overlaps the main stream
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     void* pinned_est = nullptr;        // host landing area of the side stream's predictions
